@@ -1,0 +1,228 @@
+"""Benchmark of the MI355X pipeline-operator path (one GPU).
+
+Headline workload: ScanSelect(day <= cutoff, ~98%) -> GroupByAggregate over
+two enum keys (3 x 2 = 6 groups) with SUM and AVG of four float64 and three
+int32 measures plus COUNT(*), on 60M resident rows. A step is one
+kg_scan_group_agg call over all rows (the result, 6 groups, is read back).
+
+Also reported ("extra"): join build (16M unique int64 keys), join probe
+(256M int64 keys, 10% misses, uniform), and the probe fused with GROUP BY the
+matched build key (SUM(int64), COUNT(*)).
+
+Prints ONE JSON line.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "probe/agg rows/s + HBM GB/s fraction; batched vs plain queries/s at p99"
+HBM_PEAK_GBS = 8000.0
+SCAN_AGG_BYTES_PER_ROW = 4 + 2 * 4 + 4 * 8 + 3 * 4  # day, 2 enum codes, 4 f64, 3 i32 = 56 B
+PROBE_BYTES_PER_ROW = 8 + 64                       # streamed key + one 64 B bucket
+PROBE_AGG_BYTES_PER_ROW = 8 + 8 + 64 + 64          # key + value + bucket + accumulator update
+BUILD_BYTES_PER_ROW = 8 + 4 + 64                   # key + payload + one 64 B insert
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=60_000_000)
+    ap.add_argument("--cpu-rows", type=int, default=60_000_000, help="rows of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-join", action="store_true")
+    ap.add_argument("--build-rows", type=int, default=1 << 24)
+    ap.add_argument("--probe-rows", type=int, default=1 << 28)
+    ap.add_argument("--theta", type=float, default=0.0)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def roofline(bytes_per_launch, ms, traffic=None):
+    gbs = bytes_per_launch / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (profiles/traffic.json), or None when not measured."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def bench_scan_agg(ctx, a):
+    import torch
+    from kushdb_amd import abi
+    from kushdb_amd.runtime import Column
+    from kushdb_amd.workloads import scan_agg_table
+    n = a.rows
+    log(f"scan-agg: generating {n} rows")
+    t = scan_agg_table(n, device="cuda:0")
+    cols = [Column(abi.KG_DATE, t["day"]), Column(abi.KG_ENUM, t["g1"]), Column(abi.KG_ENUM, t["g2"])]
+    cols += [Column(abi.KG_REAL, t[f"m{i}"]) for i in range(1, 5)]
+    cols += [Column(abi.KG_INT, t[f"i{i}"]) for i in range(1, 4)]
+    aggs = [(abi.KG_AGG_SUM, 3 + i) for i in range(7)] + [(abi.KG_AGG_AVG, 3 + i) for i in range(7)]
+    aggs.append((abi.KG_AGG_COUNT, -1))
+    preds = [(0, abi.KG_LEQ, t["cutoff"])]
+    keys = [(1, 3), (2, 2)]
+    for _ in range(a.warmup):
+        ctx.scan_group_agg(cols, preds, keys, aggs)
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    ctx.kernel_stats(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = ctx.scan_group_agg(cols, preds, keys, aggs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms, launches = ctx.kernel_stats(reset=True)
+    ctx.set_timing(False)
+    assert launches == a.steps, (launches, a.steps)
+    groups = len(res[0])
+    return t, wall, kms / launches, groups
+
+
+def cpu_baseline_scan_agg(t, nrows):
+    """The oracle's single-threaded scan + group-by (same operator chain and
+    evaluation order as the reference's generated code) on the first nrows."""
+    import numpy as np
+    from oracle import pyoracle as po
+    h = {k: (v[:nrows].cpu().numpy() if hasattr(v, "cpu") else v) for k, v in t.items()}
+    fm = [np.ascontiguousarray(h[f"m{i}"]) for i in range(1, 5)]
+    im = [np.ascontiguousarray(h[f"i{i}"]) for i in range(1, 4)]
+    fp = (ctypes.c_void_p * 4)(*[x.ctypes.data for x in fm])
+    ip = (ctypes.c_void_p * 3)(*[x.ctypes.data for x in im])
+    out = np.zeros((16, 17), dtype=np.int64)
+    n = len(h["day"])
+    t0 = time.perf_counter()
+    po.lib().kor_q_scan_agg(n, h["day"].ctypes.data, h["cutoff"], h["g1"].ctypes.data, h["g2"].ctypes.data, 4, fp,
+                            3, ip, 16, out.ctypes.data)
+    dt = time.perf_counter() - t0
+    return n / dt, dt, n
+
+
+def bench_join(ctx, a):
+    import torch
+    from kushdb_amd import abi
+    from kushdb_amd.runtime import Column, HashTable
+    from kushdb_amd.workloads import join_tables
+    nb, npr = a.build_rows, a.probe_rows
+    log(f"join: generating build {nb} / probe {npr} (theta={a.theta})")
+    bk, pay, pk, pv = join_tables(nb, npr, theta=a.theta, device="cuda:0")
+    bcol, pcol = Column(abi.KG_BIGINT, bk), Column(abi.KG_BIGINT, pk)
+    out = {}
+    steps = max(1, min(a.steps, 5))
+    # build
+    build_ms = []
+    for i in range(steps + 1):
+        ht = HashTable(ctx, nb)
+        torch.cuda.synchronize()
+        ctx.set_timing(True)
+        ctx.kernel_stats(reset=True)
+        ht.insert(bcol)
+        kms, _ = ctx.kernel_stats(reset=True)
+        ctx.set_timing(False)
+        if i > 0:
+            build_ms.append(kms)
+        if i < steps:
+            ht.free()
+    bms = sum(build_ms) / len(build_ms)
+    out["join_build"] = {"rows": nb, "kernel_ms": round(bms, 3), "rows_per_s": nb / (bms * 1e-3),
+                         "roofline": roofline(nb * BUILD_BYTES_PER_ROW, bms, measured_traffic("k_ht_build"))}
+    # probe (join cardinality)
+    ht.probe_count(pcol)
+    ctx.set_timing(True)
+    ctx.kernel_stats(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        card = ht.probe_count(pcol)
+    wall = time.perf_counter() - t0
+    kms, launches = ctx.kernel_stats(reset=True)
+    ctx.set_timing(False)
+    pms = kms / launches
+    out["join_probe"] = {"rows": npr, "theta": a.theta, "matches": card, "kernel_ms": round(pms, 3),
+                         "ms_per_step": round(wall / steps * 1e3, 3), "rows_per_s": npr / (pms * 1e-3),
+                         "roofline": roofline(npr * PROBE_BYTES_PER_ROW, pms, measured_traffic("k_ht_probe_count"))}
+    # probe fused with GROUP BY matched key
+    ht.probe_agg(pcol, pv, max_groups=nb)
+    ctx.set_timing(True)
+    ctx.kernel_stats(reset=True)
+    for _ in range(steps):
+        k, s, c, _ = ht.probe_agg(pcol, pv, max_groups=nb)
+    kms, launches = ctx.kernel_stats(reset=True)
+    ctx.set_timing(False)
+    ams = kms / launches
+    out["join_probe_agg"] = {"rows": npr, "groups": int(k.numel()), "kernel_ms": round(ams, 3),
+                             "rows_per_s": npr / (ams * 1e-3),
+                             "roofline": roofline(npr * PROBE_AGG_BYTES_PER_ROW, ams,
+                                                  measured_traffic("k_ht_probe_agg"))}
+    ht.free()
+    del bk, pay, pk, pv
+    torch.cuda.empty_cache()
+    return out
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design note, "
+              "not a launched configuration.", file=sys.stderr)
+        return 2
+    import torch
+    from kushdb_amd.runtime import Context
+    torch.cuda.init()
+    ctx = Context(0)
+    t, wall, kms, groups = bench_scan_agg(ctx, a)
+    n = a.rows
+    rows_per_s = n * a.steps / wall
+    result = {
+        "metric": METRIC,
+        "value": rows_per_s,
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "scan-select (day <= cutoff, ~98%) + group-by 2 enum keys (6 groups): "
+                               "SUM/AVG of 4 f64 + 3 int32 measures, COUNT(*)",
+                   "rows": n, "groups": groups, "bytes_per_row": SCAN_AGG_BYTES_PER_ROW},
+        "roofline": dict(roofline(n * SCAN_AGG_BYTES_PER_ROW, kms, measured_traffic("k_group_agg_reg")),
+                         kernel="k_group_agg_reg", kernel_ms=round(kms, 4)),
+    }
+    if not a.no_cpu:
+        log("cpu baseline (oracle, 1 thread)")
+        cps, cdt, cn = cpu_baseline_scan_agg(t, min(a.cpu_rows, n))
+        result["cpu_baseline"] = {"value": cps, "unit": "rows/s", "cores": 1, "kind": "port",
+                                  "sample": f"first {cn} rows of the same table, {cdt:.2f} s, "
+                                            f"{platform.processor() or platform.machine()}"}
+    del t
+    torch.cuda.empty_cache()
+    if not a.no_join:
+        result["extra"] = bench_join(ctx, a)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

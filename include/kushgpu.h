@@ -1,0 +1,213 @@
+/*
+ * kushgpu — MI355X (gfx950) execution of KushDB's pipeline operators behind a
+ * C ABI.
+ *
+ * KushDB's generated pipelines call runtime C++ functions by absolute address
+ * (khir ProgramBuilder::DeclareExternalFunction, khir/program_builder.h:125;
+ * bound in compile/forward_declare.cc:18-60). Those calls are per tuple
+ * (runtime/hash_table.h:22-38, runtime/aggregate_hash_table.h:28-49) and
+ * several hot loops are inline IR rather than calls at all (the aggregate probe
+ * loop, compile/proxy/aggregate_hash_table.cc:187-243; the join bucket loop,
+ * compile/translators/hash_join_translator.cc:105-139). Per-tuple calls into a
+ * GPU are pointless, so this ABI sits one level up, at the pipeline seam of
+ * execution/pipeline.cc:23-35: one call per pipeline body over a whole column
+ * range, replacing the body_N(start, end) morsel loop of
+ * execution/executable_query.cc:118-144 for the operators below.
+ *
+ * Conventions
+ *  - Every function returns a kg_status; on failure kg_last_error(ctx) holds a
+ *    message. Nothing throws across this boundary (the reference throws C++
+ *    exceptions, e.g. runtime/file_manager.cc:25-53).
+ *  - Pointers prefixed d_ (and kg_col.data / kg_col.nulls) are device memory;
+ *    everything else is host memory. Input buffers are read-only during a call.
+ *  - Work is ordered on the context's HIP stream. Calls that return host
+ *    values (counts, small result sets) synchronize that stream; calls that only
+ *    write device buffers do not.
+ *  - One context is externally synchronized: do not call it from two threads
+ *    at once.
+ */
+#ifndef KUSHGPU_H_
+#define KUSHGPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+enum kg_status {
+  KG_OK = 0,
+  KG_ERR_INVALID = 1,     /* bad argument */
+  KG_ERR_HIP = 2,         /* HIP runtime error */
+  KG_ERR_OOM = 3,         /* device allocation failed */
+  KG_ERR_CAPACITY = 4,    /* an output or table capacity was exceeded */
+  KG_ERR_UNSUPPORTED = 5, /* a type / operator combination this path does not run */
+};
+
+/* SQL type ids; same order as catalog::TypeId (catalog/sql_type.h:7-16). */
+enum kg_type {
+  KG_SMALLINT = 0,
+  KG_INT = 1,
+  KG_BIGINT = 2,
+  KG_REAL = 3,
+  KG_DATE = 4,
+  KG_TEXT = 5,
+  KG_BOOLEAN = 6,
+  KG_ENUM = 7,
+};
+
+/* Comparison operators (plan::BinaryArithmeticExpressionType EQ..GEQ as used
+ * by ScanSelect / SimdScanSelect, simd_scan_select_translator.cc:402-442). */
+enum kg_cmp { KG_EQ = 0, KG_NEQ = 1, KG_LT = 2, KG_LEQ = 3, KG_GT = 4, KG_GEQ = 5 };
+
+/* Aggregate kinds (plan::AggregateType; compile/proxy/aggregator.cc). */
+enum kg_agg_kind { KG_AGG_SUM = 0, KG_AGG_MIN = 1, KG_AGG_MAX = 2, KG_AGG_AVG = 3, KG_AGG_COUNT = 4 };
+
+/* A resident column: `data` holds n values of the C type of `type` (int16,
+ * int32, int64, double, int8); `nulls` is an int8 array with 1 = NULL, or NULL
+ * for a non-nullable column. Mirrors runtime::ColumnData::*ColumnData plus the
+ * `_null.kdb` int8 column (runtime/column_data.h:20-58). */
+typedef struct kg_col {
+  int32_t type;
+  int32_t reserved;
+  const void* data;
+  const int8_t* nulls;
+} kg_col;
+
+/* One comparison `cols[col] op literal`; ilit for integer-like columns, flit
+ * for REAL. A NULL value fails the comparison (scan_select_translator.cc:129-133). */
+typedef struct kg_pred {
+  int32_t col;
+  int32_t op;
+  int64_t ilit;
+  double flit;
+} kg_pred;
+
+/* One aggregate over cols[col]; col = -1 with KG_AGG_COUNT is COUNT(*). */
+typedef struct kg_agg {
+  int32_t kind;
+  int32_t col;
+} kg_agg;
+
+typedef struct kg_ctx kg_ctx;
+typedef struct kg_hash_table kg_hash_table;
+
+/* ---- context -------------------------------------------------------------
+ * Replaces the per-query state (execution/query_state.h:26-41) and the
+ * process-wide FileManager column cache (runtime/file_manager.cc:22-64): a
+ * context owns a device, a stream and a grow-only workspace. */
+int kg_ctx_create(int device, kg_ctx** out);
+int kg_ctx_destroy(kg_ctx* ctx);
+const char* kg_last_error(const kg_ctx* ctx);
+/* Order the context's work on the caller's hipStream_t (NULL = the null
+ * stream). A context starts with a stream of its own. */
+int kg_ctx_set_stream(kg_ctx* ctx, void* hip_stream);
+int kg_ctx_synchronize(kg_ctx* ctx);
+/* 1 (default): FP64 reductions use a fixed order that depends only on n (and
+ * is reproduced by the CPU oracle); 0: order may depend on timing. */
+int kg_ctx_set_deterministic(kg_ctx* ctx, int on);
+/* Kernel timing (no counterpart in the reference, whose timing is wall clock
+ * around whole queries, util/time_execute.h:22-57): when on, every entry point
+ * brackets its dominant kernel with hipEvents on the context's stream.
+ * kg_ctx_kernel_stats synchronizes, returns the summed kernel time in ms and
+ * the number of timed launches since the last reset, and resets if asked. */
+int kg_ctx_set_timing(kg_ctx* ctx, int on);
+int kg_ctx_kernel_stats(kg_ctx* ctx, double* total_ms, int64_t* launches, int reset);
+
+/* ---- scan-select -----------------------------------------------------------
+ * ScanSelect (compile/translators/scan_select_translator.cc:92-156): rows where
+ * every predicate holds (conjunction = 1), or SimdScanSelect's disjunctive form
+ * (simd_scan_select_translator.cc:444-454, conjunction = 0). Writes the
+ * qualifying row ids to d_out_rows in ascending order — the order the 8-wide
+ * permutation compaction (util/permute.cc:10-25) produces — and their number to
+ * *out_count. d_out_rows must hold n entries. n < 2^31. */
+int kg_scan_select(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
+                   const kg_pred* preds, int32_t npreds, int32_t conjunction,
+                   int32_t* d_out_rows, int64_t* out_count);
+
+/* ---- scan-select + group-by aggregate (dense code keys) -----------------------
+ * ScanSelect -> GroupByAggregate (compile/translators/group_by_aggregate_translator.cc:36-111)
+ * where every group key is a dense int32 code column (ENUM / small INT domain):
+ * key_cols[k] indexes cols, key_card[k] bounds its codes to [0, key_card[k]).
+ * The groups are emitted in insertion order (first qualifying row), which is
+ * the order AggregateHashTable::ForEach walks its payloads
+ * (compile/proxy/aggregate_hash_table.cc:245-291). For each group:
+ * out_keys[g*nkeys+k] are the codes, out_vals[g*naggs+a] the aggregate as an
+ * 8-byte value (integers sign-extended in the SUM/MIN/MAX child type, REAL and
+ * AVG as IEEE double bits, COUNT as int64) and out_nulls[g*naggs+a] its NULL flag.
+ * Semantics follow compile/proxy/aggregator.cc: SUM wraps in the child type
+ * (plan/expression/aggregate_expression.cc:21-27), AVG is over non-NULL values,
+ * COUNT(col) counts non-NULL values plus one when the group's first row is NULL
+ * (CountAggregator::Initialize stores 1 unconditionally, aggregator.cc:285-287).
+ * With nkeys = 0 this is the ungrouped AggregateOperator (aggregate_translator.cc:34-126):
+ * one group, or zero groups on empty input. */
+int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
+                      const kg_pred* preds, int32_t npreds,
+                      const int32_t* key_cols, const int32_t* key_card, int32_t nkeys,
+                      const kg_agg* aggs, int32_t naggs, int64_t max_groups,
+                      int64_t* out_ngroups, int32_t* out_keys, int64_t* out_vals,
+                      int8_t* out_nulls);
+
+/* ---- hash join -----------------------------------------------------------------
+ * HBM-resident open-addressing replacement for runtime::HashTable
+ * (runtime/hash_table.h:12-38, .cc:13-67) as used by HashJoinTranslator
+ * (compile/translators/hash_join_translator.cc:34-139). Keys are one column of
+ * type SMALLINT/INT/BIGINT/DATE/ENUM/BOOLEAN/REAL (REAL keys compare as doubles:
+ * -0.0 equals +0.0, NaN matches nothing). */
+
+/* Create an empty table able to hold `capacity_rows` build rows
+ * (runtime::HashTable::Create, hash_table.cc:13-16). Keys live in 64-byte
+ * buckets of eight 8-byte slots; load_factor (0 < lf <= 0.9, 0 = default 0.5)
+ * sets slots = capacity_rows / lf. */
+int kg_ht_create(kg_ctx* ctx, int64_t capacity_rows, double load_factor, kg_hash_table** out);
+/* Insert build rows (hash_join_translator.cc:85-103; runtime::HashTable::Insert,
+ * hash_table.cc:18-27). d_rows (may be NULL) lists the build row ids to insert,
+ * e.g. a kg_scan_select output; otherwise rows 0..nrows-1. Rows with a NULL key
+ * are skipped. May be called several times before probing. */
+int kg_ht_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t* d_rows,
+                int64_t nrows);
+/* Probe (hash_join_translator.cc:105-139; runtime::HashTable::GetBucket,
+ * hash_table.cc:29-36 plus CheckEquality :57-77): every (probe row, build row)
+ * pair with equal keys. Pairs are written in no particular order; the set equals
+ * the reference's after sorting. *out_count receives the total number of pairs
+ * even when it exceeds out_cap (then KG_ERR_CAPACITY is returned and only
+ * out_cap pairs are written). */
+int kg_ht_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, const int32_t* d_rows,
+                int64_t nrows, int32_t* d_out_probe, int32_t* d_out_build, int64_t out_cap,
+                int64_t* out_count);
+/* Join cardinality only (the number of pairs kg_ht_probe would emit). */
+int kg_ht_probe_count(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key,
+                      const int32_t* d_rows, int64_t nrows, int64_t* out_count);
+/* Join fused with GroupByAggregate on the matched build key:
+ * GROUP BY build.key with SUM(d_vals) (BIGINT, wrapping) and COUNT(*), over all
+ * join pairs. Writes at most max_groups groups (key, sum, count) to the device
+ * arrays; *out_ngroups receives the number of groups. Group order is unspecified
+ * (the reference's insertion order is recovered by the caller, if needed, from
+ * the first probe row per group written to d_out_first when it is non-NULL). */
+int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, const int64_t* d_vals,
+                    int64_t nrows, int64_t max_groups, int64_t* d_out_key, int64_t* d_out_sum,
+                    int64_t* d_out_count, int32_t* d_out_first, int64_t* out_ngroups);
+/* Number of build rows inserted so far. */
+int kg_ht_size(kg_ctx* ctx, const kg_hash_table* ht, int64_t* out_rows);
+/* Release the table (runtime::HashTable::Free, hash_table.cc:53-59). */
+int kg_ht_free(kg_ctx* ctx, kg_hash_table* ht);
+
+/* ---- high-cardinality group-by ---------------------------------------------------
+ * GroupByAggregate on one BIGINT key with SUM(BIGINT) and COUNT(*), for group
+ * counts too large for an on-chip table: replaces the salted linear-probe
+ * runtime::AggregateHashTable (runtime/aggregate_hash_table.cc:9-135) and its
+ * generated UpdateOrInsert loop (compile/proxy/aggregate_hash_table.cc:187-243)
+ * with an HBM open-addressing table updated by 64-bit atomics. Writes at most
+ * max_groups groups to the device arrays in unspecified order, and the first
+ * input row of each group to d_out_first (if non-NULL) so the caller can restore
+ * insertion order. */
+int kg_group_by_i64(kg_ctx* ctx, const int64_t* d_keys, const int64_t* d_vals, int64_t n,
+                    int64_t max_groups, int64_t* d_out_key, int64_t* d_out_sum,
+                    int64_t* d_out_count, int32_t* d_out_first, int64_t* out_ngroups);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KUSHGPU_H_ */

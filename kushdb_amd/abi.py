@@ -1,0 +1,96 @@
+"""ctypes declarations of the kushgpu C ABI (include/kushgpu.h).
+
+Loading fails loudly when libkushgpu.so is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkushgpu.so")
+
+# enum kg_status
+KG_OK, KG_ERR_INVALID, KG_ERR_HIP, KG_ERR_OOM, KG_ERR_CAPACITY, KG_ERR_UNSUPPORTED = range(6)
+# enum kg_type (catalog::TypeId order)
+KG_SMALLINT, KG_INT, KG_BIGINT, KG_REAL, KG_DATE, KG_TEXT, KG_BOOLEAN, KG_ENUM = range(8)
+# enum kg_cmp
+KG_EQ, KG_NEQ, KG_LT, KG_LEQ, KG_GT, KG_GEQ = range(6)
+# enum kg_agg_kind
+KG_AGG_SUM, KG_AGG_MIN, KG_AGG_MAX, KG_AGG_AVG, KG_AGG_COUNT = range(5)
+
+
+class KgCol(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("data", ctypes.c_void_p), ("nulls", ctypes.c_void_p)]
+
+
+class KgPred(ctypes.Structure):
+    _fields_ = [("col", ctypes.c_int32), ("op", ctypes.c_int32),
+                ("ilit", ctypes.c_int64), ("flit", ctypes.c_double)]
+
+
+class KgAgg(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("col", ctypes.c_int32)]
+
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+_pi32 = ctypes.POINTER(ctypes.c_int32)
+_pi8 = ctypes.POINTER(ctypes.c_int8)
+
+# name -> (restype, argtypes); every symbol include/kushgpu.h declares.
+SIGNATURES = {
+    "kg_ctx_create": (_i32, [_i32, ctypes.POINTER(_vp)]),
+    "kg_ctx_destroy": (_i32, [_vp]),
+    "kg_last_error": (ctypes.c_char_p, [_vp]),
+    "kg_ctx_set_stream": (_i32, [_vp, _vp]),
+    "kg_ctx_synchronize": (_i32, [_vp]),
+    "kg_ctx_set_deterministic": (_i32, [_vp, _i32]),
+    "kg_ctx_set_timing": (_i32, [_vp, _i32]),
+    "kg_ctx_kernel_stats": (_i32, [_vp, ctypes.POINTER(ctypes.c_double), _pi64, _i32]),
+    "kg_scan_select": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32, _i32,
+                              _vp, _pi64]),
+    "kg_scan_group_agg": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32,
+                                 _pi32, _pi32, _i32, ctypes.POINTER(KgAgg), _i32, _i64, _pi64, _pi32, _pi64,
+                                 _pi8]),
+    "kg_ht_create": (_i32, [_vp, _i64, ctypes.c_double, ctypes.POINTER(_vp)]),
+    "kg_ht_build": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64]),
+    "kg_ht_probe": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64, _vp, _vp, _i64, _pi64]),
+    "kg_ht_probe_count": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64, _pi64]),
+    "kg_ht_probe_agg": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64, _i64, _vp, _vp, _vp, _vp, _pi64]),
+    "kg_ht_size": (_i32, [_vp, _vp, _pi64]),
+    "kg_ht_free": (_i32, [_vp, _vp]),
+    "kg_group_by_i64": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _pi64]),
+}
+
+# Helpers exported next to the ABI (not part of include/kushgpu.h's contract).
+EXTRA_SIGNATURES = {
+    "kg_agg_blocks_for": (_i32, [_i64]),
+}
+
+_lib = None
+
+
+def load():
+    """Returns the loaded library, raising if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m kushdb_amd.build` "
+            "(there is no CPU fallback for the GPU path)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in {**SIGNATURES, **EXTRA_SIGNATURES}.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class KgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"kushgpu status {status}: {msg}")
+        self.status = status

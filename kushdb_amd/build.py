@@ -1,0 +1,73 @@
+"""Build recipe for the kushgpu shared library (gfx950) and the CPU oracle.
+
+    python -m kushdb_amd.build            # libkushgpu.so + oracle
+    python -m kushdb_amd.build --lib-only
+
+The library is compiled in-tree to kushdb_amd/libkushgpu.so with hipcc for
+gfx950 only. Packed-FP32 instruction selection is disabled for every HIP
+translation unit.
+"""
+import argparse
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "kushdb_amd")
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libkushgpu.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+SOURCES = [
+    "kg_context.hip",
+    "kg_scan.hip",
+    "kg_join.hip",
+]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    "-Wno-unused-result",
+    "-Wno-unused-value",
+]
+
+
+def build_lib(verbose=False):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-o", tmp, *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(verbose=False):
+    cmd = ["make", "-s", "-C", ORACLE_DIR]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib-only", action="store_true")
+    ap.add_argument("-v", "--verbose", action="store_true")
+    a = ap.parse_args(argv)
+    build_lib(a.verbose)
+    if not a.lib_only:
+        build_oracle(a.verbose)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,171 @@
+// Context, errors and workspace for the kushgpu C ABI.
+#include <cstdarg>
+#include <new>
+
+#include "kg_internal.h"
+
+int kg_fail(kg_ctx* ctx, int status, const char* fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return status;
+}
+
+int kg_work_reserve(kg_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->work_bytes) return KG_OK;
+  // Outstanding kernels may still read the old buffer.
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->work) KG_HIP_CHECK(ctx, hipFree(ctx->work));
+  ctx->work = nullptr;
+  ctx->work_bytes = 0;
+  size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
+  if (hipMalloc(&ctx->work, want) != hipSuccess) {
+    (void)hipGetLastError();
+    return kg_fail(ctx, KG_ERR_OOM, "workspace allocation of %zu bytes failed", want);
+  }
+  ctx->work_bytes = want;
+  return KG_OK;
+}
+
+int kg_pinned_reserve(kg_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->pinned_bytes) return KG_OK;
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->pinned) KG_HIP_CHECK(ctx, hipHostFree(ctx->pinned));
+  ctx->pinned = nullptr;
+  ctx->pinned_bytes = 0;
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  KG_HIP_CHECK(ctx, hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+  ctx->pinned_bytes = want;
+  return KG_OK;
+}
+
+static hipEvent_t take_event(kg_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void kg_time_begin(kg_ctx* ctx) {
+  if (!ctx->timing) return;
+  hipEvent_t a = take_event(ctx), b = take_event(ctx);
+  if (!a || !b) return;
+  (void)hipEventRecord(a, ctx->stream);
+  ctx->ev_pending.push_back({a, b});
+}
+
+void kg_time_end(kg_ctx* ctx) {
+  if (!ctx->timing || ctx->ev_pending.empty()) return;
+  (void)hipEventRecord(ctx->ev_pending.back().second, ctx->stream);
+}
+
+extern "C" {
+
+int kg_ctx_set_timing(kg_ctx* ctx, int on) {
+  if (!ctx) return KG_ERR_INVALID;
+  ctx->timing = on ? 1 : 0;
+  return KG_OK;
+}
+
+int kg_ctx_kernel_stats(kg_ctx* ctx, double* total_ms, int64_t* launches, int reset) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, total_ms && launches, "null output pointer");
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  double t = 0;
+  for (auto& pr : ctx->ev_pending) {
+    float ms = 0;
+    KG_HIP_CHECK(ctx, hipEventElapsedTime(&ms, pr.first, pr.second));
+    t += ms;
+  }
+  *total_ms = t;
+  *launches = static_cast<int64_t>(ctx->ev_pending.size());
+  if (reset) {
+    for (auto& pr : ctx->ev_pending) {
+      ctx->ev_pool.push_back(pr.first);
+      ctx->ev_pool.push_back(pr.second);
+    }
+    ctx->ev_pending.clear();
+  }
+  return KG_OK;
+}
+
+int kg_ctx_create(int device, kg_ctx** out) {
+  if (!out) return KG_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return KG_ERR_HIP;
+  }
+  if (device < 0 || device >= count) return KG_ERR_INVALID;
+  kg_ctx* ctx = new (std::nothrow) kg_ctx();
+  if (!ctx) return KG_ERR_OOM;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete ctx;
+    return KG_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    return KG_ERR_HIP;
+  }
+  ctx->own_stream = true;
+  *out = ctx;
+  return KG_OK;
+}
+
+int kg_ctx_destroy(kg_ctx* ctx) {
+  if (!ctx) return KG_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& pr : ctx->ev_pending) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->work) hipFree(ctx->work);
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return KG_OK;
+}
+
+const char* kg_last_error(const kg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int kg_ctx_set_stream(kg_ctx* ctx, void* hip_stream) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (ctx->own_stream) {
+    KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    KG_HIP_CHECK(ctx, hipStreamDestroy(ctx->stream));
+    ctx->own_stream = false;
+    ctx->stream = nullptr;
+  }
+  ctx->stream = static_cast<hipStream_t>(hip_stream);
+  return KG_OK;
+}
+
+int kg_ctx_synchronize(kg_ctx* ctx) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return KG_OK;
+}
+
+int kg_ctx_set_deterministic(kg_ctx* ctx, int on) {
+  if (!ctx) return KG_ERR_INVALID;
+  ctx->deterministic = on ? 1 : 0;
+  return KG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Internal definitions shared by the kushgpu HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "kushgpu.h"
+
+struct kg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int deterministic = 1;
+  int num_cus = 256;
+  std::string err;
+  // Grow-only device workspace (block partials, counters) and a small pinned
+  // host buffer for scalar read-backs.
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  // Dominant-kernel timing: event pairs recorded since the last stats reset.
+  int timing = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+};
+
+// Records a timing start/stop on ctx->stream when timing is on (no-op otherwise).
+// Usage: kg_time_begin(ctx); launch...; kg_time_end(ctx);
+void kg_time_begin(kg_ctx* ctx);
+void kg_time_end(kg_ctx* ctx);
+
+// Sets ctx->err and returns the status.
+int kg_fail(kg_ctx* ctx, int status, const char* fmt, ...);
+
+// Ensures ctx->work holds at least `bytes`; returns KG_OK or an error status.
+int kg_work_reserve(kg_ctx* ctx, size_t bytes);
+int kg_pinned_reserve(kg_ctx* ctx, size_t bytes);
+
+#define KG_HIP_CHECK(ctx, expr)                                                       \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return kg_fail((ctx), KG_ERR_HIP, "%s failed: %s (%s:%d)", #expr,              \
+                     hipGetErrorString(e_), __FILE__, __LINE__);                      \
+  } while (0)
+
+#define KG_LAUNCH_CHECK(ctx)                                                          \
+  do {                                                                                \
+    hipError_t e_ = hipGetLastError();                                                \
+    if (e_ != hipSuccess)                                                             \
+      return kg_fail((ctx), KG_ERR_HIP, "kernel launch failed: %s (%s:%d)",           \
+                     hipGetErrorString(e_), __FILE__, __LINE__);                      \
+  } while (0)
+
+#define KG_REQUIRE(ctx, cond, ...)                                                    \
+  do {                                                                                \
+    if (!(cond)) return kg_fail((ctx), KG_ERR_INVALID, __VA_ARGS__);                  \
+  } while (0)
+
+namespace kg {
+
+constexpr int kWave = 64;
+
+// Width in bytes of a column's values for the SQL type ids of kushgpu.h.
+inline int type_width(int32_t type) {
+  switch (type) {
+    case KG_SMALLINT: return 2;
+    case KG_INT:
+    case KG_DATE:
+    case KG_ENUM: return 4;
+    case KG_BIGINT:
+    case KG_REAL: return 8;
+    case KG_BOOLEAN: return 1;
+    default: return 0;
+  }
+}
+
+// Column access on the device: integers sign-extended to int64, REAL as double.
+__device__ __forceinline__ int64_t load_int(int32_t type, const void* p, int64_t i) {
+  switch (type) {
+    case KG_SMALLINT: return static_cast<const int16_t*>(p)[i];
+    case KG_BIGINT: return static_cast<const int64_t*>(p)[i];
+    case KG_BOOLEAN: return static_cast<const int8_t*>(p)[i];
+    default: return static_cast<const int32_t*>(p)[i];
+  }
+}
+
+__device__ __forceinline__ bool cmp_i64(int32_t op, int64_t x, int64_t l) {
+  switch (op) {
+    case KG_EQ: return x == l;
+    case KG_NEQ: return x != l;
+    case KG_LT: return x < l;
+    case KG_LEQ: return x <= l;
+    case KG_GT: return x > l;
+    default: return x >= l;
+  }
+}
+
+__device__ __forceinline__ bool cmp_f64(int32_t op, double x, double l) {
+  switch (op) {
+    case KG_EQ: return x == l;
+    case KG_NEQ: return x != l;
+    case KG_LT: return x < l;
+    case KG_LEQ: return x <= l;
+    case KG_GT: return x > l;
+    default: return x >= l;
+  }
+}
+
+// 64-bit finalizer (MurmurHash3 fmix64) used to place keys in the device
+// tables. Placement is not visible in results, so it need not match the
+// reference's 32-bit HashCombine bucket choice.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace kg

@@ -1,0 +1,880 @@
+// Scan-select and scan-select + group-by aggregation over dense code keys.
+//
+// Reference behaviour: ScanSelectTranslator (compile/translators/scan_select_translator.cc:92-156),
+// SimdScanSelectTranslator (simd_scan_select_translator.cc:94-532),
+// GroupByAggregateTranslator (group_by_aggregate_translator.cc:36-111),
+// AggregateTranslator (aggregate_translator.cc:34-126), aggregators
+// (compile/proxy/aggregator.cc:14-298).
+//
+// Layout and kernels:
+//  * Columns are resident in HBM as plain typed arrays (one per column) with an
+//    optional int8 null array, the .kdb / _null.kdb layout.
+//  * Selection: a tile of 4096 rows per workgroup; each wave evaluates 64 rows,
+//    ballots the result and compacts with a popcount prefix (the wave-wide form
+//    of the reference's 8-wide mask -> permutation-table compaction).
+//  * Low-cardinality aggregation (<= 8 dense groups): every lane keeps private
+//    per-group partials in registers, folds its rows in a fixed order, and the
+//    workgroup combines them with a fixed tree (lanes, then waves); a second
+//    kernel adds the per-workgroup partials in workgroup order. The order
+//    depends only on n, so FP64 results are reproducible and the CPU oracle
+//    restates it exactly.
+//  * Up to kMaxLdsGroups groups: a workgroup-private table in LDS updated with
+//    LDS atomics, merged into a global table with global atomics.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "kg_internal.h"
+
+namespace kg {
+
+constexpr int kMaxPreds = 8;
+constexpr int kMaxMeasures = 8;
+constexpr int kBlock = 256;
+constexpr int kSelTile = 4096;          // rows per workgroup in the selection kernels
+constexpr int kAggMaxBlocks = 2048;     // workgroups of the fixed-order aggregation
+constexpr int kAggRowsPerThreadMin = 8; // n / (kBlock * this) workgroups, capped above
+constexpr int kMaxLdsGroups = 1024;
+
+struct PredSet {
+  int32_t n;
+  int32_t conj;
+  int32_t type[kMaxPreds];
+  int32_t op[kMaxPreds];
+  const void* data[kMaxPreds];
+  const int8_t* nulls[kMaxPreds];
+  int64_t ilit[kMaxPreds];
+  double flit[kMaxPreds];
+};
+
+__device__ __forceinline__ bool eval_preds(const PredSet& p, int64_t r) {
+  bool pass = p.conj ? true : (p.n == 0);
+  for (int i = 0; i < p.n; i++) {
+    bool v;
+    if (p.type[i] == KG_REAL) {
+      v = cmp_f64(p.op[i], static_cast<const double*>(p.data[i])[r], p.flit[i]);
+    } else {
+      v = cmp_i64(p.op[i], load_int(p.type[i], p.data[i], r), p.ilit[i]);
+    }
+    if (p.nulls[i] && p.nulls[i][r]) v = false;
+    pass = p.conj ? (pass && v) : (pass || v);
+  }
+  return pass;
+}
+
+// ---------------------------------------------------------------------------
+// Selection
+
+__global__ __launch_bounds__(kBlock) void k_select_count(PredSet p, int64_t n, int32_t* tile_counts) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSelTile;
+  int cnt = 0;
+#pragma unroll 4
+  for (int k = 0; k < kSelTile / kBlock; k++) {
+    const int64_t r = base + k * kBlock + threadIdx.x;
+    if (r < n && eval_preds(p, r)) cnt++;
+  }
+  // Wave reduction, then across the workgroup's waves.
+  for (int s = kWave / 2; s >= 1; s >>= 1) cnt += __shfl_xor(cnt, s);
+  __shared__ int wsum[kBlock / kWave];
+  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kBlock / kWave; w++) t += wsum[w];
+    tile_counts[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of ntiles counts by one workgroup; total written to *total.
+__global__ __launch_bounds__(1024) void k_exclusive_scan(const int32_t* in, int64_t* out, int64_t m,
+                                                         int64_t* total) {
+  __shared__ int64_t partial[1024];
+  const int64_t per = (m + 1023) / 1024;
+  const int64_t lo = threadIdx.x * per;
+  const int64_t hi = lo + per < m ? lo + per : m;
+  int64_t s = 0;
+  for (int64_t i = lo; i < hi; i++) s += in[i];
+  partial[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int64_t v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0;
+    __syncthreads();
+    partial[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int64_t run = threadIdx.x == 0 ? 0 : partial[threadIdx.x - 1];
+  for (int64_t i = lo; i < hi; i++) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (threadIdx.x == 1023) *total = partial[1023];
+}
+
+__global__ __launch_bounds__(kBlock) void k_select_write(PredSet p, int64_t n, const int64_t* tile_offsets,
+                                                         int32_t* out_rows) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSelTile;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  __shared__ int wcnt[2][kBlock / kWave];
+  int64_t pos = tile_offsets[blockIdx.x];
+  for (int k = 0; k < kSelTile / kBlock; k++) {
+    const int64_t r = base + k * kBlock + threadIdx.x;
+    const bool pass = r < n && eval_preds(p, r);
+    const uint64_t mask = __ballot(pass);
+    const int below = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wcnt[k & 1][wave] = __popcll(mask);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < kBlock / kWave; w++) {
+      const int c = wcnt[k & 1][w];
+      before += w < wave ? c : 0;
+      total += c;
+    }
+    if (pass) out_rows[pos + before + below] = static_cast<int32_t>(r);
+    pos += total;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Aggregation over dense group codes
+
+// One measure column: the aggregate input cols[col] (times Float(cols[col2])
+// when col2 is set).
+struct MeasureSet {
+  int32_t nm;
+  int32_t is_real[kMaxMeasures];  // accumulate as double (else exact int64)
+  int32_t type[kMaxMeasures];
+  int32_t type2[kMaxMeasures];
+  const void* data[kMaxMeasures];
+  const void* data2[kMaxMeasures];  // optional second factor
+  const int8_t* nulls[kMaxMeasures];
+  const int8_t* nulls2[kMaxMeasures];
+};
+
+struct KeySet {
+  int32_t nk;
+  const int32_t* data[2];
+  const int8_t* nulls[2];
+  int32_t card1;  // group = code0 * card1 + code1
+};
+
+__device__ __forceinline__ int group_of(const KeySet& k, int64_t r) {
+  if (k.nk == 0) return 0;
+  if (k.nk == 1) return k.data[0][r];
+  return k.data[0][r] * k.card1 + k.data[1][r];
+}
+
+// Loads measure m of row r: bits of a double (is_real) or an int64, and its NULL flag.
+__device__ __forceinline__ uint64_t load_measure(const MeasureSet& M, int m, int64_t r, bool& null) {
+  null = (M.nulls[m] && M.nulls[m][r]) || (M.data2[m] && M.nulls2[m] && M.nulls2[m][r]);
+  if (M.is_real[m]) {
+    double x = M.type[m] == KG_REAL ? static_cast<const double*>(M.data[m])[r]
+                                    : static_cast<double>(load_int(M.type[m], M.data[m], r));
+    if (M.data2[m]) {
+      double y = M.type2[m] == KG_REAL ? static_cast<const double*>(M.data2[m])[r]
+                                       : static_cast<double>(load_int(M.type2[m], M.data2[m], r));
+      x = x * y;
+    }
+    return static_cast<uint64_t>(__double_as_longlong(x));
+  }
+  return static_cast<uint64_t>(load_int(M.type[m], M.data[m], r));
+}
+
+// Per-workgroup output record for group g: counts[g], first row, and per
+// measure: sum (double bits or int64), non-NULL count, min, max.
+struct AggOut {
+  uint64_t* sum;     // [nblocks][G][nm]
+  uint64_t* mn;      // [nblocks][G][nm] (minmax only)
+  uint64_t* mx;      // [nblocks][G][nm] (minmax only)
+  uint32_t* nn;      // [nblocks][G][nm] non-NULL counts
+  uint32_t* rows;    // [nblocks][G]
+  int32_t* first;    // [G] atomicMin across workgroups (init INT32_MAX)
+};
+
+__device__ __forceinline__ uint64_t fadd_bits(uint64_t a, uint64_t b) {
+  return static_cast<uint64_t>(__double_as_longlong(__longlong_as_double(a) + __longlong_as_double(b)));
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int s) {
+  const uint32_t lo = __shfl_xor(static_cast<uint32_t>(v), s);
+  const uint32_t hi = __shfl_xor(static_cast<uint32_t>(v >> 32), s);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Order-preserving min/max on the 8-byte value (double or int64).
+__device__ __forceinline__ bool val_less(bool is_real, uint64_t a, uint64_t b) {
+  return is_real ? __longlong_as_double(a) < __longlong_as_double(b)
+                 : static_cast<int64_t>(a) < static_cast<int64_t>(b);
+}
+
+// NN: some measure is nullable, so non-NULL counts are kept per measure
+// (otherwise every measure's count is the group's row count).
+// G is the register budget (compile time); Grt <= G is the number of dense
+// groups, which sizes the per-workgroup output records.
+template <int G, bool MINMAX, bool NN>
+__global__ __launch_bounds__(kBlock) void k_group_agg_reg(PredSet P, KeySet K, MeasureSet M, int64_t n, int Grt,
+                                                           AggOut out) {
+  uint64_t sum[G][kMaxMeasures];
+  uint64_t mn[MINMAX ? G : 1][MINMAX ? kMaxMeasures : 1];
+  uint64_t mx[MINMAX ? G : 1][MINMAX ? kMaxMeasures : 1];
+  uint32_t nn[NN ? G : 1][NN ? kMaxMeasures : 1];
+  uint32_t rows[G];
+  int32_t first[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    rows[g] = 0;
+    first[g] = INT32_MAX;
+#pragma unroll
+    for (int m = 0; m < kMaxMeasures; m++) {
+      sum[g][m] = 0;  // +0.0 and int64 0 share the bit pattern
+      if constexpr (NN) nn[g][m] = 0;
+      if constexpr (MINMAX) {
+        mn[g][m] = 0;
+        mx[g][m] = 0;
+      }
+    }
+  }
+  const int64_t T = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; r < n; r += T) {
+    if (!eval_preds(P, r)) continue;
+    const int grp = group_of(K, r);
+    if (grp < 0 || grp >= Grt) continue;  // codes outside [0, card) are not groups
+    uint64_t v[kMaxMeasures];
+    bool vnull[kMaxMeasures];
+#pragma unroll
+    for (int m = 0; m < kMaxMeasures; m++)
+      if (m < M.nm) v[m] = load_measure(M, m, r, vnull[m]);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (grp != g) continue;
+      const bool first_row = rows[g] == 0;
+      rows[g]++;
+      first[g] = min(first[g], static_cast<int32_t>(r));
+#pragma unroll
+      for (int m = 0; m < kMaxMeasures; m++) {
+        if (m >= M.nm || vnull[m]) continue;
+        bool fresh = first_row;
+        if constexpr (NN) {
+          fresh = nn[g][m] == 0;
+          nn[g][m]++;
+        }
+        sum[g][m] = M.is_real[m] ? fadd_bits(sum[g][m], v[m]) : sum[g][m] + v[m];
+        if constexpr (MINMAX) {
+          if (fresh || val_less(M.is_real[m], v[m], mn[g][m])) mn[g][m] = v[m];
+          if (fresh || val_less(M.is_real[m], mx[g][m], v[m])) mx[g][m] = v[m];
+        }
+      }
+    }
+  }
+
+  // Fixed-order workgroup reduction: lanes by p[l] += p[l ^ s], s = 32..1
+  // (every lane ends with the same tree result), then waves by the same tree.
+  __shared__ uint64_t wred[kBlock / kWave][G * kMaxMeasures * (MINMAX ? 3 : 1)];
+  __shared__ uint32_t wcnt[kBlock / kWave][G * (kMaxMeasures + 1)];
+  __shared__ int32_t wfirst[kBlock / kWave][G];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+#pragma unroll
+    for (int m = 0; m < kMaxMeasures; m++) {
+      if (m >= M.nm) continue;
+      uint64_t s = sum[g][m];
+      uint32_t c = NN ? nn[g][m < kMaxMeasures ? m : 0] : rows[g];
+      uint64_t lo = MINMAX ? mn[g][m] : 0, hi = MINMAX ? mx[g][m] : 0;
+      for (int st = kWave / 2; st >= 1; st >>= 1) {
+        const uint64_t os = shfl_xor_u64(s, st);
+        const uint32_t oc = __shfl_xor(c, st);
+        // Tree order: the lower lane of each pair is the left operand.
+        if (M.is_real[m]) {
+          s = (lane & st) ? fadd_bits(os, s) : fadd_bits(s, os);
+        } else {
+          s = s + os;
+        }
+        if constexpr (MINMAX) {
+          const uint64_t olo = shfl_xor_u64(lo, st), ohi = shfl_xor_u64(hi, st);
+          if (oc > 0 && (c == 0 || val_less(M.is_real[m], olo, lo))) lo = olo;
+          if (oc > 0 && (c == 0 || val_less(M.is_real[m], hi, ohi))) hi = ohi;
+        }
+        c += oc;
+      }
+      if (lane == 0) {
+        wred[wave][(g * kMaxMeasures + m) * (MINMAX ? 3 : 1)] = s;
+        if constexpr (MINMAX) {
+          wred[wave][(g * kMaxMeasures + m) * 3 + 1] = lo;
+          wred[wave][(g * kMaxMeasures + m) * 3 + 2] = hi;
+        }
+        wcnt[wave][g * (kMaxMeasures + 1) + m] = c;
+      }
+    }
+    uint32_t rc = rows[g];
+    int32_t fr = first[g];
+    for (int st = kWave / 2; st >= 1; st >>= 1) {
+      rc += __shfl_xor(rc, st);
+      fr = min(fr, __shfl_xor(fr, st));
+    }
+    if (lane == 0) {
+      wcnt[wave][g * (kMaxMeasures + 1) + kMaxMeasures] = rc;
+      wfirst[wave][g] = fr;
+    }
+  }
+  __syncthreads();
+  constexpr int NW = kBlock / kWave;
+  for (int idx = threadIdx.x; idx < G * kMaxMeasures; idx += kBlock) {
+    const int g = idx / kMaxMeasures, m = idx % kMaxMeasures;
+    if (m >= M.nm || g >= Grt) continue;
+    const int stride = MINMAX ? 3 : 1;
+    uint64_t p[NW], plo[NW], phi[NW];
+    uint32_t pc[NW];
+    for (int w = 0; w < NW; w++) {
+      p[w] = wred[w][idx * stride];
+      pc[w] = wcnt[w][g * (kMaxMeasures + 1) + m];
+      if constexpr (MINMAX) {
+        plo[w] = wred[w][idx * 3 + 1];
+        phi[w] = wred[w][idx * 3 + 2];
+      }
+    }
+    for (int st = NW / 2; st >= 1; st >>= 1) {
+      for (int w = 0; w < st; w++) {
+        p[w] = M.is_real[m] ? fadd_bits(p[w], p[w + st]) : p[w] + p[w + st];
+        if constexpr (MINMAX) {
+          if (pc[w + st] > 0 && (pc[w] == 0 || val_less(M.is_real[m], plo[w + st], plo[w]))) plo[w] = plo[w + st];
+          if (pc[w + st] > 0 && (pc[w] == 0 || val_less(M.is_real[m], phi[w], phi[w + st]))) phi[w] = phi[w + st];
+        }
+        pc[w] += pc[w + st];
+      }
+    }
+    const int64_t o = (static_cast<int64_t>(blockIdx.x) * Grt + g) * M.nm + m;
+    out.sum[o] = p[0];
+    out.nn[o] = pc[0];
+    if constexpr (MINMAX) {
+      out.mn[o] = plo[0];
+      out.mx[o] = phi[0];
+    }
+  }
+  for (int g = threadIdx.x; g < Grt; g += kBlock) {
+    uint32_t rc = 0;
+    int32_t fr = INT32_MAX;
+    for (int w = 0; w < NW; w++) {
+      rc += wcnt[w][g * (kMaxMeasures + 1) + kMaxMeasures];
+      fr = min(fr, wfirst[w][g]);
+    }
+    out.rows[static_cast<int64_t>(blockIdx.x) * Grt + g] = rc;
+    if (fr != INT32_MAX) atomicMin(&out.first[g], fr);
+  }
+}
+
+// Adds the per-workgroup partials in workgroup order. One thread per
+// (group, measure); thread G*nm + g handles the row count of group g.
+struct AggFinal {
+  uint64_t sum, mn, mx;
+  uint64_t nn;
+};
+
+__global__ void k_group_agg_final(const AggOut in, int nblocks, int G, MeasureSet M, bool minmax,
+                                  AggFinal* fin, uint64_t* rows_out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nm = M.nm;
+  if (idx < G * nm) {
+    const int g = idx / nm, m = idx % nm;
+    uint64_t s = 0, lo = 0, hi = 0, c = 0;
+    for (int b = 0; b < nblocks; b++) {
+      const int64_t o = (static_cast<int64_t>(b) * G + g) * nm + m;
+      const uint32_t bc = in.nn[o];
+      if (bc == 0) continue;
+      s = M.is_real[m] ? fadd_bits(s, in.sum[o]) : s + in.sum[o];
+      if (minmax) {
+        if (c == 0 || val_less(M.is_real[m], in.mn[o], lo)) lo = in.mn[o];
+        if (c == 0 || val_less(M.is_real[m], hi, in.mx[o])) hi = in.mx[o];
+      }
+      c += bc;
+    }
+    fin[idx] = AggFinal{s, lo, hi, c};
+  } else if (idx < G * nm + G) {
+    const int g = idx - G * nm;
+    uint64_t rc = 0;
+    for (int b = 0; b < nblocks; b++) rc += in.rows[static_cast<int64_t>(b) * G + g];
+    rows_out[g] = rc;
+  }
+}
+
+// Workgroup-private LDS table for up to kMaxLdsGroups groups (any order of
+// FP additions: not reproducible bit for bit, within FP64 rounding of it).
+template <bool MINMAX>
+__global__ __launch_bounds__(kBlock) void k_group_agg_lds(PredSet P, KeySet K, MeasureSet M, int64_t n,
+                                                           int G, AggFinal* gfin, unsigned long long* grows,
+                                                           int32_t* gfirst) {
+  extern __shared__ unsigned long long lds[];
+  const int nm = M.nm;
+  unsigned long long* lsum = lds;                        // [G][nm]
+  unsigned long long* lnn = lsum + G * nm;               // [G][nm]
+  unsigned long long* lrows = lnn + G * nm;              // [G]
+  unsigned long long* lfirst = lrows + G;                // [G]
+  unsigned long long* lmn = lfirst + G;                  // [G][nm] (MINMAX)
+  unsigned long long* lmx = lmn + (MINMAX ? G * nm : 0); // [G][nm] (MINMAX)
+  const int words = G * nm * 2 + G * 2 + (MINMAX ? 2 * G * nm : 0);
+  for (int i = threadIdx.x; i < words; i += kBlock) lds[i] = 0;
+  for (int i = threadIdx.x; i < G; i += kBlock) lfirst[i] = INT32_MAX;
+  if (MINMAX)
+    for (int i = threadIdx.x; i < G * nm; i += kBlock) {
+      lmn[i] = M.is_real[i % nm] ? static_cast<unsigned long long>(__double_as_longlong(__builtin_inf()))
+                                 : static_cast<unsigned long long>(INT64_MAX);
+      lmx[i] = M.is_real[i % nm] ? static_cast<unsigned long long>(__double_as_longlong(-__builtin_inf()))
+                                 : static_cast<unsigned long long>(INT64_MIN);
+    }
+  __syncthreads();
+  const int64_t T = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; r < n; r += T) {
+    if (!eval_preds(P, r)) continue;
+    const int g = group_of(K, r);
+    if (g < 0 || g >= G) continue;  // codes outside [0, card) are not groups
+    atomicAdd(&lrows[g], 1ull);
+    atomicMin(&lfirst[g], static_cast<unsigned long long>(r));
+    for (int m = 0; m < nm; m++) {
+      bool vn;
+      const uint64_t v = load_measure(M, m, r, vn);
+      if (vn) continue;
+      atomicAdd(&lnn[g * nm + m], 1ull);
+      if (M.is_real[m]) {
+        atomicAdd(reinterpret_cast<double*>(&lsum[g * nm + m]), __longlong_as_double(v));
+        if (MINMAX) {
+          atomicMin(reinterpret_cast<double*>(&lmn[g * nm + m]), __longlong_as_double(v));
+          atomicMax(reinterpret_cast<double*>(&lmx[g * nm + m]), __longlong_as_double(v));
+        }
+      } else {
+        atomicAdd(&lsum[g * nm + m], static_cast<unsigned long long>(v));
+        if (MINMAX) {
+          atomicMin(reinterpret_cast<long long*>(&lmn[g * nm + m]), static_cast<long long>(v));
+          atomicMax(reinterpret_cast<long long*>(&lmx[g * nm + m]), static_cast<long long>(v));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < G; i += kBlock) {
+    if (lrows[i] == 0) continue;
+    atomicAdd(&grows[i], lrows[i]);
+    atomicMin(&gfirst[i], static_cast<int32_t>(lfirst[i]));
+  }
+  for (int i = threadIdx.x; i < G * nm; i += kBlock) {
+    if (lnn[i] == 0) continue;
+    const int m = i % nm;
+    AggFinal* f = &gfin[i];
+    atomicAdd(reinterpret_cast<unsigned long long*>(&f->nn), lnn[i]);
+    if (M.is_real[m]) {
+      atomicAdd(reinterpret_cast<double*>(&f->sum), __longlong_as_double(lsum[i]));
+      if (MINMAX) {
+        atomicMin(reinterpret_cast<double*>(&f->mn), __longlong_as_double(lmn[i]));
+        atomicMax(reinterpret_cast<double*>(&f->mx), __longlong_as_double(lmx[i]));
+      }
+    } else {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&f->sum), lsum[i]);
+      if (MINMAX) {
+        atomicMin(reinterpret_cast<long long*>(&f->mn), static_cast<long long>(lmn[i]));
+        atomicMax(reinterpret_cast<long long*>(&f->mx), static_cast<long long>(lmx[i]));
+      }
+    }
+  }
+}
+
+__global__ void k_init_lds_final(AggFinal* fin, int G, MeasureSet M, unsigned long long* rows, int32_t* first) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nm = M.nm;
+  if (i < G * nm) {
+    const bool real = M.is_real[i % nm];
+    fin[i].sum = 0;
+    fin[i].nn = 0;
+    fin[i].mn = real ? static_cast<uint64_t>(__double_as_longlong(__builtin_inf())) : static_cast<uint64_t>(INT64_MAX);
+    fin[i].mx = real ? static_cast<uint64_t>(__double_as_longlong(-__builtin_inf())) : static_cast<uint64_t>(INT64_MIN);
+  }
+  if (i < G) {
+    rows[i] = 0;
+    first[i] = INT32_MAX;
+  }
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// For COUNT(col): whether each group's first row has a NULL in `nulls`.
+__global__ void k_first_null(const int32_t* first, int G, const int8_t* nulls, int8_t* out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G) out[g] = (first[g] != INT32_MAX && nulls) ? nulls[first[g]] : 0;
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+namespace {
+
+int build_preds(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_pred* preds, int32_t npreds,
+                int32_t conj, PredSet& p) {
+  KG_REQUIRE(ctx, npreds >= 0 && npreds <= kMaxPreds, "at most %d predicates (got %d)", kMaxPreds, npreds);
+  KG_REQUIRE(ctx, npreds == 0 || preds, "preds is NULL");
+  std::memset(&p, 0, sizeof(p));
+  p.n = npreds;
+  p.conj = conj ? 1 : 0;
+  for (int i = 0; i < npreds; i++) {
+    const kg_pred& q = preds[i];
+    KG_REQUIRE(ctx, q.col >= 0 && q.col < ncols, "predicate %d references column %d of %d", i, q.col, ncols);
+    const kg_col& c = cols[q.col];
+    KG_REQUIRE(ctx, c.data, "predicate column %d has no data", q.col);
+    if (c.type == KG_TEXT) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "TEXT predicates are not supported here");
+    KG_REQUIRE(ctx, type_width(c.type) > 0, "predicate column %d has bad type %d", q.col, c.type);
+    KG_REQUIRE(ctx, q.op >= KG_EQ && q.op <= KG_GEQ, "bad comparison op %d", q.op);
+    p.type[i] = c.type;
+    p.op[i] = q.op;
+    p.data[i] = c.data;
+    p.nulls[i] = c.nulls;
+    p.ilit[i] = q.ilit;
+    p.flit[i] = q.flit;
+  }
+  return KG_OK;
+}
+
+}  // namespace
+
+extern "C" int kg_scan_select(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
+                              const kg_pred* preds, int32_t npreds, int32_t conjunction,
+                              int32_t* d_out_rows, int64_t* out_count) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count %lld out of range", (long long)n);
+  KG_REQUIRE(ctx, out_count, "out_count is NULL");
+  KG_REQUIRE(ctx, n == 0 || d_out_rows, "d_out_rows is NULL");
+  PredSet p;
+  int st = build_preds(ctx, cols, ncols, preds, npreds, conjunction, p);
+  if (st != KG_OK) return st;
+  *out_count = 0;
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const int64_t ntiles = ceil_div(n, kSelTile);
+  const size_t need = ntiles * 4 + ntiles * 8 + 64;
+  if ((st = kg_work_reserve(ctx, need)) != KG_OK) return st;
+  if ((st = kg_pinned_reserve(ctx, 64)) != KG_OK) return st;
+  int32_t* counts = static_cast<int32_t*>(ctx->work);
+  int64_t* offsets = reinterpret_cast<int64_t*>(static_cast<char*>(ctx->work) + ((ntiles * 4 + 7) / 8) * 8);
+  int64_t* total = offsets + ntiles;
+  hipLaunchKernelGGL(k_select_count, dim3(ntiles), dim3(kBlock), 0, ctx->stream, p, n, counts);
+  KG_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, ctx->stream, counts, offsets, ntiles, total);
+  KG_LAUNCH_CHECK(ctx);
+  kg_time_begin(ctx);
+  hipLaunchKernelGGL(k_select_write, dim3(ntiles), dim3(kBlock), 0, ctx->stream, p, n, offsets, d_out_rows);
+  kg_time_end(ctx);
+  KG_LAUNCH_CHECK(ctx);
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, total, 8, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *out_count = *static_cast<int64_t*>(ctx->pinned);
+  return KG_OK;
+}
+
+namespace {
+
+// Host view of one requested aggregate.
+struct AggPlan {
+  int32_t kind;
+  int32_t measure;   // index into MeasureSet, -1 for COUNT(*)
+  int32_t out_type;  // SQL type of the result
+  const int8_t* count_nulls;  // COUNT(col): the column's NULL array
+};
+
+template <int G, bool MM>
+void launch_reg(hipStream_t s, int nb, const PredSet& P, const KeySet& K, const MeasureSet& M, int64_t n,
+                int Grt, const AggOut& o) {
+  bool nullable = false;
+  for (int m = 0; m < M.nm; m++) nullable |= M.nulls[m] != nullptr || M.nulls2[m] != nullptr;
+  if (nullable)
+    hipLaunchKernelGGL((k_group_agg_reg<G, MM, true>), dim3(nb), dim3(kBlock), 0, s, P, K, M, n, Grt, o);
+  else
+    hipLaunchKernelGGL((k_group_agg_reg<G, MM, false>), dim3(nb), dim3(kBlock), 0, s, P, K, M, n, Grt, o);
+}
+
+bool launch_reg_any(int G, bool mm, hipStream_t s, int nb, const PredSet& P, const KeySet& K,
+                    const MeasureSet& M, int64_t n, const AggOut& o) {
+  if (mm) {
+    if (G <= 1) launch_reg<1, true>(s, nb, P, K, M, n, G, o);
+    else if (G <= 2) launch_reg<2, true>(s, nb, P, K, M, n, G, o);
+    else if (G <= 4) launch_reg<4, true>(s, nb, P, K, M, n, G, o);
+    else return false;
+  } else {
+    if (G <= 1) launch_reg<1, false>(s, nb, P, K, M, n, G, o);
+    else if (G <= 2) launch_reg<2, false>(s, nb, P, K, M, n, G, o);
+    else if (G <= 4) launch_reg<4, false>(s, nb, P, K, M, n, G, o);
+    else if (G <= 6) launch_reg<6, false>(s, nb, P, K, M, n, G, o);
+    else if (G <= 8) launch_reg<8, false>(s, nb, P, K, M, n, G, o);
+    else return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+// Number of workgroups of the fixed-order aggregation for n rows; the CPU
+// oracle's deterministic restatement takes the same value.
+extern "C" int kg_agg_blocks_for(int64_t n) {
+  int64_t nb = ceil_div(n, static_cast<int64_t>(kBlock) * kAggRowsPerThreadMin);
+  if (nb < 1) nb = 1;
+  if (nb > kAggMaxBlocks) nb = kAggMaxBlocks;
+  return static_cast<int>(nb);
+}
+
+extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
+                                 const kg_pred* preds, int32_t npreds, const int32_t* key_cols,
+                                 const int32_t* key_card, int32_t nkeys, const kg_agg* aggs, int32_t naggs,
+                                 int64_t max_groups, int64_t* out_ngroups, int32_t* out_keys,
+                                 int64_t* out_vals, int8_t* out_nulls) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count %lld out of range", (long long)n);
+  KG_REQUIRE(ctx, out_ngroups && (naggs == 0 || (aggs && out_vals && out_nulls)), "null output pointer");
+  KG_REQUIRE(ctx, nkeys >= 0 && nkeys <= 2, "1 or 2 dense key columns (got %d)", nkeys);
+  KG_REQUIRE(ctx, naggs >= 0 && naggs <= 16, "at most 16 aggregates (got %d)", naggs);
+  KG_REQUIRE(ctx, nkeys == 0 || (key_cols && key_card && out_keys), "key arrays are NULL");
+  PredSet P;
+  int st = build_preds(ctx, cols, ncols, preds, npreds, 1, P);
+  if (st != KG_OK) return st;
+
+  KeySet K;
+  std::memset(&K, 0, sizeof(K));
+  K.nk = nkeys;
+  int64_t G = 1;
+  for (int k = 0; k < nkeys; k++) {
+    KG_REQUIRE(ctx, key_cols[k] >= 0 && key_cols[k] < ncols, "bad key column %d", key_cols[k]);
+    const kg_col& c = cols[key_cols[k]];
+    if (c.type != KG_INT && c.type != KG_ENUM && c.type != KG_DATE)
+      return kg_fail(ctx, KG_ERR_UNSUPPORTED, "dense group keys must be 32-bit codes (type %d)", c.type);
+    if (c.nulls) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "nullable dense group keys are not supported");
+    KG_REQUIRE(ctx, key_card[k] > 0, "key cardinality must be positive");
+    K.data[k] = static_cast<const int32_t*>(c.data);
+    G *= key_card[k];
+  }
+  if (nkeys == 2) K.card1 = key_card[1];
+  if (G > kMaxLdsGroups)
+    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "%lld dense groups exceed the on-chip limit %d", (long long)G,
+                   kMaxLdsGroups);
+
+  // Distinct measures and the aggregate plan.
+  MeasureSet M;
+  std::memset(&M, 0, sizeof(M));
+  std::vector<AggPlan> plan(naggs);
+  bool minmax = false;
+  for (int a = 0; a < naggs; a++) {
+    const kg_agg& q = aggs[a];
+    AggPlan& pl = plan[a];
+    pl.kind = q.kind;
+    pl.measure = -1;
+    pl.count_nulls = nullptr;
+    KG_REQUIRE(ctx, q.kind >= KG_AGG_SUM && q.kind <= KG_AGG_COUNT, "bad aggregate kind %d", q.kind);
+    if (q.kind == KG_AGG_COUNT && q.col < 0) {
+      pl.out_type = KG_BIGINT;
+      continue;
+    }
+    KG_REQUIRE(ctx, q.col >= 0 && q.col < ncols, "aggregate %d references column %d", a, q.col);
+    const kg_col& c = cols[q.col];
+    KG_REQUIRE(ctx, c.data, "aggregate column %d has no data", q.col);
+    if (c.type == KG_TEXT || c.type == KG_BOOLEAN)
+      return kg_fail(ctx, KG_ERR_UNSUPPORTED, "aggregate over type %d is not supported here", c.type);
+    if (q.kind == KG_AGG_COUNT) {
+      pl.out_type = KG_BIGINT;
+      pl.count_nulls = c.nulls;
+    } else if (q.kind == KG_AGG_AVG) {
+      pl.out_type = KG_REAL;
+    } else {
+      if (q.kind == KG_AGG_SUM && (c.type == KG_DATE || c.type == KG_ENUM))
+        return kg_fail(ctx, KG_ERR_INVALID, "cannot compute sum of non-numeric col");
+      pl.out_type = c.type;
+    }
+    if (q.kind == KG_AGG_MIN || q.kind == KG_AGG_MAX) minmax = true;
+    if (q.kind == KG_AGG_MIN || q.kind == KG_AGG_MAX) {
+      if (c.type == KG_ENUM)
+        return kg_fail(ctx, KG_ERR_UNSUPPORTED, "MIN/MAX over ENUM compares decoded strings");
+    }
+    const bool real = c.type == KG_REAL;
+    int found = -1;
+    for (int m = 0; m < M.nm; m++)
+      if (M.data[m] == c.data && !M.data2[m]) found = m;
+    if (found < 0) {
+      KG_REQUIRE(ctx, M.nm < kMaxMeasures, "at most %d distinct aggregate inputs", kMaxMeasures);
+      found = M.nm++;
+      M.is_real[found] = real;
+      M.type[found] = c.type;
+      M.data[found] = c.data;
+      M.nulls[found] = c.nulls;
+    }
+    pl.measure = found;
+  }
+
+  *out_ngroups = 0;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const int nm = M.nm;
+  std::vector<AggFinal> fin(G * (nm > 0 ? nm : 1));
+  std::vector<uint64_t> rowsv(G);
+  std::vector<int32_t> firstv(G);
+  std::vector<std::vector<int8_t>> first_null(naggs);
+
+  const bool use_reg = G <= (minmax ? 4 : 8);
+  if (use_reg) {
+    const int nb = kg_agg_blocks_for(n);
+    const size_t per_block = static_cast<size_t>(G) * (nm > 0 ? nm : 1);
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+      size_t o = off;
+      off += (bytes + 255) / 256 * 256;
+      return o;
+    };
+    const size_t o_sum = carve(nb * per_block * 8), o_mn = carve(minmax ? nb * per_block * 8 : 8),
+                 o_mx = carve(minmax ? nb * per_block * 8 : 8), o_nn = carve(nb * per_block * 4),
+                 o_rows = carve(static_cast<size_t>(nb) * G * 4), o_first = carve(G * 4),
+                 o_fin = carve(G * (nm > 0 ? nm : 1) * sizeof(AggFinal)), o_rfin = carve(G * 8),
+                 o_fnull = carve(G * 16 + 16);
+    if ((st = kg_work_reserve(ctx, off)) != KG_OK) return st;
+    char* w = static_cast<char*>(ctx->work);
+    AggOut o;
+    o.sum = reinterpret_cast<uint64_t*>(w + o_sum);
+    o.mn = reinterpret_cast<uint64_t*>(w + o_mn);
+    o.mx = reinterpret_cast<uint64_t*>(w + o_mx);
+    o.nn = reinterpret_cast<uint32_t*>(w + o_nn);
+    o.rows = reinterpret_cast<uint32_t*>(w + o_rows);
+    o.first = reinterpret_cast<int32_t*>(w + o_first);
+    hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(G, 256)), dim3(256), 0, ctx->stream, o.first, INT32_MAX, G);
+    KG_LAUNCH_CHECK(ctx);
+    kg_time_begin(ctx);
+    const bool launched = launch_reg_any(static_cast<int>(G), minmax, ctx->stream, nb, P, K, M, n, o);
+    kg_time_end(ctx);
+    if (!launched)
+      return kg_fail(ctx, KG_ERR_UNSUPPORTED, "internal: no register kernel for %lld groups", (long long)G);
+    KG_LAUNCH_CHECK(ctx);
+    AggFinal* dfin = reinterpret_cast<AggFinal*>(w + o_fin);
+    uint64_t* drows = reinterpret_cast<uint64_t*>(w + o_rfin);
+    const int threads = static_cast<int>(G * nm + G);
+    hipLaunchKernelGGL(k_group_agg_final, dim3(ceil_div(threads, 256)), dim3(256), 0, ctx->stream, o, nb,
+                       static_cast<int>(G), M, minmax, dfin, drows);
+    KG_LAUNCH_CHECK(ctx);
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(fin.data(), dfin, G * nm * sizeof(AggFinal), hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(rowsv.data(), drows, G * 8, hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(firstv.data(), o.first, G * 4, hipMemcpyDeviceToHost, ctx->stream));
+    int8_t* dfn = reinterpret_cast<int8_t*>(w + o_fnull);
+    for (int a = 0; a < naggs; a++) {
+      if (plan[a].kind != KG_AGG_COUNT || !plan[a].count_nulls) continue;
+      first_null[a].resize(G);
+      hipLaunchKernelGGL(k_first_null, dim3(ceil_div(G, 256)), dim3(256), 0, ctx->stream, o.first,
+                         static_cast<int>(G), plan[a].count_nulls, dfn);
+      KG_LAUNCH_CHECK(ctx);
+      KG_HIP_CHECK(ctx, hipMemcpyAsync(first_null[a].data(), dfn, G, hipMemcpyDeviceToHost, ctx->stream));
+      KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  } else {
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+      size_t o = off;
+      off += (bytes + 255) / 256 * 256;
+      return o;
+    };
+    const size_t o_fin = carve(G * (nm > 0 ? nm : 1) * sizeof(AggFinal)), o_rows = carve(G * 8),
+                 o_first = carve(G * 4), o_fnull = carve(G + 16);
+    if ((st = kg_work_reserve(ctx, off)) != KG_OK) return st;
+    char* w = static_cast<char*>(ctx->work);
+    AggFinal* dfin = reinterpret_cast<AggFinal*>(w + o_fin);
+    unsigned long long* drows = reinterpret_cast<unsigned long long*>(w + o_rows);
+    int32_t* dfirst = reinterpret_cast<int32_t*>(w + o_first);
+    const int init_threads = static_cast<int>(std::max<int64_t>(G * nm, G));
+    hipLaunchKernelGGL(k_init_lds_final, dim3(ceil_div(init_threads, 256)), dim3(256), 0, ctx->stream, dfin,
+                       static_cast<int>(G), M, drows, dfirst);
+    KG_LAUNCH_CHECK(ctx);
+    const size_t lds_bytes = (G * nm * 2 + G * 2 + (minmax ? 2 * G * nm : 0)) * 8;
+    KG_REQUIRE(ctx, lds_bytes <= 64 * 1024, "group table of %zu bytes exceeds LDS budget", lds_bytes);
+    int nb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(ceil_div(n, kBlock * 16), 1), ctx->num_cus * 4));
+    kg_time_begin(ctx);
+    if (minmax)
+      hipLaunchKernelGGL(k_group_agg_lds<true>, dim3(nb), dim3(kBlock), lds_bytes, ctx->stream, P, K, M, n,
+                         static_cast<int>(G), dfin, drows, dfirst);
+    else
+      hipLaunchKernelGGL(k_group_agg_lds<false>, dim3(nb), dim3(kBlock), lds_bytes, ctx->stream, P, K, M, n,
+                         static_cast<int>(G), dfin, drows, dfirst);
+    kg_time_end(ctx);
+    KG_LAUNCH_CHECK(ctx);
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(fin.data(), dfin, G * nm * sizeof(AggFinal), hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(rowsv.data(), drows, G * 8, hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(firstv.data(), dfirst, G * 4, hipMemcpyDeviceToHost, ctx->stream));
+    int8_t* dfn = reinterpret_cast<int8_t*>(w + o_fnull);
+    for (int a = 0; a < naggs; a++) {
+      if (plan[a].kind != KG_AGG_COUNT || !plan[a].count_nulls) continue;
+      first_null[a].resize(G);
+      hipLaunchKernelGGL(k_first_null, dim3(ceil_div(G, 256)), dim3(256), 0, ctx->stream, dfirst,
+                         static_cast<int>(G), plan[a].count_nulls, dfn);
+      KG_LAUNCH_CHECK(ctx);
+      KG_HIP_CHECK(ctx, hipMemcpyAsync(first_null[a].data(), dfn, G, hipMemcpyDeviceToHost, ctx->stream));
+      KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+
+  // Groups present, in insertion order (first qualifying row).
+  std::vector<int32_t> present;
+  for (int64_t g = 0; g < G; g++)
+    if (rowsv[g] > 0) present.push_back(static_cast<int32_t>(g));
+  std::sort(present.begin(), present.end(), [&](int32_t a, int32_t b) { return firstv[a] < firstv[b]; });
+  const int64_t ng = static_cast<int64_t>(present.size());
+  *out_ngroups = ng;
+  if (ng > max_groups) return kg_fail(ctx, KG_ERR_CAPACITY, "%lld groups exceed max_groups %lld", (long long)ng, (long long)max_groups);
+  for (int64_t i = 0; i < ng; i++) {
+    const int32_t g = present[i];
+    if (nkeys == 1) out_keys[i] = g;
+    if (nkeys == 2) {
+      out_keys[i * 2] = g / K.card1;
+      out_keys[i * 2 + 1] = g % K.card1;
+    }
+    for (int a = 0; a < naggs; a++) {
+      const AggPlan& pl = plan[a];
+      int64_t& v = out_vals[i * naggs + a];
+      int8_t& vn = out_nulls[i * naggs + a];
+      vn = 0;
+      if (pl.kind == KG_AGG_COUNT) {
+        if (pl.measure < 0) {
+          v = static_cast<int64_t>(rowsv[g]);
+        } else {
+          const AggFinal& f = fin[g * nm + pl.measure];
+          v = static_cast<int64_t>(f.nn) + (first_null[a].empty() ? 0 : first_null[a][g]);
+        }
+        continue;
+      }
+      const AggFinal& f = fin[g * nm + pl.measure];
+      if (f.nn == 0) {
+        vn = 1;
+        v = 0;
+        continue;
+      }
+      const bool real = M.is_real[pl.measure];
+      switch (pl.kind) {
+        case KG_AGG_SUM:
+          if (real) {
+            v = static_cast<int64_t>(f.sum);
+          } else if (pl.out_type == KG_INT) {
+            v = static_cast<int32_t>(static_cast<uint32_t>(f.sum));
+          } else if (pl.out_type == KG_SMALLINT) {
+            v = static_cast<int16_t>(static_cast<uint16_t>(f.sum));
+          } else {
+            v = static_cast<int64_t>(f.sum);
+          }
+          break;
+        case KG_AGG_MIN: v = static_cast<int64_t>(f.mn); break;
+        case KG_AGG_MAX: v = static_cast<int64_t>(f.mx); break;
+        case KG_AGG_AVG: {
+          double s;
+          if (real) {
+            std::memcpy(&s, &f.sum, 8);
+          } else {
+            s = static_cast<double>(static_cast<int64_t>(f.sum));
+          }
+          const double avg = s / static_cast<double>(f.nn);
+          std::memcpy(&v, &avg, 8);
+          break;
+        }
+      }
+    }
+  }
+  return KG_OK;
+}

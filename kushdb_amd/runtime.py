@@ -1,0 +1,224 @@
+"""Python host layer over the kushgpu C ABI.
+
+torch tensors provide device memory and the stream; every operator runs in
+libkushgpu.so (HIP, gfx950). Names follow the reference's operators:
+ScanSelect, HashJoin (runtime::HashTable), GroupByAggregate
+(runtime::AggregateHashTable).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import abi
+from .abi import KgAgg, KgCol, KgError, KgPred
+
+_TORCH_TYPE = {
+    abi.KG_SMALLINT: torch.int16,
+    abi.KG_INT: torch.int32,
+    abi.KG_BIGINT: torch.int64,
+    abi.KG_REAL: torch.float64,
+    abi.KG_DATE: torch.int32,
+    abi.KG_BOOLEAN: torch.int8,
+    abi.KG_ENUM: torch.int32,
+}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+class Column:
+    """A resident column: typed device tensor + optional int8 NULL tensor."""
+
+    def __init__(self, sql_type, data, nulls=None):
+        want = _TORCH_TYPE[sql_type]
+        if data.dtype != want:
+            raise TypeError(f"column of type {sql_type} needs {want}, got {data.dtype}")
+        if not data.is_cuda or not data.is_contiguous():
+            raise ValueError("columns must be contiguous device tensors")
+        if nulls is not None and (nulls.dtype != torch.int8 or nulls.numel() != data.numel() or not nulls.is_cuda):
+            raise ValueError("nulls must be an int8 device tensor of the column's length")
+        self.type = sql_type
+        self.data = data
+        self.nulls = nulls
+
+    def __len__(self):
+        return self.data.numel()
+
+    def desc(self):
+        return KgCol(self.type, 0, self.data.data_ptr(),
+                     self.nulls.data_ptr() if self.nulls is not None else 0)
+
+
+class Context:
+    """One device + stream (kg_ctx). Work is ordered on torch's current stream."""
+
+    def __init__(self, device=0, deterministic=True):
+        self.lib = abi.load()
+        h = ctypes.c_void_p()
+        st = self.lib.kg_ctx_create(device, ctypes.byref(h))
+        if st != abi.KG_OK:
+            raise KgError(st, "kg_ctx_create failed (is a GPU visible?)")
+        self.h = h
+        self.device = device
+        self.check(self.lib.kg_ctx_set_stream(self.h, ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)))
+        self.check(self.lib.kg_ctx_set_deterministic(self.h, 1 if deterministic else 0))
+
+    def check(self, st):
+        if st != abi.KG_OK:
+            raise KgError(st, self.lib.kg_last_error(self.h).decode())
+        return st
+
+    def close(self):
+        if self.h:
+            self.lib.kg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        self.check(self.lib.kg_ctx_synchronize(self.h))
+
+    def set_timing(self, on):
+        self.check(self.lib.kg_ctx_set_timing(self.h, 1 if on else 0))
+
+    def kernel_stats(self, reset=True):
+        """(summed ms, launches) of the dominant kernels since the last reset."""
+        ms = ctypes.c_double()
+        k = ctypes.c_int64()
+        self.check(self.lib.kg_ctx_kernel_stats(self.h, ctypes.byref(ms), ctypes.byref(k), 1 if reset else 0))
+        return ms.value, k.value
+
+    # -- ScanSelect -------------------------------------------------------
+    def scan_select(self, cols, preds, conjunction=True, out=None):
+        """Row ids (int32, ascending) passing `preds` = [(col_idx, op, literal)]."""
+        n = len(cols[0]) if cols else 0
+        carr = (KgCol * max(len(cols), 1))(*[c.desc() for c in cols])
+        parr = (KgPred * max(len(preds), 1))(*[_pred(cols, p) for p in preds])
+        if out is None:
+            out = torch.empty(max(n, 1), dtype=torch.int32, device=f"cuda:{self.device}")
+        cnt = ctypes.c_int64()
+        self.check(self.lib.kg_scan_select(self.h, n, carr, len(cols), parr, len(preds), 1 if conjunction else 0,
+                                           _ptr(out), ctypes.byref(cnt)))
+        return out[: cnt.value]
+
+    # -- ScanSelect -> GroupByAggregate over dense code keys ---------------
+    def scan_group_agg(self, cols, preds, keys, aggs, max_groups=4096):
+        """keys = [(col_idx, cardinality)], aggs = [(kind, col_idx)].
+
+        Returns (codes int32 [G, nkeys], values int64 [G, naggs], nulls int8 [G, naggs])
+        in insertion order (first qualifying row of each group)."""
+        n = len(cols[0]) if cols else 0
+        carr = (KgCol * max(len(cols), 1))(*[c.desc() for c in cols])
+        parr = (KgPred * max(len(preds), 1))(*[_pred(cols, p) for p in preds])
+        kc = (ctypes.c_int32 * max(len(keys), 1))(*[k[0] for k in keys])
+        kd = (ctypes.c_int32 * max(len(keys), 1))(*[k[1] for k in keys])
+        aarr = (KgAgg * max(len(aggs), 1))(*[KgAgg(k, c) for k, c in aggs])
+        codes = np.zeros((max_groups, max(len(keys), 1)), dtype=np.int32)
+        vals = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int64)
+        nulls = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int8)
+        ng = ctypes.c_int64()
+        self.check(self.lib.kg_scan_group_agg(
+            self.h, n, carr, len(cols), parr, len(preds), kc, kd, len(keys), aarr, len(aggs), max_groups,
+            ctypes.byref(ng), codes.ctypes.data_as(abi._pi32), vals.ctypes.data_as(abi._pi64),
+            nulls.ctypes.data_as(abi._pi8)))
+        g = ng.value
+        return codes[:g, : len(keys)], vals[:g, : len(aggs)], nulls[:g, : len(aggs)]
+
+    # -- GroupByAggregate on a BIGINT key (high cardinality) ----------------
+    def group_by_i64(self, keys, vals, max_groups=None, with_first=False):
+        n = keys.numel()
+        if max_groups is None:
+            max_groups = max(n, 1)
+        dev = keys.device
+        ok = torch.empty(max_groups, dtype=torch.int64, device=dev)
+        osum = torch.empty(max_groups, dtype=torch.int64, device=dev)
+        ocnt = torch.empty(max_groups, dtype=torch.int64, device=dev)
+        ofirst = torch.empty(max_groups, dtype=torch.int32, device=dev) if with_first else None
+        ng = ctypes.c_int64()
+        self.check(self.lib.kg_group_by_i64(self.h, _ptr(keys), _ptr(vals), n, max_groups, _ptr(ok), _ptr(osum),
+                                            _ptr(ocnt), _ptr(ofirst), ctypes.byref(ng)))
+        g = ng.value
+        return ok[:g], osum[:g], ocnt[:g], (ofirst[:g] if with_first else None)
+
+
+def _pred(cols, p):
+    ci, op, lit = p
+    if cols[ci].type == abi.KG_REAL:
+        return KgPred(ci, op, 0, float(lit))
+    return KgPred(ci, op, int(lit), 0.0)
+
+
+class HashTable:
+    """HBM open-addressing join table (replaces runtime::HashTable)."""
+
+    def __init__(self, ctx, capacity_rows, load_factor=0.5):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.kg_ht_create(ctx.h, capacity_rows, load_factor, ctypes.byref(h)))
+        self.h = h
+
+    def insert(self, key_col, rows=None, nrows=None):
+        """Build side: insert key_col's rows (or the selected `rows`)."""
+        if nrows is None:
+            nrows = rows.numel() if rows is not None else len(key_col)
+        d = key_col.desc()
+        self.ctx.check(self.ctx.lib.kg_ht_build(self.ctx.h, self.h, ctypes.byref(d), _ptr(rows), nrows))
+
+    def probe(self, key_col, rows=None, out_cap=None):
+        """All (probe row, build row) pairs with equal keys (unordered)."""
+        nrows = rows.numel() if rows is not None else len(key_col)
+        if out_cap is None:
+            out_cap = self.probe_count(key_col, rows)
+        dev = key_col.data.device
+        op = torch.empty(max(out_cap, 1), dtype=torch.int32, device=dev)
+        ob = torch.empty(max(out_cap, 1), dtype=torch.int32, device=dev)
+        cnt = ctypes.c_int64()
+        d = key_col.desc()
+        self.ctx.check(self.ctx.lib.kg_ht_probe(self.ctx.h, self.h, ctypes.byref(d), _ptr(rows), nrows, _ptr(op),
+                                                _ptr(ob), out_cap, ctypes.byref(cnt)))
+        return op[: cnt.value], ob[: cnt.value]
+
+    def probe_count(self, key_col, rows=None):
+        nrows = rows.numel() if rows is not None else len(key_col)
+        cnt = ctypes.c_int64()
+        d = key_col.desc()
+        self.ctx.check(self.ctx.lib.kg_ht_probe_count(self.ctx.h, self.h, ctypes.byref(d), _ptr(rows), nrows,
+                                                      ctypes.byref(cnt)))
+        return cnt.value
+
+    def probe_agg(self, key_col, vals, max_groups, with_first=False):
+        """GROUP BY matched build key: (key, SUM(vals), COUNT(*)[, first probe row])."""
+        dev = key_col.data.device
+        ok = torch.empty(max(max_groups, 1), dtype=torch.int64, device=dev)
+        osum = torch.empty(max(max_groups, 1), dtype=torch.int64, device=dev)
+        ocnt = torch.empty(max(max_groups, 1), dtype=torch.int64, device=dev)
+        ofirst = torch.empty(max(max_groups, 1), dtype=torch.int32, device=dev) if with_first else None
+        ng = ctypes.c_int64()
+        d = key_col.desc()
+        self.ctx.check(self.ctx.lib.kg_ht_probe_agg(self.ctx.h, self.h, ctypes.byref(d), _ptr(vals), len(key_col),
+                                                    max_groups, _ptr(ok), _ptr(osum), _ptr(ocnt), _ptr(ofirst),
+                                                    ctypes.byref(ng)))
+        g = ng.value
+        return ok[:g], osum[:g], ocnt[:g], (ofirst[:g] if with_first else None)
+
+    def size(self):
+        v = ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.kg_ht_size(self.ctx.h, self.h, ctypes.byref(v)))
+        return v.value
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.kg_ht_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,922 @@
+// ORACLE — test infrastructure, not product code. Loaded only by tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker.
+//
+// CPU restatement of KushDB's pipeline-operator semantics. Each function cites
+// the reference file:line it restates. How each part is pinned independently
+// of its own formula:
+//   kor_hash_combine   - against the reference's own runtime/hash_table.cc
+//                        compiled into oracle/_ref (tests/test_oracle_ref.py)
+//                        and hand-computed known answers.
+//   kor_agg_hash       - known answers computed by hand from the published
+//                        MurmurHash64A constants; result-invisible (placement).
+//   kor_build/split_date - reference date_test.cc known answers, round trip
+//                        over 1900..2100 against Python's datetime ordinal, and
+//                        oracle/_ref.
+//   kor_scan_select    - reference fixtures scan_select/*, simd_scan_select/*
+//                        (expected .tbl) and numpy boolean masks.
+//   kor_hash_join      - reference fixtures hash_join/* and a nested-loop join
+//                        on small inputs.
+//   kor_group_by /
+//   kor_aggregate      - reference fixtures group_by_aggregate/int,
+//                        aggregate/{int,empty}, tpch/q01; np.add.at/bincount
+//                        for integer SUM/COUNT; a Python left fold for FP64
+//                        SUM and the running-mean AVG on small inputs.
+//   kor_fast_forward   - reference column_index_bucket_test.cc known answers,
+//                        np.searchsorted, and oracle/_ref.
+//   kor_intersect_*    - oracle/_ref (reference column_index_bucket.cc) and
+//                        np.intersect1d on sorted unique lists.
+//   kor_tuple_dedup    - reference tuple_idx_table_test.cc known answers
+//                        (dedup + byte-lexicographic order), and oracle/_ref.
+//   kor_memory_index_build - np.argsort(kind="stable") grouping.
+//   kor_kdbindex_get   - the reference fixture .kdbindex files against a
+//                        linear scan of the matching .kdb column.
+//   kor_q_*            - compositions of the above, checked against numpy.
+//   kor_dtree_scan_agg - the GPU deterministic order; checked against math.fsum
+//                        (|err| bound) and against kor_q_scan_agg (1e-12 rel).
+#include "kush_oracle.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+#include <string>
+
+namespace {
+
+// Reads the integer value of a non-REAL column (sign-extending), as the
+// reference's typed ColumnData::Get* functions do (runtime/column_data.cc:98-108).
+inline int64_t load_int(const kor_col& c, int64_t i) {
+  switch (c.type) {
+    case KOR_SMALLINT: return static_cast<const int16_t*>(c.data)[i];
+    case KOR_INT:
+    case KOR_DATE:
+    case KOR_ENUM: return static_cast<const int32_t*>(c.data)[i];
+    case KOR_BIGINT: return static_cast<const int64_t*>(c.data)[i];
+    case KOR_BOOLEAN: return static_cast<const int8_t*>(c.data)[i];
+    default: return 0;
+  }
+}
+inline double load_real(const kor_col& c, int64_t i) { return static_cast<const double*>(c.data)[i]; }
+inline bool is_null(const kor_col& c, int64_t i) { return c.nulls && c.nulls[i]; }
+
+// 8-byte "bits" representation of a value: integers sign-extended, REAL as IEEE bits.
+inline int64_t load_bits(const kor_col& c, int64_t i) {
+  if (c.type == KOR_REAL) {
+    int64_t b;
+    double d = load_real(c, i);
+    std::memcpy(&b, &d, 8);
+    return b;
+  }
+  return load_int(c, i);
+}
+
+inline double bits_to_f64(int64_t b) {
+  double d;
+  std::memcpy(&d, &b, 8);
+  return d;
+}
+inline int64_t f64_to_bits(double d) {
+  int64_t b;
+  std::memcpy(&b, &d, 8);
+  return b;
+}
+
+// Equal(lhs, rhs) of compile/proxy/evaluate.cc:404-468: NULL == NULL is true,
+// NULL vs non-NULL is false, otherwise typed equality.
+inline bool sql_equal(const kor_col& a, int64_t i, const kor_col& b, int64_t j) {
+  bool an = is_null(a, i), bn = is_null(b, j);
+  if (an && bn) return true;
+  if (an || bn) return false;
+  if (a.type == KOR_REAL) return load_real(a, i) == load_real(b, j);
+  return load_int(a, i) == load_int(b, j);
+}
+
+}  // namespace
+
+extern "C" {
+
+// runtime/hash_table.cc:65-67: *hash ^= v + 0x9e3779b9 + (*hash << 6) + (*hash >> 2),
+// evaluated in int64 and stored back into an int32, i.e. all arithmetic mod 2^32
+// with an arithmetic right shift of the signed 32-bit hash.
+void kor_hash_combine(int32_t* hash, int64_t v) {
+  const uint32_t h = static_cast<uint32_t>(*hash);
+  const uint32_t shl = h << 6;
+  const uint32_t sar = static_cast<uint32_t>(*hash >> 2);
+  const uint32_t mix = static_cast<uint32_t>(v) + 0x9e3779b9u + shl + sar;
+  *hash = static_cast<int32_t>(h ^ mix);
+}
+
+// Per-type SQLValue::Hash(): zext for SMALLINT/INT/DATE/ENUM/BOOLEAN
+// (compile/proxy/value/int.cc:227,347-349; date.cc:68; enum.cc:58-60; bool.cc:52),
+// identity for BIGINT (int.cc:487), fptosi truncation for REAL (float.cc:136-138).
+int64_t kor_value_hash(int32_t type, int64_t bits) {
+  switch (type) {
+    case KOR_SMALLINT: return static_cast<int64_t>(static_cast<uint16_t>(bits));
+    case KOR_INT:
+    case KOR_DATE:
+    case KOR_ENUM: return static_cast<int64_t>(static_cast<uint32_t>(bits));
+    case KOR_BOOLEAN: return static_cast<int64_t>(bits & 1);
+    case KOR_BIGINT: return bits;
+    case KOR_REAL: {
+      double d = bits_to_f64(bits);
+      // fptosi: out-of-range is undefined in IR; x86 cvttsd2si yields INT64_MIN.
+      if (!(d > -9.2233720368547758e18 && d < 9.2233720368547758e18)) return INT64_MIN;
+      return static_cast<int64_t>(d);
+    }
+    default: return 0;
+  }
+}
+
+// compile/proxy/aggregate_hash_table.cc:313-333 (NULL key hashes as 0); all
+// products wrap mod 2^64 and >> is a logical shift (khir I64_RSHIFT lowers to
+// LShr, khir/llvm/llvm_backend.cc:743-750).
+int64_t kor_agg_hash(const int64_t* key_hashes, const int8_t* key_nulls, int32_t nkeys) {
+  const uint64_t m = 0xc6a4a7935bd1e995ull;
+  uint64_t h = 0;
+  for (int32_t i = 0; i < nkeys; i++) {
+    uint64_t k = (key_nulls && key_nulls[i]) ? 0 : static_cast<uint64_t>(key_hashes[i]);
+    k *= m;
+    k ^= k >> 47;
+    k *= m;
+    h ^= k;
+    h *= m;
+    h += 0xe6546b64ull;
+  }
+  return static_cast<int64_t>(h);
+}
+
+// runtime/column_index.cc:29-42: murmur-style mix with the top bit set.
+uint64_t kor_index_hash(uint64_t k) {
+  const uint64_t m = 0xc6a4a7935bd1e995ull;
+  const int r = 47;
+  uint64_t h = 0x8445d61a4e774912ull ^ (8 * m);
+  k *= m;
+  k ^= k >> r;
+  k *= m;
+  h ^= k;
+  h *= m;
+  h ^= h >> r;
+  h *= m;
+  h ^= h >> r;
+  return h | (1ull << 63);
+}
+
+// runtime/date.cc:10-25 (Julian day number from a civil date).
+int32_t kor_build_date(int32_t y, int32_t m, int32_t d) {
+  uint32_t year = y, month = m, day = d;
+  if (month > 2) {
+    month += 1;
+    year += 4800;
+  } else {
+    month += 13;
+    year += 4799;
+  }
+  int32_t century = year / 100;
+  int32_t julian = year * 365 - 32167;
+  julian += year / 4 - century + century / 4;
+  julian += 7834 * month / 256 + day;
+  return julian;
+}
+
+// runtime/date.cc:29-45 (civil date from a Julian day number).
+void kor_split_date(int32_t jd, int32_t* year, int32_t* month, int32_t* day) {
+  uint32_t julian = jd;
+  julian += 32044;
+  uint32_t quad = julian / 146097;
+  uint32_t extra = (julian - quad * 146097) * 4 + 3;
+  julian += 60 + quad * 3 + extra / 146097;
+  quad = julian / 1461;
+  julian -= quad * 1461;
+  int32_t y = julian * 4 / 1461;
+  julian = ((y != 0) ? ((julian + 305) % 365) : ((julian + 306) % 366)) + 123;
+  y += quad * 4;
+  *year = y - 4800;
+  quad = julian * 2141 / 65536;
+  *day = julian - 7834 * quad / 256;
+  *month = (quad + 10) % 12 + 1;
+}
+
+// Scan-select: compile/translators/scan_select_translator.cc:92-156 evaluates
+// each conjunct per row and rejects on NULL or false (:129-133).
+// SimdScanSelect (simd_scan_select_translator.cc:94-532) combines the per-column
+// int32 compares with AND (conjunction) or OR (:444-454); conjunction == 0
+// selects that disjunctive form. The surviving row ids are emitted in ascending
+// order, which is the order the 8-wide permutation compaction produces
+// (util/permute.cc:10-25 lists set bits low to high).
+int64_t kor_scan_select(int64_t n, const kor_col* cols, int32_t npreds, const kor_pred* preds,
+                        int32_t conjunction, int32_t* out_rows) {
+  int64_t out = 0;
+  for (int64_t i = 0; i < n; i++) {
+    bool pass = conjunction ? true : (npreds == 0);
+    for (int32_t p = 0; p < npreds; p++) {
+      const kor_pred& pr = preds[p];
+      const kor_col& c = cols[pr.col];
+      bool v;
+      if (is_null(c, i)) {
+        v = false;
+      } else if (c.type == KOR_REAL) {
+        double x = load_real(c, i), l = pr.flit;
+        switch (pr.op) {
+          case KOR_EQ: v = x == l; break;
+          case KOR_NEQ: v = x != l; break;
+          case KOR_LT: v = x < l; break;
+          case KOR_LEQ: v = x <= l; break;
+          case KOR_GT: v = x > l; break;
+          default: v = x >= l; break;
+        }
+      } else {
+        int64_t x = load_int(c, i), l = pr.ilit;
+        switch (pr.op) {
+          case KOR_EQ: v = x == l; break;
+          case KOR_NEQ: v = x != l; break;
+          case KOR_LT: v = x < l; break;
+          case KOR_LEQ: v = x <= l; break;
+          case KOR_GT: v = x > l; break;
+          default: v = x >= l; break;
+        }
+      }
+      if (conjunction) {
+        if (!v) { pass = false; break; }
+      } else if (v) {
+        pass = true;
+        break;
+      }
+    }
+    if (pass) out_rows[out++] = static_cast<int32_t>(i);
+  }
+  return out;
+}
+
+// Hash join.
+// Build (compile/translators/hash_join_translator.cc:85-103): rows with any
+// NULL key are skipped; the hash is HashCombine over the keys' Hash() starting
+// from 0 (compile/proxy/hash_table.cc:117-136); runtime::HashTable::Insert
+// appends the row to the bucket vector of that 32-bit hash
+// (runtime/hash_table.cc:18-27).
+// Probe (hash_join_translator.cc:105-139; proxy/hash_table.cc:138-157): a NULL
+// probe key hashes as 0; GetBucket returns the vector for the hash; every entry
+// is compared on all keys with Equal() and emitted in bucket insertion order.
+// build_rows / probe_rows (nullable) give the surviving row ids of each side in
+// pipeline order; pairs are emitted in probe order x bucket order.
+int64_t kor_hash_join(int32_t nkeys, const kor_col* build_keys, int64_t nb, const int32_t* build_rows,
+                      const kor_col* probe_keys, int64_t np, const int32_t* probe_rows,
+                      int64_t out_cap, int32_t* out_probe, int32_t* out_build) {
+  std::unordered_map<int32_t, std::vector<int32_t>> table;
+  for (int64_t bi = 0; bi < nb; bi++) {
+    const int64_t r = build_rows ? build_rows[bi] : bi;
+    bool any_null = false;
+    for (int32_t k = 0; k < nkeys; k++) any_null |= is_null(build_keys[k], r);
+    if (any_null) continue;
+    int32_t h = 0;
+    for (int32_t k = 0; k < nkeys; k++)
+      kor_hash_combine(&h, kor_value_hash(build_keys[k].type, load_bits(build_keys[k], r)));
+    table[h].push_back(static_cast<int32_t>(r));
+  }
+  int64_t out = 0;
+  for (int64_t pi = 0; pi < np; pi++) {
+    const int64_t r = probe_rows ? probe_rows[pi] : pi;
+    int32_t h = 0;
+    for (int32_t k = 0; k < nkeys; k++) {
+      const int64_t v = is_null(probe_keys[k], r)
+                            ? 0
+                            : kor_value_hash(probe_keys[k].type, load_bits(probe_keys[k], r));
+      kor_hash_combine(&h, v);
+    }
+    auto it = table.find(h);
+    if (it == table.end()) continue;
+    for (int32_t b : it->second) {
+      bool eq = true;
+      for (int32_t k = 0; k < nkeys && eq; k++) eq = sql_equal(build_keys[k], b, probe_keys[k], r);
+      if (!eq) continue;
+      if (out < out_cap) {
+        out_probe[out] = static_cast<int32_t>(r);
+        out_build[out] = b;
+      }
+      out++;
+    }
+  }
+  return out;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One aggregator's state: compile/proxy/aggregator.cc. value holds the running
+// SUM/MIN/MAX/AVG value (8-byte bits), cnt the AVG record count (a double,
+// :213-218) or the COUNT value.
+struct AggState {
+  int64_t value;
+  bool null;
+  double avg_count;
+  int64_t count;
+};
+
+// Wrapping addition in the SUM's own type (aggregator.cc:44-71: SUM returns
+// the child type, plan/expression/aggregate_expression.cc:21-27).
+inline int64_t wrap_add(int32_t type, int64_t a, int64_t b) {
+  switch (type) {
+    case KOR_SMALLINT:
+      return static_cast<int16_t>(static_cast<uint16_t>(a) + static_cast<uint16_t>(b));
+    case KOR_INT:
+      return static_cast<int32_t>(static_cast<uint32_t>(a) + static_cast<uint32_t>(b));
+    default:
+      return static_cast<int64_t>(static_cast<uint64_t>(a) + static_cast<uint64_t>(b));
+  }
+}
+
+// AverageAggregator::ToFloat (aggregator.cc:221-243).
+inline double to_float(const kor_col& c, int64_t r) {
+  return c.type == KOR_REAL ? load_real(c, r) : static_cast<double>(load_int(c, r));
+}
+
+// Initialize (aggregator.cc:27-29, 98-100, 205-219, 285-287).
+void agg_init(const kor_agg& a, int64_t r, AggState& s) {
+  const bool cnt_star = a.input.data == nullptr;
+  const bool in_null = !cnt_star && is_null(a.input, r);
+  switch (a.kind) {
+    case KOR_AGG_SUM:
+    case KOR_AGG_MIN:
+    case KOR_AGG_MAX:
+      s.null = in_null;
+      s.value = in_null ? 0 : load_bits(a.input, r);
+      break;
+    case KOR_AGG_AVG:
+      s.null = in_null;
+      s.value = in_null ? f64_to_bits(0.0) : f64_to_bits(to_float(a.input, r));
+      s.avg_count = 1.0;
+      break;
+    case KOR_AGG_COUNT:
+      // CountAggregator::Initialize stores 1 without looking at NULL (:285-287).
+      s.null = false;
+      s.count = 1;
+      break;
+  }
+}
+
+// Update (aggregator.cc:31-80 SUM, 102-171 MIN/MAX, 245-268 AVG, 289-296 COUNT).
+void agg_update(const kor_agg& a, int64_t r, AggState& s) {
+  const bool cnt_star = a.input.data == nullptr;
+  if (!cnt_star && is_null(a.input, r)) return;
+  switch (a.kind) {
+    case KOR_AGG_SUM: {
+      const int64_t nx = load_bits(a.input, r);
+      if (s.null) {
+        s.value = nx;
+        s.null = false;
+      } else if (a.input.type == KOR_REAL) {
+        s.value = f64_to_bits(bits_to_f64(s.value) + bits_to_f64(nx));
+      } else {
+        s.value = wrap_add(a.input.type, s.value, nx);
+      }
+      break;
+    }
+    case KOR_AGG_MIN:
+    case KOR_AGG_MAX: {
+      const bool is_min = a.kind == KOR_AGG_MIN;
+      const int64_t nx = load_bits(a.input, r);
+      if (s.null) {
+        s.value = nx;
+        s.null = false;
+      } else if (a.input.type == KOR_REAL) {
+        const double v1 = bits_to_f64(nx), v2 = bits_to_f64(s.value);
+        if (is_min ? v1 < v2 : v2 < v1) s.value = nx;
+      } else {
+        if (is_min ? nx < s.value : s.value < nx) s.value = nx;
+      }
+      break;
+    }
+    case KOR_AGG_AVG: {
+      const double x = to_float(a.input, r);
+      if (s.null) {
+        s.value = f64_to_bits(x);
+        s.null = false;
+      } else {
+        const double next_count = s.avg_count + 1;
+        const double cma = bits_to_f64(s.value);
+        s.avg_count = next_count;
+        s.value = f64_to_bits(cma + (x - cma) / next_count);
+      }
+      break;
+    }
+    case KOR_AGG_COUNT:
+      s.count += 1;
+      break;
+  }
+}
+
+void agg_emit(const kor_agg& a, const AggState& s, int64_t* v, int8_t* vn) {
+  if (a.kind == KOR_AGG_COUNT) {
+    *v = s.count;
+    *vn = 0;
+  } else {
+    *v = s.value;
+    *vn = s.null;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Group-by aggregate: the salted linear-probing directory of
+// runtime/aggregate_hash_table.cc:9-135 driven by the generated
+// UpdateOrInsert loop of compile/proxy/aggregate_hash_table.cc:187-243:
+//   resize first when size == capacity or size > capacity / 1.5 (:188-201);
+//   hash = Hash(keys) (:313-333), salt = hash >> 48 (:335-337);
+//   probe from hash & mask: an empty entry (block index 0) inserts a new payload
+//   and Initializes it; a salt match followed by CheckEq on every key updates.
+// Payloads are addressed by their insertion ordinal (the reference's
+// (block_idx, block_offset) pair enumerates payloads in that same order, and
+// ForEach (:245-291) emits groups in it). Resize (runtime :59-91) re-inserts
+// payloads in ordinal order from their stored hash.
+// rows (nullable) lists the input row ids in pipeline order.
+// out_keys/out_key_nulls: [group][key]; out_vals/out_val_nulls: [group][agg]
+// with 8-byte values (integers sign-extended, REAL as IEEE bits).
+int64_t kor_group_by(int64_t n, const int32_t* rows, int32_t nkeys, const kor_col* keys,
+                     int32_t naggs, const kor_agg* aggs, int64_t max_groups,
+                     int64_t* out_keys, int8_t* out_key_nulls, int64_t* out_vals,
+                     int8_t* out_val_nulls) {
+  uint64_t capacity = 1024, mask = capacity - 1;
+  std::vector<uint64_t> entries(capacity, 0);  // salt << 48 | ordinal + 1
+  std::vector<uint64_t> hashes;                 // payload-stored hash
+  std::vector<int64_t> first_row;               // row whose keys the payload holds
+  std::vector<AggState> states;
+  std::vector<int64_t> key_hash(nkeys);
+  std::vector<int8_t> key_null(nkeys);
+
+  for (int64_t ii = 0; ii < n; ii++) {
+    const int64_t r = rows ? rows[ii] : ii;
+    const uint64_t size = hashes.size();
+    if (size == capacity || static_cast<double>(size) > static_cast<double>(capacity) / 1.5) {
+      capacity *= 2;
+      mask = capacity - 1;
+      std::vector<uint64_t> ne(capacity, 0);
+      for (uint64_t g = 0; g < hashes.size(); g++) {
+        uint64_t idx = hashes[g] & mask;
+        while ((ne[idx] & 0xFFFFFFFFull) > 0) idx = (idx + 1) & mask;
+        ne[idx] = ((hashes[g] >> 48) << 48) | (g + 1);
+      }
+      entries.swap(ne);
+    }
+    for (int32_t k = 0; k < nkeys; k++) {
+      key_null[k] = is_null(keys[k], r);
+      key_hash[k] = key_null[k] ? 0 : kor_value_hash(keys[k].type, load_bits(keys[k], r));
+    }
+    const uint64_t hash = static_cast<uint64_t>(kor_agg_hash(key_hash.data(), key_null.data(), nkeys));
+    const uint64_t salt = hash >> 48;
+    uint64_t idx = hash & mask;
+    while (true) {
+      const uint64_t e = entries[idx];
+      if ((e & 0xFFFFFFFFull) == 0) {
+        const uint64_t g = hashes.size();
+        entries[idx] = (salt << 48) | (g + 1);
+        hashes.push_back(hash);
+        first_row.push_back(r);
+        for (int32_t a = 0; a < naggs; a++) {
+          AggState s{};
+          agg_init(aggs[a], r, s);
+          states.push_back(s);
+        }
+        break;
+      }
+      if ((e >> 48) == salt) {
+        const uint64_t g = (e & 0xFFFFFFFFull) - 1;
+        bool eq = true;
+        for (int32_t k = 0; k < nkeys && eq; k++) eq = sql_equal(keys[k], first_row[g], keys[k], r);
+        if (eq) {
+          for (int32_t a = 0; a < naggs; a++) agg_update(aggs[a], r, states[g * naggs + a]);
+          break;
+        }
+      }
+      idx = (idx + 1) & mask;
+    }
+  }
+
+  const int64_t ng = static_cast<int64_t>(hashes.size());
+  for (int64_t g = 0; g < ng && g < max_groups; g++) {
+    for (int32_t k = 0; k < nkeys; k++) {
+      out_key_nulls[g * nkeys + k] = is_null(keys[k], first_row[g]);
+      out_keys[g * nkeys + k] = load_bits(keys[k], first_row[g]);
+    }
+    for (int32_t a = 0; a < naggs; a++)
+      agg_emit(aggs[a], states[g * naggs + a], &out_vals[g * naggs + a], &out_val_nulls[g * naggs + a]);
+  }
+  return ng;
+}
+
+// Ungrouped aggregate (compile/translators/aggregate_translator.cc:34-126): the
+// first row Initializes, later rows Update; with no input rows every output is
+// NULL (:82-88). Returns 1 when a row was seen, 0 for empty input.
+int32_t kor_aggregate(int64_t n, const int32_t* rows, int32_t naggs, const kor_agg* aggs,
+                      int64_t* out_vals, int8_t* out_val_nulls) {
+  if (n == 0) {
+    for (int32_t a = 0; a < naggs; a++) {
+      out_vals[a] = 0;
+      out_val_nulls[a] = 1;
+    }
+    return 0;
+  }
+  std::vector<AggState> st(naggs);
+  for (int64_t ii = 0; ii < n; ii++) {
+    const int64_t r = rows ? rows[ii] : ii;
+    for (int32_t a = 0; a < naggs; a++) {
+      if (ii == 0) agg_init(aggs[a], r, st[a]);
+      else agg_update(aggs[a], r, st[a]);
+    }
+  }
+  for (int32_t a = 0; a < naggs; a++) agg_emit(aggs[a], st[a], &out_vals[a], &out_val_nulls[a]);
+  return 1;
+}
+
+// runtime/column_index_bucket.cc:33-57: lower_bound by binary search; INT32_MAX
+// for an empty bucket and when every element is < prev_tuple.
+int32_t kor_fast_forward(const int32_t* data, int32_t size, int32_t prev_tuple) {
+  if (size == 0) return INT32_MAX;
+  int start = 0, end = size - 1;
+  int32_t next_greater_idx = INT32_MAX;
+  while (start <= end) {
+    const int mid = (start + end) / 2;
+    if (data[mid] < prev_tuple) {
+      start = mid + 1;
+    } else {
+      next_greater_idx = mid;
+      end = mid - 1;
+    }
+  }
+  return next_greater_idx;
+}
+
+// runtime/column_index_bucket.cc:90-182 (BucketListSortedIntersectionPopulateResult):
+// one chunk of at most result_max ids of the k-way intersection, advancing state.
+int32_t kor_intersect_chunk(int32_t nlists, const int32_t* const* lists, const int32_t* sizes,
+                            int32_t* state, int32_t* result, int32_t result_max) {
+  if (nlists == 0) return 0;
+  if (nlists == 1) {
+    int32_t rs = 0;
+    while (state[0] < sizes[0] && rs < result_max) result[rs++] = lists[0][state[0]++];
+    return rs;
+  }
+  if (nlists == 2) {
+    int32_t rs = 0;
+    while (state[0] < sizes[0] && state[1] < sizes[1] && rs < result_max) {
+      const int32_t left = lists[0][state[0]], right = lists[1][state[1]];
+      if (left < right) {
+        state[0] = kor_fast_forward(lists[0], sizes[0], right);
+      } else if (right < left) {
+        state[1] = kor_fast_forward(lists[1], sizes[1], left);
+      } else {
+        result[rs++] = left;
+        state[0]++;
+        state[1]++;
+      }
+    }
+    return rs;
+  }
+  for (int32_t i = 0; i < nlists; i++)
+    if (state[i] >= sizes[i]) return 0;
+  int32_t min_list = 0;
+  for (int32_t i = 1; i < nlists; i++) {
+    if (sizes[i] - state[i] < sizes[min_list] - state[min_list]) min_list = i;
+  }
+  int32_t rs = 0;
+  while (state[min_list] < sizes[min_list]) {
+    if (rs == result_max) break;
+    const int32_t candidate = lists[min_list][state[min_list]];
+    bool all_equal = true;
+    for (int32_t i = 0; i < nlists; i++) {
+      if (i == min_list) continue;
+      state[i] = kor_fast_forward(lists[i], sizes[i], candidate);
+      if (state[i] >= sizes[i]) return rs;
+      const int32_t current = lists[i][state[i]];
+      if (current > candidate) {
+        all_equal = false;
+        state[min_list] = kor_fast_forward(lists[min_list], sizes[min_list], current);
+        break;
+      }
+    }
+    if (all_equal) {
+      result[rs++] = candidate;
+      state[min_list]++;
+    }
+  }
+  return rs;
+}
+
+// The full intersection as the Skinner table functions drain it
+// (compile/translators/permutable_skinner_join_translator.cc:435-472): Init
+// fast-forwards every list to next_tuple (column_index_bucket.cc:59-65), then
+// chunks of `chunk` ids are pulled until a chunk comes back empty.
+int64_t kor_intersect_all(int32_t nlists, const int32_t* const* lists, const int32_t* sizes,
+                          int32_t next_tuple, int32_t chunk, int32_t* out, int64_t out_cap) {
+  std::vector<int32_t> state(nlists);
+  for (int32_t i = 0; i < nlists; i++) state[i] = kor_fast_forward(lists[i], sizes[i], next_tuple);
+  std::vector<int32_t> buf(chunk);
+  int64_t total = 0;
+  while (true) {
+    const int32_t got = kor_intersect_chunk(nlists, lists, sizes, state.data(), buf.data(), chunk);
+    if (got == 0) break;
+    for (int32_t i = 0; i < got; i++) {
+      if (total < out_cap) out[total] = buf[i];
+      total++;
+    }
+  }
+  return total;
+}
+
+// runtime/tuple_idx_table/tuple_idx_table.cc:19-99: Insert returns false for a
+// key already present; keys are the raw bytes of the int32 array and iteration
+// is byte-lexicographic over them (key.cc:19-41, tuple_idx_table_test.cc:28-71).
+// is_new[i] = Insert(tuples[i]); sorted_out receives the distinct tuples in
+// iteration order. Returns the number of distinct tuples.
+int64_t kor_tuple_dedup(int64_t n, int32_t len, const int32_t* tuples, uint8_t* is_new,
+                        int32_t* sorted_out) {
+  const size_t bytes = static_cast<size_t>(len) * 4;
+  std::unordered_set<std::string> seen;
+  std::vector<std::string> keys;
+  for (int64_t i = 0; i < n; i++) {
+    std::string k(reinterpret_cast<const char*>(tuples + i * len), bytes);
+    const bool fresh = seen.insert(k).second;
+    if (is_new) is_new[i] = fresh;
+    if (fresh) keys.push_back(std::move(k));
+  }
+  std::sort(keys.begin(), keys.end(), [&](const std::string& a, const std::string& b) {
+    return std::memcmp(a.data(), b.data(), bytes) < 0;
+  });
+  if (sorted_out)
+    for (size_t i = 0; i < keys.size(); i++) std::memcpy(sorted_out + i * len, keys[i].data(), bytes);
+  return static_cast<int64_t>(keys.size());
+}
+
+// MemoryColumnIndex: compile/proxy/memory_column_index.cc:332-388 appends each
+// tuple id to its key's bucket in scan order, so each bucket is ascending;
+// GetBucket (:282-330) returns it. Output is CSR: keys in first-appearance
+// order, offsets[k]..offsets[k+1] into tids.
+int64_t kor_memory_index_build(int64_t n, const int32_t* keys, const int32_t* rows, int64_t max_keys,
+                               int32_t* out_keys, int64_t* out_offsets, int32_t* out_tids) {
+  std::unordered_map<int32_t, int64_t> id;
+  std::vector<int32_t> order;
+  std::vector<std::vector<int32_t>> buckets;
+  for (int64_t i = 0; i < n; i++) {
+    const int32_t r = rows ? rows[i] : static_cast<int32_t>(i);
+    auto it = id.find(keys[r]);
+    int64_t b;
+    if (it == id.end()) {
+      b = static_cast<int64_t>(buckets.size());
+      id.emplace(keys[r], b);
+      order.push_back(keys[r]);
+      buckets.emplace_back();
+    } else {
+      b = it->second;
+    }
+    buckets[b].push_back(r);
+  }
+  const int64_t nk = static_cast<int64_t>(buckets.size());
+  if (nk <= max_keys) {
+    int64_t off = 0;
+    for (int64_t b = 0; b < nk; b++) {
+      out_keys[b] = order[b];
+      out_offsets[b] = off;
+      std::memcpy(out_tids + off, buckets[b].data(), buckets[b].size() * 4);
+      off += static_cast<int64_t>(buckets[b].size());
+    }
+    out_offsets[nk] = off;
+  }
+  return nk;
+}
+
+// runtime/column_index.cc:73-95 (GetImpl): walk the chain at
+// array[Hash(ToUInt64(key)) & mask]; entries are {K key; u64 size; u64 next;
+// int32 values[]} with K stored at its natural width and alignment
+// (runtime/column_index.h:22-34). ToUInt64 zero-extends integers and bit-copies
+// doubles (:44-65).
+int64_t kor_kdbindex_get(const uint8_t* file, int32_t key_type, int64_t key_bits,
+                         int32_t* out, int64_t out_cap) {
+  uint64_t mask;
+  std::memcpy(&mask, file, 8);
+  uint64_t k64;
+  size_t ksz;
+  switch (key_type) {
+    case KOR_SMALLINT: k64 = static_cast<uint16_t>(key_bits); ksz = 2; break;
+    case KOR_BOOLEAN: k64 = static_cast<uint8_t>(key_bits); ksz = 1; break;
+    case KOR_INT:
+    case KOR_DATE:
+    case KOR_ENUM: k64 = static_cast<uint32_t>(key_bits); ksz = 4; break;
+    default: k64 = static_cast<uint64_t>(key_bits); ksz = 8; break;
+  }
+  uint64_t pos;
+  std::memcpy(&pos, file + 8 + 8 * (kor_index_hash(k64) & mask), 8);
+  while (pos > 0) {
+    const uint8_t* e = file + pos;
+    bool match = std::memcmp(e, &key_bits, ksz) == 0;  // little-endian low bytes
+    uint64_t size, next;
+    std::memcpy(&size, e + 8, 8);
+    std::memcpy(&next, e + 16, 8);
+    if (match) {
+      for (uint64_t i = 0; i < size && static_cast<int64_t>(i) < out_cap; i++)
+        std::memcpy(out + i, e + 24 + 4 * i, 4);
+      return static_cast<int64_t>(size);
+    }
+    pos = next;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Workload compositions: the operator chains the CPU baseline legs time, built
+// from the restatements above but with the column types fixed, so the hot loop
+// is what the reference's generated code would run per row.
+
+// Scan-select (day <= cutoff) -> GroupByAggregate(g1, g2; SUM and AVG of every
+// measure, COUNT(*)). Per group, out holds 2 + 2*nf + 2*ni + 1 8-byte values:
+// g1, g2, SUM(f)[nf] (f64 bits), AVG(f)[nf], SUM(i)[ni] (int32, sign-extended),
+// AVG(i)[ni], COUNT. Groups in insertion order.
+int64_t kor_q_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
+                       const int32_t* g2, int32_t nf, const double* const* fm, int32_t ni,
+                       const int32_t* const* im, int64_t max_groups, int64_t* out) {
+  std::vector<kor_col> keys = {{KOR_ENUM, g1, nullptr}, {KOR_ENUM, g2, nullptr}};
+  std::vector<kor_agg> aggs;
+  for (int32_t f = 0; f < nf; f++) aggs.push_back({KOR_AGG_SUM, {KOR_REAL, fm[f], nullptr}});
+  for (int32_t f = 0; f < nf; f++) aggs.push_back({KOR_AGG_AVG, {KOR_REAL, fm[f], nullptr}});
+  for (int32_t i = 0; i < ni; i++) aggs.push_back({KOR_AGG_SUM, {KOR_INT, im[i], nullptr}});
+  for (int32_t i = 0; i < ni; i++) aggs.push_back({KOR_AGG_AVG, {KOR_INT, im[i], nullptr}});
+  aggs.push_back({KOR_AGG_COUNT, {KOR_INT, nullptr, nullptr}});
+  kor_col dcol{KOR_DATE, day, nullptr};
+  kor_pred p{0, KOR_LEQ, cutoff, 0.0};
+  std::vector<int32_t> rows(n);
+  const int64_t ns = kor_scan_select(n, &dcol, 1, &p, 1, rows.data());
+  const int32_t na = static_cast<int32_t>(aggs.size());
+  std::vector<int64_t> ok(max_groups * 2), ov(max_groups * na);
+  std::vector<int8_t> okn(max_groups * 2), ovn(max_groups * na);
+  const int64_t ng = kor_group_by(ns, rows.data(), 2, keys.data(), na, aggs.data(), max_groups,
+                                  ok.data(), okn.data(), ov.data(), ovn.data());
+  const int64_t w = 2 + na;
+  for (int64_t g = 0; g < ng && g < max_groups; g++) {
+    out[g * w + 0] = ok[g * 2 + 0];
+    out[g * w + 1] = ok[g * 2 + 1];
+    for (int32_t a = 0; a < na; a++) out[g * w + 2 + a] = ov[g * na + a];
+  }
+  return ng;
+}
+
+// HashJoin(build keys -> probe keys) on unique BIGINT build keys: the number of
+// joined pairs and the wrapping int64 sum of the probe-side value over them.
+int64_t kor_q_join_count_sum(int64_t nb, const int64_t* bkeys, int64_t np, const int64_t* pkeys,
+                             const int64_t* pvals, int64_t* out_matches, int64_t* out_sum) {
+  std::unordered_map<int32_t, std::vector<int64_t>> table;
+  table.reserve(nb);
+  for (int64_t i = 0; i < nb; i++) {
+    int32_t h = 0;
+    kor_hash_combine(&h, bkeys[i]);
+    table[h].push_back(bkeys[i]);
+  }
+  int64_t matches = 0;
+  uint64_t sum = 0;
+  for (int64_t j = 0; j < np; j++) {
+    int32_t h = 0;
+    kor_hash_combine(&h, pkeys[j]);
+    auto it = table.find(h);
+    if (it == table.end()) continue;
+    for (int64_t k : it->second) {
+      if (k == pkeys[j]) {
+        matches++;
+        sum += static_cast<uint64_t>(pvals[j]);
+      }
+    }
+  }
+  *out_matches = matches;
+  *out_sum = static_cast<int64_t>(sum);
+  return matches;
+}
+
+// HashJoin (unique BIGINT build keys) -> GroupByAggregate(matched key;
+// SUM(probe value) BIGINT, COUNT(*)). Groups in insertion order.
+int64_t kor_q_join_agg(int64_t nb, const int64_t* bkeys, int64_t np, const int64_t* pkeys,
+                       const int64_t* pvals, int64_t max_groups, int64_t* out_key,
+                       int64_t* out_sum, int64_t* out_cnt) {
+  std::unordered_map<int32_t, std::vector<int64_t>> table;
+  table.reserve(nb);
+  for (int64_t i = 0; i < nb; i++) {
+    int32_t h = 0;
+    kor_hash_combine(&h, bkeys[i]);
+    table[h].push_back(bkeys[i]);
+  }
+  std::vector<int64_t> mrow;
+  mrow.reserve(np);
+  std::vector<int64_t> mkey;
+  mkey.reserve(np);
+  for (int64_t j = 0; j < np; j++) {
+    int32_t h = 0;
+    kor_hash_combine(&h, pkeys[j]);
+    auto it = table.find(h);
+    if (it == table.end()) continue;
+    for (int64_t k : it->second)
+      if (k == pkeys[j]) {
+        mrow.push_back(j);
+        mkey.push_back(k);
+      }
+  }
+  const int64_t nm = static_cast<int64_t>(mrow.size());
+  std::vector<int64_t> vals(nm);
+  for (int64_t i = 0; i < nm; i++) vals[i] = pvals[mrow[i]];
+  kor_col key{KOR_BIGINT, mkey.data(), nullptr};
+  kor_agg aggs[2] = {{KOR_AGG_SUM, {KOR_BIGINT, vals.data(), nullptr}},
+                     {KOR_AGG_COUNT, {KOR_INT, nullptr, nullptr}}};
+  std::vector<int64_t> ok(max_groups), ov(max_groups * 2);
+  std::vector<int8_t> okn(max_groups), ovn(max_groups * 2);
+  const int64_t ng = kor_group_by(nm, nullptr, 1, &key, 2, aggs, max_groups, ok.data(), okn.data(),
+                                  ov.data(), ovn.data());
+  for (int64_t g = 0; g < ng && g < max_groups; g++) {
+    out_key[g] = ok[g];
+    out_sum[g] = ov[g * 2];
+    out_cnt[g] = ov[g * 2 + 1];
+  }
+  return ng;
+}
+
+// Star query: ScanSelect(fact: dlo <= day < dhi) -> HashJoin(build = dim on pk,
+// probe = fact on fk) -> GroupByAggregate(cat; SUM(price * Float(qty)),
+// COUNT(*)). Groups in insertion order (the caller applies ORDER BY).
+int64_t kor_q_star(int64_t nf, const int32_t* fk, const int32_t* qty, const double* price,
+                   const int32_t* day, int32_t dlo, int32_t dhi, int64_t nd, const int32_t* pk,
+                   const int32_t* cat, int64_t max_groups, int32_t* out_cat, double* out_sum,
+                   int64_t* out_cnt) {
+  kor_col dcol{KOR_DATE, day, nullptr};
+  kor_pred preds[2] = {{0, KOR_GEQ, dlo, 0.0}, {0, KOR_LT, dhi, 0.0}};
+  std::vector<int32_t> rows(nf);
+  const int64_t ns = kor_scan_select(nf, &dcol, 2, preds, 1, rows.data());
+  kor_col bk{KOR_INT, pk, nullptr}, prk{KOR_INT, fk, nullptr};
+  std::vector<int32_t> op(ns + 16), ob(ns + 16);
+  int64_t nj = kor_hash_join(1, &bk, nd, nullptr, &prk, ns, rows.data(), ns + 16, op.data(), ob.data());
+  if (nj > ns + 16) {
+    op.resize(nj);
+    ob.resize(nj);
+    nj = kor_hash_join(1, &bk, nd, nullptr, &prk, ns, rows.data(), nj, op.data(), ob.data());
+  }
+  std::vector<int32_t> jcat(nj);
+  std::vector<double> jval(nj);
+  for (int64_t i = 0; i < nj; i++) {
+    jcat[i] = cat[ob[i]];
+    jval[i] = price[op[i]] * static_cast<double>(qty[op[i]]);
+  }
+  kor_col key{KOR_ENUM, jcat.data(), nullptr};
+  kor_agg aggs[2] = {{KOR_AGG_SUM, {KOR_REAL, jval.data(), nullptr}},
+                     {KOR_AGG_COUNT, {KOR_INT, nullptr, nullptr}}};
+  std::vector<int64_t> ok(max_groups), ov(max_groups * 2);
+  std::vector<int8_t> okn(max_groups), ovn(max_groups * 2);
+  const int64_t ng = kor_group_by(nj, nullptr, 1, &key, 2, aggs, max_groups, ok.data(), okn.data(),
+                                  ov.data(), ovn.data());
+  for (int64_t g = 0; g < ng && g < max_groups; g++) {
+    out_cat[g] = static_cast<int32_t>(ok[g]);
+    out_sum[g] = bits_to_f64(ov[g * 2]);
+    out_cnt[g] = ov[g * 2 + 1];
+  }
+  return ng;
+}
+
+// The fixed reduction order of the GPU scan-aggregate kernel, restated on the
+// CPU so that its deterministic mode can be compared bit for bit:
+//   T = nblocks * block_threads threads; thread t folds (from +0.0) the
+//   qualifying rows r = t + k*T, k ascending, into per-group partials;
+//   each 64-lane wave combines its lanes by the tree p[l] += p[l + s] for
+//   s = 32, 16, ..., 1; the block's waves are combined by the same tree over
+//   wave index (s = nwaves/2 .. 1); block partials are added in block order
+//   to +0.0. out_sums[g * nf + f].
+void kor_dtree_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
+                        const int32_t* g2, int32_t card2, int32_t ngroups, int32_t nf,
+                        const double* const* fm, int32_t nblocks, int32_t block_threads,
+                        double* out_sums) {
+  const int64_t T = static_cast<int64_t>(nblocks) * block_threads;
+  const int32_t W = 64, nwaves = block_threads / W;
+  const int32_t width = ngroups * nf;
+  std::vector<double> total(width, 0.0);
+  std::vector<double> lanes(static_cast<size_t>(block_threads) * width);
+  for (int32_t b = 0; b < nblocks; b++) {
+    std::fill(lanes.begin(), lanes.end(), 0.0);
+    for (int32_t l = 0; l < block_threads; l++) {
+      double* acc = &lanes[static_cast<size_t>(l) * width];
+      for (int64_t r = static_cast<int64_t>(b) * block_threads + l; r < n; r += T) {
+        if (day[r] > cutoff) continue;
+        const int32_t g = g1[r] * card2 + g2[r];
+        for (int32_t f = 0; f < nf; f++) acc[g * nf + f] += fm[f][r];
+      }
+    }
+    for (int32_t w = 0; w < nwaves; w++) {
+      for (int32_t s = W / 2; s >= 1; s /= 2)
+        for (int32_t l = 0; l < s; l++)
+          for (int32_t x = 0; x < width; x++)
+            lanes[(size_t)(w * W + l) * width + x] += lanes[(size_t)(w * W + l + s) * width + x];
+    }
+    for (int32_t s = nwaves / 2; s >= 1; s /= 2)
+      for (int32_t w = 0; w < s; w++)
+        for (int32_t x = 0; x < width; x++)
+          lanes[(size_t)(w * W) * width + x] += lanes[(size_t)((w + s) * W) * width + x];
+    for (int32_t x = 0; x < width; x++) total[x] += lanes[x];
+  }
+  for (int32_t x = 0; x < width; x++) out_sums[x] = total[x];
+}
+
+}  // extern "C"

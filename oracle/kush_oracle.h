@@ -1,0 +1,133 @@
+// ORACLE — test infrastructure, not product code.
+//
+// A CPU restatement of KushDB's pipeline-operator semantics (scan-select, hash
+// join, group-by aggregate, column-index buckets, tuple-id dedup). It is the
+// checker for the HIP path: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it. The product library (kushdb_amd/libkushgpu.so)
+// never links, calls or falls back to anything here.
+//
+// Every function cites the reference file:line whose behaviour it restates.
+// Pinning (how each function is checked independently of its own formula) is
+// listed per function in the header comment of kush_oracle.cc.
+#pragma once
+
+#include <cstdint>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Mirrors catalog::TypeId (catalog/sql_type.h:7-16).
+enum kor_type {
+  KOR_SMALLINT = 0,
+  KOR_INT = 1,
+  KOR_BIGINT = 2,
+  KOR_REAL = 3,
+  KOR_DATE = 4,
+  KOR_TEXT = 5,
+  KOR_BOOLEAN = 6,
+  KOR_ENUM = 7,
+};
+
+// Mirrors plan::AggregateType (plan/expression/aggregate_expression.h).
+enum kor_agg_kind {
+  KOR_AGG_SUM = 0,
+  KOR_AGG_MIN = 1,
+  KOR_AGG_MAX = 2,
+  KOR_AGG_AVG = 3,
+  KOR_AGG_COUNT = 4,
+};
+
+// Mirrors plan::BinaryArithmeticExpressionType comparison subset.
+enum kor_cmp { KOR_EQ = 0, KOR_NEQ = 1, KOR_LT = 2, KOR_LEQ = 3, KOR_GT = 4, KOR_GEQ = 5 };
+
+// A typed column. data points at n values of the C type for `type`
+// (int16 / int32 / int64 / double / int8); nulls is an int8 array (1 = NULL)
+// or nullptr for a non-nullable column.
+typedef struct {
+  int32_t type;
+  const void* data;
+  const int8_t* nulls;
+} kor_col;
+
+typedef struct {
+  int32_t kind;   // kor_agg_kind
+  kor_col input;  // input.data == nullptr means COUNT(*)
+} kor_agg;
+
+typedef struct {
+  int32_t col;    // index into the column array passed alongside
+  int32_t op;     // kor_cmp
+  int64_t ilit;   // literal for integer-like columns
+  double flit;    // literal for REAL columns
+} kor_pred;
+
+// ---- hashing -------------------------------------------------------------
+void kor_hash_combine(int32_t* hash, int64_t v);
+int64_t kor_value_hash(int32_t type, int64_t bits);
+int64_t kor_agg_hash(const int64_t* key_hashes, const int8_t* key_nulls, int32_t nkeys);
+uint64_t kor_index_hash(uint64_t k);
+
+// ---- dates ---------------------------------------------------------------
+int32_t kor_build_date(int32_t y, int32_t m, int32_t d);
+void kor_split_date(int32_t jd, int32_t* y, int32_t* m, int32_t* d);
+
+// ---- scan-select ---------------------------------------------------------
+int64_t kor_scan_select(int64_t n, const kor_col* cols, int32_t npreds, const kor_pred* preds,
+                        int32_t conjunction, int32_t* out_rows);
+
+// ---- hash join -----------------------------------------------------------
+int64_t kor_hash_join(int32_t nkeys, const kor_col* build_keys, int64_t nb, const int32_t* build_rows,
+                      const kor_col* probe_keys, int64_t np, const int32_t* probe_rows,
+                      int64_t out_cap, int32_t* out_probe, int32_t* out_build);
+
+// ---- group-by aggregate --------------------------------------------------
+int64_t kor_group_by(int64_t n, const int32_t* rows, int32_t nkeys, const kor_col* keys,
+                     int32_t naggs, const kor_agg* aggs, int64_t max_groups,
+                     int64_t* out_keys, int8_t* out_key_nulls, int64_t* out_vals,
+                     int8_t* out_val_nulls);
+int32_t kor_aggregate(int64_t n, const int32_t* rows, int32_t naggs, const kor_agg* aggs,
+                      int64_t* out_vals, int8_t* out_val_nulls);
+
+// ---- column-index buckets / intersection -----------------------------------
+int32_t kor_fast_forward(const int32_t* data, int32_t size, int32_t prev_tuple);
+int32_t kor_intersect_chunk(int32_t nlists, const int32_t* const* lists, const int32_t* sizes,
+                            int32_t* state, int32_t* result, int32_t result_max);
+int64_t kor_intersect_all(int32_t nlists, const int32_t* const* lists, const int32_t* sizes,
+                          int32_t next_tuple, int32_t chunk, int32_t* out, int64_t out_cap);
+
+// ---- tuple-id dedup ------------------------------------------------------
+int64_t kor_tuple_dedup(int64_t n, int32_t len, const int32_t* tuples, uint8_t* is_new,
+                        int32_t* sorted_out);
+
+// ---- memory column index -------------------------------------------------
+int64_t kor_memory_index_build(int64_t n, const int32_t* keys, const int32_t* rows, int64_t max_keys,
+                               int32_t* out_keys, int64_t* out_offsets, int32_t* out_tids);
+
+// ---- disk .kdbindex lookup -----------------------------------------------
+int64_t kor_kdbindex_get(const uint8_t* file, int32_t key_type, int64_t key_bits,
+                         int32_t* out, int64_t out_cap);
+
+// ---- workload compositions (CPU baseline legs) ------------------------------
+int64_t kor_q_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
+                       const int32_t* g2, int32_t nf, const double* const* fm, int32_t ni,
+                       const int32_t* const* im, int64_t max_groups, int64_t* out);
+int64_t kor_q_join_count_sum(int64_t nb, const int64_t* bkeys, int64_t np, const int64_t* pkeys,
+                             const int64_t* pvals, int64_t* out_matches, int64_t* out_sum);
+int64_t kor_q_join_agg(int64_t nb, const int64_t* bkeys, int64_t np, const int64_t* pkeys,
+                       const int64_t* pvals, int64_t max_groups, int64_t* out_key,
+                       int64_t* out_sum, int64_t* out_cnt);
+int64_t kor_q_star(int64_t nf, const int32_t* fk, const int32_t* qty, const double* price,
+                   const int32_t* day, int32_t dlo, int32_t dhi, int64_t nd, const int32_t* pk,
+                   const int32_t* cat, int64_t max_groups, int32_t* out_cat, double* out_sum,
+                   int64_t* out_cnt);
+
+// ---- deterministic fixed-order reduction (GPU deterministic mode order) ----
+void kor_dtree_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
+                        const int32_t* g2, int32_t card2, int32_t ngroups, int32_t nf,
+                        const double* const* fm, int32_t nblocks, int32_t block_threads,
+                        double* out_sums);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,265 @@
+"""ORACLE — test infrastructure only (tests/, __graft_entry__.smoke(), and
+bench.py's cpu_baseline leg). ctypes bindings of liboracle.so, the CPU
+restatement of KushDB's operator semantics (kush_oracle.cc), plus the optional
+reference-runtime pin library _ref/libkushref.so.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libkushref.so")
+
+# kor_type / kor_agg_kind / kor_cmp (same numbering as kushgpu.h)
+SMALLINT, INT, BIGINT, REAL, DATE, TEXT, BOOLEAN, ENUM = range(8)
+SUM, MIN, MAX, AVG, COUNT = range(5)
+EQ, NEQ, LT, LEQ, GT, GEQ = range(6)
+
+_NP = {SMALLINT: np.int16, INT: np.int32, BIGINT: np.int64, REAL: np.float64, DATE: np.int32,
+       BOOLEAN: np.int8, ENUM: np.int32}
+
+
+class KorCol(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("data", ctypes.c_void_p), ("nulls", ctypes.c_void_p)]
+
+
+class KorAgg(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("input", KorCol)]
+
+
+class KorPred(ctypes.Structure):
+    _fields_ = [("col", ctypes.c_int32), ("op", ctypes.c_int32), ("ilit", ctypes.c_int64),
+                ("flit", ctypes.c_double)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
+        _lib = ctypes.CDLL(ORACLE_SO)
+        L = _lib
+        L.kor_hash_combine.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
+        L.kor_value_hash.restype = ctypes.c_int64
+        L.kor_value_hash.argtypes = [ctypes.c_int32, ctypes.c_int64]
+        L.kor_agg_hash.restype = ctypes.c_int64
+        L.kor_agg_hash.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]
+        L.kor_index_hash.restype = ctypes.c_uint64
+        L.kor_index_hash.argtypes = [ctypes.c_uint64]
+        L.kor_build_date.restype = ctypes.c_int32
+        L.kor_build_date.argtypes = [ctypes.c_int32] * 3
+        L.kor_split_date.argtypes = [ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int32)] * 3
+        L.kor_scan_select.restype = ctypes.c_int64
+        L.kor_scan_select.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                      ctypes.c_int32, ctypes.c_void_p]
+        L.kor_hash_join.restype = ctypes.c_int64
+        L.kor_hash_join.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.kor_group_by.restype = ctypes.c_int64
+        L.kor_group_by.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                   ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.kor_aggregate.restype = ctypes.c_int32
+        L.kor_aggregate.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.kor_fast_forward.restype = ctypes.c_int32
+        L.kor_fast_forward.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+        L.kor_intersect_all.restype = ctypes.c_int64
+        L.kor_intersect_all.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]
+        L.kor_tuple_dedup.restype = ctypes.c_int64
+        L.kor_tuple_dedup.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]
+        L.kor_memory_index_build.restype = ctypes.c_int64
+        L.kor_memory_index_build.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.kor_kdbindex_get.restype = ctypes.c_int64
+        L.kor_kdbindex_get.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_int64]
+        L.kor_q_scan_agg.restype = ctypes.c_int64
+        L.kor_q_scan_agg.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                     ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.kor_q_join_count_sum.restype = ctypes.c_int64
+        L.kor_q_join_count_sum.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64)]
+        L.kor_q_join_agg.restype = ctypes.c_int64
+        L.kor_q_join_agg.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+        L.kor_q_star.restype = ctypes.c_int64
+        L.kor_q_star.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p]
+        L.kor_dtree_scan_agg.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
+    return _lib
+
+
+def ref():
+    """The reference-runtime pin library, or None when it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        _ref = ctypes.CDLL(REF_SO)
+        R = _ref
+        R.ref_hash_combine.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
+        R.ref_build_date.restype = ctypes.c_int32
+        R.ref_build_date.argtypes = [ctypes.c_int32] * 3
+        R.ref_split_date.argtypes = [ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int32)] * 3
+        R.ref_fast_forward.restype = ctypes.c_int32
+        R.ref_fast_forward.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+        R.ref_intersect_all.restype = ctypes.c_int64
+        R.ref_intersect_all.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]
+        R.ref_tuple_dedup.restype = ctypes.c_int64
+        R.ref_tuple_dedup.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]
+        R.ref_hash_join_i64.restype = ctypes.c_int64
+        R.ref_hash_join_i64.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                        ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        R.ref_group_by_i64.restype = ctypes.c_int64
+        R.ref_group_by_i64.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Col:
+    """numpy column (+ optional int8 nulls) for the oracle."""
+
+    def __init__(self, typ, data, nulls=None):
+        self.type = typ
+        self.data = np.ascontiguousarray(data, dtype=_NP[typ])
+        self.nulls = None if nulls is None else np.ascontiguousarray(nulls, dtype=np.int8)
+
+    def kor(self):
+        return KorCol(self.type, self.data.ctypes.data, _p(self.nulls))
+
+
+def hash_combine(h, v):
+    x = ctypes.c_int32(h)
+    lib().kor_hash_combine(ctypes.byref(x), int(v))
+    return x.value
+
+
+def build_date(y, m, d):
+    return lib().kor_build_date(y, m, d)
+
+
+def split_date(jd):
+    y, m, d = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    lib().kor_split_date(jd, ctypes.byref(y), ctypes.byref(m), ctypes.byref(d))
+    return y.value, m.value, d.value
+
+
+def scan_select(cols, preds, conjunction=True):
+    n = len(cols[0].data)
+    ca = (KorCol * len(cols))(*[c.kor() for c in cols])
+    pa = (KorPred * max(len(preds), 1))(*[
+        KorPred(ci, op, 0 if cols[ci].type == REAL else int(l), float(l) if cols[ci].type == REAL else 0.0)
+        for ci, op, l in preds])
+    out = np.empty(max(n, 1), dtype=np.int32)
+    k = lib().kor_scan_select(n, ca, len(preds), pa, 1 if conjunction else 0, out.ctypes.data)
+    return out[:k].copy()
+
+
+def hash_join(build_keys, probe_keys, build_rows=None, probe_rows=None):
+    """build_keys/probe_keys: lists of Col. Returns (probe_rows, build_rows) pairs in reference order."""
+    nk = len(build_keys)
+    ba = (KorCol * nk)(*[c.kor() for c in build_keys])
+    pa = (KorCol * nk)(*[c.kor() for c in probe_keys])
+    nb = len(build_rows) if build_rows is not None else len(build_keys[0].data)
+    np_ = len(probe_rows) if probe_rows is not None else len(probe_keys[0].data)
+    br = None if build_rows is None else np.ascontiguousarray(build_rows, dtype=np.int32)
+    pr = None if probe_rows is None else np.ascontiguousarray(probe_rows, dtype=np.int32)
+    cap = max(np_, 16)
+    while True:
+        op = np.empty(cap, dtype=np.int32)
+        ob = np.empty(cap, dtype=np.int32)
+        k = lib().kor_hash_join(nk, ba, nb, _p(br), pa, np_, _p(pr), cap, op.ctypes.data, ob.ctypes.data)
+        if k <= cap:
+            return op[:k].copy(), ob[:k].copy()
+        cap = k
+
+
+def group_by(keys, aggs, rows=None, max_groups=1 << 20):
+    """keys: list of Col; aggs: list of (kind, Col or None for COUNT(*)).
+
+    Returns (key_vals int64 [G,K], key_nulls, vals int64 [G,A], val_nulls) in insertion order.
+    REAL values come back as IEEE bits (view as float64)."""
+    n = len(rows) if rows is not None else (len(keys[0].data) if keys else
+                                            len(next(c for _, c in aggs if c is not None).data))
+    rr = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    ka = (KorCol * max(len(keys), 1))(*[c.kor() for c in keys])
+    aa = (KorAgg * max(len(aggs), 1))(*[KorAgg(k, c.kor() if c is not None else KorCol(INT, None, None))
+                                        for k, c in aggs])
+    ok = np.zeros((max_groups, max(len(keys), 1)), dtype=np.int64)
+    okn = np.zeros((max_groups, max(len(keys), 1)), dtype=np.int8)
+    ov = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int64)
+    ovn = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int8)
+    g = lib().kor_group_by(n, _p(rr), len(keys), ka, len(aggs), aa, max_groups, ok.ctypes.data, okn.ctypes.data,
+                           ov.ctypes.data, ovn.ctypes.data)
+    g = min(g, max_groups)
+    return ok[:g, :len(keys)], okn[:g, :len(keys)], ov[:g, :len(aggs)], ovn[:g, :len(aggs)]
+
+
+def aggregate(aggs, n, rows=None):
+    rr = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    nn = len(rr) if rr is not None else n
+    aa = (KorAgg * max(len(aggs), 1))(*[KorAgg(k, c.kor() if c is not None else KorCol(INT, None, None))
+                                        for k, c in aggs])
+    ov = np.zeros(max(len(aggs), 1), dtype=np.int64)
+    ovn = np.zeros(max(len(aggs), 1), dtype=np.int8)
+    got = lib().kor_aggregate(nn, _p(rr), len(aggs), aa, ov.ctypes.data, ovn.ctypes.data)
+    return got, ov[:len(aggs)], ovn[:len(aggs)]
+
+
+def fast_forward(data, prev):
+    a = np.ascontiguousarray(data, dtype=np.int32)
+    return lib().kor_fast_forward(a.ctypes.data if len(a) else None, len(a), prev)
+
+
+def intersect_all(lists, next_tuple=0, chunk=64, fn=None):
+    arrs = [np.ascontiguousarray(x, dtype=np.int32) for x in lists]
+    ptrs = (ctypes.c_void_p * max(len(arrs), 1))(*[a.ctypes.data for a in arrs])
+    sizes = (ctypes.c_int32 * max(len(arrs), 1))(*[len(a) for a in arrs])
+    cap = max([len(a) for a in arrs] + [1])
+    out = np.empty(cap, dtype=np.int32)
+    f = fn or lib().kor_intersect_all
+    k = f(len(arrs), ptrs, sizes, next_tuple, chunk, out.ctypes.data, cap)
+    return out[:k].copy()
+
+
+def tuple_dedup(tuples, fn=None):
+    t = np.ascontiguousarray(tuples, dtype=np.int32)
+    n, ln = t.shape
+    is_new = np.zeros(max(n, 1), dtype=np.uint8)
+    out = np.zeros((max(n, 1), ln), dtype=np.int32)
+    f = fn or lib().kor_tuple_dedup
+    k = f(n, ln, t.ctypes.data, is_new.ctypes.data, out.ctypes.data)
+    return is_new[:n].astype(bool), out[:k].copy()
+
+
+def kdbindex_get(file_bytes, key_type, key):
+    buf = np.frombuffer(file_bytes, dtype=np.uint8)
+    if key_type == REAL:
+        bits = int(np.array([key], dtype=np.float64).view(np.int64)[0])
+    else:
+        bits = int(key)
+    cap = 1 << 20
+    out = np.empty(cap, dtype=np.int32)
+    k = lib().kor_kdbindex_get(buf.ctypes.data, key_type, bits, out.ctypes.data, cap)
+    return out[:k].copy()

@@ -1,0 +1,47 @@
+"""Copies the reference's own end-to-end fixture data (column files and the
+expected result tables) used by the parity tests into tests/golden/ref/.
+
+These are data files the reference's tests hold (end_to_end_test/data/*.kdb,
+*_null.kdb, *.kdbenum, *.kdbindex and end_to_end_test/<op>/*_expected.tbl);
+no reference source is copied. Run in the build container:
+
+    python tests/golden/copy_fixtures.py [/root/reference]
+"""
+import os
+import shutil
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEST = os.path.join(HERE, "ref")
+
+DATA = (
+    [f"info_{c}{s}" for c in ("id", "cheated", "date", "zscore", "num1", "num2") for s in (".kdb", "_null.kdb")]
+    + ["info_id.kdbindex", "info_num2.kdbindex", "info_date.kdbindex", "info_zscore.kdbindex",
+       "people_id.kdb", "people_id_null.kdb", "n_nationkey.kdb",
+       "l_shipdate.kdb", "l_quantity.kdb", "l_extendedprice.kdb", "l_discount.kdb", "l_tax.kdb",
+       "l_returnflag_enum.kdb", "l_linestatus_enum.kdb",
+       "l_returnflag_enum.kdbenum", "l_linestatus_enum.kdbenum"]
+)
+
+EXPECTED = (
+    [f"hash_join/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real")]
+    + [f"scan_select/{t}_expected.tbl" for t in ("int", "bigint", "date", "real", "smallint", "multiple_eq")]
+    + [f"group_by_aggregate/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "real", "date")]
+    + ["aggregate/int_expected.tbl", "aggregate/empty_expected.tbl", "tpch/q01_expected.tbl"]
+    + [f"simd_scan_select/{t}_expected.tbl" for t in ("int", "date_range", "multiple_col")]
+)
+
+
+def main(ref="/root/reference"):
+    e2e = os.path.join(ref, "end_to_end_test")
+    os.makedirs(os.path.join(DEST, "data"), exist_ok=True)
+    for f in DATA:
+        shutil.copyfile(os.path.join(e2e, "data", f), os.path.join(DEST, "data", f))
+    for f in EXPECTED:
+        d = os.path.join(DEST, os.path.dirname(f))
+        os.makedirs(d, exist_ok=True)
+        shutil.copyfile(os.path.join(e2e, f), os.path.join(DEST, f))
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:])

@@ -1,0 +1,160 @@
+"""Hash join (build / probe / probe fused with GROUP BY the matched build key)
+and the high-cardinality group-by on the HIP path, against the oracle.
+
+Bars: join pairs, cardinalities, counts and integer sums bit-exact after
+sorting; groups (key, SUM(BIGINT), COUNT) bit-exact as sets, insertion order
+recovered from the first probe row."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from kushdb_amd import abi
+from kushdb_amd.runtime import Column, Context, HashTable
+from kushdb_amd.workloads import join_tables
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return Context(0)
+
+
+def col64(x):
+    return Column(abi.KG_BIGINT, torch.as_tensor(np.ascontiguousarray(x, dtype=np.int64)).to(DEV))
+
+
+def sorted_pairs(p, b):
+    a = np.stack([np.asarray(p), np.asarray(b)], 1)
+    return a[np.lexsort((a[:, 1], a[:, 0]))]
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.5, 1.0])
+def test_join_pairs_and_agg_vs_oracle(ctx, theta):
+    nb, npr = 100_000, 1_000_000
+    print(f"join theta={theta} nb={nb} np={npr}", flush=True)
+    bk, pay, pk, pv = join_tables(nb, npr, theta=theta, device=DEV, seed=99 + int(theta * 10))
+    ht = HashTable(ctx, nb)
+    ht.insert(Column(abi.KG_BIGINT, bk))
+    assert ht.size() == nb
+    pc = Column(abi.KG_BIGINT, pk)
+    count = ht.probe_count(pc)
+    gp, gb = ht.probe(pc)
+    hb, hp, hv = bk.cpu().numpy(), pk.cpu().numpy(), pv.cpu().numpy()
+    op, ob = po.hash_join([po.Col(po.BIGINT, hb)], [po.Col(po.BIGINT, hp)])
+    assert count == len(op) == len(gp)
+    assert np.array_equal(sorted_pairs(gp.cpu().numpy(), gb.cpu().numpy()), sorted_pairs(op, ob))
+    # closed form: build keys are 0..nb-1, so a probe matches iff its key < nb
+    assert count == int((hp < nb).sum())
+    # fused probe + GROUP BY matched key: SUM(v), COUNT(*)
+    k, s, c, first = ht.probe_agg(pc, pv, max_groups=nb, with_first=True)
+    k, s, c, first = (x.cpu().numpy() for x in (k, s, c, first))
+    order = np.argsort(first, kind="stable")
+    k, s, c = k[order], s[order], c[order]
+    ok = np.empty(nb, dtype=np.int64)
+    osum = np.empty(nb, dtype=np.int64)
+    ocnt = np.empty(nb, dtype=np.int64)
+    g = po.lib().kor_q_join_agg(nb, hb.ctypes.data, npr, hp.ctypes.data, hv.ctypes.data, nb, ok.ctypes.data,
+                                osum.ctypes.data, ocnt.ctypes.data)
+    assert len(k) == g
+    assert np.array_equal(k, ok[:g]) and np.array_equal(s, osum[:g]) and np.array_equal(c, ocnt[:g])
+    ht.free()
+
+
+def test_join_duplicates_nulls_and_edge_keys(ctx):
+    rng = np.random.default_rng(5)
+    b = rng.integers(-50, 50, 5000).astype(np.int64)
+    b[:7] = np.iinfo(np.int64).min  # the free-slot marker value is a legal key
+    b[7:9] = np.iinfo(np.int64).max
+    p = rng.integers(-60, 60, 20000).astype(np.int64)
+    p[:3] = np.iinfo(np.int64).min
+    p[3] = np.iinfo(np.int64).max
+    bn = (rng.random(5000) < 0.1).astype(np.int8)
+    pn = (rng.random(20000) < 0.1).astype(np.int8)
+    bcol = Column(abi.KG_BIGINT, torch.as_tensor(b).to(DEV), torch.as_tensor(bn).to(DEV))
+    pcol = Column(abi.KG_BIGINT, torch.as_tensor(p).to(DEV), torch.as_tensor(pn).to(DEV))
+    ht = HashTable(ctx, 5000)
+    ht.insert(bcol)
+    assert ht.size() == int((bn == 0).sum())
+    gp, gb = ht.probe(pcol)
+    op, ob = po.hash_join([po.Col(po.BIGINT, b, bn)], [po.Col(po.BIGINT, p, pn)])
+    assert len(gp) == len(op) > 0
+    assert np.array_equal(sorted_pairs(gp.cpu().numpy(), gb.cpu().numpy()), sorted_pairs(op, ob))
+    assert ht.probe_count(pcol) == len(op)
+    ht.free()
+
+
+@pytest.mark.parametrize("typ,npdt", [(abi.KG_INT, np.int32), (abi.KG_SMALLINT, np.int16), (abi.KG_REAL, np.float64)])
+def test_join_other_key_types(ctx, typ, npdt):
+    rng = np.random.default_rng(int(typ) + 11)
+    if npdt == np.float64:
+        b = np.round(rng.normal(size=3000), 2)
+        p = np.round(rng.normal(size=9000), 2)
+        b[:3] = -0.0
+        p[:5] = 0.0
+        p[5] = np.nan
+    else:
+        b = rng.integers(-300, 300, 3000).astype(npdt)
+        p = rng.integers(-300, 300, 9000).astype(npdt)
+    ht = HashTable(ctx, 3000)
+    ht.insert(Column(typ, torch.as_tensor(b).to(DEV)))
+    gp, gb = ht.probe(Column(typ, torch.as_tensor(p).to(DEV)))
+    ot = {abi.KG_INT: po.INT, abi.KG_SMALLINT: po.SMALLINT, abi.KG_REAL: po.REAL}[typ]
+    op, ob = po.hash_join([po.Col(ot, b)], [po.Col(ot, p)])
+    assert np.array_equal(sorted_pairs(gp.cpu().numpy(), gb.cpu().numpy()), sorted_pairs(op, ob))
+    ht.free()
+
+
+def test_join_empty_sides(ctx):
+    ht = HashTable(ctx, 10)
+    assert ht.probe_count(col64(np.arange(100))) == 0
+    ht.insert(col64(np.arange(10)))
+    assert ht.probe_count(col64(np.zeros(0))) == 0
+    ht.free()
+
+
+def test_join_capacity_error(ctx):
+    ht = HashTable(ctx, 10)
+    with pytest.raises(abi.KgError):
+        ht.insert(col64(np.arange(11)))
+    ht.free()
+
+
+@pytest.mark.parametrize("ngroups,n", [(1, 1000), (1000, 200_000), (150_000, 1_000_000)])
+def test_group_by_i64_vs_oracle(ctx, ngroups, n):
+    rng = np.random.default_rng(ngroups)
+    keys = rng.integers(0, ngroups, n).astype(np.int64) * 7919 - 3
+    keys[: min(5, n)] = np.iinfo(np.int64).min
+    vals = rng.integers(-2**62, 2**62, n).astype(np.int64)
+    k, s, c, first = ctx.group_by_i64(torch.as_tensor(keys).to(DEV), torch.as_tensor(vals).to(DEV),
+                                      with_first=True)
+    k, s, c, first = (x.cpu().numpy() for x in (k, s, c, first))
+    order = np.argsort(first, kind="stable")
+    kv, _, v, _ = po.group_by([po.Col(po.BIGINT, keys)], [(po.SUM, po.Col(po.BIGINT, vals)), (po.COUNT, None)],
+                              max_groups=len(k) + 1)
+    assert len(k) == len(kv)
+    assert np.array_equal(k[order], kv[:, 0])
+    assert np.array_equal(s[order], v[:, 0]) and np.array_equal(c[order], v[:, 1])
+
+
+def test_full_size_join_properties(ctx):
+    """16M build keys / 256M probes, uniform: cardinality by closed form, fused
+    GROUP BY totals equal the matched rows and their value sum (exact int64)."""
+    nb, npr = 1 << 24, 1 << 28
+    print(f"join full nb={nb} np={npr}", flush=True)
+    bk, pay, pk, pv = join_tables(nb, npr, theta=0.0, device=DEV)
+    ht = HashTable(ctx, nb)
+    ht.insert(Column(abi.KG_BIGINT, bk))
+    pc = Column(abi.KG_BIGINT, pk)
+    hit = pk < nb
+    want = int(hit.sum().item())
+    assert ht.probe_count(pc) == want
+    k, s, c, _ = ht.probe_agg(pc, pv, max_groups=nb)
+    assert int(c.sum().item()) == want
+    assert int(s.sum().item()) == int(pv[hit].sum().item())
+    assert len(k) == int(torch.unique(pk[hit]).numel())
+    ht.free()

@@ -1,0 +1,38 @@
+"""The oracle (CPU restatement) against the reference's own end-to-end
+fixtures: the checked-in column files and *_expected.tbl outputs."""
+import pytest
+
+import kdb
+import queries
+from engines import OracleEngine
+
+E = OracleEngine()
+
+
+@pytest.mark.parametrize("t", sorted(queries.HASH_JOIN))
+def test_hash_join(t):
+    assert sorted(queries.hash_join(E, t)) == sorted(kdb.expected(f"hash_join/{t}_expected.tbl"))
+
+
+@pytest.mark.parametrize("t", sorted(queries.SCAN_SELECT))
+def test_scan_select(t):
+    assert sorted(queries.scan_select(E, t)) == sorted(kdb.expected(f"scan_select/{t}_expected.tbl"))
+
+
+@pytest.mark.parametrize("t", sorted(queries.GROUP_BY))
+def test_group_by(t):
+    assert sorted(queries.group_by(E, t)) == sorted(kdb.expected(f"group_by_aggregate/{t}_expected.tbl"))
+
+
+@pytest.mark.parametrize("t", ["int", "empty"])
+def test_aggregate(t):
+    assert sorted(queries.aggregate(E, t)) == sorted(kdb.expected(f"aggregate/{t}_expected.tbl"))
+
+
+@pytest.mark.parametrize("t", ["int", "date_range", "multiple_col"])
+def test_simd_scan_select(t):
+    assert sorted(queries.simd_scan_select(E, t)) == sorted(kdb.expected(f"simd_scan_select/{t}_expected.tbl"))
+
+
+def test_tpch_q01():
+    queries.check_q01(queries.tpch_q01(E), kdb.expected("tpch/q01_expected.tbl"))
