@@ -84,10 +84,14 @@ typedef struct kg_pred {
   double flit;
 } kg_pred;
 
-/* One aggregate over cols[col]; col = -1 with KG_AGG_COUNT is COUNT(*). */
+/* One aggregate over cols[col]; col = -1 with KG_AGG_COUNT is COUNT(*).
+ * col2 >= 0 makes the input the REAL product cols[col] * Float(cols[col2])
+ * (e.g. SUM(price * Float(qty))); -1 otherwise. */
 typedef struct kg_agg {
   int32_t kind;
   int32_t col;
+  int32_t col2;
+  int32_t reserved;
 } kg_agg;
 
 typedef struct kg_ctx kg_ctx;

@@ -29,7 +29,8 @@ class KgPred(ctypes.Structure):
 
 
 class KgAgg(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int32), ("col", ctypes.c_int32)]
+    _fields_ = [("kind", ctypes.c_int32), ("col", ctypes.c_int32), ("col2", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 _vp = ctypes.c_void_p

@@ -24,43 +24,10 @@
 #include <cstring>
 #include <vector>
 
-#include "kg_internal.h"
+#include "kg_ops.h"
 
 namespace kg {
 
-constexpr int kMaxPreds = 8;
-constexpr int kMaxMeasures = 8;
-constexpr int kBlock = 256;
-constexpr int kSelTile = 4096;          // rows per workgroup in the selection kernels
-constexpr int kAggMaxBlocks = 2048;     // workgroups of the fixed-order aggregation
-constexpr int kAggRowsPerThreadMin = 8; // n / (kBlock * this) workgroups, capped above
-constexpr int kMaxLdsGroups = 1024;
-
-struct PredSet {
-  int32_t n;
-  int32_t conj;
-  int32_t type[kMaxPreds];
-  int32_t op[kMaxPreds];
-  const void* data[kMaxPreds];
-  const int8_t* nulls[kMaxPreds];
-  int64_t ilit[kMaxPreds];
-  double flit[kMaxPreds];
-};
-
-__device__ __forceinline__ bool eval_preds(const PredSet& p, int64_t r) {
-  bool pass = p.conj ? true : (p.n == 0);
-  for (int i = 0; i < p.n; i++) {
-    bool v;
-    if (p.type[i] == KG_REAL) {
-      v = cmp_f64(p.op[i], static_cast<const double*>(p.data[i])[r], p.flit[i]);
-    } else {
-      v = cmp_i64(p.op[i], load_int(p.type[i], p.data[i], r), p.ilit[i]);
-    }
-    if (p.nulls[i] && p.nulls[i][r]) v = false;
-    pass = p.conj ? (pass && v) : (pass || v);
-  }
-  return pass;
-}
 
 // ---------------------------------------------------------------------------
 // Selection
@@ -138,74 +105,6 @@ __global__ __launch_bounds__(kBlock) void k_select_write(PredSet p, int64_t n, c
 // ---------------------------------------------------------------------------
 // Aggregation over dense group codes
 
-// One measure column: the aggregate input cols[col] (times Float(cols[col2])
-// when col2 is set).
-struct MeasureSet {
-  int32_t nm;
-  int32_t is_real[kMaxMeasures];  // accumulate as double (else exact int64)
-  int32_t type[kMaxMeasures];
-  int32_t type2[kMaxMeasures];
-  const void* data[kMaxMeasures];
-  const void* data2[kMaxMeasures];  // optional second factor
-  const int8_t* nulls[kMaxMeasures];
-  const int8_t* nulls2[kMaxMeasures];
-};
-
-struct KeySet {
-  int32_t nk;
-  const int32_t* data[2];
-  const int8_t* nulls[2];
-  int32_t card1;  // group = code0 * card1 + code1
-};
-
-__device__ __forceinline__ int group_of(const KeySet& k, int64_t r) {
-  if (k.nk == 0) return 0;
-  if (k.nk == 1) return k.data[0][r];
-  return k.data[0][r] * k.card1 + k.data[1][r];
-}
-
-// Loads measure m of row r: bits of a double (is_real) or an int64, and its NULL flag.
-__device__ __forceinline__ uint64_t load_measure(const MeasureSet& M, int m, int64_t r, bool& null) {
-  null = (M.nulls[m] && M.nulls[m][r]) || (M.data2[m] && M.nulls2[m] && M.nulls2[m][r]);
-  if (M.is_real[m]) {
-    double x = M.type[m] == KG_REAL ? static_cast<const double*>(M.data[m])[r]
-                                    : static_cast<double>(load_int(M.type[m], M.data[m], r));
-    if (M.data2[m]) {
-      double y = M.type2[m] == KG_REAL ? static_cast<const double*>(M.data2[m])[r]
-                                       : static_cast<double>(load_int(M.type2[m], M.data2[m], r));
-      x = x * y;
-    }
-    return static_cast<uint64_t>(__double_as_longlong(x));
-  }
-  return static_cast<uint64_t>(load_int(M.type[m], M.data[m], r));
-}
-
-// Per-workgroup output record for group g: counts[g], first row, and per
-// measure: sum (double bits or int64), non-NULL count, min, max.
-struct AggOut {
-  uint64_t* sum;     // [nblocks][G][nm]
-  uint64_t* mn;      // [nblocks][G][nm] (minmax only)
-  uint64_t* mx;      // [nblocks][G][nm] (minmax only)
-  uint32_t* nn;      // [nblocks][G][nm] non-NULL counts
-  uint32_t* rows;    // [nblocks][G]
-  int32_t* first;    // [G] atomicMin across workgroups (init INT32_MAX)
-};
-
-__device__ __forceinline__ uint64_t fadd_bits(uint64_t a, uint64_t b) {
-  return static_cast<uint64_t>(__double_as_longlong(__longlong_as_double(a) + __longlong_as_double(b)));
-}
-
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int s) {
-  const uint32_t lo = __shfl_xor(static_cast<uint32_t>(v), s);
-  const uint32_t hi = __shfl_xor(static_cast<uint32_t>(v >> 32), s);
-  return (static_cast<uint64_t>(hi) << 32) | lo;
-}
-
-// Order-preserving min/max on the 8-byte value (double or int64).
-__device__ __forceinline__ bool val_less(bool is_real, uint64_t a, uint64_t b) {
-  return is_real ? __longlong_as_double(a) < __longlong_as_double(b)
-                 : static_cast<int64_t>(a) < static_cast<int64_t>(b);
-}
 
 // NN: some measure is nullable, so non-NULL counts are kept per measure
 // (otherwise every measure's count is the group's row count).
@@ -366,10 +265,6 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_reg(PredSet P, KeySet K, M
 
 // Adds the per-workgroup partials in workgroup order. One thread per
 // (group, measure); thread G*nm + g handles the row count of group g.
-struct AggFinal {
-  uint64_t sum, mn, mx;
-  uint64_t nn;
-};
 
 __global__ void k_group_agg_final(const AggOut in, int nblocks, int G, MeasureSet M, bool minmax,
                                   AggFinal* fin, uint64_t* rows_out) {
@@ -508,7 +403,7 @@ __global__ void k_first_null(const int32_t* first, int G, const int8_t* nulls, i
 
 using namespace kg;
 
-namespace {
+namespace kg {
 
 int build_preds(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_pred* preds, int32_t npreds,
                 int32_t conj, PredSet& p) {
@@ -535,7 +430,7 @@ int build_preds(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_pred* p
   return KG_OK;
 }
 
-}  // namespace
+}  // namespace kg
 
 extern "C" int kg_scan_select(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
                               const kg_pred* preds, int32_t npreds, int32_t conjunction,
@@ -572,14 +467,6 @@ extern "C" int kg_scan_select(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_
 }
 
 namespace {
-
-// Host view of one requested aggregate.
-struct AggPlan {
-  int32_t kind;
-  int32_t measure;   // index into MeasureSet, -1 for COUNT(*)
-  int32_t out_type;  // SQL type of the result
-  const int8_t* count_nulls;  // COUNT(col): the column's NULL array
-};
 
 template <int G, bool MM>
 void launch_reg(hipStream_t s, int nb, const PredSet& P, const KeySet& K, const MeasureSet& M, int64_t n,
@@ -655,56 +542,10 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
     return kg_fail(ctx, KG_ERR_UNSUPPORTED, "%lld dense groups exceed the on-chip limit %d", (long long)G,
                    kMaxLdsGroups);
 
-  // Distinct measures and the aggregate plan.
   MeasureSet M;
-  std::memset(&M, 0, sizeof(M));
-  std::vector<AggPlan> plan(naggs);
+  std::vector<AggPlan> plan;
   bool minmax = false;
-  for (int a = 0; a < naggs; a++) {
-    const kg_agg& q = aggs[a];
-    AggPlan& pl = plan[a];
-    pl.kind = q.kind;
-    pl.measure = -1;
-    pl.count_nulls = nullptr;
-    KG_REQUIRE(ctx, q.kind >= KG_AGG_SUM && q.kind <= KG_AGG_COUNT, "bad aggregate kind %d", q.kind);
-    if (q.kind == KG_AGG_COUNT && q.col < 0) {
-      pl.out_type = KG_BIGINT;
-      continue;
-    }
-    KG_REQUIRE(ctx, q.col >= 0 && q.col < ncols, "aggregate %d references column %d", a, q.col);
-    const kg_col& c = cols[q.col];
-    KG_REQUIRE(ctx, c.data, "aggregate column %d has no data", q.col);
-    if (c.type == KG_TEXT || c.type == KG_BOOLEAN)
-      return kg_fail(ctx, KG_ERR_UNSUPPORTED, "aggregate over type %d is not supported here", c.type);
-    if (q.kind == KG_AGG_COUNT) {
-      pl.out_type = KG_BIGINT;
-      pl.count_nulls = c.nulls;
-    } else if (q.kind == KG_AGG_AVG) {
-      pl.out_type = KG_REAL;
-    } else {
-      if (q.kind == KG_AGG_SUM && (c.type == KG_DATE || c.type == KG_ENUM))
-        return kg_fail(ctx, KG_ERR_INVALID, "cannot compute sum of non-numeric col");
-      pl.out_type = c.type;
-    }
-    if (q.kind == KG_AGG_MIN || q.kind == KG_AGG_MAX) minmax = true;
-    if (q.kind == KG_AGG_MIN || q.kind == KG_AGG_MAX) {
-      if (c.type == KG_ENUM)
-        return kg_fail(ctx, KG_ERR_UNSUPPORTED, "MIN/MAX over ENUM compares decoded strings");
-    }
-    const bool real = c.type == KG_REAL;
-    int found = -1;
-    for (int m = 0; m < M.nm; m++)
-      if (M.data[m] == c.data && !M.data2[m]) found = m;
-    if (found < 0) {
-      KG_REQUIRE(ctx, M.nm < kMaxMeasures, "at most %d distinct aggregate inputs", kMaxMeasures);
-      found = M.nm++;
-      M.is_real[found] = real;
-      M.type[found] = c.type;
-      M.data[found] = c.data;
-      M.nulls[found] = c.nulls;
-    }
-    pl.measure = found;
-  }
+  if ((st = build_measures(ctx, cols, ncols, aggs, naggs, M, plan, minmax)) != KG_OK) return st;
 
   *out_ngroups = 0;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -712,6 +553,7 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   std::vector<AggFinal> fin(G * (nm > 0 ? nm : 1));
   std::vector<uint64_t> rowsv(G);
   std::vector<int32_t> firstv(G);
+  std::vector<int64_t> first64(G);
   std::vector<std::vector<int8_t>> first_null(naggs);
 
   const bool use_reg = G <= (minmax ? 4 : 8);
@@ -812,20 +654,107 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
     KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   }
 
-  // Groups present, in insertion order (first qualifying row).
+  for (int64_t g = 0; g < G; g++) first64[g] = firstv[g];
+  return emit_groups(ctx, G, nkeys, K.card1, M, plan, fin, rowsv, first64, first_null, max_groups, out_ngroups,
+                     out_keys, out_vals, out_nulls);
+}
+
+namespace kg {
+
+int build_measures(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_agg* aggs, int32_t naggs,
+                   MeasureSet& M, std::vector<AggPlan>& plan, bool& minmax) {
+  std::memset(&M, 0, sizeof(M));
+  plan.assign(naggs, AggPlan{});
+  minmax = false;
+  auto check_col = [&](int32_t ci, const char* what) -> int {
+    KG_REQUIRE(ctx, ci >= 0 && ci < ncols, "%s references column %d of %d", what, ci, ncols);
+    const kg_col& c = cols[ci];
+    KG_REQUIRE(ctx, c.data, "%s column %d has no data", what, ci);
+    if (c.type == KG_TEXT || c.type == KG_BOOLEAN)
+      return kg_fail(ctx, KG_ERR_UNSUPPORTED, "aggregate over type %d is not supported here", c.type);
+    KG_REQUIRE(ctx, type_width(c.type) > 0, "column %d has bad type %d", ci, c.type);
+    return KG_OK;
+  };
+  for (int a = 0; a < naggs; a++) {
+    const kg_agg& q = aggs[a];
+    AggPlan& pl = plan[a];
+    pl.kind = q.kind;
+    pl.measure = -1;
+    pl.count_nulls = nullptr;
+    KG_REQUIRE(ctx, q.kind >= KG_AGG_SUM && q.kind <= KG_AGG_COUNT, "bad aggregate kind %d", q.kind);
+    if (q.kind == KG_AGG_COUNT && q.col < 0) {
+      pl.out_type = KG_BIGINT;
+      continue;
+    }
+    int st = check_col(q.col, "aggregate");
+    if (st != KG_OK) return st;
+    const kg_col& c = cols[q.col];
+    const kg_col* c2 = nullptr;
+    if (q.col2 >= 0) {
+      if ((st = check_col(q.col2, "aggregate factor")) != KG_OK) return st;
+      c2 = &cols[q.col2];
+      if (c.type == KG_DATE || c.type == KG_ENUM || c2->type == KG_DATE || c2->type == KG_ENUM)
+        return kg_fail(ctx, KG_ERR_INVALID, "products of non-numeric columns");
+    }
+    // The input's SQL type: a product is REAL (price * Float(qty)).
+    const int32_t in_type = c2 ? KG_REAL : c.type;
+    if (q.kind == KG_AGG_COUNT) {
+      pl.out_type = KG_BIGINT;
+      pl.count_nulls = c2 ? nullptr : c.nulls;
+      if (c2 && (c.nulls || c2->nulls))
+        return kg_fail(ctx, KG_ERR_UNSUPPORTED, "COUNT over a product of nullable columns");
+    } else if (q.kind == KG_AGG_AVG) {
+      pl.out_type = KG_REAL;
+    } else {
+      if (q.kind == KG_AGG_SUM && (in_type == KG_DATE || in_type == KG_ENUM))
+        return kg_fail(ctx, KG_ERR_INVALID, "cannot compute sum of non-numeric col");
+      if ((q.kind == KG_AGG_MIN || q.kind == KG_AGG_MAX) && in_type == KG_ENUM)
+        return kg_fail(ctx, KG_ERR_UNSUPPORTED, "MIN/MAX over ENUM compares decoded strings");
+      pl.out_type = in_type;
+    }
+    if (q.kind == KG_AGG_MIN || q.kind == KG_AGG_MAX) minmax = true;
+    int found = -1;
+    for (int m = 0; m < M.nm; m++)
+      if (M.data[m] == c.data && M.data2[m] == (c2 ? c2->data : nullptr)) found = m;
+    if (found < 0) {
+      KG_REQUIRE(ctx, M.nm < kMaxMeasures, "at most %d distinct aggregate inputs", kMaxMeasures);
+      found = M.nm++;
+      M.is_real[found] = in_type == KG_REAL;
+      M.type[found] = c.type;
+      M.data[found] = c.data;
+      M.nulls[found] = c.nulls;
+      if (c2) {
+        M.type2[found] = c2->type;
+        M.data2[found] = c2->data;
+        M.nulls2[found] = c2->nulls;
+      }
+    }
+    pl.measure = found;
+  }
+  return KG_OK;
+}
+
+int emit_groups(kg_ctx* ctx, int64_t G, int32_t nkeys, int32_t card1, const MeasureSet& M,
+                const std::vector<AggPlan>& plan, const std::vector<AggFinal>& fin,
+                const std::vector<uint64_t>& rowsv, const std::vector<int64_t>& firstv,
+                const std::vector<std::vector<int8_t>>& first_null, int64_t max_groups, int64_t* out_ngroups,
+                int32_t* out_keys, int64_t* out_vals, int8_t* out_nulls) {
+  const int naggs = static_cast<int>(plan.size());
+  const int nm = M.nm;
   std::vector<int32_t> present;
   for (int64_t g = 0; g < G; g++)
     if (rowsv[g] > 0) present.push_back(static_cast<int32_t>(g));
   std::sort(present.begin(), present.end(), [&](int32_t a, int32_t b) { return firstv[a] < firstv[b]; });
   const int64_t ng = static_cast<int64_t>(present.size());
   *out_ngroups = ng;
-  if (ng > max_groups) return kg_fail(ctx, KG_ERR_CAPACITY, "%lld groups exceed max_groups %lld", (long long)ng, (long long)max_groups);
+  if (ng > max_groups)
+    return kg_fail(ctx, KG_ERR_CAPACITY, "%lld groups exceed max_groups %lld", (long long)ng, (long long)max_groups);
   for (int64_t i = 0; i < ng; i++) {
     const int32_t g = present[i];
     if (nkeys == 1) out_keys[i] = g;
     if (nkeys == 2) {
-      out_keys[i * 2] = g / K.card1;
-      out_keys[i * 2 + 1] = g % K.card1;
+      out_keys[i * 2] = g / card1;
+      out_keys[i * 2 + 1] = g % card1;
     }
     for (int a = 0; a < naggs; a++) {
       const AggPlan& pl = plan[a];
@@ -836,6 +765,8 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
         if (pl.measure < 0) {
           v = static_cast<int64_t>(rowsv[g]);
         } else {
+          // CountAggregator::Initialize stores 1 even for a NULL first value
+          // (compile/proxy/aggregator.cc:285-287).
           const AggFinal& f = fin[g * nm + pl.measure];
           v = static_cast<int64_t>(f.nn) + (first_null[a].empty() ? 0 : first_null[a][g]);
         }
@@ -850,6 +781,7 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
       const bool real = M.is_real[pl.measure];
       switch (pl.kind) {
         case KG_AGG_SUM:
+          // SUM keeps the child type and wraps in it (aggregator.cc:44-71).
           if (real) {
             v = static_cast<int64_t>(f.sum);
           } else if (pl.out_type == KG_INT) {
@@ -878,3 +810,5 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   }
   return KG_OK;
 }
+
+}  // namespace kg

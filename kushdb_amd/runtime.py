@@ -118,7 +118,7 @@ class Context:
         parr = (KgPred * max(len(preds), 1))(*[_pred(cols, p) for p in preds])
         kc = (ctypes.c_int32 * max(len(keys), 1))(*[k[0] for k in keys])
         kd = (ctypes.c_int32 * max(len(keys), 1))(*[k[1] for k in keys])
-        aarr = (KgAgg * max(len(aggs), 1))(*[KgAgg(k, c) for k, c in aggs])
+        aarr = _aggs(aggs)
         codes = np.zeros((max_groups, max(len(keys), 1)), dtype=np.int32)
         vals = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int64)
         nulls = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int8)
@@ -145,6 +145,11 @@ class Context:
                                             _ptr(ocnt), _ptr(ofirst), ctypes.byref(ng)))
         g = ng.value
         return ok[:g], osum[:g], ocnt[:g], (ofirst[:g] if with_first else None)
+
+
+def _aggs(aggs):
+    """[(kind, col)] or [(kind, col, col2)] -> kg_agg array."""
+    return (KgAgg * max(len(aggs), 1))(*[KgAgg(a[0], a[1], a[2] if len(a) > 2 else -1, 0) for a in aggs])
 
 
 def _pred(cols, p):
