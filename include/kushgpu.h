@@ -197,6 +197,24 @@ int kg_ht_size(kg_ctx* ctx, const kg_hash_table* ht, int64_t* out_rows);
 /* Release the table (runtime::HashTable::Free, hash_table.cc:53-59). */
 int kg_ht_free(kg_ctx* ctx, kg_hash_table* ht);
 
+/* ---- star pipeline: scan-select -> join probe -> group-by ----------------------
+ * ScanSelect over the probe-side columns (cols/preds), HashJoin probe of
+ * cols[probe_key_col] into `ht`, then GroupByAggregate keyed by the matched
+ * build row's dense code build_codes[build_row] in [0, group_card), with
+ * aggregates over probe-side columns (kg_agg as in kg_scan_group_agg, products
+ * allowed). One kernel; the group table lives in LDS. Reference chain:
+ * ScanSelectTranslator (scan_select_translator.cc:92-156) ->
+ * HashJoinTranslator::Consume (hash_join_translator.cc:105-139) ->
+ * GroupByAggregateTranslator::Consume (group_by_aggregate_translator.cc:102-111).
+ * Groups come out in the reference's insertion order (first join output row);
+ * out_keys[g] is the group code. FP64 sums are added in timing-dependent order
+ * (within FP64 rounding of the sequential fold). */
+int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
+                           const kg_pred* preds, int32_t npreds, const kg_hash_table* ht,
+                           int32_t probe_key_col, const kg_col* build_codes, int32_t group_card,
+                           const kg_agg* aggs, int32_t naggs, int64_t max_groups, int64_t* out_ngroups,
+                           int32_t* out_keys, int64_t* out_vals, int8_t* out_nulls);
+
 /* ---- high-cardinality group-by ---------------------------------------------------
  * GroupByAggregate on one BIGINT key with SUM(BIGINT) and COUNT(*), for group
  * counts too large for an on-chip table: replaces the salted linear-probe

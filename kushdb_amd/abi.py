@@ -62,6 +62,9 @@ SIGNATURES = {
     "kg_ht_probe_agg": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64, _i64, _vp, _vp, _vp, _vp, _pi64]),
     "kg_ht_size": (_i32, [_vp, _vp, _pi64]),
     "kg_ht_free": (_i32, [_vp, _vp]),
+    "kg_scan_join_group_agg": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32, _vp,
+                                      _i32, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgAgg), _i32, _i64, _pi64,
+                                      _pi32, _pi64, _pi8]),
     "kg_group_by_i64": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _pi64]),
 }
 

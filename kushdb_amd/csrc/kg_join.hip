@@ -20,7 +20,7 @@
 #include <cstring>
 #include <new>
 
-#include "kg_internal.h"
+#include "kg_ops.h"
 
 namespace kg {
 
@@ -313,6 +313,145 @@ __global__ __launch_bounds__(kJBlock) void k_group_by_i64(int64_t* keys, Acc* ac
     atomicAdd(&acc[slot].count, 1ull);
     if (FIRST) atomicMin(&first[slot], static_cast<int32_t>(r));
   }
+}
+
+// Star pipeline in one kernel: ScanSelect on the probe (fact) side, probe of
+// the join table, group code of the matched build row, aggregation into a
+// workgroup-private LDS table, merged into a global table with atomics.
+// Reference chain: ScanSelectTranslator -> HashJoinTranslator::Consume(right)
+// (hash_join_translator.cc:105-139) -> GroupByAggregateTranslator::Consume
+// (group_by_aggregate_translator.cc:102-111). `first` records the first join
+// output row of each group as (probe row << 32 | build row), the order in
+// which the reference's aggregate table inserts groups.
+template <bool MINMAX>
+__global__ __launch_bounds__(kJBlock) void k_join_group_agg(PredSet P, kg_col key, TableView t,
+                                                             const int32_t* codes, int G, MeasureSet M, int64_t n,
+                                                             AggFinal* gfin, unsigned long long* grows,
+                                                             unsigned long long* gfirst) {
+  extern __shared__ unsigned long long lds[];
+  const int nm = M.nm;
+  unsigned long long* lsum = lds;
+  unsigned long long* lnn = lsum + G * nm;
+  unsigned long long* lrows = lnn + G * nm;
+  unsigned long long* lfirst = lrows + G;
+  unsigned long long* lmn = lfirst + G;
+  unsigned long long* lmx = lmn + (MINMAX ? G * nm : 0);
+  const int words = G * nm * 2 + G * 2 + (MINMAX ? 2 * G * nm : 0);
+  for (int i = threadIdx.x; i < words; i += kJBlock) lds[i] = 0;
+  for (int i = threadIdx.x; i < G; i += kJBlock) lfirst[i] = ~0ull;
+  if (MINMAX)
+    for (int i = threadIdx.x; i < G * nm; i += kJBlock) {
+      const bool real = M.is_real[i % nm];
+      lmn[i] = real ? static_cast<unsigned long long>(__double_as_longlong(__builtin_inf()))
+                    : static_cast<unsigned long long>(INT64_MAX);
+      lmx[i] = real ? static_cast<unsigned long long>(__double_as_longlong(-__builtin_inf()))
+                    : static_cast<unsigned long long>(INT64_MIN);
+    }
+  __syncthreads();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kJBlock;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x; r < n; r += stride) {
+    if (!eval_preds(P, r)) continue;
+    int64_t k;
+    if (!load_key(key, r, k)) continue;
+    uint64_t v[kMaxMeasures];
+    bool vnull[kMaxMeasures];
+#pragma unroll
+    for (int m = 0; m < kMaxMeasures; m++)
+      if (m < nm) v[m] = load_measure(M, m, r, vnull[m]);
+    auto consume = [&](int32_t brow) {
+      const int g = codes[brow];
+      if (g < 0 || g >= G) return;
+      atomicAdd(&lrows[g], 1ull);
+      atomicMin(&lfirst[g], (static_cast<unsigned long long>(r) << 32) | static_cast<uint32_t>(brow));
+#pragma unroll
+      for (int m = 0; m < kMaxMeasures; m++) {
+        if (m >= nm || vnull[m]) continue;
+        atomicAdd(&lnn[g * nm + m], 1ull);
+        if (M.is_real[m]) {
+          atomicAdd(reinterpret_cast<double*>(&lsum[g * nm + m]), __longlong_as_double(v[m]));
+          if (MINMAX) {
+            atomicMin(reinterpret_cast<double*>(&lmn[g * nm + m]), __longlong_as_double(v[m]));
+            atomicMax(reinterpret_cast<double*>(&lmx[g * nm + m]), __longlong_as_double(v[m]));
+          }
+        } else {
+          atomicAdd(&lsum[g * nm + m], static_cast<unsigned long long>(v[m]));
+          if (MINMAX) {
+            atomicMin(reinterpret_cast<long long*>(&lmn[g * nm + m]), static_cast<long long>(v[m]));
+            atomicMax(reinterpret_cast<long long*>(&lmx[g * nm + m]), static_cast<long long>(v[m]));
+          }
+        }
+      }
+    };
+    if (k == kEmpty) {
+      unsigned long long ns = *t.side_count;
+      if (ns > static_cast<unsigned long long>(kSideCap)) ns = kSideCap;
+      for (unsigned long long j = 0; j < ns; j++) consume(t.side_rows[j]);
+      continue;
+    }
+    uint64_t b = bucket_of(k, t.nbuckets);
+    for (uint64_t scanned = 0; scanned < t.nbuckets; scanned++) {
+      const longlong2* bk = reinterpret_cast<const longlong2*>(t.keys + b * kSlots);
+      bool empty = false;
+#pragma unroll
+      for (int q = 0; q < kSlots / 2; q++) {
+        const longlong2 kv = bk[q];
+        if (kv.x == k) consume(t.rows[b * kSlots + 2 * q]);
+        if (kv.y == k) consume(t.rows[b * kSlots + 2 * q + 1]);
+        empty |= kv.x == kEmpty || kv.y == kEmpty;
+      }
+      if (empty) break;
+      b = (b + 1 == t.nbuckets) ? 0 : b + 1;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < G; i += kJBlock) {
+    if (lrows[i] == 0) continue;
+    atomicAdd(&grows[i], lrows[i]);
+    atomicMin(&gfirst[i], lfirst[i]);
+  }
+  for (int i = threadIdx.x; i < G * nm; i += kJBlock) {
+    if (lnn[i] == 0) continue;
+    const int m = i % nm;
+    AggFinal* f = &gfin[i];
+    atomicAdd(reinterpret_cast<unsigned long long*>(&f->nn), lnn[i]);
+    if (M.is_real[m]) {
+      atomicAdd(reinterpret_cast<double*>(&f->sum), __longlong_as_double(lsum[i]));
+      if (MINMAX) {
+        atomicMin(reinterpret_cast<double*>(&f->mn), __longlong_as_double(lmn[i]));
+        atomicMax(reinterpret_cast<double*>(&f->mx), __longlong_as_double(lmx[i]));
+      }
+    } else {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&f->sum), lsum[i]);
+      if (MINMAX) {
+        atomicMin(reinterpret_cast<long long*>(&f->mn), static_cast<long long>(lmn[i]));
+        atomicMax(reinterpret_cast<long long*>(&f->mx), static_cast<long long>(lmx[i]));
+      }
+    }
+  }
+}
+
+__global__ void k_init_join_final(AggFinal* fin, int G, MeasureSet M, unsigned long long* rows,
+                                  unsigned long long* first) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nm = M.nm;
+  if (i < G * nm) {
+    const bool real = M.is_real[i % nm];
+    fin[i].sum = 0;
+    fin[i].nn = 0;
+    fin[i].mn = real ? static_cast<uint64_t>(__double_as_longlong(__builtin_inf())) : static_cast<uint64_t>(INT64_MAX);
+    fin[i].mx = real ? static_cast<uint64_t>(__double_as_longlong(-__builtin_inf())) : static_cast<uint64_t>(INT64_MIN);
+  }
+  if (i < G) {
+    rows[i] = 0;
+    first[i] = ~0ull;
+  }
+}
+
+// COUNT(col) over the probe side: whether each group's first join row has a
+// NULL in `nulls` (indexed by probe row = first >> 32).
+__global__ void k_first_null_join(const unsigned long long* first, int G, const int8_t* nulls, int8_t* out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G) out[g] = (first[g] != ~0ull && nulls) ? nulls[first[g] >> 32] : 0;
 }
 
 inline int grid_for(kg_ctx* ctx, int64_t work, int per_thread) {
@@ -638,4 +777,91 @@ extern "C" int kg_group_by_i64(kg_ctx* ctx, const int64_t* d_keys, const int64_t
     return kg_fail(ctx, KG_ERR_CAPACITY, "%lld groups exceed max_groups %lld", (long long)*out_ngroups,
                    (long long)max_groups);
   return KG_OK;
+}
+
+extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
+                                      const kg_pred* preds, int32_t npreds, const kg_hash_table* ht,
+                                      int32_t probe_key_col, const kg_col* build_codes, int32_t group_card,
+                                      const kg_agg* aggs, int32_t naggs, int64_t max_groups, int64_t* out_ngroups,
+                                      int32_t* out_keys, int64_t* out_vals, int8_t* out_nulls) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, ht, "table is NULL");
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count out of range");
+  KG_REQUIRE(ctx, out_ngroups && out_keys && (naggs == 0 || (aggs && out_vals && out_nulls)), "null output pointer");
+  KG_REQUIRE(ctx, naggs >= 0 && naggs <= 16, "at most 16 aggregates");
+  KG_REQUIRE(ctx, probe_key_col >= 0 && probe_key_col < ncols, "bad probe key column %d", probe_key_col);
+  KG_REQUIRE(ctx, build_codes && build_codes->data, "build group codes are NULL");
+  if (build_codes->type != KG_INT && build_codes->type != KG_ENUM)
+    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "build group codes must be INT/ENUM codes");
+  if (build_codes->nulls) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "nullable build group codes");
+  KG_REQUIRE(ctx, group_card > 0, "group cardinality must be positive");
+  int st = check_key(ctx, &cols[probe_key_col]);
+  if (st != KG_OK) return st;
+  PredSet P;
+  if ((st = build_preds(ctx, cols, ncols, preds, npreds, 1, P)) != KG_OK) return st;
+  MeasureSet M;
+  std::vector<AggPlan> plan;
+  bool minmax = false;
+  if ((st = build_measures(ctx, cols, ncols, aggs, naggs, M, plan, minmax)) != KG_OK) return st;
+  const int G = group_card;
+  const int nm = M.nm;
+  const size_t lds_bytes = (static_cast<size_t>(G) * nm * 2 + G * 2 + (minmax ? 2 * G * nm : 0)) * 8;
+  if (lds_bytes > 64 * 1024)
+    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "group table of %zu bytes exceeds the LDS budget", lds_bytes);
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  *out_ngroups = 0;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_fin = carve(static_cast<size_t>(G) * (nm > 0 ? nm : 1) * sizeof(AggFinal)), o_rows = carve(G * 8),
+               o_first = carve(G * 8), o_fnull = carve(G + 16);
+  if ((st = kg_work_reserve(ctx, off)) != KG_OK) return st;
+  char* w = static_cast<char*>(ctx->work);
+  AggFinal* dfin = reinterpret_cast<AggFinal*>(w + o_fin);
+  unsigned long long* drows = reinterpret_cast<unsigned long long*>(w + o_rows);
+  unsigned long long* dfirst = reinterpret_cast<unsigned long long*>(w + o_first);
+  const int init_threads = std::max(G * nm, G);
+  hipLaunchKernelGGL(k_init_join_final, dim3(ceil_div(init_threads, 256)), dim3(256), 0, ctx->stream, dfin, G, M,
+                     drows, dfirst);
+  KG_LAUNCH_CHECK(ctx);
+  if (n > 0) {
+    const int nb = grid_for(ctx, n, 16);
+    kg_time_begin(ctx);
+    if (minmax)
+      hipLaunchKernelGGL(k_join_group_agg<true>, dim3(nb), dim3(kJBlock), lds_bytes, ctx->stream, P,
+                         cols[probe_key_col], ht->view(), static_cast<const int32_t*>(build_codes->data), G, M, n,
+                         dfin, drows, dfirst);
+    else
+      hipLaunchKernelGGL(k_join_group_agg<false>, dim3(nb), dim3(kJBlock), lds_bytes, ctx->stream, P,
+                         cols[probe_key_col], ht->view(), static_cast<const int32_t*>(build_codes->data), G, M, n,
+                         dfin, drows, dfirst);
+    kg_time_end(ctx);
+    KG_LAUNCH_CHECK(ctx);
+  }
+  std::vector<AggFinal> fin(static_cast<size_t>(G) * (nm > 0 ? nm : 1));
+  std::vector<uint64_t> rowsv(G), firstu(G);
+  std::vector<std::vector<int8_t>> first_null(naggs);
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(fin.data(), dfin, static_cast<size_t>(G) * nm * sizeof(AggFinal),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(rowsv.data(), drows, G * 8, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(firstu.data(), dfirst, G * 8, hipMemcpyDeviceToHost, ctx->stream));
+  int8_t* dfn = reinterpret_cast<int8_t*>(w + o_fnull);
+  for (int a = 0; a < naggs; a++) {
+    if (plan[a].kind != KG_AGG_COUNT || !plan[a].count_nulls) continue;
+    first_null[a].resize(G);
+    hipLaunchKernelGGL(k_first_null_join, dim3(ceil_div(G, 256)), dim3(256), 0, ctx->stream, dfirst, G,
+                       plan[a].count_nulls, dfn);
+    KG_LAUNCH_CHECK(ctx);
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(first_null[a].data(), dfn, G, hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int64_t> first64(G);
+  // (probe row << 32 | build row) < 2^63 for every present group.
+  for (int g = 0; g < G; g++) first64[g] = static_cast<int64_t>(firstu[g]);
+  return emit_groups(ctx, G, 1, 0, M, plan, fin, rowsv, first64, first_null, max_groups, out_ngroups, out_keys,
+                     out_vals, out_nulls);
 }

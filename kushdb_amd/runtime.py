@@ -212,6 +212,26 @@ class HashTable:
         g = ng.value
         return ok[:g], osum[:g], ocnt[:g], (ofirst[:g] if with_first else None)
 
+    def scan_probe_group_agg(self, cols, preds, probe_key_col, build_codes, group_card, aggs, max_groups=4096):
+        """ScanSelect(cols, preds) -> probe cols[probe_key_col] -> GROUP BY
+        build_codes[build row] with aggs over probe-side columns. Returns
+        (codes [G], values int64 [G, naggs], nulls [G, naggs]) in insertion order."""
+        ctx = self.ctx
+        n = len(cols[0]) if cols else 0
+        carr = (KgCol * max(len(cols), 1))(*[c.desc() for c in cols])
+        parr = (KgPred * max(len(preds), 1))(*[_pred(cols, p) for p in preds])
+        codes = np.zeros(max_groups, dtype=np.int32)
+        vals = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int64)
+        nulls = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int8)
+        ng = ctypes.c_int64()
+        bc = build_codes.desc()
+        ctx.check(ctx.lib.kg_scan_join_group_agg(
+            ctx.h, n, carr, len(cols), parr, len(preds), self.h, probe_key_col, ctypes.byref(bc), group_card,
+            _aggs(aggs), len(aggs), max_groups, ctypes.byref(ng), codes.ctypes.data_as(abi._pi32),
+            vals.ctypes.data_as(abi._pi64), nulls.ctypes.data_as(abi._pi8)))
+        g = ng.value
+        return codes[:g], vals[:g, : len(aggs)], nulls[:g, : len(aggs)]
+
     def size(self):
         v = ctypes.c_int64()
         self.ctx.check(self.ctx.lib.kg_ht_size(self.ctx.h, self.h, ctypes.byref(v)))
