@@ -228,6 +228,57 @@ int kg_group_by_i64(kg_ctx* ctx, const int64_t* d_keys, const int64_t* d_vals, i
                     int64_t max_groups, int64_t* d_out_key, int64_t* d_out_sum,
                     int64_t* d_out_count, int32_t* d_out_first, int64_t* out_ngroups);
 
+/* ---- column indexes (Skinner joins) --------------------------------------------
+ * Device replacement for runtime::MemoryColumnIndex
+ * (runtime/memory_column_index.h:18-56, compile/proxy/memory_column_index.cc:282-428)
+ * and, for unfiltered children, the disk index runtime::ColumnIndex
+ * (runtime/column_index.cc:73-95): key -> bucket of tuple ids in ascending
+ * order. Keys are one SMALLINT/INT/DATE/ENUM column; NULL keys are not indexed
+ * (permutable_skinner_join_translator.cc:972-994). The tuple id of a row is its
+ * position in d_rows (its position in the materialized child), or its row id
+ * when d_rows is NULL. */
+typedef struct kg_index kg_index;
+int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_rows, int64_t nrows, kg_index** out);
+/* Device tuple-id array (grouped by key, ascending per key), its length and
+ * the number of distinct keys. */
+int kg_index_info(kg_ctx* ctx, const kg_index* idx, const int32_t** d_tids, int64_t* ntids, int64_t* nkeys);
+/* Batched GetBucket (compile/proxy/memory_column_index.cc:282-330; EMPTY for a
+ * miss, runtime/column_index_bucket.cc:13-14): for each probe key (rows of
+ * `key`, or d_rows[i]) the bucket as a device pointer into the tid array and a
+ * size (0 for a missing or NULL key). Writes device arrays only. */
+int kg_index_lookup(kg_ctx* ctx, const kg_index* idx, const kg_col* key, const int32_t* d_rows, int64_t n,
+                    const int32_t** d_out_ptr, int32_t* d_out_count);
+int kg_index_free(kg_ctx* ctx, kg_index* idx);
+
+/* Batched sorted intersection (runtime/column_index_bucket.cc:59-182,
+ * BucketListSortedIntersectionInit + PopulateResult drained to the end): task
+ * t intersects the k ascending lists d_lists[t*k .. t*k+k-1] (sizes d_sizes)
+ * from d_next_tuple[t] on (NULL = from the start). Results are written
+ * ascending per task at d_out + d_out_offsets[t]; d_out_offsets has ntasks+1
+ * entries. The reference's chunked form can repeat the last id of a 64-id
+ * chunk when three or more lists are intersected; this returns the set, which
+ * is what the downstream tuple dedup keeps. */
+int kg_intersect_batch(kg_ctx* ctx, int64_t ntasks, int32_t k, const int32_t* const* d_lists,
+                       const int32_t* d_sizes, const int32_t* d_next_tuple, int64_t* d_out_offsets,
+                       int32_t* d_out, int64_t out_cap, int64_t* out_total);
+
+/* ---- tuple-id set ----------------------------------------------------------------
+ * Device replacement for runtime::TupleIdxTable (runtime/tuple_idx_table/
+ * tuple_idx_table.h:18-51), the result dedup of the Skinner valid-tuple handler
+ * (permutable_skinner_join_translator.cc:842-912): a set of int32[len] tuples
+ * (len <= 8). */
+typedef struct kg_tuple_set kg_tuple_set;
+int kg_tuple_set_create(kg_ctx* ctx, int32_t len, int64_t capacity, kg_tuple_set** out);
+/* Insert n tuples (d_tuples[n*len]). d_is_new[i] (optional) = Insert()'s
+ * result when the batch is inserted in order: 1 for the first occurrence of a
+ * tuple not in the set before the call. *out_new = number of tuples added. */
+int kg_tuple_set_insert(kg_ctx* ctx, kg_tuple_set* s, const int32_t* d_tuples, int64_t n, uint8_t* d_is_new,
+                        int64_t* out_new);
+/* Copy the distinct tuples to d_out (unordered; the reference's byte-wise
+ * iteration order is a sort of this output). */
+int kg_tuple_set_extract(kg_ctx* ctx, const kg_tuple_set* s, int32_t* d_out, int64_t cap, int64_t* out_n);
+int kg_tuple_set_free(kg_ctx* ctx, kg_tuple_set* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,15 @@ SIGNATURES = {
     "kg_scan_join_group_agg": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32, _vp,
                                       _i32, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgAgg), _i32, _i64, _pi64,
                                       _pi32, _pi64, _pi8]),
+    "kg_index_build": (_i32, [_vp, ctypes.POINTER(KgCol), _vp, _i64, ctypes.POINTER(_vp)]),
+    "kg_index_info": (_i32, [_vp, _vp, ctypes.POINTER(_vp), _pi64, _pi64]),
+    "kg_index_lookup": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64, _vp, _vp]),
+    "kg_index_free": (_i32, [_vp, _vp]),
+    "kg_intersect_batch": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _pi64]),
+    "kg_tuple_set_create": (_i32, [_vp, _i32, _i64, ctypes.POINTER(_vp)]),
+    "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),
+    "kg_tuple_set_extract": (_i32, [_vp, _vp, _vp, _i64, _pi64]),
+    "kg_tuple_set_free": (_i32, [_vp, _vp]),
     "kg_group_by_i64": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _pi64]),
 }
 

@@ -22,6 +22,7 @@ SOURCES = [
     "kg_context.hip",
     "kg_scan.hip",
     "kg_join.hip",
+    "kg_index.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

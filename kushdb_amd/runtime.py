@@ -247,3 +247,121 @@ class HashTable:
             self.free()
         except Exception:
             pass
+
+
+class ColumnIndex:
+    """Device key -> ascending tuple-id buckets (replaces runtime::MemoryColumnIndex)."""
+
+    def __init__(self, ctx, key_col, rows=None):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        nrows = rows.numel() if rows is not None else len(key_col)
+        d = key_col.desc()
+        ctx.check(ctx.lib.kg_index_build(ctx.h, ctypes.byref(d), _ptr(rows), nrows, ctypes.byref(h)))
+        self.h = h
+        p = ctypes.c_void_p()
+        nt = ctypes.c_int64()
+        nk = ctypes.c_int64()
+        ctx.check(ctx.lib.kg_index_info(ctx.h, self.h, ctypes.byref(p), ctypes.byref(nt), ctypes.byref(nk)))
+        self.tids_ptr = p.value or 0
+        self.ntids = nt.value
+        self.nkeys = nk.value
+
+    def lookup(self, key_col, rows=None):
+        """Per probe key: (device pointers int64 tensor, sizes int32 tensor)."""
+        n = rows.numel() if rows is not None else len(key_col)
+        dev = key_col.data.device
+        ptrs = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+        cnts = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        d = key_col.desc()
+        self.ctx.check(self.ctx.lib.kg_index_lookup(self.ctx.h, self.h, ctypes.byref(d), _ptr(rows), n, _ptr(ptrs),
+                                                    _ptr(cnts)))
+        return ptrs[:n], cnts[:n]
+
+    def offsets_of(self, ptrs):
+        """Device pointers from lookup() as element offsets into the tid array."""
+        return (ptrs - self.tids_ptr) // 4
+
+    def tids(self):
+        """The tid array as a tensor view (device)."""
+        if self.ntids == 0:
+            return torch.empty(0, dtype=torch.int32, device=f"cuda:{self.ctx.device}")
+        return _device_view_i32(self.tids_ptr, self.ntids, self.ctx.device)
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.kg_index_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _DeviceArray:
+    """__cuda_array_interface__ over library-owned device memory (no copy)."""
+
+    def __init__(self, ptr, n, typestr):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, True), "version": 2}
+
+
+def _device_view_i32(ptr, n, device):
+    """int32 tensor view of n values at a library-owned device address."""
+    return torch.as_tensor(_DeviceArray(ptr, n, "<i4"), device=f"cuda:{device}")
+
+
+def intersect_batch(ctx, ptrs, sizes, k, next_tuple=None, out_cap=None):
+    """ptrs/sizes: [ntasks*k] device int64 pointers / int32 sizes. Returns
+    (offsets [ntasks+1] int64, ids int32)."""
+    ntasks = sizes.numel() // k
+    dev = sizes.device
+    offs = torch.empty(ntasks + 1, dtype=torch.int64, device=dev)
+    if out_cap is None:
+        out_cap = int(sizes.view(ntasks, k).min(dim=1).values.sum().item()) if ntasks else 0
+    out = torch.empty(max(out_cap, 1), dtype=torch.int32, device=dev)
+    tot = ctypes.c_int64()
+    ctx.check(ctx.lib.kg_intersect_batch(ctx.h, ntasks, k, _ptr(ptrs), _ptr(sizes), _ptr(next_tuple), _ptr(offs),
+                                         _ptr(out), out_cap, ctypes.byref(tot)))
+    return offs, out[: tot.value]
+
+
+class TupleSet:
+    """Device set of int32[len] tuples (replaces runtime::TupleIdxTable)."""
+
+    def __init__(self, ctx, length, capacity):
+        self.ctx = ctx
+        self.len = length
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.kg_tuple_set_create(ctx.h, length, capacity, ctypes.byref(h)))
+        self.h = h
+        self.size = 0
+
+    def insert(self, tuples):
+        """tuples: [n, len] int32 device tensor. Returns is_new (uint8 tensor)."""
+        n = tuples.shape[0]
+        is_new = torch.empty(max(n, 1), dtype=torch.uint8, device=tuples.device)
+        k = ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.kg_tuple_set_insert(self.ctx.h, self.h, _ptr(tuples.contiguous()), n,
+                                                        _ptr(is_new), ctypes.byref(k)))
+        self.size += k.value
+        return is_new[:n]
+
+    def extract(self):
+        out = torch.empty((max(self.size, 1), self.len), dtype=torch.int32, device=f"cuda:{self.ctx.device}")
+        k = ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.kg_tuple_set_extract(self.ctx.h, self.h, _ptr(out), max(self.size, 1),
+                                                         ctypes.byref(k)))
+        return out[: k.value]
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.kg_tuple_set_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
