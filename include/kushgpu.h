@@ -94,6 +94,22 @@ typedef struct kg_agg {
   int32_t reserved;
 } kg_agg;
 
+/* A resident TEXT column: value i is bytes[offsets[i] .. offsets[i+1]); the
+ * slot table of the .kdb TEXT format (runtime/column_data.cc:175-229)
+ * flattened to prefix offsets (n+1 of them). nulls as in kg_col. */
+typedef struct kg_text_col {
+  const int64_t* offsets;
+  const uint8_t* bytes;
+  const int8_t* nulls;
+} kg_text_col;
+
+/* TEXT operators: comparisons with a literal, then the string predicates of
+ * runtime/string.cc:14-82 (StartsWith, EndsWith, Contains, Like). */
+enum kg_text_op {
+  KG_TEXT_EQ = 0, KG_TEXT_NEQ = 1, KG_TEXT_LT = 2, KG_TEXT_LEQ = 3, KG_TEXT_GT = 4, KG_TEXT_GEQ = 5,
+  KG_TEXT_STARTS_WITH = 6, KG_TEXT_ENDS_WITH = 7, KG_TEXT_CONTAINS = 8, KG_TEXT_LIKE = 9,
+};
+
 typedef struct kg_ctx kg_ctx;
 typedef struct kg_hash_table kg_hash_table;
 
@@ -278,6 +294,48 @@ int kg_tuple_set_insert(kg_ctx* ctx, kg_tuple_set* s, const int32_t* d_tuples, i
  * iteration order is a sort of this output). */
 int kg_tuple_set_extract(kg_ctx* ctx, const kg_tuple_set* s, int32_t* d_out, int64_t cap, int64_t* out_n);
 int kg_tuple_set_free(kg_ctx* ctx, kg_tuple_set* s);
+
+/* ---- TEXT ------------------------------------------------------------------------
+ * ScanSelect with one TEXT predicate `col op pattern` over rows 0..n-1 or over
+ * the selection d_rows_in[0..n): qualifying row ids in input order to
+ * d_rows_out. Semantics: byte-wise string_view comparisons and
+ * StartsWith/EndsWith/Contains (runtime/string.cc:14-82); KG_TEXT_LIKE is the
+ * anchored match of the planner's regex (plan/planner.cc:284-323: '%' any
+ * sequence, '_' one character, everything else literal). The planner's
+ * rewrites of LIKE '%x' to STARTS_WITH(x) and 'x%' to ENDS_WITH(x)
+ * (planner.cc:249-269) happen before this call. Pattern <= 240 bytes. */
+int kg_text_select(kg_ctx* ctx, const kg_text_col* col, int32_t op, const char* pattern, int32_t plen,
+                   const int32_t* d_rows_in, int64_t n, int32_t* d_rows_out, int64_t* out_count);
+/* MIN (is_max = 0) or MAX over the non-NULL values of col at rows d_rows[0..n)
+ * (or 0..n-1): *out_pos = the input position of the first minimal value (-1
+ * when all are NULL / n = 0), its bytes copied to out_buf (up to buf_cap) and
+ * its length to *out_len (MinMaxAggregator over TEXT, aggregator.cc:155-160). */
+int kg_text_minmax(kg_ctx* ctx, const kg_text_col* col, const int32_t* d_rows, int64_t n, int32_t is_max,
+                   int64_t* out_pos, char* out_buf, int64_t buf_cap, int64_t* out_len);
+
+/* ---- index-nested-loop join step (Skinner table functions) ----------------------
+ * Extends each partial join tuple d_tuples[i*width .. +width) (tuple ids per
+ * table) by every tuple id of idx's bucket for the key src_key[row], where
+ * row = d_src_rows[tuple[src_pos]] (or tuple[src_pos] when d_src_rows is
+ * NULL): the GetBucket + handler chain of
+ * permutable_skinner_join_translator.cc:359-433. Output tuples have width+1
+ * ids, grouped by input tuple in input order, bucket order (ascending ids)
+ * inside a group. *out_n receives the total even when it exceeds out_cap
+ * (KG_ERR_CAPACITY). */
+int kg_index_join_expand(kg_ctx* ctx, int64_t n, int32_t width, const int32_t* d_tuples, int32_t src_pos,
+                         const kg_col* src_key, const int32_t* d_src_rows, const kg_index* idx,
+                         int64_t out_cap, int32_t* d_out, int64_t* out_n);
+
+/* Row ids of one table in a tuple batch: d_out[i] = d_sel[tuple_i[pos]] (or
+ * tuple_i[pos] when d_sel is NULL) — the tuple id -> materialized row step the
+ * Skinner valid-tuple handler does before emitting
+ * (permutable_skinner_join_translator.cc:842-912). Device output only. */
+int kg_tuple_rows(kg_ctx* ctx, const int32_t* d_tuples, int32_t width, int32_t pos, const int32_t* d_sel,
+                  int64_t n, int32_t* d_out);
+/* Materialize src[d_rows[i]] (and its NULL flag when d_out_nulls is set) for a
+ * fixed-width column (runtime::ColumnData::Get* per row,
+ * runtime/column_data.cc:96-108, done once per batch). Device output only. */
+int kg_gather(kg_ctx* ctx, const kg_col* src, const int32_t* d_rows, int64_t n, void* d_out, int8_t* d_out_nulls);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,15 @@ class KgPred(ctypes.Structure):
                 ("ilit", ctypes.c_int64), ("flit", ctypes.c_double)]
 
 
+class KgTextCol(ctypes.Structure):
+    _fields_ = [("offsets", ctypes.c_void_p), ("bytes", ctypes.c_void_p), ("nulls", ctypes.c_void_p)]
+
+
+# enum kg_text_op
+(KG_TEXT_EQ, KG_TEXT_NEQ, KG_TEXT_LT, KG_TEXT_LEQ, KG_TEXT_GT, KG_TEXT_GEQ, KG_TEXT_STARTS_WITH, KG_TEXT_ENDS_WITH,
+ KG_TEXT_CONTAINS, KG_TEXT_LIKE) = range(10)
+
+
 class KgAgg(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("col", ctypes.c_int32), ("col2", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -74,6 +83,11 @@ SIGNATURES = {
     "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),
     "kg_tuple_set_extract": (_i32, [_vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_free": (_i32, [_vp, _vp]),
+    "kg_text_select": (_i32, [_vp, ctypes.POINTER(KgTextCol), _i32, ctypes.c_char_p, _i32, _vp, _i64, _vp, _pi64]),
+    "kg_text_minmax": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, _i64, _i32, _pi64, ctypes.c_char_p, _i64, _pi64]),
+    "kg_index_join_expand": (_i32, [_vp, _i64, _i32, _vp, _i32, ctypes.POINTER(KgCol), _vp, _vp, _i64, _vp, _pi64]),
+    "kg_tuple_rows": (_i32, [_vp, _vp, _i32, _i32, _vp, _i64, _vp]),
+    "kg_gather": (_i32, [_vp, ctypes.POINTER(KgCol), _vp, _i64, _vp, _vp]),
     "kg_group_by_i64": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _pi64]),
 }
 

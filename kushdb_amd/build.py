@@ -23,6 +23,7 @@ SOURCES = [
     "kg_scan.hip",
     "kg_join.hip",
     "kg_index.hip",
+    "kg_text.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -493,6 +493,15 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
   return KG_OK;
 }
 
+void kg::index_dir(const kg_index* idx, const int64_t** keys, const int64_t** offset, const int32_t** count,
+                   uint64_t* nbuckets, const int32_t** tids) {
+  *keys = idx->dir.keys;
+  *offset = idx->dir.offset;
+  *count = idx->dir.count;
+  *nbuckets = idx->dir.nbuckets;
+  *tids = idx->tids;
+}
+
 extern "C" int kg_index_info(kg_ctx* ctx, const kg_index* idx, const int32_t** d_tids, int64_t* ntids,
                              int64_t* nkeys) {
   if (!ctx) return KG_ERR_INVALID;

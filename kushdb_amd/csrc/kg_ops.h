@@ -147,4 +147,8 @@ int emit_groups(kg_ctx* ctx, int64_t G, int32_t nkeys, int32_t card1, const Meas
                 const std::vector<std::vector<int8_t>>& first_null, int64_t max_groups, int64_t* out_ngroups,
                 int32_t* out_keys, int64_t* out_vals, int8_t* out_nulls);
 
+// Directory and tid array of a column index (kg_index.hip).
+void index_dir(const kg_index* idx, const int64_t** keys, const int64_t** offset, const int32_t** count,
+               uint64_t* nbuckets, const int32_t** tids);
+
 }  // namespace kg

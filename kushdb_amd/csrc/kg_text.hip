@@ -1,0 +1,464 @@
+// TEXT predicates, MIN/MAX over TEXT, and the index-nested-loop expansion
+// step of a multi-way join.
+//
+// Reference behaviour:
+//  * runtime/string.cc:14-82: StartsWith (rfind(p, 0) == 0), EndsWith,
+//    Contains, Equals and the comparisons are byte-wise string_view
+//    operations; Like is an anchored regex full match built by the planner
+//    (plan/planner.cc:284-323): '%' -> ".*", '_' -> ".", every other pattern
+//    byte literal. (The planner rewrites LIKE '%x' to STARTS_WITH and 'x%' to
+//    ENDS_WITH, planner.cc:249-269; callers pass the rewritten operator.)
+//  * MIN/MAX over TEXT compare with string_view's operator< (unsigned bytes,
+//    a proper prefix first) and keep the first value among equals
+//    (compile/proxy/aggregator.cc:155-160).
+//  * The Skinner table functions extend a partial tuple by the bucket of the
+//    next table's index for the join key (permutable_skinner_join_translator.cc:359-433).
+//
+// TEXT layout on the device: offsets[n+1] (int64) into a byte heap, plus an
+// optional int8 NULL array — the slot table of the .kdb TEXT format
+// (runtime/column_data.cc:175-229) flattened to prefix offsets.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "kg_ops.h"
+
+namespace kg {
+
+constexpr int kTBlock = 256;
+constexpr int kMaxPattern = 240;
+
+struct Pattern {
+  int32_t op;
+  int32_t len;
+  uint8_t p[kMaxPattern];
+};
+
+// a < b on unsigned bytes, a proper prefix sorting first; returns -1/0/1.
+__device__ __forceinline__ int str_cmp(const uint8_t* a, int64_t na, const uint8_t* b, int64_t nb) {
+  const int64_t m = na < nb ? na : nb;
+  for (int64_t i = 0; i < m; i++) {
+    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+  }
+  return na < nb ? -1 : (na > nb ? 1 : 0);
+}
+
+// Anchored LIKE: '%' matches any sequence, '_' exactly one byte, other bytes
+// themselves. Greedy matching with backtracking to the last '%' (linear in
+// practice, O(n*m) worst case), equivalent to the anchored regex full match.
+__device__ bool like_match(const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
+  int64_t i = 0, star_i = -1;
+  int32_t j = 0, star_j = -1;
+  while (i < n) {
+    if (j < m && p[j] == '%') {
+      star_j = j++;
+      star_i = i;
+    } else if (j < m && (p[j] == '_' || p[j] == s[i])) {
+      i++;
+      j++;
+    } else if (star_j >= 0) {
+      j = star_j + 1;
+      i = ++star_i;
+    } else {
+      return false;
+    }
+  }
+  while (j < m && p[j] == '%') j++;
+  return j == m;
+}
+
+__device__ bool text_pred(const Pattern& P, const uint8_t* s, int64_t n) {
+  const uint8_t* p = P.p;
+  const int32_t m = P.len;
+  switch (P.op) {
+    case KG_TEXT_STARTS_WITH:
+      if (n < m) return false;
+      for (int32_t i = 0; i < m; i++)
+        if (s[i] != p[i]) return false;
+      return true;
+    case KG_TEXT_ENDS_WITH:
+      if (n < m) return false;
+      for (int32_t i = 0; i < m; i++)
+        if (s[n - m + i] != p[i]) return false;
+      return true;
+    case KG_TEXT_CONTAINS: {
+      if (m == 0) return true;
+      for (int64_t st = 0; st + m <= n; st++) {
+        int32_t i = 0;
+        while (i < m && s[st + i] == p[i]) i++;
+        if (i == m) return true;
+      }
+      return false;
+    }
+    case KG_TEXT_LIKE:
+      return like_match(s, n, p, m);
+    default: {
+      const int c = str_cmp(s, n, p, m);
+      switch (P.op) {
+        case KG_TEXT_EQ: return c == 0;
+        case KG_TEXT_NEQ: return c != 0;
+        case KG_TEXT_LT: return c < 0;
+        case KG_TEXT_LEQ: return c <= 0;
+        case KG_TEXT_GT: return c > 0;
+        default: return c >= 0;
+      }
+    }
+  }
+}
+
+__global__ void k_text_flags(kg_text_col col, Pattern P, const int32_t* sel, int64_t n, int32_t* ids,
+                             uint8_t* flags) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t r = sel ? sel[i] : i;
+    bool ok = false;
+    if (!(col.nulls && col.nulls[r])) {
+      const int64_t a = col.offsets[r], b = col.offsets[r + 1];
+      ok = text_pred(P, col.bytes + a, b - a);
+    }
+    ids[i] = static_cast<int32_t>(r);
+    flags[i] = ok ? 1 : 0;
+  }
+}
+
+// Position (in the input order) of the MIN (or MAX) non-NULL string; among
+// equal strings the earliest position wins. One candidate per thread, then
+// a tree over the workgroup; per-workgroup winners go to `best`.
+__device__ __forceinline__ bool better(const kg_text_col& c, const int32_t* sel, int64_t a, int64_t b, bool is_max) {
+  // true when position a should replace position b (b < 0 = none yet)
+  if (a < 0) return false;
+  if (b < 0) return true;
+  const int64_t ra = sel ? sel[a] : a, rb = sel ? sel[b] : b;
+  const int cmp = str_cmp(c.bytes + c.offsets[ra], c.offsets[ra + 1] - c.offsets[ra], c.bytes + c.offsets[rb],
+                          c.offsets[rb + 1] - c.offsets[rb]);
+  if (cmp != 0) return is_max ? cmp > 0 : cmp < 0;
+  return a < b;
+}
+
+__global__ __launch_bounds__(kTBlock) void k_text_best(kg_text_col c, const int32_t* sel, int64_t n, int is_max,
+                                                        int64_t* best) {
+  __shared__ int64_t cand[kTBlock];
+  int64_t mine = -1;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kTBlock;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kTBlock + threadIdx.x; i < n; i += stride) {
+    const int64_t r = sel ? sel[i] : i;
+    if (c.nulls && c.nulls[r]) continue;
+    if (better(c, sel, i, mine, is_max)) mine = i;
+  }
+  cand[threadIdx.x] = mine;
+  __syncthreads();
+  for (int s = kTBlock / 2; s >= 1; s >>= 1) {
+    if (threadIdx.x < s) {
+      const int64_t o = cand[threadIdx.x + s];
+      if (better(c, sel, o, cand[threadIdx.x], is_max)) cand[threadIdx.x] = o;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) best[blockIdx.x] = cand[0];
+}
+
+__global__ void k_text_best_final(kg_text_col c, const int32_t* sel, const int64_t* best, int nb, int is_max,
+                                  int64_t* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int64_t b = -1;
+  for (int i = 0; i < nb; i++)
+    if (better(c, sel, best[i], b, is_max)) b = best[i];
+  *out = b;
+}
+
+// --- index-nested-loop expansion --------------------------------------------
+
+struct Expand {
+  int32_t width;          // partial tuple width
+  int32_t src_pos;        // tuple position holding the probing table's tuple id
+  kg_col key;             // key column of that table (indexed by row id)
+  const int32_t* src_rows;  // that table's selection: tuple id -> row id (NULL = identity)
+  const int32_t* tids;    // the index's tid array
+};
+
+struct DirView {
+  const int64_t* keys;
+  const int64_t* offset;
+  const int32_t* count;
+  uint64_t nbuckets;
+};
+
+__device__ __forceinline__ void dir_find(const DirView& d, int64_t k, int64_t& off, int32_t& cnt) {
+  off = 0;
+  cnt = 0;
+  uint64_t b = __umul64hi(mix64(static_cast<uint64_t>(k)), d.nbuckets);
+  for (uint64_t probes = 0; probes < d.nbuckets; probes++) {
+    bool empty = false;
+    for (int s = 0; s < 8; s++) {
+      const int64_t v = d.keys[b * 8 + s];
+      if (v == k) {
+        off = d.offset[b * 8 + s];
+        cnt = d.count[b * 8 + s];
+        return;
+      }
+      empty |= v == INT64_MIN;
+    }
+    if (empty) return;
+    b = (b + 1 == d.nbuckets) ? 0 : b + 1;
+  }
+}
+
+// Pass 1: bucket (offset, size) per partial tuple.
+__global__ void k_expand_count(Expand e, DirView d, const int32_t* tuples, int64_t n, int64_t* boff, int64_t* bcnt) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int32_t tid = tuples[i * e.width + e.src_pos];
+    const int64_t r = e.src_rows ? e.src_rows[tid] : tid;
+    int64_t off = 0;
+    int32_t cnt = 0;
+    if (!(e.key.nulls && e.key.nulls[r])) dir_find(d, static_cast<int32_t>(load_int(e.key.type, e.key.data, r)), off, cnt);
+    boff[i] = off;
+    bcnt[i] = cnt;
+  }
+}
+
+// Pass 2: one thread per output tuple; its source tuple by binary search over
+// the exclusive-scanned counts (balanced under skewed bucket sizes).
+__global__ void k_expand_write(Expand e, const int32_t* tuples, int64_t n, const int64_t* boff,
+                               const int64_t* starts, int64_t total, int32_t* out) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int w1 = e.width + 1;
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < total; j += stride) {
+    int64_t lo = 0, hi = n;  // last i with starts[i] <= j
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (starts[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    const int64_t i = lo;
+    for (int q = 0; q < e.width; q++) out[j * w1 + q] = tuples[i * e.width + q];
+    out[j * w1 + e.width] = e.tids[boff[i] + (j - starts[i])];
+  }
+}
+
+// Row ids of one position of a tuple batch: out[i] = sel[t[i*width+pos]].
+__global__ void k_tuple_rows(const int32_t* t, int32_t width, int32_t pos, const int32_t* sel, int64_t n,
+                             int32_t* out) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int32_t v = t[i * width + pos];
+    out[i] = sel ? sel[v] : v;
+  }
+}
+
+// Materialize rows[i] of a fixed-width column (and its NULL flag).
+__global__ void k_gather(const uint8_t* src, const int8_t* src_nulls, int w, const int32_t* rows, int64_t n,
+                         uint8_t* dst, int8_t* dst_nulls) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t r = rows[i];
+    switch (w) {
+      case 1: dst[i] = src[r]; break;
+      case 2: reinterpret_cast<int16_t*>(dst)[i] = reinterpret_cast<const int16_t*>(src)[r]; break;
+      case 4: reinterpret_cast<int32_t*>(dst)[i] = reinterpret_cast<const int32_t*>(src)[r]; break;
+      default: reinterpret_cast<int64_t*>(dst)[i] = reinterpret_cast<const int64_t*>(src)[r]; break;
+    }
+    if (dst_nulls) dst_nulls[i] = src_nulls ? src_nulls[r] : 0;
+  }
+}
+
+inline int tgrid(kg_ctx* ctx, int64_t work) {
+  int64_t g = ceil_div(std::max<int64_t>(work, 1), kTBlock * 4);
+  return static_cast<int>(std::min<int64_t>(g, static_cast<int64_t>(ctx->num_cus) * 16));
+}
+
+}  // namespace kg
+
+using namespace kg;
+
+namespace {
+
+int check_text(kg_ctx* ctx, const kg_text_col* c) {
+  KG_REQUIRE(ctx, c && c->offsets, "text column is NULL");
+  return KG_OK;
+}
+
+}  // namespace
+
+extern "C" int kg_text_select(kg_ctx* ctx, const kg_text_col* col, int32_t op, const char* pattern, int32_t plen,
+                              const int32_t* d_rows_in, int64_t n, int32_t* d_rows_out, int64_t* out_count) {
+  if (!ctx) return KG_ERR_INVALID;
+  int st = check_text(ctx, col);
+  if (st != KG_OK) return st;
+  KG_REQUIRE(ctx, op >= KG_TEXT_EQ && op <= KG_TEXT_LIKE, "bad text operator %d", op);
+  KG_REQUIRE(ctx, plen >= 0 && plen <= kMaxPattern, "pattern length %d out of 0..%d", plen, kMaxPattern);
+  KG_REQUIRE(ctx, plen == 0 || pattern, "pattern is NULL");
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count out of range");
+  KG_REQUIRE(ctx, out_count && (n == 0 || d_rows_out), "null output pointer");
+  *out_count = 0;
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  Pattern P{};
+  P.op = op;
+  P.len = plen;
+  if (plen) std::memcpy(P.p, pattern, plen);
+  size_t cub_sel = 0;
+  hipcub::DeviceSelect::Flagged(nullptr, cub_sel, static_cast<int32_t*>(nullptr), static_cast<uint8_t*>(nullptr),
+                                static_cast<int32_t*>(nullptr), static_cast<int64_t*>(nullptr), static_cast<int>(n),
+                                ctx->stream);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_ids = carve(n * 4), o_fl = carve(n), o_cnt = carve(16), o_cub = carve(cub_sel);
+  if ((st = kg_work_reserve(ctx, off)) != KG_OK) return st;
+  char* w = static_cast<char*>(ctx->work);
+  int32_t* ids = reinterpret_cast<int32_t*>(w + o_ids);
+  uint8_t* fl = reinterpret_cast<uint8_t*>(w + o_fl);
+  int64_t* cnt = reinterpret_cast<int64_t*>(w + o_cnt);
+  kg_time_begin(ctx);
+  hipLaunchKernelGGL(k_text_flags, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream, *col, P, d_rows_in, n, ids, fl);
+  KG_LAUNCH_CHECK(ctx);
+  size_t cb = cub_sel;
+  KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(w + o_cub, cb, ids, fl, d_rows_out, cnt, static_cast<int>(n),
+                                                  ctx->stream));
+  kg_time_end(ctx);
+  int64_t h = 0;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *out_count = h;
+  return KG_OK;
+}
+
+extern "C" int kg_text_minmax(kg_ctx* ctx, const kg_text_col* col, const int32_t* d_rows, int64_t n, int32_t is_max,
+                              int64_t* out_pos, char* out_buf, int64_t buf_cap, int64_t* out_len) {
+  if (!ctx) return KG_ERR_INVALID;
+  int st = check_text(ctx, col);
+  if (st != KG_OK) return st;
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count out of range");
+  KG_REQUIRE(ctx, out_pos && out_len, "null output pointer");
+  *out_pos = -1;
+  *out_len = 0;
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const int nb = tgrid(ctx, n);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_best = carve(nb * 8), o_out = carve(16);
+  if ((st = kg_work_reserve(ctx, off)) != KG_OK) return st;
+  char* w = static_cast<char*>(ctx->work);
+  int64_t* best = reinterpret_cast<int64_t*>(w + o_best);
+  int64_t* dout = reinterpret_cast<int64_t*>(w + o_out);
+  kg_time_begin(ctx);
+  hipLaunchKernelGGL(k_text_best, dim3(nb), dim3(kTBlock), 0, ctx->stream, *col, d_rows, n, is_max ? 1 : 0, best);
+  hipLaunchKernelGGL(k_text_best_final, dim3(1), dim3(64), 0, ctx->stream, *col, d_rows, best, nb, is_max ? 1 : 0,
+                     dout);
+  kg_time_end(ctx);
+  KG_LAUNCH_CHECK(ctx);
+  int64_t pos = -1;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(&pos, dout, 8, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *out_pos = pos;
+  if (pos < 0) return KG_OK;
+  int32_t row = static_cast<int32_t>(pos);
+  if (d_rows) KG_HIP_CHECK(ctx, hipMemcpy(&row, d_rows + pos, 4, hipMemcpyDeviceToHost));
+  int64_t ab[2];
+  KG_HIP_CHECK(ctx, hipMemcpy(ab, col->offsets + row, 16, hipMemcpyDeviceToHost));
+  *out_len = ab[1] - ab[0];
+  if (out_buf && buf_cap > 0) {
+    const int64_t m = std::min(buf_cap, *out_len);
+    if (m > 0) KG_HIP_CHECK(ctx, hipMemcpy(out_buf, col->bytes + ab[0], m, hipMemcpyDeviceToHost));
+  }
+  return KG_OK;
+}
+
+extern "C" int kg_index_join_expand(kg_ctx* ctx, int64_t n, int32_t width, const int32_t* d_tuples, int32_t src_pos,
+                                    const kg_col* src_key, const int32_t* d_src_rows, const kg_index* idx,
+                                    int64_t out_cap, int32_t* d_out, int64_t* out_n) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, idx && src_key && src_key->data && out_n, "null pointer argument");
+  KG_REQUIRE(ctx, width >= 1 && width <= 15 && src_pos >= 0 && src_pos < width, "bad width / position");
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "tuple count out of range");
+  KG_REQUIRE(ctx, n == 0 || d_tuples, "d_tuples is NULL");
+  KG_REQUIRE(ctx, out_cap == 0 || d_out, "d_out is NULL");
+  if (src_key->type != KG_INT && src_key->type != KG_DATE && src_key->type != KG_ENUM && src_key->type != KG_SMALLINT)
+    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "join keys must be 16/32-bit integers");
+  *out_n = 0;
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  DirView d;
+  const int32_t* tids = nullptr;
+  index_dir(idx, &d.keys, &d.offset, &d.count, &d.nbuckets, &tids);
+  Expand e{width, src_pos, *src_key, d_src_rows, tids};
+  size_t cub_scan = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, cub_scan, static_cast<int64_t*>(nullptr), static_cast<int64_t*>(nullptr),
+                                   static_cast<int>(n + 1), ctx->stream);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_boff = carve(n * 8), o_bcnt = carve((n + 1) * 8), o_start = carve((n + 1) * 8),
+               o_cub = carve(cub_scan);
+  int st = kg_work_reserve(ctx, off);
+  if (st != KG_OK) return st;
+  char* w = static_cast<char*>(ctx->work);
+  int64_t* boff = reinterpret_cast<int64_t*>(w + o_boff);
+  int64_t* bcnt = reinterpret_cast<int64_t*>(w + o_bcnt);
+  int64_t* starts = reinterpret_cast<int64_t*>(w + o_start);
+  KG_HIP_CHECK(ctx, hipMemsetAsync(bcnt + n, 0, 8, ctx->stream));
+  kg_time_begin(ctx);
+  hipLaunchKernelGGL(k_expand_count, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream, e, d, d_tuples, n, boff, bcnt);
+  KG_LAUNCH_CHECK(ctx);
+  size_t cb = cub_scan;
+  KG_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(w + o_cub, cb, bcnt, starts, static_cast<int>(n + 1), ctx->stream));
+  int64_t total = 0;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(&total, starts + n, 8, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *out_n = total;
+  if (total > out_cap) {
+    kg_time_end(ctx);
+    return kg_fail(ctx, KG_ERR_CAPACITY, "%lld joined tuples exceed out_cap %lld", (long long)total,
+                   (long long)out_cap);
+  }
+  if (total > 0)
+    hipLaunchKernelGGL(k_expand_write, dim3(tgrid(ctx, total)), dim3(kTBlock), 0, ctx->stream, e, d_tuples, n, boff,
+                       starts, total, d_out);
+  kg_time_end(ctx);
+  KG_LAUNCH_CHECK(ctx);
+  return KG_OK;
+}
+
+extern "C" int kg_tuple_rows(kg_ctx* ctx, const int32_t* d_tuples, int32_t width, int32_t pos, const int32_t* d_sel,
+                             int64_t n, int32_t* d_out) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, width >= 1 && pos >= 0 && pos < width, "bad width / position");
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "tuple count out of range");
+  KG_REQUIRE(ctx, n == 0 || (d_tuples && d_out), "null pointer argument");
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(k_tuple_rows, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream, d_tuples, width, pos, d_sel, n,
+                     d_out);
+  KG_LAUNCH_CHECK(ctx);
+  return KG_OK;
+}
+
+extern "C" int kg_gather(kg_ctx* ctx, const kg_col* src, const int32_t* d_rows, int64_t n, void* d_out,
+                         int8_t* d_out_nulls) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, src && src->data, "source column is NULL");
+  const int w = type_width(src->type);
+  if (w == 0) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "gather of type %d", src->type);
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count out of range");
+  KG_REQUIRE(ctx, n == 0 || (d_rows && d_out), "null pointer argument");
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(k_gather, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream,
+                     static_cast<const uint8_t*>(src->data), src->nulls, w, d_rows, n, static_cast<uint8_t*>(d_out),
+                     d_out_nulls);
+  KG_LAUNCH_CHECK(ctx);
+  return KG_OK;
+}

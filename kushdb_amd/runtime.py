@@ -365,3 +365,93 @@ class TupleSet:
             self.free()
         except Exception:
             pass
+
+
+class TextColumn:
+    """A resident TEXT column: int64 offsets [n+1] + uint8 heap (+ int8 NULLs)."""
+
+    def __init__(self, offsets, data, nulls=None):
+        if offsets.dtype != torch.int64 or data.dtype != torch.uint8:
+            raise TypeError("TEXT columns need int64 offsets and a uint8 heap")
+        self.offsets = offsets
+        self.data = data if data.numel() else torch.zeros(1, dtype=torch.uint8, device=offsets.device)
+        self.nulls = nulls
+
+    @staticmethod
+    def from_strings(strings, device="cuda:0", nulls=None):
+        import numpy as _np
+        enc = [s.encode() for s in strings]
+        off = _np.zeros(len(enc) + 1, dtype=_np.int64)
+        off[1:] = _np.cumsum([len(b) for b in enc])
+        heap = _np.frombuffer(b"".join(enc), dtype=_np.uint8) if off[-1] else _np.zeros(1, dtype=_np.uint8)
+        return TextColumn(torch.as_tensor(off).to(device), torch.as_tensor(heap.copy()).to(device),
+                          None if nulls is None else torch.as_tensor(nulls).to(device))
+
+    def __len__(self):
+        return self.offsets.numel() - 1
+
+    def desc(self):
+        return abi.KgTextCol(self.offsets.data_ptr(), self.data.data_ptr(),
+                             self.nulls.data_ptr() if self.nulls is not None else 0)
+
+
+def text_select(ctx, col, op, pattern, rows=None):
+    """Row ids (input order) whose value satisfies `col op pattern`."""
+    n = rows.numel() if rows is not None else len(col)
+    out = torch.empty(max(n, 1), dtype=torch.int32, device=col.offsets.device)
+    pb = pattern.encode() if isinstance(pattern, str) else bytes(pattern)
+    cnt = ctypes.c_int64()
+    d = col.desc()
+    ctx.check(ctx.lib.kg_text_select(ctx.h, ctypes.byref(d), op, pb, len(pb), _ptr(rows), n, _ptr(out),
+                                     ctypes.byref(cnt)))
+    return out[: cnt.value]
+
+
+def text_minmax(ctx, col, rows=None, is_max=False, cap=4096):
+    """(position, bytes) of the first MIN (MAX) non-NULL value, or (-1, None)."""
+    n = rows.numel() if rows is not None else len(col)
+    pos = ctypes.c_int64()
+    ln = ctypes.c_int64()
+    buf = ctypes.create_string_buffer(cap)
+    d = col.desc()
+    ctx.check(ctx.lib.kg_text_minmax(ctx.h, ctypes.byref(d), _ptr(rows), n, 1 if is_max else 0, ctypes.byref(pos),
+                                     buf, cap, ctypes.byref(ln)))
+    if pos.value < 0:
+        return -1, None
+    return pos.value, buf.raw[: min(ln.value, cap)]
+
+
+def index_join_expand(ctx, tuples, src_pos, src_key, src_rows, index, out_cap=None):
+    """Extend [n, width] partial tuples by the index bucket of each tuple's key."""
+    n, width = tuples.shape
+    total = ctypes.c_int64()
+    d = src_key.desc()
+    if out_cap is None:
+        # First call sizes the output (KG_ERR_CAPACITY reports the total).
+        st = ctx.lib.kg_index_join_expand(ctx.h, n, width, _ptr(tuples), src_pos, ctypes.byref(d), _ptr(src_rows),
+                                          index.h, 0, ctypes.c_void_p(0), ctypes.byref(total))
+        if st not in (abi.KG_OK, abi.KG_ERR_CAPACITY):
+            ctx.check(st)
+        out_cap = total.value
+    out = torch.empty((max(out_cap, 1), width + 1), dtype=torch.int32, device=tuples.device)
+    ctx.check(ctx.lib.kg_index_join_expand(ctx.h, n, width, _ptr(tuples), src_pos, ctypes.byref(d), _ptr(src_rows),
+                                           index.h, out_cap, _ptr(out), ctypes.byref(total)))
+    return out[: total.value]
+
+
+def tuple_rows(ctx, tuples, pos, sel=None):
+    """Row ids of tuple position `pos` (through the table's selection)."""
+    n, width = tuples.shape
+    out = torch.empty(max(n, 1), dtype=torch.int32, device=tuples.device)
+    ctx.check(ctx.lib.kg_tuple_rows(ctx.h, _ptr(tuples), width, pos, _ptr(sel), n, _ptr(out)))
+    return out[:n]
+
+
+def gather(ctx, col, rows):
+    """Column `col` materialized at `rows` (a new Column)."""
+    n = rows.numel()
+    out = torch.empty(max(n, 1), dtype=col.data.dtype, device=col.data.device)
+    nl = torch.empty(max(n, 1), dtype=torch.int8, device=col.data.device) if col.nulls is not None else None
+    d = col.desc()
+    ctx.check(ctx.lib.kg_gather(ctx.h, ctypes.byref(d), _ptr(rows), n, _ptr(out), _ptr(nl)))
+    return Column(col.type, out[:n], nl[:n] if nl is not None else None)
