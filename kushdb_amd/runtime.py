@@ -304,7 +304,7 @@ class _DeviceArray:
     """__cuda_array_interface__ over library-owned device memory (no copy)."""
 
     def __init__(self, ptr, n, typestr):
-        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, True), "version": 2}
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False), "version": 2}
 
 
 def _device_view_i32(ptr, n, device):

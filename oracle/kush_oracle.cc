@@ -33,6 +33,12 @@
 //   kor_q_*            - compositions of the above, checked against numpy.
 //   kor_dtree_scan_agg - the GPU deterministic order; checked against math.fsum
 //                        (|err| bound) and against kor_q_scan_agg (1e-12 rel).
+//   kor_text_pred      - StartsWith/EndsWith/Contains/compare against Python's
+//                        str methods; LIKE against Python's re.fullmatch of the
+//                        planner's escaped regex (plan/planner.cc:284-323) on
+//                        ASCII data. (re2 itself is absent: general LIKE is
+//                        pinned through that equivalent regex, not re2.)
+//   kor_text_minmax    - min()/max() over Python bytes.
 #include "kush_oracle.h"
 
 #include <algorithm>
@@ -723,6 +729,99 @@ int64_t kor_kdbindex_get(const uint8_t* file, int32_t key_type, int64_t key_bits
     pos = next;
   }
   return 0;
+}
+
+// String predicates of runtime/string.cc:14-82 over byte strings (string_view
+// semantics: unsigned byte comparison, a proper prefix orders first), and the
+// planner's LIKE (plan/planner.cc:284-323): an anchored regex in which '%' is
+// ".*", '_' is "." and every other byte is literal (regex metacharacters are
+// escaped). The regex is evaluated here by dynamic programming over
+// (string position, pattern position).
+enum kor_text_op {
+  KOR_TEXT_EQ = 0, KOR_TEXT_NEQ, KOR_TEXT_LT, KOR_TEXT_LEQ, KOR_TEXT_GT, KOR_TEXT_GEQ,
+  KOR_TEXT_STARTS_WITH, KOR_TEXT_ENDS_WITH, KOR_TEXT_CONTAINS, KOR_TEXT_LIKE,
+};
+
+static int kor_text_cmp(const uint8_t* a, int64_t na, const uint8_t* b, int64_t nb) {
+  const int64_t m = na < nb ? na : nb;
+  const int c = m ? std::memcmp(a, b, m) : 0;
+  if (c != 0) return c < 0 ? -1 : 1;
+  return na < nb ? -1 : (na > nb ? 1 : 0);
+}
+
+static bool kor_like(const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
+  // ok[j]: pattern prefix p[0..j) matches string prefix s[0..i).
+  std::vector<char> ok(m + 1, 0), nx(m + 1, 0);
+  ok[0] = 1;
+  for (int32_t j = 1; j <= m; j++) ok[j] = ok[j - 1] && p[j - 1] == '%';
+  for (int64_t i = 1; i <= n; i++) {
+    nx[0] = 0;
+    for (int32_t j = 1; j <= m; j++) {
+      const uint8_t c = p[j - 1];
+      if (c == '%') nx[j] = nx[j - 1] || ok[j];
+      else nx[j] = ok[j - 1] && (c == '_' || c == s[i - 1]);
+    }
+    ok.swap(nx);
+  }
+  return ok[m] != 0;
+}
+
+int32_t kor_text_pred(int32_t op, const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
+  switch (op) {
+    case KOR_TEXT_STARTS_WITH: return n >= m && (m == 0 || std::memcmp(s, p, m) == 0);
+    case KOR_TEXT_ENDS_WITH: return n >= m && (m == 0 || std::memcmp(s + n - m, p, m) == 0);
+    case KOR_TEXT_CONTAINS: {
+      if (m == 0) return 1;
+      for (int64_t i = 0; i + m <= n; i++)
+        if (std::memcmp(s + i, p, m) == 0) return 1;
+      return 0;
+    }
+    case KOR_TEXT_LIKE: return kor_like(s, n, p, m);
+    default: {
+      const int c = kor_text_cmp(s, n, p, m);
+      switch (op) {
+        case KOR_TEXT_EQ: return c == 0;
+        case KOR_TEXT_NEQ: return c != 0;
+        case KOR_TEXT_LT: return c < 0;
+        case KOR_TEXT_LEQ: return c <= 0;
+        case KOR_TEXT_GT: return c > 0;
+        default: return c >= 0;
+      }
+    }
+  }
+}
+
+// ScanSelect with one TEXT predicate over rows (or 0..n-1); NULL rejects.
+int64_t kor_text_select(int64_t n, const int64_t* offsets, const uint8_t* bytes, const int8_t* nulls, int32_t op,
+                        const uint8_t* p, int32_t m, const int32_t* rows, int32_t* out) {
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t r = rows ? rows[i] : i;
+    if (nulls && nulls[r]) continue;
+    if (kor_text_pred(op, bytes + offsets[r], offsets[r + 1] - offsets[r], p, m)) out[k++] = static_cast<int32_t>(r);
+  }
+  return k;
+}
+
+// MinMaxAggregator over TEXT (compile/proxy/aggregator.cc:102-171): the first
+// non-NULL value initializes, later values replace it only when strictly
+// smaller (larger). Returns the winning input position, or -1.
+int64_t kor_text_minmax(int64_t n, const int64_t* offsets, const uint8_t* bytes, const int8_t* nulls,
+                        const int32_t* rows, int32_t is_max) {
+  int64_t best = -1;
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t r = rows ? rows[i] : i;
+    if (nulls && nulls[r]) continue;
+    if (best < 0) {
+      best = i;
+      continue;
+    }
+    const int64_t b = rows ? rows[best] : best;
+    const int c = kor_text_cmp(bytes + offsets[r], offsets[r + 1] - offsets[r], bytes + offsets[b],
+                               offsets[b + 1] - offsets[b]);
+    if (is_max ? c > 0 : c < 0) best = i;
+  }
+  return best;
 }
 
 // ---------------------------------------------------------------------------

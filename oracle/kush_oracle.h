@@ -108,6 +108,13 @@ int64_t kor_memory_index_build(int64_t n, const int32_t* keys, const int32_t* ro
 int64_t kor_kdbindex_get(const uint8_t* file, int32_t key_type, int64_t key_bits,
                          int32_t* out, int64_t out_cap);
 
+// ---- TEXT ------------------------------------------------------------------
+int32_t kor_text_pred(int32_t op, const uint8_t* s, int64_t n, const uint8_t* p, int32_t m);
+int64_t kor_text_select(int64_t n, const int64_t* offsets, const uint8_t* bytes, const int8_t* nulls, int32_t op,
+                        const uint8_t* p, int32_t m, const int32_t* rows, int32_t* out);
+int64_t kor_text_minmax(int64_t n, const int64_t* offsets, const uint8_t* bytes, const int8_t* nulls,
+                        const int32_t* rows, int32_t is_max);
+
 // ---- workload compositions (CPU baseline legs) ------------------------------
 int64_t kor_q_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
                        const int32_t* g2, int32_t nf, const double* const* fm, int32_t ni,
