@@ -346,8 +346,8 @@ int read_u64(kg_ctx* ctx, const void* d, void* h, size_t bytes) {
   return KG_OK;
 }
 
-int check_index_key(kg_ctx* ctx, const kg_col* key) {
-  KG_REQUIRE(ctx, key && key->data, "key column is NULL");
+int check_index_key(kg_ctx* ctx, const kg_col* key, int64_t n) {
+  KG_REQUIRE(ctx, key && (n == 0 || key->data), "key column is NULL");
   switch (key->type) {
     case KG_SMALLINT:
     case KG_INT:
@@ -380,7 +380,7 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
   if (!ctx) return KG_ERR_INVALID;
   KG_REQUIRE(ctx, out, "out is NULL");
   KG_REQUIRE(ctx, nrows >= 0 && nrows < INT32_MAX, "row count out of range");
-  int st = check_index_key(ctx, key);
+  int st = check_index_key(ctx, key, nrows);
   if (st != KG_OK) return st;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   kg_index* idx = new (std::nothrow) kg_index();
@@ -518,7 +518,7 @@ extern "C" int kg_index_lookup(kg_ctx* ctx, const kg_index* idx, const kg_col* k
   KG_REQUIRE(ctx, idx, "index is NULL");
   KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count out of range");
   KG_REQUIRE(ctx, n == 0 || (d_out_ptr && d_out_count), "output arrays are NULL");
-  int st = check_index_key(ctx, key);
+  int st = check_index_key(ctx, key, n);
   if (st != KG_OK) return st;
   if (n == 0) return KG_OK;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));

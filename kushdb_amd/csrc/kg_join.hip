@@ -487,8 +487,9 @@ int alloc(kg_ctx* ctx, void** p, size_t bytes) {
   return KG_OK;
 }
 
-int check_key(kg_ctx* ctx, const kg_col* key) {
-  KG_REQUIRE(ctx, key && key->data, "key column is NULL");
+// A column's data may be NULL only when it has no rows (an empty device tensor).
+int check_key(kg_ctx* ctx, const kg_col* key, int64_t n) {
+  KG_REQUIRE(ctx, key && (n == 0 || key->data), "key column is NULL");
   switch (key->type) {
     case KG_SMALLINT:
     case KG_INT:
@@ -551,7 +552,7 @@ extern "C" int kg_ht_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, co
   if (!ctx) return KG_ERR_INVALID;
   KG_REQUIRE(ctx, ht, "table is NULL");
   KG_REQUIRE(ctx, nrows >= 0 && nrows < INT32_MAX, "row count out of range");
-  int st = check_key(ctx, key);
+  int st = check_key(ctx, key, nrows);
   if (st != KG_OK) return st;
   if (nrows == 0) return KG_OK;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -576,7 +577,7 @@ int probe_common(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, int64_
   if (!ctx) return KG_ERR_INVALID;
   KG_REQUIRE(ctx, ht, "table is NULL");
   KG_REQUIRE(ctx, nrows >= 0 && nrows < INT32_MAX, "row count out of range");
-  int st = check_key(ctx, key);
+  int st = check_key(ctx, key, nrows);
   if (st != KG_OK) return st;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   return KG_OK;
@@ -795,14 +796,14 @@ extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols
     return kg_fail(ctx, KG_ERR_UNSUPPORTED, "build group codes must be INT/ENUM codes");
   if (build_codes->nulls) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "nullable build group codes");
   KG_REQUIRE(ctx, group_card > 0, "group cardinality must be positive");
-  int st = check_key(ctx, &cols[probe_key_col]);
+  int st = check_key(ctx, &cols[probe_key_col], n);
   if (st != KG_OK) return st;
   PredSet P;
-  if ((st = build_preds(ctx, cols, ncols, preds, npreds, 1, P)) != KG_OK) return st;
+  if ((st = build_preds(ctx, cols, ncols, preds, npreds, 1, n, P)) != KG_OK) return st;
   MeasureSet M;
   std::vector<AggPlan> plan;
   bool minmax = false;
-  if ((st = build_measures(ctx, cols, ncols, aggs, naggs, M, plan, minmax)) != KG_OK) return st;
+  if ((st = build_measures(ctx, cols, ncols, aggs, naggs, n, M, plan, minmax)) != KG_OK) return st;
   const int G = group_card;
   const int nm = M.nm;
   const size_t lds_bytes = (static_cast<size_t>(G) * nm * 2 + G * 2 + (minmax ? 2 * G * nm : 0)) * 8;

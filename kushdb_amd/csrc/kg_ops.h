@@ -130,12 +130,13 @@ struct AggPlan {
 };
 
 // Validates predicates against cols and fills a PredSet.
+// (Column data may be NULL only when n == 0.)
 int build_preds(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_pred* preds, int32_t npreds,
-                int32_t conj, PredSet& p);
+                int32_t conj, int64_t n, PredSet& p);
 
 // Validates aggregates, dedups their inputs into measures, and returns the
 // plan; minmax tells whether any MIN/MAX is requested.
-int build_measures(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_agg* aggs, int32_t naggs,
+int build_measures(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_agg* aggs, int32_t naggs, int64_t n,
                    MeasureSet& M, std::vector<AggPlan>& plan, bool& minmax);
 
 // Emits the groups with rows[g] > 0 in ascending first[g] order (insertion

@@ -406,7 +406,7 @@ using namespace kg;
 namespace kg {
 
 int build_preds(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_pred* preds, int32_t npreds,
-                int32_t conj, PredSet& p) {
+                int32_t conj, int64_t n, PredSet& p) {
   KG_REQUIRE(ctx, npreds >= 0 && npreds <= kMaxPreds, "at most %d predicates (got %d)", kMaxPreds, npreds);
   KG_REQUIRE(ctx, npreds == 0 || preds, "preds is NULL");
   std::memset(&p, 0, sizeof(p));
@@ -416,7 +416,7 @@ int build_preds(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_pred* p
     const kg_pred& q = preds[i];
     KG_REQUIRE(ctx, q.col >= 0 && q.col < ncols, "predicate %d references column %d of %d", i, q.col, ncols);
     const kg_col& c = cols[q.col];
-    KG_REQUIRE(ctx, c.data, "predicate column %d has no data", q.col);
+    KG_REQUIRE(ctx, n == 0 || c.data, "predicate column %d has no data", q.col);
     if (c.type == KG_TEXT) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "TEXT predicates are not supported here");
     KG_REQUIRE(ctx, type_width(c.type) > 0, "predicate column %d has bad type %d", q.col, c.type);
     KG_REQUIRE(ctx, q.op >= KG_EQ && q.op <= KG_GEQ, "bad comparison op %d", q.op);
@@ -440,7 +440,7 @@ extern "C" int kg_scan_select(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_
   KG_REQUIRE(ctx, out_count, "out_count is NULL");
   KG_REQUIRE(ctx, n == 0 || d_out_rows, "d_out_rows is NULL");
   PredSet p;
-  int st = build_preds(ctx, cols, ncols, preds, npreds, conjunction, p);
+  int st = build_preds(ctx, cols, ncols, preds, npreds, conjunction, n, p);
   if (st != KG_OK) return st;
   *out_count = 0;
   if (n == 0) return KG_OK;
@@ -520,7 +520,7 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   KG_REQUIRE(ctx, naggs >= 0 && naggs <= 16, "at most 16 aggregates (got %d)", naggs);
   KG_REQUIRE(ctx, nkeys == 0 || (key_cols && key_card && out_keys), "key arrays are NULL");
   PredSet P;
-  int st = build_preds(ctx, cols, ncols, preds, npreds, 1, P);
+  int st = build_preds(ctx, cols, ncols, preds, npreds, 1, n, P);
   if (st != KG_OK) return st;
 
   KeySet K;
@@ -545,7 +545,7 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   MeasureSet M;
   std::vector<AggPlan> plan;
   bool minmax = false;
-  if ((st = build_measures(ctx, cols, ncols, aggs, naggs, M, plan, minmax)) != KG_OK) return st;
+  if ((st = build_measures(ctx, cols, ncols, aggs, naggs, n, M, plan, minmax)) != KG_OK) return st;
 
   *out_ngroups = 0;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -661,7 +661,7 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
 
 namespace kg {
 
-int build_measures(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_agg* aggs, int32_t naggs,
+int build_measures(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_agg* aggs, int32_t naggs, int64_t n,
                    MeasureSet& M, std::vector<AggPlan>& plan, bool& minmax) {
   std::memset(&M, 0, sizeof(M));
   plan.assign(naggs, AggPlan{});
@@ -669,7 +669,7 @@ int build_measures(kg_ctx* ctx, const kg_col* cols, int32_t ncols, const kg_agg*
   auto check_col = [&](int32_t ci, const char* what) -> int {
     KG_REQUIRE(ctx, ci >= 0 && ci < ncols, "%s references column %d of %d", what, ci, ncols);
     const kg_col& c = cols[ci];
-    KG_REQUIRE(ctx, c.data, "%s column %d has no data", what, ci);
+    KG_REQUIRE(ctx, n == 0 || c.data, "%s column %d has no data", what, ci);
     if (c.type == KG_TEXT || c.type == KG_BOOLEAN)
       return kg_fail(ctx, KG_ERR_UNSUPPORTED, "aggregate over type %d is not supported here", c.type);
     KG_REQUIRE(ctx, type_width(c.type) > 0, "column %d has bad type %d", ci, c.type);

@@ -378,7 +378,7 @@ extern "C" int kg_index_join_expand(kg_ctx* ctx, int64_t n, int32_t width, const
                                     const kg_col* src_key, const int32_t* d_src_rows, const kg_index* idx,
                                     int64_t out_cap, int32_t* d_out, int64_t* out_n) {
   if (!ctx) return KG_ERR_INVALID;
-  KG_REQUIRE(ctx, idx && src_key && src_key->data && out_n, "null pointer argument");
+  KG_REQUIRE(ctx, idx && src_key && (n == 0 || src_key->data) && out_n, "null pointer argument");
   KG_REQUIRE(ctx, width >= 1 && width <= 15 && src_pos >= 0 && src_pos < width, "bad width / position");
   KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "tuple count out of range");
   KG_REQUIRE(ctx, n == 0 || d_tuples, "d_tuples is NULL");

@@ -96,3 +96,80 @@ def join_tables(nbuild=1 << 24, nprobe=1 << 28, theta=0.0, miss=0.1, device="cud
     del missmask
     pvals = torch.randint(0, 1 << 32, (nprobe,), generator=g, device=device, dtype=torch.int64)
     return bkeys, payload, pkeys, pvals
+
+
+SKINNER_SIZES = (200_000, 1_000_000, 4_000_000, 25_000_000, 50_000_000)
+
+
+def vocabulary(nwords=50_000, seed=SEED + 3):
+    """nwords distinct lowercase ASCII words of 3..10 letters."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    words = set()
+    while len(words) < nwords:
+        ln = int(rng.integers(3, 11))
+        words.add("".join(chr(97 + int(c)) for c in rng.integers(0, 26, ln)))
+    return sorted(words)
+
+
+def text_column(n, vocab, rng):
+    """n strings of 8..40 chars: vocabulary words joined by spaces. Returns
+    (offsets int64 [n+1], heap uint8, python list)."""
+    import numpy as np
+    out = []
+    widx = rng.integers(0, len(vocab), n * 8)
+    target = rng.integers(8, 41, n)
+    k = 0
+    for i in range(n):
+        s = vocab[widx[k]]
+        k += 1
+        while len(s) < target[i]:
+            s = s + " " + vocab[widx[k % len(widx)]]
+            k += 1
+        out.append(s[: target[i]] if len(s) > 40 else s)
+    enc = [x.encode() for x in out]
+    off = np.zeros(n + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(b) for b in enc])
+    heap = np.frombuffer(b"".join(enc), dtype=np.uint8).copy()
+    return off, heap, out
+
+
+def skinner_tables(scale=1.0, theta=0.0, device="cuda", seed=SEED + 4):
+    """Five chained tables t0..t4 of SKINNER_SIZES * scale rows. Every table has
+    a dense int32 `id` and an int32 `v` U{0..999}; t_i (i >= 1) has `fk`
+    referencing t_{i-1}.id (uniform or Zipf(theta)); t0 has a TEXT `name`, t1 a
+    TEXT `s`. Filters (1-10%): t0.name STARTS_WITH "a", t1.s LIKE "%e_a%"
+    (the planner's general LIKE form), t4.v < 50.
+
+    Returns (device tables, filter spec, host copies)."""
+    import numpy as np
+    from . import abi
+    from .runtime import Column, TextColumn
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    sizes = [max(4, int(n * scale)) for n in SKINNER_SIZES]
+    vocab = vocabulary(seed=seed)
+    tabs, htabs = [], []
+    for i, n in enumerate(sizes):
+        t = {"id": torch.arange(n, dtype=torch.int32, device=device),
+             "v": torch.randint(0, 1000, (n,), generator=g, device=device, dtype=torch.int32)}
+        if i > 0:
+            t["fk"] = zipf_ranks(n, sizes[i - 1], theta, g, device).to(torch.int32)
+            if theta > 0:  # spread the hot keys over the referenced table
+                perm = torch.randperm(sizes[i - 1], generator=g, device=device).to(torch.int32)
+                t["fk"] = perm[t["fk"].long()]
+        ht = {k: v.cpu().numpy() for k, v in t.items()}
+        dt = {k: Column(abi.KG_INT, v) for k, v in t.items()}
+        if i in (0, 1):
+            name = "name" if i == 0 else "s"
+            off, heap, lst = text_column(n, vocab, rng)
+            dt[name] = TextColumn(torch.as_tensor(off).to(device), torch.as_tensor(heap).to(device))
+            ht[name + "_off"], ht[name + "_heap"], ht[name + "_list"] = off, heap, lst
+        tabs.append(dt)
+        htabs.append(ht)
+    spec = [[("text", "name", abi.KG_TEXT_STARTS_WITH, "a")],
+            [("text", "s", abi.KG_TEXT_LIKE, "%e_a%")],
+            [], [],
+            [("int", "v", abi.KG_LT, 50)]]
+    return tabs, spec, htabs

@@ -100,6 +100,15 @@ def lib():
                                  ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p]
+        L.kor_text_pred.restype = ctypes.c_int32
+        L.kor_text_pred.argtypes = [ctypes.c_int32, ctypes.c_char_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int32]
+        L.kor_text_select.restype = ctypes.c_int64
+        L.kor_text_select.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, ctypes.c_void_p,
+                                      ctypes.c_void_p]
+        L.kor_text_minmax.restype = ctypes.c_int64
+        L.kor_text_minmax.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_int32]
         L.kor_dtree_scan_agg.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]
@@ -263,3 +272,29 @@ def kdbindex_get(file_bytes, key_type, key):
     out = np.empty(cap, dtype=np.int32)
     k = lib().kor_kdbindex_get(buf.ctypes.data, key_type, bits, out.ctypes.data, cap)
     return out[:k].copy()
+
+
+class TextCol:
+    """numpy TEXT column: int64 offsets [n+1] + uint8 heap (+ int8 nulls)."""
+
+    def __init__(self, offsets, heap, nulls=None):
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.heap = np.ascontiguousarray(heap if len(heap) else np.zeros(1, np.uint8), dtype=np.uint8)
+        self.nulls = None if nulls is None else np.ascontiguousarray(nulls, dtype=np.int8)
+
+
+def text_select(col, op, pattern, rows=None):
+    n = len(rows) if rows is not None else len(col.offsets) - 1
+    rr = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    out = np.empty(max(n, 1), dtype=np.int32)
+    pb = pattern.encode() if isinstance(pattern, str) else bytes(pattern)
+    k = lib().kor_text_select(n, col.offsets.ctypes.data, col.heap.ctypes.data, _p(col.nulls), op, pb, len(pb),
+                              _p(rr), out.ctypes.data)
+    return out[:k].copy()
+
+
+def text_minmax(col, rows=None, is_max=False):
+    n = len(rows) if rows is not None else len(col.offsets) - 1
+    rr = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    return lib().kor_text_minmax(n, col.offsets.ctypes.data, col.heap.ctypes.data, _p(col.nulls), _p(rr),
+                                 1 if is_max else 0)
