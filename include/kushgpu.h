@@ -127,6 +127,14 @@ int kg_ctx_synchronize(kg_ctx* ctx);
 /* 1 (default): FP64 reductions use a fixed order that depends only on n (and
  * is reproduced by the CPU oracle); 0: order may depend on timing. */
 int kg_ctx_set_deterministic(kg_ctx* ctx, int on);
+/* The FP64 addition order of the last kg_scan_group_agg call that used the
+ * fixed-order kernels: T = *nblocks * 256 lanes; lane t folds, from +0.0, the
+ * qualifying rows of chunks c = t + k*T (k ascending) of *vec consecutive rows
+ * (ascending); lanes combine by the tree p[l] += p[l+s], s = 32..1, within a
+ * 64-lane wave, then the 4 waves by the same tree; workgroup partials are
+ * added in workgroup order. No counterpart in the reference (whose order is
+ * the arrival order); it lets a checker restate the GPU order exactly. */
+int kg_ctx_agg_order(const kg_ctx* ctx, int64_t n, int32_t* nblocks, int32_t* vec);
 /* Kernel timing (no counterpart in the reference, whose timing is wall clock
  * around whole queries, util/time_execute.h:22-57): when on, every entry point
  * brackets its dominant kernel with hipEvents on the context's stream.

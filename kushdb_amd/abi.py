@@ -58,6 +58,7 @@ SIGNATURES = {
     "kg_ctx_synchronize": (_i32, [_vp]),
     "kg_ctx_set_deterministic": (_i32, [_vp, _i32]),
     "kg_ctx_set_timing": (_i32, [_vp, _i32]),
+    "kg_ctx_agg_order": (_i32, [_vp, _i64, _pi32, _pi32]),
     "kg_ctx_kernel_stats": (_i32, [_vp, ctypes.POINTER(ctypes.c_double), _pi64, _i32]),
     "kg_scan_select": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32, _i32,
                               _vp, _pi64]),

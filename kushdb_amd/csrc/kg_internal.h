@@ -15,6 +15,9 @@ struct kg_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int deterministic = 1;
+  // Rows per lane step of the last fixed-order aggregation (1 or 4): with
+  // kg_agg_blocks_for(n) it defines that aggregation's FP64 addition order.
+  int last_agg_vec = 1;
   int num_cus = 256;
   std::string err;
   // Grow-only device workspace (block partials, counters) and a small pinned

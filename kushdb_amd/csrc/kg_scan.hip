@@ -266,30 +266,215 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_reg(PredSet P, KeySet K, M
 // Adds the per-workgroup partials in workgroup order. One thread per
 // (group, measure); thread G*nm + g handles the row count of group g.
 
-__global__ void k_group_agg_final(const AggOut in, int nblocks, int G, MeasureSet M, bool minmax,
-                                  AggFinal* fin, uint64_t* rows_out) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+// Adds the per-workgroup partials in workgroup order. One 64-lane workgroup
+// per output: (group, measure) outputs first, then the row count of each
+// group. The lanes stage all partials of that output in LDS in parallel, then
+// lane 0 folds them in workgroup order (the order that defines the FP64
+// result); the integer row counts are summed in any order.
+__global__ __launch_bounds__(64) void k_group_agg_final(const AggOut in, int nblocks, int G, MeasureSet M,
+                                                        bool minmax, AggFinal* fin, uint64_t* rows_out) {
+  extern __shared__ uint64_t stage[];
+  const int idx = blockIdx.x;
   const int nm = M.nm;
+  const int lane = threadIdx.x;
   if (idx < G * nm) {
     const int g = idx / nm, m = idx % nm;
-    uint64_t s = 0, lo = 0, hi = 0, c = 0;
-    for (int b = 0; b < nblocks; b++) {
+    uint64_t* ssum = stage;
+    uint64_t* snn = stage + nblocks;
+    uint64_t* smn = stage + 2 * nblocks;
+    uint64_t* smx = stage + 3 * nblocks;
+    for (int b = lane; b < nblocks; b += 64) {
       const int64_t o = (static_cast<int64_t>(b) * G + g) * nm + m;
-      const uint32_t bc = in.nn[o];
-      if (bc == 0) continue;
-      s = M.is_real[m] ? fadd_bits(s, in.sum[o]) : s + in.sum[o];
+      ssum[b] = in.sum[o];
+      snn[b] = in.nn[o];
       if (minmax) {
-        if (c == 0 || val_less(M.is_real[m], in.mn[o], lo)) lo = in.mn[o];
-        if (c == 0 || val_less(M.is_real[m], hi, in.mx[o])) hi = in.mx[o];
+        smn[b] = in.mn[o];
+        smx[b] = in.mx[o];
       }
-      c += bc;
     }
-    fin[idx] = AggFinal{s, lo, hi, c};
+    __syncthreads();
+    if (lane == 0) {
+      const bool real = M.is_real[m];
+      uint64_t s = 0, lo = 0, hi = 0, c = 0;
+      for (int b = 0; b < nblocks; b++) {
+        const uint64_t bc = snn[b];
+        if (bc == 0) continue;
+        s = real ? fadd_bits(s, ssum[b]) : s + ssum[b];
+        if (minmax) {
+          if (c == 0 || val_less(real, smn[b], lo)) lo = smn[b];
+          if (c == 0 || val_less(real, hi, smx[b])) hi = smx[b];
+        }
+        c += bc;
+      }
+      fin[idx] = AggFinal{s, lo, hi, c};
+    }
   } else if (idx < G * nm + G) {
     const int g = idx - G * nm;
     uint64_t rc = 0;
-    for (int b = 0; b < nblocks; b++) rc += in.rows[static_cast<int64_t>(b) * G + g];
-    rows_out[g] = rc;
+    for (int b = lane; b < nblocks; b += 64) rc += in.rows[static_cast<int64_t>(b) * G + g];
+    for (int st = 32; st >= 1; st >>= 1) rc += __shfl_xor(rc, st);
+    if (lane == 0) rows_out[g] = rc;
+  }
+}
+
+// Fast path of the dense group-by: every predicate and key column is a
+// non-NULL 32-bit integer column, every measure a non-NULL REAL or INT
+// (32-bit) column, all 16-byte aligned; conjunctive predicates; no MIN/MAX.
+// Each lane takes kVec consecutive rows per step with 16-byte loads; rows of a
+// chunk are folded in ascending order, chunks c = t + k*T in ascending k.
+constexpr int kVec = 4;
+constexpr int kFastPreds = 2;
+
+struct FastSet {
+  int32_t npred;
+  int32_t op[kFastPreds];
+  int64_t lit[kFastPreds];
+  const int32_t* pcol[kFastPreds];
+  int32_t nk;
+  const int32_t* key[2];
+  int32_t card1;
+  int32_t nm;
+  int32_t real[kMaxMeasures];
+  const void* mp[kMaxMeasures];
+};
+
+__device__ __forceinline__ void ld_i32x4(const int32_t* p, int64_t r0, int64_t n, bool full, int32_t (&o)[kVec]) {
+  if (full) {
+    const int4 x = *reinterpret_cast<const int4*>(p + r0);
+    o[0] = x.x;
+    o[1] = x.y;
+    o[2] = x.z;
+    o[3] = x.w;
+  } else {
+#pragma unroll
+    for (int v = 0; v < kVec; v++) o[v] = r0 + v < n ? p[r0 + v] : 0;
+  }
+}
+
+template <int G, int NM>
+__global__ __launch_bounds__(kBlock) void k_group_agg_v4(FastSet S, int64_t n, int Grt, AggOut out) {
+  uint64_t acc[G][NM];
+  uint32_t rows[G];
+  int32_t first[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    rows[g] = 0;
+    first[g] = INT32_MAX;
+#pragma unroll
+    for (int m = 0; m < NM; m++) acc[g][m] = 0;
+  }
+  const int64_t T = static_cast<int64_t>(gridDim.x) * kBlock;
+  const int64_t nchunks = (n + kVec - 1) / kVec;
+  for (int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; c < nchunks; c += T) {
+    const int64_t r0 = c * kVec;
+    const bool full = r0 + kVec <= n;
+    int32_t pv[kFastPreds][kVec], kv[2][kVec];
+    uint64_t mv[NM][kVec];
+#pragma unroll
+    for (int i = 0; i < kFastPreds; i++)
+      if (i < S.npred) ld_i32x4(S.pcol[i], r0, n, full, pv[i]);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+      if (i < S.nk) ld_i32x4(S.key[i], r0, n, full, kv[i]);
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      if (m >= S.nm) continue;
+      if (S.real[m]) {
+        const double* p = static_cast<const double*>(S.mp[m]);
+        if (full) {
+          const longlong2 a = *reinterpret_cast<const longlong2*>(p + r0);
+          const longlong2 b = *reinterpret_cast<const longlong2*>(p + r0 + 2);
+          mv[m][0] = a.x;
+          mv[m][1] = a.y;
+          mv[m][2] = b.x;
+          mv[m][3] = b.y;
+        } else {
+#pragma unroll
+          for (int v = 0; v < kVec; v++)
+            mv[m][v] = r0 + v < n ? static_cast<uint64_t>(__double_as_longlong(p[r0 + v])) : 0;
+        }
+      } else {
+        int32_t t[kVec];
+        ld_i32x4(static_cast<const int32_t*>(S.mp[m]), r0, n, full, t);
+#pragma unroll
+        for (int v = 0; v < kVec; v++) mv[m][v] = static_cast<uint64_t>(static_cast<int64_t>(t[v]));
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kVec; v++) {
+      bool pass = r0 + v < n;
+#pragma unroll
+      for (int i = 0; i < kFastPreds; i++)
+        if (i < S.npred) pass = pass && cmp_i64(S.op[i], pv[i][v], S.lit[i]);
+      const int grp = S.nk == 0 ? 0 : (S.nk == 1 ? kv[0][v] : kv[0][v] * S.card1 + kv[1][v]);
+      pass = pass && grp >= 0 && grp < Grt;
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        if (!(pass && grp == g)) continue;
+        rows[g]++;
+        first[g] = min(first[g], static_cast<int32_t>(r0 + v));
+#pragma unroll
+        for (int m = 0; m < NM; m++) {
+          if (m >= S.nm) continue;
+          acc[g][m] = S.real[m] ? fadd_bits(acc[g][m], mv[m][v]) : acc[g][m] + mv[m][v];
+        }
+      }
+    }
+  }
+
+  // Same fixed-order workgroup reduction as k_group_agg_reg.
+  __shared__ uint64_t wred[kBlock / kWave][G * NM];
+  __shared__ uint32_t wcnt[kBlock / kWave][G];
+  __shared__ int32_t wfirst[kBlock / kWave][G];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+#pragma unroll
+    for (int m = 0; m < NM; m++) {
+      if (m >= S.nm) continue;
+      uint64_t sv = acc[g][m];
+      for (int st = kWave / 2; st >= 1; st >>= 1) {
+        const uint64_t os = shfl_xor_u64(sv, st);
+        sv = S.real[m] ? ((lane & st) ? fadd_bits(os, sv) : fadd_bits(sv, os)) : sv + os;
+      }
+      if (lane == 0) wred[wave][g * NM + m] = sv;
+    }
+    uint32_t rc = rows[g];
+    int32_t fr = first[g];
+    for (int st = kWave / 2; st >= 1; st >>= 1) {
+      rc += __shfl_xor(rc, st);
+      fr = min(fr, __shfl_xor(fr, st));
+    }
+    if (lane == 0) {
+      wcnt[wave][g] = rc;
+      wfirst[wave][g] = fr;
+    }
+  }
+  __syncthreads();
+  constexpr int NW = kBlock / kWave;
+  for (int idx = threadIdx.x; idx < G * NM; idx += kBlock) {
+    const int g = idx / NM, m = idx % NM;
+    if (m >= S.nm || g >= Grt) continue;
+    uint64_t p[NW];
+    for (int w = 0; w < NW; w++) p[w] = wred[w][idx];
+    for (int st = NW / 2; st >= 1; st >>= 1)
+      for (int w = 0; w < st; w++) p[w] = S.real[m] ? fadd_bits(p[w], p[w + st]) : p[w] + p[w + st];
+    uint32_t cnt = 0;
+    for (int w = 0; w < NW; w++) cnt += wcnt[w][g];
+    const int64_t o = (static_cast<int64_t>(blockIdx.x) * Grt + g) * S.nm + m;
+    out.sum[o] = p[0];
+    out.nn[o] = cnt;
+  }
+  for (int g = threadIdx.x; g < Grt; g += kBlock) {
+    uint32_t rc = 0;
+    int32_t fr = INT32_MAX;
+    for (int w = 0; w < NW; w++) {
+      rc += wcnt[w][g];
+      fr = min(fr, wfirst[w][g]);
+    }
+    out.rows[static_cast<int64_t>(blockIdx.x) * Grt + g] = rc;
+    if (fr != INT32_MAX) atomicMin(&out.first[g], fr);
   }
 }
 
@@ -497,15 +682,73 @@ bool launch_reg_any(int G, bool mm, hipStream_t s, int nb, const PredSet& P, con
   return true;
 }
 
+template <int G>
+void launch_v4_g(hipStream_t s, int nb, const FastSet& S, int64_t n, int Grt, const AggOut& o) {
+  if (S.nm <= 2)
+    hipLaunchKernelGGL((k_group_agg_v4<G, 2>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
+  else if (S.nm <= 4)
+    hipLaunchKernelGGL((k_group_agg_v4<G, 4>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
+  else
+    hipLaunchKernelGGL((k_group_agg_v4<G, 8>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
+}
+
+bool launch_v4(int G, hipStream_t s, int nb, const FastSet& S, int64_t n, const AggOut& o) {
+  if (G <= 1) launch_v4_g<1>(s, nb, S, n, G, o);
+  else if (G <= 2) launch_v4_g<2>(s, nb, S, n, G, o);
+  else if (G <= 4) launch_v4_g<4>(s, nb, S, n, G, o);
+  else if (G <= 6) launch_v4_g<6>(s, nb, S, n, G, o);
+  else if (G <= 8) launch_v4_g<8>(s, nb, S, n, G, o);
+  else return false;
+  return true;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+// The fast path applies to this predicate / key / measure set.
+bool fast_set(const PredSet& P, const KeySet& K, const MeasureSet& M, bool minmax, FastSet& S) {
+  if (minmax || !P.conj || P.n > kFastPreds) return false;
+  std::memset(&S, 0, sizeof(S));
+  auto is_i32 = [](int32_t t) { return t == KG_INT || t == KG_DATE || t == KG_ENUM; };
+  S.npred = P.n;
+  for (int i = 0; i < P.n; i++) {
+    if (!is_i32(P.type[i]) || P.nulls[i] || !aligned16(P.data[i])) return false;
+    S.op[i] = P.op[i];
+    S.lit[i] = P.ilit[i];
+    S.pcol[i] = static_cast<const int32_t*>(P.data[i]);
+  }
+  S.nk = K.nk;
+  for (int k = 0; k < K.nk; k++) {
+    if (!aligned16(K.data[k])) return false;
+    S.key[k] = K.data[k];
+  }
+  S.card1 = K.card1;
+  S.nm = M.nm;
+  for (int m = 0; m < M.nm; m++) {
+    if (M.nulls[m] || M.data2[m] || !aligned16(M.data[m])) return false;
+    if (M.type[m] == KG_REAL) S.real[m] = 1;
+    else if (M.type[m] == KG_INT) S.real[m] = 0;
+    else return false;
+    S.mp[m] = M.data[m];
+  }
+  return true;
+}
+
 }  // namespace
 
 // Number of workgroups of the fixed-order aggregation for n rows; the CPU
 // oracle's deterministic restatement takes the same value.
 extern "C" int kg_agg_blocks_for(int64_t n) {
-  int64_t nb = ceil_div(n, static_cast<int64_t>(kBlock) * kAggRowsPerThreadMin);
+  int64_t nb = ceil_div(n, static_cast<int64_t>(kBlock) * kAggRowsPerThreadMin * 4);
   if (nb < 1) nb = 1;
   if (nb > kAggMaxBlocks) nb = kAggMaxBlocks;
   return static_cast<int>(nb);
+}
+
+extern "C" int kg_ctx_agg_order(const kg_ctx* ctx, int64_t n, int32_t* nblocks, int32_t* vec) {
+  if (!ctx || !nblocks || !vec) return KG_ERR_INVALID;
+  *nblocks = kg_agg_blocks_for(n);
+  *vec = ctx->last_agg_vec;
+  return KG_OK;
 }
 
 extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
@@ -582,16 +825,21 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
     o.first = reinterpret_cast<int32_t*>(w + o_first);
     hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(G, 256)), dim3(256), 0, ctx->stream, o.first, INT32_MAX, G);
     KG_LAUNCH_CHECK(ctx);
+    FastSet S;
+    const bool fast = fast_set(P, K, M, minmax, S);
     kg_time_begin(ctx);
-    const bool launched = launch_reg_any(static_cast<int>(G), minmax, ctx->stream, nb, P, K, M, n, o);
+    const bool launched = fast ? launch_v4(static_cast<int>(G), ctx->stream, nb, S, n, o)
+                               : launch_reg_any(static_cast<int>(G), minmax, ctx->stream, nb, P, K, M, n, o);
     kg_time_end(ctx);
+    ctx->last_agg_vec = fast ? kVec : 1;
     if (!launched)
       return kg_fail(ctx, KG_ERR_UNSUPPORTED, "internal: no register kernel for %lld groups", (long long)G);
     KG_LAUNCH_CHECK(ctx);
     AggFinal* dfin = reinterpret_cast<AggFinal*>(w + o_fin);
     uint64_t* drows = reinterpret_cast<uint64_t*>(w + o_rfin);
-    const int threads = static_cast<int>(G * nm + G);
-    hipLaunchKernelGGL(k_group_agg_final, dim3(ceil_div(threads, 256)), dim3(256), 0, ctx->stream, o, nb,
+    const int outputs = static_cast<int>(G * nm + G);
+    const size_t stage_bytes = static_cast<size_t>(nb) * 8 * (minmax ? 4 : 2);
+    hipLaunchKernelGGL(k_group_agg_final, dim3(outputs), dim3(64), stage_bytes, ctx->stream, o, nb,
                        static_cast<int>(G), M, minmax, dfin, drows);
     KG_LAUNCH_CHECK(ctx);
     KG_HIP_CHECK(ctx, hipMemcpyAsync(fin.data(), dfin, G * nm * sizeof(AggFinal), hipMemcpyDeviceToHost, ctx->stream));

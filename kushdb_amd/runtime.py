@@ -87,6 +87,13 @@ class Context:
     def set_timing(self, on):
         self.check(self.lib.kg_ctx_set_timing(self.h, 1 if on else 0))
 
+    def agg_order(self, n):
+        """(workgroups, rows per lane step) of the last fixed-order aggregation over n rows."""
+        nb = ctypes.c_int32()
+        vec = ctypes.c_int32()
+        self.check(self.lib.kg_ctx_agg_order(self.h, n, ctypes.byref(nb), ctypes.byref(vec)))
+        return nb.value, vec.value
+
     def kernel_stats(self, reset=True):
         """(summed ms, launches) of the dominant kernels since the last reset."""
         ms = ctypes.c_double()

@@ -976,31 +976,35 @@ int64_t kor_q_star(int64_t nf, const int32_t* fk, const int32_t* qty, const doub
   return ng;
 }
 
-// The fixed reduction order of the GPU scan-aggregate kernel, restated on the
-// CPU so that its deterministic mode can be compared bit for bit:
-//   T = nblocks * block_threads threads; thread t folds (from +0.0) the
-//   qualifying rows r = t + k*T, k ascending, into per-group partials;
-//   each 64-lane wave combines its lanes by the tree p[l] += p[l + s] for
-//   s = 32, 16, ..., 1; the block's waves are combined by the same tree over
-//   wave index (s = nwaves/2 .. 1); block partials are added in block order
-//   to +0.0. out_sums[g * nf + f].
+// The fixed reduction order of the GPU scan-aggregate kernels, restated on
+// the CPU so that their results can be compared bit for bit:
+//   T = nblocks * block_threads threads; rows are taken in chunks of `vec`
+//   consecutive rows; thread t folds (from +0.0) the qualifying rows of chunks
+//   c = t + k*T, k ascending, rows of a chunk in ascending order, into
+//   per-group partials; each 64-lane wave combines its lanes by the tree
+//   p[l] += p[l + s] for s = 32, 16, ..., 1; the block's waves are combined by
+//   the same tree over wave index (s = nwaves/2 .. 1); block partials are added
+//   in block order to +0.0. out_sums[g * nf + f].
 void kor_dtree_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
                         const int32_t* g2, int32_t card2, int32_t ngroups, int32_t nf,
-                        const double* const* fm, int32_t nblocks, int32_t block_threads,
+                        const double* const* fm, int32_t nblocks, int32_t block_threads, int32_t vec,
                         double* out_sums) {
   const int64_t T = static_cast<int64_t>(nblocks) * block_threads;
   const int32_t W = 64, nwaves = block_threads / W;
   const int32_t width = ngroups * nf;
+  const int64_t nchunks = (n + vec - 1) / vec;
   std::vector<double> total(width, 0.0);
   std::vector<double> lanes(static_cast<size_t>(block_threads) * width);
   for (int32_t b = 0; b < nblocks; b++) {
     std::fill(lanes.begin(), lanes.end(), 0.0);
     for (int32_t l = 0; l < block_threads; l++) {
       double* acc = &lanes[static_cast<size_t>(l) * width];
-      for (int64_t r = static_cast<int64_t>(b) * block_threads + l; r < n; r += T) {
-        if (day[r] > cutoff) continue;
-        const int32_t g = g1[r] * card2 + g2[r];
-        for (int32_t f = 0; f < nf; f++) acc[g * nf + f] += fm[f][r];
+      for (int64_t c = static_cast<int64_t>(b) * block_threads + l; c < nchunks; c += T) {
+        for (int64_t r = c * vec; r < (c + 1) * vec && r < n; r++) {
+          if (day[r] > cutoff) continue;
+          const int32_t g = g1[r] * card2 + g2[r];
+          for (int32_t f = 0; f < nf; f++) acc[g * nf + f] += fm[f][r];
+        }
       }
     }
     for (int32_t w = 0; w < nwaves; w++) {

@@ -132,7 +132,7 @@ int64_t kor_q_star(int64_t nf, const int32_t* fk, const int32_t* qty, const doub
 // ---- deterministic fixed-order reduction (GPU deterministic mode order) ----
 void kor_dtree_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
                         const int32_t* g2, int32_t card2, int32_t ngroups, int32_t nf,
-                        const double* const* fm, int32_t nblocks, int32_t block_threads,
+                        const double* const* fm, int32_t nblocks, int32_t block_threads, int32_t vec,
                         double* out_sums);
 
 #ifdef __cplusplus

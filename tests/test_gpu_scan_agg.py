@@ -53,10 +53,10 @@ def oracle_seq(h, n):
 def oracle_dtree(ctx, h, n):
     fm = [np.ascontiguousarray(h[f"m{i}"]) for i in range(1, 5)]
     fp = (ctypes.c_void_p * 4)(*[a.ctypes.data for a in fm])
-    nb = ctx.lib.kg_agg_blocks_for(n)
+    nb, vec = ctx.agg_order(n)
     out = np.zeros(6 * 4)
     po.lib().kor_dtree_scan_agg(n, h["day"].ctypes.data, h["cutoff"], h["g1"].ctypes.data, h["g2"].ctypes.data, 2,
-                                6, 4, fp, nb, 256, out.ctypes.data)
+                                6, 4, fp, nb, 256, vec, out.ctypes.data)
     return out.reshape(6, 4)
 
 
@@ -87,6 +87,8 @@ def test_scan_agg_vs_oracle(ctx, n):
         # COUNT(*)
         assert gv[14] == wv[14]
     # Deterministic order: bit-exact against the oracle's restatement of it.
+    # (These aligned, non-NULL INT/REAL columns take the 4-row vector path.)
+    assert ctx.agg_order(n)[1] == 4
     dt = oracle_dtree(ctx, h, n)
     for g in range(len(want)):
         code = codes[g, 0] * 2 + codes[g, 1]

@@ -236,7 +236,7 @@ def test_dtree_order_vs_fsum():
     m = [rng.uniform(0, 1e5, n) for _ in range(2)]
     ptrs = (ctypes.c_void_p * 2)(*[a.ctypes.data for a in m])
     out = np.zeros(12)
-    po.lib().kor_dtree_scan_agg(n, day.ctypes.data, 97, g1.ctypes.data, g2.ctypes.data, 2, 6, 2, ptrs, 7, 256,
+    po.lib().kor_dtree_scan_agg(n, day.ctypes.data, 97, g1.ctypes.data, g2.ctypes.data, 2, 6, 2, ptrs, 7, 256, 4,
                                 out.ctypes.data)
     sel = day <= 97
     for g in range(6):
