@@ -243,24 +243,59 @@ __global__ void k_fill_i32j(int32_t* p, int32_t v, int64_t n) {
 }
 
 // Compaction of the occupied accumulator slots into (key, sum, count, first).
-// Slot order is not meaningful; a wave-aggregated counter assigns positions.
+// Each workgroup owns a contiguous range of slots: it counts its live slots,
+// takes one range of output positions with a single atomic, then writes them
+// in slot order (wave ballot + popcount prefix, wave offsets through LDS).
 __global__ __launch_bounds__(kJBlock) void k_collect(const int64_t* keys, const Acc* acc, const int32_t* first,
                                                       int64_t nslots, int64_t side_key_slot,
                                                       unsigned long long* count, int64_t cap,
                                                       int64_t* out_key, int64_t* out_sum, int64_t* out_cnt,
                                                       int32_t* out_first) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * kJBlock;
   const int64_t total = nslots + 1;
-  const int64_t lim = ((total + kWave - 1) / kWave) * kWave;
-  for (int64_t s = static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x; s < lim; s += stride) {
-    const bool live = s < total && acc[s].count > 0;
-    const unsigned long long pos = wave_append(count, live ? 1u : 0u);
-    if (live && pos < static_cast<unsigned long long>(cap)) {
-      out_key[pos] = s == side_key_slot ? kEmpty : keys[s];
-      out_sum[pos] = static_cast<int64_t>(acc[s].sum);
-      out_cnt[pos] = static_cast<int64_t>(acc[s].count);
-      if (out_first) out_first[pos] = first[s];
+  const int64_t chunk = (total + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = static_cast<int64_t>(blockIdx.x) * chunk;
+  const int64_t hi = lo + chunk < total ? lo + chunk : total;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  constexpr int NW = kJBlock / kWave;
+  __shared__ unsigned int wsum[NW];
+  __shared__ unsigned long long base;
+  unsigned int mine = 0;
+  for (int64_t s = lo + threadIdx.x; s < hi; s += kJBlock) mine += acc[s].count > 0 ? 1u : 0u;
+  for (int st = kWave / 2; st >= 1; st >>= 1) mine += __shfl_xor(mine, st);
+  if (lane == 0) wsum[wave] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < NW; w++) t += wsum[w];
+    base = t ? atomicAdd(count, t) : 0;
+  }
+  __syncthreads();
+  unsigned long long pos = base;
+  __shared__ unsigned int wc[2][NW];
+  int it = 0;
+  for (int64_t s0 = lo; s0 < hi; s0 += kJBlock, it++) {
+    const int64_t s = s0 + threadIdx.x;
+    const bool live = s < hi && acc[s].count > 0;
+    const uint64_t m = __ballot(live);
+    if (lane == 0) wc[it & 1][wave] = __popcll(m);
+    __syncthreads();
+    unsigned int before = 0, tot = 0;
+    for (int w = 0; w < NW; w++) {
+      const unsigned int c = wc[it & 1][w];
+      before += w < wave ? c : 0;
+      tot += c;
     }
+    if (live) {
+      const unsigned long long p = pos + before + __popcll(m & ((1ull << lane) - 1ull));
+      if (p < static_cast<unsigned long long>(cap)) {
+        out_key[p] = s == side_key_slot ? kEmpty : keys[s];
+        out_sum[p] = static_cast<int64_t>(acc[s].sum);
+        out_cnt[p] = static_cast<int64_t>(acc[s].count);
+        if (out_first) out_first[p] = first[s];
+      }
+    }
+    pos += tot;
   }
 }
 
