@@ -321,7 +321,10 @@ __global__ __launch_bounds__(64) void k_group_agg_final(const AggOut in, int nbl
 // non-NULL 32-bit integer column, every measure a non-NULL REAL or INT
 // (32-bit) column, all 16-byte aligned; conjunctive predicates; no MIN/MAX.
 // Each lane takes kVec consecutive rows per step with 16-byte loads; rows of a
-// chunk are folded in ascending order, chunks c = t + k*T in ascending k.
+// chunk are folded in ascending order, chunks c = t + k*T in ascending k (the
+// last, partial chunk belongs to lane nfull % T after its full chunks).
+// REAL and INT measures are accumulated in separate register arrays whose
+// sizes (NF, NI) are compile-time.
 constexpr int kVec = 4;
 constexpr int kFastPreds = 2;
 
@@ -333,97 +336,107 @@ struct FastSet {
   int32_t nk;
   const int32_t* key[2];
   int32_t card1;
+  int32_t nf, ni;              // REAL / INT measures in use (<= NF / NI)
+  const double* fcol[4];
+  const int32_t* icol[4];
+  int32_t fidx[4], iidx[4];    // their positions in the MeasureSet order
   int32_t nm;
-  int32_t real[kMaxMeasures];
-  const void* mp[kMaxMeasures];
 };
 
-__device__ __forceinline__ void ld_i32x4(const int32_t* p, int64_t r0, int64_t n, bool full, int32_t (&o)[kVec]) {
-  if (full) {
-    const int4 x = *reinterpret_cast<const int4*>(p + r0);
-    o[0] = x.x;
-    o[1] = x.y;
-    o[2] = x.z;
-    o[3] = x.w;
-  } else {
+template <int G, int NF, int NI>
+struct FastAcc {
+  double f[G][NF > 0 ? NF : 1];
+  int64_t i[G][NI > 0 ? NI : 1];
+  uint32_t rows[G];
+  int32_t first[G];
+};
+
+template <int G, int NF, int NI, bool FULL>
+__device__ __forceinline__ void fast_chunk(const FastSet& S, int64_t r0, int64_t n, int Grt,
+                                           FastAcc<G, NF, NI>& A) {
+  int32_t pv[kFastPreds][kVec], kv[2][kVec], iv[NI > 0 ? NI : 1][kVec];
+  double fv[NF > 0 ? NF : 1][kVec];
+  auto ld32 = [&](const int32_t* p, int32_t (&o)[kVec]) {
+    if (FULL) {
+      const int4 x = *reinterpret_cast<const int4*>(p + r0);
+      o[0] = x.x;
+      o[1] = x.y;
+      o[2] = x.z;
+      o[3] = x.w;
+    } else {
 #pragma unroll
-    for (int v = 0; v < kVec; v++) o[v] = r0 + v < n ? p[r0 + v] : 0;
+      for (int v = 0; v < kVec; v++) o[v] = r0 + v < n ? p[r0 + v] : 0;
+    }
+  };
+#pragma unroll
+  for (int q = 0; q < kFastPreds; q++)
+    if (q < S.npred) ld32(S.pcol[q], pv[q]);
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+    if (q < S.nk) ld32(S.key[q], kv[q]);
+#pragma unroll
+  for (int m = 0; m < NI; m++)
+    if (m < S.ni) ld32(S.icol[m], iv[m]);
+#pragma unroll
+  for (int m = 0; m < NF; m++) {
+    if (m >= S.nf) continue;
+    const double* p = S.fcol[m];
+    if (FULL) {
+      const double2 x = *reinterpret_cast<const double2*>(p + r0);
+      const double2 y = *reinterpret_cast<const double2*>(p + r0 + 2);
+      fv[m][0] = x.x;
+      fv[m][1] = x.y;
+      fv[m][2] = y.x;
+      fv[m][3] = y.y;
+    } else {
+#pragma unroll
+      for (int v = 0; v < kVec; v++) fv[m][v] = r0 + v < n ? p[r0 + v] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < kVec; v++) {
+    bool pass = FULL || r0 + v < n;
+#pragma unroll
+    for (int q = 0; q < kFastPreds; q++)
+      if (q < S.npred) pass = pass && cmp_i64(S.op[q], pv[q][v], S.lit[q]);
+    const int grp = S.nk == 0 ? 0 : (S.nk == 1 ? kv[0][v] : kv[0][v] * S.card1 + kv[1][v]);
+    pass = pass && grp >= 0 && grp < Grt;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (pass && grp == g) {
+        A.rows[g]++;
+        A.first[g] = min(A.first[g], static_cast<int32_t>(r0 + v));
+#pragma unroll
+        for (int m = 0; m < NF; m++) A.f[g][m] += fv[m][v];
+#pragma unroll
+        for (int m = 0; m < NI; m++) A.i[g][m] += iv[m][v];
+      }
+    }
   }
 }
 
-template <int G, int NM>
+template <int G, int NF, int NI>
 __global__ __launch_bounds__(kBlock) void k_group_agg_v4(FastSet S, int64_t n, int Grt, AggOut out) {
-  uint64_t acc[G][NM];
-  uint32_t rows[G];
-  int32_t first[G];
+  FastAcc<G, NF, NI> A;
 #pragma unroll
   for (int g = 0; g < G; g++) {
-    rows[g] = 0;
-    first[g] = INT32_MAX;
+    A.rows[g] = 0;
+    A.first[g] = INT32_MAX;
 #pragma unroll
-    for (int m = 0; m < NM; m++) acc[g][m] = 0;
+    for (int m = 0; m < NF; m++) A.f[g][m] = 0.0;
+#pragma unroll
+    for (int m = 0; m < NI; m++) A.i[g][m] = 0;
   }
   const int64_t T = static_cast<int64_t>(gridDim.x) * kBlock;
-  const int64_t nchunks = (n + kVec - 1) / kVec;
-  for (int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; c < nchunks; c += T) {
-    const int64_t r0 = c * kVec;
-    const bool full = r0 + kVec <= n;
-    int32_t pv[kFastPreds][kVec], kv[2][kVec];
-    uint64_t mv[NM][kVec];
-#pragma unroll
-    for (int i = 0; i < kFastPreds; i++)
-      if (i < S.npred) ld_i32x4(S.pcol[i], r0, n, full, pv[i]);
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-      if (i < S.nk) ld_i32x4(S.key[i], r0, n, full, kv[i]);
-#pragma unroll
-    for (int m = 0; m < NM; m++) {
-      if (m >= S.nm) continue;
-      if (S.real[m]) {
-        const double* p = static_cast<const double*>(S.mp[m]);
-        if (full) {
-          const longlong2 a = *reinterpret_cast<const longlong2*>(p + r0);
-          const longlong2 b = *reinterpret_cast<const longlong2*>(p + r0 + 2);
-          mv[m][0] = a.x;
-          mv[m][1] = a.y;
-          mv[m][2] = b.x;
-          mv[m][3] = b.y;
-        } else {
-#pragma unroll
-          for (int v = 0; v < kVec; v++)
-            mv[m][v] = r0 + v < n ? static_cast<uint64_t>(__double_as_longlong(p[r0 + v])) : 0;
-        }
-      } else {
-        int32_t t[kVec];
-        ld_i32x4(static_cast<const int32_t*>(S.mp[m]), r0, n, full, t);
-#pragma unroll
-        for (int v = 0; v < kVec; v++) mv[m][v] = static_cast<uint64_t>(static_cast<int64_t>(t[v]));
-      }
-    }
-#pragma unroll
-    for (int v = 0; v < kVec; v++) {
-      bool pass = r0 + v < n;
-#pragma unroll
-      for (int i = 0; i < kFastPreds; i++)
-        if (i < S.npred) pass = pass && cmp_i64(S.op[i], pv[i][v], S.lit[i]);
-      const int grp = S.nk == 0 ? 0 : (S.nk == 1 ? kv[0][v] : kv[0][v] * S.card1 + kv[1][v]);
-      pass = pass && grp >= 0 && grp < Grt;
-#pragma unroll
-      for (int g = 0; g < G; g++) {
-        if (!(pass && grp == g)) continue;
-        rows[g]++;
-        first[g] = min(first[g], static_cast<int32_t>(r0 + v));
-#pragma unroll
-        for (int m = 0; m < NM; m++) {
-          if (m >= S.nm) continue;
-          acc[g][m] = S.real[m] ? fadd_bits(acc[g][m], mv[m][v]) : acc[g][m] + mv[m][v];
-        }
-      }
-    }
-  }
+  const int64_t nfull = n / kVec;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+#pragma unroll 2
+  for (int64_t c = t; c < nfull; c += T) fast_chunk<G, NF, NI, true>(S, c * kVec, n, Grt, A);
+  if (nfull * kVec < n && t == nfull % T) fast_chunk<G, NF, NI, false>(S, nfull * kVec, n, Grt, A);
 
   // Same fixed-order workgroup reduction as k_group_agg_reg.
-  __shared__ uint64_t wred[kBlock / kWave][G * NM];
+  constexpr int NFI = (NF > 0 ? NF : 1) + (NI > 0 ? NI : 1);
+  __shared__ uint64_t wred[kBlock / kWave][G * NFI];
   __shared__ uint32_t wcnt[kBlock / kWave][G];
   __shared__ int32_t wfirst[kBlock / kWave][G];
   const int lane = threadIdx.x & (kWave - 1);
@@ -431,17 +444,23 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_v4(FastSet S, int64_t n, i
 #pragma unroll
   for (int g = 0; g < G; g++) {
 #pragma unroll
-    for (int m = 0; m < NM; m++) {
-      if (m >= S.nm) continue;
-      uint64_t sv = acc[g][m];
+    for (int m = 0; m < NF; m++) {
+      double sv = A.f[g][m];
       for (int st = kWave / 2; st >= 1; st >>= 1) {
-        const uint64_t os = shfl_xor_u64(sv, st);
-        sv = S.real[m] ? ((lane & st) ? fadd_bits(os, sv) : fadd_bits(sv, os)) : sv + os;
+        const double os = __longlong_as_double(
+            static_cast<long long>(shfl_xor_u64(static_cast<uint64_t>(__double_as_longlong(sv)), st)));
+        sv = (lane & st) ? os + sv : sv + os;
       }
-      if (lane == 0) wred[wave][g * NM + m] = sv;
+      if (lane == 0) wred[wave][g * NFI + m] = static_cast<uint64_t>(__double_as_longlong(sv));
     }
-    uint32_t rc = rows[g];
-    int32_t fr = first[g];
+#pragma unroll
+    for (int m = 0; m < NI; m++) {
+      int64_t sv = A.i[g][m];
+      for (int st = kWave / 2; st >= 1; st >>= 1) sv += static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(sv), st));
+      if (lane == 0) wred[wave][g * NFI + (NF > 0 ? NF : 1) + m] = static_cast<uint64_t>(sv);
+    }
+    uint32_t rc = A.rows[g];
+    int32_t fr = A.first[g];
     for (int st = kWave / 2; st >= 1; st >>= 1) {
       rc += __shfl_xor(rc, st);
       fr = min(fr, __shfl_xor(fr, st));
@@ -453,16 +472,19 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_v4(FastSet S, int64_t n, i
   }
   __syncthreads();
   constexpr int NW = kBlock / kWave;
-  for (int idx = threadIdx.x; idx < G * NM; idx += kBlock) {
-    const int g = idx / NM, m = idx % NM;
-    if (m >= S.nm || g >= Grt) continue;
+  for (int idx = threadIdx.x; idx < G * NFI; idx += kBlock) {
+    const int g = idx / NFI, j = idx % NFI;
+    const bool is_f = j < (NF > 0 ? NF : 1);
+    const int m = is_f ? j : j - (NF > 0 ? NF : 1);
+    if (g >= Grt || (is_f ? m >= S.nf : m >= S.ni)) continue;
     uint64_t p[NW];
     for (int w = 0; w < NW; w++) p[w] = wred[w][idx];
     for (int st = NW / 2; st >= 1; st >>= 1)
-      for (int w = 0; w < st; w++) p[w] = S.real[m] ? fadd_bits(p[w], p[w + st]) : p[w] + p[w + st];
+      for (int w = 0; w < st; w++) p[w] = is_f ? fadd_bits(p[w], p[w + st]) : p[w] + p[w + st];
     uint32_t cnt = 0;
     for (int w = 0; w < NW; w++) cnt += wcnt[w][g];
-    const int64_t o = (static_cast<int64_t>(blockIdx.x) * Grt + g) * S.nm + m;
+    const int mi = is_f ? S.fidx[m] : S.iidx[m];
+    const int64_t o = (static_cast<int64_t>(blockIdx.x) * Grt + g) * S.nm + mi;
     out.sum[o] = p[0];
     out.nn[o] = cnt;
   }
@@ -682,14 +704,24 @@ bool launch_reg_any(int G, bool mm, hipStream_t s, int nb, const PredSet& P, con
   return true;
 }
 
+template <int G, int NF>
+void launch_v4_gf(hipStream_t s, int nb, const FastSet& S, int64_t n, int Grt, const AggOut& o) {
+  if (S.ni == 0)
+    hipLaunchKernelGGL((k_group_agg_v4<G, NF, 0>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
+  else if (S.ni <= 1)
+    hipLaunchKernelGGL((k_group_agg_v4<G, NF, 1>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
+  else if (S.ni <= 3)
+    hipLaunchKernelGGL((k_group_agg_v4<G, NF, 3>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
+  else
+    hipLaunchKernelGGL((k_group_agg_v4<G, NF, 4>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
+}
+
 template <int G>
 void launch_v4_g(hipStream_t s, int nb, const FastSet& S, int64_t n, int Grt, const AggOut& o) {
-  if (S.nm <= 2)
-    hipLaunchKernelGGL((k_group_agg_v4<G, 2>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
-  else if (S.nm <= 4)
-    hipLaunchKernelGGL((k_group_agg_v4<G, 4>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
-  else
-    hipLaunchKernelGGL((k_group_agg_v4<G, 8>), dim3(nb), dim3(kBlock), 0, s, S, n, Grt, o);
+  if (S.nf == 0) launch_v4_gf<G, 0>(s, nb, S, n, Grt, o);
+  else if (S.nf <= 1) launch_v4_gf<G, 1>(s, nb, S, n, Grt, o);
+  else if (S.nf <= 2) launch_v4_gf<G, 2>(s, nb, S, n, Grt, o);
+  else launch_v4_gf<G, 4>(s, nb, S, n, Grt, o);
 }
 
 bool launch_v4(int G, hipStream_t s, int nb, const FastSet& S, int64_t n, const AggOut& o) {
@@ -725,10 +757,17 @@ bool fast_set(const PredSet& P, const KeySet& K, const MeasureSet& M, bool minma
   S.nm = M.nm;
   for (int m = 0; m < M.nm; m++) {
     if (M.nulls[m] || M.data2[m] || !aligned16(M.data[m])) return false;
-    if (M.type[m] == KG_REAL) S.real[m] = 1;
-    else if (M.type[m] == KG_INT) S.real[m] = 0;
-    else return false;
-    S.mp[m] = M.data[m];
+    if (M.type[m] == KG_REAL) {
+      if (S.nf == 4) return false;
+      S.fidx[S.nf] = m;
+      S.fcol[S.nf++] = static_cast<const double*>(M.data[m]);
+    } else if (M.type[m] == KG_INT) {
+      if (S.ni == 4) return false;
+      S.iidx[S.ni] = m;
+      S.icol[S.ni++] = static_cast<const int32_t*>(M.data[m]);
+    } else {
+      return false;
+    }
   }
   return true;
 }
