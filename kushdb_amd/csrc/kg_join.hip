@@ -127,6 +127,97 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long* co
   return base + (incl - want);
 }
 
+// One probe (held by a 4-lane group) from its first bucket on. kv0 is this
+// lane's 16 bytes of the first bucket, already loaded by the caller so that
+// several probes' first loads are in flight together. Must be called by the
+// whole wave (ballots).
+template <int MODE, bool FIRST>
+__device__ __forceinline__ void probe_one(const TableView& t, const ProbeOut& o, int64_t r, int64_t k, bool active,
+                                          uint64_t b, longlong2 kv0, int sub, int grp,
+                                          unsigned long long& local) {
+  unsigned nmatch = 0;
+  int64_t first_slot = -1;
+  // The side list holds build rows whose key is the free-slot marker.
+  if (active && k == kEmpty) {
+    active = false;
+    if (sub == 0) {
+      unsigned long long ns = *t.side_count;
+      if (ns > static_cast<unsigned long long>(kSideCap)) ns = kSideCap;  // overflowed build (reported there)
+      if (MODE == kCount) local += ns;
+      if (MODE == kAgg && ns) {
+        nmatch = static_cast<unsigned>(ns);
+        first_slot = static_cast<int64_t>(t.nbuckets) * kSlots;
+      }
+      if (MODE == kEmit) {
+        const unsigned long long pos = atomicAdd(o.count, ns);
+        for (unsigned long long j = 0; j < ns; j++)
+          if (pos + j < static_cast<unsigned long long>(o.cap)) {
+            o.out_probe[pos + j] = static_cast<int32_t>(r);
+            o.out_build[pos + j] = t.side_rows[j];
+          }
+      }
+    }
+  }
+  uint64_t scanned = 0;
+  bool first_bucket = true;
+  while (__any(active)) {
+    bool m0 = false, m1 = false, e = false;
+    int64_t slot0 = 0;
+    if (active) {
+      slot0 = static_cast<int64_t>(b) * kSlots + sub * 2;
+      const longlong2 kv = first_bucket ? kv0 : *reinterpret_cast<const longlong2*>(t.keys + slot0);
+      m0 = kv.x == k;
+      m1 = kv.y == k;
+      e = kv.x == kEmpty || kv.y == kEmpty;
+    }
+    first_bucket = false;
+    const uint64_t bm0 = __ballot(m0), bm1 = __ballot(m1), be = __ballot(e);
+    const unsigned g0 = (bm0 >> (grp * kProbeLanes)) & 0xF;
+    const unsigned g1 = (bm1 >> (grp * kProbeLanes)) & 0xF;
+    const unsigned ge = (be >> (grp * kProbeLanes)) & 0xF;
+    if (MODE == kCount) {
+      if (active && sub == 0) local += __popc(g0) + __popc(g1);
+    } else if (MODE == kEmit) {
+      const unsigned want = (m0 ? 1u : 0u) + (m1 ? 1u : 0u);
+      const unsigned long long pos = wave_append(o.count, active ? want : 0u);
+      if (m0 && pos < static_cast<unsigned long long>(o.cap)) {
+        o.out_probe[pos] = static_cast<int32_t>(r);
+        o.out_build[pos] = t.rows[slot0];
+      }
+      if (m1 && pos + (m0 ? 1 : 0) < static_cast<unsigned long long>(o.cap)) {
+        o.out_probe[pos + (m0 ? 1 : 0)] = static_cast<int32_t>(r);
+        o.out_build[pos + (m0 ? 1 : 0)] = t.rows[slot0 + 1];
+      }
+    } else {
+      if (active) {
+        nmatch += __popc(g0) + __popc(g1);
+        if (first_slot < 0 && (g0 | g1)) {
+          // Slot order within the bucket: lane sub holds slots 2*sub, 2*sub+1.
+          unsigned bits = 0;
+          for (int q = 0; q < kProbeLanes; q++) bits |= (((g0 >> q) & 1u) << (2 * q)) | (((g1 >> q) & 1u) << (2 * q + 1));
+          first_slot = static_cast<int64_t>(b) * kSlots + (__ffs(bits) - 1);
+        }
+      }
+    }
+    if (active) {
+      scanned++;
+      if (ge || scanned >= t.nbuckets) active = false;
+      b = (b + 1 == t.nbuckets) ? 0 : b + 1;
+    }
+  }
+  if (MODE == kAgg && sub == 0 && nmatch) {
+    Acc* a = &o.acc[first_slot];
+    atomicAdd(&a->sum, static_cast<unsigned long long>(o.vals[r]) * nmatch);
+    atomicAdd(&a->count, static_cast<unsigned long long>(nmatch));
+    if (FIRST) atomicMin(&o.first[first_slot], static_cast<int32_t>(r));
+  }
+}
+
+// Probe kernel: every 4-lane group handles kUnroll probes per wave step; the
+// first-bucket loads of all of them are issued before any is examined, so a
+// wave keeps 16 * kUnroll random 64-byte reads in flight.
+constexpr int kUnroll = 4;
+
 template <int MODE, bool FIRST>
 __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, const int32_t* sel, int64_t n,
                                                        ProbeOut o) {
@@ -134,92 +225,36 @@ __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, c
   const int sub = lane & (kProbeLanes - 1);
   const int grp = lane / kProbeLanes;
   constexpr int kGroups = kWave / kProbeLanes;
+  constexpr int kStep = kGroups * kUnroll;
   const int64_t gwave = (static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x) / kWave;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * kJBlock / kWave;
   unsigned long long local = 0;
-  for (int64_t base = gwave * kGroups; base < n; base += nwaves * kGroups) {
-    const int64_t i = base + grp;
-    int64_t r = -1, k = 0;
-    bool active = false;
-    if (i < n) {
-      r = sel ? sel[i] : i;
-      active = load_key(key, r, k);
-    }
-    unsigned nmatch = 0;
-    int64_t first_slot = -1;
-    // The side list holds build rows whose key is the free-slot marker.
-    if (active && k == kEmpty) {
-      active = false;
-      if (sub == 0) {
-        unsigned long long ns = *t.side_count;
-        if (ns > static_cast<unsigned long long>(kSideCap)) ns = kSideCap;  // overflowed build (reported there)
-        if (MODE == kCount) local += ns;
-        if (MODE == kAgg && ns) {
-          nmatch = static_cast<unsigned>(ns);
-          first_slot = static_cast<int64_t>(t.nbuckets) * kSlots;
-        }
-        if (MODE == kEmit) {
-          const unsigned long long pos = atomicAdd(o.count, ns);
-          for (unsigned long long j = 0; j < ns; j++)
-            if (pos + j < static_cast<unsigned long long>(o.cap)) {
-              o.out_probe[pos + j] = static_cast<int32_t>(r);
-              o.out_build[pos + j] = t.side_rows[j];
-            }
-        }
+  for (int64_t base = gwave * kStep; base < n; base += nwaves * kStep) {
+    int64_t r[kUnroll], k[kUnroll];
+    bool active[kUnroll];
+    uint64_t b[kUnroll];
+    longlong2 kv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int64_t i = base + u * kGroups + grp;
+      r[u] = -1;
+      k[u] = 0;
+      active[u] = false;
+      if (i < n) {
+        r[u] = sel ? sel[i] : i;
+        active[u] = load_key(key, r[u], k[u]);
       }
+      b[u] = active[u] ? bucket_of(k[u], t.nbuckets) : 0;
     }
-    uint64_t b = active ? bucket_of(k, t.nbuckets) : 0;
-    uint64_t scanned = 0;
-    while (__any(active)) {
-      bool m0 = false, m1 = false, e = false;
-      int64_t slot0 = 0;
-      if (active) {
-        slot0 = static_cast<int64_t>(b) * kSlots + sub * 2;
-        const longlong2 kv = *reinterpret_cast<const longlong2*>(t.keys + slot0);
-        m0 = kv.x == k;
-        m1 = kv.y == k;
-        e = kv.x == kEmpty || kv.y == kEmpty;
-      }
-      const uint64_t bm0 = __ballot(m0), bm1 = __ballot(m1), be = __ballot(e);
-      const unsigned g0 = (bm0 >> (grp * kProbeLanes)) & 0xF;
-      const unsigned g1 = (bm1 >> (grp * kProbeLanes)) & 0xF;
-      const unsigned ge = (be >> (grp * kProbeLanes)) & 0xF;
-      if (MODE == kCount) {
-        if (active && sub == 0) local += __popc(g0) + __popc(g1);
-      } else if (MODE == kEmit) {
-        const unsigned want = (m0 ? 1u : 0u) + (m1 ? 1u : 0u);
-        const unsigned long long pos = wave_append(o.count, active ? want : 0u);
-        if (m0 && pos < static_cast<unsigned long long>(o.cap)) {
-          o.out_probe[pos] = static_cast<int32_t>(r);
-          o.out_build[pos] = t.rows[slot0];
-        }
-        if (m1 && pos + (m0 ? 1 : 0) < static_cast<unsigned long long>(o.cap)) {
-          o.out_probe[pos + (m0 ? 1 : 0)] = static_cast<int32_t>(r);
-          o.out_build[pos + (m0 ? 1 : 0)] = t.rows[slot0 + 1];
-        }
-      } else {
-        if (active) {
-          nmatch += __popc(g0) + __popc(g1);
-          if (first_slot < 0 && (g0 | g1)) {
-            // Slot order within the bucket: lane sub holds slots 2*sub, 2*sub+1.
-            unsigned bits = 0;
-            for (int q = 0; q < kProbeLanes; q++) bits |= (((g0 >> q) & 1u) << (2 * q)) | (((g1 >> q) & 1u) << (2 * q + 1));
-            first_slot = static_cast<int64_t>(b) * kSlots + (__ffs(bits) - 1);
-          }
-        }
-      }
-      if (active) {
-        scanned++;
-        if (ge || scanned >= t.nbuckets) active = false;
-        b = (b + 1 == t.nbuckets) ? 0 : b + 1;
-      }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      kv[u] = make_longlong2(kEmpty, kEmpty);
+      if (active[u] && k[u] != kEmpty)
+        kv[u] = *reinterpret_cast<const longlong2*>(t.keys + static_cast<int64_t>(b[u]) * kSlots + sub * 2);
     }
-    if (MODE == kAgg && sub == 0 && nmatch) {
-      Acc* a = &o.acc[first_slot];
-      atomicAdd(&a->sum, static_cast<unsigned long long>(o.vals[r]) * nmatch);
-      atomicAdd(&a->count, static_cast<unsigned long long>(nmatch));
-      if (FIRST) atomicMin(&o.first[first_slot], static_cast<int32_t>(r));
-    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++)
+      probe_one<MODE, FIRST>(t, o, r[u], k[u], active[u], b[u], kv[u], sub, grp, local);
   }
   if (MODE == kCount) {
     for (int s = kWave / 2; s >= 1; s >>= 1) local += __shfl_xor(local, s);
@@ -633,7 +668,7 @@ extern "C" int kg_ht_probe_count(kg_ctx* ctx, const kg_hash_table* ht, const kg_
   ProbeOut o{};
   o.count = cnt;
   kg_time_begin(ctx);
-  hipLaunchKernelGGL((k_ht_probe<kCount, false>), dim3(grid_for(ctx, nrows, 16)), dim3(kJBlock), 0, ctx->stream,
+  hipLaunchKernelGGL((k_ht_probe<kCount, false>), dim3(grid_for(ctx, nrows, 64)), dim3(kJBlock), 0, ctx->stream,
                      ht->view(), *key, d_rows, nrows, o);
   kg_time_end(ctx);
   KG_LAUNCH_CHECK(ctx);
@@ -662,7 +697,7 @@ extern "C" int kg_ht_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* k
   o.out_build = d_out_build;
   o.cap = out_cap;
   kg_time_begin(ctx);
-  hipLaunchKernelGGL((k_ht_probe<kEmit, false>), dim3(grid_for(ctx, nrows, 16)), dim3(kJBlock), 0, ctx->stream,
+  hipLaunchKernelGGL((k_ht_probe<kEmit, false>), dim3(grid_for(ctx, nrows, 64)), dim3(kJBlock), 0, ctx->stream,
                      ht->view(), *key, d_rows, nrows, o);
   kg_time_end(ctx);
   KG_LAUNCH_CHECK(ctx);
@@ -704,10 +739,10 @@ extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_co
     o.vals = d_vals;
     kg_time_begin(ctx);
     if (d_out_first)
-      hipLaunchKernelGGL((k_ht_probe<kAgg, true>), dim3(grid_for(ctx, nrows, 16)), dim3(kJBlock), 0, ctx->stream,
+      hipLaunchKernelGGL((k_ht_probe<kAgg, true>), dim3(grid_for(ctx, nrows, 64)), dim3(kJBlock), 0, ctx->stream,
                          t->view(), *key, nullptr, nrows, o);
     else
-      hipLaunchKernelGGL((k_ht_probe<kAgg, false>), dim3(grid_for(ctx, nrows, 16)), dim3(kJBlock), 0, ctx->stream,
+      hipLaunchKernelGGL((k_ht_probe<kAgg, false>), dim3(grid_for(ctx, nrows, 64)), dim3(kJBlock), 0, ctx->stream,
                          t->view(), *key, nullptr, nrows, o);
     kg_time_end(ctx);
     KG_LAUNCH_CHECK(ctx);

@@ -294,13 +294,22 @@ __global__ __launch_bounds__(64) void k_group_agg_final(const AggOut in, int nbl
     }
     __syncthreads();
     if (lane == 0) {
+      // A workgroup without rows contributes +0.0 (its partial is never
+      // -0.0: every lane starts from +0.0), so folding every workgroup's
+      // partial unconditionally gives the same bits and keeps the LDS reads
+      // independent of the branch.
       const bool real = M.is_real[m];
       uint64_t s = 0, lo = 0, hi = 0, c = 0;
+      if (real) {
+        double d = 0.0;
+        for (int b = 0; b < nblocks; b++) d += __longlong_as_double(static_cast<long long>(ssum[b]));
+        s = static_cast<uint64_t>(__double_as_longlong(d));
+      } else {
+        for (int b = 0; b < nblocks; b++) s += ssum[b];
+      }
       for (int b = 0; b < nblocks; b++) {
         const uint64_t bc = snn[b];
-        if (bc == 0) continue;
-        s = real ? fadd_bits(s, ssum[b]) : s + ssum[b];
-        if (minmax) {
+        if (minmax && bc) {
           if (c == 0 || val_less(real, smn[b], lo)) lo = smn[b];
           if (c == 0 || val_less(real, hi, smx[b])) hi = smx[b];
         }
