@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--build-rows", type=int, default=1 << 24)
     ap.add_argument("--probe-rows", type=int, default=1 << 28)
     ap.add_argument("--theta", type=float, default=0.0)
+    ap.add_argument("--lf", type=float, default=0.5, help="join table load factor")
     return ap.parse_args()
 
 
@@ -131,7 +132,7 @@ def bench_join(ctx, a):
     # build
     build_ms = []
     for i in range(steps + 1):
-        ht = HashTable(ctx, nb)
+        ht = HashTable(ctx, nb, load_factor=a.lf)
         torch.cuda.synchronize()
         ctx.set_timing(True)
         ctx.kernel_stats(reset=True)
@@ -143,7 +144,7 @@ def bench_join(ctx, a):
         if i < steps:
             ht.free()
     bms = sum(build_ms) / len(build_ms)
-    out["join_build"] = {"rows": nb, "kernel_ms": round(bms, 3), "rows_per_s": nb / (bms * 1e-3),
+    out["join_build"] = {"rows": nb, "load_factor": a.lf, "kernel_ms": round(bms, 3), "rows_per_s": nb / (bms * 1e-3),
                          "roofline": roofline(nb * BUILD_BYTES_PER_ROW, bms, measured_traffic("k_ht_build"))}
     # probe (join cardinality)
     ht.probe_count(pcol)

@@ -47,6 +47,28 @@ __device__ __forceinline__ uint64_t bucket_of(int64_t key, uint64_t nbuckets) {
   return __umul64hi(mix64(static_cast<uint64_t>(key)), nbuckets);
 }
 
+// Streamed probe-side key: a non-temporal load, so the key stream does not
+// displace the table from the caches.
+__device__ __forceinline__ bool load_key_nt(const kg_col& c, int64_t r, int64_t& key) {
+  if (c.nulls && c.nulls[r]) return false;
+  if (c.type == KG_BIGINT) {
+    key = __builtin_nontemporal_load(static_cast<const int64_t*>(c.data) + r);
+    return true;
+  }
+  if (c.type == KG_INT || c.type == KG_DATE || c.type == KG_ENUM) {
+    key = __builtin_nontemporal_load(static_cast<const int32_t*>(c.data) + r);
+    return true;
+  }
+  if (c.type == KG_REAL) {
+    const double d = __builtin_nontemporal_load(static_cast<const double*>(c.data) + r);
+    if (d != d) return false;
+    key = d == 0.0 ? 0 : __double_as_longlong(d);
+    return true;
+  }
+  key = load_int(c.type, c.data, r);
+  return true;
+}
+
 // Join key of row r as 64 bits. REAL keys compare as doubles in the reference
 // (Equal, compile/proxy/evaluate.cc:449-453): -0.0 is folded onto +0.0 and a
 // NaN key never matches, so it is treated like NULL.
@@ -242,7 +264,7 @@ __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, c
       active[u] = false;
       if (i < n) {
         r[u] = sel ? sel[i] : i;
-        active[u] = load_key(key, r[u], k[u]);
+        active[u] = load_key_nt(key, r[u], k[u]);
       }
       b[u] = active[u] ? bucket_of(k[u], t.nbuckets) : 0;
     }

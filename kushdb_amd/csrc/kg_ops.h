@@ -14,7 +14,7 @@ constexpr int kMaxPreds = 8;
 constexpr int kMaxMeasures = 8;
 constexpr int kBlock = 256;
 constexpr int kSelTile = 4096;          // rows per workgroup in the selection kernels
-constexpr int kAggMaxBlocks = 2048;     // workgroups of the fixed-order aggregation
+constexpr int kAggMaxBlocks = 1024;     // workgroups of the fixed-order aggregation
 constexpr int kAggRowsPerThreadMin = 8; // n / (kBlock * this) workgroups, capped above
 constexpr int kMaxLdsGroups = 1024;
 
@@ -90,10 +90,10 @@ __device__ __forceinline__ uint64_t load_measure(const MeasureSet& M, int m, int
 // Per-workgroup output record for group g: counts[g], first row, and per
 // measure: sum (double bits or int64), non-NULL count, min, max.
 struct AggOut {
-  uint64_t* sum;     // [nblocks][G][nm]
-  uint64_t* mn;      // [nblocks][G][nm] (minmax only)
-  uint64_t* mx;      // [nblocks][G][nm] (minmax only)
-  uint32_t* nn;      // [nblocks][G][nm] non-NULL counts
+  uint64_t* sum;     // [G][nm][nblocks]
+  uint64_t* mn;      // [G][nm][nblocks] (minmax only)
+  uint64_t* mx;      // [G][nm][nblocks] (minmax only)
+  uint32_t* nn;      // [G][nm][nblocks] non-NULL counts
   uint32_t* rows;    // [nblocks][G]
   int32_t* first;    // [G] atomicMin across workgroups (init INT32_MAX)
 };

@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_reg(PredSet P, KeySet K, M
         pc[w] += pc[w + st];
       }
     }
-    const int64_t o = (static_cast<int64_t>(blockIdx.x) * Grt + g) * M.nm + m;
+    const int64_t o = (static_cast<int64_t>(g) * M.nm + m) * gridDim.x + blockIdx.x;
     out.sum[o] = p[0];
     out.nn[o] = pc[0];
     if constexpr (MINMAX) {
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(64) void k_group_agg_final(const AggOut in, int nbl
     uint64_t* smn = stage + 2 * nblocks;
     uint64_t* smx = stage + 3 * nblocks;
     for (int b = lane; b < nblocks; b += 64) {
-      const int64_t o = (static_cast<int64_t>(b) * G + g) * nm + m;
+      const int64_t o = (static_cast<int64_t>(g) * nm + m) * nblocks + b;
       ssum[b] = in.sum[o];
       snn[b] = in.nn[o];
       if (minmax) {
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_v4(FastSet S, int64_t n, i
     uint32_t cnt = 0;
     for (int w = 0; w < NW; w++) cnt += wcnt[w][g];
     const int mi = is_f ? S.fidx[m] : S.iidx[m];
-    const int64_t o = (static_cast<int64_t>(blockIdx.x) * Grt + g) * S.nm + mi;
+    const int64_t o = (static_cast<int64_t>(g) * S.nm + mi) * gridDim.x + blockIdx.x;
     out.sum[o] = p[0];
     out.nn[o] = cnt;
   }
