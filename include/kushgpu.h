@@ -232,7 +232,10 @@ int kg_ht_free(kg_ctx* ctx, kg_hash_table* ht);
  * GroupByAggregateTranslator::Consume (group_by_aggregate_translator.cc:102-111).
  * Groups come out in the reference's insertion order (first join output row);
  * out_keys[g] is the group code. FP64 sums are added in timing-dependent order
- * (within FP64 rounding of the sequential fold). */
+ * (within FP64 rounding of the sequential fold). With build_codes NULL and
+ * group_card 1 this is the ungrouped AggregateOperator over the join output
+ * (aggregate_translator.cc:34-126): one group, or none when nothing joins, and
+ * out_keys is not written. */
 int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
                            const kg_pred* preds, int32_t npreds, const kg_hash_table* ht,
                            int32_t probe_key_col, const kg_col* build_codes, int32_t group_card,
@@ -344,6 +347,110 @@ int kg_tuple_rows(kg_ctx* ctx, const int32_t* d_tuples, int32_t width, int32_t p
  * fixed-width column (runtime::ColumnData::Get* per row,
  * runtime/column_data.cc:96-108, done once per batch). Device output only. */
 int kg_gather(kg_ctx* ctx, const kg_col* src, const int32_t* d_rows, int64_t n, void* d_out, int8_t* d_out_nulls);
+
+/* ---- batched execution -----------------------------------------------------------
+ * Many queries' pipelines in one persistent kernel fed by device work queues,
+ * the four kernels of a batch (fills, build phase, scan/probe phase, finalize)
+ * captured once as a HIP graph per kg_batch. No counterpart in the reference,
+ * which runs one query at a time (execution/executable_query.cc:199-255); each
+ * batched query is the pipeline a host plan would otherwise run through the
+ * plain entry points:
+ *   KG_Q_SCAN_AGG  kg_scan_group_agg(n, cols, preds, key_cols, key_card, aggs);
+ *   KG_Q_JOIN_AGG  kg_scan_select(build_cols, build_preds) -> kg_ht_build(build
+ *                  key) -> kg_scan_join_group_agg(n, cols, preds, probe key,
+ *                  build code column or none = one group, aggs).
+ * Results use the plain entry points' formats and semantics (SUM wrapping, AVG,
+ * COUNT(col), insertion order). Queries with equal table_id and n read the same
+ * rows and share tile reads. FP64 sums are added in a timing-dependent order
+ * (within FP64 rounding of the plain path); integers, counts and group order
+ * are exact. MIN/MAX and more than 16 aggregates are not batched
+ * (KG_ERR_UNSUPPORTED in that query's result; run it on the plain path). */
+typedef struct kg_batch kg_batch;
+enum kg_query_kind { KG_Q_SCAN_AGG = 0, KG_Q_JOIN_AGG = 1 };
+typedef struct kg_query {
+  int32_t kind;                 /* kg_query_kind */
+  int32_t table_id;             /* scan/probe table identity for tile sharing */
+  int64_t n;                    /* scan/probe rows */
+  const kg_col* cols;           /* scan/probe columns */
+  int32_t ncols;
+  int32_t npreds;
+  const kg_pred* preds;         /* conjunctive */
+  const int32_t* key_cols;      /* SCAN_AGG: dense code keys, as kg_scan_group_agg */
+  const int32_t* key_card;
+  int32_t nkeys;
+  int32_t naggs;
+  const kg_agg* aggs;           /* over cols */
+  int32_t probe_key_col;        /* JOIN_AGG: probe key in cols */
+  int32_t build_key_col;        /* JOIN_AGG: build key in build_cols */
+  int64_t build_n;
+  const kg_col* build_cols;
+  int32_t build_ncols;
+  int32_t build_npreds;
+  const kg_pred* build_preds;
+  int32_t build_code_col;       /* group code column in build_cols, or -1: one group */
+  int32_t group_card;           /* codes are in [0, group_card) */
+  double load_factor;           /* join table load factor; 0 = 0.5 */
+} kg_query;
+typedef struct kg_query_result {
+  int32_t status;               /* KG_OK or this query's error */
+  int32_t reserved;
+  int64_t max_groups;           /* capacity of keys/vals/nulls, in groups (input) */
+  int64_t ngroups;              /* groups produced */
+  int32_t* keys;                /* [max_groups * nkeys] (JOIN_AGG with codes: 1 key) */
+  int64_t* vals;                /* [max_groups * naggs] */
+  int8_t* nulls;                /* [max_groups * naggs] */
+} kg_query_result;
+typedef struct kg_batch_opts {
+  int64_t arena_bytes;          /* device bytes for one batch's tables, accumulators, results; 0 = 1 GiB */
+  int64_t reserved;
+} kg_batch_opts;
+/* A batch slot with its own stream, arena and captured graph. */
+int kg_batch_create(kg_ctx* ctx, const kg_batch_opts* opts, kg_batch** out);
+int kg_batch_free(kg_batch* b);
+/* Projected device bytes of one query in a batch arena (join table included),
+ * the admission measure of the serving loop. */
+int kg_batch_footprint(const kg_query* q, int64_t* bytes);
+int64_t kg_batch_arena_bytes(const kg_batch* b);
+/* Launch up to 256 queries on the batch's stream and return. Per-query errors
+ * (validation, arena capacity) are reported in kg_batch_collect's results. */
+int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq);
+/* *done = 1 when the submitted batch has finished on the device. */
+int kg_batch_poll(kg_batch* b, int32_t* done);
+/* Wait for the submitted batch and write each query's result (res[nq]). */
+int kg_batch_collect(kg_batch* b, kg_query_result* res);
+/* kg_batch_submit + kg_batch_collect. */
+int kg_batch_run(kg_batch* b, const kg_query* qs, int32_t nq, kg_query_result* res);
+
+/* ---- serving loop ----------------------------------------------------------------
+ * No counterpart in the reference (one query per process run,
+ * util/time_execute.h:22-57). Replays queries arriving at arrival_s[i] seconds
+ * (ascending) after the call starts, on columns already resident in device
+ * memory, and records each query's latency (completion - arrival) and result.
+ * KG_SERVE_PLAIN runs queries one at a time, in arrival order, through the
+ * plain entry points. KG_SERVE_BATCHED keeps up to `slots` batches in flight
+ * on their own streams (continuous batching): whenever a slot is free, the
+ * queries that have arrived are admitted in arrival order while their projected
+ * device footprint (kg_batch_footprint) fits the slot's arena and the batch
+ * holds at most max_batch queries; a query the batch engine does not support
+ * is re-run on the plain path. */
+enum kg_serve_mode { KG_SERVE_PLAIN = 0, KG_SERVE_BATCHED = 1 };
+typedef struct kg_serve_opts {
+  int32_t mode;                 /* kg_serve_mode */
+  int32_t slots;                /* batches in flight; 0 = 2 */
+  int32_t max_batch;            /* queries per batch; 0 = 128 */
+  int32_t reserved;
+  int64_t arena_bytes;          /* per slot; 0 = 1 GiB */
+} kg_serve_opts;
+typedef struct kg_serve_stats {
+  double wall_s;                /* first arrival to last completion */
+  double qps;
+  double p50_s, p99_s, max_s;   /* latency percentiles */
+  double mean_batch;            /* queries per batch (batched mode) */
+  int64_t batches;
+  int64_t fallbacks;            /* queries re-run on the plain path */
+} kg_serve_stats;
+int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const double* arrival_s, const kg_serve_opts* opts,
+             kg_query_result* res, double* latency_s, kg_serve_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,45 @@ class KgAgg(ctypes.Structure):
                 ("reserved", ctypes.c_int32)]
 
 
+# enum kg_query_kind / kg_serve_mode
+KG_Q_SCAN_AGG, KG_Q_JOIN_AGG = range(2)
+KG_SERVE_PLAIN, KG_SERVE_BATCHED = range(2)
+
+
+class KgQuery(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("table_id", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("cols", ctypes.POINTER(KgCol)), ("ncols", ctypes.c_int32), ("npreds", ctypes.c_int32),
+                ("preds", ctypes.POINTER(KgPred)), ("key_cols", ctypes.POINTER(ctypes.c_int32)),
+                ("key_card", ctypes.POINTER(ctypes.c_int32)), ("nkeys", ctypes.c_int32), ("naggs", ctypes.c_int32),
+                ("aggs", ctypes.POINTER(KgAgg)), ("probe_key_col", ctypes.c_int32),
+                ("build_key_col", ctypes.c_int32), ("build_n", ctypes.c_int64),
+                ("build_cols", ctypes.POINTER(KgCol)), ("build_ncols", ctypes.c_int32),
+                ("build_npreds", ctypes.c_int32), ("build_preds", ctypes.POINTER(KgPred)),
+                ("build_code_col", ctypes.c_int32), ("group_card", ctypes.c_int32),
+                ("load_factor", ctypes.c_double)]
+
+
+class KgQueryResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("reserved", ctypes.c_int32), ("max_groups", ctypes.c_int64),
+                ("ngroups", ctypes.c_int64), ("keys", ctypes.c_void_p), ("vals", ctypes.c_void_p),
+                ("nulls", ctypes.c_void_p)]
+
+
+class KgBatchOpts(ctypes.Structure):
+    _fields_ = [("arena_bytes", ctypes.c_int64), ("reserved", ctypes.c_int64)]
+
+
+class KgServeOpts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("arena_bytes", ctypes.c_int64)]
+
+
+class KgServeStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("qps", ctypes.c_double), ("p50_s", ctypes.c_double),
+                ("p99_s", ctypes.c_double), ("max_s", ctypes.c_double), ("mean_batch", ctypes.c_double),
+                ("batches", ctypes.c_int64), ("fallbacks", ctypes.c_int64)]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -90,6 +129,17 @@ SIGNATURES = {
     "kg_tuple_rows": (_i32, [_vp, _vp, _i32, _i32, _vp, _i64, _vp]),
     "kg_gather": (_i32, [_vp, ctypes.POINTER(KgCol), _vp, _i64, _vp, _vp]),
     "kg_group_by_i64": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _pi64]),
+    "kg_batch_create": (_i32, [_vp, ctypes.POINTER(KgBatchOpts), ctypes.POINTER(_vp)]),
+    "kg_batch_free": (_i32, [_vp]),
+    "kg_batch_footprint": (_i32, [ctypes.POINTER(KgQuery), _pi64]),
+    "kg_batch_arena_bytes": (_i64, [_vp]),
+    "kg_batch_submit": (_i32, [_vp, ctypes.POINTER(KgQuery), _i32]),
+    "kg_batch_poll": (_i32, [_vp, _pi32]),
+    "kg_batch_collect": (_i32, [_vp, ctypes.POINTER(KgQueryResult)]),
+    "kg_batch_run": (_i32, [_vp, ctypes.POINTER(KgQuery), _i32, ctypes.POINTER(KgQueryResult)]),
+    "kg_serve": (_i32, [_vp, ctypes.POINTER(KgQuery), _i32, ctypes.POINTER(ctypes.c_double),
+                        ctypes.POINTER(KgServeOpts), ctypes.POINTER(KgQueryResult),
+                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(KgServeStats)]),
 }
 
 # Helpers exported next to the ABI (not part of include/kushgpu.h's contract).

@@ -20,69 +20,9 @@
 #include <cstring>
 #include <new>
 
-#include "kg_ops.h"
+#include "kg_table.h"
 
 namespace kg {
-
-constexpr int64_t kEmpty = INT64_MIN;
-constexpr int kSlots = 8;         // slots per 64-byte bucket
-constexpr int kProbeLanes = 4;    // lanes cooperating on one bucket
-constexpr int kJBlock = 256;
-constexpr int64_t kSideCap = 1 << 20;
-
-struct Acc {
-  unsigned long long sum;
-  unsigned long long count;
-};
-
-struct TableView {
-  int64_t* keys;
-  int32_t* rows;
-  uint64_t nbuckets;
-  int32_t* side_rows;
-  unsigned long long* side_count;
-};
-
-__device__ __forceinline__ uint64_t bucket_of(int64_t key, uint64_t nbuckets) {
-  return __umul64hi(mix64(static_cast<uint64_t>(key)), nbuckets);
-}
-
-// Streamed probe-side key: a non-temporal load, so the key stream does not
-// displace the table from the caches.
-__device__ __forceinline__ bool load_key_nt(const kg_col& c, int64_t r, int64_t& key) {
-  if (c.nulls && c.nulls[r]) return false;
-  if (c.type == KG_BIGINT) {
-    key = __builtin_nontemporal_load(static_cast<const int64_t*>(c.data) + r);
-    return true;
-  }
-  if (c.type == KG_INT || c.type == KG_DATE || c.type == KG_ENUM) {
-    key = __builtin_nontemporal_load(static_cast<const int32_t*>(c.data) + r);
-    return true;
-  }
-  if (c.type == KG_REAL) {
-    const double d = __builtin_nontemporal_load(static_cast<const double*>(c.data) + r);
-    if (d != d) return false;
-    key = d == 0.0 ? 0 : __double_as_longlong(d);
-    return true;
-  }
-  key = load_int(c.type, c.data, r);
-  return true;
-}
-
-// Join key of row r as 64 bits. REAL keys compare as doubles in the reference
-// (Equal, compile/proxy/evaluate.cc:449-453): -0.0 is folded onto +0.0 and a
-// NaN key never matches, so it is treated like NULL.
-__device__ __forceinline__ bool load_key(const kg_col& c, int64_t r, int64_t& key) {
-  if (c.nulls && c.nulls[r]) return false;
-  if (c.type == KG_REAL) {
-    const double d = static_cast<const double*>(c.data)[r];
-    if (d != d) return false;
-    key = d == 0.0 ? 0 : __double_as_longlong(d);
-    return true;
-  }
-  key = load_int(c.type, c.data, r);
-  return true;
-}
 
 __global__ __launch_bounds__(kJBlock) void k_ht_build(TableView t, kg_col key, const int32_t* sel, int64_t n,
                                                        unsigned long long* inserted, int* overflow) {
@@ -451,7 +391,7 @@ __global__ __launch_bounds__(kJBlock) void k_join_group_agg(PredSet P, kg_col ke
     for (int m = 0; m < kMaxMeasures; m++)
       if (m < nm) v[m] = load_measure(M, m, r, vnull[m]);
     auto consume = [&](int32_t brow) {
-      const int g = codes[brow];
+      const int g = codes ? codes[brow] : 0;
       if (g < 0 || g >= G) return;
       atomicAdd(&lrows[g], 1ull);
       atomicMin(&lfirst[g], (static_cast<unsigned long long>(r) << 32) | static_cast<uint32_t>(brow));
@@ -883,11 +823,16 @@ extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols
   KG_REQUIRE(ctx, out_ngroups && out_keys && (naggs == 0 || (aggs && out_vals && out_nulls)), "null output pointer");
   KG_REQUIRE(ctx, naggs >= 0 && naggs <= 16, "at most 16 aggregates");
   KG_REQUIRE(ctx, probe_key_col >= 0 && probe_key_col < ncols, "bad probe key column %d", probe_key_col);
-  KG_REQUIRE(ctx, build_codes && build_codes->data, "build group codes are NULL");
-  if (build_codes->type != KG_INT && build_codes->type != KG_ENUM)
-    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "build group codes must be INT/ENUM codes");
-  if (build_codes->nulls) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "nullable build group codes");
   KG_REQUIRE(ctx, group_card > 0, "group cardinality must be positive");
+  // No code column: the ungrouped AggregateOperator over the join output.
+  KG_REQUIRE(ctx, build_codes || group_card == 1, "build group codes are NULL");
+  if (build_codes) {
+    KG_REQUIRE(ctx, build_codes->data, "build group codes have no data");
+    if (build_codes->type != KG_INT && build_codes->type != KG_ENUM)
+      return kg_fail(ctx, KG_ERR_UNSUPPORTED, "build group codes must be INT/ENUM codes");
+    if (build_codes->nulls) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "nullable build group codes");
+  }
+  const int32_t* codes_data = build_codes ? static_cast<const int32_t*>(build_codes->data) : nullptr;
   int st = check_key(ctx, &cols[probe_key_col], n);
   if (st != KG_OK) return st;
   PredSet P;
@@ -925,11 +870,11 @@ extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols
     kg_time_begin(ctx);
     if (minmax)
       hipLaunchKernelGGL(k_join_group_agg<true>, dim3(nb), dim3(kJBlock), lds_bytes, ctx->stream, P,
-                         cols[probe_key_col], ht->view(), static_cast<const int32_t*>(build_codes->data), G, M, n,
+                         cols[probe_key_col], ht->view(), codes_data, G, M, n,
                          dfin, drows, dfirst);
     else
       hipLaunchKernelGGL(k_join_group_agg<false>, dim3(nb), dim3(kJBlock), lds_bytes, ctx->stream, P,
-                         cols[probe_key_col], ht->view(), static_cast<const int32_t*>(build_codes->data), G, M, n,
+                         cols[probe_key_col], ht->view(), codes_data, G, M, n,
                          dfin, drows, dfirst);
     kg_time_end(ctx);
     KG_LAUNCH_CHECK(ctx);
@@ -955,6 +900,6 @@ extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols
   std::vector<int64_t> first64(G);
   // (probe row << 32 | build row) < 2^63 for every present group.
   for (int g = 0; g < G; g++) first64[g] = static_cast<int64_t>(firstu[g]);
-  return emit_groups(ctx, G, 1, 0, M, plan, fin, rowsv, first64, first_null, max_groups, out_ngroups, out_keys,
-                     out_vals, out_nulls);
+  return emit_groups(ctx, G, build_codes ? 1 : 0, 0, M, plan, fin, rowsv, first64, first_null, max_groups,
+                     out_ngroups, out_keys, out_vals, out_nulls);
 }

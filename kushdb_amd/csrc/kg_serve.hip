@@ -1,0 +1,249 @@
+// Serving loop: replays a stream of query arrivals against resident columns,
+// either one query at a time through the plain entry points, or with
+// continuous batching over several kg_batch slots, and records per-query
+// latency. No counterpart in the reference, which runs one query per process
+// (util/time_execute.h:22-57).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <vector>
+
+#include "kg_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+// Grow-only device buffer for build-side selections of the plain path.
+struct Scratch {
+  int32_t* sel = nullptr;
+  int64_t cap = 0;
+  ~Scratch() {
+    if (sel) (void)hipFree(sel);
+  }
+  int reserve(kg_ctx* ctx, int64_t n) {
+    if (n <= cap) return KG_OK;
+    if (sel) (void)hipFree(sel);
+    sel = nullptr;
+    cap = 0;
+    if (hipMalloc(reinterpret_cast<void**>(&sel), static_cast<size_t>(std::max<int64_t>(n, 1)) * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      return kg_fail(ctx, KG_ERR_OOM, "selection buffer of %lld rows", (long long)n);
+    }
+    cap = n;
+    return KG_OK;
+  }
+};
+
+// One query through the plain entry points (what a host plan would call).
+int run_plain(kg_ctx* ctx, const kg_query& q, kg_query_result& R, Scratch& S) {
+  R.ngroups = 0;
+  if (q.kind == KG_Q_SCAN_AGG)
+    return kg_scan_group_agg(ctx, q.n, q.cols, q.ncols, q.preds, q.npreds, q.key_cols, q.key_card, q.nkeys, q.aggs,
+                             q.naggs, R.max_groups, &R.ngroups, R.keys, R.vals, R.nulls);
+  if (q.kind != KG_Q_JOIN_AGG) return kg_fail(ctx, KG_ERR_INVALID, "bad query kind %d", q.kind);
+  KG_REQUIRE(ctx, q.build_cols && q.build_key_col >= 0 && q.build_key_col < q.build_ncols, "bad build key column");
+  const int32_t* sel = nullptr;
+  int64_t nb = q.build_n;
+  int st;
+  if (q.build_npreds > 0) {
+    if ((st = S.reserve(ctx, q.build_n)) != KG_OK) return st;
+    if ((st = kg_scan_select(ctx, q.build_n, q.build_cols, q.build_ncols, q.build_preds, q.build_npreds, 1, S.sel,
+                             &nb)) != KG_OK)
+      return st;
+    sel = S.sel;
+  }
+  kg_hash_table* ht = nullptr;
+  if ((st = kg_ht_create(ctx, nb, q.load_factor, &ht)) != KG_OK) return st;
+  st = kg_ht_build(ctx, ht, &q.build_cols[q.build_key_col], sel, nb);
+  if (st == KG_OK) {
+    const kg_col* codes = q.build_code_col >= 0 ? &q.build_cols[q.build_code_col] : nullptr;
+    st = kg_scan_join_group_agg(ctx, q.n, q.cols, q.ncols, q.preds, q.npreds, ht, q.probe_key_col, codes,
+                                codes ? q.group_card : 1, q.aggs, q.naggs, R.max_groups, &R.ngroups, R.keys, R.vals,
+                                R.nulls);
+  }
+  kg_ht_free(ctx, ht);
+  return st;
+}
+
+double percentile(std::vector<double> v, double p) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  // Nearest rank.
+  size_t k = static_cast<size_t>(std::ceil(p * static_cast<double>(v.size())));
+  k = std::min(std::max<size_t>(k, 1), v.size());
+  return v[k - 1];
+}
+
+struct Slot {
+  kg_batch* b = nullptr;
+  bool busy = false;
+  std::vector<int32_t> ids;
+  std::vector<kg_query> qbuf;
+  std::vector<kg_query_result> rbuf;
+};
+
+}  // namespace
+
+extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const double* arrival_s,
+                        const kg_serve_opts* opts, kg_query_result* res, double* latency_s, kg_serve_stats* stats) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, nq >= 0 && (nq == 0 || (qs && arrival_s && res && latency_s)), "null argument");
+  KG_REQUIRE(ctx, opts, "options are NULL");
+  for (int i = 1; i < nq; i++) KG_REQUIRE(ctx, arrival_s[i] >= arrival_s[i - 1], "arrivals must be ascending");
+  const bool batched = opts->mode == KG_SERVE_BATCHED;
+  KG_REQUIRE(ctx, batched || opts->mode == KG_SERVE_PLAIN, "bad serving mode %d", opts->mode);
+  const int nslots = opts->slots > 0 ? opts->slots : 2;
+  const int max_batch = opts->max_batch > 0 ? std::min(opts->max_batch, 256) : 128;
+  KG_REQUIRE(ctx, nslots <= 8, "at most 8 batch slots");
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  Scratch S;
+  std::vector<Slot> slots;
+  kg_serve_stats st_{};
+  auto cleanup = [&]() {
+    for (Slot& s : slots)
+      if (s.b) kg_batch_free(s.b);
+  };
+  if (batched) {
+    kg_batch_opts bo{};
+    bo.arena_bytes = opts->arena_bytes;
+    slots.resize(nslots);
+    for (Slot& s : slots) {
+      const int st = kg_batch_create(ctx, &bo, &s.b);
+      if (st != KG_OK) {
+        cleanup();
+        return st;
+      }
+    }
+  }
+  // Pre-fault the plain path's code and the batch graphs before the clock starts.
+  KG_HIP_CHECK(ctx, hipDeviceSynchronize());
+  std::vector<double> done_at(nq, -1.0);
+  std::deque<int32_t> pending;
+  int32_t next = 0, ndone = 0;
+  int first_err = KG_OK;
+  const Clock::time_point t0 = Clock::now();
+  auto finish = [&](int32_t i) {
+    done_at[i] = seconds_since(t0);
+    latency_s[i] = done_at[i] - arrival_s[i];
+    ndone++;
+  };
+  auto plain = [&](int32_t i) {
+    res[i].status = run_plain(ctx, qs[i], res[i], S);
+    if (res[i].status != KG_OK && first_err == KG_OK) first_err = res[i].status;
+    finish(i);
+  };
+  while (ndone < nq) {
+    const double now = seconds_since(t0);
+    while (next < nq && arrival_s[next] <= now) pending.push_back(next++);
+    bool progressed = false;
+    if (!batched) {
+      if (!pending.empty()) {
+        const int32_t i = pending.front();
+        pending.pop_front();
+        plain(i);
+        progressed = true;
+      }
+    } else {
+      // Completions.
+      for (Slot& s : slots) {
+        if (!s.busy) continue;
+        int32_t done = 0;
+        int st = kg_batch_poll(s.b, &done);
+        if (st != KG_OK) {
+          cleanup();
+          return st;
+        }
+        if (!done) continue;
+        if ((st = kg_batch_collect(s.b, s.rbuf.data())) != KG_OK) {
+          cleanup();
+          return st;
+        }
+        s.busy = false;
+        progressed = true;
+        for (size_t k = 0; k < s.ids.size(); k++) {
+          const int32_t i = s.ids[k];
+          res[i].status = s.rbuf[k].status;
+          res[i].ngroups = s.rbuf[k].ngroups;
+          if (res[i].status == KG_ERR_UNSUPPORTED) {
+            st_.fallbacks++;
+            plain(i);
+          } else {
+            if (res[i].status != KG_OK && first_err == KG_OK) first_err = res[i].status;
+            finish(i);
+          }
+        }
+      }
+      // Admission into free slots.
+      for (Slot& s : slots) {
+        if (s.busy || pending.empty()) continue;
+        s.ids.clear();
+        s.qbuf.clear();
+        s.rbuf.clear();
+        const int64_t arena = kg_batch_arena_bytes(s.b);
+        int64_t used = 0;
+        while (!pending.empty() && static_cast<int>(s.ids.size()) < max_batch) {
+          const int32_t i = pending.front();
+          int64_t fp = 0;
+          kg_batch_footprint(&qs[i], &fp);
+          if (fp > arena) {
+            // Never fits a slot: run it alone on the plain path.
+            pending.pop_front();
+            st_.fallbacks++;
+            plain(i);
+            continue;
+          }
+          if (used + fp > arena) break;
+          used += fp;
+          pending.pop_front();
+          s.ids.push_back(i);
+          s.qbuf.push_back(qs[i]);
+          s.rbuf.push_back(res[i]);
+        }
+        if (s.ids.empty()) continue;
+        const int st = kg_batch_submit(s.b, s.qbuf.data(), static_cast<int32_t>(s.ids.size()));
+        if (st != KG_OK) {
+          cleanup();
+          return st;
+        }
+        s.busy = true;
+        st_.batches++;
+        st_.mean_batch += static_cast<double>(s.ids.size());
+        progressed = true;
+      }
+    }
+    if (!progressed) {
+      bool busy = false;
+      for (const Slot& s : slots) busy |= s.busy;
+      if (!busy && pending.empty() && next < nq) {
+        // Idle until the next arrival (sleep most of the gap, spin the rest).
+        const double gap = arrival_s[next] - seconds_since(t0);
+        if (gap > 200e-6) std::this_thread::sleep_for(std::chrono::duration<double>(gap - 100e-6));
+      } else {
+        std::this_thread::yield();
+      }
+    }
+  }
+  cleanup();
+  std::vector<double> lat(latency_s, latency_s + nq);
+  double last = 0;
+  for (double d : done_at) last = std::max(last, d);
+  st_.wall_s = nq > 0 ? last - arrival_s[0] : 0;
+  st_.qps = st_.wall_s > 0 ? nq / st_.wall_s : 0;
+  st_.p50_s = percentile(lat, 0.50);
+  st_.p99_s = percentile(lat, 0.99);
+  st_.max_s = percentile(lat, 1.0);
+  if (st_.batches > 0) st_.mean_batch /= static_cast<double>(st_.batches);
+  if (stats) *stats = st_;
+  (void)first_err;
+  return KG_OK;
+}

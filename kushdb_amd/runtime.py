@@ -231,9 +231,9 @@ class HashTable:
         vals = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int64)
         nulls = np.zeros((max_groups, max(len(aggs), 1)), dtype=np.int8)
         ng = ctypes.c_int64()
-        bc = build_codes.desc()
+        bc = ctypes.byref(build_codes.desc()) if build_codes is not None else None
         ctx.check(ctx.lib.kg_scan_join_group_agg(
-            ctx.h, n, carr, len(cols), parr, len(preds), self.h, probe_key_col, ctypes.byref(bc), group_card,
+            ctx.h, n, carr, len(cols), parr, len(preds), self.h, probe_key_col, bc, group_card,
             _aggs(aggs), len(aggs), max_groups, ctypes.byref(ng), codes.ctypes.data_as(abi._pi32),
             vals.ctypes.data_as(abi._pi64), nulls.ctypes.data_as(abi._pi8)))
         g = ng.value
@@ -462,3 +462,136 @@ def gather(ctx, col, rows):
     d = col.desc()
     ctx.check(ctx.lib.kg_gather(ctx.h, ctypes.byref(d), _ptr(rows), n, _ptr(out), _ptr(nl)))
     return Column(col.type, out[:n], nl[:n] if nl is not None else None)
+
+
+# -- batched execution and serving -----------------------------------------------
+
+class Query:
+    """One batched pipeline (kg_query). Build with Query.scan_agg / Query.join_agg."""
+
+    def __init__(self, kind, cols, preds, aggs, keys=(), table_id=0, probe_key=-1, build_cols=(), build_key=-1,
+                 build_preds=(), build_codes=-1, group_card=1, load_factor=0.0):
+        self.kind = kind
+        self.cols = list(cols)
+        self.build_cols = list(build_cols)
+        self.naggs = len(aggs)
+        if kind == abi.KG_Q_SCAN_AGG:
+            self.nkeys_out = len(keys)
+        else:
+            self.nkeys_out = 1 if build_codes >= 0 else 0
+        carr = (KgCol * max(len(cols), 1))(*[c.desc() for c in cols])
+        parr = (KgPred * max(len(preds), 1))(*[_pred(cols, p) for p in preds])
+        kc = (ctypes.c_int32 * max(len(keys), 1))(*[k[0] for k in keys])
+        kd = (ctypes.c_int32 * max(len(keys), 1))(*[k[1] for k in keys])
+        aarr = _aggs(aggs)
+        barr = (KgCol * max(len(build_cols), 1))(*[c.desc() for c in build_cols])
+        bparr = (KgPred * max(len(build_preds), 1))(*[_pred(build_cols, p) for p in build_preds])
+        self._keep = (carr, parr, kc, kd, aarr, barr, bparr)
+        n = len(cols[0]) if cols else 0
+        bn = len(build_cols[0]) if build_cols else 0
+        self.q = abi.KgQuery(kind, table_id, n, carr, len(cols), len(preds), parr, kc, kd, len(keys), len(aggs),
+                             aarr, probe_key, build_key, bn, barr, len(build_cols), len(build_preds), bparr,
+                             build_codes, group_card, load_factor)
+
+    @staticmethod
+    def scan_agg(cols, preds, keys, aggs, table_id=0):
+        """ScanSelect -> GroupByAggregate on dense keys [(col, card)] (as Context.scan_group_agg)."""
+        return Query(abi.KG_Q_SCAN_AGG, cols, preds, aggs, keys=keys, table_id=table_id)
+
+    @staticmethod
+    def join_agg(cols, preds, probe_key, build_cols, build_key, aggs, build_preds=(), build_codes=-1, group_card=1,
+                 table_id=0, load_factor=0.0):
+        """ScanSelect(build) -> build; ScanSelect(probe) -> probe -> GROUP BY
+        build_cols[build_codes] (or one group when build_codes < 0)."""
+        return Query(abi.KG_Q_JOIN_AGG, cols, preds, aggs, table_id=table_id, probe_key=probe_key,
+                     build_cols=build_cols, build_key=build_key, build_preds=build_preds, build_codes=build_codes,
+                     group_card=group_card if build_codes >= 0 else 1, load_factor=load_factor)
+
+    def footprint(self):
+        b = ctypes.c_int64()
+        abi.load().kg_batch_footprint(ctypes.byref(self.q), ctypes.byref(b))
+        return b.value
+
+
+def _result_buffers(queries, max_groups):
+    res = (abi.KgQueryResult * max(len(queries), 1))()
+    bufs = []
+    for i, q in enumerate(queries):
+        keys = np.zeros((max_groups, max(q.nkeys_out, 1)), dtype=np.int32)
+        vals = np.zeros((max_groups, max(q.naggs, 1)), dtype=np.int64)
+        nulls = np.zeros((max_groups, max(q.naggs, 1)), dtype=np.int8)
+        res[i] = abi.KgQueryResult(0, 0, max_groups, 0, keys.ctypes.data, vals.ctypes.data, nulls.ctypes.data)
+        bufs.append((keys, vals, nulls))
+    return res, bufs
+
+
+def _results(queries, res, bufs):
+    out = []
+    for i, q in enumerate(queries):
+        r = res[i]
+        keys, vals, nulls = bufs[i]
+        g = r.ngroups if r.status == abi.KG_OK else 0
+        out.append((r.status, keys[:g, : q.nkeys_out], vals[:g, : q.naggs], nulls[:g, : q.naggs]))
+    return out
+
+
+class Batch:
+    """A batch slot (kg_batch): its own stream, device arena and captured graph."""
+
+    def __init__(self, ctx, arena_bytes=0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        opts = abi.KgBatchOpts(arena_bytes, 0)
+        ctx.check(ctx.lib.kg_batch_create(ctx.h, ctypes.byref(opts), ctypes.byref(h)))
+        self.h = h
+        self._inflight = None
+
+    def submit(self, queries, max_groups=4096):
+        qarr = (abi.KgQuery * max(len(queries), 1))(*[q.q for q in queries])
+        self.ctx.check(self.ctx.lib.kg_batch_submit(self.h, qarr, len(queries)))
+        self._inflight = (list(queries), qarr, max_groups)
+
+    def poll(self):
+        d = ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.kg_batch_poll(self.h, ctypes.byref(d)))
+        return bool(d.value)
+
+    def collect(self):
+        """[(status, codes, values, nulls)] per query, in submission order."""
+        queries, _, max_groups = self._inflight
+        res, bufs = _result_buffers(queries, max_groups)
+        self.ctx.check(self.ctx.lib.kg_batch_collect(self.h, res))
+        self._inflight = None
+        return _results(queries, res, bufs)
+
+    def run(self, queries, max_groups=4096):
+        self.submit(queries, max_groups)
+        return self.collect()
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.kg_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def serve(ctx, queries, arrivals, mode, slots=2, max_batch=128, arena_bytes=0, max_groups=64):
+    """Replays `queries` arriving at `arrivals` seconds (ascending). Returns
+    (results as Batch.collect, latencies in s, stats dict)."""
+    n = len(queries)
+    qarr = (abi.KgQuery * max(n, 1))(*[q.q for q in queries])
+    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    lat = np.zeros(max(n, 1), dtype=np.float64)
+    res, bufs = _result_buffers(queries, max_groups)
+    opts = abi.KgServeOpts(mode, slots, max_batch, 0, arena_bytes)
+    st = abi.KgServeStats()
+    ctx.check(ctx.lib.kg_serve(ctx.h, qarr, n, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               ctypes.byref(opts), res, lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               ctypes.byref(st)))
+    stats = {k: getattr(st, k) for k, _ in abi.KgServeStats._fields_}
+    return _results(queries, res, bufs), lat[:n], stats

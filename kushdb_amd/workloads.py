@@ -173,3 +173,128 @@ def skinner_tables(scale=1.0, theta=0.0, device="cuda", seed=SEED + 4):
             [], [],
             [("int", "v", abi.KG_LT, 50)]]
     return tabs, spec, htabs
+
+
+# -- batched / served mix -----------------------------------------------------------
+# Resident tables: fact tables of uneven size with the columns below, one 10k-row
+# dimension (pk dense, cat over 25 codes) and one build table of int64 keys (a
+# permutation of 0..nbuild-1). Fact k64 keys are uniform over 0..1.1*nbuild-1
+# (10% misses).
+SERVE_FACT_SIZES = (10_000, 100_000, 1_000_000, 4_000_000, 16_000_000, 50_000_000)
+SERVE_BUILD_ROWS = 1 << 20
+FACT_COLS = ("day", "g1", "g2", "fk", "m1", "m2", "m3", "m4", "i1", "i2", "i3", "k64")
+SERVE_TEMPLATES = ("c1_scan_agg", "c1_small", "two_filter", "two_filter_ungrouped", "c0_star", "star_dim_filter",
+                   "c2_join", "selective_join")
+
+
+def serve_tables(fact_sizes=SERVE_FACT_SIZES, ndim=10_000, nbuild=SERVE_BUILD_ROWS, device="cuda", seed=SEED + 5):
+    facts = []
+    for j, n in enumerate(fact_sizes):
+        g = _gen(device, seed + 16 * j)
+        t = {
+            "day": torch.randint(DAY0, DAY0 + NDAYS, (n,), generator=g, device=device, dtype=torch.int32),
+            "g1": torch.randint(0, 3, (n,), generator=g, device=device, dtype=torch.int32),
+            "g2": torch.randint(0, 2, (n,), generator=g, device=device, dtype=torch.int32),
+            "fk": torch.randint(0, ndim, (n,), generator=g, device=device, dtype=torch.int32),
+            "m1": 1.0 + torch.rand((n,), generator=g, device=device, dtype=torch.float64) * (1e5 - 1.0),
+        }
+        for i in range(2, 5):
+            t[f"m{i}"] = torch.rand((n,), generator=g, device=device, dtype=torch.float64) * 1e5
+        for i in range(1, 4):
+            t[f"i{i}"] = torch.randint(1, 51, (n,), generator=g, device=device, dtype=torch.int32)
+        t["k64"] = torch.randint(0, int(nbuild * 1.1), (n,), generator=g, device=device, dtype=torch.int64)
+        facts.append(t)
+    g = _gen(device, seed + 7)
+    dim = {"pk": torch.arange(ndim, device=device, dtype=torch.int32),
+           "cat": torch.randint(0, 25, (ndim,), generator=g, device=device, dtype=torch.int32)}
+    build = {"key": torch.randperm(nbuild, generator=g, device=device).to(torch.int64),
+             "payload": torch.randint(0, 1 << 30, (nbuild,), generator=g, device=device, dtype=torch.int32)}
+    return {"facts": facts, "dim": dim, "build": build}
+
+
+def serve_queries(nq, fact_sizes=SERVE_FACT_SIZES, nbuild=SERVE_BUILD_ROWS, seed=SEED + 6, templates=None):
+    """nq query specs: template, fact table index and parameters. Column
+    indices refer to FACT_COLS, the dimension's (pk, cat) and the build
+    table's (key, payload). Templates and tables are drawn uniformly."""
+    import numpy as np
+    from . import abi
+    rng = np.random.RandomState(seed & 0x7FFFFFFF)
+    names = list(templates or SERVE_TEMPLATES)
+    S, A, C = abi.KG_AGG_SUM, abi.KG_AGG_AVG, abi.KG_AGG_COUNT
+    D = DAY0
+    specs = []
+    for _ in range(nq):
+        tpl = names[rng.randint(len(names))]
+        table = int(rng.randint(len(fact_sizes)))
+        s = {"template": tpl, "table": table, "kind": "scan", "keys": [], "preds": []}
+        if tpl == "c1_scan_agg":
+            s["preds"] = [(0, abi.KG_LEQ, D + int(NDAYS * rng.uniform(0.8, 1.0)))]
+            s["keys"] = [(1, 3), (2, 2)]
+            s["aggs"] = [(S, c) for c in range(4, 11)] + [(A, c) for c in range(4, 11)] + [(C, -1)]
+        elif tpl == "c1_small":
+            s["preds"] = [(0, abi.KG_LEQ, D + int(NDAYS * rng.uniform(0.3, 1.0)))]
+            s["keys"] = [(1, 3)]
+            s["aggs"] = [(S, 4), (A, 9), (C, -1)]
+        elif tpl == "two_filter":
+            d0 = D + int(rng.randint(0, int(NDAYS * 0.7)))
+            s["preds"] = [(0, abi.KG_GEQ, d0), (0, abi.KG_LT, d0 + int(NDAYS * rng.uniform(0.1, 0.3))),
+                          (8, abi.KG_LEQ, int(rng.randint(5, 51)))]
+            s["keys"] = [(2, 2)]
+            s["aggs"] = [(S, 5), (S, 10), (C, -1)]
+        elif tpl == "two_filter_ungrouped":
+            s["preds"] = [(0, abi.KG_GEQ, D + int(rng.randint(0, int(NDAYS * 0.9)))),
+                          (1, abi.KG_EQ, int(rng.randint(0, 3)))]
+            s["aggs"] = [(S, 6), (A, 7), (C, -1)]
+        elif tpl in ("c0_star", "star_dim_filter", "selective_join"):
+            s.update(kind="join", build="dim", probe_key=3, build_key=0, build_codes=1, group_card=25,
+                     build_preds=[])
+            if tpl == "c0_star":
+                d0 = D + int(rng.randint(0, NDAYS - 767))
+                s["preds"] = [(0, abi.KG_GEQ, d0), (0, abi.KG_LT, d0 + 767)]
+                s["aggs"] = [(S, 4, 8), (C, -1)]          # SUM(price * qty), COUNT(*)
+            elif tpl == "star_dim_filter":
+                d0 = D + int(rng.randint(0, NDAYS - 767))
+                s["preds"] = [(0, abi.KG_GEQ, d0), (0, abi.KG_LT, d0 + 767)]
+                s["build_preds"] = [(1, abi.KG_LT, int(rng.randint(5, 26)))]
+                s["aggs"] = [(S, 5), (C, -1)]
+            else:
+                d0 = D + int(rng.randint(0, int(NDAYS * 0.9)))
+                w = max(1, int(NDAYS * rng.uniform(0.01, 0.1)))
+                s["preds"] = [(0, abi.KG_GEQ, d0), (0, abi.KG_LT, d0 + w), (9, abi.KG_LT, int(rng.randint(10, 51)))]
+                s["aggs"] = [(C, -1), (S, 8)]
+        elif tpl == "c2_join":
+            lo = int(rng.randint(0, nbuild // 2))
+            hi = lo + int(nbuild * rng.uniform(0.1, 0.5))
+            s.update(kind="join", build="build", probe_key=11, build_key=0, build_codes=-1, group_card=1,
+                     build_preds=[(0, abi.KG_GEQ, lo), (0, abi.KG_LT, hi)])
+            s["preds"] = [(0, abi.KG_GEQ, D + int(rng.randint(0, int(NDAYS * 0.5))))]
+            s["aggs"] = [(C, -1), (S, 10)]
+        else:
+            raise ValueError(tpl)
+        specs.append(s)
+    return specs
+
+
+def serve_columns(tables):
+    """Column objects of serve_tables: (facts [FACT_COLS order], dim, build)."""
+    from . import abi
+    from .runtime import Column
+    types = {"day": abi.KG_DATE, "g1": abi.KG_ENUM, "g2": abi.KG_ENUM, "fk": abi.KG_INT, "m1": abi.KG_REAL,
+             "m2": abi.KG_REAL, "m3": abi.KG_REAL, "m4": abi.KG_REAL, "i1": abi.KG_INT, "i2": abi.KG_INT,
+             "i3": abi.KG_INT, "k64": abi.KG_BIGINT}
+    facts = [[Column(types[c], t[c]) for c in FACT_COLS] for t in tables["facts"]]
+    dim = [Column(abi.KG_INT, tables["dim"]["pk"]), Column(abi.KG_ENUM, tables["dim"]["cat"])]
+    build = [Column(abi.KG_BIGINT, tables["build"]["key"]), Column(abi.KG_INT, tables["build"]["payload"])]
+    return facts, dim, build
+
+
+def make_query(spec, facts, dim, build, load_factor=0.0):
+    """runtime.Query for a spec of serve_queries over serve_columns."""
+    from .runtime import Query
+    cols = facts[spec["table"]]
+    if spec["kind"] == "scan":
+        return Query.scan_agg(cols, spec["preds"], spec["keys"], spec["aggs"], table_id=spec["table"])
+    bcols = dim if spec["build"] == "dim" else build
+    return Query.join_agg(cols, spec["preds"], spec["probe_key"], bcols, spec["build_key"], spec["aggs"],
+                          build_preds=spec["build_preds"], build_codes=spec["build_codes"],
+                          group_card=spec["group_card"], table_id=spec["table"], load_factor=load_factor)

@@ -1,0 +1,132 @@
+"""Batched execution (kg_batch_*) and the serving loop (kg_serve) on the HIP
+path: every query of a mixed batch against the oracle's composed operators,
+each query's result alone equal to its result in the mix, repeated submits on
+one slot, unsupported queries reported per query, and both serving modes
+returning the same results."""
+import numpy as np
+import pytest
+import torch
+
+from kushdb_amd import abi
+from kushdb_amd.runtime import Batch, Context, Query, serve
+from kushdb_amd.workloads import make_query, serve_columns, serve_queries, serve_tables
+from serve_oracle import assert_same, host_tables, oracle_spec
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (3_000, 40_000, 250_000)
+NBUILD = 100_000
+
+
+@pytest.fixture(scope="module")
+def env():
+    ctx = Context(0)
+    tables = serve_tables(SIZES, nbuild=NBUILD, device="cuda:0")
+    torch.cuda.synchronize()
+    cols = serve_columns(tables)
+    return ctx, tables, cols, host_tables(tables)
+
+
+def _queries(cols, specs):
+    facts, dim, build = cols
+    return [make_query(s, facts, dim, build) for s in specs]
+
+
+def test_batch_mix_vs_oracle(env):
+    ctx, _, cols, H = env
+    specs = serve_queries(48, SIZES, NBUILD, seed=11)
+    print("batch mix of", len(specs), flush=True)
+    b = Batch(ctx, arena_bytes=256 << 20)
+    got = b.run(_queries(cols, specs), max_groups=64)
+    for s, g in zip(specs, got):
+        assert_same(s, g, oracle_spec(s, H), what="mix")
+    b.free()
+
+
+def test_alone_equals_mix(env):
+    ctx, _, cols, _ = env
+    specs = serve_queries(32, SIZES, NBUILD, seed=12)
+    print("alone vs mix", flush=True)
+    b = Batch(ctx, arena_bytes=256 << 20)
+    mix = b.run(_queries(cols, specs), max_groups=64)
+    for s, m in zip(specs, mix):
+        alone = b.run(_queries(cols, [s]), max_groups=64)[0]
+        assert_same(s, alone, (m[1], m[2], m[3]), what="alone")
+    b.free()
+
+
+def test_repeated_submits_and_empty(env):
+    ctx, _, cols, H = env
+    b = Batch(ctx, arena_bytes=64 << 20)
+    assert b.run([], max_groups=8) == []
+    specs = serve_queries(8, SIZES, NBUILD, seed=13)
+    first = b.run(_queries(cols, specs), max_groups=64)
+    for _ in range(3):
+        again = b.run(_queries(cols, specs), max_groups=64)
+        for s, x, y in zip(specs, first, again):
+            assert_same(s, y, (x[1], x[2], x[3]), what="repeat")
+    # all rows filtered out; an empty probe side
+    facts, dim, _ = cols
+    q_none = Query.scan_agg(facts[0], [(0, abi.KG_LT, 0)], [(1, 3)], [(abi.KG_AGG_COUNT, -1)], table_id=0)
+    q_nojoin = Query.join_agg(facts[1], [(0, abi.KG_LT, 0)], 3, dim, 0, [(abi.KG_AGG_COUNT, -1)], table_id=1)
+    r = b.run([q_none, q_nojoin], max_groups=8)
+    assert r[0][0] == abi.KG_OK and len(r[0][2]) == 0
+    assert r[1][0] == abi.KG_OK and len(r[1][2]) == 0
+    b.free()
+
+
+def test_unsupported_and_bad_codes_per_query(env):
+    ctx, _, cols, H = env
+    facts, dim, _ = cols
+    b = Batch(ctx, arena_bytes=64 << 20)
+    ok_spec = serve_queries(1, SIZES, NBUILD, seed=14, templates=["c1_small"])[0]
+    q_ok = _queries(cols, [ok_spec])[0]
+    q_min = Query.scan_agg(facts[0], [], [(1, 3)], [(abi.KG_AGG_MIN, 4)], table_id=0)
+    # fk codes reach 9999: a group cardinality of 10 is violated
+    q_bad = Query.scan_agg(facts[0], [], [(3, 10)], [(abi.KG_AGG_COUNT, -1)], table_id=0)
+    # the same on the register path (at most 8 groups)
+    q_bad8 = Query.scan_agg(facts[2], [(0, abi.KG_GEQ, 0)], [(3, 8)], [(abi.KG_AGG_SUM, 4)], table_id=2)
+    r = b.run([q_min, q_ok, q_bad, q_bad8], max_groups=64)
+    assert r[0][0] == abi.KG_ERR_UNSUPPORTED
+    assert r[2][0] == abi.KG_ERR_INVALID
+    assert r[3][0] == abi.KG_ERR_INVALID
+    assert_same(ok_spec, r[1], oracle_spec(ok_spec, H), what="ok next to failures")
+    b.free()
+
+
+def test_serve_plain_and_batched_agree(env):
+    ctx, _, cols, H = env
+    specs = serve_queries(64, SIZES, NBUILD, seed=15)
+    rng = np.random.RandomState(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / 4000.0, size=len(specs)))
+    print("serve plain", flush=True)
+    plain, lat_p, st_p = serve(ctx, _queries(cols, specs), arrivals, abi.KG_SERVE_PLAIN, max_groups=64)
+    print("serve batched", st_p, flush=True)
+    batched, lat_b, st_b = serve(ctx, _queries(cols, specs), arrivals, abi.KG_SERVE_BATCHED, slots=2, max_batch=32,
+                                 arena_bytes=128 << 20, max_groups=64)
+    print("served", st_b, flush=True)
+    for s, p, q in zip(specs, plain, batched):
+        assert_same(s, p, oracle_spec(s, H), what="plain")
+        assert_same(s, q, (p[1], p[2], p[3]), what="batched vs plain")
+    assert (lat_p > 0).all() and (lat_b > 0).all()
+    assert st_b["batches"] >= 1 and st_b["fallbacks"] == 0
+    assert st_p["p99_s"] >= st_p["p50_s"] > 0
+
+
+def test_unaligned_columns_take_the_generic_path(env):
+    """Columns that start off a 16-byte boundary cannot use vector loads; the
+    same query over row-shifted views must still match the oracle."""
+    ctx, tables, cols, H = env
+    from kushdb_amd.runtime import Column
+    from kushdb_amd.workloads import FACT_COLS
+    facts, dim, build = cols
+    t = tables["facts"][2]
+    shifted = [Column(c.type, c.data[1:]) for c in facts[2]]
+    spec = serve_queries(1, SIZES, NBUILD, seed=16, templates=["c1_scan_agg"])[0]
+    spec["table"] = 2
+    q = Query.scan_agg(shifted, spec["preds"], spec["keys"], spec["aggs"], table_id=7)
+    b = Batch(ctx, arena_bytes=64 << 20)
+    got = b.run([q], max_groups=64)[0]
+    Hs = {"facts": [{k: v[1:] for k, v in H["facts"][2].items()}] * 3, "dim": H["dim"], "build": H["build"]}
+    assert_same(spec, got, oracle_spec(spec, Hs), what="unaligned")
+    b.free()
