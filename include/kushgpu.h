@@ -416,8 +416,17 @@ int64_t kg_batch_arena_bytes(const kg_batch* b);
 int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq);
 /* *done = 1 when the submitted batch has finished on the device. */
 int kg_batch_poll(kg_batch* b, int32_t* done);
+/* Queries complete individually while the batch runs: the device publishes
+ * query i's result as soon as its last work item is done (tables are scanned
+ * smallest first). *ready = 1 once query i (submission order) can be
+ * collected with kg_batch_collect_query. */
+int kg_batch_query_ready(kg_batch* b, int32_t i, int32_t* ready);
+int kg_batch_collect_query(kg_batch* b, int32_t i, kg_query_result* res);
 /* Wait for the submitted batch and write each query's result (res[nq]). */
 int kg_batch_collect(kg_batch* b, kg_query_result* res);
+/* Wait for the submitted batch without writing results (after collecting
+ * every query with kg_batch_collect_query); the slot can take a new batch. */
+int kg_batch_release(kg_batch* b);
 /* kg_batch_submit + kg_batch_collect. */
 int kg_batch_run(kg_batch* b, const kg_query* qs, int32_t nq, kg_query_result* res);
 
@@ -430,16 +439,20 @@ int kg_batch_run(kg_batch* b, const kg_query* qs, int32_t nq, kg_query_result* r
  * plain entry points. KG_SERVE_BATCHED keeps up to `slots` batches in flight
  * on their own streams (continuous batching): whenever a slot is free, the
  * queries that have arrived are admitted in arrival order while their projected
- * device footprint (kg_batch_footprint) fits the slot's arena and the batch
- * holds at most max_batch queries; a query the batch engine does not support
- * is re-run on the plain path. */
+ * device footprint (kg_batch_footprint) fits the slot's arena, the batch holds
+ * at most max_batch queries and its rows (scanned + build) stay within
+ * max_batch_rows (the first query is always admitted), which bounds how long a
+ * batch keeps its slot; a query the batch engine does not support
+ * is re-run on the plain path. Each query completes as soon as its own work
+ * is done (kg_batch_query_ready), not at the end of its batch. */
 enum kg_serve_mode { KG_SERVE_PLAIN = 0, KG_SERVE_BATCHED = 1 };
 typedef struct kg_serve_opts {
   int32_t mode;                 /* kg_serve_mode */
-  int32_t slots;                /* batches in flight; 0 = 2 */
+  int32_t slots;                /* batches in flight; 0 = 3 */
   int32_t max_batch;            /* queries per batch; 0 = 128 */
   int32_t reserved;
   int64_t arena_bytes;          /* per slot; 0 = 1 GiB */
+  int64_t max_batch_rows;       /* projected work cap of a batch: scanned + build rows; 0 = none */
 } kg_serve_opts;
 typedef struct kg_serve_stats {
   double wall_s;                /* first arrival to last completion */

@@ -72,7 +72,7 @@ class KgBatchOpts(ctypes.Structure):
 
 class KgServeOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("arena_bytes", ctypes.c_int64)]
+                ("reserved", ctypes.c_int32), ("arena_bytes", ctypes.c_int64), ("max_batch_rows", ctypes.c_int64)]
 
 
 class KgServeStats(ctypes.Structure):
@@ -136,6 +136,9 @@ SIGNATURES = {
     "kg_batch_submit": (_i32, [_vp, ctypes.POINTER(KgQuery), _i32]),
     "kg_batch_poll": (_i32, [_vp, _pi32]),
     "kg_batch_collect": (_i32, [_vp, ctypes.POINTER(KgQueryResult)]),
+    "kg_batch_query_ready": (_i32, [_vp, _i32, _pi32]),
+    "kg_batch_collect_query": (_i32, [_vp, _i32, ctypes.POINTER(KgQueryResult)]),
+    "kg_batch_release": (_i32, [_vp]),
     "kg_batch_run": (_i32, [_vp, ctypes.POINTER(KgQuery), _i32, ctypes.POINTER(KgQueryResult)]),
     "kg_serve": (_i32, [_vp, ctypes.POINTER(KgQuery), _i32, ctypes.POINTER(ctypes.c_double),
                         ctypes.POINTER(KgServeOpts), ctypes.POINTER(KgQueryResult),

@@ -50,7 +50,7 @@ constexpr int kBRegG = 8;             // register path: at most this many groups
 constexpr int kBLdsWords = 2048;      // LDS path: 8-byte words of the item table
 constexpr int kBFastPreds = 4;
 constexpr int kBMaxAggs = 16;
-constexpr int kBMaxFills = 3 * kBMaxQueries + 8;
+constexpr int kBMaxFills = 4 * kBMaxQueries + 8;
 
 enum BFlag : uint32_t { kFlagEmptyKey = 1u, kFlagBadCode = 2u, kFlagOverflow = 4u };
 
@@ -92,7 +92,11 @@ struct BQuery {
   uint64_t* rows;
   uint64_t* first;
   uint32_t* flag;
-  char* res;            // result area (copied back)
+  char* res;            // result area (pinned host memory, written by the fold)
+  unsigned long long* rem;  // scan/probe items not yet finished
+  int32_t* done;        // host-visible completion word, set to `seq` after the fold
+  int32_t seq;          // batch sequence number
+  int32_t items;        // scan/probe items of the query (0: folded by k_batch_final)
   int32_t naggs;
   int32_t first_shift;  // first >> first_shift = the scanned / probe row
   const int8_t* cnull[kBMaxAggs];  // COUNT(col): NULL array of the column
@@ -168,9 +172,14 @@ __global__ __launch_bounds__(kBThreads) void k_batch_clear(const BHeader* h) {
   }
 }
 
-// Takes the next item of queue `q` for the whole workgroup.
-__device__ __forceinline__ int32_t take_item(unsigned int* head, int32_t* s_it) {
-  if (threadIdx.x == 0) *s_it = static_cast<int32_t>(atomicAdd(head, 1u));
+// Takes the next item (of `total`) of a queue for the whole workgroup.
+__device__ __forceinline__ int32_t take_item(unsigned int* head, int32_t total, int32_t* s_it) {
+  if (threadIdx.x == 0) {
+    // An exhausted queue is detected without an atomic (heads only grow).
+    uint32_t v = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (static_cast<int32_t>(v) < total) v = atomicAdd(head, 1u);
+    *s_it = static_cast<int32_t>(v);
+  }
   __syncthreads();
   // Uniform across the workgroup: keeps descriptor reads on the scalar path.
   const int32_t it = __builtin_amdgcn_readfirstlane(*s_it);
@@ -183,7 +192,7 @@ __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQu
   __shared__ int32_t s_it;
   const int32_t total = h->bpre[h->nbq];
   while (true) {
-    const int32_t it = take_item(&h->bhead, &s_it);
+    const int32_t it = take_item(&h->bhead, total, &s_it);
     if (it >= total) break;
     const int j = find_prefix(h->bpre, h->nbq, it);
     const BQuery& d = qs[h->bq[j]];
@@ -757,6 +766,66 @@ __device__ __forceinline__ void item_join(const BQuery& d, int64_t lo, int64_t h
   __builtin_amdgcn_wave_barrier();
 }
 
+// Folds query d's accumulator copies into its result area (pinned host
+// memory) and publishes the completion word. Called by one wave, after an
+// acquire, once every item of the query has added its partials. The copies are
+// read with device-scope atomic loads (they were written by atomics from every
+// XCD).
+__device__ __forceinline__ uint64_t coherent_load(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void fold_query(const BQuery& d) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int G = d.G, nm = d.nm;
+  const ResLayout L = res_layout(G, nm, d.naggs);
+  AggFinal* fin = reinterpret_cast<AggFinal*>(d.res + L.fin);
+  uint64_t* rows = reinterpret_cast<uint64_t*>(d.res + L.rows);
+  uint64_t* first = reinterpret_cast<uint64_t*>(d.res + L.first);
+  int8_t* fnull = reinterpret_cast<int8_t*>(d.res + L.fnull);
+  if (lane == 0) *reinterpret_cast<uint32_t*>(d.res) = __hip_atomic_load(d.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = lane; i < G * nm; i += kWave) {
+    const bool real = d.M.is_real[i % nm];
+    uint64_t s = 0, c = 0;
+    for (int sl = 0; sl < kBSlots; sl++) {
+      s = acc_add(real, s, coherent_load(&d.sum[static_cast<int64_t>(sl) * G * nm + i]));
+      c += coherent_load(&d.nn[static_cast<int64_t>(sl) * G * nm + i]);
+    }
+    fin[i] = AggFinal{s, 0, 0, c};
+  }
+  for (int g = lane; g < G; g += kWave) {
+    uint64_t c = 0;
+    for (int sl = 0; sl < kBSlots; sl++) c += coherent_load(&d.rows[static_cast<int64_t>(sl) * G + g]);
+    rows[g] = c;
+    const uint64_t f = coherent_load(&d.first[g]);
+    first[g] = f;
+    for (int a = 0; a < d.naggs; a++)
+      fnull[static_cast<int64_t>(a) * G + g] =
+          (d.cnull[a] && c > 0 && f != ~0ull) ? d.cnull[a][f >> d.first_shift] : 0;
+  }
+  // Results before the completion word, for the host polling it.
+  __threadfence_system();
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) __hip_atomic_store(d.done, d.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Ends one scan/probe item of query d: the wave that finishes the query's last
+// item folds it. The item's partials are device-scope atomics; waiting until
+// the wave's outstanding memory operations are acknowledged orders them before
+// the count, and the fold reads the copies with device-scope loads. (A device
+// release fence would also write back and invalidate the XCD's L2 on every
+// item, discarding the tiles the queries of a table share.)
+__device__ __forceinline__ void item_done(const BQuery& d) {
+  const int lane = threadIdx.x & (kWave - 1);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  unsigned long long old = 0;
+  if (lane == 0) old = atomicAdd(d.rem, ~0ull);  // - 1
+  old = __shfl(old, 0);
+  if (old == 1) fold_query(d);
+}
+
 // Wave-level form of drain(): each wave takes items on its own.
 template <typename Run>
 __device__ __forceinline__ void drain_wave(BQueues& Q, const BQuery* qs, Run&& run) {
@@ -765,8 +834,13 @@ __device__ __forceinline__ void drain_wave(BQueues& Q, const BQuery* qs, Run&& r
   int x = blockIdx.x % kBQueues;
   for (int exhausted = 0; exhausted < kBQueues;) {
     const int32_t total = Q.qpre[x][nt];
+    // Heads only grow: a queue seen at or past its end stays empty, and is
+    // skipped without an atomic (all waves finish by probing every queue).
     uint32_t v = 0;
-    if (lane == 0) v = atomicAdd(&Q.heads[x], 1u);
+    if (lane == 0) {
+      v = __hip_atomic_load(&Q.heads[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (static_cast<int32_t>(v) < total) v = atomicAdd(&Q.heads[x], 1u);
+    }
     const int32_t it = static_cast<int32_t>(__builtin_amdgcn_readfirstlane(v));
     if (it >= total) {
       x = (x + 1) % kBQueues;
@@ -781,6 +855,7 @@ __device__ __forceinline__ void drain_wave(BQueues& Q, const BQuery* qs, Run&& r
     const int64_t lo = static_cast<int64_t>(tile) * kBTile;
     const int64_t hi = lo + kBTile < d.n ? lo + kBTile : d.n;
     run(d, lo, hi);
+    item_done(d);
   }
 }
 
@@ -811,37 +886,13 @@ __global__ __launch_bounds__(kBThreads) void k_batch_gen(BHeader* h, const BQuer
   drain_wave(h->gen, qs, [&](const BQuery& d, int64_t lo, int64_t hi) { item_lds(d, lo, hi, slot, mine); });
 }
 
-// One workgroup per query: fold the accumulator copies into the result area.
-__global__ __launch_bounds__(kBThreads) void k_batch_final(const BHeader* h, const BQuery* qs) {
+// Queries without scan/probe items (no rows) are folded here, after the
+// scan/probe kernels.
+__global__ __launch_bounds__(kWave) void k_batch_final(const BHeader* h, const BQuery* qs) {
   if (static_cast<int>(blockIdx.x) >= h->nq) return;
   const BQuery& d = qs[blockIdx.x];
-  if (d.G == 0 || !d.res) return;  // not planned
-  const int G = d.G, nm = d.nm;
-  const ResLayout L = res_layout(G, nm, d.naggs);
-  AggFinal* fin = reinterpret_cast<AggFinal*>(d.res + L.fin);
-  uint64_t* rows = reinterpret_cast<uint64_t*>(d.res + L.rows);
-  uint64_t* first = reinterpret_cast<uint64_t*>(d.res + L.first);
-  int8_t* fnull = reinterpret_cast<int8_t*>(d.res + L.fnull);
-  if (threadIdx.x == 0) *reinterpret_cast<uint32_t*>(d.res) = *d.flag;
-  for (int i = threadIdx.x; i < G * nm; i += kBThreads) {
-    const bool real = d.M.is_real[i % nm];
-    uint64_t s = 0, c = 0;
-    for (int sl = 0; sl < kBSlots; sl++) {
-      s = acc_add(real, s, d.sum[static_cast<int64_t>(sl) * G * nm + i]);
-      c += d.nn[static_cast<int64_t>(sl) * G * nm + i];
-    }
-    fin[i] = AggFinal{s, 0, 0, c};
-  }
-  for (int g = threadIdx.x; g < G; g += kBThreads) {
-    uint64_t c = 0;
-    for (int sl = 0; sl < kBSlots; sl++) c += d.rows[static_cast<int64_t>(sl) * G + g];
-    rows[g] = c;
-    const uint64_t f = d.first[g];
-    first[g] = f;
-    for (int a = 0; a < d.naggs; a++)
-      fnull[static_cast<int64_t>(a) * G + g] =
-          (d.cnull[a] && c > 0 && f != ~0ull) ? d.cnull[a][f >> d.first_shift] : 0;
-  }
+  if (d.G == 0 || !d.res || d.items != 0) return;  // not planned, or folded by its last item
+  fold_query(d);
 }
 
 }  // namespace kg
@@ -903,7 +954,7 @@ size_t query_footprint(const kg_query& q) {
   const int64_t G = std::min<int64_t>(query_groups(q), kMaxLdsGroups);
   const int nm = std::min(std::max(q.naggs, 1), kMaxMeasures);
   size_t b = res_layout(static_cast<int>(G), nm, std::min(q.naggs, kBMaxAggs)).bytes;
-  b += round256(static_cast<size_t>(kBSlots) * G * (2 * nm + 1) * 8 + 8);
+  b += round256(static_cast<size_t>(kBSlots) * G * (2 * nm + 1) * 8 + 16);
   b += round256(static_cast<size_t>(G) * 8);
   if (q.kind == KG_Q_JOIN_AGG) {
     const uint64_t nb = table_buckets(std::max<int64_t>(q.build_n, 0), query_lf(q));
@@ -925,7 +976,11 @@ struct kg_batch {
   char* h_desc = nullptr;      // pinned staging of the same
   char* arena = nullptr;
   size_t arena_bytes = 0;
-  char* h_res = nullptr;       // pinned copy of the result areas
+  char* h_res = nullptr;       // result areas: coherent pinned host memory written by the device
+  char* d_res = nullptr;       // its device address
+  int32_t* h_done = nullptr;   // per-query completion words (coherent pinned host memory)
+  int32_t* d_done = nullptr;
+  int32_t seq = 0;             // sequence number of the batch in flight
   size_t res_cap = 0;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -952,7 +1007,7 @@ int capture(kg_batch* b) {
   hipLaunchKernelGGL(k_batch_fast, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
   hipLaunchKernelGGL(k_batch_join, dim3(b->grid_gen), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
   hipLaunchKernelGGL(k_batch_gen, dim3(b->grid_gen), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
-  hipLaunchKernelGGL(k_batch_final, dim3(kBMaxQueries), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
+  hipLaunchKernelGGL(k_batch_final, dim3(kBMaxQueries), dim3(kWave), 0, b->stream, b->dhdr(), b->dq());
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (e != hipSuccess) return kg_fail(ctx, KG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
@@ -1094,6 +1149,9 @@ bool layout_queues(BQueues& Q, const std::vector<int32_t>& ids, const kg_query* 
     }
     it->second.push_back(i);
   }
+  // Smallest tables first: their queries complete (and are published) early.
+  std::stable_sort(tables.begin(), tables.end(),
+                   [](const auto& a, const auto& b) { return a.first.second < b.first.second; });
   const int nt = static_cast<int>(tables.size());
   Q.nt = nt;
   Q.toff[0] = 0;
@@ -1147,11 +1205,16 @@ extern "C" int kg_batch_create(kg_ctx* ctx, const kg_batch_opts* opts, kg_batch*
     (void)hipGetLastError();
     return fail(kg_fail(ctx, KG_ERR_OOM, "device allocation of the batch arena (%zu bytes) failed", b->arena_bytes));
   }
+  const unsigned coherent = hipHostMallocCoherent | hipHostMallocMapped;
   if (hipHostMalloc(reinterpret_cast<void**>(&b->h_desc), desc_bytes(), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&b->h_res), b->res_cap, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(reinterpret_cast<void**>(&b->h_res), b->res_cap, coherent) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&b->h_done), kBMaxQueries * sizeof(int32_t), coherent) != hipSuccess ||
+      hipHostGetDevicePointer(reinterpret_cast<void**>(&b->d_res), b->h_res, 0) != hipSuccess ||
+      hipHostGetDevicePointer(reinterpret_cast<void**>(&b->d_done), b->h_done, 0) != hipSuccess) {
     (void)hipGetLastError();
     return fail(kg_fail(ctx, KG_ERR_OOM, "pinned host allocation failed"));
   }
+  std::memset(b->h_done, 0, kBMaxQueries * sizeof(int32_t));
   std::memset(b->h_desc, 0, desc_bytes());
   int st = capture(b);
   if (st != KG_OK) return fail(st);
@@ -1168,6 +1231,7 @@ extern "C" int kg_batch_free(kg_batch* b) {
   if (b->arena) (void)hipFree(b->arena);
   if (b->h_desc) (void)hipHostFree(b->h_desc);
   if (b->h_res) (void)hipHostFree(b->h_res);
+  if (b->h_done) (void)hipHostFree(b->h_done);
   if (b->done) (void)hipEventDestroy(b->done);
   if (b->ready) (void)hipEventDestroy(b->ready);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1195,8 +1259,9 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   BQuery* Q = b->hq();
   std::memset(H, 0, sizeof(BHeader));
   b->plans.assign(nq, BPlan{});
-  // Plan; lay out result areas first (contiguous, copied back), then the
-  // accumulators and tables.
+  // Plan; result areas go to the pinned result buffer, accumulators, counters
+  // and tables to the device arena.
+  b->seq++;
   std::vector<int32_t> ok;
   size_t off = 0;
   for (int i = 0; i < nq; i++) {
@@ -1219,12 +1284,13 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     ok.push_back(i);
   }
   b->res_used = off;
+  off = 0;
   int nf = 0;
   for (int i : ok) {
     BPlan& P = b->plans[i];
     BQuery& D = Q[i];
     const int G = P.G, nm = P.M.nm;
-    const size_t acc_bytes = round256(static_cast<size_t>(kBSlots) * G * (2 * std::max(nm, 1) + 1) * 8 + 8);
+    const size_t acc_bytes = round256(static_cast<size_t>(kBSlots) * G * (2 * std::max(nm, 1) + 1) * 8 + 16);
     const size_t first_bytes = round256(static_cast<size_t>(G) * 8);
     size_t need = acc_bytes + first_bytes;
     uint64_t nbuckets = 0;
@@ -1242,7 +1308,11 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     D.nn = D.sum + static_cast<size_t>(kBSlots) * G * std::max(nm, 1);
     D.rows = D.nn + static_cast<size_t>(kBSlots) * G * std::max(nm, 1);
     D.flag = reinterpret_cast<uint32_t*>(D.rows + static_cast<size_t>(kBSlots) * G);
-    H->fills[nf++] = BFill{D.sum, static_cast<int64_t>(acc_bytes / 8), 0};
+    D.rem = reinterpret_cast<unsigned long long*>(D.rows + static_cast<size_t>(kBSlots) * G + 1);
+    // Fill ranges must not overlap: the fill kernel writes them in no fixed order.
+    H->fills[nf++] = BFill{D.sum, static_cast<int64_t>(kBSlots) * G * (2 * std::max(nm, 1) + 1) + 1, 0};
+    D.items = static_cast<int32_t>(ceil_div(qs[i].n, kBTile));
+    H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.rem), 1, static_cast<uint64_t>(D.items)};
     D.first = reinterpret_cast<uint64_t*>(base + acc_bytes);
     H->fills[nf++] = BFill{D.first, G, ~0ull};
     if (P.join) {
@@ -1255,7 +1325,9 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
       H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.t.keys), static_cast<int64_t>(nbuckets * kSlots),
                              static_cast<uint64_t>(kEmpty)};
     }
-    D.res = b->arena + P.res_off;
+    D.res = b->d_res + P.res_off;
+    D.done = b->d_done + i;
+    D.seq = b->seq;
     off += need;
   }
   H->nfills = nf;
@@ -1294,8 +1366,6 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   KG_HIP_CHECK(ctx, hipStreamWaitEvent(b->stream, b->ready, 0));
   KG_HIP_CHECK(ctx, hipMemcpyAsync(b->d_desc, b->h_desc, bytes, hipMemcpyHostToDevice, b->stream));
   KG_HIP_CHECK(ctx, hipGraphLaunch(b->exec, b->stream));
-  if (b->res_used > 0)
-    KG_HIP_CHECK(ctx, hipMemcpyAsync(b->h_res, b->arena, b->res_used, hipMemcpyDeviceToHost, b->stream));
   KG_HIP_CHECK(ctx, hipEventRecord(b->done, b->stream));
   b->nq = nq;
   b->pending = true;
@@ -1320,6 +1390,83 @@ extern "C" int kg_batch_poll(kg_batch* b, int32_t* done) {
   return kg_fail(b->ctx, KG_ERR_HIP, "batch event query failed: %s", hipGetErrorString(e));
 }
 
+namespace {
+
+// Writes query i's result from its (completed) result area.
+void emit_query(kg_batch* b, int i, kg_query_result& R) {
+  kg_ctx* ctx = b->ctx;
+  BPlan& P = b->plans[i];
+  R.ngroups = 0;
+  R.status = P.status;
+  if (P.status != KG_OK) return;
+  const char* base = b->h_res + P.res_off;
+  const uint32_t flag = *reinterpret_cast<const volatile uint32_t*>(base);
+  if (flag & kFlagEmptyKey) {
+    R.status = KG_ERR_UNSUPPORTED;
+    return;
+  }
+  if (flag & kFlagBadCode) {
+    R.status = KG_ERR_INVALID;
+    return;
+  }
+  if (flag & kFlagOverflow) {
+    R.status = KG_ERR_CAPACITY;
+    return;
+  }
+  const int G = P.G, nm = P.M.nm, naggs = static_cast<int>(P.plan.size());
+  const AggFinal* f = reinterpret_cast<const AggFinal*>(base + P.L.fin);
+  std::vector<AggFinal> fin(f, f + static_cast<size_t>(G) * std::max(nm, 1));
+  const uint64_t* r = reinterpret_cast<const uint64_t*>(base + P.L.rows);
+  std::vector<uint64_t> rowsv(r, r + G);
+  const uint64_t* fr = reinterpret_cast<const uint64_t*>(base + P.L.first);
+  std::vector<int64_t> firstv(G);
+  for (int g = 0; g < G; g++) firstv[g] = static_cast<int64_t>(fr[g]);
+  std::vector<std::vector<int8_t>> first_null(naggs);
+  const int8_t* fn = reinterpret_cast<const int8_t*>(base + P.L.fnull);
+  for (int a = 0; a < naggs; a++)
+    if (P.plan[a].kind == KG_AGG_COUNT && P.plan[a].count_nulls)
+      first_null[a].assign(fn + static_cast<size_t>(a) * G, fn + static_cast<size_t>(a + 1) * G);
+  if ((naggs > 0 && (!R.vals || !R.nulls)) || (P.nkeys_out > 0 && !R.keys)) {
+    R.status = KG_ERR_INVALID;
+    return;
+  }
+  const std::string saved = ctx->err;
+  R.status = emit_groups(ctx, G, P.nkeys_out, P.card1, P.M, P.plan, fin, rowsv, firstv, first_null, R.max_groups,
+                         &R.ngroups, R.keys, R.vals, R.nulls);
+  ctx->err = saved;
+}
+
+bool query_ready(const kg_batch* b, int i) {
+  if (b->plans[i].status != KG_OK) return true;
+  return __atomic_load_n(&b->h_done[i], __ATOMIC_ACQUIRE) == b->seq;
+}
+
+}  // namespace
+
+extern "C" int kg_batch_query_ready(kg_batch* b, int32_t i, int32_t* ready) {
+  if (!b || !ready) return KG_ERR_INVALID;
+  KG_REQUIRE(b->ctx, b->pending && i >= 0 && i < b->nq, "no query %d in flight", i);
+  *ready = query_ready(b, i) ? 1 : 0;
+  return KG_OK;
+}
+
+extern "C" int kg_batch_collect_query(kg_batch* b, int32_t i, kg_query_result* res) {
+  if (!b || !res) return KG_ERR_INVALID;
+  KG_REQUIRE(b->ctx, b->pending && i >= 0 && i < b->nq, "no query %d in flight", i);
+  KG_REQUIRE(b->ctx, query_ready(b, i), "query %d has not completed", i);
+  emit_query(b, i, *res);
+  return KG_OK;
+}
+
+extern "C" int kg_batch_release(kg_batch* b) {
+  if (!b) return KG_ERR_INVALID;
+  kg_ctx* ctx = b->ctx;
+  KG_REQUIRE(ctx, b->pending, "no batch in flight");
+  b->pending = false;
+  KG_HIP_CHECK(ctx, hipEventSynchronize(b->done));
+  return KG_OK;
+}
+
 extern "C" int kg_batch_collect(kg_batch* b, kg_query_result* res) {
   if (!b) return KG_ERR_INVALID;
   kg_ctx* ctx = b->ctx;
@@ -1328,46 +1475,12 @@ extern "C" int kg_batch_collect(kg_batch* b, kg_query_result* res) {
   b->pending = false;
   KG_HIP_CHECK(ctx, hipEventSynchronize(b->done));
   for (int i = 0; i < b->nq; i++) {
-    BPlan& P = b->plans[i];
-    kg_query_result& R = res[i];
-    R.ngroups = 0;
-    R.status = P.status;
-    if (P.status != KG_OK) continue;
-    const char* base = b->h_res + P.res_off;
-    const uint32_t flag = *reinterpret_cast<const uint32_t*>(base);
-    if (flag & kFlagEmptyKey) {
-      R.status = KG_ERR_UNSUPPORTED;
+    if (!query_ready(b, i)) {
+      res[i].ngroups = 0;
+      res[i].status = kg_fail(ctx, KG_ERR_HIP, "query %d was not published by the device", i);
       continue;
     }
-    if (flag & kFlagBadCode) {
-      R.status = KG_ERR_INVALID;
-      continue;
-    }
-    if (flag & kFlagOverflow) {
-      R.status = KG_ERR_CAPACITY;
-      continue;
-    }
-    const int G = P.G, nm = P.M.nm, naggs = static_cast<int>(P.plan.size());
-    const AggFinal* f = reinterpret_cast<const AggFinal*>(base + P.L.fin);
-    std::vector<AggFinal> fin(f, f + static_cast<size_t>(G) * std::max(nm, 1));
-    const uint64_t* r = reinterpret_cast<const uint64_t*>(base + P.L.rows);
-    std::vector<uint64_t> rowsv(r, r + G);
-    const uint64_t* fr = reinterpret_cast<const uint64_t*>(base + P.L.first);
-    std::vector<int64_t> firstv(G);
-    for (int g = 0; g < G; g++) firstv[g] = static_cast<int64_t>(fr[g]);
-    std::vector<std::vector<int8_t>> first_null(naggs);
-    const int8_t* fn = reinterpret_cast<const int8_t*>(base + P.L.fnull);
-    for (int a = 0; a < naggs; a++)
-      if (P.plan[a].kind == KG_AGG_COUNT && P.plan[a].count_nulls)
-        first_null[a].assign(fn + static_cast<size_t>(a) * G, fn + static_cast<size_t>(a + 1) * G);
-    if ((naggs > 0 && (!R.vals || !R.nulls)) || (P.nkeys_out > 0 && !R.keys)) {
-      R.status = KG_ERR_INVALID;
-      continue;
-    }
-    const std::string saved = ctx->err;
-    R.status = emit_groups(ctx, G, P.nkeys_out, P.card1, P.M, P.plan, fin, rowsv, firstv, first_null, R.max_groups,
-                           &R.ngroups, R.keys, R.vals, R.nulls);
-    ctx->err = saved;
+    emit_query(b, i, res[i]);
   }
   return KG_OK;
 }

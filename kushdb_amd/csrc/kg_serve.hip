@@ -89,7 +89,8 @@ struct Slot {
   bool busy = false;
   std::vector<int32_t> ids;
   std::vector<kg_query> qbuf;
-  std::vector<kg_query_result> rbuf;
+  std::vector<char> finished;
+  int left = 0;
 };
 
 }  // namespace
@@ -102,7 +103,7 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
   for (int i = 1; i < nq; i++) KG_REQUIRE(ctx, arrival_s[i] >= arrival_s[i - 1], "arrivals must be ascending");
   const bool batched = opts->mode == KG_SERVE_BATCHED;
   KG_REQUIRE(ctx, batched || opts->mode == KG_SERVE_PLAIN, "bad serving mode %d", opts->mode);
-  const int nslots = opts->slots > 0 ? opts->slots : 2;
+  const int nslots = opts->slots > 0 ? opts->slots : 3;
   const int max_batch = opts->max_batch > 0 ? std::min(opts->max_batch, 256) : 128;
   KG_REQUIRE(ctx, nslots <= 8, "at most 8 batch slots");
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
@@ -154,26 +155,27 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
         progressed = true;
       }
     } else {
-      // Completions.
+      // Completions: each query as soon as the device has published it; the
+      // slot when its batch has finished.
       for (Slot& s : slots) {
         if (!s.busy) continue;
-        int32_t done = 0;
-        int st = kg_batch_poll(s.b, &done);
-        if (st != KG_OK) {
-          cleanup();
-          return st;
-        }
-        if (!done) continue;
-        if ((st = kg_batch_collect(s.b, s.rbuf.data())) != KG_OK) {
-          cleanup();
-          return st;
-        }
-        s.busy = false;
-        progressed = true;
         for (size_t k = 0; k < s.ids.size(); k++) {
+          if (s.finished[k]) continue;
+          int32_t ready = 0;
+          int st = kg_batch_query_ready(s.b, static_cast<int32_t>(k), &ready);
+          if (st != KG_OK) {
+            cleanup();
+            return st;
+          }
+          if (!ready) continue;
           const int32_t i = s.ids[k];
-          res[i].status = s.rbuf[k].status;
-          res[i].ngroups = s.rbuf[k].ngroups;
+          if ((st = kg_batch_collect_query(s.b, static_cast<int32_t>(k), &res[i])) != KG_OK) {
+            cleanup();
+            return st;
+          }
+          s.finished[k] = 1;
+          s.left--;
+          progressed = true;
           if (res[i].status == KG_ERR_UNSUPPORTED) {
             st_.fallbacks++;
             plain(i);
@@ -182,15 +184,27 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
             finish(i);
           }
         }
+        if (s.left == 0) {
+          int32_t done = 0;
+          int st = kg_batch_poll(s.b, &done);
+          if (st == KG_OK && done) st = kg_batch_release(s.b);
+          if (st != KG_OK) {
+            cleanup();
+            return st;
+          }
+          if (done) {
+            s.busy = false;
+            progressed = true;
+          }
+        }
       }
       // Admission into free slots.
       for (Slot& s : slots) {
         if (s.busy || pending.empty()) continue;
         s.ids.clear();
         s.qbuf.clear();
-        s.rbuf.clear();
         const int64_t arena = kg_batch_arena_bytes(s.b);
-        int64_t used = 0;
+        int64_t used = 0, rows = 0;
         while (!pending.empty() && static_cast<int>(s.ids.size()) < max_batch) {
           const int32_t i = pending.front();
           int64_t fp = 0;
@@ -203,13 +217,17 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
             continue;
           }
           if (used + fp > arena) break;
+          const int64_t qrows = qs[i].n + (qs[i].kind == KG_Q_JOIN_AGG ? qs[i].build_n : 0);
+          if (opts->max_batch_rows > 0 && !s.ids.empty() && rows + qrows > opts->max_batch_rows) break;
+          rows += qrows;
           used += fp;
           pending.pop_front();
           s.ids.push_back(i);
           s.qbuf.push_back(qs[i]);
-          s.rbuf.push_back(res[i]);
         }
         if (s.ids.empty()) continue;
+        s.finished.assign(s.ids.size(), 0);
+        s.left = static_cast<int>(s.ids.size());
         const int st = kg_batch_submit(s.b, s.qbuf.data(), static_cast<int32_t>(s.ids.size()));
         if (st != KG_OK) {
           cleanup();

@@ -564,6 +564,23 @@ class Batch:
         self._inflight = None
         return _results(queries, res, bufs)
 
+    def ready(self, i):
+        """Whether query i of the batch in flight has been published."""
+        d = ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.kg_batch_query_ready(self.h, i, ctypes.byref(d)))
+        return bool(d.value)
+
+    def collect_query(self, i):
+        """(status, codes, values, nulls) of query i, once ready(i)."""
+        queries, _, max_groups = self._inflight
+        res, bufs = _result_buffers([queries[i]], max_groups)
+        self.ctx.check(self.ctx.lib.kg_batch_collect_query(self.h, i, res))
+        return _results([queries[i]], res, bufs)[0]
+
+    def release(self):
+        self.ctx.check(self.ctx.lib.kg_batch_release(self.h))
+        self._inflight = None
+
     def run(self, queries, max_groups=4096):
         self.submit(queries, max_groups)
         return self.collect()
@@ -580,7 +597,7 @@ class Batch:
             pass
 
 
-def serve(ctx, queries, arrivals, mode, slots=2, max_batch=128, arena_bytes=0, max_groups=64):
+def serve(ctx, queries, arrivals, mode, slots=3, max_batch=128, arena_bytes=0, max_groups=64, max_batch_rows=0):
     """Replays `queries` arriving at `arrivals` seconds (ascending). Returns
     (results as Batch.collect, latencies in s, stats dict)."""
     n = len(queries)
@@ -588,7 +605,7 @@ def serve(ctx, queries, arrivals, mode, slots=2, max_batch=128, arena_bytes=0, m
     arr = np.ascontiguousarray(arrivals, dtype=np.float64)
     lat = np.zeros(max(n, 1), dtype=np.float64)
     res, bufs = _result_buffers(queries, max_groups)
-    opts = abi.KgServeOpts(mode, slots, max_batch, 0, arena_bytes)
+    opts = abi.KgServeOpts(mode, slots, max_batch, 0, arena_bytes, max_batch_rows)
     st = abi.KgServeStats()
     ctx.check(ctx.lib.kg_serve(ctx.h, qarr, n, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                ctypes.byref(opts), res, lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),

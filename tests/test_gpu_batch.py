@@ -130,3 +130,25 @@ def test_unaligned_columns_take_the_generic_path(env):
     Hs = {"facts": [{k: v[1:] for k, v in H["facts"][2].items()}] * 3, "dim": H["dim"], "build": H["build"]}
     assert_same(spec, got, oracle_spec(spec, Hs), what="unaligned")
     b.free()
+
+
+def test_queries_complete_individually(env):
+    """Each query's result is published by the device when its own last item is
+    done; collecting them one by one while the batch runs gives the oracle's
+    answers."""
+    import time
+    ctx, _, cols, H = env
+    specs = serve_queries(24, SIZES, NBUILD, seed=17)
+    b = Batch(ctx, arena_bytes=128 << 20)
+    b.submit(_queries(cols, specs), max_groups=64)
+    got = [None] * len(specs)
+    deadline = time.time() + 60
+    while any(g is None for g in got):
+        assert time.time() < deadline, "queries not published"
+        for i in range(len(specs)):
+            if got[i] is None and b.ready(i):
+                got[i] = b.collect_query(i)
+    b.release()
+    for s, g in zip(specs, got):
+        assert_same(s, g, oracle_spec(s, H), what="streamed")
+    b.free()
