@@ -40,11 +40,12 @@
 
 namespace kg {
 
-constexpr int kBTile = 4096;          // rows per scan/probe or build work item
+constexpr int kBTile = 4096;          // rows per scan/probe work item
+constexpr int kBBuildTile = 512;      // rows per build work item
 constexpr int kBThreads = 256;        // threads per workgroup
 constexpr int kBWaves = kBThreads / kWave;
 constexpr int kBQueues = 8;           // scan/probe work queues (one per XCD)
-constexpr int kBSlots = 8;            // accumulator copies per query
+constexpr int kBSlots = 64;           // accumulator copies per query (8 per XCD)
 constexpr int kBMaxQueries = 256;     // queries per batch
 constexpr int kBRegG = 8;             // register path: at most this many groups
 constexpr int kBLdsWords = 2048;      // LDS path: 8-byte words of the item table
@@ -68,11 +69,30 @@ struct BFast {
   const void* mcol[kMaxMeasures];
 };
 
+// Typed view of a join's probe side: 4-byte non-NULL predicate columns, a
+// non-NULL 4- or 8-byte integer probe key, at most two non-NULL measures
+// (REAL or 4-byte integer, optionally times a REAL / 4-byte integer factor),
+// all 16-byte aligned (vector loads).
+struct BJoinFast {
+  int32_t npred;
+  int32_t op[kBFastPreds];
+  int64_t lit[kBFastPreds];
+  const int32_t* pcol[kBFastPreds];
+  const void* key;
+  int32_t key64;
+  int32_t mkind[2];     // 1 REAL, 2 4-byte integer
+  int32_t fkind[2];     // 0 none, 1 REAL, 2 4-byte integer
+  const void* mcol[2];
+  const void* fcol[2];
+};
+
 // Device descriptor of one query.
 struct BQuery {
   int32_t kind;         // KG_Q_SCAN_AGG / KG_Q_JOIN_AGG
   int32_t reg;          // 1: register path (BFast valid)
+  int32_t jfast;        // 1: typed join path (J valid)
   BFast F;
+  BJoinFast J;
   int32_t G;            // dense groups
   int32_t nm;           // measures
   int64_t n;            // scanned / probing rows
@@ -86,7 +106,7 @@ struct BQuery {
   PredSet PB;           // build side predicates
   kg_col bkey;          // build key
   // Accumulators in the arena: sum/nn [kBSlots][G][nm], rows [kBSlots][G],
-  // first [G] (min over copies), flag (BFlag bits).
+  // first [kBSlots][G] (min over copies), flag (BFlag bits).
   uint64_t* sum;
   uint64_t* nn;
   uint64_t* rows;
@@ -127,7 +147,7 @@ struct BHeader {
   int32_t bq[kBMaxQueries];                   // build phase: query of entry j
   unsigned int bhead;
   BQueues fast;                               // register-path scans
-  BQueues join;                               // joins with at most 2 measures
+  BQueues join;                               // joins with a typed view
   BQueues gen;                                // the others
   BFill fills[kBMaxFills];
 };
@@ -196,8 +216,8 @@ __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQu
     if (it >= total) break;
     const int j = find_prefix(h->bpre, h->nbq, it);
     const BQuery& d = qs[h->bq[j]];
-    const int64_t lo = static_cast<int64_t>(it - h->bpre[j]) * kBTile;
-    const int64_t hi = lo + kBTile < d.bn ? lo + kBTile : d.bn;
+    const int64_t lo = static_cast<int64_t>(it - h->bpre[j]) * kBBuildTile;
+    const int64_t hi = lo + kBBuildTile < d.bn ? lo + kBBuildTile : d.bn;
     const TableView t = d.t;
     for (int64_t r = lo + threadIdx.x; r < hi; r += kBThreads) {
       if (!eval_preds(d.PB, r)) continue;
@@ -531,7 +551,7 @@ __device__ __forceinline__ void item_fast(const BQuery& d, int64_t lo, int64_t h
   }
   if (lane < G && myc) {
     atomicAdd(reinterpret_cast<unsigned long long*>(&d.rows[static_cast<int64_t>(slot) * G + lane]), myc);
-    atomicMin(reinterpret_cast<unsigned long long*>(&d.first[lane]), myf);
+    atomicMin(reinterpret_cast<unsigned long long*>(&d.first[static_cast<int64_t>(slot) * G + lane]), myf);
     for (int m = 0; m < nm; m++)
       atomicAdd(reinterpret_cast<unsigned long long*>(&d.nn[(static_cast<int64_t>(slot) * G + lane) * nm + m]), myc);
   }
@@ -617,7 +637,7 @@ __device__ __forceinline__ void item_lds(const BQuery& d, int64_t lo, int64_t hi
   for (int i = lane; i < G; i += kWave) {
     if (lrows[i] == 0) continue;
     atomicAdd(reinterpret_cast<unsigned long long*>(&d.rows[static_cast<int64_t>(slot) * G + i]), lrows[i]);
-    atomicMin(reinterpret_cast<unsigned long long*>(&d.first[i]), lfirst[i]);
+    atomicMin(reinterpret_cast<unsigned long long*>(&d.first[static_cast<int64_t>(slot) * G + i]), lfirst[i]);
   }
   for (int i = lane; i < G * nm; i += kWave) {
     if (lnn[i] == 0) continue;
@@ -753,7 +773,7 @@ __device__ __forceinline__ void item_join(const BQuery& d, int64_t lo, int64_t h
   for (int i = lane; i < G; i += kWave) {
     if (lrows[i] == 0) continue;
     atomicAdd(reinterpret_cast<unsigned long long*>(&d.rows[static_cast<int64_t>(slot) * G + i]), lrows[i]);
-    atomicMin(reinterpret_cast<unsigned long long*>(&d.first[i]), lfirst[i]);
+    atomicMin(reinterpret_cast<unsigned long long*>(&d.first[static_cast<int64_t>(slot) * G + i]), lfirst[i]);
   }
   for (int i = lane; i < G * nm; i += kWave) {
     if (lnn[i] == 0) continue;
@@ -761,6 +781,178 @@ __device__ __forceinline__ void item_join(const BQuery& d, int64_t lo, int64_t h
     atomicAdd(reinterpret_cast<unsigned long long*>(&d.nn[o]), lnn[i]);
     if (d.M.is_real[i % nm]) atomicAdd(reinterpret_cast<double*>(&d.sum[o]), __longlong_as_double(lsum[i]));
     else atomicAdd(reinterpret_cast<unsigned long long*>(&d.sum[o]), lsum[i]);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Typed join items (one wave per item): lane l owns the row groups
+// lo + 4 l + 256 k (k < 16) of 4 consecutive rows. Per row group, the
+// predicate, key and measure columns are read with 16-byte vector loads, then
+// the first buckets of the passing rows are loaded together, then their
+// matches' row ids and group codes. Aggregation into the wave's LDS table: row
+// count, sums, and the first output row of each group (a lane issues that
+// atomicMin once per group and item, since its rows ascend); non-NULL counts
+// equal row counts for these non-NULL measures.
+template <bool FULL>
+__device__ __forceinline__ void ld4_i64(const int64_t* col, int64_t base, int nv, int64_t (&o)[4]) {
+  if (FULL) {
+    const longlong2 a = *reinterpret_cast<const longlong2*>(col + base);
+    const longlong2 b = *reinterpret_cast<const longlong2*>(col + base + 2);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; e++) o[e] = e < nv ? col[base + e] : 0;
+  }
+}
+
+// Measure value of 4 rows as double bits (REAL or product) or int64.
+template <bool FULL>
+__device__ __forceinline__ void ld4_jmeasure(int kind, int fkind, const void* col, const void* fcol, int64_t base,
+                                             int nv, uint64_t (&o)[4]) {
+  if (kind == 1) ld4_measure<FULL, 1>(col, base, nv, o);
+  else ld4_measure<FULL, 2>(col, base, nv, o);
+  if (fkind == 0) return;
+  uint64_t f[4];
+  if (fkind == 1) ld4_measure<FULL, 1>(fcol, base, nv, f);
+  else ld4_measure<FULL, 2>(fcol, base, nv, f);
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const double x = kind == 1 ? __longlong_as_double(o[e]) : static_cast<double>(static_cast<int64_t>(o[e]));
+    const double y = fkind == 1 ? __longlong_as_double(f[e]) : static_cast<double>(static_cast<int64_t>(f[e]));
+    o[e] = static_cast<uint64_t>(__double_as_longlong(x * y));
+  }
+}
+
+template <bool FULL>
+__device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int64_t hi, int slot, uint64_t* lds) {
+  const BJoinFast& J = d.J;
+  const int G = d.G, nm = d.nm;
+  const int lane = threadIdx.x & (kWave - 1);
+  uint64_t* lsum = lds;
+  uint64_t* lrows = lsum + G * nm;
+  uint64_t* lfirst = lrows + G;
+  const int words = G * nm + G * 2;
+  for (int i = lane; i < words; i += kWave) lds[i] = i < G * nm + G ? 0ull : ~0ull;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const TableView& t = d.t;
+  // Product measures are REAL; others keep their kind.
+  const bool real0 = J.mkind[0] == 1 || J.fkind[0] != 0;
+  const bool real1 = J.mkind[1] == 1 || J.fkind[1] != 0;
+  uint64_t seen = 0;  // groups (< 64) whose first row this lane has reported
+#pragma unroll 1
+  for (int k = 0; k < kWRowGroups; k++) {
+    const int64_t base = lo + 4 * lane + static_cast<int64_t>(k) * kWave * 4;
+    const int nv = FULL ? 4 : rows_left(hi, base);
+    if (!FULL && __builtin_amdgcn_readfirstlane(__ballot(nv > 0) != 0 ? 1 : 0) == 0) break;
+    bool pass[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) pass[e] = e < nv;
+#pragma unroll
+    for (int p = 0; p < kBFastPreds; p++) {
+      if (p >= J.npred) break;
+      int32_t x[4];
+      ld4_i32<FULL>(J.pcol[p], base, nv, x);
+#pragma unroll
+      for (int e = 0; e < 4; e++) pass[e] = pass[e] && cmp_i64(J.op[p], x[e], J.lit[p]);
+    }
+    int64_t key[4];
+    if (J.key64) {
+      ld4_i64<FULL>(static_cast<const int64_t*>(J.key), base, nv, key);
+    } else {
+      int32_t k32[4];
+      ld4_i32<FULL>(static_cast<const int32_t*>(J.key), base, nv, k32);
+#pragma unroll
+      for (int e = 0; e < 4; e++) key[e] = k32[e];
+    }
+    uint64_t v0[4] = {0, 0, 0, 0}, v1[4] = {0, 0, 0, 0};
+    if (nm > 0) ld4_jmeasure<FULL>(J.mkind[0], J.fkind[0], J.mcol[0], J.fcol[0], base, nv, v0);
+    if (nm > 1) ld4_jmeasure<FULL>(J.mkind[1], J.fkind[1], J.mcol[1], J.fcol[1], base, nv, v1);
+    // First buckets of the passing rows, loaded together.
+    uint64_t b[4];
+    longlong2 kv[4][kSlots / 2];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      pass[e] = pass[e] && key[e] != kEmpty;
+      b[e] = pass[e] ? bucket_of(key[e], t.nbuckets) : 0;
+#pragma unroll
+      for (int q = 0; q < kSlots / 2; q++)
+        kv[e][q] = pass[e] ? reinterpret_cast<const longlong2*>(t.keys + b[e] * kSlots)[q]
+                           : make_longlong2(kEmpty, kEmpty);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      if (!pass[e]) continue;
+      const int64_t row = base + e;
+      auto consume = [&](int32_t brow) {
+        const int g = d.codes ? d.codes[brow] : 0;
+        if (static_cast<unsigned>(g) >= static_cast<unsigned>(G)) {
+          atomicOr(d.flag, kFlagBadCode);
+          return;
+        }
+        atomicAdd(reinterpret_cast<unsigned long long*>(&lrows[g]), 1ull);
+        if (g >= 64 || !((seen >> g) & 1ull)) {
+          atomicMin(reinterpret_cast<unsigned long long*>(&lfirst[g]),
+                    (static_cast<unsigned long long>(row) << 32) | static_cast<uint32_t>(brow));
+          if (g < 64) seen |= 1ull << g;
+        }
+        if (nm > 0) {
+          if (real0) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm]), __longlong_as_double(v0[e]));
+          else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm]), v0[e]);
+        }
+        if (nm > 1) {
+          if (real1) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm + 1]), __longlong_as_double(v1[e]));
+          else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm + 1]), v1[e]);
+        }
+      };
+      uint32_t hit = 0;
+      bool empty = false;
+#pragma unroll
+      for (int q = 0; q < kSlots / 2; q++) {
+        hit |= (kv[e][q].x == key[e] ? 1u : 0u) << (2 * q);
+        hit |= (kv[e][q].y == key[e] ? 1u : 0u) << (2 * q + 1);
+        empty |= kv[e][q].x == kEmpty || kv[e][q].y == kEmpty;
+      }
+      while (hit) {
+        const int sidx = __builtin_ctz(hit);
+        hit &= hit - 1;
+        consume(t.rows[b[e] * kSlots + sidx]);
+      }
+      if (!empty) {
+        uint64_t bb = (b[e] + 1 == t.nbuckets) ? 0 : b[e] + 1;
+        for (uint64_t scanned = 1; scanned < t.nbuckets; scanned++) {
+          const longlong2* bk = reinterpret_cast<const longlong2*>(t.keys + bb * kSlots);
+          bool emp = false;
+#pragma unroll
+          for (int q = 0; q < kSlots / 2; q++) {
+            const longlong2 x = bk[q];
+            if (x.x == key[e]) consume(t.rows[bb * kSlots + 2 * q]);
+            if (x.y == key[e]) consume(t.rows[bb * kSlots + 2 * q + 1]);
+            emp |= x.x == kEmpty || x.y == kEmpty;
+          }
+          if (emp) break;
+          bb = (bb + 1 == t.nbuckets) ? 0 : bb + 1;
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int i = lane; i < G; i += kWave) {
+    const uint64_t c = lrows[i];
+    if (c == 0) continue;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&d.rows[static_cast<int64_t>(slot) * G + i]), c);
+    atomicMin(reinterpret_cast<unsigned long long*>(&d.first[static_cast<int64_t>(slot) * G + i]), lfirst[i]);
+    for (int m = 0; m < nm; m++) {
+      const int64_t o = (static_cast<int64_t>(slot) * G + i) * nm + m;
+      atomicAdd(reinterpret_cast<unsigned long long*>(&d.nn[o]), c);
+      if (d.M.is_real[m]) atomicAdd(reinterpret_cast<double*>(&d.sum[o]), __longlong_as_double(lsum[i * nm + m]));
+      else atomicAdd(reinterpret_cast<unsigned long long*>(&d.sum[o]), lsum[i * nm + m]);
+    }
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -797,7 +989,11 @@ __device__ void fold_query(const BQuery& d) {
     uint64_t c = 0;
     for (int sl = 0; sl < kBSlots; sl++) c += coherent_load(&d.rows[static_cast<int64_t>(sl) * G + g]);
     rows[g] = c;
-    const uint64_t f = coherent_load(&d.first[g]);
+    uint64_t f = ~0ull;
+    for (int sl = 0; sl < kBSlots; sl++) {
+      const uint64_t x = coherent_load(&d.first[static_cast<int64_t>(sl) * G + g]);
+      f = x < f ? x : f;
+    }
     first[g] = f;
     for (int a = 0; a < d.naggs; a++)
       fnull[static_cast<int64_t>(a) * G + g] =
@@ -824,6 +1020,14 @@ __device__ __forceinline__ void item_done(const BQuery& d) {
   if (lane == 0) old = atomicAdd(d.rem, ~0ull);  // - 1
   old = __shfl(old, 0);
   if (old == 1) fold_query(d);
+}
+
+// Accumulator copy of the calling wave: copies are spread over the waves of
+// an XCD (slot mod 8 = the XCD of workgroup b, which is b mod 8) so that
+// concurrent flushes of one query's partials rarely hit the same address.
+__device__ __forceinline__ int acc_slot() {
+  const int wave = static_cast<int>(blockIdx.x / kBQueues) * kBWaves + static_cast<int>(threadIdx.x / kWave);
+  return static_cast<int>(blockIdx.x % kBQueues) + kBQueues * (wave % (kBSlots / kBQueues));
 }
 
 // Wave-level form of drain(): each wave takes items on its own.
@@ -861,29 +1065,33 @@ __device__ __forceinline__ void drain_wave(BQueues& Q, const BQuery* qs, Run&& r
 
 // Scan/probe phase, register-path scans (one wave per item).
 __global__ __launch_bounds__(kBThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_fast(BHeader* h, const BQuery* qs) {
-  const int slot = blockIdx.x % kBSlots;
+  const int slot = acc_slot();
   drain_wave(h->fast, qs, [&](const BQuery& d, int64_t lo, int64_t hi) {
     if (hi - lo == kBTile) item_fast<true>(d, lo, hi, slot);
     else item_fast<false>(d, lo, hi, slot);
   });
 }
 
-// Scan/probe phase, joins with at most 2 measures (one wave per item).
-__global__ __launch_bounds__(kBThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_join(BHeader* h, const BQuery* qs) {
+// Scan/probe phase, joins with a typed view (one wave per item).
+__global__ __launch_bounds__(kBThreads) void k_batch_join(BHeader* h, const BQuery* qs) {
   __shared__ uint64_t lds[kBLdsWords];
-  const int slot = blockIdx.x % kBSlots;
+  const int slot = acc_slot();
   uint64_t* mine = lds + (threadIdx.x / kWave) * kBWaveWords;
   drain_wave(h->join, qs, [&](const BQuery& d, int64_t lo, int64_t hi) {
-    item_join<2>(d, lo, hi, slot, mine);
+    if (hi - lo == kBTile) item_join_fast<true>(d, lo, hi, slot, mine);
+    else item_join_fast<false>(d, lo, hi, slot, mine);
   });
 }
 
 // Scan/probe phase, the other queries (generic LDS path, one wave per item).
 __global__ __launch_bounds__(kBThreads) void k_batch_gen(BHeader* h, const BQuery* qs) {
   __shared__ uint64_t lds[kBLdsWords];
-  const int slot = blockIdx.x % kBSlots;
+  const int slot = acc_slot();
   uint64_t* mine = lds + (threadIdx.x / kWave) * kBWaveWords;
-  drain_wave(h->gen, qs, [&](const BQuery& d, int64_t lo, int64_t hi) { item_lds(d, lo, hi, slot, mine); });
+  drain_wave(h->gen, qs, [&](const BQuery& d, int64_t lo, int64_t hi) {
+    if (d.kind == KG_Q_JOIN_AGG && d.nm <= 2) item_join<2>(d, lo, hi, slot, mine);
+    else item_lds(d, lo, hi, slot, mine);
+  });
 }
 
 // Queries without scan/probe items (no rows) are folded here, after the
@@ -955,7 +1163,7 @@ size_t query_footprint(const kg_query& q) {
   const int nm = std::min(std::max(q.naggs, 1), kMaxMeasures);
   size_t b = res_layout(static_cast<int>(G), nm, std::min(q.naggs, kBMaxAggs)).bytes;
   b += round256(static_cast<size_t>(kBSlots) * G * (2 * nm + 1) * 8 + 16);
-  b += round256(static_cast<size_t>(G) * 8);
+  b += round256(static_cast<size_t>(kBSlots) * G * 8);
   if (q.kind == KG_Q_JOIN_AGG) {
     const uint64_t nb = table_buckets(std::max<int64_t>(q.build_n, 0), query_lf(q));
     b += round256(nb * kSlots * 8) + round256(nb * kSlots * 4);
@@ -1046,6 +1254,40 @@ bool fast_view(const PredSet& P, const KeySet& K, const MeasureSet& M, BFast& F)
   return true;
 }
 
+// The typed join view, when the probe side's columns allow vector loads.
+bool join_view(const kg_query& q, const PredSet& P, const MeasureSet& M, BJoinFast& J) {
+  std::memset(&J, 0, sizeof(J));
+  auto is_i32 = [](int32_t t) { return t == KG_INT || t == KG_DATE || t == KG_ENUM; };
+  if (!P.conj || P.n > kBFastPreds || M.nm > 2) return false;
+  J.npred = P.n;
+  for (int i = 0; i < P.n; i++) {
+    if (!is_i32(P.type[i]) || P.nulls[i] || !aligned16(P.data[i])) return false;
+    J.op[i] = P.op[i];
+    J.lit[i] = P.ilit[i];
+    J.pcol[i] = static_cast<const int32_t*>(P.data[i]);
+  }
+  const kg_col& k = q.cols[q.probe_key_col];
+  if (k.nulls || !aligned16(k.data)) return false;
+  if (k.type == KG_BIGINT) J.key64 = 1;
+  else if (!is_i32(k.type)) return false;
+  J.key = k.data;
+  for (int m = 0; m < M.nm; m++) {
+    if (M.nulls[m] || !aligned16(M.data[m])) return false;
+    if (M.type[m] == KG_REAL) J.mkind[m] = 1;
+    else if (is_i32(M.type[m])) J.mkind[m] = 2;
+    else return false;
+    J.mcol[m] = M.data[m];
+    if (M.data2[m]) {
+      if (M.nulls2[m] || !aligned16(M.data2[m])) return false;
+      if (M.type2[m] == KG_REAL) J.fkind[m] = 1;
+      else if (is_i32(M.type2[m])) J.fkind[m] = 2;
+      else return false;
+      J.fcol[m] = M.data2[m];
+    }
+  }
+  return true;
+}
+
 // Validates and plans query q into the device descriptor D and host plan P.
 int plan_query(kg_ctx* ctx, const kg_query& q, BQuery& D, BPlan& P) {
   std::memset(&D, 0, sizeof(D));
@@ -1126,6 +1368,7 @@ int plan_query(kg_ctx* ctx, const kg_query& q, BQuery& D, BPlan& P) {
   for (int m = 0; m < nm; m++)
     if (P.M.nulls[m] || (P.M.data2[m] && P.M.nulls2[m])) nonnull = false;
   D.reg = (q.kind == KG_Q_SCAN_AGG && G <= kBRegG && nonnull && fast_view(D.P, D.K, P.M, D.F)) ? 1 : 0;
+  D.jfast = (q.kind == KG_Q_JOIN_AGG && join_view(q, D.P, P.M, D.J)) ? 1 : 0;
   if (!D.reg && G * nm * 2 + G * 2 > kBWaveWords)
     return kg_fail(ctx, KG_ERR_UNSUPPORTED, "%lld groups x %d measures exceed the batched LDS table", (long long)G,
                    nm);
@@ -1291,7 +1534,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     BQuery& D = Q[i];
     const int G = P.G, nm = P.M.nm;
     const size_t acc_bytes = round256(static_cast<size_t>(kBSlots) * G * (2 * std::max(nm, 1) + 1) * 8 + 16);
-    const size_t first_bytes = round256(static_cast<size_t>(G) * 8);
+    const size_t first_bytes = round256(static_cast<size_t>(kBSlots) * G * 8);
     size_t need = acc_bytes + first_bytes;
     uint64_t nbuckets = 0;
     if (P.join) {
@@ -1314,7 +1557,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     D.items = static_cast<int32_t>(ceil_div(qs[i].n, kBTile));
     H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.rem), 1, static_cast<uint64_t>(D.items)};
     D.first = reinterpret_cast<uint64_t*>(base + acc_bytes);
-    H->fills[nf++] = BFill{D.first, G, ~0ull};
+    H->fills[nf++] = BFill{D.first, static_cast<int64_t>(kBSlots) * G, ~0ull};
     if (P.join) {
       char* tb = base + acc_bytes + first_bytes;
       D.t.keys = reinterpret_cast<int64_t*>(tb);
@@ -1342,14 +1585,14 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   for (int i : live) {
     if (!b->plans[i].join || b->plans[i].bn == 0) continue;
     H->bq[nbq] = i;
-    H->bpre[nbq + 1] = H->bpre[nbq] + static_cast<int32_t>(ceil_div(b->plans[i].bn, kBTile));
+    H->bpre[nbq + 1] = H->bpre[nbq] + static_cast<int32_t>(ceil_div(b->plans[i].bn, kBBuildTile));
     nbq++;
   }
   H->nbq = nbq;
   std::vector<int32_t> fast_ids, join_ids, gen_ids;
   for (int i : live) {
     if (Q[i].reg) fast_ids.push_back(i);
-    else if (Q[i].kind == KG_Q_JOIN_AGG && Q[i].nm <= 2) join_ids.push_back(i);
+    else if (Q[i].jfast) join_ids.push_back(i);
     else gen_ids.push_back(i);
   }
   if (!layout_queues(H->fast, fast_ids, qs) || !layout_queues(H->join, join_ids, qs) ||
