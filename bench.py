@@ -63,15 +63,75 @@ def roofline(bytes_per_launch, ms, traffic=None):
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
+# Kernels whose HBM traffic the bench reports, by a substring of their name.
+TRAFFIC_KERNELS = {"k_group_agg_v4": "k_group_agg_v4<", "k_ht_build": "k_ht_build(",
+                   "k_ht_probe_count": "k_ht_probe<0,", "k_ht_probe_agg": "k_ht_probe<2,"}
+COPY_BYTES = 1 << 30
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (profiles/traffic.json), or None when not measured."""
-    p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
-        with open(p) as f:
-            return json.load(f).get(kernel, {}).get("bytes_per_launch")
-    except (OSError, ValueError):
+        with open(TRAFFIC_JSON) as f:
+            v = json.load(f)["kernels"].get(kernel, {}).get("bytes_per_launch")
+            return int(v) if v is not None else None
+    except (OSError, ValueError, KeyError):
         return None
+
+
+def copy_calibration():
+    """A plain device copy of COPY_BYTES (three launches): the known-size
+    reference for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    src = torch.ones(COPY_BYTES, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": COPY_BYTES, "launches": 3}))
+    return 0
+
+
+def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the reported kernels,
+    scaled by the copy kernel's reading of its known size, into
+    profiles/traffic.json."""
+    import csv
+
+    def per_kernel(path, counter):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            acc.setdefault(r["Kernel_Name"], {}).setdefault(r["Dispatch_Id"], 0.0)
+            acc[r["Kernel_Name"]][r["Dispatch_Id"]] += float(r["Counter_Value"])
+        return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+
+    def largest(d):
+        k = max(d, key=d.get)
+        return k, d[k]
+
+    cname, cf = largest(per_kernel(copy_fetch_csv, "FETCH_SIZE"))
+    _, cw = largest(per_kernel(copy_write_csv, "WRITE_SIZE"))
+    ff, wf = COPY_BYTES / (cf * 1024), COPY_BYTES / (cw * 1024)
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    out = {"calibration": {"kernel": cname[:80], "bytes": COPY_BYTES, "fetch_kib": cf, "write_kib": cw,
+                           "fetch_factor": ff, "write_factor": wf},
+           "kernels": {}}
+    for short, sub in TRAFFIC_KERNELS.items():
+        fk = [v for k, v in fetch.items() if sub in k]
+        wk = [v for k, v in write.items() if sub in k]
+        if not fk or not wk:
+            continue
+        f, w = sum(fk) / len(fk), sum(wk) / len(wk)
+        out["kernels"][short] = {"fetch_kib": f, "write_kib": w,
+                                 "bytes_per_launch": round((f * ff + w * wf) * 1024)}
+    with open(TRAFFIC_JSON, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+    return 0
 
 
 def bench_scan_agg(ctx, a):
@@ -234,8 +294,12 @@ def bench_serving(ctx, a):
     batched_sat = run(abi.KG_SERVE_BATCHED, None)
     log(f"saturation: plain {plain_sat['qps']:.0f} q/s, batched {batched_sat['qps']:.0f} q/s")
     base = 0.7 * plain_sat["qps"]
-    bar = run(abi.KG_SERVE_PLAIN, base)["p99_ms"]
-    plain_best, plain_runs = best_rate(abi.KG_SERVE_PLAIN, base, bar, factor=1.1, limit=2)
+    plain_at = run(abi.KG_SERVE_PLAIN, base)
+    bar = plain_at["p99_ms"]
+    # The run that sets the bar meets it; higher plain rates are tried from there.
+    more, plain_runs = best_rate(abi.KG_SERVE_PLAIN, base * 1.1, bar, factor=1.1, limit=2)
+    plain_best = more if (more and more["qps"] > plain_at["qps"]) else plain_at
+    plain_runs = [plain_at] + plain_runs
     batched = []
     for cap in a.serve_caps:
         bb, runs = best_rate(abi.KG_SERVE_BATCHED, base, bar, cap=cap)
@@ -257,6 +321,10 @@ def bench_serving(ctx, a):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--copy-calibration":
+        return copy_calibration()
+    if len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
+        return pmc_summary(*sys.argv[2:6])
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design note, "
@@ -285,8 +353,8 @@ def main():
         "config": {"workload": "scan-select (day <= cutoff, ~98%) + group-by 2 enum keys (6 groups): "
                                "SUM/AVG of 4 f64 + 3 int32 measures, COUNT(*)",
                    "rows": n, "groups": groups, "bytes_per_row": SCAN_AGG_BYTES_PER_ROW},
-        "roofline": dict(roofline(n * SCAN_AGG_BYTES_PER_ROW, kms, measured_traffic("k_group_agg_reg")),
-                         kernel="k_group_agg_reg", kernel_ms=round(kms, 4)),
+        "roofline": dict(roofline(n * SCAN_AGG_BYTES_PER_ROW, kms, measured_traffic("k_group_agg_v4")),
+                         kernel="k_group_agg_v4", kernel_ms=round(kms, 4)),
     }
     if not a.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
