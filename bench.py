@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--build-rows", type=int, default=1 << 24)
     ap.add_argument("--probe-rows", type=int, default=1 << 28)
     ap.add_argument("--theta", type=float, default=0.0)
-    ap.add_argument("--lf", type=float, default=0.5, help="join table load factor")
+    ap.add_argument("--lf", type=float, default=0.0, help="join table load factor (0: autotuned)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--serve-slots", type=int, default=3)
@@ -195,11 +195,17 @@ def bench_join(ctx, a):
     bk, pay, pk, pv = join_tables(nb, npr, theta=a.theta, device="cuda:0")
     bcol, pcol = Column(abi.KG_BIGINT, bk), Column(abi.KG_BIGINT, pk)
     out = {}
+    lf = a.lf
+    if lf == 0:
+        tune = ctx.join_tune(bcol, pcol)
+        lf = tune["load_factor"]
+        out["join_tuning"] = tune
+        log(f"join tuning: {tune}")
     steps = max(1, min(a.steps, 5))
     # build
     build_ms = []
     for i in range(steps + 1):
-        ht = HashTable(ctx, nb, load_factor=a.lf)
+        ht = HashTable(ctx, nb, load_factor=lf)
         torch.cuda.synchronize()
         ctx.set_timing(True)
         ctx.kernel_stats(reset=True)
@@ -211,7 +217,7 @@ def bench_join(ctx, a):
         if i < steps:
             ht.free()
     bms = sum(build_ms) / len(build_ms)
-    out["join_build"] = {"rows": nb, "load_factor": a.lf, "kernel_ms": round(bms, 3), "rows_per_s": nb / (bms * 1e-3),
+    out["join_build"] = {"rows": nb, "load_factor": lf, "kernel_ms": round(bms, 3), "rows_per_s": nb / (bms * 1e-3),
                          "roofline": roofline(nb * BUILD_BYTES_PER_ROW, bms, measured_traffic("k_ht_build"))}
     # probe (join cardinality)
     ht.probe_count(pcol)

@@ -127,6 +127,13 @@ int kg_ctx_synchronize(kg_ctx* ctx);
 /* 1 (default): FP64 reductions use a fixed order that depends only on n (and
  * is reproduced by the CPU oracle); 0: order may depend on timing. */
 int kg_ctx_set_deterministic(kg_ctx* ctx, int on);
+/* Join probes can first partition the probe keys by table region (histogram,
+ * scan, LDS-staged scatter) and then probe one region at a time per XCD, so
+ * each region's slice of the table is read from HBM once. regions: -1 or 0
+ * (default) probes directly, otherwise the region count (a multiple of 8, at
+ * most 1024), as kg_join_tune may choose. No counterpart in the reference (its
+ * probes are per tuple). */
+int kg_ctx_set_probe_regions(kg_ctx* ctx, int32_t regions);
 /* The FP64 addition order of the last kg_scan_group_agg call that used the
  * fixed-order kernels: T = *nblocks * 256 lanes; lane t folds, from +0.0, the
  * qualifying rows of chunks c = t + k*T (k ascending) of *vec consecutive rows
@@ -216,6 +223,31 @@ int kg_ht_probe_count(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key,
 int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, const int64_t* d_vals,
                     int64_t nrows, int64_t max_groups, int64_t* d_out_key, int64_t* d_out_sum,
                     int64_t* d_out_count, int32_t* d_out_first, int64_t* out_ngroups);
+/* ---- join autotuner ------------------------------------------------------------
+ * Picks the join table load factor, the probe workgroups per CU and the
+ * probe-key partition fan-out (kg_ctx_set_probe_regions) for a build/probe
+ * pair, per class of (build rows, build key cardinality, probe key skew): the
+ * classes are log2 of the build rows, the distinct fraction of a build-key
+ * sample (unique / few duplicates / many) and the share of the most frequent
+ * key in a probe-key sample (uniform / moderate / heavy). The first call for a
+ * class builds a table over the build keys (at most 64M rows) and times probes
+ * of a probe sample (at most 32M rows) under each candidate, by coordinate
+ * descent over load factor {0.35, 0.5, 0.65}, workgroups per CU {8, 16, 32}
+ * and regions {0, 64, 128}; later calls for the class reuse the choice. The
+ * probe settings are applied to the context; the load factor is for
+ * kg_ht_create. No counterpart in the reference (fixed std::unordered_map). */
+typedef struct kg_join_tuning {
+  double load_factor;
+  int32_t probe_wg_per_cu;
+  int32_t probe_regions;
+  double probe_ms;              /* probe time of the choice on the sample */
+  uint64_t tuning_class;        /* the (rows, cardinality, skew) class */
+  int32_t from_cache;           /* 1: an earlier call's choice */
+  int32_t trials;               /* configurations timed by this call */
+} kg_join_tuning;
+int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild, const kg_col* probe_key, int64_t nprobe,
+                 kg_join_tuning* out);
+
 /* Number of build rows inserted so far. */
 int kg_ht_size(kg_ctx* ctx, const kg_hash_table* ht, int64_t* out_rows);
 /* Release the table (runtime::HashTable::Free, hash_table.cc:53-59). */

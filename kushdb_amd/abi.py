@@ -81,6 +81,12 @@ class KgServeStats(ctypes.Structure):
                 ("batches", ctypes.c_int64), ("fallbacks", ctypes.c_int64)]
 
 
+class KgJoinTuning(ctypes.Structure):
+    _fields_ = [("load_factor", ctypes.c_double), ("probe_wg_per_cu", ctypes.c_int32),
+                ("probe_regions", ctypes.c_int32), ("probe_ms", ctypes.c_double), ("tuning_class", ctypes.c_uint64),
+                ("from_cache", ctypes.c_int32), ("trials", ctypes.c_int32)]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -96,6 +102,7 @@ SIGNATURES = {
     "kg_ctx_set_stream": (_i32, [_vp, _vp]),
     "kg_ctx_synchronize": (_i32, [_vp]),
     "kg_ctx_set_deterministic": (_i32, [_vp, _i32]),
+    "kg_ctx_set_probe_regions": (_i32, [_vp, _i32]),
     "kg_ctx_set_timing": (_i32, [_vp, _i32]),
     "kg_ctx_agg_order": (_i32, [_vp, _i64, _pi32, _pi32]),
     "kg_ctx_kernel_stats": (_i32, [_vp, ctypes.POINTER(ctypes.c_double), _pi64, _i32]),
@@ -110,6 +117,8 @@ SIGNATURES = {
     "kg_ht_probe_count": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64, _pi64]),
     "kg_ht_probe_agg": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64, _i64, _vp, _vp, _vp, _vp, _pi64]),
     "kg_ht_size": (_i32, [_vp, _vp, _pi64]),
+    "kg_join_tune": (_i32, [_vp, ctypes.POINTER(KgCol), _i64, ctypes.POINTER(KgCol), _i64,
+                            ctypes.POINTER(KgJoinTuning)]),
     "kg_ht_free": (_i32, [_vp, _vp]),
     "kg_scan_join_group_agg": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32, _vp,
                                       _i32, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgAgg), _i32, _i64, _pi64,

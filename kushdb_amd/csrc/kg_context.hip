@@ -168,4 +168,12 @@ int kg_ctx_set_deterministic(kg_ctx* ctx, int on) {
   return KG_OK;
 }
 
+int kg_ctx_set_probe_regions(kg_ctx* ctx, int32_t regions) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, regions == -1 || regions == 0 || (regions % 8 == 0 && regions <= 1024),
+             "probe regions must be -1, 0 or a multiple of 8 up to 1024 (got %d)", regions);
+  ctx->probe_regions = regions;
+  return KG_OK;
+}
+
 }  // extern "C"

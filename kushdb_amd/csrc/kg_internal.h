@@ -15,6 +15,18 @@ struct kg_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int deterministic = 1;
+  // Region count for partitioned join probes: -1 chooses by table size, 0 probes
+  // directly, otherwise a multiple of 8 (kg_ctx_set_probe_regions).
+  int probe_regions = -1;
+  int probe_wg_per_cu = 0;    // probe workgroups per CU; 0 = 32
+  // Join tuning choices per (build rows, key cardinality, skew) class.
+  struct JoinTune {
+    uint64_t cls;
+    double load_factor;
+    int32_t wg_per_cu, regions;
+    double probe_ms;
+  };
+  std::vector<JoinTune> join_tunes;
   // Rows per lane step of the last fixed-order aggregation (1 or 4): with
   // kg_agg_blocks_for(n) it defines that aggregation's FP64 addition order.
   int last_agg_vec = 1;

@@ -33,33 +33,48 @@ __global__ __launch_bounds__(kJBlock) void k_ht_build(TableView t, kg_col key, c
     int64_t k;
     if (!load_key(key, r, k)) continue;  // NULL build keys are skipped
     local++;
-    if (k == kEmpty) {
-      const unsigned long long pos = atomicAdd(t.side_count, 1ull);
-      if (pos < static_cast<unsigned long long>(kSideCap)) t.side_rows[pos] = static_cast<int32_t>(r);
-      else atomicOr(overflow, 1);
-      continue;
-    }
-    uint64_t b = bucket_of(k, t.nbuckets);
-    bool done = false;
-    for (uint64_t probes = 0; probes < t.nbuckets && !done; probes++) {
-      const int64_t* bk = t.keys + b * kSlots;
-      for (int s = 0; s < kSlots && !done; s++) {
-        if (bk[s] != kEmpty) continue;
-        const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(t.keys + b * kSlots + s),
-                                                  static_cast<unsigned long long>(kEmpty),
-                                                  static_cast<unsigned long long>(k));
-        if (prev == static_cast<unsigned long long>(kEmpty)) {
-          t.rows[b * kSlots + s] = static_cast<int32_t>(r);
-          done = true;
-        }
-      }
-      b = (b + 1 == t.nbuckets) ? 0 : b + 1;
-    }
-    if (!done) atomicOr(overflow, 1);
+    if (t.narrow) insert_narrow(t, k, static_cast<int32_t>(r), overflow);
+    else insert_wide(t, k, static_cast<int32_t>(r), overflow);
   }
   // One atomic per wave for the row count.
   for (int s = kWave / 2; s >= 1; s >>= 1) local += __shfl_xor(local, s);
   if ((threadIdx.x & (kWave - 1)) == 0 && local) atomicAdd(inserted, local);
+}
+
+// Minimum and maximum non-NULL key of the rows to insert (integer keys).
+__global__ __launch_bounds__(kJBlock) void k_key_range(kg_col key, const int32_t* sel, int64_t n,
+                                                        unsigned long long* out) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kJBlock;
+  int64_t lo = INT64_MAX, hi = INT64_MIN;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x; i < n; i += stride) {
+    const int64_t r = sel ? sel[i] : i;
+    int64_t k;
+    if (!load_key(key, r, k)) continue;
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+  }
+  for (int s = kWave / 2; s >= 1; s >>= 1) {
+    const int64_t a = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(lo), s));
+    const int64_t c = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(hi), s));
+    lo = a < lo ? a : lo;
+    hi = c > hi ? c : hi;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0 && lo <= hi) {
+    atomicMin(reinterpret_cast<long long*>(out), static_cast<long long>(lo));
+    atomicMax(reinterpret_cast<long long*>(out + 1), static_cast<long long>(hi));
+  }
+}
+
+// Re-inserts the occupied slots of a narrow table (copied to keys/rows) into
+// the wide table t.
+__global__ __launch_bounds__(kJBlock) void k_widen(TableView t, const uint32_t* keys, const int32_t* rows,
+                                                    int64_t nslots, int64_t base, int* overflow) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kJBlock;
+  for (int64_t s = static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x; s < nslots; s += stride) {
+    const uint32_t k32 = keys[s];
+    if (k32 == kEmpty32) continue;
+    insert_wide(t, static_cast<int64_t>(static_cast<uint64_t>(base) + k32), rows[s], overflow);
+  }
 }
 
 enum ProbeMode { kCount = 0, kEmit = 1, kAgg = 2 };
@@ -91,11 +106,12 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long* co
 
 // One probe (held by a 4-lane group) from its first bucket on. kv0 is this
 // lane's 16 bytes of the first bucket, already loaded by the caller so that
-// several probes' first loads are in flight together. Must be called by the
-// whole wave (ballots).
+// several probes' first loads are in flight together. r is the probe row (for
+// outputs), vi the index of its value (kAgg). Must be called by the whole wave
+// (ballots).
 template <int MODE, bool FIRST>
-__device__ __forceinline__ void probe_one(const TableView& t, const ProbeOut& o, int64_t r, int64_t k, bool active,
-                                          uint64_t b, longlong2 kv0, int sub, int grp,
+__device__ __forceinline__ void probe_one(const TableView& t, const ProbeOut& o, int64_t r, int64_t vi, int64_t k,
+                                          bool active, uint64_t b, longlong2 kv0, int sub, int grp,
                                           unsigned long long& local) {
   unsigned nmatch = 0;
   int64_t first_slot = -1;
@@ -169,7 +185,78 @@ __device__ __forceinline__ void probe_one(const TableView& t, const ProbeOut& o,
   }
   if (MODE == kAgg && sub == 0 && nmatch) {
     Acc* a = &o.acc[first_slot];
-    atomicAdd(&a->sum, static_cast<unsigned long long>(o.vals[r]) * nmatch);
+    atomicAdd(&a->sum, static_cast<unsigned long long>(o.vals[vi]) * nmatch);
+    atomicAdd(&a->count, static_cast<unsigned long long>(nmatch));
+    if (FIRST) atomicMin(&o.first[first_slot], static_cast<int32_t>(r));
+  }
+}
+
+// The same for a narrow table: each lane of the group holds 4 of the bucket's
+// 16 offsets (16 bytes); slot order within the bucket is lane-major.
+template <int MODE, bool FIRST>
+__device__ __forceinline__ void probe_one32(const TableView& t, const ProbeOut& o, int64_t r, int64_t vi, uint32_t k32,
+                                            bool active, uint64_t b, uint4 kv0, int sub, int grp,
+                                            unsigned long long& local) {
+  const uint32_t* keys = keys32(t);
+  const uint64_t nb = nbuckets32(t);
+  unsigned nmatch = 0;
+  int64_t first_slot = -1;
+  uint64_t scanned = 0;
+  bool first_bucket = true;
+  while (__any(active)) {
+    bool m0 = false, m1 = false, m2 = false, m3 = false, e = false;
+    int64_t slot0 = 0;
+    if (active) {
+      slot0 = static_cast<int64_t>(b) * kSlots32 + sub * 4;
+      const uint4 kv = first_bucket ? kv0 : *reinterpret_cast<const uint4*>(keys + slot0);
+      m0 = kv.x == k32;
+      m1 = kv.y == k32;
+      m2 = kv.z == k32;
+      m3 = kv.w == k32;
+      e = kv.x == kEmpty32 || kv.y == kEmpty32 || kv.z == kEmpty32 || kv.w == kEmpty32;
+    }
+    first_bucket = false;
+    const unsigned sh = grp * kProbeLanes;
+    const unsigned g0 = (__ballot(m0) >> sh) & 0xF, g1 = (__ballot(m1) >> sh) & 0xF;
+    const unsigned g2 = (__ballot(m2) >> sh) & 0xF, g3 = (__ballot(m3) >> sh) & 0xF;
+    const unsigned ge = (__ballot(e) >> sh) & 0xF;
+    if (MODE == kCount) {
+      if (active && sub == 0) local += __popc(g0) + __popc(g1) + __popc(g2) + __popc(g3);
+    } else if (MODE == kEmit) {
+      const unsigned want = (m0 ? 1u : 0u) + (m1 ? 1u : 0u) + (m2 ? 1u : 0u) + (m3 ? 1u : 0u);
+      const unsigned long long pos = wave_append(o.count, active ? want : 0u);
+      unsigned j = 0;
+      const bool mm[4] = {m0, m1, m2, m3};
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (!mm[q]) continue;
+        if (pos + j < static_cast<unsigned long long>(o.cap)) {
+          o.out_probe[pos + j] = static_cast<int32_t>(r);
+          o.out_build[pos + j] = t.rows[slot0 + q];
+        }
+        j++;
+      }
+    } else {
+      if (active) {
+        nmatch += __popc(g0) + __popc(g1) + __popc(g2) + __popc(g3);
+        if (first_slot < 0 && (g0 | g1 | g2 | g3)) {
+          unsigned bits = 0;
+          for (int q = 0; q < kProbeLanes; q++)
+            bits |= (((g0 >> q) & 1u) << (4 * q)) | (((g1 >> q) & 1u) << (4 * q + 1)) |
+                    (((g2 >> q) & 1u) << (4 * q + 2)) | (((g3 >> q) & 1u) << (4 * q + 3));
+          first_slot = static_cast<int64_t>(b) * kSlots32 + (__ffs(bits) - 1);
+        }
+      }
+    }
+    if (active) {
+      scanned++;
+      if (ge || scanned >= nb) active = false;
+      b = (b + 1 == nb) ? 0 : b + 1;
+    }
+  }
+  if (MODE == kAgg && sub == 0 && nmatch) {
+    Acc* a = &o.acc[first_slot];
+    atomicAdd(&a->sum, static_cast<unsigned long long>(o.vals[vi]) * nmatch);
     atomicAdd(&a->count, static_cast<unsigned long long>(nmatch));
     if (FIRST) atomicMin(&o.first[first_slot], static_cast<int32_t>(r));
   }
@@ -180,47 +267,249 @@ __device__ __forceinline__ void probe_one(const TableView& t, const ProbeOut& o,
 // wave keeps 16 * kUnroll random 64-byte reads in flight.
 constexpr int kUnroll = 4;
 
-template <int MODE, bool FIRST>
+constexpr int kXcds = 8;
+
+// Partitioned input (see k_part_scatter): probe keys grouped by table region,
+// regions laid out XCD-major; workgroup b (on XCD b mod 8) probes only its XCD's
+// regions, in order, so each region's slice of the table stays in that XCD's
+// L2 while its probes run.
+struct PartIn {
+  const uint32_t* offs;   // scanned histogram [P][nblk] (+ total), null: not partitioned
+  const int32_t* rowmap;  // probe row of each partitioned key (kEmit / kAgg first rows)
+  int32_t nblk;
+  int32_t P;
+};
+
+template <int MODE, bool FIRST, bool NARROW>
 __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, const int32_t* sel, int64_t n,
-                                                       ProbeOut o) {
+                                                       ProbeOut o, PartIn part) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sub = lane & (kProbeLanes - 1);
   const int grp = lane / kProbeLanes;
   constexpr int kGroups = kWave / kProbeLanes;
   constexpr int kStep = kGroups * kUnroll;
-  const int64_t gwave = (static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x) / kWave;
-  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * kJBlock / kWave;
+  int64_t lo = 0, hi = n, gwave, nwaves;
+  if (part.offs) {
+    const int x = blockIdx.x % kXcds;
+    const int per = part.P / kXcds;
+    lo = part.offs[static_cast<int64_t>(x) * per * part.nblk];
+    hi = part.offs[static_cast<int64_t>(x + 1) * per * part.nblk];
+    gwave = (static_cast<int64_t>(blockIdx.x / kXcds) * kJBlock + threadIdx.x) / kWave;
+    nwaves = static_cast<int64_t>(gridDim.x / kXcds) * kJBlock / kWave;
+  } else {
+    gwave = (static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x) / kWave;
+    nwaves = static_cast<int64_t>(gridDim.x) * kJBlock / kWave;
+  }
   unsigned long long local = 0;
-  for (int64_t base = gwave * kStep; base < n; base += nwaves * kStep) {
-    int64_t r[kUnroll], k[kUnroll];
-    bool active[kUnroll];
-    uint64_t b[kUnroll];
-    longlong2 kv[kUnroll];
+  // Keys of the next step are loaded while this step's buckets are in flight.
+  int64_t nr[kUnroll], nv[kUnroll], nk[kUnroll];
+  bool nact[kUnroll];
+  auto load_step = [&](int64_t base) {
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const int64_t i = base + u * kGroups + grp;
-      r[u] = -1;
-      k[u] = 0;
-      active[u] = false;
-      if (i < n) {
-        r[u] = sel ? sel[i] : i;
-        active[u] = load_key_nt(key, r[u], k[u]);
+      nr[u] = -1;
+      nv[u] = -1;
+      nk[u] = 0;
+      nact[u] = false;
+      if (i < hi) {
+        if (part.offs) {
+          nv[u] = i;
+          nr[u] = part.rowmap ? part.rowmap[i] : i;
+        } else {
+          nr[u] = sel ? sel[i] : i;
+          nv[u] = nr[u];
+        }
+        nact[u] = load_key_nt(key, part.offs ? i : nr[u], nk[u]);
       }
-      b[u] = active[u] ? bucket_of(k[u], t.nbuckets) : 0;
     }
+  };
+  int64_t base = lo + gwave * kStep;
+  if (base < hi) load_step(base);
+  for (; base < hi; base += nwaves * kStep) {
+    int64_t r[kUnroll], v[kUnroll], k[kUnroll];
+    bool active[kUnroll];
+    uint64_t b[kUnroll];
+    uint32_t k32[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
-      kv[u] = make_longlong2(kEmpty, kEmpty);
-      if (active[u] && k[u] != kEmpty)
-        kv[u] = *reinterpret_cast<const longlong2*>(t.keys + static_cast<int64_t>(b[u]) * kSlots + sub * 2);
+      r[u] = nr[u];
+      v[u] = nv[u];
+      k[u] = nk[u];
+      active[u] = nact[u];
+      k32[u] = 0;
+      // A key outside a narrow table's range cannot match: no memory access.
+      if (NARROW) active[u] = active[u] && narrow_key(t, k[u], k32[u]);
+      b[u] = active[u] ? bucket_of(k[u], NARROW ? nbuckets32(t) : t.nbuckets) : 0;
     }
+    if (NARROW) {
+      uint4 kv[kUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++)
-      probe_one<MODE, FIRST>(t, o, r[u], k[u], active[u], b[u], kv[u], sub, grp, local);
+      for (int u = 0; u < kUnroll; u++) {
+        kv[u] = make_uint4(kEmpty32, kEmpty32, kEmpty32, kEmpty32);
+        if (active[u])
+          kv[u] = *reinterpret_cast<const uint4*>(keys32(t) + static_cast<int64_t>(b[u]) * kSlots32 + sub * 4);
+      }
+      if (base + nwaves * kStep < hi) load_step(base + nwaves * kStep);
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++)
+        probe_one32<MODE, FIRST>(t, o, r[u], v[u], k32[u], active[u], b[u], kv[u], sub, grp, local);
+    } else {
+      longlong2 kv[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) {
+        kv[u] = make_longlong2(kEmpty, kEmpty);
+        if (active[u] && k[u] != kEmpty)
+          kv[u] = *reinterpret_cast<const longlong2*>(t.keys + static_cast<int64_t>(b[u]) * kSlots + sub * 2);
+      }
+      if (base + nwaves * kStep < hi) load_step(base + nwaves * kStep);
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++)
+        probe_one<MODE, FIRST>(t, o, r[u], v[u], k[u], active[u], b[u], kv[u], sub, grp, local);
+    }
   }
   if (MODE == kCount) {
     for (int s = kWave / 2; s >= 1; s >>= 1) local += __shfl_xor(local, s);
     if (lane == 0 && local) atomicAdd(o.count, local);
+  }
+}
+
+// ---- radix partitioning of probe keys by table region -------------------------
+// Region of a key = floor(bucket * P / nbuckets): regions are contiguous bucket
+// ranges, 1/P of the table each. Regions are stored XCD-major (region p at
+// position (p mod 8) * P/8 + p / 8) so that XCD x's regions are contiguous.
+// Keys that cannot match (NULL, NaN, outside a narrow table's range) are
+// dropped here. Each workgroup owns a contiguous range of input rows; the
+// histogram and the scatter walk it identically.
+constexpr int kPartTile = 2048;   // keys staged in LDS per scatter step
+
+__device__ __forceinline__ int key_region(const TableView& t, const kg_col& key, int64_t r, uint32_t P, int64_t& k) {
+  if (!load_key(key, r, k)) return -1;
+  uint64_t nb = t.nbuckets;
+  if (t.narrow) {
+    uint32_t k32;
+    if (!narrow_key(t, k, k32)) return -1;
+    nb = nbuckets32(t);
+  }
+  const uint32_t p = static_cast<uint32_t>((bucket_of(k, nb) * P) / nb);
+  return static_cast<int>((p % kXcds) * (P / kXcds) + p / kXcds);
+}
+
+__device__ __forceinline__ void block_range(int64_t n, int64_t& lo, int64_t& hi) {
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  lo = static_cast<int64_t>(blockIdx.x) * chunk;
+  hi = lo + chunk < n ? lo + chunk : n;
+  if (lo > n) lo = n;
+}
+
+__global__ __launch_bounds__(kJBlock) void k_part_hist(TableView t, kg_col key, const int32_t* sel, int64_t n,
+                                                        int P, uint32_t* hist) {
+  extern __shared__ uint32_t cnt[];
+  for (int q = threadIdx.x; q < P; q += kJBlock) cnt[q] = 0;
+  __syncthreads();
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kJBlock) {
+    int64_t k;
+    const int q = key_region(t, key, sel ? sel[i] : i, P, k);
+    if (q >= 0) atomicAdd(&cnt[q], 1u);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < P; q += kJBlock) hist[static_cast<int64_t>(q) * gridDim.x + blockIdx.x] = cnt[q];
+}
+
+// Exclusive scan of m counts in one workgroup of 1024 threads; out[m] = total.
+__global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t* in, uint32_t* out, int64_t m) {
+  __shared__ uint32_t part[1024];
+  const int64_t per = (m + 1023) / 1024;
+  const int64_t lo = threadIdx.x * per;
+  const int64_t hi = lo + per < m ? lo + per : m;
+  uint32_t sum = 0;
+  for (int64_t i = lo; i < hi; i++) sum += in[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int s = 1; s < 1024; s <<= 1) {
+    const uint32_t v = threadIdx.x >= s ? part[threadIdx.x - s] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - sum;
+  for (int64_t i = lo; i < hi; i++) {
+    const uint32_t c = in[i];
+    out[i] = run;
+    run += c;
+  }
+  if (threadIdx.x == 1023) out[m] = part[1023];
+}
+
+// Scatter: each tile of kPartTile rows is counting-sorted by region in LDS and
+// written out as one contiguous run per region (the workgroup's running cursor
+// per region starts at its scanned histogram offset).
+template <bool VALS, bool ROWS>
+__global__ __launch_bounds__(kJBlock) void k_part_scatter(TableView t, kg_col key, const int32_t* sel,
+                                                           const int64_t* vals, int64_t n, int P,
+                                                           const uint32_t* offs, int64_t* okeys, int64_t* ovals,
+                                                           int32_t* orows) {
+  extern __shared__ unsigned char smem[];
+  uint32_t* cursor = reinterpret_cast<uint32_t*>(smem);
+  uint32_t* lcnt = cursor + P;
+  uint32_t* loff = lcnt + P;
+  int64_t* tkeys = reinterpret_cast<int64_t*>(loff + P + (P & 1));
+  int64_t* tvals = tkeys + kPartTile;
+  int32_t* trows = reinterpret_cast<int32_t*>(tvals + (VALS ? kPartTile : 0));
+  uint16_t* treg = reinterpret_cast<uint16_t*>(trows + (ROWS ? kPartTile : 0));
+  constexpr int kPer = kPartTile / kJBlock;
+  for (int q = threadIdx.x; q < P; q += kJBlock) cursor[q] = offs[static_cast<int64_t>(q) * gridDim.x + blockIdx.x];
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  for (int64_t t0 = lo; t0 < hi; t0 += kPartTile) {
+    for (int q = threadIdx.x; q < P; q += kJBlock) lcnt[q] = 0;
+    __syncthreads();
+    int reg[kPer];
+    uint32_t rank[kPer];
+    int64_t kk[kPer], rr[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int64_t i = t0 + threadIdx.x + static_cast<int64_t>(j) * kJBlock;
+      reg[j] = -1;
+      rr[j] = 0;
+      if (i < hi) {
+        rr[j] = sel ? sel[i] : i;
+        reg[j] = key_region(t, key, rr[j], P, kk[j]);
+        if (reg[j] >= 0) rank[j] = atomicAdd(&lcnt[reg[j]], 1u);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t run = 0;
+      for (int q = 0; q < P; q++) {
+        loff[q] = run;
+        run += lcnt[q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (reg[j] < 0) continue;
+      const uint32_t slot = loff[reg[j]] + rank[j];
+      tkeys[slot] = kk[j];
+      if (VALS) tvals[slot] = vals[rr[j]];
+      if (ROWS) trows[slot] = static_cast<int32_t>(rr[j]);
+      treg[slot] = static_cast<uint16_t>(reg[j]);
+    }
+    __syncthreads();
+    const uint32_t total = loff[P - 1] + lcnt[P - 1];
+    for (uint32_t slot = threadIdx.x; slot < total; slot += kJBlock) {
+      const int q = treg[slot];
+      const uint32_t g = cursor[q] + (slot - loff[q]);
+      okeys[g] = tkeys[slot];
+      if (VALS) ovals[g] = tvals[slot];
+      if (ROWS) orows[g] = trows[slot];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < P; q += kJBlock) cursor[q] += lcnt[q];
+    __syncthreads();
   }
 }
 
@@ -243,7 +532,7 @@ __global__ void k_fill_i32j(int32_t* p, int32_t v, int64_t n) {
 // Each workgroup owns a contiguous range of slots: it counts its live slots,
 // takes one range of output positions with a single atomic, then writes them
 // in slot order (wave ballot + popcount prefix, wave offsets through LDS).
-__global__ __launch_bounds__(kJBlock) void k_collect(const int64_t* keys, const Acc* acc, const int32_t* first,
+__global__ __launch_bounds__(kJBlock) void k_collect(TableView t, const Acc* acc, const int32_t* first,
                                                       int64_t nslots, int64_t side_key_slot,
                                                       unsigned long long* count, int64_t cap,
                                                       int64_t* out_key, int64_t* out_sum, int64_t* out_cnt,
@@ -286,7 +575,8 @@ __global__ __launch_bounds__(kJBlock) void k_collect(const int64_t* keys, const 
     if (live) {
       const unsigned long long p = pos + before + __popcll(m & ((1ull << lane) - 1ull));
       if (p < static_cast<unsigned long long>(cap)) {
-        out_key[p] = s == side_key_slot ? kEmpty : keys[s];
+        out_key[p] = s == side_key_slot ? kEmpty
+                     : (t.narrow ? static_cast<int64_t>(static_cast<uint64_t>(t.base) + keys32(t)[s]) : t.keys[s]);
         out_sum[p] = static_cast<int64_t>(acc[s].sum);
         out_cnt[p] = static_cast<int64_t>(acc[s].count);
         if (out_first) out_first[p] = first[s];
@@ -414,26 +704,7 @@ __global__ __launch_bounds__(kJBlock) void k_join_group_agg(PredSet P, kg_col ke
         }
       }
     };
-    if (k == kEmpty) {
-      unsigned long long ns = *t.side_count;
-      if (ns > static_cast<unsigned long long>(kSideCap)) ns = kSideCap;
-      for (unsigned long long j = 0; j < ns; j++) consume(t.side_rows[j]);
-      continue;
-    }
-    uint64_t b = bucket_of(k, t.nbuckets);
-    for (uint64_t scanned = 0; scanned < t.nbuckets; scanned++) {
-      const longlong2* bk = reinterpret_cast<const longlong2*>(t.keys + b * kSlots);
-      bool empty = false;
-#pragma unroll
-      for (int q = 0; q < kSlots / 2; q++) {
-        const longlong2 kv = bk[q];
-        if (kv.x == k) consume(t.rows[b * kSlots + 2 * q]);
-        if (kv.y == k) consume(t.rows[b * kSlots + 2 * q + 1]);
-        empty |= kv.x == kEmpty || kv.y == kEmpty;
-      }
-      if (empty) break;
-      b = (b + 1 == t.nbuckets) ? 0 : b + 1;
-    }
+    table_matches(t, k, consume);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < G; i += kJBlock) {
@@ -503,9 +774,15 @@ struct kg_hash_table {
   Acc* acc = nullptr;          // lazily allocated for probe_agg
   int32_t* first = nullptr;    // lazily allocated for probe_agg with first rows
   int32_t* side_rows = nullptr;
-  unsigned long long* counters = nullptr;  // [0] inserted rows, [1] side count, [2] overflow flag, [3] scratch
+  unsigned long long* counters = nullptr;  // [0] inserted rows, [1] side count, [2] overflow flag, [3] scratch,
+                                           // [4..5] key range of a build
   int device = 0;
-  TableView view() const { return TableView{keys, rows, nbuckets, side_rows, counters + 1}; }
+  int form = -1;          // -1 until the first build with keys, then 0 wide / 1 narrow
+  int64_t base = 0;       // narrow: smallest key
+  TableView view() const {
+    TableView v{keys, rows, nbuckets, side_rows, counters + 1, form == 1 ? 1 : 0, 0, base};
+    return v;
+  }
 };
 
 namespace {
@@ -545,6 +822,73 @@ int read_counter(kg_ctx* ctx, const unsigned long long* d, unsigned long long* h
   return KG_OK;
 }
 
+// Converts a narrow table to the wide form, re-inserting its entries.
+int widen(kg_ctx* ctx, kg_hash_table* ht) {
+  const int64_t nslots = static_cast<int64_t>(ht->nbuckets) * kSlots;
+  uint32_t* tk = nullptr;
+  int32_t* tr = nullptr;
+  int st;
+  if ((st = alloc(ctx, reinterpret_cast<void**>(&tk), nslots * 4)) != KG_OK) return st;
+  if ((st = alloc(ctx, reinterpret_cast<void**>(&tr), nslots * 4)) != KG_OK) {
+    (void)hipFree(tk);
+    return st;
+  }
+  auto done = [&](int s2) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tk);
+    (void)hipFree(tr);
+    return s2;
+  };
+  if (hipMemcpyAsync(tk, ht->keys, nslots * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(tr, ht->rows, nslots * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+    return done(kg_fail(ctx, KG_ERR_HIP, "copy of the narrow table failed"));
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ctx, nslots, 4)), dim3(kJBlock), 0, ctx->stream, ht->keys, kEmpty,
+                     nslots);
+  ht->form = 0;
+  hipLaunchKernelGGL(k_widen, dim3(grid_for(ctx, nslots, 4)), dim3(kJBlock), 0, ctx->stream, ht->view(), tk, tr,
+                     nslots, ht->base, reinterpret_cast<int*>(ht->counters + 2));
+  if (hipGetLastError() != hipSuccess) return done(kg_fail(ctx, KG_ERR_HIP, "widen launch failed"));
+  return done(KG_OK);
+}
+
+// Picks the table's form before inserting a build batch: the first batch with
+// integer keys whose range spans at most 2^32 - 2 makes the table narrow; a
+// later batch outside that range (or a REAL key) turns it wide.
+int choose_form(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t* d_rows, int64_t nrows) {
+  if (ht->form == 0) return KG_OK;
+  if (key->type == KG_REAL) return ht->form == 1 ? widen(ctx, ht) : (ht->form = 0, KG_OK);
+  unsigned long long* rng = ht->counters + 4;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, rng, static_cast<unsigned long long>(INT64_MAX),
+                     1);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, rng + 1,
+                     static_cast<unsigned long long>(INT64_MIN), 1);
+  hipLaunchKernelGGL(k_key_range, dim3(grid_for(ctx, nrows, 16)), dim3(kJBlock), 0, ctx->stream, *key, d_rows, nrows,
+                     rng);
+  KG_LAUNCH_CHECK(ctx);
+  int st = kg_pinned_reserve(ctx, 64);
+  if (st != KG_OK) return st;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, rng, 16, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  const int64_t lo = static_cast<const int64_t*>(ctx->pinned)[0], hi = static_cast<const int64_t*>(ctx->pinned)[1];
+  if (lo > hi) return KG_OK;  // no non-NULL keys in this batch
+  const uint64_t limit = static_cast<uint64_t>(kEmpty32) - 1;
+  if (ht->form == -1) {
+    if (static_cast<uint64_t>(hi) - static_cast<uint64_t>(lo) <= limit) {
+      ht->form = 1;
+      ht->base = lo;
+      const int64_t words = static_cast<int64_t>(ht->nbuckets) * kSlots / 2;  // 32-bit slots / 2
+      hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, words, 4)), dim3(kJBlock), 0, ctx->stream,
+                         reinterpret_cast<unsigned long long*>(ht->keys), ~0ull, words);
+      KG_LAUNCH_CHECK(ctx);
+    } else {
+      ht->form = 0;
+    }
+    return KG_OK;
+  }
+  if (lo < ht->base || static_cast<uint64_t>(hi) - static_cast<uint64_t>(ht->base) > limit) return widen(ctx, ht);
+  return KG_OK;
+}
+
 }  // namespace
 
 extern "C" int kg_ht_create(kg_ctx* ctx, int64_t capacity_rows, double load_factor, kg_hash_table** out) {
@@ -561,6 +905,7 @@ extern "C" int kg_ht_create(kg_ctx* ctx, int64_t capacity_rows, double load_fact
   t->capacity = capacity_rows;
   const double slots = std::max<double>(static_cast<double>(capacity_rows) / load_factor, kSlots);
   t->nbuckets = static_cast<uint64_t>(ceil_div(static_cast<int64_t>(slots), kSlots));
+  t->nbuckets += t->nbuckets & 1;  // even: the narrow form has nbuckets / 2 buckets of 16 slots
   const size_t nslots = t->nbuckets * kSlots;
   int st;
   if ((st = alloc(ctx, reinterpret_cast<void**>(&t->keys), nslots * 8)) != KG_OK ||
@@ -592,6 +937,7 @@ extern "C" int kg_ht_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, co
   if ((st = read_counter(ctx, ht->counters, &inserted)) != KG_OK) return st;
   if (static_cast<int64_t>(inserted) + nrows > ht->capacity)
     return kg_fail(ctx, KG_ERR_CAPACITY, "table capacity %lld rows exceeded", (long long)ht->capacity);
+  if ((st = choose_form(ctx, ht, key, d_rows, nrows)) != KG_OK) return st;
   kg_time_begin(ctx);
   hipLaunchKernelGGL(k_ht_build, dim3(grid_for(ctx, nrows, 4)), dim3(kJBlock), 0, ctx->stream, ht->view(), *key,
                      d_rows, nrows, ht->counters, reinterpret_cast<int*>(ht->counters + 2));
@@ -604,6 +950,94 @@ extern "C" int kg_ht_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, co
 }
 
 namespace {
+
+// Probe-key partitioning: the region count (0 = probe directly). Only an
+// explicit setting (kg_ctx_set_probe_regions, e.g. from kg_join_tune)
+// partitions: measured on MI355X, the histogram + scatter passes cost more than
+// the L2 locality they buy for a 16M-key table probed 256M times (6.3 ms with
+// 128 regions against 4.8 ms direct), so the default is direct.
+int probe_regions(const kg_ctx* ctx) { return ctx->probe_regions > 0 ? ctx->probe_regions : 0; }
+
+// Probe workgroups per CU (kg_join_tune), 32 by default.
+int probe_grid(kg_ctx* ctx, int64_t n) {
+  const int per_cu = ctx->probe_wg_per_cu > 0 ? ctx->probe_wg_per_cu : 32;
+  const int64_t g = ceil_div(std::max<int64_t>(n, 1), static_cast<int64_t>(kJBlock) * 64);
+  return static_cast<int>(std::min<int64_t>(g, static_cast<int64_t>(ctx->num_cus) * per_cu));
+}
+
+template <int MODE, bool FIRST>
+void launch_probe_kernel(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const int32_t* sel, int64_t n,
+                         const ProbeOut& o, const PartIn& part, int grid) {
+  if (ht->form == 1)
+    hipLaunchKernelGGL((k_ht_probe<MODE, FIRST, true>), dim3(grid), dim3(kJBlock), 0, ctx->stream, ht->view(), key,
+                       sel, n, o, part);
+  else
+    hipLaunchKernelGGL((k_ht_probe<MODE, FIRST, false>), dim3(grid), dim3(kJBlock), 0, ctx->stream, ht->view(), key,
+                       sel, n, o, part);
+}
+
+// Probes n keys (rows sel[i] or i), directly or through a region partition of
+// the keys (histogram, scan, scatter, then an XCD-ranged probe). The whole
+// sequence is the timed region.
+template <int MODE, bool FIRST>
+int launch_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const int32_t* sel, int64_t n,
+                 ProbeOut o) {
+  const int P = probe_regions(ctx);
+  if (P == 0) {
+    kg_time_begin(ctx);
+    launch_probe_kernel<MODE, FIRST>(ctx, ht, key, sel, n, o, PartIn{}, probe_grid(ctx, n));
+    kg_time_end(ctx);
+    KG_LAUNCH_CHECK(ctx);
+    return KG_OK;
+  }
+  KG_REQUIRE(ctx, P % kXcds == 0 && P <= 65536, "region count %d must be a multiple of %d", P, kXcds);
+  const bool vals = MODE == kAgg, rows = MODE == kEmit || (MODE == kAgg && FIRST);
+  const int nblk = static_cast<int>(std::min<int64_t>(2048, std::max<int64_t>(kXcds, ceil_div(n, 1 << 16))));
+  const int64_t m = static_cast<int64_t>(P) * nblk;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o2 = off;
+    off += (bytes + 255) / 256 * 256;
+    return o2;
+  };
+  const size_t o_hist = carve(m * 4), o_offs = carve((m + 1) * 4), o_keys = carve(n * 8),
+               o_vals = carve(vals ? n * 8 : 8), o_rows = carve(rows ? n * 4 : 8);
+  int st = kg_work_reserve(ctx, off);
+  if (st != KG_OK) return st;
+  char* w = static_cast<char*>(ctx->work);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(w + o_hist);
+  uint32_t* offs = reinterpret_cast<uint32_t*>(w + o_offs);
+  int64_t* pkeys = reinterpret_cast<int64_t*>(w + o_keys);
+  int64_t* pvals = reinterpret_cast<int64_t*>(w + o_vals);
+  int32_t* prows = reinterpret_cast<int32_t*>(w + o_rows);
+  const size_t scatter_lds = static_cast<size_t>(3 * P + 1) * 4 + 8 + kPartTile * 8 + (vals ? kPartTile * 8 : 0) +
+                             (rows ? kPartTile * 4 : 0) + kPartTile * 2;
+  KG_REQUIRE(ctx, scatter_lds <= 64 * 1024, "region count %d too large", P);
+  kg_time_begin(ctx);
+  hipLaunchKernelGGL(k_part_hist, dim3(nblk), dim3(kJBlock), P * 4, ctx->stream, ht->view(), key, sel, n, P, hist);
+  hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, ctx->stream, hist, offs, m);
+  if (vals && rows)
+    hipLaunchKernelGGL((k_part_scatter<true, true>), dim3(nblk), dim3(kJBlock), scatter_lds, ctx->stream, ht->view(),
+                       key, sel, o.vals, n, P, offs, pkeys, pvals, prows);
+  else if (vals)
+    hipLaunchKernelGGL((k_part_scatter<true, false>), dim3(nblk), dim3(kJBlock), scatter_lds, ctx->stream,
+                       ht->view(), key, sel, o.vals, n, P, offs, pkeys, pvals, prows);
+  else if (rows)
+    hipLaunchKernelGGL((k_part_scatter<false, true>), dim3(nblk), dim3(kJBlock), scatter_lds, ctx->stream,
+                       ht->view(), key, sel, o.vals, n, P, offs, pkeys, pvals, prows);
+  else
+    hipLaunchKernelGGL((k_part_scatter<false, false>), dim3(nblk), dim3(kJBlock), scatter_lds, ctx->stream,
+                       ht->view(), key, sel, o.vals, n, P, offs, pkeys, pvals, prows);
+  kg_col pk{KG_BIGINT, 0, pkeys, nullptr};
+  ProbeOut po = o;
+  if (vals) po.vals = pvals;
+  PartIn part{offs, rows ? prows : nullptr, nblk, P};
+  const int grid = (probe_grid(ctx, n) + kXcds - 1) / kXcds * kXcds;
+  launch_probe_kernel<MODE, FIRST>(ctx, ht, pk, nullptr, n, po, part, grid);
+  kg_time_end(ctx);
+  KG_LAUNCH_CHECK(ctx);
+  return KG_OK;
+}
 
 int probe_common(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, int64_t nrows) {
   if (!ctx) return KG_ERR_INVALID;
@@ -629,11 +1063,7 @@ extern "C" int kg_ht_probe_count(kg_ctx* ctx, const kg_hash_table* ht, const kg_
   KG_LAUNCH_CHECK(ctx);
   ProbeOut o{};
   o.count = cnt;
-  kg_time_begin(ctx);
-  hipLaunchKernelGGL((k_ht_probe<kCount, false>), dim3(grid_for(ctx, nrows, 64)), dim3(kJBlock), 0, ctx->stream,
-                     ht->view(), *key, d_rows, nrows, o);
-  kg_time_end(ctx);
-  KG_LAUNCH_CHECK(ctx);
+  if ((st = launch_probe<kCount, false>(ctx, ht, *key, d_rows, nrows, o)) != KG_OK) return st;
   unsigned long long h = 0;
   if ((st = read_counter(ctx, cnt, &h)) != KG_OK) return st;
   *out_count = static_cast<int64_t>(h);
@@ -658,11 +1088,7 @@ extern "C" int kg_ht_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* k
   o.out_probe = d_out_probe;
   o.out_build = d_out_build;
   o.cap = out_cap;
-  kg_time_begin(ctx);
-  hipLaunchKernelGGL((k_ht_probe<kEmit, false>), dim3(grid_for(ctx, nrows, 64)), dim3(kJBlock), 0, ctx->stream,
-                     ht->view(), *key, d_rows, nrows, o);
-  kg_time_end(ctx);
-  KG_LAUNCH_CHECK(ctx);
+  if ((st = launch_probe<kEmit, false>(ctx, ht, *key, d_rows, nrows, o)) != KG_OK) return st;
   unsigned long long h = 0;
   if ((st = read_counter(ctx, cnt, &h)) != KG_OK) return st;
   *out_count = static_cast<int64_t>(h);
@@ -699,20 +1125,14 @@ extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_co
     o.acc = t->acc;
     o.first = t->first;
     o.vals = d_vals;
-    kg_time_begin(ctx);
-    if (d_out_first)
-      hipLaunchKernelGGL((k_ht_probe<kAgg, true>), dim3(grid_for(ctx, nrows, 64)), dim3(kJBlock), 0, ctx->stream,
-                         t->view(), *key, nullptr, nrows, o);
-    else
-      hipLaunchKernelGGL((k_ht_probe<kAgg, false>), dim3(grid_for(ctx, nrows, 64)), dim3(kJBlock), 0, ctx->stream,
-                         t->view(), *key, nullptr, nrows, o);
-    kg_time_end(ctx);
-    KG_LAUNCH_CHECK(ctx);
+    st = d_out_first ? launch_probe<kAgg, true>(ctx, t, *key, nullptr, nrows, o)
+                     : launch_probe<kAgg, false>(ctx, t, *key, nullptr, nrows, o);
+    if (st != KG_OK) return st;
   }
   unsigned long long* cnt = t->counters + 3;
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, cnt, 0ull, 1);
   KG_LAUNCH_CHECK(ctx);
-  hipLaunchKernelGGL(k_collect, dim3(grid_for(ctx, nslots + 1, 4)), dim3(kJBlock), 0, ctx->stream, t->keys, t->acc,
+  hipLaunchKernelGGL(k_collect, dim3(grid_for(ctx, nslots + 1, 4)), dim3(kJBlock), 0, ctx->stream, t->view(), t->acc,
                      t->first, nslots, nslots, cnt, max_groups, d_out_key, d_out_sum, d_out_count, d_out_first);
   KG_LAUNCH_CHECK(ctx);
   unsigned long long h = 0;
@@ -798,7 +1218,9 @@ extern "C" int kg_group_by_i64(kg_ctx* ctx, const int64_t* d_keys, const int64_t
     kg_time_end(ctx);
     KG_LAUNCH_CHECK(ctx);
   }
-  hipLaunchKernelGGL(k_collect, dim3(grid_for(ctx, nslots + 1, 4)), dim3(kJBlock), 0, ctx->stream, keys, acc, first,
+  TableView gv{};
+  gv.keys = keys;
+  hipLaunchKernelGGL(k_collect, dim3(grid_for(ctx, nslots + 1, 4)), dim3(kJBlock), 0, ctx->stream, gv, acc, first,
                      nslots, nslots, cnt, max_groups, d_out_key, d_out_sum, d_out_count, d_out_first);
   KG_LAUNCH_CHECK(ctx);
   unsigned long long h = 0, of = 0;
@@ -902,4 +1324,191 @@ extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols
   for (int g = 0; g < G; g++) first64[g] = static_cast<int64_t>(firstu[g]);
   return emit_groups(ctx, G, build_codes ? 1 : 0, 0, M, plan, fin, rowsv, first64, first_null, max_groups,
                      out_ngroups, out_keys, out_vals, out_nulls);
+}
+
+// ---- join autotuner -------------------------------------------------------------
+
+namespace {
+
+// Copies up to `cap` keys (rows 0..) of a column to the host as int64 (NULL /
+// NaN keys dropped) for the class statistics.
+int sample_keys(kg_ctx* ctx, const kg_col* key, int64_t n, int64_t cap, std::vector<int64_t>& out) {
+  out.clear();
+  const int64_t m = std::min(n, cap);
+  if (m <= 0) return KG_OK;
+  const int w = type_width(key->type);
+  std::vector<unsigned char> raw(static_cast<size_t>(m) * w);
+  std::vector<int8_t> nul;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(raw.data(), key->data, raw.size(), hipMemcpyDeviceToHost, ctx->stream));
+  if (key->nulls) {
+    nul.resize(m);
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(nul.data(), key->nulls, m, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  out.reserve(m);
+  for (int64_t i = 0; i < m; i++) {
+    if (!nul.empty() && nul[i]) continue;
+    const unsigned char* p = raw.data() + i * w;
+    int64_t v = 0;
+    switch (key->type) {
+      case KG_SMALLINT: v = *reinterpret_cast<const int16_t*>(p); break;
+      case KG_BOOLEAN: v = *reinterpret_cast<const int8_t*>(p); break;
+      case KG_BIGINT: v = *reinterpret_cast<const int64_t*>(p); break;
+      case KG_REAL: {
+        const double d = *reinterpret_cast<const double*>(p);
+        if (d != d) continue;
+        std::memcpy(&v, &d, 8);
+        break;
+      }
+      default: v = *reinterpret_cast<const int32_t*>(p); break;
+    }
+    out.push_back(v);
+  }
+  return KG_OK;
+}
+
+// The tuning class: log2 build rows (6 bits), distinct fraction class of the
+// build sample (0 unique, 1 few duplicates, 2 many), top-key share class of the
+// probe sample (0 < 0.1%, 1 < 5%, 2 heavier).
+uint64_t tuning_class(int64_t nbuild, std::vector<int64_t> b, std::vector<int64_t> p) {
+  int lg = 0;
+  while ((int64_t{1} << lg) < nbuild && lg < 63) lg++;
+  int card = 0;
+  if (!b.empty()) {
+    std::sort(b.begin(), b.end());
+    const double d = static_cast<double>(std::unique(b.begin(), b.end()) - b.begin()) / b.size();
+    card = d >= 0.99 ? 0 : (d >= 0.5 ? 1 : 2);
+  }
+  int skew = 0;
+  if (!p.empty()) {
+    std::sort(p.begin(), p.end());
+    int64_t best = 0, run = 0;
+    for (size_t i = 0; i < p.size(); i++) {
+      run = (i > 0 && p[i] == p[i - 1]) ? run + 1 : 1;
+      best = std::max(best, run);
+    }
+    const double top = static_cast<double>(best) / p.size();
+    skew = top < 0.001 ? 0 : (top < 0.05 ? 1 : 2);
+  }
+  return (static_cast<uint64_t>(lg) << 8) | (static_cast<uint64_t>(card) << 4) | static_cast<uint64_t>(skew);
+}
+
+// Mean time of `reps` probe_count calls (after one warm call) in ms.
+int time_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, int64_t n, int reps, double* ms) {
+  int64_t cnt = 0;
+  int st = kg_ht_probe_count(ctx, ht, key, nullptr, n, &cnt);
+  if (st != KG_OK) return st;
+  hipEvent_t a, b;
+  KG_HIP_CHECK(ctx, hipEventCreate(&a));
+  KG_HIP_CHECK(ctx, hipEventCreate(&b));
+  const int timing = ctx->timing;
+  ctx->timing = 0;
+  (void)hipEventRecord(a, ctx->stream);
+  for (int i = 0; i < reps && st == KG_OK; i++) st = kg_ht_probe_count(ctx, ht, key, nullptr, n, &cnt);
+  (void)hipEventRecord(b, ctx->stream);
+  (void)hipEventSynchronize(b);
+  float t = 0;
+  (void)hipEventElapsedTime(&t, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  ctx->timing = timing;
+  *ms = t / reps;
+  return st;
+}
+
+}  // namespace
+
+extern "C" int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild, const kg_col* probe_key,
+                            int64_t nprobe, kg_join_tuning* out) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, out && build_key && probe_key, "null argument");
+  KG_REQUIRE(ctx, nbuild >= 0 && nbuild < INT32_MAX && nprobe >= 0 && nprobe < INT32_MAX, "row count out of range");
+  int st;
+  if ((st = check_key(ctx, build_key, nbuild)) != KG_OK || (st = check_key(ctx, probe_key, nprobe)) != KG_OK)
+    return st;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  std::memset(out, 0, sizeof(*out));
+  std::vector<int64_t> bs, ps;
+  if ((st = sample_keys(ctx, build_key, nbuild, 1 << 16, bs)) != KG_OK) return st;
+  if ((st = sample_keys(ctx, probe_key, nprobe, 1 << 16, ps)) != KG_OK) return st;
+  const uint64_t cls = tuning_class(nbuild, bs, ps);
+  out->tuning_class = cls;
+  for (const auto& t : ctx->join_tunes)
+    if (t.cls == cls) {
+      out->load_factor = t.load_factor;
+      out->probe_wg_per_cu = t.wg_per_cu;
+      out->probe_regions = t.regions;
+      out->probe_ms = t.probe_ms;
+      out->from_cache = 1;
+      ctx->probe_wg_per_cu = t.wg_per_cu;
+      ctx->probe_regions = t.regions;
+      return KG_OK;
+    }
+  const int64_t nb = std::min<int64_t>(nbuild, int64_t{1} << 26);
+  const int64_t np = std::min<int64_t>(nprobe, int64_t{1} << 25);
+  double best_lf = 0.5, best_ms = 1e30;
+  int best_wg = 32, best_regions = 0, trials = 0;
+  kg_hash_table* keep = nullptr;
+  const int saved_regions = ctx->probe_regions, saved_wg = ctx->probe_wg_per_cu;
+  auto restore = [&](int s2) {
+    if (keep) kg_ht_free(ctx, keep);
+    ctx->probe_regions = saved_regions;
+    ctx->probe_wg_per_cu = saved_wg;
+    return s2;
+  };
+  // 1. load factor (direct probes, 32 workgroups per CU)
+  ctx->probe_regions = 0;
+  ctx->probe_wg_per_cu = 32;
+  for (double lf : {0.35, 0.5, 0.65}) {
+    kg_hash_table* ht = nullptr;
+    if ((st = kg_ht_create(ctx, std::max<int64_t>(nb, 1), lf, &ht)) != KG_OK) return restore(st);
+    st = kg_ht_build(ctx, ht, build_key, nullptr, nb);
+    double ms = 0;
+    if (st == KG_OK) st = time_probe(ctx, ht, probe_key, np, 2, &ms);
+    if (st != KG_OK) {
+      kg_ht_free(ctx, ht);
+      return restore(st);
+    }
+    trials++;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_lf = lf;
+      if (keep) kg_ht_free(ctx, keep);
+      keep = ht;
+    } else {
+      kg_ht_free(ctx, ht);
+    }
+  }
+  // 2. workgroups per CU, then 3. regions, on the best table
+  for (int wg : {8, 16}) {
+    ctx->probe_wg_per_cu = wg;
+    double ms = 0;
+    if ((st = time_probe(ctx, keep, probe_key, np, 2, &ms)) != KG_OK) return restore(st);
+    trials++;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_wg = wg;
+    }
+  }
+  ctx->probe_wg_per_cu = best_wg;
+  for (int reg : {64, 128}) {
+    ctx->probe_regions = reg;
+    double ms = 0;
+    if ((st = time_probe(ctx, keep, probe_key, np, 2, &ms)) != KG_OK) return restore(st);
+    trials++;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_regions = reg;
+    }
+  }
+  restore(KG_OK);
+  ctx->probe_wg_per_cu = best_wg;
+  ctx->probe_regions = best_regions;
+  ctx->join_tunes.push_back({cls, best_lf, best_wg, best_regions, best_ms});
+  out->load_factor = best_lf;
+  out->probe_wg_per_cu = best_wg;
+  out->probe_regions = best_regions;
+  out->probe_ms = best_ms;
+  out->trials = trials;
+  return KG_OK;
 }

@@ -84,6 +84,19 @@ class Context:
     def synchronize(self):
         self.check(self.lib.kg_ctx_synchronize(self.h))
 
+    def set_probe_regions(self, regions):
+        """Region count of partitioned join probes (-1 auto, 0 direct)."""
+        self.check(self.lib.kg_ctx_set_probe_regions(self.h, regions))
+
+    def join_tune(self, build_key, probe_key):
+        """Autotuned join settings for this build/probe pair (applied to the
+        context); returns a dict with the load factor to create the table with."""
+        t = abi.KgJoinTuning()
+        bd, pd = build_key.desc(), probe_key.desc()
+        self.check(self.lib.kg_join_tune(self.h, ctypes.byref(bd), len(build_key), ctypes.byref(pd), len(probe_key),
+                                         ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in abi.KgJoinTuning._fields_}
+
     def set_timing(self, on):
         self.check(self.lib.kg_ctx_set_timing(self.h, 1 if on else 0))
 

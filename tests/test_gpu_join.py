@@ -158,3 +158,115 @@ def test_full_size_join_properties(ctx):
     assert int(s.sum().item()) == int(pv[hit].sum().item())
     assert len(k) == int(torch.unique(pk[hit]).numel())
     ht.free()
+
+
+@pytest.mark.parametrize("base", [0, 1 << 40, -(1 << 62)])
+def test_join_narrow_table_and_widening(ctx, base):
+    """Build keys spanning less than 2^32 (any base) use the 32-bit-offset table;
+    a second build batch far outside that range converts it to the wide form.
+    Pairs, counts and the fused probe + GROUP BY match the oracle before and
+    after the conversion; probe keys just outside the range miss."""
+    rng = np.random.default_rng(77)
+    b1 = (base + rng.permutation(20_000)).astype(np.int64)
+    p = np.concatenate([base + rng.integers(-100, 20_100, 50_000), [base - 1, base + (1 << 32)]]).astype(np.int64)
+    v = rng.integers(0, 1 << 31, len(p)).astype(np.int64)
+    ht = HashTable(ctx, 40_000)
+    ht.insert(Column(abi.KG_BIGINT, torch.as_tensor(b1).to(DEV)))
+    pc = Column(abi.KG_BIGINT, torch.as_tensor(p).to(DEV))
+
+    def check(build):
+        gp, gb = ht.probe(pc)
+        op, ob = po.hash_join([po.Col(po.BIGINT, build)], [po.Col(po.BIGINT, p)])
+        assert ht.probe_count(pc) == len(op) == len(gp)
+        assert np.array_equal(sorted_pairs(gp.cpu().numpy(), gb.cpu().numpy()), sorted_pairs(op, ob))
+        k, s, c, _ = ht.probe_agg(pc, torch.as_tensor(v).to(DEV), max_groups=len(build))
+        k, s, c = (x.cpu().numpy() for x in (k, s, c))
+        order = np.argsort(k)
+        k, s, c = k[order], s[order], c[order]
+        wk, inv = np.unique(build[ob], return_inverse=True)
+        wc = np.bincount(inv, minlength=len(wk))
+        ws = np.zeros(len(wk), dtype=np.int64)
+        np.add.at(ws, inv, v[op])
+        assert np.array_equal(k, wk) and np.array_equal(c, wc) and np.array_equal(s, ws)
+
+    check(b1)
+    b2 = np.array([base + (1 << 33) + i for i in range(1000)] + [np.iinfo(np.int64).min], dtype=np.int64)
+    ht.insert(Column(abi.KG_BIGINT, torch.as_tensor(b2).to(DEV)))
+    p2 = np.concatenate([p, b2[:500], [np.iinfo(np.int64).min]]).astype(np.int64)
+    v = rng.integers(0, 1 << 31, len(p2)).astype(np.int64)
+    pc = Column(abi.KG_BIGINT, torch.as_tensor(p2).to(DEV))
+    p = p2
+    allb = np.concatenate([b1, b2])
+    # build row ids of the second batch restart at 0 (rows of its own column)
+    gp, gb = ht.probe(pc)
+    pairs = set(zip(gp.cpu().numpy().tolist(), gb.cpu().numpy().tolist()))
+    want = set()
+    for i, key in enumerate(p2.tolist()):
+        for j in np.nonzero(b1 == key)[0].tolist():
+            want.add((i, j))
+        for j in np.nonzero(b2 == key)[0].tolist():
+            want.add((i, j))
+    assert pairs == want
+    assert ht.size() == len(allb)
+    ht.free()
+
+
+@pytest.mark.parametrize("regions", [8, 64])
+@pytest.mark.parametrize("wide", [False, True])
+def test_partitioned_probe_matches_direct(ctx, regions, wide):
+    """Probing through a region partition of the probe keys (forced here; chosen
+    by table size otherwise) gives the same pairs, counts and grouped sums as
+    the direct probe, on narrow and wide tables, with NULL probe keys."""
+    rng = np.random.default_rng(regions + (7 if wide else 0))
+    nb, npr = 50_000, 400_000
+    b = rng.permutation(nb).astype(np.int64) * 3
+    if wide:
+        b[:5] = np.iinfo(np.int64).min      # wide form: the free-slot marker value as a key
+        b[5] = np.iinfo(np.int64).max
+    p = rng.integers(-10, 3 * nb + 10, npr).astype(np.int64)
+    p[:11] = np.iinfo(np.int64).min
+    pn = (rng.random(npr) < 0.05).astype(np.int8)
+    v = rng.integers(0, 1 << 31, npr).astype(np.int64)
+    ht = HashTable(ctx, nb)
+    ht.insert(Column(abi.KG_BIGINT, torch.as_tensor(b).to(DEV)))
+    pc = Column(abi.KG_BIGINT, torch.as_tensor(p).to(DEV), torch.as_tensor(pn).to(DEV))
+    vd = torch.as_tensor(v).to(DEV)
+    res = {}
+    for reg in (0, regions):
+        ctx.set_probe_regions(reg)
+        gp, gb = ht.probe(pc)
+        k, s, c, f = ht.probe_agg(pc, vd, max_groups=nb, with_first=True)
+        order = np.argsort(k.cpu().numpy())
+        res[reg] = (ht.probe_count(pc), sorted_pairs(gp.cpu().numpy(), gb.cpu().numpy()),
+                    *(x.cpu().numpy()[order] for x in (k, s, c, f)))
+    ctx.set_probe_regions(-1)
+    a, d = res[0], res[regions]
+    assert a[0] == d[0] == len(a[1]) > 0
+    for x, y in zip(a[1:], d[1:]):
+        assert np.array_equal(x, y)
+    op, ob = po.hash_join([po.Col(po.BIGINT, b)], [po.Col(po.BIGINT, p, pn)])
+    assert np.array_equal(a[1], sorted_pairs(op, ob))
+    ht.free()
+
+
+def test_join_tune_and_cache(ctx):
+    """The join tuner returns a valid configuration, applies it, reuses it for
+    the same (rows, cardinality, skew) class, and probes under it stay exact."""
+    bk, pay, pk, pv = join_tables(200_000, 4_000_000, theta=0.0, device=DEV, seed=5)
+    bc, pc = Column(abi.KG_BIGINT, bk), Column(abi.KG_BIGINT, pk)
+    t = ctx.join_tune(bc, pc)
+    assert t["load_factor"] in (0.35, 0.5, 0.65)
+    assert t["probe_wg_per_cu"] in (8, 16, 32) and t["probe_regions"] in (0, 64, 128)
+    assert t["from_cache"] == 0 and t["trials"] == 7 and t["probe_ms"] > 0
+    again = ctx.join_tune(bc, pc)
+    assert again["from_cache"] == 1 and again["tuning_class"] == t["tuning_class"]
+    ht = HashTable(ctx, 200_000, load_factor=t["load_factor"])
+    ht.insert(bc)
+    hp = pk.cpu().numpy()
+    assert ht.probe_count(pc) == int((hp < 200_000).sum())
+    ht.free()
+    # a heavily skewed probe side is another class
+    skewed = torch.zeros_like(pk)
+    s = ctx.join_tune(bc, Column(abi.KG_BIGINT, skewed))
+    assert s["tuning_class"] != t["tuning_class"] and s["from_cache"] == 0
+    ctx.set_probe_regions(0)
