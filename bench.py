@@ -252,6 +252,82 @@ def bench_join(ctx, a):
     return out
 
 
+C0_BYTES = 1_000_000 * (4 + 4 + 4 + 8) + 10_000 * (4 + 4)   # fk, day, qty, price; dim pk, cat
+
+
+def bench_star_c0(ctx, a):
+    """C0: 1M-row fact filtered to a 30% day range, joined with a 10k-row
+    dimension on fk = pk, grouped by the dimension's 25 categories with
+    SUM(price * Float(qty)) and COUNT(*): dim build + one fused
+    scan/probe/aggregate call per query. Latency-bound at this size."""
+    import torch
+    from kushdb_amd import abi
+    from kushdb_amd.runtime import Column, HashTable
+    from kushdb_amd.workloads import star_tables
+    f, d, (lo, hi) = star_tables(device="cuda:0")
+    cols = [Column(abi.KG_INT, f["fk"]), Column(abi.KG_INT, f["qty"]), Column(abi.KG_REAL, f["price"]),
+            Column(abi.KG_DATE, f["day"])]
+    preds = [(3, abi.KG_GEQ, lo), (3, abi.KG_LT, hi)]
+    aggs = [(abi.KG_AGG_SUM, 2, 1), (abi.KG_AGG_COUNT, -1)]
+    pk, cat = Column(abi.KG_INT, d["pk"]), Column(abi.KG_ENUM, d["cat"])
+
+    def query():
+        ht = HashTable(ctx, 10_000)
+        ht.insert(pk)
+        r = ht.scan_probe_group_agg(cols, preds, 0, cat, 25, aggs, max_groups=64)
+        ht.free()
+        return r
+    for _ in range(3):
+        query()
+    torch.cuda.synchronize()
+    steps = 20
+    ctx.set_timing(True)
+    ctx.kernel_stats(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        codes, vals, _ = query()
+    wall = (time.perf_counter() - t0) / steps
+    kms, launches = ctx.kernel_stats(reset=True)
+    ctx.set_timing(False)
+    # timed kernels per query: the dim build and the fused probe kernel
+    per_query_kernel_ms = kms / steps
+    return {"workload": "1M-row fact (30% day range) join 10k-row dim, GROUP BY 25 cats: SUM(price*qty), COUNT(*)",
+            "groups": int(len(codes)), "query_ms": round(wall * 1e3, 4), "kernel_ms": round(per_query_kernel_ms, 4),
+            "rows_per_s": 1_000_000 / wall,
+            "roofline": roofline(C0_BYTES, per_query_kernel_ms)}
+
+
+def bench_chain_c3(ctx, a):
+    """C3: five chained tables (200k, 1M, 4M, 25M, 50M rows), filters incl.
+    STARTS_WITH / LIKE on TEXT and v < 50, chain join through device column
+    indexes in the order t0..t4, tuple-id dedup, MIN(t0.name), MIN(t4.v)."""
+    import torch
+    from kushdb_amd.chain import chain_query
+    from kushdb_amd.workloads import SKINNER_SIZES, skinner_tables
+    log("C3: generating chain tables")
+    tabs, spec, htabs = skinner_tables(scale=1.0, theta=0.0, device="cuda:0")
+    torch.cuda.synchronize()
+    order = [0, 1, 2, 3, 4]
+    chain_query(ctx, tabs, spec, order)
+    torch.cuda.synchronize()
+    steps = 3
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tuples, mn_text, mn_int, stats = chain_query(ctx, tabs, spec, order)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    n_in = sum(SKINNER_SIZES)
+    # referenced bytes: filter columns of t0 (name: 8-B offset + bytes), t1 (s), t4 (v: 4 B); join
+    # keys of the selected rows (4 B each); 64 B per index insert, index probe and tuple insert.
+    text_bytes = sum(int(htabs[i][c + "_heap"].nbytes) + 8 * len(htabs[i]["id"]) for i, c in ((0, "name"), (1, "s")))
+    alg = (text_bytes + 4 * SKINNER_SIZES[4] + 4 * sum(stats["selected"]) +
+           64 * (stats["index_inserts"] + stats["index_probes"] + stats["tuples"]))
+    return {"workload": "5-table chain join (200k..50M rows), TEXT prefix + LIKE + int filters, device indexes, "
+                        "tuple dedup, MIN(text), MIN(int)", "order": order, "stats": stats,
+            "query_ms": round(wall * 1e3, 3), "rows_per_s": n_in / wall,
+            "min_int": mn_int, "roofline": roofline(alg, wall * 1e3)}
+
+
 def bench_serving(ctx, a):
     """Batched/served mix: Poisson arrivals of queries from 8 templates over
     resident fact tables of 10k..50M rows. Plain one-at-a-time GPU path vs
@@ -373,6 +449,10 @@ def main():
     result["extra"] = {}
     if not a.no_join:
         result["extra"].update(bench_join(ctx, a))
+    if not a.no_join:
+        result["extra"]["c0_star"] = bench_star_c0(ctx, a)
+        result["extra"]["c3_chain"] = bench_chain_c3(ctx, a)
+        torch.cuda.empty_cache()
     if not a.no_serve:
         result["extra"]["serving"] = bench_serving(ctx, a)
     print(json.dumps(result), flush=True)

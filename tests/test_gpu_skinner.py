@@ -14,6 +14,7 @@ import torch
 from kushdb_amd import abi
 from kushdb_amd.runtime import (Column, ColumnIndex, Context, TextColumn, TupleSet, gather, index_join_expand,
                                 text_minmax, text_select, tuple_rows)
+from kushdb_amd.chain import chain_filters, chain_join, chain_query
 from kushdb_amd.workloads import skinner_tables
 from oracle import pyoracle as po
 
@@ -22,49 +23,11 @@ DEV = "cuda:0"
 
 
 def gpu_filters(ctx, tabs, spec):
-    """Selection (row ids) of every table."""
-    sels = []
-    for ti, (t, f) in enumerate(zip(tabs, spec)):
-        rows = None
-        for kind, col, op, lit in f:
-            if kind == "text":
-                rows = text_select(ctx, t[col], op, lit, rows)
-            else:
-                c = t[col]
-                if rows is None:
-                    rows = ctx.scan_select([c], [(0, op, lit)])
-                else:
-                    g = gather(ctx, c, rows)
-                    keep = ctx.scan_select([g], [(0, op, lit)])
-                    rows = rows[keep.long()]
-        if rows is None:
-            rows = torch.arange(len(next(iter(t.values()))), dtype=torch.int32, device=DEV)
-        sels.append(rows.contiguous())
-    return sels
+    return chain_filters(ctx, tabs, spec)
 
 
 def gpu_chain(ctx, tabs, sels, order):
-    """Join along the chain in `order` (indices of tables, each adjacent to a
-    joined one). Returns [n, 5] tuple ids (positions in each selection)."""
-    ntab = len(tabs)
-    first = order[0]
-    cur = torch.arange(sels[first].numel(), dtype=torch.int32, device=DEV).view(-1, 1)
-    cols = [first]
-    for nxt in order[1:]:
-        # chain edge: table i.fk -> table i-1.id (i = 1..4)
-        if nxt - 1 in cols:      # nxt.fk = (nxt-1).id: probe with (nxt-1).id, index on nxt.fk
-            src = nxt - 1
-            src_key = tabs[src]["id"]
-            idx = ColumnIndex(ctx, tabs[nxt]["fk"], sels[nxt])
-        else:                    # (nxt+1).fk = nxt.id: probe with (nxt+1).fk, index on nxt.id
-            src = nxt + 1
-            src_key = tabs[src]["fk"]
-            idx = ColumnIndex(ctx, tabs[nxt]["id"], sels[nxt])
-        cur = index_join_expand(ctx, cur.contiguous(), cols.index(src), src_key, sels[src], idx)
-        idx.free()
-        cols.append(nxt)
-    perm = [cols.index(t) for t in range(ntab)]
-    return cur[:, perm].contiguous()
+    return chain_join(ctx, tabs, sels, order)[0]
 
 
 def host_filters(ht, spec):
@@ -135,3 +98,18 @@ def test_chain_join_orders_agree():
     a = run_case(0.01, 0.0, [4, 3, 2, 1, 0])
     b = run_case(0.01, 0.0, [2, 1, 3, 0, 4])
     assert a == b
+
+
+def test_chain_query_matches_oracle():
+    """The packaged pipeline (filters, chain join, dedup, MIN) on the same data."""
+    ctx = Context(0)
+    tabs, spec, htabs = skinner_tables(scale=0.01, theta=1.0, device=DEV, seed=21)
+    tuples, mn_text, mn_int, stats = chain_query(ctx, tabs, spec, [0, 1, 2, 3, 4])
+    hsels = [host_filters(ht, sp) for ht, sp in zip(htabs, spec)]
+    want = oracle_chain(htabs, hsels)
+    got = tuples.cpu().numpy()
+    assert np.array_equal(got[np.lexsort(got.T[::-1])], want[np.lexsort(want.T[::-1])])
+    assert stats["tuples"] == len(want) and stats["selected"] == [len(h) for h in hsels]
+    if len(want):
+        assert mn_text.decode() == min(htabs[0]["name_list"][r] for r in hsels[0][want[:, 0]])
+        assert mn_int == htabs[4]["v"][hsels[4][want[:, 4]]].min()
