@@ -10,25 +10,28 @@
 //             HashJoin probe -> GroupByAggregate on a build-side code column or
 //             the ungrouped Aggregate (hash_join_translator.cc:85-139), as
 //             kg_ht_build + kg_scan_join_group_agg.
-// A batch runs four kernels: fills (accumulators, table slots), the build
-// phase (work items = (query, build tile)), the scan/probe phase, and a
-// finalize that folds each query's accumulator copies into its result area,
-// which is copied back in one transfer.
+// A batch runs as one captured graph: fills (accumulators, table slots), the
+// build phase (work items = (query, build tile)), the scan/probe kernels, and
+// a finalize for queries without scan/probe items. A query's result area lives
+// in pinned host memory; the wave that finishes its last item folds the
+// accumulator copies into it and publishes a completion word.
 //
-// Scan/probe work items are (query, tile of kBTile rows). Queries over the same
-// table share its tiles: the items of tile k of a table are adjacent and sit in
-// queue k mod 8, one queue per XCD (workgroup b starts on queue b mod 8, the
-// XCD the dispatcher places it on), so the queries reading tile k run back to
-// back on one XCD and all but the first read it from that XCD's L2. A
-// workgroup drains its own queue, then the others, and exits when all eight are
-// empty; there is no inter-workgroup waiting, so any grid size is correct.
+// Scans with dense code keys and non-NULL columns (the register path) run as
+// shared scans: the queries over one table form groups whose tiles (512 or
+// 1024 rows) are dealt round robin to the workgroups; the group's columns are
+// staged once into LDS and every query of the group reads the tile from
+// there. Other scan/probe work items are (query, tile of kBTile rows); queries
+// over the same table share its tiles through L2: the items of tile k of a table are adjacent and
+// sit in queue k mod 8, one queue per XCD (workgroup b starts on queue b mod 8,
+// the XCD the dispatcher places it on). A workgroup drains its own queue, then
+// the others, and exits when all eight are empty; there is no inter-workgroup
+// waiting, so any grid size is correct.
 //
-// Per item, a query with at most 8 groups and non-NULL measures accumulates in
-// lane-private registers (the register path), reduced across the wave by a
-// butterfly and across the workgroup in LDS; other queries use a
-// workgroup-private LDS table updated with LDS atomics (up to kBLdsWords 8-byte
-// words). Item partials go to one of eight accumulator copies per query with
-// global atomics; the finalize adds the eight copies. FP64 sums are therefore
+// Register-path partials live in lane registers, reduced across the wave by a
+// butterfly; other queries use a wave-private LDS table updated with LDS
+// atomics (up to kBLdsWords 8-byte words per workgroup). Partials go to one of
+// kBSlots accumulator copies per query with global atomics; the wave that
+// finishes a query's last item folds the copies. FP64 sums are therefore
 // added in a timing-dependent order: a query's result in a mix equals its
 // result alone within FP64 rounding (integers, counts and group order exactly).
 #include <algorithm>
@@ -67,6 +70,10 @@ struct BFast {
   const int32_t* key[2];
   int32_t real[kMaxMeasures];
   const void* mcol[kMaxMeasures];
+  // Columns in the staged tile of the query's shared-scan group.
+  int8_t pcx[kBFastPreds];
+  int8_t kcx[2];
+  int8_t mcx[kMaxMeasures];
 };
 
 // Typed view of a join's probe side: 4-byte non-NULL predicate columns, a
@@ -131,6 +138,13 @@ struct BFill {
 // Scan/probe work of one kernel: tables (queries sharing table_id and n) and
 // one queue per XCD. Queue x holds, table by table, the tiles k = x mod 8,
 // each followed by all of that table's queries.
+// Shared-scan work: groups (SGroup) in order, and the prefix of their tile
+// counts; tile t of the sequence is taken by workgroup t mod gridDim.x.
+struct STiles {
+  int32_t ng;
+  int32_t tpre[kBMaxQueries + 1];
+};
+
 struct BQueues {
   int32_t nt;
   int32_t qpre[kBQueues][kBMaxQueries + 1];   // per queue, item prefix over tables
@@ -146,7 +160,7 @@ struct BHeader {
   int32_t bpre[kBMaxQueries + 1];             // build phase: item prefix over bq entries
   int32_t bq[kBMaxQueries];                   // build phase: query of entry j
   unsigned int bhead;
-  BQueues fast;                               // register-path scans
+  STiles shared;                              // register-path scans, by group
   BQueues join;                               // joins with a typed view
   BQueues gen;                                // the others
   BFill fills[kBMaxFills];
@@ -295,20 +309,11 @@ __device__ __forceinline__ void visit_rows(const BQuery& d, int64_t lo, int64_t 
   }
 }
 
-// Register path (SCAN_AGG with a BFast view, at most kBRegG groups), one wave
-// per item: lane l owns the row groups lo + 4 l + 256 k (k < 16) of 4
-// consecutive rows, read with 16-byte vector loads, four row groups' loads in
-// flight at once. A first pass turns predicates and group codes into a 64-bit
-// row mask and packed 4-bit codes and counts rows and first rows per group; the
-// measures are then summed two at a time into lane-private registers, so each
-// column is read once. Partials are reduced across the wave by a butterfly and
-// added to accumulator copy `slot` by one atomic instruction per value kind.
-// Waves work independently (no LDS, no barriers).
+// Vector-load helpers of the typed join path: lane l owns the row groups
+// lo + 4 l + 256 k (k < kWRowGroups) of 4 consecutive rows of a kBTile item,
+// read with 16-byte vector loads (FULL) or element by element at the end of a
+// table.
 constexpr int kWRowGroups = kBTile / (kWave * 4);   // row groups per lane
-constexpr int kWBatch = 2;                          // row groups loaded together
-constexpr int kWBatches = kWRowGroups / kWBatch;
-constexpr int kWBatchRows = 4 * kWBatch;            // rows per lane per batch
-static_assert(kWBatchRows * 4 * kWBatches == 256, "4-bit codes of all rows fill codes[4]");
 
 template <bool FULL>
 __device__ __forceinline__ void ld4_i32(const int32_t* col, int64_t base, int nv, int32_t (&o)[4]) {
@@ -370,202 +375,326 @@ __device__ __forceinline__ int rows_left(int64_t hi, int64_t base) {
   return static_cast<int>(left < 0 ? 0 : (left > 4 ? 4 : left));
 }
 
-// Packed 4-bit codes of all rows of a lane: batch kb's kWBatchRows codes are
-// bits [32 kb, 32 kb + 32) of the 256-bit value (w0, w1, w2, w3).
-struct Codes {
-  uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-  __device__ __forceinline__ uint32_t get(int kb) const {
-    const int word = kb >> 1;
-    const uint64_t x = word == 0 ? w0 : (word == 1 ? w1 : (word == 2 ? w2 : w3));
-    return static_cast<uint32_t>(x >> (32 * (kb & 1)));
-  }
-  __device__ __forceinline__ void put(int kb, uint32_t c) {
-    const uint64_t part = static_cast<uint64_t>(c) << (32 * (kb & 1));
-    const int word = kb >> 1;
-    w0 |= word == 0 ? part : 0;
-    w1 |= word == 1 ? part : 0;
-    w2 |= word == 2 ? part : 0;
-    w3 |= word == 3 ? part : 0;
-  }
-};
-static_assert(kWBatchRows * 4 == 32, "one 32-bit code word per batch");
+// Shared scans (SCAN_AGG with a BFast view, at most kBRegG groups). The
+// register-path queries over one table form groups whose columns, together at
+// most kSRowBytes bytes per row, are staged tile by tile (512 or 1024 rows)
+// into the workgroup's LDS with direct global-to-LDS loads; every query of the
+// group then reads the tile from LDS, so the group reads the table from HBM
+// once, and the queries' per-row work runs at LDS latency. The work of a group
+// is split into at most kSWaves units, one per wave: a unit is (query, pair of
+// its measures, half of its groups when it has more than four, slice of the
+// tile's rows). Lane l handles row groups of 4
+// consecutive rows (LDS vector reads), evaluates the predicates and the group
+// code, and adds the unit's measures into lane registers with select-adds.
+// The partials stay in registers across all tiles of the group the workgroup
+// takes, and are reduced (butterfly) and added to accumulator copy `slot` once
+// per stay.
+constexpr int kSThreads = 512;        // threads per shared-scan workgroup (two per CU)
+constexpr int kSWaves = kSThreads / kWave;
+constexpr int kSBufBytes = 32768;    // LDS of a staged tile (two per workgroup)
+constexpr int kSMaxTile = 1024;       // rows per tile when a row takes <= 32 bytes (else 512)
+constexpr int kSRowBytes = kSBufBytes / (kSMaxTile / 2);
+constexpr int kSMaxCols = 16;
+constexpr int kSMaxQ = kSWaves;       // queries per group
+constexpr int kSUnitM = 2;            // measures per unit
+constexpr int kSUnitG = 4;            // groups per unit
+constexpr int kSMaxRG = kSMaxTile / (kWave * 4);   // row groups per lane in a whole-tile unit
 
-// Sums measures m0 (kind K0) and m0 + 1 (kind K1, 0 = absent) over the rows in
-// `mask` and adds the wave's totals to accumulator copy `slot`.
-template <bool FULL, int K0, int K1>
-__device__ __forceinline__ void wave_chunk(const BQuery& d, int64_t lo, int64_t hi, int m0, uint64_t mask,
-                           const Codes& codes, int slot) {
-  const BFast& F = d.F;
-  const int G = d.G, nm = d.nm;
-  const int lane = threadIdx.x & (kWave - 1);
-  uint64_t acc[2][kBRegG];
+// A group of register-path queries over one table, and its units.
+struct SGroup {
+  int64_t n;                          // table rows
+  int32_t nq, ncol, rows, nunits;     // rows: tile rows
+  int32_t q[kSMaxQ];
+  const void* col[kSMaxCols];
+  int32_t width[kSMaxCols];           // bytes per value (4 or 8)
+  int32_t off[kSMaxCols];             // LDS byte offset of the column's tile
+  // Unit w: query q[uq[w]], measures um[w] and um[w] + 1 (those below nm),
+  // groups ug[w] .. ug[w] + 3 (those below G), tile rows [ur[w], ur[w] + 256
+  // urg[w]); ucnt[w]: it also counts rows and first rows of its groups (one
+  // unit per query, group and row does).
+  int8_t uq[kSWaves], um[kSWaves], ug[kSWaves], urg[kSWaves], ucnt[kSWaves];
+  int16_t ur[kSWaves];
+};
+
+// LDS pointers (address space 3), so that tile reads compile to ds_read.
+typedef __attribute__((address_space(3))) char lds_char;
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const v4i lds_v4i;
+typedef __attribute__((address_space(3))) const v2d lds_v2d;
+
+__device__ __forceinline__ void lds4_i32(const lds_char* col, int row, int32_t (&o)[4]) {
+  const v4i x = *reinterpret_cast<lds_v4i*>(col + row * 4);
+  o[0] = x[0];
+  o[1] = x[1];
+  o[2] = x[2];
+  o[3] = x[3];
+}
+
+// K = 1: REAL (double bits), K = 2: 4-byte integer (sign-extended).
+template <int K>
+__device__ __forceinline__ void lds4_measure(const lds_char* col, int row, uint64_t (&o)[4]) {
+  if (K == 1) {
+    const v2d a = *reinterpret_cast<lds_v2d*>(col + row * 8);
+    const v2d b = *reinterpret_cast<lds_v2d*>(col + row * 8 + 16);
+    o[0] = static_cast<uint64_t>(__double_as_longlong(a[0]));
+    o[1] = static_cast<uint64_t>(__double_as_longlong(a[1]));
+    o[2] = static_cast<uint64_t>(__double_as_longlong(b[0]));
+    o[3] = static_cast<uint64_t>(__double_as_longlong(b[1]));
+  } else {
+    int32_t t[4];
+    lds4_i32(col, row, t);
 #pragma unroll
-  for (int gg = 0; gg < kBRegG; gg++) acc[0][gg] = acc[1][gg] = 0;
-  // Two load buffers: batch kb + 1 is in flight while batch kb is summed.
-  static_assert(kWBatches % 2 == 0, "batches are processed in pairs");
-  uint64_t va[kWBatch][2][4], vb[kWBatch][2][4];
-  auto load = [&](int kb, uint64_t (&dst)[kWBatch][2][4]) {
-#pragma unroll
-    for (int kk = 0; kk < kWBatch; kk++) {
-      const int64_t base = lo + 4 * lane + static_cast<int64_t>(kb * kWBatch + kk) * kWave * 4;
-      const int nv = FULL ? 4 : rows_left(hi, base);
-      ld4_measure<FULL, K0>(F.mcol[m0], base, nv, dst[kk][0]);
-      if (K1) ld4_measure<FULL, K1>(F.mcol[m0 + 1], base, nv, dst[kk][1]);
-    }
-  };
-  auto sum = [&](int kb, const uint64_t (&cur)[kWBatch][2][4]) {
-    const uint32_t mw = static_cast<uint32_t>(mask >> (kWBatchRows * kb));
-    const uint32_t cw = codes.get(kb);
-#pragma unroll
-    for (int kk = 0; kk < kWBatch; kk++)
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int bit = 4 * kk + e;
-        if (!((mw >> bit) & 1u)) continue;
-        const int g = static_cast<int>((cw >> (4 * bit)) & 15u);
-        // Branch-free select-add (a dynamically indexed update would move acc
-        // to scratch). Adding +0.0 or 0 leaves an accumulator unchanged: they
-        // start at +0.0 / 0 and never become -0.0.
-#pragma unroll
-        for (int gg = 0; gg < kBRegG; gg++) {
-          if (gg >= G) break;
-          acc[0][gg] = kadd<K0>(acc[0][gg], gg == g ? cur[kk][0][e] : 0);
-          if (K1) acc[1][gg] = kadd<K1>(acc[1][gg], gg == g ? cur[kk][1][e] : 0);
-        }
-      }
-  };
-  load(0, va);
-#pragma unroll 1
-  for (int kb = 0; kb < kWBatches; kb += 2) {
-    load(kb + 1, vb);
-    sum(kb, va);
-    if (kb + 2 < kWBatches) load(kb + 2, va);
-    sum(kb + 1, vb);
-  }
-  uint64_t mine0 = 0, mine1 = 0;
-#pragma unroll
-  for (int gg = 0; gg < kBRegG; gg++) {
-    if (gg >= G) break;
-    const uint64_t a0 = wave_sum<K0>(acc[0][gg]);
-    if (lane == gg) mine0 = a0;
-    if (K1) {
-      const uint64_t a1 = wave_sum<K1>(acc[1][gg]);
-      if (lane == gg) mine1 = a1;
-    }
-  }
-  if (lane < G) {
-    const int64_t o = (static_cast<int64_t>(slot) * G + lane) * nm + m0;
-    if (mine0) atomic_acc<K0>(&d.sum[o], mine0);
-    if (K1 && mine1) atomic_acc<K1>(&d.sum[o + 1], mine1);
+    for (int e = 0; e < 4; e++) o[e] = static_cast<uint64_t>(static_cast<int64_t>(t[e]));
   }
 }
 
-template <bool FULL>
-__device__ __forceinline__ void item_fast(const BQuery& d, int64_t lo, int64_t hi, int slot) {
-  const BFast& F = d.F;
-  const int G = d.G, nm = d.nm;
-  const int lane = threadIdx.x & (kWave - 1);
-  uint64_t mask = 0;
-  Codes codes;
-  uint32_t cnt[kBRegG];
-  uint64_t first[kBRegG];
+// Partials of one unit, kept in lane registers while the workgroup stays on
+// the unit's group: per measure and group the sum (double bits or wrapping
+// int64), per group the row count and first row.
+struct UnitAcc {
+  uint64_t s[kSUnitM][kSUnitG];
+  uint32_t cnt[kSUnitG];
+  uint32_t first[kSUnitG];  // table rows are below 2^31
+};
+
+__device__ __forceinline__ void unit_reset(UnitAcc& A) {
 #pragma unroll
-  for (int g = 0; g < kBRegG; g++) {
-    cnt[g] = 0;
-    first[g] = ~0ull;
+  for (int g = 0; g < kSUnitG; g++) {
+#pragma unroll
+    for (int m = 0; m < kSUnitM; m++) A.s[m][g] = 0;
+    A.cnt[g] = 0;
+    A.first[g] = ~0u;
   }
+}
+
+// What a wave reads of the descriptors, loaded once per stay into
+// wave-uniform registers (the descriptors live in global memory that the
+// kernel also writes, so the compiler would otherwise re-read them with
+// vector loads inside the row loops). CodeDesc: the query whose row codes the
+// wave computes (phase A); UnitDesc: the wave's unit (phase B).
+struct CodeDesc {
+  int q, npred, nk, card1, G, rg0, rgs;   // local query; row groups rg0, rg0 + rgs, ...
+  int op[kBFastPreds];
+  int64_t lit[kBFastPreds];
+  int poff[kBFastPreds];          // LDS byte offsets of the columns in the tile
+  int koff[2];
+};
+
+struct UnitDesc {
+  int q, G, gb, gc, nm, nmu, m0, counts, r0, rg;
+  uint32_t real;                  // bit i: measure m0 + i is REAL
+  int moff[kSUnitM];
+};
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
+  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// Phase A of wave w: local query w mod nq, row groups w / nq + k * (waves per
+// query) of the tile.
+__device__ __forceinline__ void code_desc(const BQuery* qs, const SGroup& S, int w, CodeDesc& c) {
+  const int nq = uni(S.nq);
+  c.q = w % nq;
+  c.rg0 = w / nq;
+  c.rgs = kSWaves / nq;
+  const BQuery& d = qs[uni(S.q[c.q])];
+  const BFast& F = d.F;
+  c.npred = uni(F.npred);
+  c.nk = uni(F.nk);
+  c.card1 = uni(F.card1);
+  c.G = uni(d.G);
+#pragma unroll
+  for (int p = 0; p < kBFastPreds; p++) {
+    const bool on = p < c.npred;
+    c.op[p] = on ? uni(F.op[p]) : 0;
+    c.lit[p] = on ? uni64(F.lit[p]) : 0;
+    c.poff[p] = on ? uni(S.off[F.pcx[p]]) : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < 2; k++) c.koff[k] = k < c.nk ? uni(S.off[F.kcx[k]]) : 0;
+}
+
+// Unit w of group S, whose query is d.
+__device__ __forceinline__ void unit_desc(const BQuery& d, const SGroup& S, int w, UnitDesc& u) {
+  const BFast& F = d.F;
+  u.q = uni(S.uq[w]);
+  u.G = uni(d.G);
+  u.gb = uni(S.ug[w]);
+  u.gc = min(u.G - u.gb, kSUnitG);
+  u.nm = uni(d.nm);
+  u.m0 = uni(S.um[w]);
+  u.counts = uni(S.ucnt[w]);
+  u.r0 = uni(S.ur[w]);
+  u.rg = uni(S.urg[w]);
+  uint32_t real = 0;
+  int nmu = 0;
+#pragma unroll
+  for (int i = 0; i < kSUnitM; i++) {
+    const int m = u.m0 + i;
+    const bool on = m < u.nm;
+    u.moff[i] = on ? uni(S.off[F.mcx[m]]) : 0;
+    real |= (on && F.real[m]) ? 1u << i : 0u;
+    nmu += on ? 1 : 0;
+  }
+  u.nmu = uni(nmu);
+  u.real = uni(static_cast<int>(real));
+}
+
+// Phase A: the group code of each of the tile's rows for query c.q (-1: the
+// row fails or is past the vr valid rows), one byte per row in codes[c.q].
+// Returns whether a qualifying row had a code outside [0, G).
+__device__ __forceinline__ bool tile_codes(const CodeDesc& c, const lds_char* sh, int rows, int vr,
+                                           __attribute__((address_space(3))) uint32_t* codes) {
+  const int lane = threadIdx.x & (kWave - 1);
   bool bad = false;
-#pragma unroll 1
-  for (int kb = 0; kb < kWBatches; kb++) {
-    uint32_t mw = 0;
-    uint32_t cw = 0;
+  for (int j = c.rg0; j < rows / (kWave * 4); j += c.rgs) {
+    const int row = 4 * (j * kWave + lane);
+    bool ok[4];
 #pragma unroll
-    for (int kk = 0; kk < kWBatch; kk++) {
-      const int64_t base = lo + 4 * lane + static_cast<int64_t>(kb * kWBatch + kk) * kWave * 4;
-      const int nv = FULL ? 4 : rows_left(hi, base);
-      bool pass[4];
+    for (int e = 0; e < 4; e++) ok[e] = row + e < vr;
 #pragma unroll
-      for (int e = 0; e < 4; e++) pass[e] = e < nv;
+    for (int p = 0; p < kBFastPreds; p++) {
+      if (p >= c.npred) break;
+      int32_t x[4];
+      lds4_i32(sh + c.poff[p], row, x);
 #pragma unroll
-      for (int p = 0; p < kBFastPreds; p++) {
-        if (p >= F.npred) break;
-        int32_t x[4];
-        ld4_i32<FULL>(F.pcol[p], base, nv, x);
+      for (int e = 0; e < 4; e++) ok[e] = ok[e] && cmp_i64(c.op[p], x[e], c.lit[p]);
+    }
+    int32_t g[4] = {0, 0, 0, 0};
+    if (c.nk >= 1) {
+      int32_t a[4];
+      lds4_i32(sh + c.koff[0], row, a);
+      if (c.nk == 2) {
+        int32_t b[4];
+        lds4_i32(sh + c.koff[1], row, b);
 #pragma unroll
-        for (int e = 0; e < 4; e++) pass[e] = pass[e] && cmp_i64(F.op[p], x[e], F.lit[p]);
+        for (int e = 0; e < 4; e++) g[e] = a[e] * c.card1 + b[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++) g[e] = a[e];
       }
-      int32_t g[4] = {0, 0, 0, 0};
-      if (F.nk >= 1) {
-        int32_t a[4];
-        ld4_i32<FULL>(F.key[0], base, nv, a);
-        if (F.nk == 2) {
-          int32_t b[4];
-          ld4_i32<FULL>(F.key[1], base, nv, b);
+    }
+    uint32_t word = 0;
 #pragma unroll
-          for (int e = 0; e < 4; e++) g[e] = a[e] * F.card1 + b[e];
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; e++) g[e] = a[e];
-        }
+    for (int e = 0; e < 4; e++) {
+      if (ok[e] && static_cast<unsigned>(g[e]) >= static_cast<unsigned>(c.G)) {
+        bad = true;
+        ok[e] = false;
       }
+      word |= (ok[e] ? static_cast<uint32_t>(g[e]) : 0xFFu) << (8 * e);
+    }
+    codes[c.q * (kSMaxTile / 4) + row / 4] = word;
+  }
+  return bad;
+}
+
+// Phase B: adds the unit's rows of the staged tile (first table row lo) to A,
+// for GC groups gb .. gb + GC - 1 (a compile-time count, so the select-adds
+// carry no per-group branches), taking each row's group from the phase A
+// codes. Rows that fail, or belong to other groups, get a local group outside
+// [0, GC), so a select-add tests the local group alone; INT measures add
+// x * (g == gg) with one 32x32+64 multiply-add, REAL measures a select and an
+// add (a multiply by 0.0 would turn an infinite value into NaN in the other
+// groups). Adding +0.0 or 0 leaves a sum unchanged: sums start at +0.0 / 0 and
+// never become -0.0. Every accumulator index is static: a dynamically indexed
+// update would move A to scratch.
+template <int GC>
+__device__ __forceinline__ void unit_scan(const UnitDesc& u, const lds_char* sh, int64_t lo,
+                                          const __attribute__((address_space(3))) uint32_t* codes, UnitAcc& A) {
+  const int nmu = u.nmu;
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int j = 0; j < kSMaxRG; j++) {
+    if (j >= u.rg) break;
+    const int row = u.r0 + 4 * (j * kWave + lane);
+    const uint32_t word = codes[u.q * (kSMaxTile / 4) + row / 4];
+    if (word == 0xFFFFFFFFu) continue;  // no qualifying row
+    int32_t g[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) g[e] = static_cast<int32_t>(static_cast<int8_t>(word >> (8 * e))) - u.gb;
+    if (u.counts) {
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        if (!pass[e]) continue;
-        if (static_cast<unsigned>(g[e]) >= static_cast<unsigned>(G)) {
-          bad = true;
-          continue;
-        }
-        const int bit = 4 * kk + e;
-        mw |= 1u << bit;
-        cw |= static_cast<uint32_t>(g[e]) << (4 * bit);
-        const uint64_t row = static_cast<uint64_t>(base + e);
+        const uint32_t r = static_cast<uint32_t>(lo + row + e);
 #pragma unroll
-        for (int gg = 0; gg < kBRegG; gg++) {
-          if (gg >= G) break;
-          cnt[gg] += gg == g[e] ? 1u : 0u;
-          first[gg] = (gg == g[e] && row < first[gg]) ? row : first[gg];
+        for (int gg = 0; gg < GC; gg++) {
+          const bool hit = gg == g[e];
+          A.cnt[gg] += hit ? 1u : 0u;
+          A.first[gg] = min(A.first[gg], hit ? r : ~0u);
         }
       }
     }
-    mask |= static_cast<uint64_t>(mw) << (kWBatchRows * kb);
-    codes.put(kb, cw);
-  }
-  if (bad) atomicOr(d.flag, kFlagBadCode);
-  // Rows, first rows and each measure's non-NULL count (= rows: the measures
-  // are non-NULL), one lane per group.
-  uint64_t myc = 0, myf = ~0ull;
 #pragma unroll
-  for (int gg = 0; gg < kBRegG; gg++) {
-    if (gg >= G) break;
-    uint32_t c = cnt[gg];
-    uint64_t f = first[gg];
-    for (int s = kWave / 2; s >= 1; s >>= 1) {
-      c += __shfl_xor(c, s);
-      const uint64_t o = shfl_xor_u64(f, s);
-      f = o < f ? o : f;
-    }
-    if (lane == gg) {
-      myc = c;
-      myf = f;
+    for (int i = 0; i < kSUnitM; i++) {
+      if (i >= nmu) break;
+      const lds_char* col = sh + u.moff[i];
+      if ((u.real >> i) & 1u) {
+        uint64_t v[4];
+        lds4_measure<1>(col, row, v);
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+#pragma unroll
+          for (int gg = 0; gg < GC; gg++) A.s[i][gg] = fadd_bits(A.s[i][gg], gg == g[e] ? v[e] : 0);
+      } else {
+        int32_t x[4];
+        lds4_i32(col, row, x);
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+#pragma unroll
+          for (int gg = 0; gg < GC; gg++)
+            A.s[i][gg] += static_cast<uint64_t>(static_cast<int64_t>(x[e]) * static_cast<int64_t>(gg == g[e]));
+      }
     }
   }
-  if (lane < G && myc) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(&d.rows[static_cast<int64_t>(slot) * G + lane]), myc);
-    atomicMin(reinterpret_cast<unsigned long long*>(&d.first[static_cast<int64_t>(slot) * G + lane]), myf);
-    for (int m = 0; m < nm; m++)
-      atomicAdd(reinterpret_cast<unsigned long long*>(&d.nn[(static_cast<int64_t>(slot) * G + lane) * nm + m]), myc);
+}
+
+// Reduces a unit's partials across the wave and adds them to query d's
+// accumulator copy `slot`, lane l for group gb + l.
+__device__ __forceinline__ void unit_flush(const BQuery& d, const UnitDesc& u, const UnitAcc& A, int slot) {
+  const int G = u.G, nm = u.nm, gc = u.gc;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t base = static_cast<int64_t>(slot) * G + u.gb + lane;   // (slot, group) of lane < gc
+  if (u.counts) {
+    uint64_t myc = 0, myf = ~0ull;
+#pragma unroll
+    for (int gg = 0; gg < kSUnitG; gg++) {
+      if (gg >= gc) break;
+      uint32_t c = A.cnt[gg];
+      uint32_t f = A.first[gg];
+      for (int s = kWave / 2; s >= 1; s >>= 1) {
+        c += __shfl_xor(c, s);
+        f = min(f, static_cast<uint32_t>(__shfl_xor(static_cast<int>(f), s)));
+      }
+      if (lane == gg) {
+        myc = c;
+        myf = f == ~0u ? ~0ull : f;
+      }
+    }
+    if (lane < gc && myc) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&d.rows[base]), myc);
+      atomicMin(reinterpret_cast<unsigned long long*>(&d.first[base]), myf);
+      for (int m = 0; m < nm; m++) atomicAdd(reinterpret_cast<unsigned long long*>(&d.nn[base * nm + m]), myc);
+    }
   }
-  if (__builtin_amdgcn_readfirstlane(__ballot(mask != 0) != 0 ? 1 : 0) == 0) return;  // no qualifying rows
-  for (int m0 = 0; m0 < nm; m0 += 2) {
-    const int k0 = F.real[m0] ? 1 : 2;
-    const int k1 = m0 + 1 < nm ? (F.real[m0 + 1] ? 1 : 2) : 0;
-    switch (k0 * 3 + k1) {
-      case 3: wave_chunk<FULL, 1, 0>(d, lo, hi, m0, mask, codes, slot); break;
-      case 4: wave_chunk<FULL, 1, 1>(d, lo, hi, m0, mask, codes, slot); break;
-      case 5: wave_chunk<FULL, 1, 2>(d, lo, hi, m0, mask, codes, slot); break;
-      case 6: wave_chunk<FULL, 2, 0>(d, lo, hi, m0, mask, codes, slot); break;
-      case 7: wave_chunk<FULL, 2, 1>(d, lo, hi, m0, mask, codes, slot); break;
-      default: wave_chunk<FULL, 2, 2>(d, lo, hi, m0, mask, codes, slot); break;
+#pragma unroll
+  for (int i = 0; i < kSUnitM; i++) {
+    if (i >= u.nmu) break;
+    const bool real = (u.real >> i) & 1u;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int gg = 0; gg < kSUnitG; gg++) {
+      if (gg >= gc) break;
+      const uint64_t a = real ? wave_sum<1>(A.s[i][gg]) : wave_sum<2>(A.s[i][gg]);
+      if (lane == gg) mine = a;
+    }
+    if (lane < gc && mine) {
+      if (real) atomic_acc<1>(&d.sum[base * nm + u.m0 + i], mine);
+      else atomic_acc<2>(&d.sum[base * nm + u.m0 + i], mine);
     }
   }
 }
@@ -1063,13 +1192,178 @@ __device__ __forceinline__ void drain_wave(BQueues& Q, const BQuery* qs, Run&& r
   }
 }
 
-// Scan/probe phase, register-path scans (one wave per item).
-__global__ __launch_bounds__(kBThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_fast(BHeader* h, const BQuery* qs) {
-  const int slot = acc_slot();
-  drain_wave(h->fast, qs, [&](const BQuery& d, int64_t lo, int64_t hi) {
-    if (hi - lo == kBTile) item_fast<true>(d, lo, hi, slot);
-    else item_fast<false>(d, lo, hi, slot);
-  });
+// Ends a workgroup's stay on group S after `ntiles` of its tiles: the units'
+// partials go to the accumulator copies, and each query's remaining-tile count
+// drops by ntiles once every wave's partials are acknowledged; the wave that
+// takes a count to zero folds that query.
+__device__ __forceinline__ void group_leave(const SGroup& S, const BQuery* qs, bool active, const UnitDesc& u,
+                                            const UnitAcc& A, const CodeDesc& c, bool cbad, int slot,
+                                            int32_t ntiles) {
+  const int wave = uni(static_cast<int>(threadIdx.x / kWave));
+  const int lane = threadIdx.x & (kWave - 1);
+  if (active) unit_flush(qs[uni(S.q[u.q])], u, A, slot);
+  if (__builtin_amdgcn_readfirstlane(__ballot(cbad) != 0 ? 1 : 0) && lane == 0)
+    atomicOr(qs[uni(S.q[c.q])].flag, kFlagBadCode);
+  // The partials' atomics are acknowledged (and not moved past the wait)
+  // before the barrier that precedes the count.
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __syncthreads();
+  if (wave < S.nq) {
+    const BQuery& d = qs[uni(S.q[wave])];
+    unsigned long long old = 0;
+    if (lane == 0) old = atomicAdd(d.rem, static_cast<unsigned long long>(-static_cast<long long>(ntiles)));
+    old = __shfl(old, 0);
+    if (old == static_cast<unsigned long long>(ntiles)) fold_query(d);
+  }
+}
+
+// Scan/probe phase, shared scans. The groups' tiles form one sequence
+// (groups in order, smallest tables first) dealt round robin: workgroup b takes
+// tiles b, b + B, b + 2B, ... (B = gridDim.x), so no work queue is needed and
+// each workgroup knows its next tile; that tile is staged into the second LDS
+// buffer while the current one is scanned. Per tile, phase A writes every
+// query's row codes to LDS and phase B runs the units: wave w runs unit w of
+// the group and keeps its partials in registers for as long as the workgroup
+// takes tiles of that group; they are flushed when the workgroup's next tile
+// belongs to another group, and at the end. Two workgroups of 8 waves per CU.
+__global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_shared(
+    BHeader* h, const BQuery* qs, const SGroup* gs) {
+  __shared__ __attribute__((aligned(16))) char sh[2][kSBufBytes];
+  // Tile prefix, rows and tile rows of every group, read once: finding a
+  // tile's group and rows then takes LDS reads only.
+  __shared__ int32_t s_tpre[kBMaxQueries + 1];
+  __shared__ int64_t s_n[kBMaxQueries];
+  __shared__ int32_t s_rows[kBMaxQueries];
+  // Phase A output: per query of the group, one group-code byte per tile row.
+  __shared__ uint32_t s_codes[kSMaxQ * kSMaxTile / 4];
+  const int wave = uni(static_cast<int>(threadIdx.x / kWave));
+  const int lane = threadIdx.x & (kWave - 1);
+  // Accumulator copy: slot mod 8 = the workgroup's XCD (b mod 8), spread over
+  // the XCD's waves.
+  const int slot = static_cast<int>(blockIdx.x % kBQueues) +
+                   kBQueues * static_cast<int>((blockIdx.x / kBQueues * kSWaves + wave) % (kBSlots / kBQueues));
+  const STiles& T = h->shared;
+  const int ng = T.ng;
+  for (int i = threadIdx.x; i <= ng; i += kSThreads) s_tpre[i] = T.tpre[i];
+  for (int i = threadIdx.x; i < ng; i += kSThreads) {
+    s_n[i] = gs[i].n;
+    s_rows[i] = gs[i].rows;
+  }
+  __syncthreads();
+  const int32_t total = s_tpre[ng];
+  const int32_t step = static_cast<int32_t>(gridDim.x);
+  UnitAcc A;
+  unit_reset(A);
+  UnitDesc u{};
+  CodeDesc cd{};
+  int cur = -1;
+  bool active = false, cbad = false;
+  int32_t ntiles = 0;
+  // Tile t of the sequence: group g (searched upward from the previous one:
+  // a workgroup's tiles ascend), first row lo, valid rows vr.
+  auto decode = [&](int32_t t, int& g, int64_t& lo, int& vr) {
+    while (g + 1 < ng && s_tpre[g + 1] <= t) g++;
+    const int rows = s_rows[g];
+    lo = static_cast<int64_t>(t - s_tpre[g]) * rows;
+    const int64_t n = s_n[g];
+    vr = static_cast<int>(n - lo < rows ? n - lo : rows);
+  };
+  // Staging: whole 16-byte chunks with direct global-to-LDS loads (one 1 KB
+  // wave instruction covers 64 chunks; the tile's instructions are dealt round
+  // robin over the waves, across columns), the bytes of a trailing partial
+  // chunk with 4-byte loads; rows past vr are left unwritten (the units never
+  // read them). Lane c holds staging column c's address, width and LDS offset
+  // of group cg, reloaded when the staged group changes.
+  int cg = -1, cncol = 0;
+  const char* ccol = nullptr;
+  int cw = 0, coff = 0;
+  auto stage = [&](int g, int64_t lo, int vr, char* buf) {
+    if (g != cg) {
+      const SGroup& S = gs[g];
+      cncol = uni(S.ncol);
+      ccol = lane < kSMaxCols ? static_cast<const char*>(S.col[lane]) : nullptr;
+      cw = lane < kSMaxCols ? S.width[lane] : 0;
+      coff = lane < kSMaxCols ? S.off[lane] : 0;
+      cg = g;
+    }
+    const int rows = s_rows[g];
+    int base = 0;
+    for (int c = 0; c < cncol; c++) {
+      const uint64_t pa = reinterpret_cast<uint64_t>(ccol);
+      const uint32_t plo = __builtin_amdgcn_readlane(static_cast<uint32_t>(pa), c);
+      const uint32_t phi = __builtin_amdgcn_readlane(static_cast<uint32_t>(pa >> 32), c);
+      const int w = __builtin_amdgcn_readlane(cw, c);
+      const int off = __builtin_amdgcn_readlane(coff, c);
+      const char* src = reinterpret_cast<const char*>((static_cast<uint64_t>(phi) << 32) | plo) + lo * w;
+      char* dst = buf + off;
+      const int bytes = vr * w;
+      const int full = bytes >> 4;
+      const int ninst = (rows * w) >> 10;
+      for (int i = (wave - base % kSWaves + kSWaves) % kSWaves; i < ninst; i += kSWaves) {
+        const int chunk = i * kWave + lane;
+        if (chunk < full) __builtin_amdgcn_global_load_lds(src + chunk * 16, dst + i * 1024, 16, 0, 0);
+      }
+      base += ninst;
+      const int tail = (bytes & 15) >> 2;  // 4-byte words of a partial last chunk
+      if (static_cast<int>(threadIdx.x) < tail)
+        reinterpret_cast<int32_t*>(dst + full * 16)[threadIdx.x] =
+            reinterpret_cast<const int32_t*>(src + full * 16)[threadIdx.x];
+    }
+  };
+  int32_t t = static_cast<int32_t>(blockIdx.x);
+  int g = 0, vr = 0;
+  int64_t lo = 0;
+  if (t < total) {
+    decode(t, g, lo, vr);
+    stage(g, lo, vr, sh[0]);
+  }
+  for (int k = 0; t < total; k++) {
+    // This tile's staging is complete, and no wave still reads the other buffer.
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (g != cur) {
+      if (cur >= 0) {
+        group_leave(gs[cur], qs, active, u, A, cd, cbad, slot, ntiles);
+        unit_reset(A);
+        cbad = false;
+      }
+      cur = g;
+      ntiles = 0;
+      const SGroup& S = gs[g];
+      active = wave < uni(S.nunits);
+      if (active) unit_desc(qs[uni(S.q[S.uq[wave]])], S, wave, u);
+      code_desc(qs, S, wave, cd);
+    }
+    const int32_t tn = t + step;
+    int gn = g, vrn = 0;
+    int64_t lon = 0;
+    if (tn < total) {
+      decode(tn, gn, lon, vrn);
+      stage(gn, lon, vrn, sh[(k + 1) & 1]);
+    }
+    const lds_char* buf = (const lds_char*)(sh[k & 1]);
+    __attribute__((address_space(3))) uint32_t* codes = (__attribute__((address_space(3))) uint32_t*)(s_codes);
+    cbad |= tile_codes(cd, buf, s_rows[g], vr, codes);
+    __syncthreads();  // the codes are complete
+    if (active) {
+      switch (u.gc) {
+        case 1: unit_scan<1>(u, buf, lo, codes, A); break;
+        case 2: unit_scan<2>(u, buf, lo, codes, A); break;
+        case 3: unit_scan<3>(u, buf, lo, codes, A); break;
+        default: unit_scan<4>(u, buf, lo, codes, A); break;
+      }
+    }
+    ntiles++;
+    t = tn;
+    g = gn;
+    lo = lon;
+    vr = vrn;
+  }
+  if (cur >= 0) group_leave(gs[cur], qs, active, u, A, cd, cbad, slot, ntiles);
 }
 
 // Scan/probe phase, joins with a typed view (one wave per item).
@@ -1180,6 +1474,7 @@ struct kg_batch {
   hipEvent_t ready = nullptr;  // the context stream's work before a submit
   int grid = 0;
   int grid_gen = 0;
+  int grid_shared = 0;
   char* d_desc = nullptr;      // BHeader, then BQuery[kBMaxQueries]
   char* h_desc = nullptr;      // pinned staging of the same
   char* arena = nullptr;
@@ -1201,18 +1496,24 @@ struct kg_batch {
   BQuery* hq() { return reinterpret_cast<BQuery*>(h_desc + round256(sizeof(BHeader))); }
   BHeader* dhdr() { return reinterpret_cast<BHeader*>(d_desc); }
   BQuery* dq() { return reinterpret_cast<BQuery*>(d_desc + round256(sizeof(BHeader))); }
+  size_t groups_off() const { return round256(sizeof(BHeader)) + round256(sizeof(BQuery) * kBMaxQueries); }
+  SGroup* hg() { return reinterpret_cast<SGroup*>(h_desc + groups_off()); }
+  SGroup* dg() { return reinterpret_cast<SGroup*>(d_desc + groups_off()); }
 };
 
 namespace {
 
-size_t desc_bytes() { return round256(sizeof(BHeader)) + round256(sizeof(BQuery) * kBMaxQueries); }
+size_t desc_bytes() {
+  return round256(sizeof(BHeader)) + round256(sizeof(BQuery) * kBMaxQueries) + round256(sizeof(SGroup) * kBMaxQueries);
+}
 
 int capture(kg_batch* b) {
   kg_ctx* ctx = b->ctx;
   KG_HIP_CHECK(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(k_batch_clear, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr());
   hipLaunchKernelGGL(k_batch_build, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
-  hipLaunchKernelGGL(k_batch_fast, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
+  hipLaunchKernelGGL(k_batch_shared, dim3(b->grid_shared), dim3(kSThreads), 0, b->stream, b->dhdr(), b->dq(),
+                     b->dg());
   hipLaunchKernelGGL(k_batch_join, dim3(b->grid_gen), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
   hipLaunchKernelGGL(k_batch_gen, dim3(b->grid_gen), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
   hipLaunchKernelGGL(k_batch_final, dim3(kBMaxQueries), dim3(kWave), 0, b->stream, b->dhdr(), b->dq());
@@ -1223,6 +1524,8 @@ int capture(kg_batch* b) {
   KG_HIP_CHECK(ctx, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
   return KG_OK;
 }
+
+bool fits_group(const BQuery& D);
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
@@ -1368,6 +1671,7 @@ int plan_query(kg_ctx* ctx, const kg_query& q, BQuery& D, BPlan& P) {
   for (int m = 0; m < nm; m++)
     if (P.M.nulls[m] || (P.M.data2[m] && P.M.nulls2[m])) nonnull = false;
   D.reg = (q.kind == KG_Q_SCAN_AGG && G <= kBRegG && nonnull && fast_view(D.P, D.K, P.M, D.F)) ? 1 : 0;
+  if (D.reg && !fits_group(D)) D.reg = 0;  // its columns alone exceed a staged tile
   D.jfast = (q.kind == KG_Q_JOIN_AGG && join_view(q, D.P, P.M, D.J)) ? 1 : 0;
   if (!D.reg && G * nm * 2 + G * 2 > kBWaveWords)
     return kg_fail(ctx, KG_ERR_UNSUPPORTED, "%lld groups x %d measures exceed the batched LDS table", (long long)G,
@@ -1417,6 +1721,141 @@ bool layout_queues(BQueues& Q, const std::vector<int32_t>& ids, const kg_query* 
   return true;
 }
 
+// Columns (pointer, bytes per value) a register-path query reads.
+int fast_columns(const BQuery& D, std::pair<const void*, int> (&cols)[kSMaxCols]) {
+  int n = 0;
+  auto add = [&](const void* p, int w) {
+    for (int c = 0; c < n; c++)
+      if (cols[c].first == p) return;
+    if (n < kSMaxCols) cols[n] = {p, w};
+    n++;
+  };
+  for (int p = 0; p < D.F.npred; p++) add(D.F.pcol[p], 4);
+  for (int k = 0; k < D.F.nk; k++) add(D.F.key[k], 4);
+  for (int m = 0; m < D.nm; m++) add(D.F.mcol[m], D.F.real[m] ? 8 : 4);
+  return n;
+}
+
+int fast_row_bytes(const BQuery& D) {
+  std::pair<const void*, int> cols[kSMaxCols];
+  const int n = fast_columns(D, cols);
+  if (n > kSMaxCols) return kSBufBytes;
+  int bytes = 0;
+  for (int c = 0; c < n; c++) bytes += cols[c].second;
+  return bytes;
+}
+
+// Whether a register-path query fits one shared-scan group on its own.
+bool fits_group(const BQuery& D) { return fast_row_bytes(D) <= kSRowBytes; }
+
+// Tile rows of a register-path query's group: 1024 when the query's columns
+// take at most 32 bytes per row, else 512 (a staged tile is kSBufBytes).
+int tile_rows(const BQuery& D) { return fast_row_bytes(D) * kSMaxTile <= kSBufBytes ? kSMaxTile : kSMaxTile / 2; }
+
+// Groups the register-path queries by table (table_id, n) and tile rows,
+// smallest tables first; a group holds at most kSMaxQ queries whose columns
+// together fit a staged tile (a query that does not fit starts the next group
+// of its table). Fills the queries' column indices and the tile sequence.
+bool layout_groups(STiles& ST, SGroup* gs, const std::vector<int32_t>& ids, const kg_query* qs, BQuery* Q) {
+  std::vector<std::pair<std::pair<int32_t, std::pair<int64_t, int>>, std::vector<int32_t>>> tables;
+  for (int i : ids) {
+    if (qs[i].n == 0) continue;
+    const std::pair<int32_t, std::pair<int64_t, int>> key{qs[i].table_id, {qs[i].n, tile_rows(Q[i])}};
+    auto it = std::find_if(tables.begin(), tables.end(), [&](const auto& e) { return e.first == key; });
+    if (it == tables.end()) {
+      tables.push_back({key, {}});
+      it = tables.end() - 1;
+    }
+    it->second.push_back(i);
+  }
+  std::stable_sort(tables.begin(), tables.end(),
+                   [](const auto& a, const auto& b) { return a.first.second.first < b.first.second.first; });
+  int ng = 0;
+  for (const auto& t : tables) {
+    const int rows = t.first.second.second;
+    const int max_bytes = kSBufBytes / rows;
+    SGroup* cur = nullptr;
+    int bytes = 0, units = 0;
+    for (int i : t.second) {
+      BQuery& D = Q[i];
+      const int mu = std::max(1, (D.nm + kSUnitM - 1) / kSUnitM) * ((D.G + kSUnitG - 1) / kSUnitG);  // its units
+      std::pair<const void*, int> cols[kSMaxCols];
+      const int nc = fast_columns(D, cols);
+      auto find = [](const SGroup* g, const void* p) {
+        for (int c = 0; c < g->ncol; c++)
+          if (g->col[c] == p) return c;
+        return -1;
+      };
+      int add_cols = 0, add_bytes = 0;
+      for (int c = 0; c < nc && cur; c++)
+        if (find(cur, cols[c].first) < 0) {
+          add_cols++;
+          add_bytes += cols[c].second;
+        }
+      if (!cur || cur->nq == kSMaxQ || units + mu > kSWaves || cur->ncol + add_cols > kSMaxCols ||
+          bytes + add_bytes > max_bytes) {
+        if (ng == kBMaxQueries) return false;
+        cur = &gs[ng++];
+        std::memset(cur, 0, sizeof(SGroup));
+        cur->n = t.first.second.first;
+        cur->rows = rows;
+        bytes = 0;
+        units = 0;
+      }
+      units += mu;
+      auto index = [&](const void* p, int w) {
+        int c = find(cur, p);
+        if (c < 0) {
+          c = cur->ncol++;
+          cur->col[c] = p;
+          cur->width[c] = w;
+          bytes += w;
+        }
+        return static_cast<int8_t>(c);
+      };
+      for (int p = 0; p < D.F.npred; p++) D.F.pcx[p] = index(D.F.pcol[p], 4);
+      for (int k = 0; k < D.F.nk; k++) D.F.kcx[k] = index(D.F.key[k], 4);
+      for (int m = 0; m < D.nm; m++) D.F.mcx[m] = index(D.F.mcol[m], D.F.real[m] ? 8 : 4);
+      cur->q[cur->nq++] = i;
+    }
+  }
+  ST.ng = ng;
+  ST.tpre[0] = 0;
+  for (int g = 0; g < ng; g++) {
+    SGroup& S = gs[g];
+    int off = 0;
+    for (int c = 0; c < S.ncol; c++) {
+      S.off[c] = off;
+      off += S.rows * S.width[c];
+    }
+    // Units: each query's measures in pairs, then every pair over rs row
+    // slices (whole row groups of 4 rows per lane), at most one unit per wave.
+    auto pairs = [&](int q) { return std::max(1, (Q[S.q[q]].nm + kSUnitM - 1) / kSUnitM); };
+    auto halves = [&](int q) { return (Q[S.q[q]].G + kSUnitG - 1) / kSUnitG; };
+    int mu = 0;
+    for (int q = 0; q < S.nq; q++) mu += pairs(q) * halves(q);
+    int rs = 1;
+    while (mu * rs * 2 <= kSWaves && S.rows / (rs * 2) >= kWave * 4) rs *= 2;
+    int w = 0;
+    for (int q = 0; q < S.nq; q++)
+      for (int k = 0; k < pairs(q); k++)
+        for (int hf = 0; hf < halves(q); hf++)
+          for (int r = 0; r < rs; r++, w++) {
+            S.uq[w] = static_cast<int8_t>(q);
+            S.um[w] = static_cast<int8_t>(k * kSUnitM);
+            S.ug[w] = static_cast<int8_t>(hf * kSUnitG);
+            S.ur[w] = static_cast<int16_t>(r * (S.rows / rs));
+            S.urg[w] = static_cast<int8_t>(S.rows / rs / (kWave * 4));
+            S.ucnt[w] = k == 0 ? 1 : 0;
+          }
+    S.nunits = w;
+    const int64_t tiles = ceil_div(S.n, S.rows);
+    if (ST.tpre[g] + tiles >= INT32_MAX) return false;
+    ST.tpre[g + 1] = ST.tpre[g] + static_cast<int32_t>(tiles);
+  }
+  return true;
+}
+
 }  // namespace
 
 extern "C" int kg_batch_create(kg_ctx* ctx, const kg_batch_opts* opts, kg_batch** out) {
@@ -1433,6 +1872,7 @@ extern "C" int kg_batch_create(kg_ctx* ctx, const kg_batch_opts* opts, kg_batch*
   b->res_cap = std::min<size_t>(b->arena_bytes, size_t{64} << 20);
   b->grid = ctx->num_cus * 3;
   b->grid_gen = ctx->num_cus * 8;
+  b->grid_shared = ctx->num_cus * 2;
   auto fail = [&](int st) {
     kg_batch_free(b);
     return st;
@@ -1554,7 +1994,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     D.rem = reinterpret_cast<unsigned long long*>(D.rows + static_cast<size_t>(kBSlots) * G + 1);
     // Fill ranges must not overlap: the fill kernel writes them in no fixed order.
     H->fills[nf++] = BFill{D.sum, static_cast<int64_t>(kBSlots) * G * (2 * std::max(nm, 1) + 1) + 1, 0};
-    D.items = static_cast<int32_t>(ceil_div(qs[i].n, kBTile));
+    D.items = static_cast<int32_t>(ceil_div(qs[i].n, D.reg ? tile_rows(D) : kBTile));
     H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.rem), 1, static_cast<uint64_t>(D.items)};
     D.first = reinterpret_cast<uint64_t*>(base + acc_bytes);
     H->fills[nf++] = BFill{D.first, static_cast<int64_t>(kBSlots) * G, ~0ull};
@@ -1595,7 +2035,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     else if (Q[i].jfast) join_ids.push_back(i);
     else gen_ids.push_back(i);
   }
-  if (!layout_queues(H->fast, fast_ids, qs) || !layout_queues(H->join, join_ids, qs) ||
+  if (!layout_groups(H->shared, b->hg(), fast_ids, qs, Q) || !layout_queues(H->join, join_ids, qs) ||
       !layout_queues(H->gen, gen_ids, qs))
     return kg_fail(ctx, KG_ERR_INVALID, "batch too large");
   // Queries that failed planning keep a zero descriptor: G = 0, nothing to fold.
@@ -1608,6 +2048,9 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   KG_HIP_CHECK(ctx, hipEventRecord(b->ready, ctx->stream));
   KG_HIP_CHECK(ctx, hipStreamWaitEvent(b->stream, b->ready, 0));
   KG_HIP_CHECK(ctx, hipMemcpyAsync(b->d_desc, b->h_desc, bytes, hipMemcpyHostToDevice, b->stream));
+  if (H->shared.ng)
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(b->dg(), b->hg(), sizeof(SGroup) * H->shared.ng, hipMemcpyHostToDevice,
+                                     b->stream));
   KG_HIP_CHECK(ctx, hipGraphLaunch(b->exec, b->stream));
   KG_HIP_CHECK(ctx, hipEventRecord(b->done, b->stream));
   b->nq = nq;

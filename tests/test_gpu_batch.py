@@ -55,6 +55,24 @@ def test_alone_equals_mix(env):
     b.free()
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 7, 9])
+def test_shared_scan_group_sizes(env, k):
+    """k register-path scans over one table: groups of one to four queries
+    (one to four tile slices per query), several groups per table, and the
+    tail tile (250k rows is not a multiple of the tile)."""
+    ctx, _, cols, H = env
+    specs = serve_queries(64, SIZES, NBUILD, seed=31,
+                          templates=["c1_small", "two_filter_ungrouped", "two_filter", "c1_scan_agg"])[:k]
+    for s in specs:
+        s["table"] = 2
+    print("shared scan group of", k, flush=True)
+    b = Batch(ctx, arena_bytes=64 << 20)
+    got = b.run(_queries(cols, specs), max_groups=64)
+    for s, g in zip(specs, got):
+        assert_same(s, g, oracle_spec(s, H), what=f"group of {k}")
+    b.free()
+
+
 def test_repeated_submits_and_empty(env):
     ctx, _, cols, H = env
     b = Batch(ctx, arena_bytes=64 << 20)
