@@ -54,9 +54,13 @@ constexpr int kBRegG = 8;             // register path: at most this many groups
 constexpr int kBLdsWords = 2048;      // LDS path: 8-byte words of the item table
 constexpr int kBFastPreds = 4;
 constexpr int kBMaxAggs = 16;
-constexpr int kBMaxFills = 4 * kBMaxQueries + 8;
+constexpr int kBMaxFills = 6 * kBMaxQueries + 8;
+constexpr int kBMapMax = 65536;       // direct-map joins: at most this many key offsets
+constexpr uint32_t kMapEmpty = 0xFFFFFFFFu;
 
-enum BFlag : uint32_t { kFlagEmptyKey = 1u, kFlagBadCode = 2u, kFlagOverflow = 4u };
+// kFlagNoMap is internal (the direct map of a join cannot be used) and is not
+// reported.
+enum BFlag : uint32_t { kFlagEmptyKey = 1u, kFlagBadCode = 2u, kFlagOverflow = 4u, kFlagNoMap = 8u };
 
 // Register-path view of a scan: 4-byte non-NULL predicate, key and INT
 // measure columns, non-NULL REAL measures, all 16-byte aligned (vector loads).
@@ -108,6 +112,13 @@ struct BQuery {
   MeasureSet M;
   kg_col pkey;          // JOIN_AGG probe key
   const int32_t* codes; // JOIN_AGG build row -> group code (nullptr: group 0)
+  // Direct map of a typed join with a small build side: entry key - min for
+  // keys in [min, min + msize) = group code << 24 | build row (kMapEmpty: no
+  // row), usable when the qualifying build keys are unique and span at most
+  // msize values. krange = {-min, max} of the qualifying build keys.
+  uint32_t* map;
+  int64_t* krange;
+  int32_t msize;
   TableView t;          // JOIN_AGG table (slots in the arena)
   int64_t bn;           // JOIN_AGG build rows
   PredSet PB;           // build side predicates
@@ -160,6 +171,10 @@ struct BHeader {
   int32_t bpre[kBMaxQueries + 1];             // build phase: item prefix over bq entries
   int32_t bq[kBMaxQueries];                   // build phase: query of entry j
   unsigned int bhead;
+  int32_t nrq;                                // key-range phase (direct-map joins only):
+  int32_t rpre[kBMaxQueries + 1];             //   item prefix over rq entries
+  int32_t rq[kBMaxQueries];                   //   query of entry j
+  unsigned int rhead;                         //   next item
   STiles shared;                              // register-path scans, by group
   BQueues join;                               // joins with a typed view
   BQueues gen;                                // the others
@@ -221,7 +236,66 @@ __device__ __forceinline__ int32_t take_item(unsigned int* head, int32_t total, 
   return it;
 }
 
-// Build phase: items (query, build tile); same insertion as k_ht_build.
+__device__ __forceinline__ uint64_t coherent_load(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Key range of a direct-map join: min and max of the qualifying build keys,
+// over the build items (before the build phase).
+__global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQuery* qs) {
+  __shared__ int32_t s_it;
+  const int32_t total = h->rpre[h->nrq];
+  const int lane = threadIdx.x & (kWave - 1);
+  while (true) {
+    const int32_t it = take_item(&h->rhead, total, &s_it);
+    if (it >= total) break;
+    const int j = find_prefix(h->rpre, h->nrq, it);
+    const BQuery& d = qs[h->rq[j]];
+    const int64_t lo = static_cast<int64_t>(it - h->rpre[j]) * kBBuildTile;
+    const int64_t hi = lo + kBBuildTile < d.bn ? lo + kBBuildTile : d.bn;
+    int64_t nmin = INT64_MIN, mx = INT64_MIN;  // max of -key, max of key
+    for (int64_t r = lo + threadIdx.x; r < hi; r += kBThreads) {
+      if (!eval_preds(d.PB, r)) continue;
+      int64_t k;
+      if (!load_key(d.bkey, r, k) || k == kEmpty) continue;
+      nmin = -k > nmin ? -k : nmin;
+      mx = k > mx ? k : mx;
+    }
+    for (int s = kWave / 2; s >= 1; s >>= 1) {
+      const int64_t a = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(nmin), s));
+      const int64_t b = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(mx), s));
+      nmin = a > nmin ? a : nmin;
+      mx = b > mx ? b : mx;
+    }
+    if (lane == 0 && mx != INT64_MIN) {
+      atomicMax(reinterpret_cast<long long*>(&d.krange[0]), static_cast<long long>(nmin));
+      atomicMax(reinterpret_cast<long long*>(&d.krange[1]), static_cast<long long>(mx));
+    }
+  }
+}
+
+// Whether a direct-map join can use its map (range known, within the map, no
+// repeated key); *mn and *span receive the map's base key and span.
+__device__ __forceinline__ bool map_usable(const BQuery& d, int64_t* mn, int64_t* span) {
+  if (!d.map) return false;
+  const int64_t nmin = static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[0])));
+  const int64_t mx = static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[1])));
+  const uint32_t fl = __hip_atomic_load(d.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (fl & kFlagNoMap) return false;
+  if (mx == INT64_MIN) {  // no qualifying build row: every probe misses
+    *mn = 0;
+    *span = 0;
+    return true;
+  }
+  *mn = -nmin;
+  const uint64_t sp = static_cast<uint64_t>(mx) - static_cast<uint64_t>(*mn) + 1;
+  *span = static_cast<int64_t>(sp);
+  return sp <= static_cast<uint64_t>(d.msize);
+}
+
+// Build phase: items (query, build tile); same insertion as k_ht_build, and
+// for a direct-map join the entry of the key in the map (a repeated key, or a
+// group code outside [0, G), turns the map off).
 __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQuery* qs) {
   __shared__ int32_t s_it;
   const int32_t total = h->bpre[h->nbq];
@@ -233,6 +307,8 @@ __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQu
     const int64_t lo = static_cast<int64_t>(it - h->bpre[j]) * kBBuildTile;
     const int64_t hi = lo + kBBuildTile < d.bn ? lo + kBBuildTile : d.bn;
     const TableView t = d.t;
+    int64_t mn = 0, span = 0;
+    const bool use_map = map_usable(d, &mn, &span);
     for (int64_t r = lo + threadIdx.x; r < hi; r += kBThreads) {
       if (!eval_preds(d.PB, r)) continue;
       int64_t k;
@@ -240,6 +316,16 @@ __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQu
       if (k == kEmpty) {
         atomicOr(d.flag, kFlagEmptyKey);
         continue;
+      }
+      if (use_map) {
+        const uint64_t off = static_cast<uint64_t>(k) - static_cast<uint64_t>(mn);
+        const int g = d.codes ? d.codes[r] : 0;
+        if (off >= static_cast<uint64_t>(span) || g < 0 || g >= d.G || r >= (int64_t{1} << 24)) {
+          atomicOr(d.flag, kFlagNoMap);
+        } else {
+          const uint32_t e = (static_cast<uint32_t>(g) << 24) | static_cast<uint32_t>(r);
+          if (atomicCAS(&d.map[off], kMapEmpty, e) != kMapEmpty) atomicOr(d.flag, kFlagNoMap);
+        }
       }
       uint64_t b = bucket_of(k, t.nbuckets);
       bool done = false;
@@ -969,6 +1055,8 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   const TableView& t = d.t;
+  int64_t mn = 0, span = 0;
+  const bool use_map = map_usable(d, &mn, &span);
   // Product measures are REAL; others keep their kind.
   const bool real0 = J.mkind[0] == 1 || J.fkind[0] != 0;
   const bool real1 = J.mkind[1] == 1 || J.fkind[1] != 0;
@@ -1001,12 +1089,44 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
     uint64_t v0[4] = {0, 0, 0, 0}, v1[4] = {0, 0, 0, 0};
     if (nm > 0) ld4_jmeasure<FULL>(J.mkind[0], J.fkind[0], J.mcol[0], J.fcol[0], base, nv, v0);
     if (nm > 1) ld4_jmeasure<FULL>(J.mkind[1], J.fkind[1], J.mcol[1], J.fcol[1], base, nv, v1);
+#pragma unroll
+    for (int e = 0; e < 4; e++) pass[e] = pass[e] && key[e] != kEmpty;
+    // Adds probe row `row` matched with build row brow of group g.
+    auto consume_g = [&](int64_t row, int32_t brow, int g, uint64_t a0, uint64_t a1) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&lrows[g]), 1ull);
+      if (g >= 64 || !((seen >> g) & 1ull)) {
+        atomicMin(reinterpret_cast<unsigned long long*>(&lfirst[g]),
+                  (static_cast<unsigned long long>(row) << 32) | static_cast<uint32_t>(brow));
+        if (g < 64) seen |= 1ull << g;
+      }
+      if (nm > 0) {
+        if (real0) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm]), __longlong_as_double(a0));
+        else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm]), a0);
+      }
+      if (nm > 1) {
+        if (real1) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm + 1]), __longlong_as_double(a1));
+        else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm + 1]), a1);
+      }
+    };
+    if (use_map) {
+      // Direct map: one 4-byte entry per probe, the four loaded together.
+      uint32_t me[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const uint64_t off = static_cast<uint64_t>(key[e]) - static_cast<uint64_t>(mn);
+        me[e] = (pass[e] && off < static_cast<uint64_t>(span)) ? d.map[off] : kMapEmpty;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (me[e] != kMapEmpty)
+          consume_g(base + e, static_cast<int32_t>(me[e] & 0xFFFFFFu), static_cast<int>(me[e] >> 24), v0[e], v1[e]);
+      continue;
+    }
     // First buckets of the passing rows, loaded together.
     uint64_t b[4];
     longlong2 kv[4][kSlots / 2];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      pass[e] = pass[e] && key[e] != kEmpty;
       b[e] = pass[e] ? bucket_of(key[e], t.nbuckets) : 0;
 #pragma unroll
       for (int q = 0; q < kSlots / 2; q++)
@@ -1023,20 +1143,7 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
           atomicOr(d.flag, kFlagBadCode);
           return;
         }
-        atomicAdd(reinterpret_cast<unsigned long long*>(&lrows[g]), 1ull);
-        if (g >= 64 || !((seen >> g) & 1ull)) {
-          atomicMin(reinterpret_cast<unsigned long long*>(&lfirst[g]),
-                    (static_cast<unsigned long long>(row) << 32) | static_cast<uint32_t>(brow));
-          if (g < 64) seen |= 1ull << g;
-        }
-        if (nm > 0) {
-          if (real0) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm]), __longlong_as_double(v0[e]));
-          else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm]), v0[e]);
-        }
-        if (nm > 1) {
-          if (real1) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm + 1]), __longlong_as_double(v1[e]));
-          else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm + 1]), v1[e]);
-        }
+        consume_g(row, brow, g, v0[e], v1[e]);
       };
       uint32_t hit = 0;
       bool empty = false;
@@ -1092,10 +1199,6 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
 // acquire, once every item of the query has added its partials. The copies are
 // read with device-scope atomic loads (they were written by atomics from every
 // XCD).
-__device__ __forceinline__ uint64_t coherent_load(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ void fold_query(const BQuery& d) {
   const int lane = threadIdx.x & (kWave - 1);
   const int G = d.G, nm = d.nm;
@@ -1452,6 +1555,16 @@ int64_t query_groups(const kg_query& q) {
 
 // Device bytes one query takes in a batch arena: result area, accumulator
 // copies, and the join table's key and row slots.
+// Direct-map entries of a join (0: no map): typed joins whose build side has
+// at most kBMapMax rows and at most 254 groups; twice the rows, rounded up to
+// a power of two, capped at kBMapMax.
+int32_t map_entries(bool typed_join, int64_t bn, int64_t G) {
+  if (!typed_join || bn <= 0 || bn > kBMapMax || G > 254) return 0;
+  int32_t m = 2;
+  while (m < 2 * bn && m < kBMapMax) m *= 2;
+  return m;
+}
+
 size_t query_footprint(const kg_query& q) {
   const int64_t G = std::min<int64_t>(query_groups(q), kMaxLdsGroups);
   const int nm = std::min(std::max(q.naggs, 1), kMaxMeasures);
@@ -1461,6 +1574,8 @@ size_t query_footprint(const kg_query& q) {
   if (q.kind == KG_Q_JOIN_AGG) {
     const uint64_t nb = table_buckets(std::max<int64_t>(q.build_n, 0), query_lf(q));
     b += round256(nb * kSlots * 8) + round256(nb * kSlots * 4);
+    const int32_t m = map_entries(true, q.build_n, G);
+    if (m) b += round256(static_cast<size_t>(m) * 4) + 256;
   }
   return b;
 }
@@ -1511,6 +1626,7 @@ int capture(kg_batch* b) {
   kg_ctx* ctx = b->ctx;
   KG_HIP_CHECK(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(k_batch_clear, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr());
+  hipLaunchKernelGGL(k_batch_range, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
   hipLaunchKernelGGL(k_batch_build, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
   hipLaunchKernelGGL(k_batch_shared, dim3(b->grid_shared), dim3(kSThreads), 0, b->stream, b->dhdr(), b->dq(),
                      b->dg());
@@ -1977,9 +2093,11 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     const size_t first_bytes = round256(static_cast<size_t>(kBSlots) * G * 8);
     size_t need = acc_bytes + first_bytes;
     uint64_t nbuckets = 0;
+    const int32_t msize = map_entries(P.join && D.jfast, P.bn, G);
     if (P.join) {
       nbuckets = table_buckets(P.bn, query_lf(qs[i]));
       need += round256(nbuckets * kSlots * 8) + round256(nbuckets * kSlots * 4);
+      if (msize) need += round256(static_cast<size_t>(msize) * 4) + 256;
     }
     if (off + need > b->arena_bytes) {
       P.status = KG_ERR_CAPACITY;
@@ -2007,6 +2125,17 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
       D.t.side_count = nullptr;
       H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.t.keys), static_cast<int64_t>(nbuckets * kSlots),
                              static_cast<uint64_t>(kEmpty)};
+      D.map = nullptr;
+      D.krange = nullptr;
+      D.msize = 0;
+      if (msize) {
+        char* mb = tb + round256(nbuckets * kSlots * 8) + round256(nbuckets * kSlots * 4);
+        D.map = reinterpret_cast<uint32_t*>(mb);
+        D.krange = reinterpret_cast<int64_t*>(mb + round256(static_cast<size_t>(msize) * 4));
+        D.msize = msize;
+        H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.map), msize / 2, ~0ull};
+        H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.krange), 2, static_cast<uint64_t>(INT64_MIN)};
+      }
     }
     D.res = b->d_res + P.res_off;
     D.done = b->d_done + i;
@@ -2029,6 +2158,15 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     nbq++;
   }
   H->nbq = nbq;
+  int nrq = 0;
+  H->rpre[0] = 0;
+  for (int j = 0; j < nbq; j++) {
+    if (!Q[H->bq[j]].map) continue;
+    H->rq[nrq] = H->bq[j];
+    H->rpre[nrq + 1] = H->rpre[nrq] + (H->bpre[j + 1] - H->bpre[j]);
+    nrq++;
+  }
+  H->nrq = nrq;
   std::vector<int32_t> fast_ids, join_ids, gen_ids;
   for (int i : live) {
     if (Q[i].reg) fast_ids.push_back(i);

@@ -73,6 +73,55 @@ def test_shared_scan_group_sizes(env, k):
     b.free()
 
 
+@pytest.mark.parametrize("case", ["dup_keys", "sparse_keys", "no_build_rows", "negative_keys"])
+def test_small_build_direct_map(env, case):
+    """Star joins over a small dimension use a direct map from key offset to
+    (group, build row) when the qualifying keys are unique and dense; repeated
+    keys or a span beyond the map fall back to the hash table, an empty build
+    side matches nothing. Every case against the oracle, alone and in a mix."""
+    from kushdb_amd.runtime import Column
+    ctx, tables, cols, H = env
+    facts, dim, build = cols
+    pk = tables["dim"]["pk"]
+    if case == "dup_keys":
+        pk = pk // 2                      # every key twice
+    elif case == "sparse_keys":
+        pk = pk * 1000                    # span 10^7: beyond the map
+    elif case == "negative_keys":
+        pk = pk - 5000                    # dense, negative base
+    pk = pk.contiguous()
+    ddim = [Column(abi.KG_INT, pk), dim[1]]
+    Hd = dict(H, dim={"pk": pk.cpu().numpy(), "cat": H["dim"]["cat"]})
+    specs = serve_queries(40, SIZES, NBUILD, seed=41, templates=["c0_star", "star_dim_filter", "selective_join"])[:6]
+    if case == "no_build_rows":
+        for s_ in specs:
+            s_["template"] = "star_dim_filter"
+            s_["build_preds"] = [(1, abi.KG_LT, 0)]
+            s_["aggs"] = [(abi.KG_AGG_SUM, 5), (abi.KG_AGG_COUNT, -1)]
+    if case in ("sparse_keys", "negative_keys"):
+        # fact keys that hit the transformed dimension keys
+        fk = [Column(c.type, c.data) for c in facts[2]]
+        fkv = tables["facts"][2]["fk"]
+        fk[3] = Column(abi.KG_INT, (fkv * 1000 if case == "sparse_keys" else fkv - 5000).contiguous())
+        fcols = [f if i != 2 else fk for i, f in enumerate(facts)]
+        Hf = [dict(h) for h in Hd["facts"]]
+        Hf[2]["fk"] = fk[3].data.cpu().numpy()
+        Hd = dict(Hd, facts=Hf)
+    else:
+        fcols = facts
+    for s_ in specs:
+        s_["table"] = 2
+    print("direct map case", case, flush=True)
+    b = Batch(ctx, arena_bytes=64 << 20)
+    qs = [make_query(s_, fcols, ddim, build) for s_ in specs]
+    mix = b.run(qs, max_groups=64)
+    for s_, g in zip(specs, mix):
+        assert_same(s_, g, oracle_spec(s_, Hd), what=f"{case} mix")
+    alone = b.run(qs[:1], max_groups=64)[0]
+    assert_same(specs[0], alone, oracle_spec(specs[0], Hd), what=f"{case} alone")
+    b.free()
+
+
 def test_repeated_submits_and_empty(env):
     ctx, _, cols, H = env
     b = Batch(ctx, arena_bytes=64 << 20)
