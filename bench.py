@@ -46,7 +46,9 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--serve-slots", type=int, default=3)
-    ap.add_argument("--serve-caps", type=int, nargs="+", default=[100_000_000, 400_000_000, 0],
+    ap.add_argument("--serve-saturation-only", action="store_true",
+                    help="serving: one batched pass of the mix at once (for kernel traces)")
+    ap.add_argument("--serve-caps", type=int, nargs="+", default=[100_000_000, 0],
                     help="batch work caps (rows) tried for the batched mode; 0 = no cap")
     ap.add_argument("--serve-max-batch", type=int, default=128)
     ap.add_argument("--serve-arena", type=int, default=1024, help="MiB of device arena per batch slot")
@@ -358,20 +360,37 @@ def bench_serving(ctx, a):
                 "p99_ms": st["p99_s"] * 1e3, "mean_batch": st["mean_batch"], "batches": st["batches"],
                 "fallbacks": st["fallbacks"]}
 
-    def best_rate(mode, start, bar, cap=0, factor=1.25, limit=64):
-        """Highest offered rate (start * factor^k) whose p99 stays within bar."""
+    def best_rate(mode, start, bar, cap=0, factor=1.25, limit=64, refine=0):
+        """Highest offered rate (start * factor^k, then `refine` geometric
+        bisection steps below the first failing rate) whose p99 stays within
+        bar."""
         best, runs, rate = None, [], start
+        failed = None
         while rate <= start * limit:
             r = run(mode, rate, cap)
             runs.append(r)
             if r["p99_ms"] > bar or r["qps"] < 0.9 * rate:
+                failed = rate
                 break
             best = r
             rate *= factor
+        lo = best["offered"] if best else None
+        for _ in range(refine if (best and failed) else 0):
+            mid = (lo * failed) ** 0.5
+            r = run(mode, mid, cap)
+            runs.append(r)
+            if r["p99_ms"] > bar or r["qps"] < 0.9 * mid:
+                failed = mid
+            else:
+                best, lo = r, mid
         return best, runs
 
-    run(abi.KG_SERVE_PLAIN, None)
     run(abi.KG_SERVE_BATCHED, None)
+    if a.serve_saturation_only:  # one batched pass of the mix (for kernel traces)
+        batched_sat = run(abi.KG_SERVE_BATCHED, None)
+        log(f"saturation: batched {batched_sat['qps']:.0f} q/s")
+        return {"batched_saturation": batched_sat}
+    run(abi.KG_SERVE_PLAIN, None)
     plain_sat = run(abi.KG_SERVE_PLAIN, None)
     batched_sat = run(abi.KG_SERVE_BATCHED, None)
     log(f"saturation: plain {plain_sat['qps']:.0f} q/s, batched {batched_sat['qps']:.0f} q/s")
@@ -379,12 +398,12 @@ def bench_serving(ctx, a):
     plain_at = run(abi.KG_SERVE_PLAIN, base)
     bar = plain_at["p99_ms"]
     # The run that sets the bar meets it; higher plain rates are tried from there.
-    more, plain_runs = best_rate(abi.KG_SERVE_PLAIN, base * 1.1, bar, factor=1.1, limit=2)
+    more, plain_runs = best_rate(abi.KG_SERVE_PLAIN, base * 1.1, bar, factor=1.1, limit=2, refine=2)
     plain_best = more if (more and more["qps"] > plain_at["qps"]) else plain_at
     plain_runs = [plain_at] + plain_runs
     batched = []
     for cap in a.serve_caps:
-        bb, runs = best_rate(abi.KG_SERVE_BATCHED, base, bar, cap=cap)
+        bb, runs = best_rate(abi.KG_SERVE_BATCHED, base, bar, cap=cap, refine=3)
         batched.append({"cap_rows": cap, "best": bb, "runs": runs})
         log(f"batched cap {cap}: best {bb['qps'] if bb else None} q/s within p99 {bar:.2f} ms")
     cands = [x["best"] for x in batched if x["best"]]

@@ -482,7 +482,7 @@ constexpr int kSMaxTile = 1024;       // rows per tile when a row takes <= 32 by
 constexpr int kSRowBytes = kSBufBytes / (kSMaxTile / 2);
 constexpr int kSMaxCols = 16;
 constexpr int kSMaxQ = kSWaves;       // queries per group
-constexpr int kSUnitM = 2;            // measures per unit
+constexpr int kSUnitM = 4;            // measures per unit
 constexpr int kSUnitG = 4;            // groups per unit
 constexpr int kSMaxRG = kSMaxTile / (kWave * 4);   // row groups per lane in a whole-tile unit
 
@@ -1864,19 +1864,15 @@ int fast_row_bytes(const BQuery& D) {
 // Whether a register-path query fits one shared-scan group on its own.
 bool fits_group(const BQuery& D) { return fast_row_bytes(D) <= kSRowBytes; }
 
-// Tile rows of a register-path query's group: 1024 when the query's columns
-// take at most 32 bytes per row, else 512 (a staged tile is kSBufBytes).
-int tile_rows(const BQuery& D) { return fast_row_bytes(D) * kSMaxTile <= kSBufBytes ? kSMaxTile : kSMaxTile / 2; }
-
-// Groups the register-path queries by table (table_id, n) and tile rows,
-// smallest tables first; a group holds at most kSMaxQ queries whose columns
-// together fit a staged tile (a query that does not fit starts the next group
-// of its table). Fills the queries' column indices and the tile sequence.
+// Groups the register-path queries by table (table_id, n), smallest tables
+// first; a group holds at most kSMaxQ queries whose columns together fit a
+// staged tile (a query that does not fit starts the next group of its table).
+// Fills the queries' column indices and item counts and the tile sequence.
 bool layout_groups(STiles& ST, SGroup* gs, const std::vector<int32_t>& ids, const kg_query* qs, BQuery* Q) {
   std::vector<std::pair<std::pair<int32_t, std::pair<int64_t, int>>, std::vector<int32_t>>> tables;
   for (int i : ids) {
     if (qs[i].n == 0) continue;
-    const std::pair<int32_t, std::pair<int64_t, int>> key{qs[i].table_id, {qs[i].n, tile_rows(Q[i])}};
+    const std::pair<int32_t, std::pair<int64_t, int>> key{qs[i].table_id, {qs[i].n, 0}};
     auto it = std::find_if(tables.begin(), tables.end(), [&](const auto& e) { return e.first == key; });
     if (it == tables.end()) {
       tables.push_back({key, {}});
@@ -1888,8 +1884,9 @@ bool layout_groups(STiles& ST, SGroup* gs, const std::vector<int32_t>& ids, cons
                    [](const auto& a, const auto& b) { return a.first.second.first < b.first.second.first; });
   int ng = 0;
   for (const auto& t : tables) {
-    const int rows = t.first.second.second;
-    const int max_bytes = kSBufBytes / rows;
+    // A group's tiles take 1024 rows while its columns take at most 32 bytes
+    // per row, else 512 (at most kSRowBytes bytes per row).
+    const int max_bytes = kSRowBytes;
     SGroup* cur = nullptr;
     int bytes = 0, units = 0;
     for (int i : t.second) {
@@ -1914,7 +1911,6 @@ bool layout_groups(STiles& ST, SGroup* gs, const std::vector<int32_t>& ids, cons
         cur = &gs[ng++];
         std::memset(cur, 0, sizeof(SGroup));
         cur->n = t.first.second.first;
-        cur->rows = rows;
         bytes = 0;
         units = 0;
       }
@@ -1939,6 +1935,10 @@ bool layout_groups(STiles& ST, SGroup* gs, const std::vector<int32_t>& ids, cons
   ST.tpre[0] = 0;
   for (int g = 0; g < ng; g++) {
     SGroup& S = gs[g];
+    int rb = 0;
+    for (int c = 0; c < S.ncol; c++) rb += S.width[c];
+    S.rows = rb * kSMaxTile <= kSBufBytes ? kSMaxTile : kSMaxTile / 2;
+    for (int q = 0; q < S.nq; q++) Q[S.q[q]].items = static_cast<int32_t>(ceil_div(S.n, S.rows));
     int off = 0;
     for (int c = 0; c < S.ncol; c++) {
       S.off[c] = off;
@@ -2085,6 +2085,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   b->res_used = off;
   off = 0;
   int nf = 0;
+  std::vector<int> rem_fill(nq, -1);  // fill entry of each query's remaining-item count
   for (int i : ok) {
     BPlan& P = b->plans[i];
     BQuery& D = Q[i];
@@ -2112,7 +2113,8 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     D.rem = reinterpret_cast<unsigned long long*>(D.rows + static_cast<size_t>(kBSlots) * G + 1);
     // Fill ranges must not overlap: the fill kernel writes them in no fixed order.
     H->fills[nf++] = BFill{D.sum, static_cast<int64_t>(kBSlots) * G * (2 * std::max(nm, 1) + 1) + 1, 0};
-    D.items = static_cast<int32_t>(ceil_div(qs[i].n, D.reg ? tile_rows(D) : kBTile));
+    D.items = static_cast<int32_t>(ceil_div(qs[i].n, kBTile));  // shared scans: set by layout_groups
+    rem_fill[i] = nf;
     H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.rem), 1, static_cast<uint64_t>(D.items)};
     D.first = reinterpret_cast<uint64_t*>(base + acc_bytes);
     H->fills[nf++] = BFill{D.first, static_cast<int64_t>(kBSlots) * G, ~0ull};
@@ -2176,6 +2178,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   if (!layout_groups(H->shared, b->hg(), fast_ids, qs, Q) || !layout_queues(H->join, join_ids, qs) ||
       !layout_queues(H->gen, gen_ids, qs))
     return kg_fail(ctx, KG_ERR_INVALID, "batch too large");
+  for (int i : fast_ids) H->fills[rem_fill[i]].v = static_cast<uint64_t>(Q[i].items);
   // Queries that failed planning keep a zero descriptor: G = 0, nothing to fold.
   for (int i = 0; i < nq; i++)
     if (b->plans[i].status != KG_OK) {
