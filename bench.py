@@ -68,7 +68,8 @@ def roofline(bytes_per_launch, ms, traffic=None):
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
 # Kernels whose HBM traffic the bench reports, by a substring of their name.
 TRAFFIC_KERNELS = {"k_group_agg_v4": "k_group_agg_v4<", "k_ht_build": "k_ht_build(",
-                   "k_ht_probe_count": "k_ht_probe<0,", "k_ht_probe_agg": "k_ht_probe<2,"}
+                   "k_ht_probe_count": "k_ht_probe<0,", "k_ht_probe_agg": "k_ht_probe<2,",
+                   "k_batch_shared": "k_batch_shared(", "k_batch_join": "k_batch_join("}
 COPY_BYTES = 1 << 30
 
 
@@ -103,13 +104,16 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
     import csv
 
     def per_kernel(path, counter):
+        """Per kernel, the largest dispatch's reading: the bench's full-size
+        launch (smaller launches of the same kernel are tuning samples or
+        other workloads)."""
         acc = {}
         for r in csv.DictReader(open(path)):
             if r["Counter_Name"] != counter:
                 continue
             acc.setdefault(r["Kernel_Name"], {}).setdefault(r["Dispatch_Id"], 0.0)
             acc[r["Kernel_Name"]][r["Dispatch_Id"]] += float(r["Counter_Value"])
-        return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+        return {k: max(v.values()) for k, v in acc.items()}
 
     def largest(d):
         k = max(d, key=d.get)
@@ -127,7 +131,7 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
         wk = [v for k, v in write.items() if sub in k]
         if not fk or not wk:
             continue
-        f, w = sum(fk) / len(fk), sum(wk) / len(wk)
+        f, w = max(fk), max(wk)
         out["kernels"][short] = {"fetch_kib": f, "write_kib": w,
                                  "bytes_per_launch": round((f * ff + w * wf) * 1024)}
     with open(TRAFFIC_JSON, "w") as fh:
