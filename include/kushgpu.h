@@ -497,6 +497,35 @@ typedef struct kg_serve_stats {
 int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const double* arrival_s, const kg_serve_opts* opts,
              kg_query_result* res, double* latency_s, kg_serve_stats* stats);
 
+/* ---- on-disk columns (FileManager / ColumnData / Enum / ColumnIndex) ---------
+ * The reference's files, loaded into HBM-resident columns of the context. */
+
+/* Replaces FileManager::Open + ColumnData::Open{Int8,Int16,Int32,Int64,Float64}
+ * (runtime/file_manager.cc:22-60, runtime/column_data.cc:24-56): the typed
+ * .kdb array at `path` (2, 4, 8 or 1 bytes per value for SMALLINT,
+ * INT/DATE/ENUM, BIGINT/REAL, BOOLEAN) and, when "<stem>_null.kdb" exists
+ * beside it, its int8 NULL flags (util/load.h:70-90). The column stays
+ * resident in device memory, cached by path, until kg_file_release or the
+ * context is destroyed; later calls for the same path return it again. */
+int kg_file_column(kg_ctx* ctx, const char* path, int32_t type, kg_col* out, int64_t* out_n);
+/* ColumnData::OpenText + GetText (runtime/column_data.cc:56,110-114): a TEXT
+ * column file (int32 cardinality, 16-byte {length, offset} slots, NUL-ended
+ * strings; :175-229) as device offsets [n + 1] + bytes (+ NULL flags). */
+int kg_file_text(kg_ctx* ctx, const char* path, kg_text_col* out, int64_t* out_n);
+/* Frees the resident column loaded from `path` (NULL: every loaded column). */
+int kg_file_release(kg_ctx* ctx, const char* path);
+/* Enum dictionary (runtime/enum.cc:26-131, .kdbenum): *out_count = the number
+ * of codes; with buf non-NULL the strings of codes 0..count-1 are copied back
+ * to back into buf (host memory) with offsets[code .. code + 1] delimiting
+ * each (offsets holds max_codes + 1 >= count + 1 entries). */
+int kg_file_enum(kg_ctx* ctx, const char* path, char* buf, int64_t buf_cap, int64_t* offsets, int32_t max_codes,
+                 int32_t* out_count);
+/* ColumnIndex::Open + Get* (runtime/column_index.cc:150-240, .kdbindex with
+ * SMALLINT, INT, DATE or ENUM keys, those kg_index_build takes; other key
+ * types return KG_ERR_UNSUPPORTED): the file's key -> tuple-id lists as a
+ * device column index, equal to kg_index_build over the listed tuples. */
+int kg_file_index(kg_ctx* ctx, const char* path, int32_t key_type, kg_index** out);
+
 #ifdef __cplusplus
 }
 #endif

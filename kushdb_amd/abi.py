@@ -127,6 +127,11 @@ SIGNATURES = {
     "kg_index_info": (_i32, [_vp, _vp, ctypes.POINTER(_vp), _pi64, _pi64]),
     "kg_index_lookup": (_i32, [_vp, _vp, ctypes.POINTER(KgCol), _vp, _i64, _vp, _vp]),
     "kg_index_free": (_i32, [_vp, _vp]),
+    "kg_file_column": (_i32, [_vp, ctypes.c_char_p, _i32, ctypes.POINTER(KgCol), _pi64]),
+    "kg_file_text": (_i32, [_vp, ctypes.c_char_p, ctypes.POINTER(KgTextCol), _pi64]),
+    "kg_file_release": (_i32, [_vp, ctypes.c_char_p]),
+    "kg_file_enum": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p, _i64, _pi64, _i32, _pi32]),
+    "kg_file_index": (_i32, [_vp, ctypes.c_char_p, _i32, ctypes.POINTER(_vp)]),
     "kg_intersect_batch": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_create": (_i32, [_vp, _i32, _i64, ctypes.POINTER(_vp)]),
     "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),

@@ -26,6 +26,7 @@ SOURCES = [
     "kg_text.hip",
     "kg_batch.hip",
     "kg_serve.hip",
+    "kg_file.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

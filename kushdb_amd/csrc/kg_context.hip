@@ -134,6 +134,7 @@ int kg_ctx_destroy(kg_ctx* ctx) {
     (void)hipEventDestroy(pr.second);
   }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  for (auto& f : ctx->files) kg_file_col_free(f);
   if (ctx->work) hipFree(ctx->work);
   if (ctx->pinned) hipHostFree(ctx->pinned);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);

@@ -42,7 +42,21 @@ struct kg_ctx {
   int timing = 0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+  // Columns loaded from the reference's on-disk files, resident by path until
+  // released (kg_file.hip); freed with the context.
+  struct FileCol {
+    std::string path;
+    int32_t type = 0;
+    int64_t n = 0;
+    void* data = nullptr;       // typed array, or TEXT bytes
+    int8_t* nulls = nullptr;
+    int64_t* offsets = nullptr; // TEXT: n + 1 offsets into data
+  };
+  std::vector<FileCol> files;
 };
+
+// Frees the device memory of a loaded file column.
+void kg_file_col_free(kg_ctx::FileCol& f);
 
 // Records a timing start/stop on ctx->stream when timing is on (no-op otherwise).
 // Usage: kg_time_begin(ctx); launch...; kg_time_end(ctx);

@@ -6,6 +6,7 @@ ScanSelect, HashJoin (runtime::HashTable), GroupByAggregate
 (runtime::AggregateHashTable).
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -26,6 +27,18 @@ _TORCH_TYPE = {
 
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+_FILE_DTYPES = {abi.KG_SMALLINT: "<i2", abi.KG_INT: "<i4", abi.KG_DATE: "<i4", abi.KG_ENUM: "<i4",
+                abi.KG_BIGINT: "<i8", abi.KG_REAL: "<f8", abi.KG_BOOLEAN: "|i1"}
+
+
+def _device_view(ptr, n, typestr, device):
+    """A torch tensor viewing n values at device address ptr (no copy)."""
+    if not ptr or n == 0:
+        import numpy as _np
+        return torch.empty(0, dtype=torch.from_numpy(_np.zeros(0, dtype=typestr)).dtype, device=device)
+    return torch.as_tensor(_DeviceArray(ptr, n, typestr), device=device)
 
 
 class Column:
@@ -83,6 +96,51 @@ class Context:
 
     def synchronize(self):
         self.check(self.lib.kg_ctx_synchronize(self.h))
+
+    # -- on-disk columns (reference .kdb / .kdbenum / .kdbindex files) --------------
+    def file_column(self, path, sql_type):
+        """The typed column file at path (and its _null.kdb flags) resident in
+        device memory owned by the context (until file_release)."""
+        d = abi.KgCol()
+        n = ctypes.c_int64()
+        self.check(self.lib.kg_file_column(self.h, str(path).encode(), sql_type, ctypes.byref(d), ctypes.byref(n)))
+        dev = f"cuda:{self.device}"
+        data = _device_view(d.data or 0, n.value, _FILE_DTYPES[sql_type], dev)
+        nulls = _device_view(d.nulls, n.value, "|i1", dev) if d.nulls else None
+        return Column(sql_type, data, nulls)
+
+    def file_text(self, path):
+        """The TEXT column file at path as offsets + bytes (+ NULL flags)."""
+        d = abi.KgTextCol()
+        n = ctypes.c_int64()
+        self.check(self.lib.kg_file_text(self.h, str(path).encode(), ctypes.byref(d), ctypes.byref(n)))
+        dev = f"cuda:{self.device}"
+        off = _device_view(d.offsets, n.value + 1, "<i8", dev)
+        nbytes = int(off[-1].item()) if n.value else 0
+        data = _device_view(d.bytes, nbytes, "|u1", dev)
+        nulls = _device_view(d.nulls, n.value, "|i1", dev) if d.nulls else None
+        return TextColumn(off, data, nulls)
+
+    def file_release(self, path=None):
+        self.check(self.lib.kg_file_release(self.h, None if path is None else str(path).encode()))
+
+    def file_enum(self, path):
+        """Strings of codes 0..count-1 of a .kdbenum dictionary."""
+        cnt = ctypes.c_int32()
+        p = str(path).encode()
+        self.check(self.lib.kg_file_enum(self.h, p, None, 0, None, 0, ctypes.byref(cnt)))
+        cap = max(1, os.path.getsize(path))
+        buf = ctypes.create_string_buffer(cap)
+        offs = (ctypes.c_int64 * (cnt.value + 1))()
+        self.check(self.lib.kg_file_enum(self.h, p, buf, cap, offs, cnt.value, ctypes.byref(cnt)))
+        raw = buf.raw
+        return [raw[offs[i]:offs[i + 1]].decode() for i in range(cnt.value)]
+
+    def file_index(self, path, key_type):
+        """The .kdbindex file at path as a device column index."""
+        h = ctypes.c_void_p()
+        self.check(self.lib.kg_file_index(self.h, str(path).encode(), key_type, ctypes.byref(h)))
+        return ColumnIndex.from_handle(self, h)
 
     def set_probe_regions(self, regions):
         """Region count of partitioned join probes (-1 auto, 0 direct)."""
@@ -286,6 +344,20 @@ class ColumnIndex:
         self.tids_ptr = p.value or 0
         self.ntids = nt.value
         self.nkeys = nk.value
+
+    @classmethod
+    def from_handle(cls, ctx, h):
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.h = h
+        p = ctypes.c_void_p()
+        nt = ctypes.c_int64()
+        nk = ctypes.c_int64()
+        ctx.check(ctx.lib.kg_index_info(ctx.h, self.h, ctypes.byref(p), ctypes.byref(nt), ctypes.byref(nk)))
+        self.tids_ptr = p.value or 0
+        self.ntids = nt.value
+        self.nkeys = nk.value
+        return self
 
     def lookup(self, key_col, rows=None):
         """Per probe key: (device pointers int64 tensor, sizes int32 tensor)."""

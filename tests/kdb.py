@@ -48,6 +48,24 @@ def enum_dict(name):
     return out
 
 
+def index_entries(name, typ):
+    """key -> tuple ids of end_to_end_test/data/<name>.kdbindex: {u64 mask;
+    u64 array[mask + 1]} then entries {key padded to 8 B; u64 size; u64 next;
+    int32 values[size]} back to back (runtime/column_index.cc:150-240)."""
+    raw = open(os.path.join(DATA, name + ".kdbindex"), "rb").read()
+    mask = int(np.frombuffer(raw, dtype=np.uint64, count=1, offset=0)[0])
+    off = 8 + 8 * (mask + 1)
+    dt = np.dtype(_DT[typ])
+    out = {}
+    while off < len(raw):
+        key = np.frombuffer(raw, dtype=dt, count=1, offset=off)[0].item()
+        size = int(np.frombuffer(raw, dtype=np.uint64, count=1, offset=off + 8)[0])
+        out[key] = np.frombuffer(raw, dtype=np.int32, count=size, offset=off + 24).tolist()
+        off += 24 + 4 * size
+    assert off == len(raw)
+    return out
+
+
 def expected(rel):
     with open(os.path.join(REF, rel)) as f:
         return [ln for ln in f.read().split("\n") if ln != ""]
