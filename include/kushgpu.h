@@ -526,6 +526,47 @@ int kg_file_enum(kg_ctx* ctx, const char* path, char* buf, int64_t buf_cap, int6
  * device column index, equal to kg_index_build over the listed tuples. */
 int kg_file_index(kg_ctx* ctx, const char* path, int32_t key_type, kg_index** out);
 
+/* ---- ORDER BY and result rows -------------------------------------------------- */
+
+/* Replaces OrderByTranslator's buffer sort (order_by_translator.cc:35-110 over
+ * runtime/vector.cc:45-110 with LessThan, compile/proxy/evaluate.cc:339-400):
+ * d_perm[0..n) = the input positions in ORDER BY keys[0], ..., keys[nkeys-1]
+ * order, ascending[k] != 0 for ASC. NULLs sort after every value ascending and
+ * before descending; ENUM keys order by their decoded strings (enum_rank[k]: a
+ * device array of each code's string rank, kg_enum_ranks); TEXT keys (type
+ * KG_TEXT, the column in text_keys[k]) by unsigned bytes; BOOLEAN keys are
+ * rejected as in the reference. Rows with equal keys come out in reverse input
+ * order, which is what the reference's merge (the right run's element first
+ * unless comp(left, right)) produces. enum_rank / text_keys may be NULL when
+ * no key needs them. */
+int kg_order_by(kg_ctx* ctx, int64_t n, const kg_col* keys, int32_t nkeys, const int32_t* ascending,
+                const int32_t* const* enum_rank, const kg_text_col* text_keys, int32_t* d_perm);
+/* Rank of each dictionary string (codes 0..count-1, strings back to back in
+ * `strings` delimited by offsets[code .. code + 1]) in std::string order:
+ * the order Enum values compare in (compile/proxy/evaluate.cc:390-393). Host
+ * memory in and out. */
+int kg_enum_ranks(const char* strings, const int64_t* offsets, int32_t count, int32_t* out_rank);
+
+/* One column of result rows in host memory. ENUM: dict / dict_off[code ..
+ * code + 1] delimit each code's string (dict_n codes); TEXT: dict / dict_off[row
+ * .. row + 1] delimit each row's string. */
+typedef struct kg_fmt_col {
+  int32_t type;
+  int32_t dict_n;
+  const void* data;
+  const int8_t* nulls;
+  const char* dict;
+  const int64_t* dict_off;
+} kg_fmt_col;
+/* The OutputTranslator's printing (output_translator.cc:25-31,
+ * runtime/printer.cc:13-35) of n rows, in `order` (host positions) or input
+ * order: each value followed by '|', NULL as the empty string, REAL with six
+ * decimals (std::fixed), DATE as YYYY-MM-DD, BOOLEAN as t / f, '\n' after each
+ * row. *out_len = the text's length; with buf NULL only the length is
+ * computed (KG_ERR_CAPACITY when cap is smaller). Host memory only. */
+int kg_format_rows(const kg_fmt_col* cols, int32_t ncols, int64_t n, const int32_t* order, char* buf, int64_t cap,
+                   int64_t* out_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,11 @@ class KgTextCol(ctypes.Structure):
     _fields_ = [("offsets", ctypes.c_void_p), ("bytes", ctypes.c_void_p), ("nulls", ctypes.c_void_p)]
 
 
+class KgFmtCol(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("dict_n", ctypes.c_int32), ("data", ctypes.c_void_p),
+                ("nulls", ctypes.c_void_p), ("dict", ctypes.c_void_p), ("dict_off", ctypes.c_void_p)]
+
+
 # enum kg_text_op
 (KG_TEXT_EQ, KG_TEXT_NEQ, KG_TEXT_LT, KG_TEXT_LEQ, KG_TEXT_GT, KG_TEXT_GEQ, KG_TEXT_STARTS_WITH, KG_TEXT_ENDS_WITH,
  KG_TEXT_CONTAINS, KG_TEXT_LIKE) = range(10)
@@ -132,6 +137,10 @@ SIGNATURES = {
     "kg_file_release": (_i32, [_vp, ctypes.c_char_p]),
     "kg_file_enum": (_i32, [_vp, ctypes.c_char_p, ctypes.c_char_p, _i64, _pi64, _i32, _pi32]),
     "kg_file_index": (_i32, [_vp, ctypes.c_char_p, _i32, ctypes.POINTER(_vp)]),
+    "kg_order_by": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, _pi32, ctypes.POINTER(_vp),
+                           ctypes.POINTER(KgTextCol), _vp]),
+    "kg_enum_ranks": (_i32, [_vp, _pi64, _i32, _pi32]),
+    "kg_format_rows": (_i32, [ctypes.POINTER(KgFmtCol), _i32, _i64, _vp, _vp, _i64, _pi64]),
     "kg_intersect_batch": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_create": (_i32, [_vp, _i32, _i64, ctypes.POINTER(_vp)]),
     "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),

@@ -27,6 +27,7 @@ SOURCES = [
     "kg_batch.hip",
     "kg_serve.hip",
     "kg_file.hip",
+    "kg_order.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -697,3 +697,96 @@ def serve(ctx, queries, arrivals, mode, slots=3, max_batch=128, arena_bytes=0, m
                                ctypes.byref(st)))
     stats = {k: getattr(st, k) for k, _ in abi.KgServeStats._fields_}
     return _results(queries, res, bufs), lat[:n], stats
+
+
+# -- ORDER BY and result rows -----------------------------------------------------------
+def enum_ranks(strings):
+    """Rank of each dictionary string (by code) in the order ENUM values compare."""
+    enc = [s.encode() for s in strings]
+    off = np.zeros(len(enc) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(b) for b in enc])
+    buf = ctypes.create_string_buffer(b"".join(enc) or b"\0")
+    out = np.empty(max(len(enc), 1), dtype=np.int32)
+    st = abi.load().kg_enum_ranks(buf, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(enc),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    if st != abi.KG_OK:
+        raise KgError(st, "kg_enum_ranks failed")
+    return out[:len(enc)]
+
+
+def order_by(ctx, keys, ascending, dictionaries=None):
+    """Device int32 permutation: input positions in ORDER BY order. keys:
+    Column or TextColumn objects of equal length; dictionaries: {key index:
+    strings by code} for ENUM keys."""
+    n = len(keys[0])
+    dictionaries = dictionaries or {}
+    dev = f"cuda:{ctx.device}"
+    cols = (abi.KgCol * len(keys))()
+    texts = (abi.KgTextCol * len(keys))()
+    ranks = (ctypes.c_void_p * len(keys))()
+    keep = []
+    for i, k in enumerate(keys):
+        if isinstance(k, TextColumn):
+            cols[i] = abi.KgCol(abi.KG_TEXT, 0, 0, 0)
+            texts[i] = k.desc()
+        else:
+            cols[i] = k.desc()
+            if k.type == abi.KG_ENUM:
+                r = torch.as_tensor(enum_ranks(dictionaries[i])).to(dev)
+                keep.append(r)
+                ranks[i] = r.data_ptr()
+    asc = (ctypes.c_int32 * len(keys))(*[1 if a else 0 for a in ascending])
+    perm = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    ctx.check(ctx.lib.kg_order_by(ctx.h, n, cols, len(keys), asc, ranks, texts, _ptr(perm)))
+    return perm[:n]
+
+
+def format_rows(columns, order=None):
+    """Result rows as the reference prints them. columns: (sql_type, values,
+    nulls or None, dictionary) with host numpy values (ENUM: codes and the
+    strings by code; TEXT: a list of str)."""
+    n = len(columns[0][1]) if columns else 0
+    fc = (abi.KgFmtCol * max(len(columns), 1))()
+    keep = []
+    for i, (typ, vals, nulls, dictionary) in enumerate(columns):
+        if typ == abi.KG_TEXT:
+            strings = [s.encode() for s in vals]
+            off = np.zeros(len(strings) + 1, dtype=np.int64)
+            off[1:] = np.cumsum([len(b) for b in strings])
+            blob = ctypes.create_string_buffer(b"".join(strings) or b"\0")
+            data = np.zeros(1, dtype=np.int8)
+            keep += [off, blob, data]
+            fc[i] = abi.KgFmtCol(typ, 0, data.ctypes.data, 0, ctypes.addressof(blob), off.ctypes.data)
+        else:
+            vals = np.ascontiguousarray(vals)
+            keep.append(vals)
+            d = doff = None
+            dn = 0
+            if typ == abi.KG_ENUM:
+                enc = [s.encode() for s in dictionary]
+                doff = np.zeros(len(enc) + 1, dtype=np.int64)
+                doff[1:] = np.cumsum([len(b) for b in enc])
+                d = ctypes.create_string_buffer(b"".join(enc) or b"\0")
+                dn = len(enc)
+                keep += [doff, d]
+            fc[i] = abi.KgFmtCol(typ, dn, vals.ctypes.data, 0, ctypes.addressof(d) if d is not None else 0,
+                                 doff.ctypes.data if doff is not None else 0)
+        if nulls is not None:
+            nl = np.ascontiguousarray(nulls, dtype=np.int8)
+            keep.append(nl)
+            fc[i].nulls = nl.ctypes.data
+    ordp = 0
+    if order is not None:
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        keep.append(order)
+        ordp = order.ctypes.data
+    lib = abi.load()
+    ln = ctypes.c_int64()
+    st = lib.kg_format_rows(fc, len(columns), n, ordp, None, 0, ctypes.byref(ln))
+    if st != abi.KG_OK:
+        raise KgError(st, "kg_format_rows failed")
+    buf = ctypes.create_string_buffer(max(ln.value, 1))
+    st = lib.kg_format_rows(fc, len(columns), n, ordp, buf, ln.value, ctypes.byref(ln))
+    if st != abi.KG_OK:
+        raise KgError(st, "kg_format_rows failed")
+    return buf.raw[:ln.value].decode()

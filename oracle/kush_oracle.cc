@@ -39,6 +39,11 @@
 //                        ASCII data. (re2 itself is absent: general LIKE is
 //                        pinned through that equivalent regex, not re2.)
 //   kor_text_minmax    - min()/max() over Python bytes.
+//   kor_order_by       - reference fixtures order_by/* (expected .tbl, both
+//                        directions), the reference's own Vector::Sort
+//                        (runtime/vector.cc) in oracle/_ref on random keys
+//                        with ties and NULLs, and Python's stable sorted() of
+//                        the reversed input on small inputs.
 #include "kush_oracle.h"
 
 #include <algorithm>
@@ -822,6 +827,67 @@ int64_t kor_text_minmax(int64_t n, const int64_t* offsets, const uint8_t* bytes,
     if (is_max ? c > 0 : c < 0) best = i;
   }
   return best;
+}
+
+// ---------------------------------------------------------------------------
+// ORDER BY (compile/translators/order_by_translator.cc:50-80): rows are sorted
+// by the reference's top-down merge sort (runtime/vector.cc:44-110), which
+// copies the left element only when comp(left, right) holds, so rows equal on
+// every key end in reverse input order. comp walks the keys: a < b returns
+// asc, b < a returns !asc, all equal returns false. LessThan
+// (compile/proxy/evaluate.cc:339-395): NULL is never less, a non-NULL value is
+// less than NULL; ENUM compares the decoded strings, TEXT bytes as
+// std::string_view, BOOLEAN throws.
+
+static int kor_key_less(const kor_sort_key& k, int32_t a, int32_t b) {
+  const bool na = k.nulls && k.nulls[a], nb = k.nulls && k.nulls[b];
+  if (na) return 0;
+  if (nb) return 1;
+  switch (k.type) {
+    case KOR_SMALLINT: return static_cast<const int16_t*>(k.data)[a] < static_cast<const int16_t*>(k.data)[b];
+    case KOR_INT:
+    case KOR_DATE: return static_cast<const int32_t*>(k.data)[a] < static_cast<const int32_t*>(k.data)[b];
+    case KOR_BIGINT: return static_cast<const int64_t*>(k.data)[a] < static_cast<const int64_t*>(k.data)[b];
+    case KOR_REAL: return static_cast<const double*>(k.data)[a] < static_cast<const double*>(k.data)[b];
+    case KOR_TEXT:
+      return kor_text_cmp(k.str_heap + k.str_off[a], k.str_off[a + 1] - k.str_off[a], k.str_heap + k.str_off[b],
+                          k.str_off[b + 1] - k.str_off[b]) < 0;
+    default: {  // KOR_ENUM
+      const int32_t ca = static_cast<const int32_t*>(k.data)[a], cb = static_cast<const int32_t*>(k.data)[b];
+      return kor_text_cmp(k.str_heap + k.str_off[ca], k.str_off[ca + 1] - k.str_off[ca], k.str_heap + k.str_off[cb],
+                          k.str_off[cb + 1] - k.str_off[cb]) < 0;
+    }
+  }
+}
+
+static bool kor_sort_comp(const kor_sort_key* keys, int32_t nkeys, int32_t a, int32_t b) {
+  for (int32_t i = 0; i < nkeys; i++) {
+    if (kor_key_less(keys[i], a, b)) return keys[i].asc != 0;
+    if (kor_key_less(keys[i], b, a)) return keys[i].asc == 0;
+  }
+  return false;
+}
+
+static void kor_merge_sort(const kor_sort_key* keys, int32_t nkeys, int32_t* v, std::vector<int32_t>& tmp,
+                           int64_t l, int64_t r) {
+  if (l >= r) return;
+  const int64_t m = l + (r - l) / 2;
+  kor_merge_sort(keys, nkeys, v, tmp, l, m);
+  kor_merge_sort(keys, nkeys, v, tmp, m + 1, r);
+  int64_t i = l, j = m + 1, k = 0;
+  while (i <= m && j <= r) tmp[k++] = kor_sort_comp(keys, nkeys, v[i], v[j]) ? v[i++] : v[j++];
+  while (i <= m) tmp[k++] = v[i++];
+  while (j <= r) tmp[k++] = v[j++];
+  std::memcpy(v + l, tmp.data(), static_cast<size_t>(k) * sizeof(int32_t));
+}
+
+int32_t kor_order_by(int64_t n, const kor_sort_key* keys, int32_t nkeys, int32_t* out) {
+  for (int32_t i = 0; i < nkeys; i++)
+    if (keys[i].type == KOR_BOOLEAN) return -1;
+  for (int64_t i = 0; i < n; i++) out[i] = static_cast<int32_t>(i);
+  std::vector<int32_t> tmp(static_cast<size_t>(n > 0 ? n : 1));
+  kor_merge_sort(keys, nkeys, out, tmp, 0, n - 1);
+  return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -115,6 +115,23 @@ int64_t kor_text_select(int64_t n, const int64_t* offsets, const uint8_t* bytes,
 int64_t kor_text_minmax(int64_t n, const int64_t* offsets, const uint8_t* bytes, const int8_t* nulls,
                         const int32_t* rows, int32_t is_max);
 
+// ---- ORDER BY --------------------------------------------------------------
+// One sort key. TEXT: str_off[row .. row + 1] delimit the row's bytes in
+// str_heap (data unused); ENUM: data holds int32 codes and str_off[code ..
+// code + 1] delimit the code's dictionary string; other types: data holds the
+// typed values. nulls: int8 flags or nullptr.
+typedef struct {
+  int32_t type;
+  int32_t asc;
+  const void* data;
+  const int8_t* nulls;
+  const int64_t* str_off;
+  const uint8_t* str_heap;
+} kor_sort_key;
+
+// Row positions 0..n-1 in ORDER BY order. Returns 0, or -1 for a BOOLEAN key.
+int32_t kor_order_by(int64_t n, const kor_sort_key* keys, int32_t nkeys, int32_t* out);
+
 // ---- workload compositions (CPU baseline legs) ------------------------------
 int64_t kor_q_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
                        const int32_t* g2, int32_t nf, const double* const* fm, int32_t ni,

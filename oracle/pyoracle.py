@@ -29,6 +29,16 @@ class KorAgg(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("input", KorCol)]
 
 
+class KorSortKey(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("asc", ctypes.c_int32), ("data", ctypes.c_void_p),
+                ("nulls", ctypes.c_void_p), ("str_off", ctypes.c_void_p), ("str_heap", ctypes.c_void_p)]
+
+
+class RefSortKey(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("asc", ctypes.c_int32), ("data", ctypes.c_void_p),
+                ("nulls", ctypes.c_void_p)]
+
+
 class KorPred(ctypes.Structure):
     _fields_ = [("col", ctypes.c_int32), ("op", ctypes.c_int32), ("ilit", ctypes.c_int64),
                 ("flit", ctypes.c_double)]
@@ -109,6 +119,8 @@ def lib():
         L.kor_text_minmax.restype = ctypes.c_int64
         L.kor_text_minmax.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_int32]
+        L.kor_order_by.restype = ctypes.c_int32
+        L.kor_order_by.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
         L.kor_dtree_scan_agg.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -137,6 +149,7 @@ def ref():
         R.ref_hash_join_i64.restype = ctypes.c_int64
         R.ref_hash_join_i64.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        R.ref_sort_rows.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
         R.ref_group_by_i64.restype = ctypes.c_int64
         R.ref_group_by_i64.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -150,10 +163,11 @@ def _p(a):
 class Col:
     """numpy column (+ optional int8 nulls) for the oracle."""
 
-    def __init__(self, typ, data, nulls=None):
+    def __init__(self, typ, data, nulls=None, dictionary=None):
         self.type = typ
         self.data = np.ascontiguousarray(data, dtype=_NP[typ])
         self.nulls = None if nulls is None else np.ascontiguousarray(nulls, dtype=np.int8)
+        self.dictionary = dictionary  # ENUM: strings by code
 
     def kor(self):
         return KorCol(self.type, self.data.ctypes.data, _p(self.nulls))
@@ -299,3 +313,53 @@ def text_minmax(col, rows=None, is_max=False):
     rr = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
     return lib().kor_text_minmax(n, col.offsets.ctypes.data, col.heap.ctypes.data, _p(col.nulls), _p(rr),
                                  1 if is_max else 0)
+
+
+def _strings(strs):
+    enc = [x if isinstance(x, bytes) else x.encode() for x in strs]
+    off = np.zeros(len(enc) + 1, dtype=np.int64)
+    off[1:] = np.cumsum([len(b) for b in enc])
+    heap = np.frombuffer(b"".join(enc) + b"\0", dtype=np.uint8).copy()
+    return off, heap
+
+
+def order_by(keys):
+    """keys: list of (Col or TextCol-like tuple, asc). A Col of type ENUM
+    carries its dictionary as col.dictionary (strings by code); a TEXT key is
+    ("text", strings, nulls). Returns row positions in ORDER BY order, or None
+    when a key is BOOLEAN (the reference throws)."""
+    ks = (KorSortKey * max(len(keys), 1))()
+    keep = []
+    n = None
+    for i, (c, asc) in enumerate(keys):
+        if isinstance(c, tuple) and c[0] == "text":
+            _, strs, nulls = c
+            off, heap = _strings(strs)
+            nl = None if nulls is None else np.ascontiguousarray(nulls, dtype=np.int8)
+            keep += [off, heap, nl]
+            ks[i] = KorSortKey(TEXT, 1 if asc else 0, None, _p(nl), off.ctypes.data, heap.ctypes.data)
+            n = len(strs)
+        else:
+            so = sh = None
+            if c.type == ENUM:
+                off, heap = _strings(c.dictionary)
+                keep += [off, heap]
+                so, sh = off.ctypes.data, heap.ctypes.data
+            ks[i] = KorSortKey(c.type, 1 if asc else 0, c.data.ctypes.data, _p(c.nulls), so, sh)
+            n = len(c.data)
+    out = np.empty(max(n, 1), dtype=np.int32)
+    if lib().kor_order_by(n, ks, len(keys), out.ctypes.data) != 0:
+        return None
+    return out[:n].copy()
+
+
+def ref_sort_rows(keys):
+    """The reference's Vector::Sort over row positions (oracle/_ref), for
+    typed non-string keys: list of (Col, asc)."""
+    ks = (RefSortKey * max(len(keys), 1))()
+    for i, (c, asc) in enumerate(keys):
+        ks[i] = RefSortKey(c.type, 1 if asc else 0, c.data.ctypes.data, _p(c.nulls))
+    n = len(keys[0][0].data)
+    out = np.empty(max(n, 1), dtype=np.int32)
+    ref().ref_sort_rows(n, ks, len(keys), out.ctypes.data)
+    return out[:n].copy()

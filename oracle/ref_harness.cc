@@ -6,7 +6,8 @@
 //
 // The harness drives:
 //   runtime/hash_table.cc          HashCombine, Create/Insert/GetBucket/Free
-//   runtime/vector.cc              bucket vectors
+//   runtime/vector.cc              bucket vectors; Sort (the ORDER BY merge
+//                                  sort) over row positions
 //   runtime/aggregate_hash_table.cc Init/Resize/AllocateNewPage/ConstructEntry/
 //                                  GetEntry/GetPayload/Size/Free
 //   runtime/column_index_bucket.cc FastForwardBucket, sorted intersection
@@ -199,6 +200,54 @@ int64_t ref_group_by_i64(int64_t n, const int64_t* keys, const int64_t* vals, in
   }
   AggregateHashTable::Free(&ht);
   return ng == g ? ng : -1;
+}
+
+// ORDER BY's sort (compile/translators/order_by_translator.cc:50-80): the
+// reference's Vector::Sort over 4-byte row positions with a comparison
+// function that walks the keys as the generated comparator does, NULLs
+// ordered by compile/proxy/evaluate.cc:339-395 (NULL never less, non-NULL
+// less than NULL). Keys are typed integer / REAL columns (no strings here;
+// TEXT and ENUM ordering is pinned through the fixtures).
+struct RefSortKey {
+  int32_t type;  // 0 SMALLINT, 1 INT, 2 BIGINT, 3 REAL, 4 DATE
+  int32_t asc;
+  const void* data;
+  const int8_t* nulls;
+};
+static const RefSortKey* g_sort_keys;
+static int32_t g_sort_nkeys;
+
+static bool ref_key_less(const RefSortKey& k, int32_t a, int32_t b) {
+  if (k.nulls && k.nulls[a]) return false;
+  if (k.nulls && k.nulls[b]) return true;
+  switch (k.type) {
+    case 0: return static_cast<const int16_t*>(k.data)[a] < static_cast<const int16_t*>(k.data)[b];
+    case 2: return static_cast<const int64_t*>(k.data)[a] < static_cast<const int64_t*>(k.data)[b];
+    case 3: return static_cast<const double*>(k.data)[a] < static_cast<const double*>(k.data)[b];
+    default: return static_cast<const int32_t*>(k.data)[a] < static_cast<const int32_t*>(k.data)[b];
+  }
+}
+
+static bool ref_sort_comp(int8_t* x, int8_t* y) {
+  int32_t a, b;
+  std::memcpy(&a, x, 4);
+  std::memcpy(&b, y, 4);
+  for (int32_t i = 0; i < g_sort_nkeys; i++) {
+    if (ref_key_less(g_sort_keys[i], a, b)) return g_sort_keys[i].asc != 0;
+    if (ref_key_less(g_sort_keys[i], b, a)) return g_sort_keys[i].asc == 0;
+  }
+  return false;
+}
+
+void ref_sort_rows(int32_t n, const RefSortKey* keys, int32_t nkeys, int32_t* out) {
+  g_sort_keys = keys;
+  g_sort_nkeys = nkeys;
+  Vector::Vector vec;
+  Vector::Create(&vec, 4, n > 0 ? n : 1);
+  for (int32_t i = 0; i < n; i++) std::memcpy(Vector::PushBack(&vec), &i, 4);
+  Vector::Sort(&vec, ref_sort_comp);
+  for (int32_t i = 0; i < n; i++) std::memcpy(&out[i], Vector::Get(&vec, i), 4);
+  Vector::Free(&vec);
 }
 
 }  // extern "C"

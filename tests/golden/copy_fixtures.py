@@ -20,7 +20,9 @@ DATA = (
        "people_id.kdb", "people_id_null.kdb", "n_nationkey.kdb",
        "l_shipdate.kdb", "l_quantity.kdb", "l_extendedprice.kdb", "l_discount.kdb", "l_tax.kdb",
        "l_returnflag_enum.kdb", "l_linestatus_enum.kdb",
-       "l_returnflag_enum.kdbenum", "l_linestatus_enum.kdbenum"]
+       "l_returnflag_enum.kdbenum", "l_linestatus_enum.kdbenum",
+       "people_name.kdb", "people_name_null.kdb", "people_name_enum.kdb", "people_name_enum_null.kdb",
+       "people_name_enum.kdbenum"]
 )
 
 EXPECTED = (
@@ -29,6 +31,7 @@ EXPECTED = (
     + [f"group_by_aggregate/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "real", "date")]
     + ["aggregate/int_expected.tbl", "aggregate/empty_expected.tbl", "tpch/q01_expected.tbl"]
     + [f"simd_scan_select/{t}_expected.tbl" for t in ("int", "date_range", "multiple_col")]
+    + [f"order_by/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text")]
 )
 
 

@@ -30,6 +30,19 @@ def col(name, typ):
     return v, nulls
 
 
+def text(name):
+    """Returns (strings, nulls or None) of a TEXT column file: {int32 count;
+    4 bytes padding} then {int32 length; 4 bytes padding; uint64 offset} per
+    row, offsets from the file start (runtime/column_data.cc:175-229)."""
+    raw = open(os.path.join(DATA, name + ".kdb"), "rb").read()
+    n = int(np.frombuffer(raw, dtype=np.int32, count=1, offset=0)[0])
+    slots = np.frombuffer(raw, dtype=np.dtype([("len", "<i4"), ("pad", "<i4"), ("off", "<u8")]), count=n, offset=8)
+    out = [raw[int(o):int(o) + int(ln)].decode() for ln, o in zip(slots["len"], slots["off"])]
+    npath = os.path.join(DATA, name + "_null.kdb")
+    nulls = np.fromfile(npath, dtype=np.int8) if os.path.exists(npath) else None
+    return out, nulls
+
+
 def enum_dict(name):
     """Code -> string of a .kdbenum file (reads the entry array by code)."""
     raw = open(os.path.join(DATA, name + ".kdbenum"), "rb").read()
