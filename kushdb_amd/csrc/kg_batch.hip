@@ -43,7 +43,7 @@
 
 namespace kg {
 
-constexpr int kBTile = 4096;          // rows per scan/probe work item
+constexpr int kBTile = 16384;         // rows per scan/probe work item
 constexpr int kBBuildTile = 512;      // rows per build work item
 constexpr int kBThreads = 256;        // threads per workgroup
 constexpr int kBWaves = kBThreads / kWave;
@@ -55,7 +55,7 @@ constexpr int kBLdsWords = 2048;      // LDS path: 8-byte words of the item tabl
 constexpr int kBFastPreds = 4;
 constexpr int kBMaxAggs = 16;
 constexpr int kBMaxFills = 6 * kBMaxQueries + 8;
-constexpr int kBMapMax = 65536;       // direct-map joins: at most this many key offsets
+constexpr int kBMapMax = 1 << 21;     // direct-map joins: at most this many key offsets
 constexpr uint32_t kMapEmpty = 0xFFFFFFFFu;
 
 // kFlagNoMap is internal (the direct map of a join cannot be used) and is not
@@ -170,11 +170,9 @@ struct BHeader {
   int32_t nq, nfills, nbq, pad;
   int32_t bpre[kBMaxQueries + 1];             // build phase: item prefix over bq entries
   int32_t bq[kBMaxQueries];                   // build phase: query of entry j
-  unsigned int bhead;
   int32_t nrq;                                // key-range phase (direct-map joins only):
   int32_t rpre[kBMaxQueries + 1];             //   item prefix over rq entries
   int32_t rq[kBMaxQueries];                   //   query of entry j
-  unsigned int rhead;                         //   next item
   STiles shared;                              // register-path scans, by group
   BQueues join;                               // joins with a typed view
   BQueues gen;                                // the others
@@ -221,34 +219,21 @@ __global__ __launch_bounds__(kBThreads) void k_batch_clear(const BHeader* h) {
   }
 }
 
-// Takes the next item (of `total`) of a queue for the whole workgroup.
-__device__ __forceinline__ int32_t take_item(unsigned int* head, int32_t total, int32_t* s_it) {
-  if (threadIdx.x == 0) {
-    // An exhausted queue is detected without an atomic (heads only grow).
-    uint32_t v = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (static_cast<int32_t>(v) < total) v = atomicAdd(head, 1u);
-    *s_it = static_cast<int32_t>(v);
-  }
-  __syncthreads();
-  // Uniform across the workgroup: keeps descriptor reads on the scalar path.
-  const int32_t it = __builtin_amdgcn_readfirstlane(*s_it);
-  __syncthreads();
-  return it;
-}
-
 __device__ __forceinline__ uint64_t coherent_load(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Key range of a direct-map join: min and max of the qualifying build keys,
-// over the build items (before the build phase).
+// over the build items (before the build phase). Items are dealt by a grid
+// stride (no shared counter); each workgroup reduces its item's range and
+// touches the query's two words only when it widens them, so the atomics do
+// not queue up on one address.
 __global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQuery* qs) {
-  __shared__ int32_t s_it;
+  __shared__ int64_t s_red[2][kBWaves];
   const int32_t total = h->rpre[h->nrq];
   const int lane = threadIdx.x & (kWave - 1);
-  while (true) {
-    const int32_t it = take_item(&h->rhead, total, &s_it);
-    if (it >= total) break;
+  const int wave = threadIdx.x / kWave;
+  for (int32_t it = blockIdx.x; it < total; it += gridDim.x) {
     const int j = find_prefix(h->rpre, h->nrq, it);
     const BQuery& d = qs[h->rq[j]];
     const int64_t lo = static_cast<int64_t>(it - h->rpre[j]) * kBBuildTile;
@@ -262,15 +247,29 @@ __global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQu
       mx = k > mx ? k : mx;
     }
     for (int s = kWave / 2; s >= 1; s >>= 1) {
-      const int64_t a = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(nmin), s));
-      const int64_t b = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(mx), s));
-      nmin = a > nmin ? a : nmin;
-      mx = b > mx ? b : mx;
+      const int64_t x = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(nmin), s));
+      const int64_t y = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(mx), s));
+      nmin = x > nmin ? x : nmin;
+      mx = y > mx ? y : mx;
     }
-    if (lane == 0 && mx != INT64_MIN) {
-      atomicMax(reinterpret_cast<long long*>(&d.krange[0]), static_cast<long long>(nmin));
-      atomicMax(reinterpret_cast<long long*>(&d.krange[1]), static_cast<long long>(mx));
+    if (lane == 0) {
+      s_red[0][wave] = nmin;
+      s_red[1][wave] = mx;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < kBWaves; w++) {
+        nmin = s_red[0][w] > nmin ? s_red[0][w] : nmin;
+        mx = s_red[1][w] > mx ? s_red[1][w] : mx;
+      }
+      if (mx != INT64_MIN) {
+        if (nmin > static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[0]))))
+          atomicMax(reinterpret_cast<long long*>(&d.krange[0]), static_cast<long long>(nmin));
+        if (mx > static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[1]))))
+          atomicMax(reinterpret_cast<long long*>(&d.krange[1]), static_cast<long long>(mx));
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -295,13 +294,11 @@ __device__ __forceinline__ bool map_usable(const BQuery& d, int64_t* mn, int64_t
 
 // Build phase: items (query, build tile); same insertion as k_ht_build, and
 // for a direct-map join the entry of the key in the map (a repeated key, or a
-// group code outside [0, G), turns the map off).
+// group code outside [0, G), turns the map off). Items are dealt by a grid
+// stride.
 __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQuery* qs) {
-  __shared__ int32_t s_it;
   const int32_t total = h->bpre[h->nbq];
-  while (true) {
-    const int32_t it = take_item(&h->bhead, total, &s_it);
-    if (it >= total) break;
+  for (int32_t it = blockIdx.x; it < total; it += gridDim.x) {
     const int j = find_prefix(h->bpre, h->nbq, it);
     const BQuery& d = qs[h->bq[j]];
     const int64_t lo = static_cast<int64_t>(it - h->bpre[j]) * kBBuildTile;
@@ -1626,8 +1623,8 @@ int capture(kg_batch* b) {
   kg_ctx* ctx = b->ctx;
   KG_HIP_CHECK(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(k_batch_clear, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr());
-  hipLaunchKernelGGL(k_batch_range, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
-  hipLaunchKernelGGL(k_batch_build, dim3(b->grid), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
+  hipLaunchKernelGGL(k_batch_range, dim3(b->grid_gen), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
+  hipLaunchKernelGGL(k_batch_build, dim3(b->grid_gen), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
   hipLaunchKernelGGL(k_batch_shared, dim3(b->grid_shared), dim3(kSThreads), 0, b->stream, b->dhdr(), b->dq(),
                      b->dg());
   hipLaunchKernelGGL(k_batch_join, dim3(b->grid_gen), dim3(kBThreads), 0, b->stream, b->dhdr(), b->dq());
