@@ -567,6 +567,73 @@ typedef struct kg_fmt_col {
 int kg_format_rows(const kg_fmt_col* cols, int32_t ncols, int64_t n, const int32_t* order, char* buf, int64_t cap,
                    int64_t* out_len);
 
+/* ---- general expressions ---------------------------------------------------
+ * The ExpressionTranslator (compile/translators/expression_translator.cc:43-360)
+ * and the evaluation rules it calls (compile/proxy/evaluate.cc:11-338) over a
+ * column set, evaluated on the device. An expression is an array of nodes in
+ * which every child precedes its parent; the last node is the root. Types
+ * follow the plan's rules (plan/expression/arithmetic_expression.cc:11-78):
+ * both operands of a binary node have the same type, except ENUM against a
+ * TEXT literal; comparisons, AND/OR and the string predicates are BOOLEAN;
+ * ADD/SUB/MUL keep the operand type (integer types wrap); DIV takes REAL.
+ * NULL rules as the reference's: arithmetic, comparisons and the string
+ * predicates are NULL when an operand is; AND is NULL when its left operand
+ * is NULL, else FALSE when it is FALSE, else the right operand; OR is the
+ * right operand when the left is NULL or FALSE, else TRUE; NOT keeps NULL;
+ * CASE takes the ELSE branch for a NULL condition; IS_NULL is never NULL; an
+ * integer-to-REAL conversion turns NULL into a non-NULL 0.0; LIKE and IN on
+ * a NULL value are a non-NULL FALSE; EXTRACT(YEAR) keeps NULL. */
+enum kg_expr_op {
+  KG_E_COL = 0,          /* column a of the column set */
+  KG_E_LIT = 1,          /* literal: ilit (integer types, DATE as a Julian day,
+                            BOOLEAN, ENUM code), flit (REAL), str/str_len (TEXT);
+                            flags & KG_E_NULL: the NULL literal */
+  KG_E_ADD = 2, KG_E_SUB = 3, KG_E_MUL = 4, KG_E_DIV = 5,
+  KG_E_EQ = 6, KG_E_NEQ = 7, KG_E_LT = 8, KG_E_LEQ = 9, KG_E_GT = 10, KG_E_GEQ = 11,
+  KG_E_AND = 12, KG_E_OR = 13,
+  KG_E_STARTS_WITH = 14, KG_E_ENDS_WITH = 15, KG_E_CONTAINS = 16,
+  KG_E_NOT = 17, KG_E_IS_NULL = 18,
+  KG_E_CASE = 19,        /* CASE WHEN a THEN b ELSE c */
+  KG_E_INT_TO_REAL = 20,
+  KG_E_EXTRACT_YEAR = 21,
+  KG_E_LIKE = 22,        /* a LIKE str (the SQL pattern: '%', '_') */
+  KG_E_IN = 23           /* ENUM a IN (values[0 .. nvalues)) */
+};
+#define KG_E_NULL 1
+
+typedef struct kg_expr_node {
+  int32_t op;            /* kg_expr_op */
+  int32_t type;          /* result type (kg_type) */
+  int32_t a, b, c;       /* children (node indices), or the column (KG_E_COL) */
+  int32_t flags;
+  int64_t ilit;
+  double flit;
+  const char* str;       /* host bytes */
+  int32_t str_len;
+  int32_t nvalues;
+  const int32_t* values; /* host ENUM codes of KG_E_IN */
+} kg_expr_node;
+
+/* ENUM dictionary of a column (host strings by code), for the operators that
+ * compare decoded strings (evaluate.cc:222-300, expression_translator.cc:275-296). */
+typedef struct kg_enum_dict {
+  const char* bytes;
+  const int64_t* offsets;  /* count + 1 */
+  int32_t count;
+} kg_enum_dict;
+
+/* Evaluates the expression for rows d_rows[0 .. n) (or 0 .. n) of the column
+ * set into out (device data of the root type's width and int8 NULL flags,
+ * both n long). cols[i].type KG_TEXT takes text_cols[i]; dicts may be NULL
+ * when no ENUM string operator is used. At most 64 nodes. */
+int kg_expr_eval(kg_ctx* ctx, int64_t n, const kg_col* cols, const kg_text_col* text_cols, const kg_enum_dict* dicts,
+                 int32_t ncols, const kg_expr_node* nodes, int32_t nnodes, const int32_t* d_rows, kg_col* out);
+/* SelectTranslator (compile/translators/select_translator.cc:23-37): the rows
+ * (ascending) whose BOOLEAN expression is TRUE and not NULL. */
+int kg_expr_select(kg_ctx* ctx, int64_t n, const kg_col* cols, const kg_text_col* text_cols,
+                   const kg_enum_dict* dicts, int32_t ncols, const kg_expr_node* nodes, int32_t nnodes,
+                   const int32_t* d_rows, int32_t* d_out_rows, int64_t* out_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,23 @@ class KgFmtCol(ctypes.Structure):
                 ("nulls", ctypes.c_void_p), ("dict", ctypes.c_void_p), ("dict_off", ctypes.c_void_p)]
 
 
+class KgExprNode(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int32), ("type", ctypes.c_int32), ("a", ctypes.c_int32), ("b", ctypes.c_int32),
+                ("c", ctypes.c_int32), ("flags", ctypes.c_int32), ("ilit", ctypes.c_int64), ("flit", ctypes.c_double),
+                ("str", ctypes.c_void_p), ("str_len", ctypes.c_int32), ("nvalues", ctypes.c_int32),
+                ("values", ctypes.c_void_p)]
+
+
+class KgEnumDict(ctypes.Structure):
+    _fields_ = [("bytes", ctypes.c_void_p), ("offsets", ctypes.c_void_p), ("count", ctypes.c_int32)]
+
+
+# enum kg_expr_op
+(KG_E_COL, KG_E_LIT, KG_E_ADD, KG_E_SUB, KG_E_MUL, KG_E_DIV, KG_E_EQ, KG_E_NEQ, KG_E_LT, KG_E_LEQ, KG_E_GT, KG_E_GEQ,
+ KG_E_AND, KG_E_OR, KG_E_STARTS_WITH, KG_E_ENDS_WITH, KG_E_CONTAINS, KG_E_NOT, KG_E_IS_NULL, KG_E_CASE,
+ KG_E_INT_TO_REAL, KG_E_EXTRACT_YEAR, KG_E_LIKE, KG_E_IN) = range(24)
+KG_E_NULL = 1
+
 # enum kg_text_op
 (KG_TEXT_EQ, KG_TEXT_NEQ, KG_TEXT_LT, KG_TEXT_LEQ, KG_TEXT_GT, KG_TEXT_GEQ, KG_TEXT_STARTS_WITH, KG_TEXT_ENDS_WITH,
  KG_TEXT_CONTAINS, KG_TEXT_LIKE) = range(10)
@@ -141,6 +158,10 @@ SIGNATURES = {
                            ctypes.POINTER(KgTextCol), _vp]),
     "kg_enum_ranks": (_i32, [_vp, _pi64, _i32, _pi32]),
     "kg_format_rows": (_i32, [ctypes.POINTER(KgFmtCol), _i32, _i64, _vp, _vp, _i64, _pi64]),
+    "kg_expr_eval": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), ctypes.POINTER(KgTextCol), ctypes.POINTER(KgEnumDict),
+                            _i32, ctypes.POINTER(KgExprNode), _i32, _vp, ctypes.POINTER(KgCol)]),
+    "kg_expr_select": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), ctypes.POINTER(KgTextCol),
+                              ctypes.POINTER(KgEnumDict), _i32, ctypes.POINTER(KgExprNode), _i32, _vp, _vp, _pi64]),
     "kg_intersect_batch": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_create": (_i32, [_vp, _i32, _i64, ctypes.POINTER(_vp)]),
     "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),

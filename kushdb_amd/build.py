@@ -28,6 +28,7 @@ SOURCES = [
     "kg_serve.hip",
     "kg_file.hip",
     "kg_order.hip",
+    "kg_expr.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "kg_ops.h"
+#include "kg_strops.h"
 
 namespace kg {
 
@@ -35,76 +36,8 @@ struct Pattern {
   uint8_t p[kMaxPattern];
 };
 
-// a < b on unsigned bytes, a proper prefix sorting first; returns -1/0/1.
-__device__ __forceinline__ int str_cmp(const uint8_t* a, int64_t na, const uint8_t* b, int64_t nb) {
-  const int64_t m = na < nb ? na : nb;
-  for (int64_t i = 0; i < m; i++) {
-    if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
-  }
-  return na < nb ? -1 : (na > nb ? 1 : 0);
-}
-
-// Anchored LIKE: '%' matches any sequence, '_' exactly one byte, other bytes
-// themselves. Greedy matching with backtracking to the last '%' (linear in
-// practice, O(n*m) worst case), equivalent to the anchored regex full match.
-__device__ bool like_match(const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
-  int64_t i = 0, star_i = -1;
-  int32_t j = 0, star_j = -1;
-  while (i < n) {
-    if (j < m && p[j] == '%') {
-      star_j = j++;
-      star_i = i;
-    } else if (j < m && (p[j] == '_' || p[j] == s[i])) {
-      i++;
-      j++;
-    } else if (star_j >= 0) {
-      j = star_j + 1;
-      i = ++star_i;
-    } else {
-      return false;
-    }
-  }
-  while (j < m && p[j] == '%') j++;
-  return j == m;
-}
-
-__device__ bool text_pred(const Pattern& P, const uint8_t* s, int64_t n) {
-  const uint8_t* p = P.p;
-  const int32_t m = P.len;
-  switch (P.op) {
-    case KG_TEXT_STARTS_WITH:
-      if (n < m) return false;
-      for (int32_t i = 0; i < m; i++)
-        if (s[i] != p[i]) return false;
-      return true;
-    case KG_TEXT_ENDS_WITH:
-      if (n < m) return false;
-      for (int32_t i = 0; i < m; i++)
-        if (s[n - m + i] != p[i]) return false;
-      return true;
-    case KG_TEXT_CONTAINS: {
-      if (m == 0) return true;
-      for (int64_t st = 0; st + m <= n; st++) {
-        int32_t i = 0;
-        while (i < m && s[st + i] == p[i]) i++;
-        if (i == m) return true;
-      }
-      return false;
-    }
-    case KG_TEXT_LIKE:
-      return like_match(s, n, p, m);
-    default: {
-      const int c = str_cmp(s, n, p, m);
-      switch (P.op) {
-        case KG_TEXT_EQ: return c == 0;
-        case KG_TEXT_NEQ: return c != 0;
-        case KG_TEXT_LT: return c < 0;
-        case KG_TEXT_LEQ: return c <= 0;
-        case KG_TEXT_GT: return c > 0;
-        default: return c >= 0;
-      }
-    }
-  }
+__device__ __forceinline__ bool text_pred(const Pattern& P, const uint8_t* s, int64_t n) {
+  return text_match(P.op, s, n, P.p, P.len);
 }
 
 __global__ void k_text_flags(kg_text_col col, Pattern P, const int32_t* sel, int64_t n, int32_t* ids,

@@ -39,6 +39,13 @@
 //                        ASCII data. (re2 itself is absent: general LIKE is
 //                        pinned through that equivalent regex, not re2.)
 //   kor_text_minmax    - min()/max() over Python bytes.
+//   kor_expr_eval      - reference fixtures select/* (every type, sorted rows),
+//                        tpch/q06 (filter + SUM of a product) and tpch/q01
+//                        (derived measures), the CASE group key of
+//                        group_by_aggregate/int; numpy wrapping arithmetic and
+//                        comparisons, an enumerated truth table of the AND/OR/NOT
+//                        NULL rules, Python's re.fullmatch for LIKE and
+//                        datetime for EXTRACT(YEAR).
 //   kor_order_by       - reference fixtures order_by/* (expected .tbl, both
 //                        directions), the reference's own Vector::Sort
 //                        (runtime/vector.cc) in oracle/_ref on random keys
@@ -887,6 +894,217 @@ int32_t kor_order_by(int64_t n, const kor_sort_key* keys, int32_t nkeys, int32_t
   for (int64_t i = 0; i < n; i++) out[i] = static_cast<int32_t>(i);
   std::vector<int32_t> tmp(static_cast<size_t>(n > 0 ? n : 1));
   kor_merge_sort(keys, nkeys, out, tmp, 0, n - 1);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// General expressions: the ExpressionTranslator (compile/translators/
+// expression_translator.cc:43-360) with compile/proxy/evaluate.cc:11-338, one
+// row at a time, children before parents. Values are int64 (integers
+// sign-extended, DATE Julian day, BOOLEAN 0/1, ENUM code, TEXT row) or REAL.
+
+namespace {
+
+struct XV {
+  int64_t i = 0;
+  double f = 0.0;
+  bool null = false;
+};
+
+std::string kor_dict_str(const kor_expr_col& c, int64_t code) {
+  return std::string(c.dict + c.dict_off[code], static_cast<size_t>(c.dict_off[code + 1] - c.dict_off[code]));
+}
+
+bool kor_like_rec(const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
+  // Plain recursive definition of the anchored pattern ('%' any run, '_' one byte).
+  if (m == 0) return n == 0;
+  if (p[0] == '%') {
+    for (int64_t k = 0; k <= n; k++)
+      if (kor_like_rec(s + k, n - k, p + 1, m - 1)) return true;
+    return false;
+  }
+  if (n == 0) return false;
+  if (p[0] != '_' && p[0] != s[0]) return false;
+  return kor_like_rec(s + 1, n - 1, p + 1, m - 1);
+}
+
+int64_t kor_year(int64_t jd) {
+  int32_t y, m, d;
+  kor_split_date(static_cast<int32_t>(jd), &y, &m, &d);
+  return y;
+}
+
+}  // namespace
+
+int32_t kor_expr_eval(int64_t n, const kor_expr_col* cols, int32_t ncols, const kor_expr_node* nodes, int32_t nn,
+                      const int32_t* rows, int64_t* out, int8_t* out_nulls) {
+  std::vector<XV> v(static_cast<size_t>(nn));
+  auto str_of = [&](int node, int64_t row, std::string& s) {
+    const kor_expr_node& N = nodes[node];
+    if (N.op == 1) {  // literal
+      s.assign(N.str ? N.str : "", static_cast<size_t>(N.str_len));
+    } else if (N.type == KOR_ENUM) {
+      s = kor_dict_str(cols[N.a], v[node].i);
+    } else {  // TEXT column
+      const kor_expr_col& c = cols[N.a];
+      s.assign(reinterpret_cast<const char*>(c.heap + c.offsets[row]),
+               static_cast<size_t>(c.offsets[row + 1] - c.offsets[row]));
+    }
+  };
+  auto enum_col = [&](int node) -> int { return nodes[node].op == 0 ? nodes[node].a : -1; };
+  for (int64_t i = 0; i < n; i++) {
+    const int64_t r = rows ? rows[i] : i;
+    for (int k = 0; k < nn; k++) {
+      const kor_expr_node& N = nodes[k];
+      XV x;
+      const XV A = (N.op >= 2 && N.a >= 0) ? v[N.a] : XV();
+      const XV B = (N.op >= 2 && N.op <= 16 && N.b >= 0) ? v[N.b] : XV();
+      switch (N.op) {
+        case 0: {  // column
+          if (N.a < 0 || N.a >= ncols) return -1;
+          const kor_expr_col& c = cols[N.a];
+          x.null = c.nulls && c.nulls[r];
+          switch (c.type) {
+            case KOR_SMALLINT: x.i = static_cast<const int16_t*>(c.data)[r]; break;
+            case KOR_INT:
+            case KOR_DATE:
+            case KOR_ENUM: x.i = static_cast<const int32_t*>(c.data)[r]; break;
+            case KOR_BIGINT: x.i = static_cast<const int64_t*>(c.data)[r]; break;
+            case KOR_REAL: x.f = static_cast<const double*>(c.data)[r]; break;
+            case KOR_BOOLEAN: x.i = static_cast<const int8_t*>(c.data)[r] ? 1 : 0; break;
+            default: x.i = r; break;  // TEXT
+          }
+          break;
+        }
+        case 1:  // literal
+          x.i = N.ilit;
+          x.f = N.flit;
+          x.null = (N.flags & 1) != 0;
+          break;
+        case 2:
+        case 3:
+        case 4:
+        case 5: {  // ADD SUB MUL DIV: NULL when either operand is (evaluate.cc:90-135)
+          const int32_t t = nodes[N.a].type;
+          x.null = A.null || B.null;
+          if (x.null) break;
+          if (t == KOR_REAL) {
+            x.f = N.op == 2 ? A.f + B.f : N.op == 3 ? A.f - B.f : N.op == 4 ? A.f * B.f : A.f / B.f;
+          } else {
+            if (N.op == 5) return -1;  // DIV takes REAL only
+            const uint64_t a = static_cast<uint64_t>(A.i), b = static_cast<uint64_t>(B.i);
+            const uint64_t w = N.op == 2 ? a + b : N.op == 3 ? a - b : a * b;
+            x.i = t == KOR_SMALLINT ? static_cast<int16_t>(w) : t == KOR_INT ? static_cast<int32_t>(w) : static_cast<int64_t>(w);
+          }
+          break;
+        }
+        case 6: case 7: case 8: case 9: case 10: case 11:
+        case 14: case 15: case 16: {  // comparisons and string predicates
+          x.null = A.null || B.null;
+          if (x.null) break;
+          const int32_t ta = nodes[N.a].type, tb = nodes[N.b].type;
+          int c = 0;
+          bool strop = N.op >= 14, res = false;
+          if (ta == KOR_TEXT || (ta == KOR_ENUM && (tb == KOR_TEXT || (N.op >= 8 && N.op <= 16)))) {
+            std::string sa, sb;
+            if (ta == KOR_ENUM) {
+              const int ca = enum_col(N.a);
+              if (ca < 0 || !cols[ca].dict) return -1;
+              sa = kor_dict_str(cols[ca], A.i);
+              if (tb == KOR_ENUM) sb = kor_dict_str(cols[ca], B.i);
+              else str_of(N.b, r, sb);
+            } else {
+              str_of(N.a, r, sa);
+              str_of(N.b, r, sb);
+            }
+            const uint8_t* pa = reinterpret_cast<const uint8_t*>(sa.data());
+            const uint8_t* pb = reinterpret_cast<const uint8_t*>(sb.data());
+            const int64_t la = static_cast<int64_t>(sa.size()), lb = static_cast<int64_t>(sb.size());
+            if (strop) {
+              if (N.op == 14) res = sa.compare(0, sb.size(), sb) == 0 && la >= lb;
+              else if (N.op == 15) res = la >= lb && sa.compare(sa.size() - sb.size(), sb.size(), sb) == 0;
+              else res = sa.find(sb) != std::string::npos;
+            } else {
+              c = kor_text_cmp(pa, la, pb, lb);
+            }
+          } else if (ta == KOR_REAL) {
+            c = A.f < B.f ? -1 : (A.f > B.f ? 1 : (A.f == B.f ? 0 : 2));  // 2: unordered
+          } else {
+            c = A.i < B.i ? -1 : (A.i > B.i ? 1 : 0);
+          }
+          if (!strop) {
+            switch (N.op) {
+              case 6: res = c == 0; break;
+              case 7: res = c != 0; break;
+              case 8: res = c == -1; break;
+              case 9: res = c == -1 || c == 0; break;
+              case 10: res = c == 1; break;
+              default: res = c == 1 || c == 0; break;
+            }
+          }
+          x.i = res ? 1 : 0;
+          break;
+        }
+        case 12:  // AND (expression_translator.cc:77-95)
+          if (A.null) x.null = true;
+          else if (A.i) x = B;
+          else x.i = 0;
+          break;
+        case 13:  // OR (expression_translator.cc:97-111)
+          if (A.null || !A.i) x = B;
+          else x.i = 1;
+          break;
+        case 17:  // NOT
+          x.null = A.null;
+          x.i = A.null ? 0 : !A.i;
+          break;
+        case 18:  // IS_NULL
+          x.i = A.null ? 1 : 0;
+          break;
+        case 19: {  // CASE: NULL condition takes ELSE (expression_translator.cc:183-195)
+          const XV& C = (!A.null && A.i) ? v[N.b] : v[N.c];
+          x = C;
+          break;
+        }
+        case 20:  // integer to REAL: NULL becomes a non-NULL 0.0 (expression_translator.cc:231-262)
+          x.f = A.null ? 0.0 : static_cast<double>(A.i);
+          break;
+        case 21:  // EXTRACT(YEAR)
+          x.null = A.null;
+          x.i = A.null ? 0 : kor_year(A.i);
+          break;
+        case 22: {  // LIKE: NULL is a non-NULL FALSE (expression_translator.cc:283-302)
+          if (A.null) break;
+          std::string sa;
+          if (nodes[N.a].type == KOR_ENUM) {
+            const int ca = enum_col(N.a);
+            if (ca < 0 || !cols[ca].dict) return -1;
+            sa = kor_dict_str(cols[ca], A.i);
+          } else {
+            str_of(N.a, r, sa);
+          }
+          x.i = kor_like_rec(reinterpret_cast<const uint8_t*>(sa.data()), static_cast<int64_t>(sa.size()),
+                             reinterpret_cast<const uint8_t*>(N.str), N.str_len)
+                    ? 1
+                    : 0;
+          break;
+        }
+        case 23: {  // ENUM IN: NULL is a non-NULL FALSE (expression_translator.cc:343-360)
+          if (A.null) break;
+          for (int j = 0; j < N.nvalues; j++)
+            if (N.values[j] == A.i) x.i = 1;
+          break;
+        }
+        default:
+          return -1;
+      }
+      v[k] = x;
+    }
+    const XV& root = v[nn - 1];
+    out_nulls[i] = root.null ? 1 : 0;
+    if (nodes[nn - 1].type == KOR_REAL) std::memcpy(&out[i], &root.f, 8);
+    else out[i] = root.i;
+  }
   return 0;
 }
 

@@ -132,6 +132,40 @@ typedef struct {
 // Row positions 0..n-1 in ORDER BY order. Returns 0, or -1 for a BOOLEAN key.
 int32_t kor_order_by(int64_t n, const kor_sort_key* keys, int32_t nkeys, int32_t* out);
 
+// ---- general expressions ---------------------------------------------------
+// One expression node (children precede parents; the last node is the root).
+// Operators and their numbering as kg_expr_op of the product header.
+typedef struct {
+  int32_t op, type;
+  int32_t a, b, c;
+  int32_t flags;          // 1: NULL literal
+  int64_t ilit;
+  double flit;
+  const char* str;
+  int32_t str_len;
+  int32_t nvalues;
+  const int32_t* values;
+} kor_expr_node;
+
+// An input column: typed data + NULL flags; TEXT: offsets/heap; ENUM string
+// operators: the dictionary (dict_off[code .. code + 1] in dict).
+typedef struct {
+  int32_t type;
+  const void* data;
+  const int8_t* nulls;
+  const int64_t* offsets;
+  const uint8_t* heap;
+  const char* dict;
+  const int64_t* dict_off;
+  int32_t dict_n;
+} kor_expr_col;
+
+// Evaluates the root for rows rows[0..n) (or 0..n): out[i] = value bits
+// (integers sign-extended, REAL bits, BOOLEAN 0/1), out_nulls[i]. Returns 0,
+// or -1 for an expression the reference rejects.
+int32_t kor_expr_eval(int64_t n, const kor_expr_col* cols, int32_t ncols, const kor_expr_node* nodes, int32_t nn,
+                      const int32_t* rows, int64_t* out, int8_t* out_nulls);
+
 // ---- workload compositions (CPU baseline legs) ------------------------------
 int64_t kor_q_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
                        const int32_t* g2, int32_t nf, const double* const* fm, int32_t ni,

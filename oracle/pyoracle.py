@@ -34,6 +34,12 @@ class KorSortKey(ctypes.Structure):
                 ("nulls", ctypes.c_void_p), ("str_off", ctypes.c_void_p), ("str_heap", ctypes.c_void_p)]
 
 
+class KorExprCol(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("data", ctypes.c_void_p), ("nulls", ctypes.c_void_p),
+                ("offsets", ctypes.c_void_p), ("heap", ctypes.c_void_p), ("dict", ctypes.c_void_p),
+                ("dict_off", ctypes.c_void_p), ("dict_n", ctypes.c_int32)]
+
+
 class RefSortKey(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("asc", ctypes.c_int32), ("data", ctypes.c_void_p),
                 ("nulls", ctypes.c_void_p)]
@@ -119,6 +125,9 @@ def lib():
         L.kor_text_minmax.restype = ctypes.c_int64
         L.kor_text_minmax.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_int32]
+        L.kor_expr_eval.restype = ctypes.c_int32
+        L.kor_expr_eval.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.kor_order_by.restype = ctypes.c_int32
         L.kor_order_by.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
         L.kor_dtree_scan_agg.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
@@ -363,3 +372,36 @@ def ref_sort_rows(keys):
     out = np.empty(max(n, 1), dtype=np.int32)
     ref().ref_sort_rows(n, ks, len(keys), out.ctypes.data)
     return out[:n].copy()
+
+
+def expr_eval(nodes_ptr, nn, cols, n, rows=None):
+    """Evaluates an expression node array (kor_expr_node layout) over host
+    columns: each a Col (ENUM may carry .dictionary) or ("text", strings,
+    nulls). Returns (int64 value bits, int8 NULL flags), or None when the
+    expression is rejected."""
+    cs = (KorExprCol * max(len(cols), 1))()
+    keep = []
+    for i, c in enumerate(cols):
+        if isinstance(c, tuple) and c[0] == "text":
+            _, strs, nulls = c
+            off, heap = _strings(strs)
+            nl = None if nulls is None else np.ascontiguousarray(nulls, dtype=np.int8)
+            keep += [off, heap, nl]
+            cs[i] = KorExprCol(TEXT, None, _p(nl), off.ctypes.data, heap.ctypes.data, None, None, 0)
+        else:
+            do = dh = None
+            dn = 0
+            if getattr(c, "dictionary", None) is not None:
+                do, dh = _strings(c.dictionary)
+                keep += [do, dh]
+                dn = len(c.dictionary)
+            cs[i] = KorExprCol(c.type, c.data.ctypes.data, _p(c.nulls), None, None,
+                               dh.ctypes.data if dh is not None else None, do.ctypes.data if do is not None else None,
+                               dn)
+    rr = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    m = len(rr) if rr is not None else n
+    out = np.zeros(max(m, 1), dtype=np.int64)
+    nulls = np.zeros(max(m, 1), dtype=np.int8)
+    if lib().kor_expr_eval(m, cs, len(cols), nodes_ptr, nn, _p(rr), out.ctypes.data, nulls.ctypes.data) != 0:
+        return None
+    return out[:m], nulls[:m]

@@ -32,6 +32,8 @@ EXPECTED = (
     + ["aggregate/int_expected.tbl", "aggregate/empty_expected.tbl", "tpch/q01_expected.tbl"]
     + [f"simd_scan_select/{t}_expected.tbl" for t in ("int", "date_range", "multiple_col")]
     + [f"order_by/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text")]
+    + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
+    + ["tpch/q06_expected.tbl"]
 )
 
 
