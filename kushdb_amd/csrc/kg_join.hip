@@ -264,8 +264,11 @@ __device__ __forceinline__ void probe_one32(const TableView& t, const ProbeOut& 
 
 // Probe kernel: every 4-lane group handles kUnroll probes per wave step; the
 // first-bucket loads of all of them are issued before any is examined, so a
-// wave keeps 16 * kUnroll random 64-byte reads in flight.
-constexpr int kUnroll = 4;
+// wave keeps 16 * kUnroll random 64-byte reads in flight. Counting probes keep
+// 8 per group in flight (256M probes into a 16M-key narrow table: 4.69 ms at 8,
+// 4.87 ms at 4, 4.86 at 2, 5.4 at 12 where registers cut occupancy).
+template <int MODE>
+constexpr int probe_unroll() { return MODE == kCount ? 8 : 4; }
 
 constexpr int kXcds = 8;
 
@@ -287,6 +290,7 @@ __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, c
   const int sub = lane & (kProbeLanes - 1);
   const int grp = lane / kProbeLanes;
   constexpr int kGroups = kWave / kProbeLanes;
+  constexpr int kUnroll = probe_unroll<MODE>();
   constexpr int kStep = kGroups * kUnroll;
   int64_t lo = 0, hi = n, gwave, nwaves;
   if (part.offs) {
