@@ -338,6 +338,66 @@ int kg_tuple_set_insert(kg_ctx* ctx, kg_tuple_set* s, const int32_t* d_tuples, i
 int kg_tuple_set_extract(kg_ctx* ctx, const kg_tuple_set* s, int32_t* d_out, int64_t cap, int64_t* out_n);
 int kg_tuple_set_free(kg_ctx* ctx, kg_tuple_set* s);
 
+/* ---- Skinner adaptive join -------------------------------------------------------
+ * Replaces runtime::ExecutePermutableSkinnerJoin
+ * (runtime/skinner_join_executor.h:16-25, skinner_join_executor.cc:840-885)
+ * with its table functions and valid-tuple handler
+ * (compile/translators/permutable_skinner_join_translator.cc:82-994): the
+ * multi-way equi-join of ntables materialized children under the conditions
+ * conds, with the join order chosen per episode by the reference's UCT agent
+ * (exploration weight 1e-5, connectivity heuristic; skinner_join_executor.cc:
+ * 559-838). An episode joins the next slice of its leftmost table's remaining
+ * tuples through the other tables on the device (column-index expansion for
+ * 16/32-bit keys, checked equality otherwise) and inserts the result tuples
+ * into a tuple-id set; the join ends when a leftmost table is complete.
+ * *out_set receives a new set (caller frees with kg_tuple_set_free) holding
+ * each result tuple once: int32[ntables] tuple ids, tuple id i of table t being
+ * position i of its child (row d_rows[i] of the key columns, or row i when
+ * d_rows is NULL). The set is the same for every join order and seed. NULL
+ * keys join nothing. Keys: SMALLINT, INT, BIGINT, REAL, DATE, BOOLEAN or ENUM
+ * (TEXT: KG_ERR_UNSUPPORTED). */
+typedef struct kg_skinner_table {
+  int64_t card;            /* tuples of the materialized child (< 2^31) */
+  const int32_t* d_rows;   /* child tuple -> row of its key columns; NULL: identity */
+} kg_skinner_table;
+typedef struct kg_skinner_cond {
+  int32_t left_table;      /* conds: left.col = right.col (Eq(ColRef, ColRef)) */
+  int32_t right_table;
+  kg_col left;             /* key columns, indexed by row */
+  kg_col right;
+} kg_skinner_cond;
+typedef struct kg_skinner_opts {
+  int64_t budget_per_episode;  /* partial tuples per episode (--budget_per_episode); 0: 2^20 */
+  int64_t seed;                /* > 0: fixed join seed (--join_seed); otherwise the clock */
+  int64_t max_episodes;        /* 0: no limit; otherwise KG_ERR_CAPACITY when not done by then */
+  int32_t nprefix;             /* tables fixed at the front of every order (prefix_order) */
+  int32_t prefix[8];
+} kg_skinner_opts;
+typedef struct kg_skinner_stats {
+  int64_t episodes;
+  int64_t distinct_orders;     /* join orders tried */
+  int64_t result_tuples;       /* valid tuples found, repeats included (num_result_tuples) */
+  int64_t distinct_tuples;     /* tuples in *out_set */
+  int64_t partial_tuples;      /* partial tuples produced by all join steps */
+  int32_t last_order[8];       /* order of the last episode */
+} kg_skinner_stats;
+int kg_skinner_join(kg_ctx* ctx, int32_t ntables, const kg_skinner_table* tables, int32_t nconds,
+                    const kg_skinner_cond* conds, const kg_skinner_opts* opts, kg_tuple_set** out_set,
+                    kg_skinner_stats* stats);
+
+/* The UCT join-order agent alone (UctJoinAgent, skinner_join_executor.cc:
+ * 779-838), host only, for callers that run their own episodes: Act() split at
+ * the environment's Execute. kg_uct_next writes the next order (ntables
+ * entries); kg_uct_reward feeds that episode's reward back (>= 0) before the
+ * next call. adjacency[i*ntables+j] != 0 when a condition joins tables i and j
+ * (either triangle suffices). Returns KG_ERR_INVALID on misuse. */
+typedef struct kg_uct kg_uct;
+int kg_uct_create(int32_t ntables, const uint8_t* adjacency, int64_t seed, const int32_t* prefix, int32_t nprefix,
+                  kg_uct** out);
+int kg_uct_next(kg_uct* u, int32_t* order);
+int kg_uct_reward(kg_uct* u, double reward);
+void kg_uct_free(kg_uct* u);
+
 /* ---- TEXT ------------------------------------------------------------------------
  * ScanSelect with one TEXT predicate `col op pattern` over rows 0..n-1 or over
  * the selection d_rows_in[0..n): qualifying row ids in input order to

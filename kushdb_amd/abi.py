@@ -109,6 +109,26 @@ class KgJoinTuning(ctypes.Structure):
                 ("from_cache", ctypes.c_int32), ("trials", ctypes.c_int32)]
 
 
+
+class KgSkinnerTable(ctypes.Structure):
+    _fields_ = [("card", ctypes.c_int64), ("d_rows", ctypes.c_void_p)]
+
+
+class KgSkinnerCond(ctypes.Structure):
+    _fields_ = [("left_table", ctypes.c_int32), ("right_table", ctypes.c_int32), ("left", KgCol), ("right", KgCol)]
+
+
+class KgSkinnerOpts(ctypes.Structure):
+    _fields_ = [("budget_per_episode", ctypes.c_int64), ("seed", ctypes.c_int64), ("max_episodes", ctypes.c_int64),
+                ("nprefix", ctypes.c_int32), ("prefix", ctypes.c_int32 * 8)]
+
+
+class KgSkinnerStats(ctypes.Structure):
+    _fields_ = [("episodes", ctypes.c_int64), ("distinct_orders", ctypes.c_int64),
+                ("result_tuples", ctypes.c_int64), ("distinct_tuples", ctypes.c_int64),
+                ("partial_tuples", ctypes.c_int64), ("last_order", ctypes.c_int32 * 8)]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -164,6 +184,12 @@ SIGNATURES = {
                               ctypes.POINTER(KgEnumDict), _i32, ctypes.POINTER(KgExprNode), _i32, _vp, _vp, _pi64]),
     "kg_intersect_batch": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_create": (_i32, [_vp, _i32, _i64, ctypes.POINTER(_vp)]),
+    "kg_skinner_join": (_i32, [_vp, _i32, ctypes.POINTER(KgSkinnerTable), _i32, ctypes.POINTER(KgSkinnerCond),
+                               ctypes.POINTER(KgSkinnerOpts), ctypes.POINTER(_vp), ctypes.POINTER(KgSkinnerStats)]),
+    "kg_uct_create": (_i32, [_i32, ctypes.POINTER(ctypes.c_uint8), _i64, _pi32, _i32, ctypes.POINTER(_vp)]),
+    "kg_uct_next": (_i32, [_vp, _pi32]),
+    "kg_uct_reward": (_i32, [_vp, ctypes.c_double]),
+    "kg_uct_free": (None, [_vp]),
     "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),
     "kg_tuple_set_extract": (_i32, [_vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_free": (_i32, [_vp, _vp]),

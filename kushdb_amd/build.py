@@ -29,6 +29,7 @@ SOURCES = [
     "kg_file.hip",
     "kg_order.hip",
     "kg_expr.hip",
+    "kg_skinner.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -82,3 +82,26 @@ def chain_query(ctx, tabs, spec, order, text_min=(0, "name"), int_min=(4, "v")):
     stats = {"selected": [int(s.numel()) for s in sels], "index_probes": probes, "index_inserts": inserts,
              "tuples": int(res.shape[0])}
     return tuples, min_text, min_int, stats
+
+
+def skinner_chain_query(ctx, tabs, spec, budget=0, seed=1, text_min=(0, "name"), int_min=(4, "v")):
+    """The same query with the join order left to the Skinner driver
+    (kg_skinner_join: UCT-chosen orders per episode, device episodes, tuple-id
+    dedup), as ExecutePermutableSkinnerJoin runs it
+    (runtime/skinner_join_executor.cc:840-885). Returns (distinct tuples [n, 5],
+    MIN text bytes or None, MIN int or None, stats)."""
+    from .runtime import skinner_join
+    sels = chain_filters(ctx, tabs, spec)
+    tables = [(s.numel(), s) for s in sels]
+    conds = [(t - 1, tabs[t - 1]["id"], t, tabs[t]["fk"]) for t in range(1, len(tabs))]
+    tuples, stats = skinner_join(ctx, tables, conds, budget=budget, seed=seed)
+    min_text = min_int = None
+    if tuples.shape[0]:
+        ti, col = text_min
+        _, min_text = text_minmax(ctx, tabs[ti][col], tuple_rows(ctx, tuples, ti, sels[ti]))
+        ii, icol = int_min
+        v = gather(ctx, tabs[ii][icol], tuple_rows(ctx, tuples, ii, sels[ii]))
+        _, vals, _ = ctx.scan_group_agg([v], [], [], [(abi.KG_AGG_MIN, 0)])
+        min_int = int(vals[0, 0])
+    stats["selected"] = [int(s.numel()) for s in sels]
+    return tuples, min_text, min_int, stats

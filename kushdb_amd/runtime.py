@@ -459,6 +459,72 @@ class TupleSet:
             pass
 
 
+    @classmethod
+    def from_handle(cls, ctx, h, length, size):
+        """Wraps a set the library created (e.g. by kg_skinner_join)."""
+        s = cls.__new__(cls)
+        s.ctx, s.len, s.h, s.size = ctx, length, h, size
+        return s
+
+
+def skinner_join(ctx, tables, conds, budget=0, seed=1, prefix=(), max_episodes=0):
+    """Adaptive multi-way equi-join (kg_skinner_join). tables: list of
+    (card, rows) where rows is an int32 device tensor mapping child tuple ->
+    key-column row, or None for the identity; conds: list of
+    (left_table, left Column, right_table, right Column). Returns
+    (distinct result tuples [n, ntables] int32 device tensor, stats dict)."""
+    nt = len(tables)
+    tab = (abi.KgSkinnerTable * nt)(*[abi.KgSkinnerTable(int(c), r.data_ptr() if r is not None else 0)
+                                      for c, r in tables])
+    cs = (abi.KgSkinnerCond * max(len(conds), 1))(*[abi.KgSkinnerCond(lt, rt, lc.desc(), rc.desc())
+                                                    for lt, lc, rt, rc in conds])
+    o = abi.KgSkinnerOpts(budget, seed, max_episodes, len(prefix), (ctypes.c_int32 * 8)(*prefix))
+    st = abi.KgSkinnerStats()
+    h = ctypes.c_void_p()
+    ctx.check(ctx.lib.kg_skinner_join(ctx.h, nt, tab, len(conds), cs, ctypes.byref(o), ctypes.byref(h),
+                                      ctypes.byref(st)))
+    ts = TupleSet.from_handle(ctx, h, nt, st.distinct_tuples)
+    tuples = ts.extract()
+    ts.free()
+    stats = {k: getattr(st, k) for k, _ in abi.KgSkinnerStats._fields_ if k != "last_order"}
+    stats["last_order"] = list(st.last_order)[:nt]
+    return tuples, stats
+
+
+class UctAgent:
+    """The reference's UCT join-order agent (kg_uct_*; host only): next()
+    returns the order of the next episode, reward(r) reports its reward."""
+
+    def __init__(self, ntables, edges, seed=1, prefix=()):
+        lib = abi.load()
+        adj = (ctypes.c_uint8 * (ntables * ntables))()
+        for a, b in edges:
+            adj[a * ntables + b] = 1
+        self.lib, self.n = lib, ntables
+        h = ctypes.c_void_p()
+        pre = (ctypes.c_int32 * max(len(prefix), 1))(*prefix)
+        st = lib.kg_uct_create(ntables, adj, seed, pre, len(prefix), ctypes.byref(h))
+        if st != abi.KG_OK:
+            raise abi.KgError(st, "kg_uct_create")
+        self.h = h
+
+    def next(self):
+        order = (ctypes.c_int32 * self.n)()
+        st = self.lib.kg_uct_next(self.h, order)
+        if st != abi.KG_OK:
+            raise abi.KgError(st, "kg_uct_next")
+        return list(order)
+
+    def reward(self, r):
+        st = self.lib.kg_uct_reward(self.h, r)
+        if st != abi.KG_OK:
+            raise abi.KgError(st, "kg_uct_reward")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.kg_uct_free(self.h)
+            self.h = None
+
 class TextColumn:
     """A resident TEXT column: int64 offsets [n+1] + uint8 heap (+ int8 NULLs)."""
 

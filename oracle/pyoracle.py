@@ -405,3 +405,83 @@ def expr_eval(nodes_ptr, nn, cols, n, rows=None):
     if lib().kor_expr_eval(m, cs, len(cols), nodes_ptr, nn, _p(rr), out.ctypes.data, nulls.ctypes.data) != 0:
         return None
     return out[:m], nulls[:m]
+
+
+def skinner_join(cards, conds, rows=None):
+    """Result set of the Skinner multi-way equi-join: every tuple of tuple ids
+    (one per table) that satisfies all conditions, each once.
+
+    Restates what the permutable Skinner join emits, independent of the join
+    order the UCT agent picks: each table function keeps the tuples of its
+    table that match the joined ones on every equality predicate
+    (compile/translators/permutable_skinner_join_translator.cc:359-433 GetBucket
+    per equality predicate, :663-832 full scan without an index, flagged
+    predicates evaluated once all their tables are joined), NULL keys are not
+    indexed and match nothing (:972-994), and the valid-tuple handler inserts
+    each full tuple into the TupleIdxTable and emits it only when new
+    (:842-912). cards[t] = tuples of table t; conds = [(lt, lkeys, lnulls, rt,
+    rkeys, rnulls)] with key arrays indexed by row; rows[t] maps tuple id -> row
+    (None: identity). Returns a sorted list of int tuples."""
+    nt = len(cards)
+    rows = rows or [None] * nt
+
+    def key(c, side, t, tid):
+        lt, lk, ln, rt, rk, rn = conds[c]
+        ks, ns = (lk, ln) if side == 0 else (rk, rn)
+        r = tid if rows[t] is None else int(rows[t][tid])
+        if ns is not None and ns[r]:
+            return None
+        v = ks[r]
+        return float(v) if isinstance(v, (float, np.floating)) else int(v)
+
+    if any(c == 0 for c in cards):
+        return []
+    # Tables in breadth-first order over the condition graph, one component
+    # after the other (the order does not change the set).
+    order, seen = [], set()
+    for s in range(nt):
+        if s in seen:
+            continue
+        seen.add(s)
+        q = [s]
+        while q:
+            t = q.pop(0)
+            order.append(t)
+            for c in conds:
+                for a, b in ((c[0], c[3]), (c[3], c[0])):
+                    if a == t and b not in seen:
+                        seen.add(b)
+                        q.append(b)
+    partial = [{order[0]: i} for i in range(cards[order[0]])]
+    for t in order[1:]:
+        links = [(ci, 0 if c[0] == t else 1) for ci, c in enumerate(conds)
+                 if (c[0] == t and c[3] in partial[0]) or (c[3] == t and c[0] in partial[0])] if partial else []
+        nxt = []
+        if links:
+            ci, side = links[0]
+            other = conds[ci][3] if side == 0 else conds[ci][0]
+            bucket = {}
+            for i in range(cards[t]):
+                k = key(ci, side, t, i)
+                if k is not None:
+                    bucket.setdefault(k, []).append(i)
+            for p in partial:
+                k = key(ci, 1 - side, other, p[other])
+                for i in ([] if k is None else bucket.get(k, [])):
+                    q = dict(p)
+                    q[t] = i
+                    nxt.append(q)
+        else:
+            nxt = [{**p, t: i} for p in partial for i in range(cards[t])]
+        for ci, side in links[1:]:
+            other = conds[ci][3] if side == 0 else conds[ci][0]
+            keep = []
+            for q in nxt:
+                a, b = key(ci, side, t, q[t]), key(ci, 1 - side, other, q[other])
+                if a is not None and b is not None and a == b:
+                    keep.append(q)
+            nxt = keep
+        partial = nxt
+        if not partial:
+            return []
+    return sorted({tuple(p[t] for t in range(nt)) for p in partial})

@@ -37,6 +37,10 @@ class OracleEngine:
         got, v, nl = po.aggregate(pa, n, rows)
         return (v, nl) if got else None
 
+    def skinner(self, cards, conds, **kw):
+        """conds: [(lt, (typ, values, nulls), rt, (typ, values, nulls))]."""
+        return po.skinner_join(cards, [(lt, lc[1], lc[2], rt, rc[1], rc[2]) for lt, lc, rt, rc in conds])
+
 
 class GpuEngine:
     name = "gpu"
@@ -64,6 +68,19 @@ class GpuEngine:
     def select(self, cols, preds, conj):
         rows = self.ctx.scan_select([self._col(c) for c in cols], preds, conj)
         return rows.cpu().numpy()
+
+    def skinner(self, cards, conds, budget=0, seed=1):
+        from kushdb_amd.runtime import skinner_join
+        cache = {}
+
+        def dev(c):
+            if id(c[1]) not in cache:
+                cache[id(c[1])] = self._col(c)
+            return cache[id(c[1])]
+        t, self.last_stats = skinner_join(self.ctx, [(c, None) for c in cards],
+                                          [(lt, dev(lc), rt, dev(rc)) for lt, lc, rt, rc in conds],
+                                          budget=budget, seed=seed)
+        return sorted(map(tuple, t.cpu().numpy().tolist()))
 
     def join(self, b, p):
         bc, pc = self._col(b), self._col(p)

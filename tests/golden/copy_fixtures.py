@@ -34,6 +34,8 @@ EXPECTED = (
     + [f"order_by/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text")]
     + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
     + ["tpch/q06_expected.tbl"]
+    + [f"skinner_join/{t}_expected.tbl" for t in ("int", "smallint", "date", "bigint", "real", "boolean",
+                                                   "enum", "multiple_tables_join")]
 )
 
 

@@ -204,3 +204,37 @@ def check_q01(got, expected_lines, tol=1e-5):
         for j in range(7):
             assert abs(float(e[2 + j]) - nums[j]) < tol, (line, nums)
         assert int(e[9]) == nums[7]
+
+
+# ------------------------------------------------------------- skinner join
+SKINNER = {
+    # end_to_end_test/skinner_join/<t>_test.cc: two scans of one column joined on equality,
+    # output both columns (enum: the EnumSchema people.name, printed as strings)
+    "int": ("info_id", "int"),
+    "smallint": ("info_num1", "smallint"),
+    "date": ("info_date", "date"),
+    "bigint": ("info_num2", "bigint"),
+    "real": ("info_zscore", "real"),
+    "boolean": ("info_cheated", "boolean"),
+    "enum": ("people_name_enum", "enum"),
+}
+
+
+def skinner(engine, t, **kw):
+    """Result lines of a skinner_join fixture query on `engine`."""
+    if t == "multiple_tables_join":
+        # multiple_tables_join_test.cc: four scans of info(id, num2);
+        # s1.id = s2.id, s2.num2 = s3.num2, s2.id = s4.id; output the four ids.
+        ident, num2 = _c("info_id", "int"), _c("info_num2", "bigint")
+        n = len(ident[1])
+        tuples = engine.skinner([n] * 4, [(0, ident, 1, ident), (1, num2, 2, num2), (1, ident, 3, ident)], **kw)
+        return [kdb.row([kdb.fmt("int", ident[1][tp[i]], ident[2] is not None and ident[2][tp[i]])
+                         for i in range(4)]) for tp in tuples]
+    name, typ = SKINNER[t]
+    c = _c(name, typ)
+    n = len(c[1])
+    tuples = engine.skinner([n, n], [(0, c, 1, c)], **kw)
+    if typ == "enum":
+        d = kdb.enum_dict(name)
+        return [kdb.row([d[int(c[1][tp[0]])], d[int(c[1][tp[1]])]]) for tp in tuples]
+    return [kdb.row([kdb.fmt(typ, c[1][tp[i]]) for i in range(2)]) for tp in tuples]
