@@ -328,10 +328,28 @@ def bench_chain_c3(ctx, a):
     text_bytes = sum(int(htabs[i][c + "_heap"].nbytes) + 8 * len(htabs[i]["id"]) for i, c in ((0, "name"), (1, "s")))
     alg = (text_bytes + 4 * SKINNER_SIZES[4] + 4 * sum(stats["selected"]) +
            64 * (stats["index_inserts"] + stats["index_probes"] + stats["tuples"]))
-    return {"workload": "5-table chain join (200k..50M rows), TEXT prefix + LIKE + int filters, device indexes, "
-                        "tuple dedup, MIN(text), MIN(int)", "order": order, "stats": stats,
-            "query_ms": round(wall * 1e3, 3), "rows_per_s": n_in / wall,
-            "min_int": mn_int, "roofline": roofline(alg, wall * 1e3)}
+    out = {"workload": "5-table chain join (200k..50M rows), TEXT prefix + LIKE + int filters, device indexes, "
+                       "tuple dedup, MIN(text), MIN(int)", "order": order, "stats": stats,
+           "query_ms": round(wall * 1e3, 3), "rows_per_s": n_in / wall,
+           "min_int": mn_int, "roofline": roofline(alg, wall * 1e3)}
+    # The same query with the join order chosen per episode by the UCT agent (kg_skinner_join).
+    from kushdb_amd.chain import skinner_chain_query
+    for budget in (1 << 20, 1 << 22):
+        skinner_chain_query(ctx, tabs, spec, budget=budget, seed=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st_tuples, st_text, st_int, sst = skinner_chain_query(ctx, tabs, spec, budget=budget, seed=1)
+        torch.cuda.synchronize()
+        swall = (time.perf_counter() - t0) / steps
+        assert st_tuples.shape[0] == tuples.shape[0] and st_int == mn_int and st_text == mn_text
+        out[f"skinner_budget_{budget}"] = {
+            "query_ms": round(swall * 1e3, 3), "rows_per_s": n_in / swall,
+            "episodes": sst["episodes"], "orders_tried": sst["distinct_orders"], "last_order": sst["last_order"],
+            "partial_tuples": sst["partial_tuples"], "result_tuples": sst["result_tuples"]}
+        log(f"C3 skinner budget {budget}: {swall * 1e3:.2f} ms, {sst['episodes']} episodes, "
+            f"{sst['distinct_orders']} orders, last {sst['last_order']}")
+    return out
 
 
 def bench_serving(ctx, a):

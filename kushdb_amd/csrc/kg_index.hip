@@ -54,12 +54,39 @@ __global__ void k_index_gather(kg_col key, const int32_t* sel, int64_t n, uint32
   }
 }
 
+// Directory from key to its run of tuple ids. Hashed: 8-slot buckets of
+// {key, offset, count}. Dense (dspan > 0, when the distinct keys span at most
+// about three times their number): offset/count indexed by key - dmin, count 0
+// for an absent key, written in key order by one streaming pass.
 struct Dir {
-  int64_t* keys;     // [nbuckets * 8]
-  int64_t* offset;   // [nbuckets * 8]
-  int32_t* count;    // [nbuckets * 8]
+  int64_t* keys;     // [nbuckets * 8] (hashed)
+  int64_t* offset;   // [nbuckets * 8] or [dspan]
+  int32_t* count;    // [nbuckets * 8] or [dspan]
   uint64_t nbuckets;
+  int64_t dmin;
+  uint64_t dspan;
 };
+
+// Dense directory: one thread per run, runs in ascending key order.
+__global__ void k_dir_dense(Dir d, const uint32_t* ukeys, const int32_t* counts, const int64_t* offsets,
+                            const int32_t* nruns) {
+  const int64_t n = *nruns;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t u = static_cast<int64_t>(static_cast<int32_t>(ukeys[i] ^ 0x80000000u)) - d.dmin;
+    d.offset[u] = offsets[i];
+    d.count[u] = counts[i];
+  }
+}
+
+// Sets *unsorted when some key is greater than the next one.
+__global__ void k_keys_sorted(const uint32_t* k, int64_t n, int* unsorted) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  bool bad = false;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i + 1 < n; i += stride)
+    bad |= k[i] > k[i + 1];
+  if (__any(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(unsorted, 1);
+}
 
 // One thread per distinct key (run): claim a directory slot.
 __global__ void k_dir_insert(Dir d, const uint32_t* ukeys, const int32_t* counts, const int64_t* offsets,
@@ -98,7 +125,13 @@ __global__ void k_dir_lookup(Dir d, const int32_t* tids, kg_col key, const int32
     const int64_t r = sel ? sel[i] : i;
     const int32_t* ptr = tids;
     int32_t cnt = 0;
-    if (!(key.nulls && key.nulls[r])) {
+    if (d.dspan && !(key.nulls && key.nulls[r])) {
+      const int64_t u = static_cast<int64_t>(static_cast<int32_t>(load_int(key.type, key.data, r))) - d.dmin;
+      if (u >= 0 && static_cast<uint64_t>(u) < d.dspan) {
+        cnt = d.count[u];
+        if (cnt) ptr = tids + d.offset[u];
+      }
+    } else if (!(key.nulls && key.nulls[r])) {
       const int64_t k = static_cast<int32_t>(load_int(key.type, key.data, r));
       uint64_t b = dir_bucket(k, d.nbuckets);
       for (uint64_t probes = 0; probes < d.nbuckets; probes++) {
@@ -432,16 +465,23 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
     hipLaunchKernelGGL(k_index_gather, dim3(igrid(ctx, n)), dim3(kIBlock), 0, ctx->stream, *key, d_rows, n, k0, t0,
                        valid);
     KG_LAUNCH_CHECK(ctx);
-    // Drop NULL keys (keeping order), then a stable sort by key keeps tuple ids ascending per key.
-    cb = cub_bytes;
-    KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(cub, cb, k0, valid, k1, nsel, static_cast<int>(n), ctx->stream));
-    cb = cub_bytes;
-    KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(cub, cb, t0, valid, t1, nsel, static_cast<int>(n), ctx->stream));
+    // Drop NULL keys (keeping order), then a stable sort by key keeps tuple ids
+    // ascending per key. Without a NULL column every key stays in place.
+    if (key->nulls) {
+      cb = cub_bytes;
+      KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(cub, cb, k0, valid, k1, nsel, static_cast<int>(n), ctx->stream));
+      cb = cub_bytes;
+      KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(cub, cb, t0, valid, t1, nsel, static_cast<int>(n), ctx->stream));
+    }
   }
-  int32_t hsel = 0;
-  if (n > 0 && (st = read_u64(ctx, nsel, &hsel, 4)) != KG_OK) {
+  int32_t hsel = static_cast<int32_t>(n);
+  if (n > 0 && key->nulls && (st = read_u64(ctx, nsel, &hsel, 4)) != KG_OK) {
     delete idx;
     return st;
+  }
+  if (n > 0 && !key->nulls) {
+    std::swap(k0, k1);
+    std::swap(t0, t1);
   }
   idx->ntids = hsel;
   if ((st = ialloc(ctx, reinterpret_cast<void**>(&idx->tids), static_cast<size_t>(hsel) * 4)) != KG_OK) {
@@ -450,8 +490,24 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
   }
   int32_t hruns = 0;
   if (hsel > 0) {
+    // Keys already in order (an id column over an ascending selection) need no
+    // sort: the stable sort of sorted keys is the identity.
+    KG_HIP_CHECK(ctx, hipMemsetAsync(flag, 0, 16, ctx->stream));
+    hipLaunchKernelGGL(k_keys_sorted, dim3(igrid(ctx, hsel)), dim3(kIBlock), 0, ctx->stream, k1, hsel, flag);
+    KG_LAUNCH_CHECK(ctx);
+    int unsorted = 0;
+    if ((st = read_u64(ctx, flag, &unsorted, 4)) != KG_OK) {
+      kg_index_free(ctx, idx);
+      return st;
+    }
     cb = cub_bytes;
-    KG_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(cub, cb, k1, k0, t1, idx->tids, hsel, 0, 32, ctx->stream));
+    if (unsorted) {
+      KG_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(cub, cb, k1, k0, t1, idx->tids, hsel, 0, 32, ctx->stream));
+    } else {
+      KG_HIP_CHECK(ctx, hipMemcpyAsync(k0, k1, static_cast<size_t>(hsel) * 4, hipMemcpyDeviceToDevice, ctx->stream));
+      KG_HIP_CHECK(ctx, hipMemcpyAsync(idx->tids, t1, static_cast<size_t>(hsel) * 4, hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+    }
     cb = cub_bytes;
     KG_HIP_CHECK(ctx, hipcub::DeviceRunLengthEncode::Encode(cub, cb, k0, k1, cnt, nruns, hsel, ctx->stream));
     if ((st = read_u64(ctx, nruns, &hruns, 4)) != KG_OK) {
@@ -462,6 +518,31 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
     KG_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cub, cb, cnt, offs, hruns, ctx->stream));
   }
   idx->nkeys = hruns;
+  if (hruns > 0) {
+    // Key range of the runs (ascending): first and last run key.
+    uint32_t ends[2] = {0, 0};
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(&ends[0], k1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(&ends[1], k1 + hruns - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t kmin = static_cast<int32_t>(ends[0] ^ 0x80000000u), kmax = static_cast<int32_t>(ends[1] ^ 0x80000000u);
+    const int64_t span = kmax - kmin + 1;
+    if (span <= 3 * static_cast<int64_t>(hruns) + 1024) {
+      idx->dir.dmin = kmin;
+      idx->dir.dspan = static_cast<uint64_t>(span);
+      if ((st = ialloc(ctx, reinterpret_cast<void**>(&idx->dir.offset), span * 8)) != KG_OK ||
+          (st = ialloc(ctx, reinterpret_cast<void**>(&idx->dir.count), span * 4)) != KG_OK) {
+        kg_index_free(ctx, idx);
+        return st;
+      }
+      KG_HIP_CHECK(ctx, hipMemsetAsync(idx->dir.count, 0, span * 4, ctx->stream));
+      hipLaunchKernelGGL(k_dir_dense, dim3(igrid(ctx, hruns)), dim3(kIBlock), 0, ctx->stream, idx->dir, k1, cnt, offs,
+                         nruns);
+      KG_LAUNCH_CHECK(ctx);
+      kg_time_end(ctx);
+      *out = idx;
+      return KG_OK;
+    }
+  }
   idx->dir.nbuckets = static_cast<uint64_t>(std::max<int64_t>(ceil_div(std::max<int64_t>(hruns, 1) * 2, kDirSlots), 1));
   const size_t ns = idx->dir.nbuckets * kDirSlots;
   if ((st = ialloc(ctx, reinterpret_cast<void**>(&idx->dir.keys), ns * 8)) != KG_OK ||
@@ -494,11 +575,13 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
 }
 
 void kg::index_dir(const kg_index* idx, const int64_t** keys, const int64_t** offset, const int32_t** count,
-                   uint64_t* nbuckets, const int32_t** tids) {
+                   uint64_t* nbuckets, int64_t* dmin, uint64_t* dspan, const int32_t** tids) {
   *keys = idx->dir.keys;
   *offset = idx->dir.offset;
   *count = idx->dir.count;
   *nbuckets = idx->dir.nbuckets;
+  *dmin = idx->dir.dmin;
+  *dspan = idx->dir.dspan;
   *tids = idx->tids;
 }
 

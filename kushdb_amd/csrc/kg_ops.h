@@ -149,7 +149,8 @@ int emit_groups(kg_ctx* ctx, int64_t G, int32_t nkeys, int32_t card1, const Meas
                 int32_t* out_keys, int64_t* out_vals, int8_t* out_nulls);
 
 // Directory and tid array of a column index (kg_index.hip).
+// Dense directories have dspan > 0 (offset/count indexed by key - dmin).
 void index_dir(const kg_index* idx, const int64_t** keys, const int64_t** offset, const int32_t** count,
-               uint64_t* nbuckets, const int32_t** tids);
+               uint64_t* nbuckets, int64_t* dmin, uint64_t* dspan, const int32_t** tids);
 
 }  // namespace kg

@@ -373,26 +373,31 @@ class SkinnerJoin {
     for (DevBuf* b : {&a_, &b_, &keep_, &pos_, &cub_}) b->release();
   }
 
-  int Prepare() {
-    // One device column index per indexable condition side over the table's
-    // materialized tuples (MemoryColumnIndex per join column of a filtered
-    // child; permutable_skinner_join_translator.cc:116-132, 934-994).
-    for (size_t c = 0; c < conds_.size(); c++)
-      for (int s = 0; s < 2; s++) {
-        const Side sd = side(c, s);
-        if (!indexable(sd.col.type) || !indexable(side(c, 1 - s).col.type)) continue;
-        for (size_t e = 0; e < c * 2 + s; e++) {  // same table and column as an earlier side: share
-          const Side o = side(e / 2, e % 2);
-          if (index_[e] && o.table == sd.table && o.col.data == sd.col.data && o.col.type == sd.col.type) {
-            index_[c * 2 + s] = index_[e];
-            shared_.insert(c * 2 + s);
-            break;
-          }
-        }
-        if (index_[c * 2 + s]) continue;
-        int st = kg_index_build(ctx_, &sd.col, tables_[sd.table].d_rows, tables_[sd.table].card, &index_[c * 2 + s]);
-        if (st != KG_OK) return st;
+  // The device column index of one condition side, built on first use over
+  // the table's materialized tuples (the reference builds a MemoryColumnIndex
+  // per join column of a filtered child up front,
+  // permutable_skinner_join_translator.cc:116-132, 934-994; here only the
+  // sides some chosen order probes are built). Sides with the same table and
+  // column share one index. Null when the side cannot be indexed.
+  int Index(int c, int s, kg_index** out) {
+    *out = nullptr;
+    const Side sd = side(c, s);
+    if (!indexable(sd.col.type) || !indexable(side(c, 1 - s).col.type)) return KG_OK;
+    if (index_[c * 2 + s]) {
+      *out = index_[c * 2 + s];
+      return KG_OK;
+    }
+    for (size_t e = 0; e < index_.size(); e++) {
+      const Side o = side(e / 2, e % 2);
+      if (index_[e] && o.table == sd.table && o.col.data == sd.col.data && o.col.type == sd.col.type) {
+        index_[c * 2 + s] = *out = index_[e];
+        shared_.insert(c * 2 + s);
+        return KG_OK;
       }
+    }
+    int st = kg_index_build(ctx_, &sd.col, tables_[sd.table].d_rows, tables_[sd.table].card, &index_[c * 2 + s]);
+    if (st != KG_OK) return st;
+    *out = index_[c * 2 + s];
     return KG_OK;
   }
 
@@ -428,11 +433,11 @@ class SkinnerJoin {
           if (side(c, s).table == t && pos_of[side(c, 1 - s).table] >= 0 && side(c, 1 - s).table != t)
             link.emplace_back(static_cast<int>(c), s);
       int via = -1;
-      for (size_t l = 0; l < link.size(); l++)
-        if (index_[link[l].first * 2 + link[l].second]) {
-          via = static_cast<int>(l);
-          break;
-        }
+      kg_index* idx = nullptr;
+      for (size_t l = 0; l < link.size() && via < 0; l++) {
+        if ((st = Index(link[l].first, link[l].second, &idx)) != KG_OK) return st;
+        if (idx) via = static_cast<int>(l);
+      }
       int64_t next = 0;
       const int32_t min_id = static_cast<int32_t>(offset_[t] + 1);
       if (via >= 0) {
@@ -441,7 +446,7 @@ class SkinnerJoin {
         for (;;) {
           const int64_t cap = static_cast<int64_t>(b_.bytes / (4 * (width + 1)));
           st = kg_index_join_expand(ctx_, cur, width, a_.as<int32_t>(), pos_of[src.table], &src.col,
-                                    tables_[src.table].d_rows, index_[c * 2 + s], cap, b_.as<int32_t>(), &next);
+                                    tables_[src.table].d_rows, idx, cap, b_.as<int32_t>(), &next);
           if (st == KG_ERR_CAPACITY && next > cap) {
             if (next > INT32_MAX) return kg_fail(ctx_, KG_ERR_CAPACITY, "skinner join: %lld partial tuples in one "
                                                  "step; lower budget_per_episode", (long long)next);
@@ -705,11 +710,8 @@ extern "C" int kg_skinner_join(kg_ctx* ctx, int32_t ntables, const kg_skinner_ta
   if (!empty) {
     SkinnerJoin sj(ctx, ntables, tables, nconds, conds, o);
     kg_uct* agent = nullptr;
-    rc = sj.Prepare();
-    if (rc == KG_OK) {
-      rc = kg_uct_create(ntables, adj.data(), o.seed, o.prefix, o.nprefix, &agent);
-      if (rc != KG_OK) rc = kg_fail(ctx, rc, "skinner join: bad join prefix");
-    }
+    rc = kg_uct_create(ntables, adj.data(), o.seed, o.prefix, o.nprefix, &agent);
+    if (rc != KG_OK) rc = kg_fail(ctx, rc, "skinner join: bad join prefix");
     std::set<std::vector<int>> orders;
     std::vector<int32_t> order(ntables);
     bool done = false;

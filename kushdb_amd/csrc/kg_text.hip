@@ -115,11 +115,21 @@ struct DirView {
   const int64_t* offset;
   const int32_t* count;
   uint64_t nbuckets;
+  int64_t dmin;     // dense directory when dspan > 0
+  uint64_t dspan;
 };
 
 __device__ __forceinline__ void dir_find(const DirView& d, int64_t k, int64_t& off, int32_t& cnt) {
   off = 0;
   cnt = 0;
+  if (d.dspan) {
+    const int64_t u = k - d.dmin;
+    if (u >= 0 && static_cast<uint64_t>(u) < d.dspan) {
+      cnt = d.count[u];
+      if (cnt) off = d.offset[u];
+    }
+    return;
+  }
   uint64_t b = __umul64hi(mix64(static_cast<uint64_t>(k)), d.nbuckets);
   for (uint64_t probes = 0; probes < d.nbuckets; probes++) {
     bool empty = false;
@@ -323,7 +333,7 @@ extern "C" int kg_index_join_expand(kg_ctx* ctx, int64_t n, int32_t width, const
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   DirView d;
   const int32_t* tids = nullptr;
-  index_dir(idx, &d.keys, &d.offset, &d.count, &d.nbuckets, &tids);
+  index_dir(idx, &d.keys, &d.offset, &d.count, &d.nbuckets, &d.dmin, &d.dspan, &tids);
   Expand e{width, src_pos, *src_key, d_src_rows, tids};
   size_t cub_scan = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, cub_scan, static_cast<int64_t*>(nullptr), static_cast<int64_t*>(nullptr),

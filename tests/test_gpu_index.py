@@ -32,11 +32,21 @@ def oracle_index(keys, rows=None):
     return {int(ok[i]): ot[oo[i]:oo[i + 1]] for i in range(nk)}
 
 
-@pytest.mark.parametrize("n,domain,with_nulls,with_sel", [(1, 5, False, False), (100_000, 1000, False, False),
-                                                         (300_000, 50_000, True, True), (2_000_000, 10, False, True)])
-def test_index_build_and_lookup(ctx, n, domain, with_nulls, with_sel):
+# Dense directories (keys spanning at most about 3x their count), hashed ones
+# (sparse keys, the extremes of INT), and keys that arrive sorted (no sort).
+@pytest.mark.parametrize("n,domain,with_nulls,with_sel,order", [
+    (1, 5, False, False, "random"), (100_000, 1000, False, False, "random"),
+    (300_000, 50_000, True, True, "random"), (2_000_000, 10, False, True, "random"),
+    (200_000, 1 << 30, False, False, "random"), (200_000, 1 << 30, True, True, "random"),
+    (50_000, 5, False, False, "extremes"), (400_000, 300_000, False, True, "sorted"),
+    (400_000, 300_000, True, False, "sorted"), (300_000, 1 << 28, False, False, "sorted")])
+def test_index_build_and_lookup(ctx, n, domain, with_nulls, with_sel, order):
     rng = np.random.default_rng(n + domain)
     keys = rng.integers(-domain, domain, n).astype(np.int32)
+    if order == "sorted":
+        keys = np.sort(keys)
+    elif order == "extremes":
+        keys = rng.choice(np.array([-2**31, 2**31 - 1, 0, 7], dtype=np.int64), n).astype(np.int32)
     nulls = (rng.random(n) < 0.05).astype(np.int8) if with_nulls else None
     sel = np.sort(rng.choice(n, size=max(1, n // 3), replace=False)).astype(np.int32) if with_sel else None
     col = Column(abi.KG_INT, t(keys), t(nulls) if nulls is not None else None)
@@ -49,7 +59,8 @@ def test_index_build_and_lookup(ctx, n, domain, with_nulls, with_sel):
     assert idx.nkeys == len(want)
     assert idx.ntids == sum(len(v) for v in want.values())
     probe = np.concatenate([np.array(list(want.keys()), dtype=np.int32),
-                            rng.integers(-2 * domain, 2 * domain, 1000).astype(np.int32)])
+                            rng.integers(-2 * domain - 5, 2 * domain + 5, 1000).astype(np.int64).clip(-2**31, 2**31 - 1)
+                            .astype(np.int32), np.array([-2**31, 2**31 - 1], dtype=np.int32)])
     ptrs, cnts = idx.lookup(Column(abi.KG_INT, t(probe)))
     offs = idx.offsets_of(ptrs).cpu().numpy()
     cnts = cnts.cpu().numpy()
