@@ -416,6 +416,22 @@ int kg_text_select(kg_ctx* ctx, const kg_text_col* col, int32_t op, const char* 
 int kg_text_minmax(kg_ctx* ctx, const kg_text_col* col, const int32_t* d_rows, int64_t n, int32_t is_max,
                    int64_t* out_pos, char* out_buf, int64_t buf_cap, int64_t* out_len);
 
+/* TEXT equi-join keys. A TEXT key has no fixed width, so a TEXT join runs as a
+ * BIGINT join on 64-bit hashes of the strings followed by an exact pair check:
+ * kg_text_hash writes the hash of every value (rows d_rows[i], or i when
+ * d_rows is NULL) to d_out and its NULL flag to d_out_nulls (optional; a NULL
+ * key joins nothing, hash_join_translator.cc:86-101). It replaces the
+ * std::hash<string_view> key hash of TEXT values (compile/proxy/value/
+ * string.cc:104-107 -> runtime/string.cc:74-77); equal strings hash equal.
+ * kg_text_pairs_equal then keeps, of n candidate pairs (a[d_rows_a[i]],
+ * b[d_rows_b[i]]), the positions i whose strings are byte-equal and not NULL
+ * (CheckEquality over TEXT keys, hash_join_translator.cc:57-77, with
+ * String::Equals, runtime/string.cc:42-46), ascending, in d_out_idx. */
+int kg_text_hash(kg_ctx* ctx, const kg_text_col* col, const int32_t* d_rows, int64_t n, int64_t* d_out,
+                 int8_t* d_out_nulls);
+int kg_text_pairs_equal(kg_ctx* ctx, const kg_text_col* a, const int32_t* d_rows_a, const kg_text_col* b,
+                        const int32_t* d_rows_b, int64_t n, int32_t* d_out_idx, int64_t* out_count);
+
 /* ---- index-nested-loop join step (Skinner table functions) ----------------------
  * Extends each partial join tuple d_tuples[i*width .. +width) (tuple ids per
  * table) by every tuple id of idx's bucket for the key src_key[row], where

@@ -190,6 +190,9 @@ SIGNATURES = {
     "kg_uct_next": (_i32, [_vp, _pi32]),
     "kg_uct_reward": (_i32, [_vp, ctypes.c_double]),
     "kg_uct_free": (None, [_vp]),
+    "kg_text_hash": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, _i64, _vp, _vp]),
+    "kg_text_pairs_equal": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, ctypes.POINTER(KgTextCol), _vp, _i64, _vp,
+                                   _pi64]),
     "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),
     "kg_tuple_set_extract": (_i32, [_vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_free": (_i32, [_vp, _vp]),

@@ -222,6 +222,43 @@ int check_text(kg_ctx* ctx, const kg_text_col* c) {
   return KG_OK;
 }
 
+// 64-bit FNV-1a over a TEXT value's bytes, finished with mix64 (fmix64).
+// Equal strings hash equal; placement only, never compared as a result.
+__device__ __forceinline__ uint64_t text_hash(const kg_text_col& c, int64_t r) {
+  const int64_t b = c.offsets[r], e = c.offsets[r + 1];
+  uint64_t h = 0xcbf29ce484222325ull;
+  for (int64_t i = b; i < e; i++) h = (h ^ c.bytes[i]) * 0x100000001b3ull;
+  return mix64(h ^ static_cast<uint64_t>(e - b));
+}
+
+__global__ void k_text_hash(kg_text_col col, const int32_t* rows, int64_t n, int64_t* out, int8_t* out_nulls) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t r = rows ? rows[i] : i;
+    const bool null = col.nulls && col.nulls[r];
+    out[i] = null ? 0 : static_cast<int64_t>(text_hash(col, r));
+    if (out_nulls) out_nulls[i] = null ? 1 : 0;
+  }
+}
+
+// Keep flag of pair i: a[ra[i]] == b[rb[i]] byte for byte, neither NULL.
+__global__ void k_text_pairs_eq(kg_text_col a, const int32_t* ra, kg_text_col b, const int32_t* rb, int64_t n,
+                                int32_t* ids, uint8_t* fl) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t x = ra ? ra[i] : i, y = rb ? rb[i] : i;
+    bool eq = !(a.nulls && a.nulls[x]) && !(b.nulls && b.nulls[y]);
+    if (eq) {
+      const int64_t ab = a.offsets[x], al = a.offsets[x + 1] - ab;
+      const int64_t bb = b.offsets[y], bl = b.offsets[y + 1] - bb;
+      eq = al == bl;
+      for (int64_t k = 0; eq && k < al; k++) eq = a.bytes[ab + k] == b.bytes[bb + k];
+    }
+    ids[i] = static_cast<int32_t>(i);
+    fl[i] = eq ? 1 : 0;
+  }
+}
+
 }  // namespace
 
 extern "C" int kg_text_select(kg_ctx* ctx, const kg_text_col* col, int32_t op, const char* pattern, int32_t plen,
@@ -403,5 +440,59 @@ extern "C" int kg_gather(kg_ctx* ctx, const kg_col* src, const int32_t* d_rows, 
                      static_cast<const uint8_t*>(src->data), src->nulls, w, d_rows, n, static_cast<uint8_t*>(d_out),
                      d_out_nulls);
   KG_LAUNCH_CHECK(ctx);
+  return KG_OK;
+}
+
+extern "C" int kg_text_hash(kg_ctx* ctx, const kg_text_col* col, const int32_t* d_rows, int64_t n, int64_t* d_out,
+                            int8_t* d_out_nulls) {
+  if (!ctx) return KG_ERR_INVALID;
+  int st = check_text(ctx, col);
+  if (st != KG_OK) return st;
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count out of range");
+  KG_REQUIRE(ctx, n == 0 || d_out, "d_out is NULL");
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(k_text_hash, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream, *col, d_rows, n, d_out,
+                     d_out_nulls);
+  KG_LAUNCH_CHECK(ctx);
+  return KG_OK;
+}
+
+extern "C" int kg_text_pairs_equal(kg_ctx* ctx, const kg_text_col* a, const int32_t* d_rows_a, const kg_text_col* b,
+                                   const int32_t* d_rows_b, int64_t n, int32_t* d_out_idx, int64_t* out_count) {
+  if (!ctx) return KG_ERR_INVALID;
+  int st = check_text(ctx, a);
+  if (st != KG_OK || (st = check_text(ctx, b)) != KG_OK) return st;
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "pair count out of range");
+  KG_REQUIRE(ctx, out_count && (n == 0 || d_out_idx), "null output pointer");
+  *out_count = 0;
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  size_t cub_sel = 0;
+  KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(nullptr, cub_sel, static_cast<int32_t*>(nullptr),
+                                                  static_cast<uint8_t*>(nullptr), static_cast<int32_t*>(nullptr),
+                                                  static_cast<int64_t*>(nullptr), static_cast<int>(n), ctx->stream));
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_ids = carve(n * 4), o_fl = carve(n), o_cnt = carve(16), o_cub = carve(cub_sel);
+  if ((st = kg_work_reserve(ctx, off)) != KG_OK) return st;
+  char* w = static_cast<char*>(ctx->work);
+  int32_t* ids = reinterpret_cast<int32_t*>(w + o_ids);
+  uint8_t* fl = reinterpret_cast<uint8_t*>(w + o_fl);
+  int64_t* cnt = reinterpret_cast<int64_t*>(w + o_cnt);
+  hipLaunchKernelGGL(k_text_pairs_eq, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream, *a, d_rows_a, *b, d_rows_b,
+                     n, ids, fl);
+  KG_LAUNCH_CHECK(ctx);
+  size_t cb = cub_sel;
+  KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(w + o_cub, cb, ids, fl, d_out_idx, cnt, static_cast<int>(n),
+                                                  ctx->stream));
+  int64_t h = 0;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *out_count = h;
   return KG_OK;
 }

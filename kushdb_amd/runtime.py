@@ -565,6 +565,30 @@ def text_select(ctx, col, op, pattern, rows=None):
     return out[: cnt.value]
 
 
+
+def text_hash(ctx, col, rows=None):
+    """64-bit hashes (int64) and NULL flags (int8) of TEXT values: the keys a
+    TEXT equi-join builds and probes (kg_text_hash)."""
+    n = rows.numel() if rows is not None else len(col)
+    dev = col.offsets.device
+    h = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+    nl = torch.empty(max(n, 1), dtype=torch.int8, device=dev)
+    d = col.desc()
+    ctx.check(ctx.lib.kg_text_hash(ctx.h, ctypes.byref(d), _ptr(rows), n, _ptr(h), _ptr(nl)))
+    return h[:n], nl[:n]
+
+
+def text_pairs_equal(ctx, a, rows_a, b, rows_b):
+    """Positions i (ascending, int32) where a[rows_a[i]] == b[rows_b[i]] and
+    neither is NULL (kg_text_pairs_equal)."""
+    n = rows_a.numel()
+    out = torch.empty(max(n, 1), dtype=torch.int32, device=a.offsets.device)
+    cnt = ctypes.c_int64()
+    da, db = a.desc(), b.desc()
+    ctx.check(ctx.lib.kg_text_pairs_equal(ctx.h, ctypes.byref(da), _ptr(rows_a.contiguous()), ctypes.byref(db),
+                                          _ptr(rows_b.contiguous()), n, _ptr(out), ctypes.byref(cnt)))
+    return out[: cnt.value]
+
 def text_minmax(ctx, col, rows=None, is_max=False, cap=4096):
     """(position, bytes) of the first MIN (MAX) non-NULL value, or (-1, None)."""
     n = rows.numel() if rows is not None else len(col)

@@ -432,6 +432,8 @@ def skinner_join(cards, conds, rows=None):
         if ns is not None and ns[r]:
             return None
         v = ks[r]
+        if isinstance(v, (str, bytes)):
+            return v
         return float(v) if isinstance(v, (float, np.floating)) else int(v)
 
     if any(c == 0 for c in cards):
@@ -485,3 +487,23 @@ def skinner_join(cards, conds, rows=None):
         if not partial:
             return []
     return sorted({tuple(p[t] for t in range(nt)) for p in partial})
+
+
+def text_hash_join(build, build_nulls, probe, probe_nulls):
+    """TEXT-key hash join: (probe_rows, build_rows) of every pair of equal,
+    non-NULL strings, probe order then build insertion order
+    (compile/translators/hash_join_translator.cc:79-139 with CheckEquality over
+    String::Equals, runtime/string.cc:42-46; NULL build keys are skipped,
+    :86-101)."""
+    buckets = {}
+    for i, v in enumerate(build):
+        if build_nulls is None or not build_nulls[i]:
+            buckets.setdefault(v, []).append(i)
+    pr, br = [], []
+    for j, v in enumerate(probe):
+        if probe_nulls is not None and probe_nulls[j]:
+            continue
+        for i in buckets.get(v, []):
+            pr.append(j)
+            br.append(i)
+    return np.array(pr, dtype=np.int32), np.array(br, dtype=np.int32)

@@ -24,6 +24,9 @@ class OracleEngine:
     def join(self, b, p):
         return po.hash_join([self._col(b)], [self._col(p)])
 
+    def join_text(self, b, p):
+        return po.text_hash_join(b[0], b[1], p[0], p[1])
+
     def group_agg(self, codes, card, aggs, rows, keys_out=False):
         keys = [po.Col(po.INT, codes)]
         pa = [(k, self._col(c) if c is not None else None) for k, c in aggs]
@@ -69,18 +72,48 @@ class GpuEngine:
         rows = self.ctx.scan_select([self._col(c) for c in cols], preds, conj)
         return rows.cpu().numpy()
 
+    def _text(self, strings, nulls):
+        from kushdb_amd.runtime import TextColumn
+        return TextColumn.from_strings(strings, device=self.dev,
+                                       nulls=None if nulls is None else np.ascontiguousarray(nulls, dtype=np.int8))
+
     def skinner(self, cards, conds, budget=0, seed=1):
-        from kushdb_amd.runtime import skinner_join
+        """TEXT conditions join on device string hashes (kg_text_hash) and the
+        result tuples are then checked string by string (kg_text_pairs_equal)."""
+        from kushdb_amd.runtime import skinner_join, text_hash, text_pairs_equal
         cache = {}
 
         def dev(c):
             if id(c[1]) not in cache:
-                cache[id(c[1])] = self._col(c)
+                if c[0] == "text":
+                    tc = self._text(c[1], c[2])
+                    h, nl = text_hash(self.ctx, tc)
+                    cache[id(c[1])] = (self.Column(self.abi.KG_BIGINT, h, nl), tc)
+                else:
+                    cache[id(c[1])] = (self._col(c), None)
             return cache[id(c[1])]
         t, self.last_stats = skinner_join(self.ctx, [(c, None) for c in cards],
-                                          [(lt, dev(lc), rt, dev(rc)) for lt, lc, rt, rc in conds],
+                                          [(lt, dev(lc)[0], rt, dev(rc)[0]) for lt, lc, rt, rc in conds],
                                           budget=budget, seed=seed)
+        for lt, lc, rt, rc in conds:
+            if lc[0] == "text" and t.shape[0]:
+                keep = text_pairs_equal(self.ctx, dev(lc)[1], t[:, lt].contiguous(), dev(rc)[1], t[:, rt].contiguous())
+                t = t[keep.long()]
         return sorted(map(tuple, t.cpu().numpy().tolist()))
+
+    def join_text(self, b, p):
+        """TEXT equi-join: BIGINT join on kg_text_hash keys, then the exact
+        pair check (kg_text_pairs_equal)."""
+        from kushdb_amd.runtime import text_hash, text_pairs_equal
+        bt, pt = self._text(*b), self._text(*p)
+        bh, bn = text_hash(self.ctx, bt)
+        ph, pn = text_hash(self.ctx, pt)
+        ht = self.HashTable(self.ctx, max(len(bt), 1))
+        ht.insert(self.Column(self.abi.KG_BIGINT, bh, bn))
+        pr, br = ht.probe(self.Column(self.abi.KG_BIGINT, ph, pn))
+        ht.free()
+        keep = text_pairs_equal(self.ctx, bt, br, pt, pr).long()
+        return pr[keep].cpu().numpy(), br[keep].cpu().numpy()
 
     def join(self, b, p):
         bc, pc = self._col(b), self._col(p)

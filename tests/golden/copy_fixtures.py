@@ -26,7 +26,8 @@ DATA = (
 )
 
 EXPECTED = (
-    [f"hash_join/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real")]
+    [f"hash_join/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "boolean", "enum",
+                                                "text")]
     + [f"scan_select/{t}_expected.tbl" for t in ("int", "bigint", "date", "real", "smallint", "multiple_eq")]
     + [f"group_by_aggregate/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "real", "date")]
     + ["aggregate/int_expected.tbl", "aggregate/empty_expected.tbl", "tpch/q01_expected.tbl"]
@@ -35,7 +36,7 @@ EXPECTED = (
     + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
     + ["tpch/q06_expected.tbl"]
     + [f"skinner_join/{t}_expected.tbl" for t in ("int", "smallint", "date", "bigint", "real", "boolean",
-                                                   "enum", "multiple_tables_join")]
+                                                   "enum", "text", "multiple_tables_join")]
 )
 
 

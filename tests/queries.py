@@ -45,13 +45,23 @@ HASH_JOIN = {
     "smallint": ("info_num1", "info_num1", "smallint"),
     "date": ("info_date", "info_date", "date"),
     "real": ("info_zscore", "info_zscore", "real"),
+    "boolean": ("info_cheated", "info_cheated", "boolean"),
+    "enum": ("people_name_enum", "people_name_enum", "enum"),   # EnumSchema people.name
+    "text": ("people_name", "people_name", "text"),
 }
 
 
 def hash_join(engine, t):
     b, p, typ = HASH_JOIN[t]
+    if typ == "text":
+        bs, ps = kdb.text(b), kdb.text(p)
+        pr, br = engine.join_text(bs, ps)
+        return [kdb.row([bs[0][br[i]], ps[0][pr[i]]]) for i in range(len(pr))]
     bc, pc = _c(b, typ), _c(p, typ)
     pr, br = engine.join(bc, pc)
+    if typ == "enum":
+        d = kdb.enum_dict(b)
+        return [kdb.row([d[int(bc[1][br[i]])], d[int(pc[1][pr[i]])]]) for i in range(len(pr))]
     lines = []
     for i in range(len(pr)):
         lines.append(kdb.row([kdb.fmt(typ, bc[1][br[i]]), kdb.fmt(typ, pc[1][pr[i]])]))
@@ -217,6 +227,7 @@ SKINNER = {
     "real": ("info_zscore", "real"),
     "boolean": ("info_cheated", "boolean"),
     "enum": ("people_name_enum", "enum"),
+    "text": ("people_name", "text"),
 }
 
 
@@ -231,10 +242,12 @@ def skinner(engine, t, **kw):
         return [kdb.row([kdb.fmt("int", ident[1][tp[i]], ident[2] is not None and ident[2][tp[i]])
                          for i in range(4)]) for tp in tuples]
     name, typ = SKINNER[t]
-    c = _c(name, typ)
+    c = ("text", *kdb.text(name)) if typ == "text" else _c(name, typ)
     n = len(c[1])
     tuples = engine.skinner([n, n], [(0, c, 1, c)], **kw)
     if typ == "enum":
         d = kdb.enum_dict(name)
         return [kdb.row([d[int(c[1][tp[0]])], d[int(c[1][tp[1]])]]) for tp in tuples]
+    if typ == "text":
+        return [kdb.row([c[1][tp[0]], c[1][tp[1]]]) for tp in tuples]
     return [kdb.row([kdb.fmt(typ, c[1][tp[i]]) for i in range(2)]) for tp in tuples]
