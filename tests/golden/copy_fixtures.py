@@ -22,16 +22,17 @@ DATA = (
        "l_returnflag_enum.kdb", "l_linestatus_enum.kdb",
        "l_returnflag_enum.kdbenum", "l_linestatus_enum.kdbenum",
        "people_name.kdb", "people_name_null.kdb", "people_name_enum.kdb", "people_name_enum_null.kdb",
-       "people_name_enum.kdbenum"]
+       "people_name_enum.kdbenum", "c_custkey.kdb"]
 )
 
 EXPECTED = (
     [f"hash_join/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "boolean", "enum",
                                                 "text")]
-    + [f"scan_select/{t}_expected.tbl" for t in ("int", "bigint", "date", "real", "smallint", "multiple_eq")]
+    + [f"scan_select/{t}_expected.tbl" for t in ("int", "bigint", "date", "real", "smallint", "multiple_eq",
+                                                  "boolean_eq", "boolean", "enum", "int_eq", "text")]
     + [f"group_by_aggregate/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "real", "date")]
     + ["aggregate/int_expected.tbl", "aggregate/empty_expected.tbl", "tpch/q01_expected.tbl"]
-    + [f"simd_scan_select/{t}_expected.tbl" for t in ("int", "date_range", "multiple_col")]
+    + [f"simd_scan_select/{t}_expected.tbl" for t in ("int", "date_range", "multiple_col", "date", "enum")]
     + [f"order_by/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text")]
     + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
     + ["tpch/q06_expected.tbl"]

@@ -21,7 +21,7 @@ from kushdb_amd.expr import (Add, And, Case, ColRef, Contains, Div, EndsWith, Eq
                              expr_select)
 from kushdb_amd.runtime import Column, Context, KgError, TextColumn, order_by
 from oracle import pyoracle as po
-from test_oracle_expr import SELECT, T, oracle_col
+from test_oracle_expr import SCAN, SELECT, T, oracle_col
 
 pytestmark = pytest.mark.gpu
 
@@ -50,6 +50,34 @@ def test_select_fixture(ctx, name):
     rows = expr_select(ctx, len(col), [col], flt(ColRef(0, T[typ])), dictionaries={0: d} if d else None)
     got = sorted(printed[r] + "|" for r in rows.cpu().numpy())
     assert got == sorted(kdb.expected(f"select/{name}_expected.tbl"))
+
+
+@pytest.mark.parametrize("name", sorted(SCAN))
+def test_scan_fixture(ctx, name):
+    """ScanSelect / SimdScanSelect fixtures: the general expression filter
+    (kg_expr_select) and, where the predicate is column-op-literal, the scan
+    fast path (kg_scan_select; ENUM literals mapped to their code through the
+    dictionary as the SIMD translator does) or kg_text_select."""
+    from kushdb_amd.runtime import text_select
+    print("scan fixture", name, flush=True)
+    f, typ, flt, fast = SCAN[name]
+    col, d = dev_col(ctx, f, typ)
+    _, printed = oracle_col(f, typ)
+    want = sorted(kdb.expected(f"{name}_expected.tbl"))
+    rows = expr_select(ctx, len(col), [col], flt(ColRef(0, T[typ])), dictionaries={0: d} if d else None)
+    assert sorted(printed[r] + "|" for r in rows.cpu().numpy()) == want
+    if fast is None:
+        return
+    if fast[0] == "text":
+        r2 = text_select(ctx, col, abi.KG_TEXT_GT, fast[2])
+    else:
+        op, lit = fast
+        if isinstance(lit, tuple) and lit[0] == "date":
+            lit = po.build_date(*lit[1:])
+        elif isinstance(lit, tuple) and lit[0] == "code":
+            lit = kdb.enum_dict(f).index(lit[1])
+        r2 = ctx.scan_select([col], [(0, op, lit)])
+    assert np.array_equal(np.sort(r2.cpu().numpy()), np.sort(rows.cpu().numpy()))
 
 
 def test_tpch_q06(ctx):

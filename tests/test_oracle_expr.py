@@ -37,6 +37,24 @@ SELECT = {
     "text": ("people_name", "text", lambda c: Gt(c, Literal("M"))),
 }
 
+# scan_select / simd_scan_select fixtures with one printed column: (column file, type, filter on
+# ColRef(0), fast-path predicate (op, literal) for kg_scan_select / kg_text_select or None)
+# (end_to_end_test/scan_select/{boolean_eq,boolean,enum,int_eq,text}_test.cc,
+#  end_to_end_test/simd_scan_select/{date,enum}_test.cc)
+SCAN = {
+    "scan_select/boolean_eq": ("info_cheated", "boolean", lambda c: Eq(c, Literal(True)), (po.EQ, 1)),
+    "scan_select/boolean": ("info_cheated", "boolean", lambda c: c, (po.EQ, 1)),
+    "scan_select/enum": ("people_name_enum", "enum", lambda c: Gt(c, Literal("M")), None),
+    "scan_select/text": ("people_name", "text", lambda c: Gt(c, Literal("M")), ("text", po.GT, "M")),
+    "scan_select/int_eq": ("c_custkey", "int", lambda c: Eq(c, Literal(555)), (po.EQ, 555)),
+    "simd_scan_select/date": ("l_shipdate", "date", lambda c: Leq(c, Literal((1998, 9, 13))),
+                              (po.LEQ, ("date", 1998, 9, 13))),
+    # ENUM equality compares codes: the literal is the dictionary code of "R" (an enum literal)
+    "simd_scan_select/enum": ("l_returnflag_enum", "enum",
+                              lambda c: Eq(c, Literal(kdb.enum_dict("l_returnflag_enum").index("R"), abi.KG_ENUM)),
+                              (po.EQ, ("code", "R"))),
+}
+
 
 def oracle_col(name, typ):
     if typ == "text":
@@ -246,3 +264,11 @@ def test_oracle_strings_like_in():
     v, nl = oracle_eval(In(E, [1, 3]), cols, n)
     assert not nl.any()
     assert v.astype(bool).tolist() == [c in (1, 3) and not z for c, z in zip(codes, en)]
+
+
+@pytest.mark.parametrize("name", sorted(SCAN))
+def test_oracle_scan_fixture(name):
+    f, typ, flt, _ = SCAN[name]
+    col, printed = oracle_col(f, typ)
+    rows = selected(flt(ColRef(0, T[typ])), [col], len(printed))
+    assert sorted(printed[r] + "|" for r in rows) == sorted(kdb.expected(f"{name}_expected.tbl"))
