@@ -30,7 +30,7 @@ EXPECTED = (
                                                 "text")]
     + [f"scan_select/{t}_expected.tbl" for t in ("int", "bigint", "date", "real", "smallint", "multiple_eq",
                                                   "boolean_eq", "boolean", "enum", "int_eq", "text")]
-    + [f"group_by_aggregate/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "real", "date")]
+    + [f"group_by_aggregate/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "real", "date", "enum", "text")]
     + ["aggregate/int_expected.tbl", "aggregate/empty_expected.tbl", "tpch/q01_expected.tbl"]
     + [f"simd_scan_select/{t}_expected.tbl" for t in ("int", "date_range", "multiple_col", "date", "enum")]
     + [f"order_by/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text")]

@@ -287,3 +287,70 @@ def test_rejected_expressions(ctx):
     with pytest.raises(KgError):
         expr_eval(ctx, n, [i, r], Lt(ColRef(0, abi.KG_BIGINT), I))   # column type mismatch
     assert expr_select(ctx, 0, [i, r], Gt(I, Literal(3))).numel() == 0
+
+
+def test_group_by_enum_fixture(ctx):
+    """group_by_aggregate/enum: GROUP BY an ENUM column with NULLs, MIN/MAX by
+    string order, COUNT(*), COUNT(name), all on the device: the key is the code
+    with NULL as one more code (CASE(IS NULL) on the device), MIN/MAX run over
+    each code's string rank (kg_enum_ranks, gathered on the device) and map back
+    through the sorted dictionary."""
+    from kushdb_amd.runtime import enum_ranks, gather
+    from test_oracle_expr import groupby_enum_rows
+    col, d = dev_col(ctx, "people_name_enum", "enum")
+    strings = kdb.enum_dict("people_name_enum")
+    card = len(strings)
+    n = len(col)
+    key = expr_eval(ctx, n, [col], Case(IsNull(ColRef(0, abi.KG_ENUM)), Literal(card, abi.KG_ENUM),
+                                        ColRef(0, abi.KG_ENUM)))
+    code0 = expr_eval(ctx, n, [col], Case(IsNull(ColRef(0, abi.KG_ENUM)), Literal(0, abi.KG_ENUM),
+                                          ColRef(0, abi.KG_ENUM)))
+    ranks = np.asarray(enum_ranks(strings), dtype=np.int32)
+    rcol = gather(ctx, Column(abi.KG_INT, torch.from_numpy(ranks).to("cuda:0")), code0.data.contiguous())
+    rank_col = Column(abi.KG_INT, rcol.data, col.nulls)
+    kc = Column(abi.KG_ENUM, key.data, None)
+    # 943 groups x 4 aggregates exceed one workgroup's LDS table budget: two
+    # passes (the group order is the first appearance in both).
+    codes, v1, n1 = ctx.scan_group_agg([kc, rank_col], [], [(0, card + 1)],
+                                       [(abi.KG_AGG_MIN, 1), (abi.KG_AGG_MAX, 1)], max_groups=card + 1)
+    codes2, v2, n2 = ctx.scan_group_agg([kc, col], [], [(0, card + 1)],
+                                        [(abi.KG_AGG_COUNT, -1), (abi.KG_AGG_COUNT, 1)], max_groups=card + 1)
+    assert np.array_equal(codes, codes2)
+    vals, nulls = np.concatenate([v1, v2], axis=1), np.concatenate([n1, n2], axis=1)
+    by_rank = [None] * card
+    for c, r in enumerate(ranks):
+        by_rank[r] = strings[c]
+    got = []
+    for g in range(len(codes)):
+        mn = "" if nulls[g, 0] else by_rank[int(vals[g, 0])]
+        mx = "" if nulls[g, 1] else by_rank[int(vals[g, 1])]
+        got.append(f"{mn}|{mx}|{int(vals[g, 2])}|{int(vals[g, 3])}|")
+    assert sorted(got) == sorted(kdb.expected("group_by_aggregate/enum_expected.tbl"))
+    assert got == groupby_enum_rows()   # the same first-appearance group order
+
+
+def test_group_by_text_fixture(ctx):
+    """group_by_aggregate/text: GROUP BY CASE(id > 500, 'hello', '') (the two
+    branches as codes 1 / 0 from a device CASE; a NULL id takes the ELSE
+    branch), MIN/MAX(name) per group with kg_text_minmax over the group's rows,
+    COUNT(*) and COUNT(name) from the device group-by (COUNT(name) over a column
+    carrying name's NULL flags)."""
+    from kushdb_amd.runtime import text_minmax
+    from test_oracle_expr import groupby_text_rows
+    ids, _ = dev_col(ctx, "people_id", "int")
+    names, _ = dev_col(ctx, "people_name", "text")
+    n = len(ids)
+    code = expr_eval(ctx, n, [ids], Case(Gt(ColRef(0, abi.KG_INT), Literal(500)), Literal(1), Literal(0)))
+    assert code.nulls is None or int(code.nulls.sum()) == 0
+    kc = Column(abi.KG_INT, code.data, None)
+    name_flags = Column(abi.KG_INT, ids.data, names.nulls)
+    codes, vals, _ = ctx.scan_group_agg([kc, name_flags], [], [(0, 2)], [(abi.KG_AGG_COUNT, -1),
+                                                                          (abi.KG_AGG_COUNT, 1)])
+    got = []
+    for g in range(len(codes)):
+        rows = ctx.scan_select([kc], [(0, abi.KG_EQ, int(codes[g, 0]))])
+        _, mn = text_minmax(ctx, names, rows)
+        _, mx = text_minmax(ctx, names, rows, is_max=True)
+        got.append(f"{(mn or b'').decode()}|{(mx or b'').decode()}|{int(vals[g, 0])}|{int(vals[g, 1])}|")
+    assert sorted(got) == sorted(kdb.expected("group_by_aggregate/text_expected.tbl"))
+    assert got == groupby_text_rows()

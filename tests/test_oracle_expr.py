@@ -272,3 +272,48 @@ def test_oracle_scan_fixture(name):
     col, printed = oracle_col(f, typ)
     rows = selected(flt(ColRef(0, T[typ])), [col], len(printed))
     assert sorted(printed[r] + "|" for r in rows) == sorted(kdb.expected(f"{name}_expected.tbl"))
+
+
+def group_minmax_count(keys, values, value_nulls, less=None):
+    """GROUP BY key with MIN(v), MAX(v), COUNT(*), COUNT(v), groups in first-appearance
+    order: NULL values are skipped by MIN/MAX (compile/proxy/aggregator.cc:14-171, ENUM and
+    TEXT by string order); COUNT(v) starts at 1 on the group's first row whatever its NULL
+    flag and then counts non-NULL values (aggregator.cc:270-298). keys: hashable per row
+    (None = the NULL key). Returns printed rows min|max|count1|count2|."""
+    g = {}
+    for i, k in enumerate(keys):
+        v = None if value_nulls is not None and value_nulls[i] else values[i]
+        if k not in g:
+            g[k] = [v, v, 1, 1]
+            continue
+        st = g[k]
+        if v is not None:
+            st[0] = v if st[0] is None or v < st[0] else st[0]
+            st[1] = v if st[1] is None or v > st[1] else st[1]
+            st[3] += 1
+        st[2] += 1
+    return [f"{'' if a is None else a}|{'' if b is None else b}|{c}|{d}|" for a, b, c, d in g.values()]
+
+
+def groupby_enum_rows():
+    """group_by_aggregate/enum_test.cc (EnumSchema): GROUP BY people.name with MIN/MAX(name),
+    COUNT(*), COUNT(name)."""
+    codes, nulls = kdb.col("people_name_enum", "enum")
+    d = kdb.enum_dict("people_name_enum")
+    keys = [None if nulls is not None and nulls[i] else int(c) for i, c in enumerate(codes)]
+    return group_minmax_count(keys, [d[int(c)] for c in codes], nulls)
+
+
+def groupby_text_rows():
+    """group_by_aggregate/text_test.cc: GROUP BY CASE(id > 500, 'hello', '') with
+    MIN/MAX(name), COUNT(*), COUNT(name); a NULL id takes the ELSE branch."""
+    ids, idn = kdb.col("people_id", "int")
+    names, nn = kdb.text("people_name")
+    keys = ["hello" if (idn is None or not idn[i]) and ids[i] > 500 else "" for i in range(len(ids))]
+    return group_minmax_count(keys, names, nn)
+
+
+@pytest.mark.parametrize("t", ["enum", "text"])
+def test_oracle_group_by_string_fixture(t):
+    got = groupby_enum_rows() if t == "enum" else groupby_text_rows()
+    assert sorted(got) == sorted(kdb.expected(f"group_by_aggregate/{t}_expected.tbl"))
