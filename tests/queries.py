@@ -251,3 +251,12 @@ def skinner(engine, t, **kw):
     if typ == "text":
         return [kdb.row([c[1][tp[0]], c[1][tp[1]]]) for tp in tuples]
     return [kdb.row([kdb.fmt(typ, c[1][tp[i]]) for i in range(2)]) for tp in tuples]
+
+
+def cross_product(engine):
+    """cross_product_test.cc: region.r_name x region.r_name (CrossProductOperator), as a
+    join with no conditions."""
+    names, nulls = kdb.text("r_name")
+    n = len(names)
+    tuples = engine.skinner([n, n], [])
+    return [kdb.row([names[a], names[b]]) for a, b in tuples]

@@ -130,3 +130,9 @@ def test_skinner_chain_query_matches_fixed_order():
         assert a0 == a1 and b0 == b1
         assert st["episodes"] >= 1 and len(st["last_order"]) == 5
     ctx.close()
+
+
+def test_cross_product_fixture(E):
+    """CrossProductOperator (cross_product_test.cc) on the device: the join with
+    no conditions, every episode a cross product of its slice."""
+    assert sorted(queries.cross_product(E)) == sorted(kdb.expected("cross_product/cross_product_expected.tbl"))

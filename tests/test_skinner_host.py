@@ -139,3 +139,7 @@ def test_uct_misuse():
     a.reward(0.0)
     with pytest.raises(abi.KgError):
         UctAgent(3, [], prefix=(0, 0))
+
+
+def test_oracle_cross_product_fixture():
+    assert sorted(queries.cross_product(E)) == sorted(kdb.expected("cross_product/cross_product_expected.tbl"))
