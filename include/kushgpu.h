@@ -287,6 +287,25 @@ int kg_group_by_i64(kg_ctx* ctx, const int64_t* d_keys, const int64_t* d_vals, i
                     int64_t max_groups, int64_t* d_out_key, int64_t* d_out_sum,
                     int64_t* d_out_count, int32_t* d_out_first, int64_t* out_ngroups);
 
+/* GROUP BY over hashed keys for any number of groups: runtime::AggregateHashTable
+ * and the generated UpdateOrInsert loop (runtime/aggregate_hash_table.cc:9-135,
+ * compile/proxy/aggregate_hash_table.cc:187-343) for keys that are not small
+ * dense codes. Rows 0..n-1 passing preds (as kg_scan_group_agg) are grouped by
+ * the key columns key_cols[0..nkeys) (1..4 fixed-width columns; NULL is a key
+ * value of its own; REAL keys compare with ==, so -0.0 and 0.0 are one group).
+ * Groups come out in insertion order (their first qualifying row ascending,
+ * ForEach's order): out_first_rows[g] is that row (the group's key values are
+ * the key columns there); out_vals / out_nulls [g * naggs + a] as
+ * kg_scan_group_agg (SUM wraps in the child type, AVG = sum / count, COUNT(col)
+ * counts the first row whatever its NULL flag). FP64 SUM/AVG add in a
+ * timing-dependent order (within 1e-12 relative of the sequential fold).
+ * max_groups sizes the table: more groups return KG_ERR_CAPACITY. Outputs are
+ * host arrays. TEXT keys group by their kg_text_hash codes. */
+int kg_hash_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols, const kg_pred* preds,
+                      int32_t npreds, const int32_t* key_cols, int32_t nkeys, const kg_agg* aggs, int32_t naggs,
+                      int64_t max_groups, int64_t* out_ngroups, int32_t* out_first_rows, int64_t* out_vals,
+                      int8_t* out_nulls);
+
 /* ---- column indexes (Skinner joins) --------------------------------------------
  * Device replacement for runtime::MemoryColumnIndex
  * (runtime/memory_column_index.h:18-56, compile/proxy/memory_column_index.cc:282-428)

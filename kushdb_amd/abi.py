@@ -190,6 +190,8 @@ SIGNATURES = {
     "kg_uct_next": (_i32, [_vp, _pi32]),
     "kg_uct_reward": (_i32, [_vp, ctypes.c_double]),
     "kg_uct_free": (None, [_vp]),
+    "kg_hash_group_agg": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32, _pi32, _i32,
+                                 ctypes.POINTER(KgAgg), _i32, _i64, _pi64, _pi32, _pi64, _pi8]),
     "kg_text_hash": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, _i64, _vp, _vp]),
     "kg_text_pairs_equal": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, ctypes.POINTER(KgTextCol), _vp, _i64, _vp,
                                    _pi64]),

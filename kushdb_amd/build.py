@@ -30,6 +30,7 @@ SOURCES = [
     "kg_order.hip",
     "kg_expr.hip",
     "kg_skinner.hip",
+    "kg_group.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -208,6 +208,27 @@ class Context:
         g = ng.value
         return codes[:g, : len(keys)], vals[:g, : len(aggs)], nulls[:g, : len(aggs)]
 
+    # -- GroupByAggregate over hashed keys (any group count) ----------------
+    def hash_group_agg(self, cols, preds, key_cols, aggs, max_groups=1 << 16):
+        """key_cols = [col_idx] (1..4), aggs = [(kind, col_idx)]. Returns
+        (first rows int32 [G], values int64 [G, naggs], nulls int8 [G, naggs])
+        in insertion order; a group's keys are the key columns at its first row."""
+        n = len(cols[0]) if cols else 0
+        carr = (KgCol * max(len(cols), 1))(*[c.desc() for c in cols])
+        parr = (KgPred * max(len(preds), 1))(*[_pred(cols, p) for p in preds])
+        kc = (ctypes.c_int32 * max(len(key_cols), 1))(*key_cols)
+        aarr = _aggs(aggs)
+        first = np.zeros(max(max_groups, 1), dtype=np.int32)
+        vals = np.zeros((max(max_groups, 1), max(len(aggs), 1)), dtype=np.int64)
+        nulls = np.zeros((max(max_groups, 1), max(len(aggs), 1)), dtype=np.int8)
+        ng = ctypes.c_int64()
+        self.check(self.lib.kg_hash_group_agg(
+            self.h, n, carr, len(cols), parr, len(preds), kc, len(key_cols), aarr, len(aggs), max_groups,
+            ctypes.byref(ng), first.ctypes.data_as(abi._pi32), vals.ctypes.data_as(abi._pi64),
+            nulls.ctypes.data_as(abi._pi8)))
+        g = ng.value
+        return first[:g], vals[:g, : len(aggs)], nulls[:g, : len(aggs)]
+
     # -- GroupByAggregate on a BIGINT key (high cardinality) ----------------
     def group_by_i64(self, keys, vals, max_groups=None, with_first=False):
         n = keys.numel()
