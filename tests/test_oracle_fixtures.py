@@ -36,3 +36,8 @@ def test_simd_scan_select(t):
 
 def test_tpch_q01():
     queries.check_q01(queries.tpch_q01(E), kdb.expected("tpch/q01_expected.tbl"))
+
+
+def test_tpch_q03():
+    import tpch_q03
+    tpch_q03.check(tpch_q03.oracle_q03(tpch_q03.columns()), kdb.expected("tpch/q03_expected.tbl"))
