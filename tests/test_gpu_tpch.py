@@ -1,5 +1,5 @@
-"""TPC-H Q3 end to end on the device operators, composed as the reference's
-q03_test.cc plan: TEXT select (kg_text_select), scan-selects (kg_scan_select),
+"""TPC-H Q3 and Q12 end to end on the device operators. Q3 as the reference's
+q03_test.cc plans it: TEXT select (kg_text_select), scan-selects (kg_scan_select),
 two hash joins (kg_ht_build / kg_ht_probe over gathered keys), the derived
 revenue (kg_expr_eval), GROUP BY over three hashed keys (kg_hash_group_agg)
 and ORDER BY revenue DESC, o_orderdate (kg_order_by). Checked against the
@@ -74,4 +74,57 @@ def test_tpch_q03():
     want = tpch_q03.oracle_q03(c)
     assert [(k, d, p) for k, _, d, p in got] == [(k, d, p) for k, _, d, p in want]
     assert np.allclose([r for _, r, _, _ in got], [r for _, r, _, _ in want], rtol=1e-12, atol=0)
+    ctx.close()
+
+
+def device_q12(ctx):
+    """Q12 on the device: the lineitem filter as one expression over a TEXT and
+    three DATE columns (kg_expr_select), the join (kg_ht_build on the selected
+    lineitem keys, kg_ht_probe with all orders), the CASE counts over the
+    orders' TEXT priority (kg_expr_eval at the joined rows), GROUP BY l_shipmode
+    through its string hash (kg_text_hash codes, kg_hash_group_agg), each
+    group's row count checked against an exact string select so that no two
+    strings share a group, ORDER BY l_shipmode (kg_order_by over TEXT)."""
+    import tpch_q12
+    from kushdb_amd.expr import And, Case, Eq, Geq, Lt, Neq, Or, expr_select
+    from kushdb_amd.runtime import text_hash
+    c = tpch_q12.columns()
+    sm = TextColumn.from_strings(c["l_shipmode"], device=DEV)
+    pri = TextColumn.from_strings(c["o_orderpriority"], device=DEV)
+    commit, receipt, ship = (dev(c[k], abi.KG_DATE) for k in ("l_commitdate", "l_receiptdate", "l_shipdate"))
+    T, D = abi.KG_TEXT, abi.KG_DATE
+    flt = And([Or([Eq(ColRef(0, T), Literal("RAIL")), Eq(ColRef(0, T), Literal("MAIL"))]),
+               Lt(ColRef(1, D), ColRef(2, D)), Lt(ColRef(3, D), ColRef(1, D)),
+               Geq(ColRef(2, D), Literal(tpch_q12.LO)), Lt(ColRef(2, D), Literal(tpch_q12.HI))])
+    rows_l = expr_select(ctx, len(c["l_orderkey"]), [sm, commit, receipt, ship], flt)
+    bk = gather(ctx, dev(c["l_orderkey"], abi.KG_INT), rows_l)
+    ht = HashTable(ctx, max(len(bk), 1))
+    ht.insert(bk)
+    orows, bi = ht.probe(dev(c["o_orderkey"], abi.KG_INT))
+    ht.free()
+    lrows = rows_l[bi.long()].contiguous()
+    codes, _ = text_hash(ctx, sm, lrows)
+    urgent = Or([Eq(ColRef(0, T), Literal("1-URGENT")), Eq(ColRef(0, T), Literal("2-HIGH"))])
+    high = expr_eval(ctx, len(c["o_orderkey"]), [pri], Case(urgent, Literal(1), Literal(0)), rows=orows.contiguous())
+    low = expr_eval(ctx, len(c["o_orderkey"]), [pri],
+                    Case(And([Neq(ColRef(0, T), Literal("1-URGENT")), Neq(ColRef(0, T), Literal("2-HIGH"))]),
+                         Literal(1), Literal(0)), rows=orows.contiguous())
+    first, vals, _ = ctx.hash_group_agg([Column(abi.KG_BIGINT, codes), high, low], [], [0],
+                                        [(abi.KG_AGG_SUM, 1), (abi.KG_AGG_SUM, 2), (abi.KG_AGG_COUNT, -1)],
+                                        max_groups=64)
+    names = [c["l_shipmode"][int(r)] for r in lrows.cpu().numpy()[first]]
+    for g, s in enumerate(names):   # every row of a group has the group's string
+        exact = text_select(ctx, sm, abi.KG_TEXT_EQ, s, lrows)
+        assert len(exact) == vals[g, 2]
+    gsm = TextColumn.from_strings(names, device=DEV)
+    perm = order_by(ctx, [gsm], [True]).cpu().numpy()
+    return [(names[i], int(vals[i, 0]), int(vals[i, 1])) for i in perm]
+
+
+def test_tpch_q12():
+    import tpch_q12
+    ctx = Context(0)
+    got = device_q12(ctx)
+    assert tpch_q12.lines(got) == kdb.expected("tpch/q12_expected.tbl")
+    assert got == tpch_q12.oracle_q12(tpch_q12.columns())
     ctx.close()

@@ -41,3 +41,8 @@ def test_tpch_q01():
 def test_tpch_q03():
     import tpch_q03
     tpch_q03.check(tpch_q03.oracle_q03(tpch_q03.columns()), kdb.expected("tpch/q03_expected.tbl"))
+
+
+def test_tpch_q12():
+    import tpch_q12
+    assert tpch_q12.lines(tpch_q12.oracle_q12(tpch_q12.columns())) == kdb.expected("tpch/q12_expected.tbl")
