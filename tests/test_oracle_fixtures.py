@@ -46,3 +46,8 @@ def test_tpch_q03():
 def test_tpch_q12():
     import tpch_q12
     assert tpch_q12.lines(tpch_q12.oracle_q12(tpch_q12.columns())) == kdb.expected("tpch/q12_expected.tbl")
+
+
+def test_tpch_q05():
+    import tpch_q05
+    tpch_q05.check(tpch_q05.oracle_q05(tpch_q05.columns()), kdb.expected("tpch/q05_expected.tbl"))
