@@ -25,7 +25,8 @@ DATA = (
        "people_name_enum.kdbenum", "c_custkey.kdb", "r_name.kdb", "c_mktsegment.kdb", "o_orderkey.kdb",
        "o_custkey.kdb", "o_orderdate.kdb", "o_shippriority.kdb", "l_orderkey.kdb", "l_shipmode.kdb",
        "l_commitdate.kdb", "l_receiptdate.kdb", "o_orderpriority.kdb", "r_regionkey.kdb", "n_nationkey.kdb",
-       "n_regionkey.kdb", "n_name.kdb", "c_nationkey.kdb", "l_suppkey.kdb", "s_suppkey.kdb", "s_nationkey.kdb"]
+       "n_regionkey.kdb", "n_name.kdb", "c_nationkey.kdb", "l_suppkey.kdb", "s_suppkey.kdb", "s_nationkey.kdb",
+       "l_partkey.kdb", "p_partkey.kdb", "p_type.kdb"]
 )
 
 EXPECTED = (
@@ -39,7 +40,7 @@ EXPECTED = (
     + [f"order_by/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text")]
     + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
     + ["tpch/q06_expected.tbl", "tpch/q03_expected.tbl", "tpch/q12_expected.tbl", "tpch/q05_expected.tbl",
-       "cross_product/cross_product_expected.tbl"]
+       "tpch/q14_expected.tbl", "cross_product/cross_product_expected.tbl"]
     + [f"skinner_join/{t}_expected.tbl" for t in ("int", "smallint", "date", "bigint", "real", "boolean",
                                                    "enum", "text", "multiple_tables_join")]
 )

@@ -1,4 +1,4 @@
-"""TPC-H Q3, Q5 and Q12 end to end on the device operators. Q3 as the reference's
+"""TPC-H Q3, Q5, Q12 and Q14 end to end on the device operators. Q3 as the reference's
 q03_test.cc plans it: TEXT select (kg_text_select), scan-selects (kg_scan_select),
 two hash joins (kg_ht_build / kg_ht_probe over gathered keys), the derived
 revenue (kg_expr_eval), GROUP BY over three hashed keys (kg_hash_group_agg)
@@ -192,4 +192,36 @@ def test_tpch_q05():
     want = tpch_q05.oracle_q05(tpch_q05.columns())
     assert [n for n, _ in got] == [n for n, _ in want]
     assert np.allclose([r for _, r in got], [r for _, r in want], rtol=1e-12, atol=0)
+    ctx.close()
+
+
+def test_tpch_q14():
+    """Q14 on the device: scan-select, join with part, the CASE over
+    STARTS_WITH(p_type) and the revenue product as expressions at the joined
+    rows, two ungrouped SUMs (kg_scan_group_agg), then 100 * a1 / a2."""
+    import tpch_q14
+    from kushdb_amd.expr import Case, Eq, StartsWith
+    c = tpch_q14.columns()
+    ctx = Context(0)
+    rows = ctx.scan_select([dev(c["l_shipdate"], abi.KG_DATE)],
+                           [(0, abi.KG_GEQ, po.build_date(*tpch_q14.LO)), (0, abi.KG_LT, po.build_date(*tpch_q14.HI))])
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["l_partkey"], abi.KG_INT), rows), dev(c["p_partkey"], abi.KG_INT))
+    li = rows[br].contiguous()
+    ext = gather(ctx, dev(c["l_extendedprice"], abi.KG_REAL), li)
+    disc = gather(ctx, dev(c["l_discount"], abi.KG_REAL), li)
+    R = abi.KG_REAL
+    rev_e = Mul(ColRef(0, R), Sub(Literal(1.0), ColRef(1, R)))
+    rev = expr_eval(ctx, len(li), [ext, disc], rev_e)
+    ptype = TextColumn.from_strings(c["p_type"], device=DEV)
+    promo = expr_eval(ctx, len(c["p_partkey"]), [ptype],
+                      Case(StartsWith(ColRef(0, abi.KG_TEXT), Literal("PROMO")), Literal(1), Literal(0)),
+                      rows=pr.int().contiguous())
+    part_rev = expr_eval(ctx, len(li), [ext, disc, Column(abi.KG_INT, promo.data)],
+                         Case(Eq(ColRef(2, abi.KG_INT), Literal(1)), rev_e, Literal(0.0)))
+    _, vals, _ = ctx.scan_group_agg([Column(R, part_rev.data), Column(R, rev.data)], [], [],
+                                    [(abi.KG_AGG_SUM, 0), (abi.KG_AGG_SUM, 1)])
+    a1, a2 = vals[0, 0:1].view(np.float64)[0], vals[0, 1:2].view(np.float64)[0]
+    got = 100.0 * a1 / a2
+    assert abs(got - float(kdb.expected("tpch/q14_expected.tbl")[0])) < 1e-5
+    assert abs(got - tpch_q14.oracle_q14(c)) <= 1e-12 * abs(got)
     ctx.close()

@@ -51,3 +51,8 @@ def test_tpch_q12():
 def test_tpch_q05():
     import tpch_q05
     tpch_q05.check(tpch_q05.oracle_q05(tpch_q05.columns()), kdb.expected("tpch/q05_expected.tbl"))
+
+
+def test_tpch_q14():
+    import tpch_q14
+    assert abs(tpch_q14.oracle_q14(tpch_q14.columns()) - float(kdb.expected("tpch/q14_expected.tbl")[0])) < 1e-5
