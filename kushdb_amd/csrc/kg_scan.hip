@@ -799,6 +799,41 @@ extern "C" int kg_ctx_agg_order(const kg_ctx* ctx, int64_t n, int32_t* nblocks, 
   return KG_OK;
 }
 
+namespace {
+
+// kg_scan_group_agg for a dense key space too large for an on-chip table:
+// kg_hash_group_agg over the same key columns, then each group's key codes
+// read at its first row.
+int dense_as_hashed(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols, const kg_pred* preds, int32_t npreds,
+                    const int32_t* key_cols, int32_t nkeys, const kg_agg* aggs, int32_t naggs, int64_t max_groups,
+                    int64_t* out_ngroups, int32_t* out_keys, int64_t* out_vals, int8_t* out_nulls) {
+  std::vector<int32_t> first(static_cast<size_t>(std::max<int64_t>(max_groups, 1)));
+  int st = kg_hash_group_agg(ctx, n, cols, ncols, preds, npreds, key_cols, nkeys, aggs, naggs, max_groups,
+                             out_ngroups, first.data(), out_vals, out_nulls);
+  if (st != KG_OK || *out_ngroups == 0) return st;
+  const int64_t G = *out_ngroups;
+  int32_t* d = nullptr;
+  if (hipMalloc(&d, static_cast<size_t>(G) * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    return kg_fail(ctx, KG_ERR_OOM, "device allocation of %lld bytes failed", (long long)G * 8);
+  }
+  std::vector<int32_t> codes(G);
+  hipError_t e = hipMemcpyAsync(d, first.data(), G * 4, hipMemcpyHostToDevice, ctx->stream);
+  for (int k = 0; k < nkeys && e == hipSuccess; k++) {
+    st = kg_gather(ctx, &cols[key_cols[k]], d, G, d + G, nullptr);
+    if (st != KG_OK) break;
+    e = hipMemcpyAsync(codes.data(), d + G, G * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    for (int64_t g = 0; g < G && e == hipSuccess; g++) out_keys[g * nkeys + k] = codes[g];
+  }
+  (void)hipFree(d);
+  if (st != KG_OK) return st;
+  if (e != hipSuccess) return kg_fail(ctx, KG_ERR_HIP, "group key read-back failed: %s", hipGetErrorString(e));
+  return KG_OK;
+}
+
+}  // namespace
+
 extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
                                  const kg_pred* preds, int32_t npreds, const int32_t* key_cols,
                                  const int32_t* key_card, int32_t nkeys, const kg_agg* aggs, int32_t naggs,
@@ -829,9 +864,10 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
     G *= key_card[k];
   }
   if (nkeys == 2) K.card1 = key_card[1];
-  if (G > kMaxLdsGroups)
-    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "%lld dense groups exceed the on-chip limit %d", (long long)G,
-                   kMaxLdsGroups);
+  // A dense table that cannot stay on chip (more than kMaxLdsGroups groups, or
+  // more LDS than one workgroup gets) runs as the hashed GROUP BY over the same
+  // key columns: the same groups, aggregates and insertion order.
+  bool hashed = G > kMaxLdsGroups;
 
   MeasureSet M;
   std::vector<AggPlan> plan;
@@ -840,6 +876,12 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
 
   *out_ngroups = 0;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if (!hashed && G > (minmax ? 4 : 8)) {
+    const size_t lds_bytes = (G * M.nm * 2 + G * 2 + (minmax ? 2 * G * M.nm : 0)) * 8;
+    hashed = lds_bytes > 64 * 1024;
+  }
+  if (hashed) return dense_as_hashed(ctx, n, cols, ncols, preds, npreds, key_cols, nkeys, aggs, naggs, max_groups,
+                                     out_ngroups, out_keys, out_vals, out_nulls);
   const int nm = M.nm;
   std::vector<AggFinal> fin(G * (nm > 0 ? nm : 1));
   std::vector<uint64_t> rowsv(G);

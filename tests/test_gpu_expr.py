@@ -317,6 +317,11 @@ def test_group_by_enum_fixture(ctx):
                                         [(abi.KG_AGG_COUNT, -1), (abi.KG_AGG_COUNT, 1)], max_groups=card + 1)
     assert np.array_equal(codes, codes2)
     vals, nulls = np.concatenate([v1, v2], axis=1), np.concatenate([n1, n2], axis=1)
+    # One call with all four: the dense table leaves LDS and runs as the hashed GROUP BY.
+    c4, v4, n4 = ctx.scan_group_agg([kc, rank_col, col], [], [(0, card + 1)],
+                                    [(abi.KG_AGG_MIN, 1), (abi.KG_AGG_MAX, 1), (abi.KG_AGG_COUNT, -1),
+                                     (abi.KG_AGG_COUNT, 2)], max_groups=card + 1)
+    assert np.array_equal(c4, codes) and np.array_equal(v4, vals) and np.array_equal(n4, nulls)
     by_rank = [None] * card
     for c, r in enumerate(ranks):
         by_rank[r] = strings[c]

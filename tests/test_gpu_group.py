@@ -113,3 +113,23 @@ def test_hash_group_capacity_and_empty(ctx):
         ctx.hash_group_agg([k], [], [0], [(abi.KG_AGG_COUNT, -1)], max_groups=10)
     first, vals, _ = ctx.hash_group_agg([k], [(0, abi.KG_LT, -5)], [0], [(abi.KG_AGG_COUNT, -1)], max_groups=10)
     assert len(first) == 0
+
+
+def test_dense_keys_beyond_on_chip_table(ctx):
+    """kg_scan_group_agg over 3000 dense codes x 2 keys (6000 groups, more than
+    an LDS table holds) takes the hashed path: the same groups in insertion
+    order, codes and values as the oracle."""
+    rng = np.random.default_rng(9)
+    n = 500_000
+    a = rng.integers(0, 3000, n).astype(np.int32)
+    b = rng.integers(0, 2, n).astype(np.int32)
+    v = rng.integers(-1000, 1000, n).astype(np.int32)
+    cols = [Column(abi.KG_INT, torch.from_numpy(x).to(DEV)) for x in (a, b, v)]
+    codes, vals, nulls = ctx.scan_group_agg(cols, [(2, abi.KG_GT, -900)], [(0, 3000), (1, 2)],
+                                            [(abi.KG_AGG_SUM, 2), (abi.KG_AGG_MIN, 2), (abi.KG_AGG_COUNT, -1)],
+                                            max_groups=6000)
+    rows = po.scan_select([po.Col(po.INT, v)], [(0, po.GT, -900)])
+    kv, _, ov, onl = po.group_by([po.Col(po.INT, a), po.Col(po.INT, b)],
+                                 [(po.SUM, po.Col(po.INT, v)), (po.MIN, po.Col(po.INT, v)), (po.COUNT, None)],
+                                 rows=rows, max_groups=6001)
+    assert np.array_equal(codes.astype(np.int64), kv) and np.array_equal(vals, ov) and np.array_equal(nulls, onl)
