@@ -26,7 +26,8 @@ DATA = (
        "o_custkey.kdb", "o_orderdate.kdb", "o_shippriority.kdb", "l_orderkey.kdb", "l_shipmode.kdb",
        "l_commitdate.kdb", "l_receiptdate.kdb", "o_orderpriority.kdb", "r_regionkey.kdb", "n_nationkey.kdb",
        "n_regionkey.kdb", "n_name.kdb", "c_nationkey.kdb", "l_suppkey.kdb", "s_suppkey.kdb", "s_nationkey.kdb",
-       "l_partkey.kdb", "p_partkey.kdb", "p_type.kdb"]
+       "l_partkey.kdb", "p_partkey.kdb", "p_type.kdb", "p_size.kdb", "p_brand.kdb", "p_container.kdb",
+       "l_shipinstruct.kdb"]
 )
 
 EXPECTED = (
@@ -40,7 +41,7 @@ EXPECTED = (
     + [f"order_by/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text")]
     + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
     + ["tpch/q06_expected.tbl", "tpch/q03_expected.tbl", "tpch/q12_expected.tbl", "tpch/q05_expected.tbl",
-       "tpch/q14_expected.tbl", "cross_product/cross_product_expected.tbl"]
+       "tpch/q14_expected.tbl", "tpch/q19_expected.tbl", "cross_product/cross_product_expected.tbl"]
     + [f"skinner_join/{t}_expected.tbl" for t in ("int", "smallint", "date", "bigint", "real", "boolean",
                                                    "enum", "text", "multiple_tables_join")]
 )

@@ -1,4 +1,4 @@
-"""TPC-H Q3, Q5, Q12 and Q14 end to end on the device operators. Q3 as the reference's
+"""TPC-H Q3, Q5, Q12, Q14 and Q19 end to end on the device operators. Q3 as the reference's
 q03_test.cc plans it: TEXT select (kg_text_select), scan-selects (kg_scan_select),
 two hash joins (kg_ht_build / kg_ht_probe over gathered keys), the derived
 revenue (kg_expr_eval), GROUP BY over three hashed keys (kg_hash_group_agg)
@@ -224,4 +224,56 @@ def test_tpch_q14():
     got = 100.0 * a1 / a2
     assert abs(got - float(kdb.expected("tpch/q14_expected.tbl")[0])) < 1e-5
     assert abs(got - tpch_q14.oracle_q14(c)) <= 1e-12 * abs(got)
+    ctx.close()
+
+
+def test_tpch_q19():
+    """Q19 on the device: the part and lineitem filters as expressions over
+    TEXT / INT / REAL columns (kg_expr_select), the join, the post-join
+    select pairing each part conjunction (a class code from the conjunctions'
+    flag columns at the joined part rows) with its quantity range, and the ungrouped SUM whose
+    empty input gives the NULL row."""
+    import tpch_q19
+    from kushdb_amd.expr import And, Case, Eq, Geq, Leq, Or, expr_select
+    c = tpch_q19.columns()
+    ctx = Context(0)
+    T, I, R = abi.KG_TEXT, abi.KG_INT, abi.KG_REAL
+    brand, cont = (TextColumn.from_strings(c[k], device=DEV) for k in ("p_brand", "p_container"))
+    size = dev(c["p_size"], I)
+
+    def part_conj(j):
+        b, ct, hi = tpch_q19.PART[j]
+        return And([Geq(ColRef(2, I), Literal(1)), Leq(ColRef(2, I), Literal(hi)), Eq(ColRef(0, T), Literal(b)),
+                    Or([Eq(ColRef(1, T), Literal(x)) for x in ct])])
+    # One flag column per conjunction (an expression takes at most 64 nodes),
+    # then their OR.
+    np_ = len(c["p_partkey"])
+    flags = [Column(I, expr_eval(ctx, np_, [brand, cont, size], Case(part_conj(j), Literal(1), Literal(0))).data)
+             for j in range(3)]
+    prows = expr_select(ctx, np_, flags, Or([Eq(ColRef(j, I), Literal(1)) for j in range(3)]))
+    instr, mode = (TextColumn.from_strings(c[k], device=DEV) for k in ("l_shipinstruct", "l_shipmode"))
+    qty = dev(c["l_quantity"], R)
+    lflt = And([Eq(ColRef(0, T), Literal("DELIVER IN PERSON")),
+                Or([Eq(ColRef(1, T), Literal("AIR")), Eq(ColRef(1, T), Literal("AIR REG"))]),
+                Or([And([Geq(ColRef(2, R), Literal(lo)), Leq(ColRef(2, R), Literal(hi))]) for lo, hi in tpch_q19.QTY])])
+    lrows = expr_select(ctx, len(c["l_partkey"]), [instr, mode, qty], lflt)
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["p_partkey"], I), prows), gather(ctx, dev(c["l_partkey"], I), lrows))
+    jp, jl = prows[br].contiguous(), lrows[pr].contiguous()
+    # class of the joined part row: 1, 2 or 3 for the conjunction it meets (the
+    # brands make them exclusive)
+    from kushdb_amd.expr import Add
+    cls = expr_eval(ctx, np_, flags, Add(Add(ColRef(0, I), Mul(ColRef(1, I), Literal(2))),
+                                         Mul(ColRef(2, I), Literal(3))), rows=jp)
+    q = gather(ctx, qty, jl)
+    keep = expr_select(ctx, len(jl), [Column(I, cls.data), q],
+                       Or([And([Eq(ColRef(0, I), Literal(j + 1)), Geq(ColRef(1, R), Literal(lo)),
+                                Leq(ColRef(1, R), Literal(hi))]) for j, (lo, hi) in enumerate(tpch_q19.QTY)]))
+    kl = jl[keep.long()].contiguous()
+    ext, disc = gather(ctx, dev(c["l_extendedprice"], R), kl), gather(ctx, dev(c["l_discount"], R), kl)
+    rev = expr_eval(ctx, len(kl), [ext, disc], Mul(ColRef(0, R), Sub(Literal(1.0), ColRef(1, R))))
+    codes, vals, nulls = ctx.scan_group_agg([Column(R, rev.data)], [], [], [(abi.KG_AGG_SUM, 0)])
+    got = None if (len(codes) == 0 or nulls[0, 0]) else float(vals[0, 0:1].view(np.float64)[0])
+    assert tpch_q19.line(got) == kdb.expected("tpch/q19_expected.tbl")[0]
+    assert got == tpch_q19.oracle_q19(c)
+    assert len(pr) > 0   # the joins produce pairs; only the post-join select empties them
     ctx.close()

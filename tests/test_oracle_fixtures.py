@@ -56,3 +56,8 @@ def test_tpch_q05():
 def test_tpch_q14():
     import tpch_q14
     assert abs(tpch_q14.oracle_q14(tpch_q14.columns()) - float(kdb.expected("tpch/q14_expected.tbl")[0])) < 1e-5
+
+
+def test_tpch_q19():
+    import tpch_q19
+    assert tpch_q19.line(tpch_q19.oracle_q19(tpch_q19.columns())) == kdb.expected("tpch/q19_expected.tbl")[0]
