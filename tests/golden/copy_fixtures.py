@@ -41,7 +41,8 @@ EXPECTED = (
     + [f"order_by/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text")]
     + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
     + ["tpch/q06_expected.tbl", "tpch/q03_expected.tbl", "tpch/q12_expected.tbl", "tpch/q05_expected.tbl",
-       "tpch/q14_expected.tbl", "tpch/q19_expected.tbl", "cross_product/cross_product_expected.tbl"]
+       "tpch/q14_expected.tbl", "tpch/q19_expected.tbl", "tpch/q18_expected.tbl",
+       "cross_product/cross_product_expected.tbl"]
     + [f"skinner_join/{t}_expected.tbl" for t in ("int", "smallint", "date", "bigint", "real", "boolean",
                                                    "enum", "text", "multiple_tables_join")]
 )

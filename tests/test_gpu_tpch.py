@@ -1,4 +1,4 @@
-"""TPC-H Q3, Q5, Q12, Q14 and Q19 end to end on the device operators. Q3 as the reference's
+"""TPC-H Q3, Q5, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
 q03_test.cc plans it: TEXT select (kg_text_select), scan-selects (kg_scan_select),
 two hash joins (kg_ht_build / kg_ht_probe over gathered keys), the derived
 revenue (kg_expr_eval), GROUP BY over three hashed keys (kg_hash_group_agg)
@@ -276,4 +276,29 @@ def test_tpch_q19():
     assert tpch_q19.line(got) == kdb.expected("tpch/q19_expected.tbl")[0]
     assert got == tpch_q19.oracle_q19(c)
     assert len(pr) > 0   # the joins produce pairs; only the post-join select empties them
+    ctx.close()
+
+
+def test_tpch_q18():
+    """Q18's subquery on the device: GROUP BY l_orderkey SUM(l_quantity) over
+    all of lineitem (kg_hash_group_agg, thousands of groups) equal to the
+    oracle group for group, then the HAVING select (kg_scan_select on the
+    sums), whose empty result empties the joins downstream (a build on no rows,
+    probed by all orders) and the output."""
+    import tpch_q18
+    c = tpch_q18.columns()
+    ctx = Context(0)
+    ok_col, qty = dev(c["l_orderkey"], abi.KG_INT), dev(c["l_quantity"], abi.KG_REAL)
+    n = len(c["l_orderkey"])
+    first, vals, _ = ctx.hash_group_agg([ok_col, qty], [], [0], [(abi.KG_AGG_SUM, 1)], max_groups=n + 1)
+    keys, sums = tpch_q18.oracle_subquery(c)
+    assert np.array_equal(c["l_orderkey"][first], keys)
+    got = vals[:, 0].view(np.float64)
+    assert np.all(np.abs(got - sums) <= 1e-12 * np.abs(sums))
+    s = Column(abi.KG_REAL, torch.from_numpy(got.copy()).to(DEV))
+    having = ctx.scan_select([s], [(0, abi.KG_GT, tpch_q18.BAR)])
+    hk = gather(ctx, Column(abi.KG_INT, torch.from_numpy(keys.astype(np.int32)).to(DEV)), having)
+    orders = dev(kdb.col("o_orderkey", "int")[0], abi.KG_INT)
+    pr, _ = _dev_join(ctx, hk, orders)
+    assert len(having) == 0 and len(pr) == 0 and kdb.expected("tpch/q18_expected.tbl") == []
     ctx.close()

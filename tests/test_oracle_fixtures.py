@@ -61,3 +61,8 @@ def test_tpch_q14():
 def test_tpch_q19():
     import tpch_q19
     assert tpch_q19.line(tpch_q19.oracle_q19(tpch_q19.columns())) == kdb.expected("tpch/q19_expected.tbl")[0]
+
+
+def test_tpch_q18():
+    import tpch_q18
+    assert tpch_q18.oracle_q18(tpch_q18.columns()) == kdb.expected("tpch/q18_expected.tbl") == []
