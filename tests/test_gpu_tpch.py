@@ -1,4 +1,4 @@
-"""TPC-H Q3, Q5, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
+"""TPC-H Q3, Q5, Q10, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
 q03_test.cc plans it: TEXT select (kg_text_select), scan-selects (kg_scan_select),
 two hash joins (kg_ht_build / kg_ht_probe over gathered keys), the derived
 revenue (kg_expr_eval), GROUP BY over three hashed keys (kg_hash_group_agg)
@@ -301,4 +301,43 @@ def test_tpch_q18():
     orders = dev(kdb.col("o_orderkey", "int")[0], abi.KG_INT)
     pr, _ = _dev_join(ctx, hk, orders)
     assert len(having) == 0 and len(pr) == 0 and kdb.expected("tpch/q18_expected.tbl") == []
+    ctx.close()
+
+
+def test_tpch_q10():
+    """Q10 on the device: the orders date range, three hash joins, the TEXT
+    l_returnflag select (kg_text_select), revenue, GROUP BY the seven customer
+    and nation columns and ORDER BY revenue DESC. Every grouping column is an
+    attribute of the joined customer row (c_custkey is unique, checked here)
+    or of its nation, so the device groups by (c_custkey, c_acctbal,
+    n_nationkey) (kg_hash_group_agg): the same groups as the seven values."""
+    import tpch_q10
+    c = tpch_q10.columns()
+    assert len(np.unique(c["c_custkey"])) == len(c["c_custkey"])
+    ctx = Context(0)
+    I, R = abi.KG_INT, abi.KG_REAL
+    ords = ctx.scan_select([dev(c["o_orderdate"], abi.KG_DATE)],
+                           [(0, abi.KG_GEQ, po.build_date(*tpch_q10.LO)), (0, abi.KG_LT, po.build_date(*tpch_q10.HI))])
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["o_custkey"], I), ords), dev(c["c_custkey"], I))
+    crow, orow = pr.int(), ords[br]
+    pr, br = _dev_join(ctx, dev(c["n_nationkey"], I), gather(ctx, dev(c["c_nationkey"], I), crow))
+    crow, orow, nrow = crow[pr].contiguous(), orow[pr].contiguous(), br.int().contiguous()
+    li = text_select(ctx, TextColumn.from_strings(c["l_returnflag"], device=DEV), abi.KG_TEXT_EQ, "R")
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["o_orderkey"], I), orow), gather(ctx, dev(c["l_orderkey"], I), li))
+    lrow, crow, nrow = li[pr].contiguous(), crow[br].contiguous(), nrow[br].contiguous()
+    ext, disc = gather(ctx, dev(c["l_extendedprice"], R), lrow), gather(ctx, dev(c["l_discount"], R), lrow)
+    rev = expr_eval(ctx, len(lrow), [ext, disc], Mul(ColRef(0, R), Sub(Literal(1.0), ColRef(1, R))))
+    keys = [gather(ctx, dev(c["c_custkey"], I), crow), gather(ctx, dev(c["c_acctbal"], R), crow),
+            gather(ctx, dev(c["n_nationkey"], I), nrow)]
+    first, vals, _ = ctx.hash_group_agg(keys + [Column(R, rev.data)], [], [0, 1, 2], [(abi.KG_AGG_SUM, 3)],
+                                        max_groups=len(lrow) + 1)
+    r = vals[:, 0].view(np.float64).copy()
+    perm = order_by(ctx, [Column(R, torch.from_numpy(r).to(DEV))], [False]).cpu().numpy()
+    fc, fn = crow.cpu().numpy()[first], nrow.cpu().numpy()[first]
+    got = [(int(c["c_custkey"][fc[i]]), c["c_name"][fc[i]], float(r[i]), float(c["c_acctbal"][fc[i]]),
+            c["n_name"][fn[i]], c["c_address"][fc[i]], c["c_phone"][fc[i]], c["c_comment"][fc[i]]) for i in perm]
+    tpch_q10.check(got, kdb.expected("tpch/q10_expected.tbl"))
+    want = tpch_q10.oracle_q10(c)
+    assert [g[:2] + g[3:] for g in got] == [w[:2] + w[3:] for w in want]
+    assert np.allclose([g[2] for g in got], [w[2] for w in want], rtol=1e-12, atol=0)
     ctx.close()

@@ -66,3 +66,8 @@ def test_tpch_q19():
 def test_tpch_q18():
     import tpch_q18
     assert tpch_q18.oracle_q18(tpch_q18.columns()) == kdb.expected("tpch/q18_expected.tbl") == []
+
+
+def test_tpch_q10():
+    import tpch_q10
+    tpch_q10.check(tpch_q10.oracle_q10(tpch_q10.columns()), kdb.expected("tpch/q10_expected.tbl"))
