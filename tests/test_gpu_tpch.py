@@ -1,4 +1,4 @@
-"""TPC-H Q3, Q5, Q10, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
+"""TPC-H Q3, Q5, Q10, Q11, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
 q03_test.cc plans it: TEXT select (kg_text_select), scan-selects (kg_scan_select),
 two hash joins (kg_ht_build / kg_ht_probe over gathered keys), the derived
 revenue (kg_expr_eval), GROUP BY over three hashed keys (kg_hash_group_agg)
@@ -340,4 +340,38 @@ def test_tpch_q10():
     want = tpch_q10.oracle_q10(c)
     assert [g[:2] + g[3:] for g in got] == [w[:2] + w[3:] for w in want]
     assert np.allclose([g[2] for g in got], [w[2] for w in want], rtol=1e-12, atol=0)
+    ctx.close()
+
+
+def test_tpch_q11():
+    """Q11 on the device: TEXT select on nation, two joins, the subquery's
+    SUM(ps_supplycost * Float(ps_availqty)) as an ungrouped product aggregate
+    (kg_scan_group_agg with a second factor), the same product grouped by
+    ps_partkey (kg_hash_group_agg), the cross-product select value > value1
+    (kg_scan_select against the subquery's value), ORDER BY value DESC."""
+    import tpch_q11
+    c = tpch_q11.columns()
+    ctx = Context(0)
+    I, R = abi.KG_INT, abi.KG_REAL
+    nat = text_select(ctx, TextColumn.from_strings(c["n_name"], device=DEV), abi.KG_TEXT_EQ, "GERMANY")
+    pr, _ = _dev_join(ctx, gather(ctx, dev(c["n_nationkey"], I), nat), dev(c["s_nationkey"], I))
+    sup = pr.int().contiguous()
+    pr, _ = _dev_join(ctx, gather(ctx, dev(c["s_suppkey"], I), sup), dev(c["ps_suppkey"], I))
+    ps = pr.int().contiguous()
+    cost, qty, part = (gather(ctx, dev(c[k], t), ps) for k, t in
+                       (("ps_supplycost", R), ("ps_availqty", I), ("ps_partkey", I)))
+    _, tot, _ = ctx.scan_group_agg([cost, qty], [], [], [(abi.KG_AGG_SUM, 0, 1)])
+    bar = float(tot[0, 0:1].view(np.float64)[0]) * 0.0001
+    first, vals, _ = ctx.hash_group_agg([part, cost, qty], [], [0], [(abi.KG_AGG_SUM, 1, 2)],
+                                        max_groups=len(ps) + 1)
+    value = Column(R, torch.from_numpy(vals[:, 0].view(np.float64).copy()).to(DEV))
+    keep = ctx.scan_select([value], [(0, abi.KG_GT, bar)]).long()
+    kv = value.data[keep].contiguous()
+    perm = order_by(ctx, [Column(R, kv)], [False]).cpu().numpy()
+    keys = c["ps_partkey"][ps.cpu().numpy()[first]][keep.cpu().numpy()]
+    got = [(int(keys[i]), float(kv[i])) for i in perm]
+    tpch_q11.check(got, kdb.expected("tpch/q11_expected.tbl"))
+    want = tpch_q11.oracle_q11(c)
+    assert [k for k, _ in got] == [k for k, _ in want]
+    assert np.allclose([v for _, v in got], [v for _, v in want], rtol=1e-12, atol=0)
     ctx.close()

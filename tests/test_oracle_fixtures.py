@@ -71,3 +71,8 @@ def test_tpch_q18():
 def test_tpch_q10():
     import tpch_q10
     tpch_q10.check(tpch_q10.oracle_q10(tpch_q10.columns()), kdb.expected("tpch/q10_expected.tbl"))
+
+
+def test_tpch_q11():
+    import tpch_q11
+    tpch_q11.check(tpch_q11.oracle_q11(tpch_q11.columns()), kdb.expected("tpch/q11_expected.tbl"))
