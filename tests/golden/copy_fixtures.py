@@ -43,7 +43,7 @@ EXPECTED = (
     + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
     + ["tpch/q06_expected.tbl", "tpch/q03_expected.tbl", "tpch/q12_expected.tbl", "tpch/q05_expected.tbl",
        "tpch/q14_expected.tbl", "tpch/q19_expected.tbl", "tpch/q18_expected.tbl",
-       "tpch/q10_expected.tbl", "tpch/q11_expected.tbl",
+       "tpch/q10_expected.tbl", "tpch/q11_expected.tbl", "tpch/q07_expected.tbl",
        "cross_product/cross_product_expected.tbl"]
     + [f"skinner_join/{t}_expected.tbl" for t in ("int", "smallint", "date", "bigint", "real", "boolean",
                                                    "enum", "text", "multiple_tables_join")]

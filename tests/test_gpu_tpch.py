@@ -1,4 +1,4 @@
-"""TPC-H Q3, Q5, Q10, Q11, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
+"""TPC-H Q3, Q5, Q7, Q10, Q11, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
 q03_test.cc plans it: TEXT select (kg_text_select), scan-selects (kg_scan_select),
 two hash joins (kg_ht_build / kg_ht_probe over gathered keys), the derived
 revenue (kg_expr_eval), GROUP BY over three hashed keys (kg_hash_group_agg)
@@ -374,4 +374,62 @@ def test_tpch_q11():
     want = tpch_q11.oracle_q11(c)
     assert [k for k, _ in got] == [k for k, _ in want]
     assert np.allclose([v for _, v in got], [v for _, v in want], rtol=1e-12, atol=0)
+    ctx.close()
+
+
+def test_tpch_q07():
+    """Q7 on the device: the nation select (TEXT), five joins, the post-join
+    select over the two nations through a per-nation code (CASE on n_name,
+    EGYPT 1 / KENYA 2, gathered at both nation rows), EXTRACT(YEAR) and
+    revenue as expressions, GROUP BY (code1, code2, year) over hashed keys,
+    ORDER BY the three (the codes order as the names do)."""
+    import tpch_q07
+    from kushdb_amd.expr import And, Case, Eq, ExtractYear, Or, expr_select
+    c = tpch_q07.columns()
+    ctx = Context(0)
+    I, R, D, T = abi.KG_INT, abi.KG_REAL, abi.KG_DATE, abi.KG_TEXT
+    names = TextColumn.from_strings(c["n_name"], device=DEV)
+    code = expr_eval(ctx, len(c["n_name"]), [names],
+                     Case(Eq(ColRef(0, T), Literal("EGYPT")), Literal(1),
+                          Case(Eq(ColRef(0, T), Literal("KENYA")), Literal(2), Literal(0))))
+    codecol = Column(I, code.data)
+    nat = ctx.scan_select([codecol], [(0, abi.KG_GT, 0)])
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["n_nationkey"], I), nat), dev(c["s_nationkey"], I))
+    srow, snat = pr.int(), nat[br]
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["n_nationkey"], I), nat), dev(c["c_nationkey"], I))
+    crow, cnat = pr.int(), nat[br]
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["c_custkey"], I), crow), dev(c["o_custkey"], I))
+    orow, onat = pr.int(), cnat[br]
+    ship = dev(c["l_shipdate"], D)
+    li = ctx.scan_select([ship], [(0, abi.KG_GEQ, po.build_date(*tpch_q07.LO)),
+                                  (0, abi.KG_LEQ, po.build_date(*tpch_q07.HI))])
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["o_orderkey"], I), orow), gather(ctx, dev(c["l_orderkey"], I), li))
+    lrow, lnat = li[pr], onat[br]
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["s_suppkey"], I), srow), gather(ctx, dev(c["l_suppkey"], I),
+                                                                               lrow.contiguous()))
+    lrow, n2, n1 = lrow[pr].contiguous(), lnat[pr].contiguous(), snat[br].contiguous()
+    c1, c2 = gather(ctx, codecol, n1), gather(ctx, codecol, n2)
+    keep = expr_select(ctx, len(lrow), [c1, c2],
+                       Or([And([Eq(ColRef(0, I), Literal(1)), Eq(ColRef(1, I), Literal(2))]),
+                           And([Eq(ColRef(0, I), Literal(2)), Eq(ColRef(1, I), Literal(1))])])).long()
+    lrow = lrow[keep].contiguous()
+    c1, c2 = Column(I, c1.data[keep].contiguous()), Column(I, c2.data[keep].contiguous())
+    sd, ext, disc = (gather(ctx, dev(c[k], t), lrow) for k, t in
+                     (("l_shipdate", D), ("l_extendedprice", R), ("l_discount", R)))
+    year = expr_eval(ctx, len(lrow), [sd], ExtractYear(ColRef(0, D)))
+    rev = expr_eval(ctx, len(lrow), [ext, disc], Mul(ColRef(0, R), Sub(Literal(1.0), ColRef(1, R))))
+    yc = Column(I, year.data)
+    first, vals, _ = ctx.hash_group_agg([c1, c2, yc, Column(R, rev.data)], [], [0, 1, 2], [(abi.KG_AGG_SUM, 3)],
+                                        max_groups=64)
+    fr = torch.from_numpy(first).to(DEV)
+    g1, g2, g3 = gather(ctx, c1, fr), gather(ctx, c2, fr), gather(ctx, yc, fr)
+    perm = order_by(ctx, [g1, g2, g3], [True, True, True]).cpu().numpy()
+    a1, a2, a3 = g1.data.cpu().numpy(), g2.data.cpu().numpy(), g3.data.cpu().numpy()
+    r = vals[:, 0].view(np.float64)
+    nm = {1: "EGYPT", 2: "KENYA"}
+    got = [(nm[int(a1[i])], nm[int(a2[i])], int(a3[i]), float(r[i])) for i in perm]
+    tpch_q07.check(got, kdb.expected("tpch/q07_expected.tbl"))
+    want = tpch_q07.oracle_q07(c)
+    assert [g[:3] for g in got] == [w[:3] for w in want]
+    assert np.allclose([g[3] for g in got], [w[3] for w in want], rtol=1e-12, atol=0)
     ctx.close()

@@ -76,3 +76,8 @@ def test_tpch_q10():
 def test_tpch_q11():
     import tpch_q11
     tpch_q11.check(tpch_q11.oracle_q11(tpch_q11.columns()), kdb.expected("tpch/q11_expected.tbl"))
+
+
+def test_tpch_q07():
+    import tpch_q07
+    tpch_q07.check(tpch_q07.oracle_q07(tpch_q07.columns()), kdb.expected("tpch/q07_expected.tbl"))
