@@ -28,7 +28,8 @@ DATA = (
        "n_regionkey.kdb", "n_name.kdb", "c_nationkey.kdb", "l_suppkey.kdb", "s_suppkey.kdb", "s_nationkey.kdb",
        "l_partkey.kdb", "p_partkey.kdb", "p_type.kdb", "p_size.kdb", "p_brand.kdb", "p_container.kdb",
        "l_shipinstruct.kdb", "c_acctbal.kdb", "c_name.kdb", "c_phone.kdb", "c_address.kdb", "c_comment.kdb",
-       "l_returnflag.kdb", "ps_partkey.kdb", "ps_suppkey.kdb", "ps_supplycost.kdb", "ps_availqty.kdb"]
+       "l_returnflag.kdb", "ps_partkey.kdb", "ps_suppkey.kdb", "ps_supplycost.kdb", "ps_availqty.kdb",
+       "p_name.kdb", "l_quantity.kdb"]
 )
 
 EXPECTED = (
@@ -43,7 +44,7 @@ EXPECTED = (
     + [f"select/{t}_expected.tbl" for t in ("int", "bigint", "smallint", "date", "real", "enum", "text", "boolean")]
     + ["tpch/q06_expected.tbl", "tpch/q03_expected.tbl", "tpch/q12_expected.tbl", "tpch/q05_expected.tbl",
        "tpch/q14_expected.tbl", "tpch/q19_expected.tbl", "tpch/q18_expected.tbl",
-       "tpch/q10_expected.tbl", "tpch/q11_expected.tbl", "tpch/q07_expected.tbl",
+       "tpch/q10_expected.tbl", "tpch/q11_expected.tbl", "tpch/q07_expected.tbl", "tpch/q09_expected.tbl",
        "cross_product/cross_product_expected.tbl"]
     + [f"skinner_join/{t}_expected.tbl" for t in ("int", "smallint", "date", "bigint", "real", "boolean",
                                                    "enum", "text", "multiple_tables_join")]

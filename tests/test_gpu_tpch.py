@@ -1,4 +1,4 @@
-"""TPC-H Q3, Q5, Q7, Q10, Q11, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
+"""TPC-H Q3, Q5, Q7, Q9, Q10, Q11, Q12, Q14, Q18 and Q19 end to end on the device operators. Q3 as the reference's
 q03_test.cc plans it: TEXT select (kg_text_select), scan-selects (kg_scan_select),
 two hash joins (kg_ht_build / kg_ht_probe over gathered keys), the derived
 revenue (kg_expr_eval), GROUP BY over three hashed keys (kg_hash_group_agg)
@@ -432,4 +432,51 @@ def test_tpch_q07():
     want = tpch_q07.oracle_q07(c)
     assert [g[:3] for g in got] == [w[:3] for w in want]
     assert np.allclose([g[3] for g in got], [w[3] for w in want], rtol=1e-12, atol=0)
+    ctx.close()
+
+
+def test_tpch_q09():
+    """Q9 on the device: CONTAINS on p_name (kg_text_select), five joins (the
+    two-key lineitem join as l_partkey plus a checked l_suppkey), EXTRACT(YEAR)
+    and the profit expression, GROUP BY (nation row, year) over hashed keys
+    (nation names are unique per nation row), ORDER BY nation (TEXT), year DESC."""
+    import tpch_q09
+    from kushdb_amd.expr import Eq, ExtractYear, expr_select
+    c = tpch_q09.columns()
+    ctx = Context(0)
+    I, R, D = abi.KG_INT, abi.KG_REAL, abi.KG_DATE
+    assert len(set(c["n_name"])) == len(c["n_name"])
+    pr, br = _dev_join(ctx, dev(c["n_nationkey"], I), dev(c["s_nationkey"], I))
+    srow, snat = pr.int(), br.int()
+    part = text_select(ctx, TextColumn.from_strings(c["p_name"], device=DEV), abi.KG_TEXT_CONTAINS, "moccasin")
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["p_partkey"], I), part), dev(c["ps_partkey"], I))
+    psrow, prow = pr.int(), part[br]
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["s_suppkey"], I), srow), gather(ctx, dev(c["ps_suppkey"], I), psrow))
+    psrow, prow, srow2, nrow = psrow[pr], prow[pr].contiguous(), srow[br], snat[br]
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["p_partkey"], I), prow), dev(c["l_partkey"], I))
+    ls = gather(ctx, dev(c["l_suppkey"], I), pr.int().contiguous())
+    ss = gather(ctx, dev(c["s_suppkey"], I), srow2[br].contiguous())
+    keep = expr_select(ctx, len(ls), [ls, ss], Eq(ColRef(0, I), ColRef(1, I))).long()
+    lrow, psrow, nrow = pr.int()[keep].contiguous(), psrow[br][keep].contiguous(), nrow[br][keep].contiguous()
+    pr, br = _dev_join(ctx, gather(ctx, dev(c["l_orderkey"], I), lrow), dev(c["o_orderkey"], I))
+    orow, lrow, psrow, nrow = pr.int(), lrow[br].contiguous(), psrow[br].contiguous(), nrow[br].contiguous()
+    ext, disc, qty = (gather(ctx, dev(c[k], R), lrow) for k in ("l_extendedprice", "l_discount", "l_quantity"))
+    cost = gather(ctx, dev(c["ps_supplycost"], R), psrow)
+    amount = expr_eval(ctx, len(lrow), [ext, disc, cost, qty],
+                       Sub(Mul(ColRef(0, R), Sub(Literal(1.0), ColRef(1, R))), Mul(ColRef(2, R), ColRef(3, R))))
+    year = expr_eval(ctx, len(orow), [gather(ctx, dev(c["o_orderdate"], D), orow)], ExtractYear(ColRef(0, D)))
+    nk, yk = Column(I, nrow), Column(I, year.data)
+    first, vals, _ = ctx.hash_group_agg([nk, yk, Column(R, amount.data)], [], [0, 1], [(abi.KG_AGG_SUM, 2)],
+                                        max_groups=1024)
+    fr = torch.from_numpy(first).to(DEV)
+    gn, gy = gather(ctx, nk, fr).data.cpu().numpy(), gather(ctx, yk, fr).data.cpu().numpy()
+    gnames = [c["n_name"][int(n)] for n in gn]
+    perm = order_by(ctx, [TextColumn.from_strings(gnames, device=DEV), Column(I, torch.from_numpy(gy).to(DEV))],
+                    [True, False]).cpu().numpy()
+    r = vals[:, 0].view(np.float64)
+    got = [(gnames[i], int(gy[i]), float(r[i])) for i in perm]
+    tpch_q09.check(got, kdb.expected("tpch/q09_expected.tbl"))
+    want = tpch_q09.oracle_q09(c)
+    assert [g[:2] for g in got] == [w[:2] for w in want]
+    assert np.allclose([g[2] for g in got], [w[2] for w in want], rtol=1e-12, atol=1e-9)
     ctx.close()

@@ -81,3 +81,8 @@ def test_tpch_q11():
 def test_tpch_q07():
     import tpch_q07
     tpch_q07.check(tpch_q07.oracle_q07(tpch_q07.columns()), kdb.expected("tpch/q07_expected.tbl"))
+
+
+def test_tpch_q09():
+    import tpch_q09
+    tpch_q09.check(tpch_q09.oracle_q09(tpch_q09.columns()), kdb.expected("tpch/q09_expected.tbl"))
