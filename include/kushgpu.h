@@ -124,8 +124,19 @@ const char* kg_last_error(const kg_ctx* ctx);
  * stream). A context starts with a stream of its own. */
 int kg_ctx_set_stream(kg_ctx* ctx, void* hip_stream);
 int kg_ctx_synchronize(kg_ctx* ctx);
-/* 1 (default): FP64 reductions use a fixed order that depends only on n (and
- * is reproduced by the CPU oracle); 0: order may depend on timing. */
+/* FP64 SUM/AVG addition order (compile/proxy/aggregator.cc:67-71,239-268
+ * fold sequentially in arrival order; no switch there).
+ * 0 (default): dense GROUP BYs with at most 8 groups (4 with MIN/MAX) add in a
+ * fixed tree order that depends only on n; the LDS table (more groups), the
+ * star pipeline (kg_scan_join_group_agg), hashed GROUP BY (kg_hash_group_agg)
+ * and batches add in a timing-dependent order (within FP64 rounding of the
+ * fixed order).
+ * 1: those paths take their REAL sums in a fixed order that depends only on
+ * the input: each group's rows (join output rows for the star pipeline) in
+ * input order, summed in chunks of 256 by a fixed lane / pairwise tree,
+ * chunk sums summed the same way (reproduced bit for bit by the CPU oracle,
+ * kor_det_sum). Batches do not take queries with REAL sums in this mode
+ * (KG_ERR_UNSUPPORTED per query; kg_serve runs them on the plain path). */
 int kg_ctx_set_deterministic(kg_ctx* ctx, int on);
 /* Join probes can first partition the probe keys by table region (histogram,
  * scan, LDS-staged scatter) and then probe one region at a time per XCD, so

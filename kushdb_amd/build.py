@@ -31,6 +31,7 @@ SOURCES = [
     "kg_expr.hip",
     "kg_skinner.hip",
     "kg_group.hip",
+    "kg_detsum.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1716,6 +1716,12 @@ int plan_query(kg_ctx* ctx, const kg_query& q, BQuery& D, BPlan& P) {
   bool minmax = false;
   if ((st = build_measures(ctx, q.cols, q.ncols, q.aggs, q.naggs, q.n, P.M, P.plan, minmax)) != KG_OK) return st;
   if (minmax) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "MIN/MAX are not batched");
+  // Batched REAL sums add in a timing-dependent order; in the deterministic
+  // mode such a query is not taken (the serving loop runs it on the plain path).
+  if (ctx->deterministic)
+    for (int m = 0; m < P.M.nm; m++)
+      if (P.M.is_real[m])
+        return kg_fail(ctx, KG_ERR_UNSUPPORTED, "REAL sums are not batched in the deterministic mode");
   D.M = P.M;
   D.kind = q.kind;
   D.n = q.n;

@@ -495,11 +495,18 @@ int plan_expr(kg_ctx* ctx, const kg_col* cols, const kg_text_col* text_cols, con
           X.tcount = d->count;
           for (int c = 0; c < d->count; c++) {
             const std::string s = dict_str(*d, c);
-            tabs.push_back(like_match(reinterpret_cast<const uint8_t*>(s.data()), static_cast<int64_t>(s.size()),
-                                      reinterpret_cast<const uint8_t*>(N.str), N.str_len)
+            tabs.push_back(text_match(KG_TEXT_LIKE, reinterpret_cast<const uint8_t*>(s.data()),
+                                      static_cast<int64_t>(s.size()), reinterpret_cast<const uint8_t*>(N.str),
+                                      N.str_len)
                                ? 1
                                : 0);
           }
+        } else if (!like_pattern_ok(reinterpret_cast<const uint8_t*>(N.str), N.str_len)) {
+          // re2 rejects a pattern that is not UTF-8, and the rejected regex
+          // matches nothing: an empty code table gives FALSE for every row.
+          X.tmode = 1;
+          X.toff = static_cast<int64_t>(tabs.size());
+          X.tcount = 0;
         } else {
           X.scol = col_of(N.a);
           KG_REQUIRE(ctx, X.scol >= 0, "node %d: LIKE on a TEXT column", k);

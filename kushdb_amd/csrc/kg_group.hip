@@ -108,10 +108,18 @@ __device__ __forceinline__ void atomic_max_f64(unsigned long long* p, double x) 
   }
 }
 
+// Every claimed slot is one group; the claim that takes the count past
+// max_groups raises *overflow, and every thread stops at its next row once it
+// sees the flag, so an input with far more distinct keys than max_groups ends
+// after about one table's worth of claims instead of probing a full table
+// for each remaining row.
 __global__ __launch_bounds__(kGBlock) void k_hash_group(PredSet P, HKeys K, MeasureSet M, int32_t minmax, int64_t n,
-                                                         GTable T, int* overflow) {
+                                                         GTable T, int* overflow, unsigned long long* claimed,
+                                                         int64_t max_groups, int32_t* row_slot) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kGBlock;
   for (int64_t r = static_cast<int64_t>(blockIdx.x) * kGBlock + threadIdx.x; r < n; r += stride) {
+    if (__hip_atomic_load(overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    if (row_slot) row_slot[r] = -1;
     if (!eval_preds(P, r)) continue;
     uint64_t s = key_hash(K, r) & T.mask;
     int64_t slot = -1;
@@ -121,6 +129,7 @@ __global__ __launch_bounds__(kGBlock) void k_hash_group(PredSet P, HKeys K, Meas
         const int32_t prev = atomicCAS(&T.owner[s], -1, static_cast<int32_t>(r));
         if (prev < 0) {
           slot = static_cast<int64_t>(s);
+          if (static_cast<int64_t>(atomicAdd(claimed, 1ull)) >= max_groups) atomicOr(overflow, 1);
           break;
         }
         o = prev;
@@ -135,6 +144,7 @@ __global__ __launch_bounds__(kGBlock) void k_hash_group(PredSet P, HKeys K, Meas
       atomicOr(overflow, 1);
       continue;
     }
+    if (row_slot) row_slot[r] = static_cast<int32_t>(slot);
     atomicAdd(&T.rows[slot], 1ull);
     atomicMin(&T.first[slot], static_cast<int32_t>(r));
     for (int m = 0; m < M.nm; m++) {
@@ -270,7 +280,11 @@ extern "C" int kg_hash_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   GTable T{};
   T.mask = nslots - 1;
   const size_t mslots = nslots * static_cast<size_t>(nm > 0 ? nm : 1);
-  void* base[8] = {};
+  bool any_real = false;
+  for (int m = 0; m < nm; m++) any_real |= M.is_real[m] != 0;
+  // Deterministic mode: each row's slot, for the fixed-order REAL sums.
+  const bool det = ctx->deterministic && any_real && n > 0;
+  void* base[9] = {};
   auto release = [&]() {
     for (void* p : base)
       if (p) (void)hipFree(p);
@@ -279,7 +293,8 @@ extern "C" int kg_hash_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
       (st = galloc(ctx, &base[2], nslots * 4)) != KG_OK || (st = galloc(ctx, &base[3], mslots * 8)) != KG_OK ||
       (st = galloc(ctx, &base[4], mslots * 8)) != KG_OK ||
       (st = galloc(ctx, &base[5], minmax ? mslots * 8 : 8)) != KG_OK ||
-      (st = galloc(ctx, &base[6], minmax ? mslots * 8 : 8)) != KG_OK || (st = galloc(ctx, &base[7], 16)) != KG_OK) {
+      (st = galloc(ctx, &base[6], minmax ? mslots * 8 : 8)) != KG_OK || (st = galloc(ctx, &base[7], 16)) != KG_OK ||
+      (st = galloc(ctx, &base[8], det ? n * 4 : 8)) != KG_OK) {
     release();
     return st;
   }
@@ -306,7 +321,8 @@ extern "C" int kg_hash_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   kg_time_begin(ctx);
   if (n > 0)
     hipLaunchKernelGGL(k_hash_group, dim3(grid(n)), dim3(kGBlock), 0, ctx->stream, P, K, M, minmax ? 1 : 0, n, T,
-                       overflow);
+                       overflow, reinterpret_cast<unsigned long long*>(static_cast<char*>(base[7]) + 8), max_groups,
+                       det ? static_cast<int32_t*>(base[8]) : nullptr);
   kg_time_end(ctx);
   if (hipGetLastError() != hipSuccess) return fail(kg_fail(ctx, KG_ERR_HIP, "group launch failed"));
   int hov = 0;
@@ -314,6 +330,9 @@ extern "C" int kg_hash_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
       hipStreamSynchronize(ctx->stream) != hipSuccess)
     return fail(kg_fail(ctx, KG_ERR_HIP, "overflow read-back failed"));
   if (hov) return fail(kg_fail(ctx, KG_ERR_CAPACITY, "more than %lld groups", (long long)max_groups));
+  if (det && (st = det_fp_sums(ctx, n, static_cast<int32_t*>(base[8]), nullptr, static_cast<int64_t>(nslots), M,
+                               reinterpret_cast<uint64_t*>(T.sum), 1)) != KG_OK)
+    return fail(st);
 
   // Used slots, compacted.
   size_t cub = 0;

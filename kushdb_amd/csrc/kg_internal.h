@@ -14,7 +14,7 @@ struct kg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  int deterministic = 1;
+  int deterministic = 0;  // kg_ctx_set_deterministic
   // Region count for partitioned join probes: -1 chooses by table size, 0 probes
   // directly, otherwise a multiple of 8 (kg_ctx_set_probe_regions).
   int probe_regions = -1;

@@ -16,6 +16,8 @@
 // bucket order claimed with a 64-bit atomicCAS, next bucket when full.
 // Probe: four lanes read one 64-byte bucket together (16 bytes each), compare
 // two keys per lane, and ballot; a bucket with a free slot ends the scan.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -737,6 +739,48 @@ __global__ __launch_bounds__(kJBlock) void k_join_group_agg(PredSet P, kg_col ke
   }
 }
 
+// Deterministic mode: the join output rows of the star pipeline as items in
+// probe-row order. Pass 1 counts each probe row's matches whose group code is
+// a group; pass 2 writes (group, probe row) per match at the row's offset.
+// Matches of one probe row carry the same measure values, so their order among
+// themselves does not change any group's sum.
+__global__ void k_star_det_count(PredSet P, kg_col key, TableView t, const int32_t* codes, int G, int64_t n,
+                                 int64_t* cnt) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r > n) return;
+  if (r == n) {  // the scan's extra element
+    cnt[n] = 0;
+    return;
+  }
+  int64_t c = 0;
+  int64_t k;
+  if (eval_preds(P, r) && load_key(key, r, k)) {
+    table_matches(t, k, [&](int32_t brow) {
+      const int g = codes ? codes[brow] : 0;
+      if (g >= 0 && g < G) c++;
+    });
+  }
+  cnt[r] = c;
+}
+
+__global__ void k_star_det_items(PredSet P, kg_col key, TableView t, const int32_t* codes, int G, int64_t n,
+                                 const int64_t* off, int32_t* gid, int32_t* src) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t o = off[r];
+  const int64_t end = off[r + 1];
+  int64_t k;
+  if (o == end || !eval_preds(P, r) || !load_key(key, r, k)) return;
+  table_matches(t, k, [&](int32_t brow) {
+    const int g = codes ? codes[brow] : 0;
+    if (g >= 0 && g < G && o < end) {
+      gid[o] = g;
+      src[o] = static_cast<int32_t>(r);
+      o++;
+    }
+  });
+}
+
 __global__ void k_init_join_final(AggFinal* fin, int G, MeasureSet M, unsigned long long* rows,
                                   unsigned long long* first) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1238,6 +1282,60 @@ extern "C" int kg_group_by_i64(kg_ctx* ctx, const int64_t* d_keys, const int64_t
   return KG_OK;
 }
 
+namespace {
+
+// The star pipeline's REAL sums in the deterministic order (det_fp_sums over
+// its join output rows), written into dfin.
+int star_det_sums(kg_ctx* ctx, int64_t n, const PredSet& P, const kg_col& key, const TableView& t,
+                  const int32_t* codes, int G, const MeasureSet& M, AggFinal* dfin) {
+  hipStream_t s = ctx->stream;
+  size_t scan_tmp = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, static_cast<int64_t*>(nullptr),
+                                         static_cast<int64_t*>(nullptr), static_cast<int>(n + 1), s);
+  char* buf = nullptr;
+  const size_t cnt_b = (n + 1) * 8, off_b = (n + 1) * 8;
+  if (hipMalloc(&buf, cnt_b + off_b + scan_tmp + 256) != hipSuccess) {
+    (void)hipGetLastError();
+    return kg_fail(ctx, KG_ERR_OOM, "deterministic star: device allocation failed");
+  }
+  int64_t* cnt = reinterpret_cast<int64_t*>(buf);
+  int64_t* off = reinterpret_cast<int64_t*>(buf + cnt_b);
+  void* tmp = buf + cnt_b + off_b;
+  int64_t N = 0;
+  int st = KG_OK;
+  hipLaunchKernelGGL(k_star_det_count, dim3(ceil_div(n + 1, 256)), dim3(256), 0, s, P, key, t, codes, G, n, cnt);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) {
+    size_t tb = scan_tmp;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, off, static_cast<int>(n + 1), s);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&N, off + n, 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  char* items = nullptr;
+  if (e == hipSuccess && N > 0) {
+    if (hipMalloc(&items, N * 8) != hipSuccess) {
+      (void)hipGetLastError();
+      st = kg_fail(ctx, KG_ERR_OOM, "deterministic star: %lld join rows", (long long)N);
+    } else {
+      int32_t* gid = reinterpret_cast<int32_t*>(items);
+      int32_t* src = gid + N;
+      hipLaunchKernelGGL(k_star_det_items, dim3(ceil_div(n, 256)), dim3(256), 0, s, P, key, t, codes, G, n, off,
+                         gid, src);
+      e = hipGetLastError();
+      if (e == hipSuccess) st = det_fp_sums(ctx, N, gid, src, G, M, reinterpret_cast<uint64_t*>(dfin),
+                                            sizeof(AggFinal) / 8);
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  if (items) (void)hipFree(items);
+  (void)hipFree(buf);
+  if (st != KG_OK) return st;
+  if (e != hipSuccess) return kg_fail(ctx, KG_ERR_HIP, "deterministic star failed: %s", hipGetErrorString(e));
+  return KG_OK;
+}
+
+}  // namespace
+
 extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
                                       const kg_pred* preds, int32_t npreds, const kg_hash_table* ht,
                                       int32_t probe_key_col, const kg_col* build_codes, int32_t group_card,
@@ -1304,6 +1402,14 @@ extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols
                          dfin, drows, dfirst);
     kg_time_end(ctx);
     KG_LAUNCH_CHECK(ctx);
+    bool any_real = false;
+    for (int m = 0; m < nm; m++) any_real |= M.is_real[m] != 0;
+    if (ctx->deterministic && any_real) {
+      // Deterministic mode: the REAL sums again, in the fixed order over the
+      // join output rows.
+      if ((st = star_det_sums(ctx, n, P, cols[probe_key_col], ht->view(), codes_data, G, M, dfin)) != KG_OK)
+        return st;
+    }
   }
   std::vector<AggFinal> fin(static_cast<size_t>(G) * (nm > 0 ? nm : 1));
   std::vector<uint64_t> rowsv(G), firstu(G);

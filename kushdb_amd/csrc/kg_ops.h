@@ -148,6 +148,14 @@ int emit_groups(kg_ctx* ctx, int64_t G, int32_t nkeys, int32_t card1, const Meas
                 const std::vector<std::vector<int8_t>>& first_null, int64_t max_groups, int64_t* out_ngroups,
                 int32_t* out_keys, int64_t* out_vals, int8_t* out_nulls);
 
+// Deterministic mode (kg_detsum.hip): for every REAL measure m of M and every
+// group g in [0, G) with items, writes the fixed-order FP64 sum of m over the
+// group's items to dst[(g * M.nm + m) * stride]. Item k (0..N-1) belongs to
+// group gid[k] (outside [0, G): none) and reads its values at row src[k]
+// (src NULL: row k); items are taken in ascending k. Synchronizes the stream.
+int det_fp_sums(kg_ctx* ctx, int64_t N, const int32_t* gid, const int32_t* src, int64_t G, const MeasureSet& M,
+                uint64_t* dst, int32_t stride);
+
 // Directory and tid array of a column index (kg_index.hip).
 // Dense directories have dspan > 0 (offset/count indexed by key - dmin).
 void index_dir(const kg_index* idx, const int64_t** keys, const int64_t** offset, const int32_t** count,

@@ -588,6 +588,19 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_lds(PredSet P, KeySet K, M
   }
 }
 
+// Deterministic mode: the group of every row (-1 when the row does not
+// qualify or its code is not a group), the items of det_fp_sums.
+__global__ void k_det_gid_dense(PredSet P, KeySet K, int64_t n, int G, int32_t* gid) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int g = -1;
+  if (eval_preds(P, r)) {
+    g = group_of(K, r);
+    if (g < 0 || g >= G) g = -1;
+  }
+  gid[r] = g;
+}
+
 __global__ void k_init_lds_final(AggFinal* fin, int G, MeasureSet M, unsigned long long* rows, int32_t* first) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nm = M.nm;
@@ -953,8 +966,11 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
       off += (bytes + 255) / 256 * 256;
       return o;
     };
+    bool any_real = false;
+    for (int m = 0; m < nm; m++) any_real |= M.is_real[m] != 0;
+    const bool det = ctx->deterministic && any_real && n > 0;
     const size_t o_fin = carve(G * (nm > 0 ? nm : 1) * sizeof(AggFinal)), o_rows = carve(G * 8),
-                 o_first = carve(G * 4), o_fnull = carve(G + 16);
+                 o_first = carve(G * 4), o_fnull = carve(G + 16), o_gid = carve(det ? n * 4 : 0);
     if ((st = kg_work_reserve(ctx, off)) != KG_OK) return st;
     char* w = static_cast<char*>(ctx->work);
     AggFinal* dfin = reinterpret_cast<AggFinal*>(w + o_fin);
@@ -976,6 +992,16 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
                          static_cast<int>(G), dfin, drows, dfirst);
     kg_time_end(ctx);
     KG_LAUNCH_CHECK(ctx);
+    if (det) {
+      // Deterministic mode: the REAL sums again, in the fixed order.
+      int32_t* gid = reinterpret_cast<int32_t*>(w + o_gid);
+      hipLaunchKernelGGL(k_det_gid_dense, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, P, K, n,
+                         static_cast<int>(G), gid);
+      KG_LAUNCH_CHECK(ctx);
+      if ((st = det_fp_sums(ctx, n, gid, nullptr, G, M, reinterpret_cast<uint64_t*>(dfin),
+                            sizeof(AggFinal) / 8)) != KG_OK)
+        return st;
+    }
     KG_HIP_CHECK(ctx, hipMemcpyAsync(fin.data(), dfin, G * nm * sizeof(AggFinal), hipMemcpyDeviceToHost, ctx->stream));
     KG_HIP_CHECK(ctx, hipMemcpyAsync(rowsv.data(), drows, G * 8, hipMemcpyDeviceToHost, ctx->stream));
     KG_HIP_CHECK(ctx, hipMemcpyAsync(firstv.data(), dfirst, G * 4, hipMemcpyDeviceToHost, ctx->stream));

@@ -67,7 +67,7 @@ class Column:
 class Context:
     """One device + stream (kg_ctx). Work is ordered on torch's current stream."""
 
-    def __init__(self, device=0, deterministic=True):
+    def __init__(self, device=0, deterministic=False):
         self.lib = abi.load()
         h = ctypes.c_void_p()
         st = self.lib.kg_ctx_create(device, ctypes.byref(h))
@@ -558,8 +558,9 @@ class TextColumn:
 
     @staticmethod
     def from_strings(strings, device="cuda:0", nulls=None):
+        """Strings (str, UTF-8 encoded, or raw bytes) as a device column."""
         import numpy as _np
-        enc = [s.encode() for s in strings]
+        enc = [s if isinstance(s, (bytes, bytearray)) else s.encode() for s in strings]
         off = _np.zeros(len(enc) + 1, dtype=_np.int64)
         off[1:] = _np.cumsum([len(b) for b in enc])
         heap = _np.frombuffer(b"".join(enc), dtype=_np.uint8) if off[-1] else _np.zeros(1, dtype=_np.uint8)

@@ -31,13 +31,22 @@
 //   kor_kdbindex_get   - the reference fixture .kdbindex files against a
 //                        linear scan of the matching .kdb column.
 //   kor_q_*            - compositions of the above, checked against numpy.
+//   kor_det_sum        - the deterministic mode's order; known answers with
+//                        every value exact in binary (so any order gives the
+//                        same sum), a Python restatement of the chunk tree on
+//                        values chosen so that a different chunking, lane
+//                        fold or tree pairing changes the sum, and math.fsum
+//                        error bounds (tests/test_oracle_ref.py).
 //   kor_dtree_scan_agg - the GPU deterministic order; checked against math.fsum
 //                        (|err| bound) and against kor_q_scan_agg (1e-12 rel).
 //   kor_text_pred      - StartsWith/EndsWith/Contains/compare against Python's
-//                        str methods; LIKE against Python's re.fullmatch of the
-//                        planner's escaped regex (plan/planner.cc:284-323) on
-//                        ASCII data. (re2 itself is absent: general LIKE is
-//                        pinned through that equivalent regex, not re2.)
+//                        str methods; LIKE against Python's re.fullmatch (no
+//                        DOTALL, so "." is one code point other than '\n', as
+//                        in re2's default UTF-8 mode) of the planner's escaped
+//                        regex (plan/planner.cc:284-323) on strings with
+//                        newlines and 2-4-byte characters, and against a bytes
+//                        regex restating re2's "." byte ranges on invalid
+//                        UTF-8. (re2 itself is absent here.)
 //   kor_text_minmax    - min()/max() over Python bytes.
 //   kor_expr_eval      - reference fixtures select/* (every type, sorted rows),
 //                        tpch/q06 (filter + SUM of a product) and tpch/q01
@@ -761,21 +770,71 @@ static int kor_text_cmp(const uint8_t* a, int64_t na, const uint8_t* b, int64_t 
   return na < nb ? -1 : (na > nb ? 1 : 0);
 }
 
-static bool kor_like(const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
-  // ok[j]: pattern prefix p[0..j) matches string prefix s[0..i).
-  std::vector<char> ok(m + 1, 0), nx(m + 1, 0);
-  ok[0] = 1;
-  for (int32_t j = 1; j <= m; j++) ok[j] = ok[j - 1] && p[j - 1] == '%';
-  for (int64_t i = 1; i <= n; i++) {
-    nx[0] = 0;
-    for (int32_t j = 1; j <= m; j++) {
-      const uint8_t c = p[j - 1];
-      if (c == '%') nx[j] = nx[j - 1] || ok[j];
-      else nx[j] = ok[j - 1] && (c == '_' || c == s[i - 1]);
+// re2's default "." over UTF-8 text ([^\n]): one byte 00-09 / 0B-7F, or a
+// sequence [C2-DF][80-BF], [E0-EF][80-BF]{2}, [F0-F4][80-BF]{3}
+// (re2/compile.cc's byte ranges for code points 80-10FFFF). Returns the
+// sequence length at s[i], 0 when "." cannot match there.
+static int kor_dot_len(const uint8_t* s, int64_t i, int64_t n) {
+  if (i >= n) return 0;
+  const uint8_t c = s[i];
+  if (c < 0x80) return c == '\n' ? 0 : 1;
+  const int len = (c >= 0xC2 && c <= 0xDF) ? 2 : (c >= 0xE0 && c <= 0xEF) ? 3 : (c >= 0xF0 && c <= 0xF4) ? 4 : 0;
+  if (len == 0 || i + len > n) return 0;
+  for (int k = 1; k < len; k++)
+    if ((s[i + k] & 0xC0) != 0x80) return 0;
+  return len;
+}
+
+// re2 parses the pattern as UTF-8 and rejects the regex (which then matches
+// nothing) on a byte sequence that does not decode: stray continuation bytes,
+// overlong forms, code points above 10FFFF, truncation (re2/parse.cc
+// StringPieceToRune).
+static bool kor_pattern_utf8(const uint8_t* p, int32_t m) {
+  for (int32_t j = 0; j < m;) {
+    const uint8_t c = p[j];
+    int len = 1;
+    uint32_t cp = c;
+    if (c >= 0x80) {
+      if (c < 0xC0 || c > 0xF7) return false;
+      len = c < 0xE0 ? 2 : (c < 0xF0 ? 3 : 4);
+      if (j + len > m) return false;
+      cp = c & (0x7F >> len);
+      for (int k = 1; k < len; k++) {
+        if ((p[j + k] & 0xC0) != 0x80) return false;
+        cp = (cp << 6) | (p[j + k] & 0x3F);
+      }
+      static const uint32_t kMin[5] = {0, 0, 0x80, 0x800, 0x10000};
+      if (cp < kMin[len] || cp > 0x10FFFF) return false;
     }
-    ok.swap(nx);
+    j += len;
   }
-  return ok[m] != 0;
+  return true;
+}
+
+// The planner's regex under RE2::FullMatch (plan/planner.cc:284-323,
+// runtime/string.cc:79-82): reach[i][j] says pattern prefix p[0..j) can
+// consume s[0..i); '%' = ".*", '_' = ".", other bytes literal.
+static bool kor_like(const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
+  if (!kor_pattern_utf8(p, m)) return false;
+  const int64_t w = m + 1;
+  std::vector<char> reach((n + 1) * w, 0);
+  reach[0] = 1;
+  for (int64_t i = 0; i <= n; i++) {
+    for (int32_t j = 0; j < m; j++) {
+      if (!reach[i * w + j]) continue;
+      const uint8_t c = p[j];
+      const int k = kor_dot_len(s, i, n);
+      if (c == '%') {
+        reach[i * w + j + 1] = 1;
+        if (k) reach[(i + k) * w + j] = 1;
+      } else if (c == '_') {
+        if (k) reach[(i + k) * w + j + 1] = 1;
+      } else if (i < n && s[i] == c) {
+        reach[(i + 1) * w + j + 1] = 1;
+      }
+    }
+  }
+  return reach[n * w + m] != 0;
 }
 
 int32_t kor_text_pred(int32_t op, const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
@@ -916,15 +975,23 @@ std::string kor_dict_str(const kor_expr_col& c, int64_t code) {
 }
 
 bool kor_like_rec(const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
-  // Plain recursive definition of the anchored pattern ('%' any run, '_' one byte).
+  // Recursive definition of the planner's regex under re2 ('%' any run of
+  // "." characters, '_' one "." character, see kor_dot_len); the caller
+  // checks the pattern's UTF-8 once.
   if (m == 0) return n == 0;
   if (p[0] == '%') {
-    for (int64_t k = 0; k <= n; k++)
+    for (int64_t k = 0;;) {
       if (kor_like_rec(s + k, n - k, p + 1, m - 1)) return true;
-    return false;
+      const int d = kor_dot_len(s, k, n);
+      if (!d) return false;
+      k += d;
+    }
   }
-  if (n == 0) return false;
-  if (p[0] != '_' && p[0] != s[0]) return false;
+  if (p[0] == '_') {
+    const int d = kor_dot_len(s, 0, n);
+    return d && kor_like_rec(s + d, n - d, p + 1, m - 1);
+  }
+  if (n == 0 || p[0] != s[0]) return false;
   return kor_like_rec(s + 1, n - 1, p + 1, m - 1);
 }
 
@@ -1083,8 +1150,9 @@ int32_t kor_expr_eval(int64_t n, const kor_expr_col* cols, int32_t ncols, const 
           } else {
             str_of(N.a, r, sa);
           }
-          x.i = kor_like_rec(reinterpret_cast<const uint8_t*>(sa.data()), static_cast<int64_t>(sa.size()),
-                             reinterpret_cast<const uint8_t*>(N.str), N.str_len)
+          x.i = kor_pattern_utf8(reinterpret_cast<const uint8_t*>(N.str), N.str_len) &&
+                        kor_like_rec(reinterpret_cast<const uint8_t*>(sa.data()), static_cast<int64_t>(sa.size()),
+                                     reinterpret_cast<const uint8_t*>(N.str), N.str_len)
                     ? 1
                     : 0;
           break;
@@ -1307,3 +1375,35 @@ void kor_dtree_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int
 }
 
 }  // extern "C"
+
+// The deterministic mode's FP64 order (kg_ctx_set_deterministic(1)) for one
+// group: v[0..n) are the group's values in input order, NULLs passed as -0.0.
+// Consecutive chunks of 256 values; in a chunk, slot l (0..63) folds values
+// 4l..4l+3 left to right, then s[l] = s[l] + s[l + d] for d = 32, 16, ..., 1
+// (missing values are -0.0, which leaves every sum unchanged); while more than
+// one chunk sum is left, the chunk sums are summed the same way.
+static double kor_det_chunk(const double* v, int64_t n) {
+  double s[64];
+  for (int l = 0; l < 64; l++) {
+    for (int e = 0; e < 4; e++) {
+      const int64_t i = 4 * l + e;
+      const double x = i < n ? v[i] : -0.0;
+      s[l] = e == 0 ? x : s[l] + x;
+    }
+  }
+  for (int d = 32; d >= 1; d >>= 1)
+    for (int l = 0; l < d; l++) s[l] = s[l] + s[l + d];
+  return s[0];
+}
+
+double kor_det_sum(const double* v, int64_t n) {
+  if (n <= 0) return -0.0;
+  std::vector<double> cur(v, v + n);
+  while (cur.size() > 256) {
+    std::vector<double> nxt;
+    for (size_t c = 0; c < cur.size(); c += 256)
+      nxt.push_back(kor_det_chunk(cur.data() + c, std::min<int64_t>(256, static_cast<int64_t>(cur.size() - c))));
+    cur.swap(nxt);
+  }
+  return kor_det_chunk(cur.data(), static_cast<int64_t>(cur.size()));
+}

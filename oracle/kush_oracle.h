@@ -186,6 +186,11 @@ void kor_dtree_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int
                         const double* const* fm, int32_t nblocks, int32_t block_threads, int32_t vec,
                         double* out_sums);
 
+// The deterministic mode's fixed FP64 order over one group's values (input
+// order, NULL as -0.0): chunks of 256, a 4-value fold per slot, a pairwise
+// tree over 64 slots, repeated over chunk sums.
+double kor_det_sum(const double* v, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -134,7 +134,21 @@ def lib():
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_void_p]
+        L.kor_det_sum.restype = ctypes.c_double
+        L.kor_det_sum.argtypes = [ctypes.c_void_p, ctypes.c_int64]
     return _lib
+
+
+def det_sum(values):
+    """The deterministic mode's fixed-order FP64 sum of one group's values
+    (input order; NULLs passed as -0.0)."""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    return lib().kor_det_sum(v.ctypes.data, len(v))
+
+
+def ref_built():
+    """Whether the reference-runtime pin library exists (without loading it)."""
+    return os.path.exists(REF_SO)
 
 
 def ref():

@@ -115,6 +115,21 @@ def test_hash_group_capacity_and_empty(ctx):
     assert len(first) == 0
 
 
+def test_hash_group_many_more_keys_than_max_groups(ctx):
+    """16M distinct keys against max_groups = 1000: the table holds 2048 slots,
+    and the call must end with KG_ERR_CAPACITY after about that many claims,
+    not probe the full table for each of the remaining rows."""
+    import time
+    k = Column(abi.KG_BIGINT, torch.randperm(1 << 24, dtype=torch.int64, device=DEV))
+    t0 = time.perf_counter()
+    with pytest.raises(abi.KgError) as ei:
+        ctx.hash_group_agg([k], [], [0], [(abi.KG_AGG_COUNT, -1)], max_groups=1000)
+    assert ei.value.status == abi.KG_ERR_CAPACITY
+    assert time.perf_counter() - t0 < 5.0
+    first, vals, _ = ctx.hash_group_agg([k], [(0, abi.KG_LT, 1000)], [0], [(abi.KG_AGG_COUNT, -1)], max_groups=1000)
+    assert len(first) == 1000 and np.all(vals[:, 0] == 1)
+
+
 def test_dense_keys_beyond_on_chip_table(ctx):
     """kg_scan_group_agg over 3000 dense codes x 2 keys (6000 groups, more than
     an LDS table holds) takes the hashed path: the same groups in insertion

@@ -232,23 +232,25 @@ def test_oracle_conversions_and_year():
 
 def test_oracle_strings_like_in():
     rng = np.random.default_rng(5)
-    words = ["", "a", "ab", "abc", "b", "ba", "banana", "bandana", "a_c", "a%c", "xyz", "abab"]
+    words = ["", "a", "ab", "abc", "b", "ba", "banana", "bandana", "a_c", "a%c", "xyz", "abab",
+             "a\nc", "\u00e9a", "a\u20acc", "ban\nana"]
     n = 600
     strs = [words[k] for k in rng.integers(0, len(words), n)]
     tn = (rng.random(n) < 0.15).astype(np.int8)
-    dic = ["apple", "banana", "cherry", "date", "apricot"]
+    dic = ["apple", "banana", "cherry", "date", "apricot", "a\u00e9c", "a\nc"]
     codes = rng.integers(0, len(dic), n).astype(np.int32)
     en = (rng.random(n) < 0.15).astype(np.int8)
     cols = [("text", strs, tn), po.Col(po.ENUM, codes, en, dictionary=dic)]
     S, E = ColRef(0, abi.KG_TEXT), ColRef(1, abi.KG_ENUM)
-    for pat in ["a%", "%a", "%an%", "a_c", "_", "", "%", "b%a", "a\\%c"]:
-        rx = "^" + "".join(".*" if ch == "%" else "." if ch == "_" else re.escape(ch) for ch in pat) + "$"
+    for pat in ["a%", "%a", "%an%", "a_c", "_", "", "%", "b%a", "a\\%c", "_a", "a%c", "ban%ana"]:
+        # re2's default "." (one code point, never a newline) = Python's without DOTALL
+        rx = r"\A" + "".join(".*" if ch == "%" else "." if ch == "_" else re.escape(ch) for ch in pat) + r"\Z"
         v, nl = oracle_eval(Like(S, pat), cols, n)
         assert not nl.any()                          # LIKE on NULL is a non-NULL FALSE
-        want = [bool(re.fullmatch(rx, s, re.S)) and not tn[i] for i, s in enumerate(strs)]
+        want = [bool(re.fullmatch(rx, s)) and not tn[i] for i, s in enumerate(strs)]
         assert v.astype(bool).tolist() == want, pat
         v, nl = oracle_eval(Like(E, pat), cols, n)
-        want = [bool(re.fullmatch(rx, dic[c], re.S)) and not en[i] for i, c in enumerate(codes)]
+        want = [bool(re.fullmatch(rx, dic[c])) and not en[i] for i, c in enumerate(codes)]
         assert v.astype(bool).tolist() == want, pat
     for mk, fn in ((StartsWith, str.startswith), (EndsWith, str.endswith), (Contains, lambda s, p: p in s)):
         for p in ["a", "an", "", "ana"]:

@@ -71,7 +71,7 @@ def _random_cols(rng, n, card):
     return cols
 
 
-@pytest.mark.skipif(po.ref() is None, reason="oracle/_ref not built (reference checkout absent)")
+@pytest.mark.skipif(not po.ref_built(), reason="oracle/_ref not built (reference checkout absent)")
 @pytest.mark.parametrize("seed", range(6))
 def test_oracle_order_by_vs_reference_sort(seed):
     rng = np.random.default_rng(seed)
