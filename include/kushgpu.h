@@ -171,6 +171,18 @@ int kg_ctx_kernel_stats(kg_ctx* ctx, double* total_ms, int64_t* launches, int re
 int kg_scan_select(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
                    const kg_pred* preds, int32_t npreds, int32_t conjunction,
                    int32_t* d_out_rows, int64_t* out_count);
+/* kg_scan_select that allocates nothing. No counterpart in the reference; it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and errors as kg_scan_select, except: n is limited to
+ * 536,870,912 rows (131072 tiles of 4096; beyond that KG_ERR_CAPACITY), and
+ * the tile counts / offsets live in a 1.5 MiB static device array per GPU,
+ * held until the process ends and shared by every call on that GPU — calls
+ * that overlap on one GPU (other streams, other threads, other contexts) race
+ * and must be serialized by the caller. Synchronizes the context's stream. */
+int kg_scan_select_noalloc(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
+                           const kg_pred* preds, int32_t npreds, int32_t conjunction,
+                           int32_t* d_out_rows, int64_t* out_count);
 
 /* ---- scan-select + group-by aggregate (dense code keys) -----------------------
  * ScanSelect -> GroupByAggregate (compile/translators/group_by_aggregate_translator.cc:36-111)

@@ -150,6 +150,8 @@ SIGNATURES = {
     "kg_ctx_kernel_stats": (_i32, [_vp, ctypes.POINTER(ctypes.c_double), _pi64, _i32]),
     "kg_scan_select": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32, _i32,
                               _vp, _pi64]),
+    "kg_scan_select_noalloc": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32, _i32,
+                                      _vp, _pi64]),
     "kg_scan_group_agg": (_i32, [_vp, _i64, ctypes.POINTER(KgCol), _i32, ctypes.POINTER(KgPred), _i32,
                                  _pi32, _pi32, _i32, ctypes.POINTER(KgAgg), _i32, _i64, _pi64, _pi32, _pi64,
                                  _pi8]),

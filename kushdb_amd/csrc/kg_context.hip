@@ -1,8 +1,38 @@
 // Context, errors and workspace for the kushgpu C ABI.
 #include <cstdarg>
+#include <mutex>
 #include <new>
 
 #include "kg_internal.h"
+
+// Device scratch of the entry points that allocate nothing
+// (kg_scan_select_noalloc): one array per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host the name is
+// only a stand-in); it uses the device address that hipGetSymbolAddress
+// returns for the current device, looked up once per device when the first
+// context on that device is created.
+static __device__ __align__(256) unsigned char scratch[kg::kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The scratch array's device address on ctx->device (current device).
+static int scratch_setup(kg_ctx* ctx) {
+  if (ctx->device >= kMaxDevices) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "device %d beyond %d", ctx->device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return kg_fail(ctx, KG_ERR_HIP, "scratch symbol address on device %d: %s", ctx->device, hipGetErrorString(e));
+    g_scratch_ptr[ctx->device] = ptr;
+  }
+  ctx->scratch = g_scratch_ptr[ctx->device];
+  return KG_OK;
+}
 
 int kg_fail(kg_ctx* ctx, int status, const char* fmt, ...) {
   if (ctx) {
@@ -121,6 +151,11 @@ int kg_ctx_create(int device, kg_ctx** out) {
     return KG_ERR_HIP;
   }
   ctx->own_stream = true;
+  if (scratch_setup(ctx) != KG_OK) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return KG_ERR_HIP;
+  }
   *out = ctx;
   return KG_OK;
 }

@@ -53,6 +53,9 @@ struct kg_ctx {
     int64_t* offsets = nullptr; // TEXT: n + 1 offsets into data
   };
   std::vector<FileCol> files;
+  // Device address of this device's static scratch array (kg_context.hip),
+  // used by the entry points that allocate nothing.
+  void* scratch = nullptr;
 };
 
 // Frees the device memory of a loaded file column.
@@ -94,6 +97,12 @@ int kg_pinned_reserve(kg_ctx* ctx, size_t bytes);
 namespace kg {
 
 constexpr int kWave = 64;
+
+// The static device scratch of the no-allocation entry points: tile counts
+// (int32) and tile offsets (int64) of kg_scan_select_noalloc for up to
+// kNoAllocMaxTiles tiles of 4096 rows, then the selected-row total.
+constexpr int64_t kNoAllocMaxTiles = 131072;
+constexpr int64_t kNoAllocScratchBytes = kNoAllocMaxTiles * 12 + 256;
 
 // Width in bytes of a column's values for the SQL type ids of kushgpu.h.
 inline int type_width(int32_t type) {

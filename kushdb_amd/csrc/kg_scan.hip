@@ -695,6 +695,47 @@ extern "C" int kg_scan_select(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_
   return KG_OK;
 }
 
+// kg_scan_select with its tile counts, offsets and total in the device's
+// static scratch array (ctx->scratch) instead of the context workspace, and
+// the total read back into a stack variable: no device or pinned allocation,
+// no event. The same kernels; every scratch word it reads was written earlier
+// in the same call (counts by k_select_count, offsets and total by the scan).
+extern "C" int kg_scan_select_noalloc(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
+                                      const kg_pred* preds, int32_t npreds, int32_t conjunction,
+                                      int32_t* d_out_rows, int64_t* out_count) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count %lld out of range", (long long)n);
+  KG_REQUIRE(ctx, out_count, "out_count is NULL");
+  KG_REQUIRE(ctx, n == 0 || d_out_rows, "d_out_rows is NULL");
+  const int64_t ntiles = ceil_div(n, kSelTile);
+  if (ntiles > kNoAllocMaxTiles)
+    return kg_fail(ctx, KG_ERR_CAPACITY,
+                   "kg_scan_select_noalloc: %lld rows exceed its scratch limit of %lld rows (use kg_scan_select)",
+                   (long long)n, (long long)(kNoAllocMaxTiles * kSelTile));
+  KG_REQUIRE(ctx, ctx->scratch, "no device scratch for device %d", ctx->device);
+  PredSet p;
+  int st = build_preds(ctx, cols, ncols, preds, npreds, conjunction, n, p);
+  if (st != KG_OK) return st;
+  *out_count = 0;
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  char* s = static_cast<char*>(ctx->scratch);
+  int32_t* counts = reinterpret_cast<int32_t*>(s);
+  int64_t* offsets = reinterpret_cast<int64_t*>(s + kNoAllocMaxTiles * 4);
+  int64_t* total = reinterpret_cast<int64_t*>(s + kNoAllocMaxTiles * 12);
+  hipLaunchKernelGGL(k_select_count, dim3(ntiles), dim3(kBlock), 0, ctx->stream, p, n, counts);
+  KG_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_exclusive_scan, dim3(1), dim3(1024), 0, ctx->stream, counts, offsets, ntiles, total);
+  KG_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_select_write, dim3(ntiles), dim3(kBlock), 0, ctx->stream, p, n, offsets, d_out_rows);
+  KG_LAUNCH_CHECK(ctx);
+  int64_t count = 0;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(&count, total, 8, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *out_count = count;
+  return KG_OK;
+}
+
 namespace {
 
 template <int G, bool MM>
