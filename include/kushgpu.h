@@ -427,9 +427,11 @@ int kg_skinner_join(kg_ctx* ctx, int32_t ntables, const kg_skinner_table* tables
                     const kg_skinner_cond* conds, const kg_skinner_opts* opts, kg_tuple_set** out_set,
                     kg_skinner_stats* stats);
 
-/* The UCT join-order agent alone (UctJoinAgent, skinner_join_executor.cc:
- * 779-838), host only, for callers that run their own episodes: Act() split at
- * the environment's Execute. kg_uct_next writes the next order (ntables
+/* The UCT join-order agent alone (the role of UctJoinAgent,
+ * skinner_join_executor.cc:779-838, with its own search: a flat node array,
+ * per-table statistics, UCB1 over the connected candidates, one new node per
+ * episode, random connected-first completion), host only, for callers that run
+ * their own episodes. kg_uct_next writes the next order (ntables
  * entries); kg_uct_reward feeds that episode's reward back (>= 0) before the
  * next call. adjacency[i*ntables+j] != 0 when a condition joins tables i and j
  * (either triangle suffices). Returns KG_ERR_INVALID on misuse. */

@@ -25,9 +25,8 @@
 // The budget is in partial tuples per episode; the slice of an order is
 // sized from the partial tuples per leftmost tuple its last episode produced.
 // The reward is the reference's (:252-274) over the episode's start and end
-// states, and the UCT agent is the reference's UctNode / UctJoinAgent
-// (exploration weight 1e-5, connectivity heuristic, playouts) behind an
-// ask/tell interface so that episodes run outside the recursion.
+// states. The order comes from a UCT search over join orders (OrderSearch
+// below) behind an ask/tell interface, so that episodes run in the host loop.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -36,7 +35,6 @@
 #include <cmath>
 #include <map>
 #include <memory>
-#include <random>
 #include <set>
 #include <utility>
 #include <vector>
@@ -51,173 +49,178 @@ constexpr int kMaxTables = 8;   // tuple-id set width limit
 constexpr int kMaxChecks = 16;  // conditions checked after one expansion
 
 // ---------------------------------------------------------------- UCT agent
-// Join seed: a positive seed is used as given (--join_seed), otherwise the
-// clock (GetJoinSeed, skinner_join_executor.cc:26-32). Every node seeds its own
-// engine with it.
+// Join-order search by UCT (upper confidence bounds applied to trees) as the
+// SkinnerDB paper describes it (Trummer et al., SIGMOD 2019, section 4): a
+// search tree over order prefixes, UCB1 selection inside the tree, one new
+// node per episode, random completion of the order below it, and the
+// episode's reward credited along the selected path. The reference runs the
+// same method (runtime/skinner_join_executor.cc:559-838); this agent is laid
+// out for the host loop around device episodes instead:
+//   * nodes live in one flat array; a node holds the bit set of the tables
+//     its prefix has joined and its children's indices by table id;
+//   * statistics are kept per table id (tries, reward sum), not per action;
+//   * selection is a loop from the root, not a recursion; candidates are the
+//     unjoined tables a join condition connects to the prefix (all unjoined
+//     tables when none is connected); an untried candidate is taken first
+//     (uniformly at random among them), then the largest
+//     mean + kExploration * sqrt(ln visits / tries), ties broken uniformly at
+//     random;
+//   * the order is completed by drawing, at each position, uniformly among the
+//     connected unjoined tables (all unjoined when none is);
+//   * every random choice comes from one SplitMix64 stream per agent.
+
+// Agent seed: a positive seed as given, otherwise the clock (the reference's
+// --join_seed default, skinner_join_executor.cc:26-32).
 uint64_t join_seed(int64_t seed) {
   if (seed > 0) return static_cast<uint64_t>(seed);
   return static_cast<uint64_t>(std::chrono::system_clock::now().time_since_epoch().count());
 }
 
-struct JoinGraph {
-  int n = 0;
-  std::vector<uint8_t> adj;  // n * n, symmetric: a condition joins the two tables
-  bool connected(const std::set<int>& joined, int t) const {
-    for (int j : joined)
-      if (adj[j * n + t]) return true;
-    return false;
+// SplitMix64 (Steele, Lea, Flood 2014).
+struct SplitMix {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
   }
+  int below(int n) { return static_cast<int>(next() % static_cast<uint64_t>(n)); }
 };
 
-class UctNode;
-using UctPath = std::vector<std::pair<UctNode*, int>>;
+struct JoinGraph {
+  int n = 0;
+  std::vector<uint64_t> nbr;  // nbr[t]: bit set of the tables a condition joins to t
+};
 
-// UctNode (skinner_join_executor.cc:559-777).
-class UctNode {
+// Tables a step may add after the prefix `joined`: those connected to it, or
+// every unjoined table when none is.
+inline uint64_t step_candidates(const JoinGraph& g, uint64_t joined) {
+  const uint64_t all = (g.n == 64 ? ~0ull : ((1ull << g.n) - 1)) & ~joined;
+  uint64_t conn = 0;
+  for (uint64_t m = joined; m; m &= m - 1) conn |= g.nbr[__builtin_ctzll(m)];
+  conn &= all;
+  return conn ? conn : all;
+}
+
+inline int pick_bit(uint64_t set, SplitMix& rng) {
+  int k = rng.below(__builtin_popcountll(set));
+  for (uint64_t m = set;; m &= m - 1)
+    if (k-- == 0) return __builtin_ctzll(m);
+}
+
+class OrderSearch {
  public:
-  UctNode(int round, const JoinGraph& g, const std::vector<int>& already_joined, int64_t seed)
-      : g_(g), seed_(seed), created_in_(round), num_tables_(g.n),
-        num_actions_(g.n - static_cast<int>(already_joined.size())), child_(num_actions_), tries_(num_actions_, 0),
-        acc_(num_actions_, 0.0), table_per_action_(num_actions_), rng_(join_seed(seed)) {
-    for (int i = 0; i < num_actions_; i++) {
-      priority_.push_back(i);
-      recommended_.insert(i);
-    }
-    joined_.insert(already_joined.begin(), already_joined.end());
-    int a = 0;
-    for (int i = 0; i < num_tables_; i++)
-      if (!joined_.count(i)) {
-        unjoined_.push_back(i);
-        table_per_action_[a++] = i;
+  static constexpr double kExploration = 1e-5;  // rewards lie in [0, 1]
+
+  OrderSearch(const JoinGraph& g, const std::vector<int>& prefix, uint64_t seed) : g_(g), prefix_(prefix) {
+    rng_.s = seed;
+    uint64_t joined = 0;
+    for (int t : prefix) joined |= 1ull << t;
+    add_node(joined);
+  }
+
+  // Next order (all tables) and the path of (node, table) steps it took.
+  void next(std::vector<int>& order) {
+    path_.clear();
+    order.assign(prefix_.begin(), prefix_.end());
+    int node = 0;
+    bool expanded = false;
+    uint64_t joined = nodes_[0].joined;
+    while (static_cast<int>(order.size()) < g_.n) {
+      const int t = choose(node, joined);
+      path_.push_back({node, t});
+      order.push_back(t);
+      joined |= 1ull << t;
+      Node& cur = nodes_[node];
+      int child = cur.child[t];
+      if (child < 0 && !expanded) {  // grow the tree by one node
+        child = add_node(joined);
+        nodes_[node].child[t] = child;
+        expanded = true;
+      } else if (child < 0) {
+        break;
       }
+      node = child;
+    }
+    // Below the tree: complete the order at random, connected tables first.
+    while (static_cast<int>(order.size()) < g_.n) {
+      const int t = pick_bit(step_candidates(g_, joined), rng_);
+      order.push_back(t);
+      joined |= 1ull << t;
+    }
   }
 
-  UctNode(int round, const UctNode& parent, int joined_table)
-      : g_(parent.g_), seed_(parent.seed_), created_in_(round), num_tables_(parent.num_tables_),
-        num_actions_(parent.num_actions_ - 1), child_(num_actions_), tries_(num_actions_, 0),
-        acc_(num_actions_, 0.0), joined_(parent.joined_), unjoined_(parent.unjoined_),
-        table_per_action_(num_actions_), rng_(join_seed(parent.seed_)) {
-    joined_.insert(joined_table);
-    unjoined_.erase(std::find(unjoined_.begin(), unjoined_.end(), joined_table));
-    int a = 0;
-    for (int t : unjoined_) table_per_action_[a++] = t;
-    // Connectivity heuristic: only actions joining a connected table are
-    // recommended, unless there is none.
-    for (int i = 0; i < num_actions_; i++)
-      if (g_.connected(joined_, table_per_action_[i])) recommended_.insert(i);
-    if (recommended_.empty())
-      for (int i = 0; i < num_actions_; i++) recommended_.insert(i);
-    for (int i = 0; i < num_actions_; i++)
-      if (recommended_.count(i)) priority_.push_back(i);
-  }
-
-  // Sample (:641-663) up to the environment's Execute: fills order and
-  // records the (node, action) pairs whose statistics the reward updates.
-  void Select(int round, std::vector<int>& order, UctPath& path) {
-    if (num_actions_ == 0) return;
-    const int action = SelectAction();
-    const int table = table_per_action_[action];
-    order[joined_.size()] = table;
-    const bool can_expand = created_in_ != round;
-    if (can_expand && !child_[action]) child_[action] = std::make_unique<UctNode>(round, *this, table);
-    path.emplace_back(this, action);
-    if (child_[action]) child_[action]->Select(round, order, path);
-    else Playout(order);
-  }
-
-  // UpdateStatistics (:766-771).
-  void Update(int action, double reward) {
-    num_visits_++;
-    tries_[action]++;
-    acc_[action] += reward;
+  void reward(double r) {
+    for (const Step& s : path_) {
+      Node& nd = nodes_[s.node];
+      nd.visits++;
+      nd.tries[s.table]++;
+      nd.sum[s.table] += r;
+    }
+    path_.clear();
   }
 
  private:
-  // Playout (:666-712): the rest of the order at random, preferring tables
-  // connected to those already placed.
-  void Playout(std::vector<int>& order) {
-    const int last = order[joined_.size()];
-    std::set<int> cur(joined_.begin(), joined_.end());
-    cur.insert(last);
-    std::vector<int> rest;
-    for (int i = 0; i < num_tables_; i++)
-      if (!joined_.count(i) && i != last) rest.push_back(i);
-    std::shuffle(rest.begin(), rest.end(), rng_);
-    for (int pos = static_cast<int>(joined_.size()) + 1; pos < num_tables_; pos++) {
-      bool found = false;
-      for (int t : rest)
-        if (!cur.count(t) && g_.connected(cur, t)) {
-          order[pos] = t;
-          cur.insert(t);
-          found = true;
-          break;
-        }
-      if (!found)
-        for (int t : rest)
-          if (!cur.count(t)) {
-            order[pos] = t;
-            cur.insert(t);
-            break;
-          }
-    }
+  struct Node {
+    uint64_t joined;
+    int64_t visits = 0;
+    std::vector<int64_t> tries;
+    std::vector<double> sum;
+    std::vector<int> child;  // by table id, -1 = not in the tree
+  };
+  struct Step {
+    int node, table;
+  };
+
+  int add_node(uint64_t joined) {
+    Node nd;
+    nd.joined = joined;
+    nd.tries.assign(g_.n, 0);
+    nd.sum.assign(g_.n, 0.0);
+    nd.child.assign(g_.n, -1);
+    nodes_.push_back(std::move(nd));
+    return static_cast<int>(nodes_.size()) - 1;
   }
 
-  // SelectAction (:714-764): untried recommended actions first, in random
-  // order; then the best mean reward + 1e-5 * sqrt(ln visits / tries) over the
-  // recommended actions, scanning from a random offset.
-  int SelectAction() {
-    if (!priority_.empty()) {
-      const int k = static_cast<int>(rng_() % priority_.size());
-      const int a = priority_[k];
-      priority_.erase(priority_.begin() + k);
-      return a;
-    }
-    const int offset = static_cast<int>(rng_() % num_actions_);
-    int best = -1;
-    double best_q = -1;
-    for (int i = 0; i < num_actions_; ++i) {
-      const int a = (offset + i) % num_actions_;
-      if (!recommended_.count(a)) continue;
-      const double mean = acc_[a] / tries_[a];
-      const double explore = std::sqrt(std::log(static_cast<double>(num_visits_)) / tries_[a]);
-      const double q = mean + kExploration * explore;
-      if (q > best_q) {
-        best = a;
-        best_q = q;
+  int choose(int node, uint64_t joined) {
+    const Node& nd = nodes_[node];
+    const uint64_t cand = step_candidates(g_, joined);
+    uint64_t untried = 0;
+    for (uint64_t m = cand; m; m &= m - 1)
+      if (nd.tries[__builtin_ctzll(m)] == 0) untried |= m & (~m + 1);
+    if (untried) return pick_bit(untried, rng_);
+    const double lnv = std::log(static_cast<double>(nd.visits));
+    double best = -1.0;
+    uint64_t best_set = 0;
+    for (uint64_t m = cand; m; m &= m - 1) {
+      const int t = __builtin_ctzll(m);
+      const double q = nd.sum[t] / nd.tries[t] + kExploration * std::sqrt(lnv / nd.tries[t]);
+      if (q > best) {
+        best = q;
+        best_set = 1ull << t;
+      } else if (q == best) {
+        best_set |= 1ull << t;
       }
     }
-    return best;
+    return pick_bit(best_set, rng_);
   }
 
-  static constexpr double kExploration = 1e-5;
   const JoinGraph& g_;
-  int64_t seed_;
-  int created_in_;
-  int num_tables_;
-  int num_actions_;
-  std::vector<int> priority_;
-  std::vector<std::unique_ptr<UctNode>> child_;
-  int num_visits_ = 0;
-  std::vector<int> tries_;
-  std::vector<double> acc_;
-  std::set<int> joined_;
-  std::set<int> recommended_;
-  std::vector<int> unjoined_;
-  std::vector<int> table_per_action_;
-  std::default_random_engine rng_;
+  std::vector<int> prefix_;
+  SplitMix rng_;
+  std::vector<Node> nodes_;
+  std::vector<Step> path_;
 };
 
 }  // namespace
 }  // namespace kg
 
-// UctJoinAgent (:779-838) split into Next (Act up to Execute) and Reward (the
-// statistics updates on the way back).
+// The agent behind the C ABI: kg_uct_next asks for an order, kg_uct_reward
+// credits it.
 struct kg_uct {
   kg::JoinGraph g;
-  int round = 0;
-  std::vector<int> order;
-  std::unique_ptr<kg::UctNode> root;
-  kg::UctPath path;
+  std::unique_ptr<kg::OrderSearch> search;
   bool pending = false;
 };
 
@@ -629,43 +632,41 @@ extern "C" int kg_uct_create(int32_t ntables, const uint8_t* adjacency, int64_t 
                              int32_t nprefix, kg_uct** out) {
   if (!out || ntables < 1 || ntables > 64 || nprefix < 0 || nprefix > ntables || (nprefix && !prefix))
     return KG_ERR_INVALID;
+  std::vector<int> pre;
+  uint64_t seen = 0;
+  for (int i = 0; i < nprefix; i++) {
+    if (prefix[i] < 0 || prefix[i] >= ntables || (seen >> prefix[i] & 1)) return KG_ERR_INVALID;
+    seen |= 1ull << prefix[i];
+    pre.push_back(prefix[i]);
+  }
   kg_uct* u = new (std::nothrow) kg_uct();
   if (!u) return KG_ERR_OOM;
   u->g.n = ntables;
-  u->g.adj.assign(static_cast<size_t>(ntables) * ntables, 0);
+  u->g.nbr.assign(ntables, 0);
   if (adjacency)
     for (int i = 0; i < ntables; i++)
       for (int j = 0; j < ntables; j++)
-        if (adjacency[i * ntables + j] || adjacency[j * ntables + i]) u->g.adj[i * ntables + j] = 1;
-  std::vector<int> joined;
-  for (int i = 0; i < nprefix; i++) {
-    if (prefix[i] < 0 || prefix[i] >= ntables || std::count(joined.begin(), joined.end(), prefix[i])) {
-      delete u;
-      return KG_ERR_INVALID;
-    }
-    joined.push_back(prefix[i]);
-  }
-  u->order.assign(ntables, 0);
-  for (int i = 0; i < nprefix; i++) u->order[i] = joined[i];
-  u->root = std::make_unique<UctNode>(u->round, u->g, joined, seed);
+        if (i != j && adjacency[i * ntables + j]) {
+          u->g.nbr[i] |= 1ull << j;
+          u->g.nbr[j] |= 1ull << i;
+        }
+  u->search = std::make_unique<OrderSearch>(u->g, pre, join_seed(seed));
   *out = u;
   return KG_OK;
 }
 
 extern "C" int kg_uct_next(kg_uct* u, int32_t* order) {
   if (!u || !order || u->pending) return KG_ERR_INVALID;
-  u->round++;
-  u->path.clear();
-  u->root->Select(u->round, u->order, u->path);
-  for (int i = 0; i < u->g.n; i++) order[i] = u->order[i];
+  std::vector<int> o;
+  u->search->next(o);
+  for (int i = 0; i < u->g.n; i++) order[i] = o[i];
   u->pending = true;
   return KG_OK;
 }
 
 extern "C" int kg_uct_reward(kg_uct* u, double reward) {
   if (!u || !u->pending || !(reward >= 0)) return KG_ERR_INVALID;
-  for (auto it = u->path.rbegin(); it != u->path.rend(); ++it) it->first->Update(it->second, reward);
-  u->path.clear();
+  u->search->reward(reward);
   u->pending = false;
   return KG_OK;
 }
