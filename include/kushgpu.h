@@ -145,6 +145,17 @@ int kg_ctx_set_deterministic(kg_ctx* ctx, int on);
  * most 1024), as kg_join_tune may choose. No counterpart in the reference (its
  * probes are per tuple). */
 int kg_ctx_set_probe_regions(kg_ctx* ctx, int32_t regions);
+/* Join builds (kg_ht_build) of many rows group their (key, row) pairs by
+ * table region first (histogram, scan, LDS-staged scatter), then build each
+ * region's slice of the table in one workgroup's LDS and write it back with
+ * coalesced stores, instead of one random CAS insert per row; the table's
+ * contents are the same (the first free slot from the home bucket on), and
+ * keys that overflow a region's last bucket are inserted directly afterwards.
+ * regions: -1 (default) builds by regions when at least 2^20 rows go into a
+ * table of at least 32 MB; 0 always inserts directly; > 0 uses at least that
+ * many regions (rounded up to a multiple of 8) for any build. No counterpart
+ * in the reference (its inserts are per tuple, runtime/hash_table.cc:18-27). */
+int kg_ctx_set_build_regions(kg_ctx* ctx, int32_t regions);
 /* The FP64 addition order of the last kg_scan_group_agg call that used the
  * fixed-order kernels: T = *nblocks * 256 lanes; lane t folds, from +0.0, the
  * qualifying rows of chunks c = t + k*T (k ascending) of *vec consecutive rows

@@ -46,6 +46,15 @@ int kg_fail(kg_ctx* ctx, int status, const char* fmt, ...) {
   return status;
 }
 
+int kg_allow_lds(kg_ctx* ctx, const void* kernel, size_t bytes) {
+  if (static_cast<int64_t>(bytes) > ctx->max_lds)
+    return kg_fail(ctx, KG_ERR_CAPACITY, "%zu bytes of LDS exceed the device's %lld", bytes,
+                   (long long)ctx->max_lds);
+  if (bytes <= 64 * 1024) return KG_OK;
+  KG_HIP_CHECK(ctx, hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+  return KG_OK;
+}
+
 int kg_work_reserve(kg_ctx* ctx, size_t bytes) {
   if (bytes <= ctx->work_bytes) return KG_OK;
   // Outstanding kernels may still read the old buffer.
@@ -145,7 +154,10 @@ int kg_ctx_create(int device, kg_ctx** out) {
     return KG_ERR_HIP;
   }
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->max_lds = static_cast<int64_t>(prop.sharedMemPerBlock);
+  }
   if (hipStreamCreate(&ctx->stream) != hipSuccess) {
     delete ctx;
     return KG_ERR_HIP;
@@ -209,6 +221,13 @@ int kg_ctx_set_probe_regions(kg_ctx* ctx, int32_t regions) {
   KG_REQUIRE(ctx, regions == -1 || regions == 0 || (regions % 8 == 0 && regions <= 1024),
              "probe regions must be -1, 0 or a multiple of 8 up to 1024 (got %d)", regions);
   ctx->probe_regions = regions;
+  return KG_OK;
+}
+
+int kg_ctx_set_build_regions(kg_ctx* ctx, int32_t regions) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, regions >= -1 && regions <= 32768, "build regions must be -1, 0 or up to 32768 (got %d)", regions);
+  ctx->build_regions = regions;
   return KG_OK;
 }
 

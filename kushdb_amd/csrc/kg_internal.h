@@ -19,6 +19,10 @@ struct kg_ctx {
   // directly, otherwise a multiple of 8 (kg_ctx_set_probe_regions).
   int probe_regions = -1;
   int probe_wg_per_cu = 0;    // probe workgroups per CU; 0 = 32
+  // Join builds: -1 chooses (region-partitioned for large builds into tables
+  // beyond the caches), 0 always inserts directly, > 0 at least that many
+  // regions (kg_ctx_set_build_regions).
+  int build_regions = -1;
   // Join tuning choices per (build rows, key cardinality, skew) class.
   struct JoinTune {
     uint64_t cls;
@@ -31,6 +35,7 @@ struct kg_ctx {
   // kg_agg_blocks_for(n) it defines that aggregation's FP64 addition order.
   int last_agg_vec = 1;
   int num_cus = 256;
+  int64_t max_lds = 64 * 1024;  // LDS bytes one workgroup may use (160 KB on gfx950)
   std::string err;
   // Grow-only device workspace (block partials, counters) and a small pinned
   // host buffer for scalar read-backs.
@@ -65,6 +70,10 @@ void kg_file_col_free(kg_ctx::FileCol& f);
 // Usage: kg_time_begin(ctx); launch...; kg_time_end(ctx);
 void kg_time_begin(kg_ctx* ctx);
 void kg_time_end(kg_ctx* ctx);
+
+// Lets `kernel` launch with up to `bytes` of dynamic LDS (beyond the 64 KB
+// default; at most ctx->max_lds).
+int kg_allow_lds(kg_ctx* ctx, const void* kernel, size_t bytes);
 
 // Sets ctx->err and returns the status.
 int kg_fail(kg_ctx* ctx, int status, const char* fmt, ...);

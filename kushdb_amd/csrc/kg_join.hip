@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 #include "kg_table.h"
 
@@ -61,9 +62,24 @@ __global__ __launch_bounds__(kJBlock) void k_key_range(kg_col key, const int32_t
     lo = a < lo ? a : lo;
     hi = c > hi ? c : hi;
   }
-  if ((threadIdx.x & (kWave - 1)) == 0 && lo <= hi) {
-    atomicMin(reinterpret_cast<long long*>(out), static_cast<long long>(lo));
-    atomicMax(reinterpret_cast<long long*>(out + 1), static_cast<long long>(hi));
+  // Across the workgroup's waves, then one atomic pair per workgroup (a pair
+  // per wave on one address serializes thousands of atomics).
+  __shared__ int64_t wlo[kJBlock / kWave], whi[kJBlock / kWave];
+  const int wave = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    wlo[wave] = lo;
+    whi[wave] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kJBlock / kWave; w++) {
+      lo = wlo[w] < lo ? wlo[w] : lo;
+      hi = whi[w] > hi ? whi[w] : hi;
+    }
+    if (lo <= hi) {
+      atomicMin(reinterpret_cast<long long*>(out), static_cast<long long>(lo));
+      atomicMax(reinterpret_cast<long long*>(out + 1), static_cast<long long>(hi));
+    }
   }
 }
 
@@ -449,27 +465,54 @@ __global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t* in, uint32_t*
   if (threadIdx.x == 1023) out[m] = part[1023];
 }
 
-// Scatter: each tile of kPartTile rows is counting-sorted by region in LDS and
+// Exclusive prefix sums of cnt[0..P) into off[0..P) by one kJBlock-thread
+// workgroup: each thread sums a run of consecutive counters, the run totals
+// are scanned across the wave (shuffles) and the waves (wtot in LDS), and each
+// thread writes its run's offsets. Ends with the workgroup synchronized.
+__device__ __forceinline__ void block_excl_scan(const uint32_t* cnt, uint32_t* off, int P, uint32_t* wtot) {
+  const int per = (P + kJBlock - 1) / kJBlock;
+  const int lo = min(static_cast<int>(threadIdx.x) * per, P), hi = min(lo + per, P);
+  uint32_t s = 0;
+  for (int q = lo; q < hi; q++) s += cnt[q];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  uint32_t incl = s;
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d);
+    if (lane >= d) incl += o;
+  }
+  if (lane == kWave - 1) wtot[wave] = incl;
+  __syncthreads();
+  uint32_t run = incl - s;
+  for (int w = 0; w < wave; w++) run += wtot[w];
+  for (int q = lo; q < hi; q++) {
+    off[q] = run;
+    run += cnt[q];
+  }
+  __syncthreads();
+}
+
+// Scatter: each tile of TILE rows is counting-sorted by region in LDS and
 // written out as one contiguous run per region (the workgroup's running cursor
 // per region starts at its scanned histogram offset).
-template <bool VALS, bool ROWS>
+template <bool VALS, bool ROWS, int TILE = kPartTile>
 __global__ __launch_bounds__(kJBlock) void k_part_scatter(TableView t, kg_col key, const int32_t* sel,
                                                            const int64_t* vals, int64_t n, int P,
                                                            const uint32_t* offs, int64_t* okeys, int64_t* ovals,
                                                            int32_t* orows) {
   extern __shared__ unsigned char smem[];
+  __shared__ uint32_t wtot[kJBlock / kWave];
   uint32_t* cursor = reinterpret_cast<uint32_t*>(smem);
   uint32_t* lcnt = cursor + P;
   uint32_t* loff = lcnt + P;
   int64_t* tkeys = reinterpret_cast<int64_t*>(loff + P + (P & 1));
-  int64_t* tvals = tkeys + kPartTile;
-  int32_t* trows = reinterpret_cast<int32_t*>(tvals + (VALS ? kPartTile : 0));
-  uint16_t* treg = reinterpret_cast<uint16_t*>(trows + (ROWS ? kPartTile : 0));
-  constexpr int kPer = kPartTile / kJBlock;
+  int64_t* tvals = tkeys + TILE;
+  int32_t* trows = reinterpret_cast<int32_t*>(tvals + (VALS ? TILE : 0));
+  uint16_t* treg = reinterpret_cast<uint16_t*>(trows + (ROWS ? TILE : 0));
+  constexpr int kPer = TILE / kJBlock;
   for (int q = threadIdx.x; q < P; q += kJBlock) cursor[q] = offs[static_cast<int64_t>(q) * gridDim.x + blockIdx.x];
   int64_t lo, hi;
   block_range(n, lo, hi);
-  for (int64_t t0 = lo; t0 < hi; t0 += kPartTile) {
+  for (int64_t t0 = lo; t0 < hi; t0 += TILE) {
     for (int q = threadIdx.x; q < P; q += kJBlock) lcnt[q] = 0;
     __syncthreads();
     int reg[kPer];
@@ -487,13 +530,7 @@ __global__ __launch_bounds__(kJBlock) void k_part_scatter(TableView t, kg_col ke
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t run = 0;
-      for (int q = 0; q < P; q++) {
-        loff[q] = run;
-        run += lcnt[q];
-      }
-    }
+    block_excl_scan(lcnt, loff, P, wtot);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kPer; j++) {
@@ -516,6 +553,140 @@ __global__ __launch_bounds__(kJBlock) void k_part_scatter(TableView t, kg_col ke
     __syncthreads();
     for (int q = threadIdx.x; q < P; q += kJBlock) cursor[q] += lcnt[q];
     __syncthreads();
+  }
+}
+
+// ---- region-partitioned build ---------------------------------------------------
+// For a large build, random CAS inserts into an HBM table cost two random
+// line round trips each (the key CAS, then the row id). Instead the (key, row)
+// pairs are first grouped by table region with the probe partitioner
+// (k_part_hist / scan / k_part_scatter: region = floor(bucket * P / buckets),
+// partitions XCD-major); then one workgroup per region builds the region's
+// slice of the table in LDS, one window of the slice at a time (W windows, each
+// at most the LDS budget): it copies the window into LDS, inserts the region's
+// keys whose home bucket lies in the window with LDS CAS (the same rule: first
+// free slot from the home bucket on), and writes the window back with
+// coalesced stores. A key that finds no free slot before its window's last
+// bucket is spilled and inserted afterwards by the global-atomic path, which
+// continues into the next window exactly as a direct insert would; so is a
+// wide table's free-marker key (it goes to the side list). The region's keys
+// are read once per window, from L2 after the first.
+constexpr int kRegionBlock = 1024;
+constexpr int kBuildTile = 4096;   // keys per scatter tile of a build
+constexpr int kItemBatch = 8;      // items per thread per step of the region kernel
+
+template <bool NARROW>
+__global__ __launch_bounds__(kRegionBlock) void k_build_region(TableView t, int P, int W, int nblk,
+                                                                const uint32_t* offs, const int64_t* pkeys,
+                                                                const int32_t* prows, int fresh,
+                                                                unsigned long long* inserted, int64_t* spill_keys,
+                                                                int32_t* spill_rows, unsigned long long* spill_count) {
+  using KeyT = typename std::conditional<NARROW, uint32_t, unsigned long long>::type;
+  constexpr int S = NARROW ? kSlots32 : kSlots;
+  const KeyT kFree = NARROW ? static_cast<KeyT>(kEmpty32) : static_cast<KeyT>(kEmpty);
+  extern __shared__ unsigned long long lds_words[];
+  const int q = blockIdx.x;
+  const int per = P / kXcds;
+  const uint64_t p = static_cast<uint64_t>(q % per) * kXcds + q / per;  // region of partition q
+  const uint64_t nb = NARROW ? nbuckets32(t) : t.nbuckets;
+  const uint64_t lo = (p * nb + P - 1) / P, hi = ((p + 1) * nb + P - 1) / P;
+  const int64_t wbk = static_cast<int64_t>((hi - lo + W - 1) / W);  // buckets per window
+  KeyT* lkey = reinterpret_cast<KeyT*>(lds_words);
+  int32_t* lrow = reinterpret_cast<int32_t*>(lkey + wbk * S);
+  KeyT* gkey = NARROW ? reinterpret_cast<KeyT*>(keys32(t)) : reinterpret_cast<KeyT*>(t.keys);
+  const int64_t a = offs[static_cast<int64_t>(q) * nblk], b = offs[static_cast<int64_t>(q + 1) * nblk];
+  for (int w = 0; w < W; w++) {
+    const int64_t wlo = static_cast<int64_t>(lo) + w * wbk;
+    const int64_t whi = min(wlo + wbk, static_cast<int64_t>(hi));
+    if (wlo >= whi) break;
+    const int64_t ns = (whi - wlo) * S;
+    const int64_t g0 = wlo * S;
+    // 16-byte transfers: a bucket is 64 bytes of keys, and S row ids
+    // (S * 4 bytes, a multiple of 16) start at a multiple of S.
+    {
+      uint4* lk4 = reinterpret_cast<uint4*>(lkey);
+      uint4* lr4 = reinterpret_cast<uint4*>(lrow);
+      const uint4* gk4 = reinterpret_cast<const uint4*>(gkey + g0);
+      const uint4* gr4 = reinterpret_cast<const uint4*>(t.rows + g0);
+      const int64_t nk4 = ns * static_cast<int64_t>(sizeof(KeyT)) / 16, nr4 = ns / 4;
+      const uint4 free4 = NARROW ? make_uint4(kEmpty32, kEmpty32, kEmpty32, kEmpty32)
+                                 : make_uint4(0u, 0x80000000u, 0u, 0x80000000u);
+      for (int64_t i = threadIdx.x; i < nk4; i += kRegionBlock) lk4[i] = fresh ? free4 : gk4[i];
+      for (int64_t i = threadIdx.x; i < nr4; i += kRegionBlock) lr4[i] = fresh ? make_uint4(0u, 0u, 0u, 0u) : gr4[i];
+    }
+    __syncthreads();
+    // kItemBatch items per thread per step, their loads issued together.
+    for (int64_t i0 = a + threadIdx.x; i0 < b; i0 += static_cast<int64_t>(kRegionBlock) * kItemBatch) {
+      int64_t kb[kItemBatch];
+      int32_t rb[kItemBatch];
+#pragma unroll
+      for (int u = 0; u < kItemBatch; u++) {
+        const int64_t i = i0 + static_cast<int64_t>(u) * kRegionBlock;
+        kb[u] = i < b ? pkeys[i] : 0;
+        rb[u] = i < b ? prows[i] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < kItemBatch; u++) {
+        if (i0 + static_cast<int64_t>(u) * kRegionBlock >= b) continue;
+        const int64_t k = kb[u];
+        bool ok;
+        KeyT kk;
+        if (NARROW) {
+          uint32_t k32;
+          ok = narrow_key(t, k, k32);
+          kk = static_cast<KeyT>(k32);
+        } else {
+          ok = k != kEmpty;
+          kk = static_cast<KeyT>(k);
+        }
+        const int64_t home = static_cast<int64_t>(bucket_of(k, nb));
+        // A key is inserted (or spilled) only by the window of its home
+        // bucket; keys no window can take are spilled by window 0.
+        if (ok ? (home < wlo || home >= whi) : w != 0) continue;
+        bool placed = false;
+        if (ok) {
+          for (int64_t bb = home - wlo; bb < whi - wlo && !placed; bb++) {
+            for (int s = 0; s < S; s++) {
+              KeyT* slot = lkey + bb * S + s;
+              if (*slot != kFree) continue;
+              if (atomicCAS(slot, kFree, kk) == kFree) {
+                lrow[bb * S + s] = rb[u];
+                placed = true;
+                break;
+              }
+            }
+          }
+        }
+        if (!placed) {
+          const unsigned long long pos = atomicAdd(spill_count, 1ull);
+          spill_keys[pos] = k;
+          spill_rows[pos] = rb[u];
+        }
+      }
+    }
+    __syncthreads();
+    {
+      const uint4* lk4 = reinterpret_cast<const uint4*>(lkey);
+      const uint4* lr4 = reinterpret_cast<const uint4*>(lrow);
+      uint4* gk4 = reinterpret_cast<uint4*>(gkey + g0);
+      uint4* gr4 = reinterpret_cast<uint4*>(t.rows + g0);
+      const int64_t nk4 = ns * static_cast<int64_t>(sizeof(KeyT)) / 16, nr4 = ns / 4;
+      for (int64_t i = threadIdx.x; i < nk4; i += kRegionBlock) gk4[i] = lk4[i];
+      for (int64_t i = threadIdx.x; i < nr4; i += kRegionBlock) gr4[i] = lr4[i];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && b > a) atomicAdd(inserted, static_cast<unsigned long long>(b - a));
+}
+
+// Inserts the spilled (key, row) pairs with the global-atomic path.
+__global__ __launch_bounds__(kJBlock) void k_build_spill(TableView t, const int64_t* keys, const int32_t* rows,
+                                                          const unsigned long long* count, int* overflow) {
+  const int64_t n = static_cast<int64_t>(*count);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kJBlock;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x; i < n; i += stride) {
+    if (t.narrow) insert_narrow(t, keys[i], rows[i], overflow);
+    else insert_wide(t, keys[i], rows[i], overflow);
   }
 }
 
@@ -910,8 +1081,8 @@ int choose_form(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t
                      1);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, rng + 1,
                      static_cast<unsigned long long>(INT64_MIN), 1);
-  hipLaunchKernelGGL(k_key_range, dim3(grid_for(ctx, nrows, 16)), dim3(kJBlock), 0, ctx->stream, *key, d_rows, nrows,
-                     rng);
+  hipLaunchKernelGGL(k_key_range, dim3(std::min(grid_for(ctx, nrows, 16), ctx->num_cus * 8)), dim3(kJBlock), 0,
+                     ctx->stream, *key, d_rows, nrows, rng);
   KG_LAUNCH_CHECK(ctx);
   int st = kg_pinned_reserve(ctx, 64);
   if (st != KG_OK) return st;
@@ -934,6 +1105,113 @@ int choose_form(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t
     return KG_OK;
   }
   if (lo < ht->base || static_cast<uint64_t>(hi) - static_cast<uint64_t>(ht->base) > limit) return widen(ctx, ht);
+  return KG_OK;
+}
+
+// Layout of a region-partitioned build: P regions (a multiple of 8) of W LDS
+// windows each; P = 0 inserts directly. Builds of fewer than 2^20 rows, or into
+// tables small enough for the L2 caches, insert directly. A window (8 B per
+// narrow slot, 12 B per wide slot) fits the LDS budget; W grows so that there
+// are at most about four regions per CU (fewer regions give longer runs in the
+// scatter, more windows per region cost the region kernel a pass over its keys
+// each).
+constexpr int64_t kRegionLdsBudget = 128 * 1024;
+
+struct BuildLayout {
+  int P = 0, W = 1;
+};
+
+BuildLayout build_layout(kg_ctx* ctx, const kg_hash_table* ht, int64_t nrows) {
+  BuildLayout L;
+  if (ctx->build_regions == 0 || (ctx->build_regions < 0 && nrows < (1 << 20))) return L;
+  const bool narrow = ht->form == 1;
+  const int S = narrow ? kSlots32 : kSlots;
+  const int64_t nb = static_cast<int64_t>(narrow ? ht->nbuckets / 2 : ht->nbuckets);
+  const int64_t slot_bytes = narrow ? 8 : 12;
+  if (ctx->build_regions < 0 && nb * S * slot_bytes < (32ll << 20)) return L;
+  const int64_t budget = std::min<int64_t>(kRegionLdsBudget, ctx->max_lds);
+  const int64_t max_wb = budget / (S * slot_bytes);  // buckets per window
+  const int64_t windows = ceil_div(nb, max_wb);
+  int64_t W = 1, P;
+  if (ctx->build_regions > 0) {
+    P = std::max<int64_t>(ctx->build_regions, ceil_div(windows, 8));
+    P = ceil_div(P, kXcds) * kXcds;
+    W = ceil_div(windows, P);
+  } else {
+    W = std::min<int64_t>(8, std::max<int64_t>(1, ceil_div(windows, 4 * ctx->num_cus)));
+    P = ceil_div(ceil_div(windows, W), kXcds) * kXcds;
+  }
+  // The scatter keeps 3 words per region in LDS beside its tile.
+  const int64_t max_p = (ctx->max_lds - (8 + kBuildTile * 14)) / 12 / kXcds * kXcds;
+  if (ctx->build_regions > 0) P = std::min(P, max_p);
+  if (P > max_p || P > nb || P < kXcds) return L;
+  W = ceil_div(windows, P);
+  if (ceil_div(ceil_div(nb, P), W) > max_wb) W++;
+  L.P = static_cast<int>(P);
+  L.W = static_cast<int>(W);
+  return L;
+}
+
+// The region-partitioned build (see k_build_region); the whole sequence is the
+// timed region.
+int region_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col& key, const int32_t* sel, int64_t n,
+                 const BuildLayout& L, bool fresh) {
+  const int P = L.P, W = L.W;
+  const bool narrow = ht->form == 1;
+  const int nblk = static_cast<int>(std::min<int64_t>(4096, std::max<int64_t>(kXcds, ceil_div(n, 4 * kBuildTile))));
+  const int64_t m = static_cast<int64_t>(P) * nblk;
+  size_t scan_tmp = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, static_cast<uint32_t*>(nullptr),
+                                         static_cast<uint32_t*>(nullptr), static_cast<int>(m + 1), ctx->stream);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o2 = off;
+    off += (bytes + 255) / 256 * 256;
+    return o2;
+  };
+  const size_t o_hist = carve((m + 1) * 4), o_offs = carve((m + 1) * 4), o_keys = carve(n * 8), o_rows = carve(n * 4),
+               o_skeys = carve(n * 8), o_srows = carve(n * 4), o_scnt = carve(8), o_tmp = carve(scan_tmp);
+  int st = kg_work_reserve(ctx, off);
+  if (st != KG_OK) return st;
+  char* w = static_cast<char*>(ctx->work);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(w + o_hist);
+  uint32_t* offs = reinterpret_cast<uint32_t*>(w + o_offs);
+  int64_t* pkeys = reinterpret_cast<int64_t*>(w + o_keys);
+  int32_t* prows = reinterpret_cast<int32_t*>(w + o_rows);
+  int64_t* skeys = reinterpret_cast<int64_t*>(w + o_skeys);
+  int32_t* srows = reinterpret_cast<int32_t*>(w + o_srows);
+  unsigned long long* scnt = reinterpret_cast<unsigned long long*>(w + o_scnt);
+  const size_t scatter_lds = static_cast<size_t>(3 * P + 1) * 4 + 8 + kBuildTile * 8 + kBuildTile * 4 + kBuildTile * 2;
+  const int S = narrow ? kSlots32 : kSlots;
+  const int64_t nb = static_cast<int64_t>(narrow ? ht->nbuckets / 2 : ht->nbuckets);
+  const size_t region_lds = static_cast<size_t>(ceil_div(ceil_div(nb, P), W)) * S * (narrow ? 8 : 12) + 64;
+  if ((st = kg_allow_lds(ctx, reinterpret_cast<const void*>(&k_part_scatter<false, true, kBuildTile>), scatter_lds)) !=
+      KG_OK)
+    return st;
+  const void* region_kernel =
+      narrow ? reinterpret_cast<const void*>(&k_build_region<true>) : reinterpret_cast<const void*>(&k_build_region<false>);
+  if ((st = kg_allow_lds(ctx, region_kernel, region_lds)) != KG_OK) return st;
+  const TableView v = ht->view();
+  kg_time_begin(ctx);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, scnt, 0ull, 1);
+  hipLaunchKernelGGL(k_fill_i32j, dim3(1), dim3(64), 0, ctx->stream, reinterpret_cast<int32_t*>(hist + m), 0, 1);
+  hipLaunchKernelGGL(k_part_hist, dim3(nblk), dim3(kJBlock), P * 4, ctx->stream, v, key, sel, n, P, hist);
+  size_t tb = scan_tmp;
+  if (hipcub::DeviceScan::ExclusiveSum(w + o_tmp, tb, hist, offs, static_cast<int>(m + 1), ctx->stream) != hipSuccess)
+    return kg_fail(ctx, KG_ERR_HIP, "build histogram scan failed");
+  hipLaunchKernelGGL((k_part_scatter<false, true, kBuildTile>), dim3(nblk), dim3(kJBlock), scatter_lds, ctx->stream, v,
+                     key, sel, static_cast<const int64_t*>(nullptr), n, P, offs, pkeys, static_cast<int64_t*>(nullptr),
+                     prows);
+  if (narrow)
+    hipLaunchKernelGGL(k_build_region<true>, dim3(P), dim3(kRegionBlock), region_lds, ctx->stream, v, P, W, nblk, offs,
+                       pkeys, prows, fresh ? 1 : 0, ht->counters, skeys, srows, scnt);
+  else
+    hipLaunchKernelGGL(k_build_region<false>, dim3(P), dim3(kRegionBlock), region_lds, ctx->stream, v, P, W, nblk,
+                       offs, pkeys, prows, fresh ? 1 : 0, ht->counters, skeys, srows, scnt);
+  hipLaunchKernelGGL(k_build_spill, dim3(ctx->num_cus), dim3(kJBlock), 0, ctx->stream, v, skeys, srows, scnt,
+                     reinterpret_cast<int*>(ht->counters + 2));
+  kg_time_end(ctx);
+  KG_LAUNCH_CHECK(ctx);
   return KG_OK;
 }
 
@@ -986,11 +1264,16 @@ extern "C" int kg_ht_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, co
   if (static_cast<int64_t>(inserted) + nrows > ht->capacity)
     return kg_fail(ctx, KG_ERR_CAPACITY, "table capacity %lld rows exceeded", (long long)ht->capacity);
   if ((st = choose_form(ctx, ht, key, d_rows, nrows)) != KG_OK) return st;
-  kg_time_begin(ctx);
-  hipLaunchKernelGGL(k_ht_build, dim3(grid_for(ctx, nrows, 4)), dim3(kJBlock), 0, ctx->stream, ht->view(), *key,
-                     d_rows, nrows, ht->counters, reinterpret_cast<int*>(ht->counters + 2));
-  kg_time_end(ctx);
-  KG_LAUNCH_CHECK(ctx);
+  const BuildLayout L = ht->form >= 0 ? build_layout(ctx, ht, nrows) : BuildLayout{};
+  if (L.P > 0) {
+    if ((st = region_build(ctx, ht, *key, d_rows, nrows, L, inserted == 0)) != KG_OK) return st;
+  } else {
+    kg_time_begin(ctx);
+    hipLaunchKernelGGL(k_ht_build, dim3(grid_for(ctx, nrows, 4)), dim3(kJBlock), 0, ctx->stream, ht->view(), *key,
+                       d_rows, nrows, ht->counters, reinterpret_cast<int*>(ht->counters + 2));
+    kg_time_end(ctx);
+    KG_LAUNCH_CHECK(ctx);
+  }
   unsigned long long overflow = 0;
   if ((st = read_counter(ctx, ht->counters + 2, &overflow)) != KG_OK) return st;
   if (overflow) return kg_fail(ctx, KG_ERR_CAPACITY, "join table overflow (side list or buckets full)");

@@ -142,6 +142,10 @@ class Context:
         self.check(self.lib.kg_file_index(self.h, str(path).encode(), key_type, ctypes.byref(h)))
         return ColumnIndex.from_handle(self, h)
 
+    def set_build_regions(self, regions):
+        """-1: region-partitioned join builds when large; 0: direct inserts; > 0: at least that many regions."""
+        self.check(self.lib.kg_ctx_set_build_regions(self.h, regions))
+
     def set_probe_regions(self, regions):
         """Region count of partitioned join probes (-1 auto, 0 direct)."""
         self.check(self.lib.kg_ctx_set_probe_regions(self.h, regions))

@@ -270,3 +270,60 @@ def test_join_tune_and_cache(ctx):
     s = ctx.join_tune(bc, Column(abi.KG_BIGINT, skewed))
     assert s["tuning_class"] != t["tuning_class"] and s["from_cache"] == 0
     ctx.set_probe_regions(0)
+
+
+@pytest.mark.parametrize("kind,regions,lf", [("narrow", -1, 0.5), ("narrow", 8192, 0.8), ("wide", -1, 0.5),
+                                             ("wide", 8192, 0.8), ("narrow_sel", -1, 0.35)])
+def test_region_build_equals_direct(kind, regions, lf):
+    """The region-partitioned build (kg_ctx_set_build_regions) against direct
+    inserts: 2.1M build rows with duplicate keys and NULLs, in two build calls
+    (the second into a non-empty table), narrow (keys within 2^32) and wide
+    (keys across int64, with the free-marker key INT64_MIN) forms, at the
+    default region size and with regions small enough (load factor 0.8) that
+    many keys run off their region's end and take the spill path. The join
+    pairs and the fused GROUP BY must be equal to the direct build's and the
+    pair set to the oracle's."""
+    rng = np.random.default_rng(sum(kind.encode()) + regions % 97)
+    nb = 2_100_000
+    if kind.startswith("narrow"):
+        keys = rng.integers(0, 1_500_000, nb) + 10**12
+    else:
+        keys = rng.integers(-2**63, 2**63 - 1, nb, dtype=np.int64)
+        keys[rng.integers(0, nb, 5)] = np.iinfo(np.int64).min
+        keys[: nb // 3] = keys[nb // 3: 2 * (nb // 3)]  # duplicates
+    nulls = (rng.random(nb) < 0.02).astype(np.int8)
+    probe = np.concatenate([keys[rng.integers(0, nb, 600_000)], rng.integers(-2**62, 2**62, 100_000)])
+    pv = rng.integers(0, 1 << 20, len(probe)).astype(np.int64)
+    bcol = Column(abi.KG_BIGINT, torch.as_tensor(keys).to(DEV), torch.as_tensor(nulls).to(DEV))
+    pcol, pvals = col64(probe), torch.as_tensor(pv).to(DEV)
+    sel = None
+    if kind == "narrow_sel":
+        sel = torch.as_tensor(np.nonzero(rng.random(nb) < 0.9)[0].astype(np.int32)).to(DEV)
+    results = []
+    for mode in (0, regions):
+        c = Context(0)
+        c.set_build_regions(mode)
+        ht = HashTable(c, nb, load_factor=lf)
+        rows = sel if sel is not None else torch.arange(nb, dtype=torch.int32, device=DEV)
+        half = rows.numel() // 2
+        print(f"region build {kind} mode={mode} lf={lf}", flush=True)
+        ht.insert(bcol, rows=rows[:half])
+        ht.insert(bcol, rows=rows[half:])
+        cnt = ht.probe_count(pcol)
+        gp, gb = ht.probe(pcol)
+        k, s, n, _ = ht.probe_agg(pcol, pvals, max_groups=nb)
+        agg = np.stack([x.cpu().numpy() for x in (k, s, n)], 1)
+        results.append((ht.size(), cnt, sorted_pairs(gp.cpu().numpy(), gb.cpu().numpy()),
+                        agg[np.lexsort((agg[:, 2], agg[:, 1], agg[:, 0]))]))
+        ht.free()
+        c.close()
+    (s0, c0, p0, a0), (s1, c1, p1, a1) = results
+    assert s0 == s1 == int((nulls[(sel.cpu().numpy() if sel is not None else np.arange(nb))] == 0).sum())
+    assert c0 == c1 and np.array_equal(p0, p1) and np.array_equal(a0, a1)
+    hs = sel.cpu().numpy() if sel is not None else np.arange(nb)
+    bmask = np.zeros(nb, dtype=bool)
+    bmask[hs] = True
+    bmask &= nulls == 0
+    op, ob = po.hash_join([po.Col(po.BIGINT, np.where(bmask, keys, 0), (~bmask).astype(np.int8))],
+                          [po.Col(po.BIGINT, probe)])
+    assert np.array_equal(p1, sorted_pairs(op, ob))
