@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--no-join", action="store_true")
     ap.add_argument("--build-rows", type=int, default=1 << 24)
     ap.add_argument("--probe-rows", type=int, default=1 << 28)
-    ap.add_argument("--theta", type=float, default=0.0)
+    ap.add_argument("--thetas", type=float, nargs="+", default=[0.0, 0.5, 1.0])
+    ap.add_argument("--cpu-probe-rows", type=int, default=1 << 25, help="probe rows of the C2 CPU baseline sample")
     ap.add_argument("--lf", type=float, default=0.0, help="join table load factor (0: autotuned)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-queries", type=int, default=512)
@@ -68,6 +69,7 @@ def roofline(bytes_per_launch, ms, traffic=None):
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
 # Kernels whose HBM traffic the bench reports, by a substring of their name.
 TRAFFIC_KERNELS = {"k_group_agg_v4": "k_group_agg_v4<", "k_ht_build": "k_ht_build(",
+                   "k_join_group_agg": "k_join_group_agg<",
                    "k_ht_probe_count": "k_ht_probe<0,", "k_ht_probe_agg": "k_ht_probe<2,",
                    "k_batch_shared": "k_batch_shared(", "k_batch_join": "k_batch_join("}
 COPY_BYTES = 1 << 30
@@ -191,14 +193,19 @@ def cpu_baseline_scan_agg(t, nrows):
     return n / dt, dt, n
 
 
-def bench_join(ctx, a):
+def bench_join_theta(ctx, a, theta, cpu):
+    """C2 at one Zipf skew: build (16M unique int64 keys), probe counting the
+    join cardinality, probe emitting every (probe row, build row) pair, and
+    the probe fused with GROUP BY the matched build key (SUM(int64), COUNT(*)),
+    each at full size. With cpu, the oracle's chained join table timed on the
+    host over the full build and a probe sample."""
     import torch
     from kushdb_amd import abi
     from kushdb_amd.runtime import Column, HashTable
     from kushdb_amd.workloads import join_tables
     nb, npr = a.build_rows, a.probe_rows
-    log(f"join: generating build {nb} / probe {npr} (theta={a.theta})")
-    bk, pay, pk, pv = join_tables(nb, npr, theta=a.theta, device="cuda:0")
+    log(f"join: generating build {nb} / probe {npr} (theta={theta})")
+    bk, pay, pk, pv = join_tables(nb, npr, theta=theta, device="cuda:0")
     bcol, pcol = Column(abi.KG_BIGINT, bk), Column(abi.KG_BIGINT, pk)
     out = {}
     lf = a.lf
@@ -206,7 +213,7 @@ def bench_join(ctx, a):
         tune = ctx.join_tune(bcol, pcol)
         lf = tune["load_factor"]
         out["join_tuning"] = tune
-        log(f"join tuning: {tune}")
+        log(f"join tuning (theta={theta}): {tune}")
     steps = max(1, min(a.steps, 5))
     # build
     build_ms = []
@@ -223,8 +230,10 @@ def bench_join(ctx, a):
         if i < steps:
             ht.free()
     bms = sum(build_ms) / len(build_ms)
-    out["join_build"] = {"rows": nb, "load_factor": lf, "kernel_ms": round(bms, 3), "rows_per_s": nb / (bms * 1e-3),
-                         "roofline": roofline(nb * BUILD_BYTES_PER_ROW, bms, measured_traffic("k_ht_build"))}
+    out["join_build"] = {"rows": nb, "theta": theta, "load_factor": lf, "kernel_ms": round(bms, 3),
+                         "rows_per_s": nb / (bms * 1e-3),
+                         "roofline": roofline(nb * BUILD_BYTES_PER_ROW, bms,
+                                              measured_traffic("k_ht_build") if theta == 0 else None)}
     # probe (join cardinality)
     ht.probe_count(pcol)
     ctx.set_timing(True)
@@ -236,9 +245,26 @@ def bench_join(ctx, a):
     kms, launches = ctx.kernel_stats(reset=True)
     ctx.set_timing(False)
     pms = kms / launches
-    out["join_probe"] = {"rows": npr, "theta": a.theta, "matches": card, "kernel_ms": round(pms, 3),
+    out["join_probe"] = {"rows": npr, "theta": theta, "matches": card, "kernel_ms": round(pms, 3),
                          "ms_per_step": round(wall / steps * 1e3, 3), "rows_per_s": npr / (pms * 1e-3),
-                         "roofline": roofline(npr * PROBE_BYTES_PER_ROW, pms, measured_traffic("k_ht_probe_count"))}
+                         "roofline": roofline(npr * PROBE_BYTES_PER_ROW, pms,
+                                              measured_traffic("k_ht_probe_count") if theta == 0 else None)}
+    # probe emitting the pairs (two int32 row ids per match)
+    op = torch.empty(card, dtype=torch.int32, device="cuda:0")
+    ob = torch.empty(card, dtype=torch.int32, device="cuda:0")
+    ht.probe(pcol, out_cap=card, out=(op, ob))
+    ctx.set_timing(True)
+    ctx.kernel_stats(reset=True)
+    for _ in range(steps):
+        gp, _ = ht.probe(pcol, out_cap=card, out=(op, ob))
+    kms, launches = ctx.kernel_stats(reset=True)
+    ctx.set_timing(False)
+    ems = kms / launches
+    assert gp.numel() == card
+    out["join_probe_emit"] = {"rows": npr, "theta": theta, "pairs": card, "kernel_ms": round(ems, 3),
+                              "rows_per_s": npr / (ems * 1e-3),
+                              "roofline": roofline(npr * PROBE_BYTES_PER_ROW + card * 8, ems)}
+    del op, ob, gp
     # probe fused with GROUP BY matched key
     ht.probe_agg(pcol, pv, max_groups=nb)
     ctx.set_timing(True)
@@ -248,13 +274,41 @@ def bench_join(ctx, a):
     kms, launches = ctx.kernel_stats(reset=True)
     ctx.set_timing(False)
     ams = kms / launches
-    out["join_probe_agg"] = {"rows": npr, "groups": int(k.numel()), "kernel_ms": round(ams, 3),
+    groups = int(k.numel())
+    # SURVEY cost model: matched rows (key 8 + value 8 + 64 B update), every
+    # probe row's key and bucket (8 + 64), 24 B per group out.
+    agg_bytes = npr * PROBE_BYTES_PER_ROW + card * (8 + 64) + groups * 24
+    out["join_probe_agg"] = {"rows": npr, "theta": theta, "groups": groups, "kernel_ms": round(ams, 3),
                              "rows_per_s": npr / (ams * 1e-3),
-                             "roofline": roofline(npr * PROBE_AGG_BYTES_PER_ROW, ams,
-                                                  measured_traffic("k_ht_probe_agg"))}
+                             "roofline": roofline(agg_bytes, ams,
+                                                  measured_traffic("k_ht_probe_agg") if theta == 0 else None)}
     ht.free()
+    if cpu:
+        from oracle import pyoracle as po
+        ns = min(a.cpu_probe_rows, npr)
+        hb, hp, hv = bk.cpu().numpy(), pk[:ns].cpu().numpy(), pv[:ns].cpu().numpy()
+        log(f"cpu baseline C2 (oracle, 1 thread): build {nb}, probe sample {ns}")
+        m, _, bs, ps = po.time_join(hb, hp, hv)
+        out["cpu_baseline"] = {"value": ns / ps, "unit": "probe rows/s", "cores": 1, "kind": "port",
+                               "build_rows_per_s": nb / bs,
+                               "sample": f"the full {nb}-key build ({bs:.2f} s) and the first {ns} probe rows "
+                                         f"({ps:.2f} s, {m} matches) of the same tables, chained join table as in "
+                                         f"runtime/hash_table.cc, {platform.processor() or platform.machine()}"}
     del bk, pay, pk, pv
     torch.cuda.empty_cache()
+    return out
+
+
+def bench_join(ctx, a):
+    """C2 at theta 0 (the keys the earlier records use) and every other skew
+    in --thetas."""
+    out = {}
+    for theta in a.thetas:
+        r = bench_join_theta(ctx, a, theta, cpu=not a.no_cpu and theta == a.thetas[0])
+        if theta == a.thetas[0]:
+            out.update(r)
+        else:
+            out[f"c2_theta_{theta}"] = r
     return out
 
 
@@ -297,10 +351,30 @@ def bench_star_c0(ctx, a):
     ctx.set_timing(False)
     # timed kernels per query: the dim build and the fused probe kernel
     per_query_kernel_ms = kms / steps
-    return {"workload": "1M-row fact (30% day range) join 10k-row dim, GROUP BY 25 cats: SUM(price*qty), COUNT(*)",
-            "groups": int(len(codes)), "query_ms": round(wall * 1e3, 4), "kernel_ms": round(per_query_kernel_ms, 4),
-            "rows_per_s": 1_000_000 / wall,
-            "roofline": roofline(C0_BYTES, per_query_kernel_ms)}
+    out = {"workload": "1M-row fact (30% day range) join 10k-row dim, GROUP BY 25 cats: SUM(price*qty), COUNT(*)",
+           "groups": int(len(codes)), "query_ms": round(wall * 1e3, 4), "kernel_ms": round(per_query_kernel_ms, 4),
+           "rows_per_s": 1_000_000 / wall,
+           "roofline": roofline(C0_BYTES, per_query_kernel_ms, measured_traffic("k_join_group_agg"))}
+    if not a.no_cpu:
+        import numpy as np
+        from oracle import pyoracle as po
+        h = {k: v.cpu().numpy() for k, v in f.items()}
+        hd = {k: v.cpu().numpy() for k, v in d.items()}
+        oc, osum, ocnt = np.zeros(64, np.int32), np.zeros(64), np.zeros(64, np.int64)
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            g = po.lib().kor_q_star(len(h["fk"]), h["fk"].ctypes.data, h["qty"].ctypes.data, h["price"].ctypes.data,
+                                    h["day"].ctypes.data, lo, hi, len(hd["pk"]), hd["pk"].ctypes.data,
+                                    hd["cat"].ctypes.data, 64, oc.ctypes.data, osum.ctypes.data, ocnt.ctypes.data)
+            times.append(time.perf_counter() - t0)
+        assert g == len(codes) and np.array_equal(np.sort(ocnt[:g]), np.sort(vals[:, 1]))
+        ct = sorted(times)[2]
+        out["cpu_baseline"] = {"value": 1_000_000 / ct, "unit": "fact rows/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole query (1M fact rows, 10k dim rows), median of 5 runs, "
+                                         f"{ct * 1e3:.1f} ms, scan-select + chained hash join + aggregate "
+                                         f"table in the reference's orders"}
+    return out
 
 
 def bench_chain_c3(ctx, a):
@@ -331,7 +405,21 @@ def bench_chain_c3(ctx, a):
     out = {"workload": "5-table chain join (200k..50M rows), TEXT prefix + LIKE + int filters, device indexes, "
                        "tuple dedup, MIN(text), MIN(int)", "order": order, "stats": stats,
            "query_ms": round(wall * 1e3, 3), "rows_per_s": n_in / wall,
-           "min_int": mn_int, "roofline": roofline(alg, wall * 1e3)}
+           "min_int": mn_int, "roofline": roofline(alg, wall * 1e3, c3_traffic())}
+    if not a.no_cpu:
+        # The oracle's chain query on the same full-size tables: its result is
+        # also the full-size check of the device result.
+        from oracle import pyoracle as po
+        log("cpu baseline C3 (oracle, 1 thread)")
+        t0 = time.perf_counter()
+        cn, cname, cvmin = po.chain_query(htabs)
+        ct = time.perf_counter() - t0
+        as_bytes = lambda x: x.encode() if isinstance(x, str) else (bytes(x) if x is not None else None)
+        same = cn == tuples.shape[0] and cvmin == mn_int and as_bytes(cname) == as_bytes(mn_text)
+        out["cpu_baseline"] = {"value": n_in / ct, "unit": "input rows/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole query over the same {n_in} rows, {ct:.2f} s: filters, four "
+                                         f"left-deep chained hash joins, MIN; result equal to the device's: {same}"}
+        out["oracle_equal"] = bool(same)
     # The same query with the join order chosen per episode by the UCT agent (kg_skinner_join).
     from kushdb_amd.chain import skinner_chain_query
     for budget in (1 << 20, 1 << 22):
@@ -350,6 +438,18 @@ def bench_chain_c3(ctx, a):
         log(f"C3 skinner budget {budget}: {swall * 1e3:.2f} ms, {sst['episodes']} episodes, "
             f"{sst['distinct_orders']} orders, last {sst['last_order']}")
     return out
+
+
+def c3_traffic():
+    """HBM bytes of one C3 chain query: every dispatch of that query, FETCH_SIZE
+    + WRITE_SIZE scaled by the copy calibration (profiles/traffic.json), or
+    None when not measured."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            v = json.load(f).get("queries", {}).get("c3_chain", {}).get("bytes_per_query")
+            return int(v) if v is not None else None
+    except (OSError, ValueError):
+        return None
 
 
 def bench_serving(ctx, a):

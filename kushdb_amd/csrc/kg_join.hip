@@ -98,14 +98,119 @@ __global__ __launch_bounds__(kJBlock) void k_widen(TableView t, const uint32_t* 
 enum ProbeMode { kCount = 0, kEmit = 1, kAgg = 2 };
 
 struct ProbeOut {
-  unsigned long long* count;   // kCount / kEmit: total pairs
+  unsigned long long* count;   // kCount / kEmit: total pairs (kEmit: set to the workgroup's LDS cursor)
   int32_t* out_probe;          // kEmit
   int32_t* out_build;          // kEmit
   int64_t cap;                 // kEmit
   Acc* acc;                    // kAgg: per-slot accumulators (+1 side slot at the end)
   int32_t* first;              // kAgg: per-slot first probe row (optional)
   const int64_t* vals;         // kAgg: probe-side values
+  unsigned long long* block_counts;      // kCount: per-workgroup totals instead of *count
+  const unsigned long long* block_offs;  // kEmit: each workgroup's first output position
+  // kAgg: the hot keys (device list) and, in the kernel, their LDS state.
+  const int64_t* hot_keys;    // kAgg: device list of hot probe keys (k_hot_keys), or NULL
+  const int32_t* hot_n;       // its length
+  int32_t nhot;
+  const int16_t* hidx;
+  const int64_t* hkey;
+  int64_t* hslot;
+  unsigned long long* hsum;
+  unsigned long long* hcnt;
+  int32_t* hfirst;
 };
+
+// kAgg under skew: the hot probe keys (found by k_hot_keys from a sample,
+// at most kHot) are accumulated per workgroup in LDS and added to the global
+// accumulators once when the workgroup ends; every other key updates the
+// global accumulators directly. Under Zipf theta = 1 about 6% of all probes
+// carry one key, and their device atomics on one slot would serialize.
+constexpr int kHot = 256;
+constexpr int kHotHash = 512;
+constexpr int kHotSample = 1 << 16;   // probe keys sampled by k_hot_keys
+constexpr int kHotTable = 4096;       // distinct sampled keys counted
+constexpr int kHotMinCount = 3;       // sampled occurrences that make a key hot
+
+__device__ __forceinline__ uint32_t hot_pos(int64_t k) {
+  return static_cast<uint32_t>(mix64(static_cast<uint64_t>(k)) >> 32) & (kHotHash - 1);
+}
+
+// One workgroup: kHotSample probe keys at an even stride counted in an LDS
+// open-addressing table (a key is claimed by a 64-bit CAS of the free marker;
+// keys that find no entry within 64 probes are not counted, nor is the free
+// marker itself); keys seen at least kHotMinCount times become the hot list.
+__global__ __launch_bounds__(1024) void k_hot_keys(kg_col key, const int32_t* sel, int64_t n, int64_t* hot,
+                                                   int32_t* nhot) {
+  __shared__ unsigned long long tkey[kHotTable];
+  __shared__ uint32_t tcnt[kHotTable];
+  __shared__ int32_t out_n;
+  for (int i = threadIdx.x; i < kHotTable; i += 1024) {
+    tkey[i] = static_cast<unsigned long long>(kEmpty);
+    tcnt[i] = 0;
+  }
+  if (threadIdx.x == 0) out_n = 0;
+  __syncthreads();
+  const int64_t stride = n > kHotSample ? n / kHotSample : 1;
+  for (int64_t s = threadIdx.x; s < kHotSample && s * stride < n; s += 1024) {
+    const int64_t r = sel ? sel[s * stride] : s * stride;
+    int64_t k;
+    if (!load_key(key, r, k) || k == kEmpty) continue;
+    uint32_t p = static_cast<uint32_t>(mix64(static_cast<uint64_t>(k))) & (kHotTable - 1);
+    for (int probes = 0; probes < 64; probes++) {
+      const unsigned long long prev = atomicCAS(&tkey[p], static_cast<unsigned long long>(kEmpty),
+                                                static_cast<unsigned long long>(k));
+      if (prev == static_cast<unsigned long long>(kEmpty) || prev == static_cast<unsigned long long>(k)) {
+        atomicAdd(&tcnt[p], 1u);
+        break;
+      }
+      p = (p + 1) & (kHotTable - 1);
+    }
+  }
+  // The most frequent keys: a histogram of floor(log2(count)) picks the
+  // smallest power-of-two threshold that admits at most kHot keys.
+  __shared__ uint32_t lhist[32];
+  __shared__ uint32_t thresh;
+  if (threadIdx.x < 32) lhist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHotTable; i += 1024)
+    if (tcnt[i] >= kHotMinCount) atomicAdd(&lhist[31 - __clz(tcnt[i])], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t above = 0, b = 32;
+    while (b > 0 && above + lhist[b - 1] <= kHot) above += lhist[--b];
+    thresh = b >= 32 ? 0xFFFFFFFFu : (1u << b);
+    if (thresh < kHotMinCount) thresh = kHotMinCount;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHotTable; i += 1024) {
+    if (tcnt[i] < thresh) continue;
+    const int j = atomicAdd(&out_n, 1);
+    if (j < kHot) hot[j] = static_cast<int64_t>(tkey[i]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *nhot = out_n < kHot ? out_n : kHot;
+}
+
+template <bool FIRST>
+__device__ __forceinline__ void agg_add(const ProbeOut& o, int64_t slot, int64_t k, unsigned long long sum,
+                                        unsigned n, int32_t r) {
+  if (o.nhot > 0) {
+    for (uint32_t p = hot_pos(k);; p = (p + 1) & (kHotHash - 1)) {
+      const int j = o.hidx[p];
+      if (j < 0) break;
+      if (o.hkey[j] == k) {
+        o.hslot[j] = slot;
+        atomicAdd(&o.hsum[j], sum);
+        atomicAdd(&o.hcnt[j], static_cast<unsigned long long>(n));
+        if (FIRST) atomicMin(&o.hfirst[j], r);
+        return;
+      }
+    }
+  }
+  Acc* a = &o.acc[slot];
+  atomicAdd(&a->sum, sum);
+  atomicAdd(&a->count, static_cast<unsigned long long>(n));
+  if (FIRST) atomicMin(&o.first[slot], r);
+}
 
 // Wave-aggregated append of `want` items per lane; returns this lane's base.
 __device__ __forceinline__ unsigned long long wave_append(unsigned long long* counter, unsigned want) {
@@ -201,12 +306,9 @@ __device__ __forceinline__ void probe_one(const TableView& t, const ProbeOut& o,
       b = (b + 1 == t.nbuckets) ? 0 : b + 1;
     }
   }
-  if (MODE == kAgg && sub == 0 && nmatch) {
-    Acc* a = &o.acc[first_slot];
-    atomicAdd(&a->sum, static_cast<unsigned long long>(o.vals[vi]) * nmatch);
-    atomicAdd(&a->count, static_cast<unsigned long long>(nmatch));
-    if (FIRST) atomicMin(&o.first[first_slot], static_cast<int32_t>(r));
-  }
+  if (MODE == kAgg && sub == 0 && nmatch)
+    agg_add<FIRST>(o, first_slot, k, static_cast<unsigned long long>(o.vals[vi]) * nmatch, nmatch,
+                   static_cast<int32_t>(r));
 }
 
 // The same for a narrow table: each lane of the group holds 4 of the bucket's
@@ -272,12 +374,9 @@ __device__ __forceinline__ void probe_one32(const TableView& t, const ProbeOut& 
       b = (b + 1 == nb) ? 0 : b + 1;
     }
   }
-  if (MODE == kAgg && sub == 0 && nmatch) {
-    Acc* a = &o.acc[first_slot];
-    atomicAdd(&a->sum, static_cast<unsigned long long>(o.vals[vi]) * nmatch);
-    atomicAdd(&a->count, static_cast<unsigned long long>(nmatch));
-    if (FIRST) atomicMin(&o.first[first_slot], static_cast<int32_t>(r));
-  }
+  if (MODE == kAgg && sub == 0 && nmatch)
+    agg_add<FIRST>(o, first_slot, static_cast<int64_t>(static_cast<uint64_t>(t.base) + k32),
+                   static_cast<unsigned long long>(o.vals[vi]) * nmatch, nmatch, static_cast<int32_t>(r));
 }
 
 // Probe kernel: every 4-lane group handles kUnroll probes per wave step; the
@@ -301,14 +400,15 @@ struct PartIn {
   int32_t P;
 };
 
-template <int MODE, bool FIRST, bool NARROW>
+template <int MODE, bool FIRST, bool NARROW, int UNROLL = probe_unroll<MODE>()>
 __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, const int32_t* sel, int64_t n,
                                                        ProbeOut o, PartIn part) {
+  // (o is this workgroup's copy: the emit cursor and the cache live in LDS.)
   const int lane = threadIdx.x & (kWave - 1);
   const int sub = lane & (kProbeLanes - 1);
   const int grp = lane / kProbeLanes;
   constexpr int kGroups = kWave / kProbeLanes;
-  constexpr int kUnroll = probe_unroll<MODE>();
+  constexpr int kUnroll = UNROLL;
   constexpr int kStep = kGroups * kUnroll;
   int64_t lo = 0, hi = n, gwave, nwaves;
   if (part.offs) {
@@ -323,6 +423,44 @@ __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, c
     nwaves = static_cast<int64_t>(gridDim.x) * kJBlock / kWave;
   }
   unsigned long long local = 0;
+  // Workgroup-local state: the emit cursor, the count partials, the
+  // accumulator cache.
+  __shared__ unsigned long long s_cur;
+  __shared__ unsigned long long s_wsum[kJBlock / kWave];
+  constexpr int kH = MODE == kAgg ? kHot : 1, kHH = MODE == kAgg ? kHotHash : 1;
+  __shared__ int16_t s_hidx[kHH];
+  __shared__ int64_t s_hkey[kH], s_hslot[kH];
+  __shared__ unsigned long long s_hsum[kH], s_hcnt[kH];
+  __shared__ int32_t s_hfirst[kH];
+  if (MODE == kEmit && o.block_offs) {
+    if (threadIdx.x == 0) s_cur = o.block_offs[blockIdx.x];
+    o.count = &s_cur;
+  }
+  if (MODE == kAgg) {
+    o.nhot = o.hot_keys ? *o.hot_n : 0;
+    for (int e = threadIdx.x; e < kHH; e += kJBlock) s_hidx[e] = -1;
+    for (int j = threadIdx.x; j < o.nhot; j += kJBlock) {
+      s_hkey[j] = o.hot_keys[j];
+      s_hslot[j] = -1;
+      s_hsum[j] = 0;
+      s_hcnt[j] = 0;
+      s_hfirst[j] = INT32_MAX;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)  // the hot keys are distinct: plain linear probing
+      for (int j = 0; j < o.nhot; j++) {
+        uint32_t p = hot_pos(s_hkey[j]);
+        while (s_hidx[p] >= 0) p = (p + 1) & (kHotHash - 1);
+        s_hidx[p] = static_cast<int16_t>(j);
+      }
+    o.hidx = s_hidx;
+    o.hkey = s_hkey;
+    o.hslot = s_hslot;
+    o.hsum = s_hsum;
+    o.hcnt = s_hcnt;
+    o.hfirst = s_hfirst;
+  }
+  __syncthreads();
   // Keys of the next step are loaded while this step's buckets are in flight.
   int64_t nr[kUnroll], nv[kUnroll], nk[kUnroll];
   bool nact[kUnroll];
@@ -392,7 +530,28 @@ __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, c
   }
   if (MODE == kCount) {
     for (int s = kWave / 2; s >= 1; s >>= 1) local += __shfl_xor(local, s);
-    if (lane == 0 && local) atomicAdd(o.count, local);
+    if (o.block_counts) {
+      if (lane == 0) s_wsum[threadIdx.x / kWave] = local;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kJBlock / kWave; w++) t += s_wsum[w];
+        o.block_counts[blockIdx.x] = t;
+      }
+    } else if (lane == 0 && local) {
+      atomicAdd(o.count, local);
+    }
+  }
+  if (MODE == kAgg) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < o.nhot; j += kJBlock) {
+      const int64_t slot = s_hslot[j];
+      if (slot < 0 || s_hcnt[j] == 0) continue;
+      Acc* a = &o.acc[slot];
+      atomicAdd(&a->sum, s_hsum[j]);
+      atomicAdd(&a->count, s_hcnt[j]);
+      if (FIRST) atomicMin(&o.first[slot], s_hfirst[j]);
+    }
   }
 }
 
@@ -991,6 +1150,7 @@ struct kg_hash_table {
   int64_t* keys = nullptr;
   int32_t* rows = nullptr;
   Acc* acc = nullptr;          // lazily allocated for probe_agg
+  int64_t* hot = nullptr;      // probe_agg: hot probe keys (kHot) + their count, lazily allocated
   int32_t* first = nullptr;    // lazily allocated for probe_agg with first rows
   int32_t* side_rows = nullptr;
   unsigned long long* counters = nullptr;  // [0] inserted rows, [1] side count, [2] overflow flag, [3] scratch,
@@ -1296,15 +1456,55 @@ int probe_grid(kg_ctx* ctx, int64_t n) {
   return static_cast<int>(std::min<int64_t>(g, static_cast<int64_t>(ctx->num_cus) * per_cu));
 }
 
-template <int MODE, bool FIRST>
+template <int MODE, bool FIRST, int UNROLL = probe_unroll<MODE>()>
 void launch_probe_kernel(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const int32_t* sel, int64_t n,
                          const ProbeOut& o, const PartIn& part, int grid) {
   if (ht->form == 1)
-    hipLaunchKernelGGL((k_ht_probe<MODE, FIRST, true>), dim3(grid), dim3(kJBlock), 0, ctx->stream, ht->view(), key,
-                       sel, n, o, part);
+    hipLaunchKernelGGL((k_ht_probe<MODE, FIRST, true, UNROLL>), dim3(grid), dim3(kJBlock), 0, ctx->stream, ht->view(),
+                       key, sel, n, o, part);
   else
-    hipLaunchKernelGGL((k_ht_probe<MODE, FIRST, false>), dim3(grid), dim3(kJBlock), 0, ctx->stream, ht->view(), key,
-                       sel, n, o, part);
+    hipLaunchKernelGGL((k_ht_probe<MODE, FIRST, false, UNROLL>), dim3(grid), dim3(kJBlock), 0, ctx->stream,
+                       ht->view(), key, sel, n, o, part);
+}
+
+// One probe pass over the given (possibly partitioned) keys. An emitting
+// probe runs twice over the same workgroup layout: counting first (one total
+// per workgroup), then, after an exclusive scan of the totals, emitting each
+// workgroup's pairs from its own offset through an LDS cursor, so no pair
+// goes through a device-wide atomic counter; the grand total goes to *o.count.
+__global__ void k_copy_u64(const unsigned long long* src, unsigned long long* dst) { *dst = *src; }
+
+template <int MODE, bool FIRST>
+int run_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const int32_t* sel, int64_t n, ProbeOut o,
+              const PartIn& part, int grid, char* scratch) {
+  if (MODE != kEmit) {
+    launch_probe_kernel<MODE, FIRST>(ctx, ht, key, sel, n, o, part, grid);
+    return KG_OK;
+  }
+  unsigned long long* bc = reinterpret_cast<unsigned long long*>(scratch);
+  unsigned long long* bo = bc + grid + 1;
+  void* tmp = bo + grid + 1;
+  size_t tb = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, bc, bo, grid + 1, ctx->stream);
+  ProbeOut oc{};
+  oc.block_counts = bc;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, bc + grid, 0ull, 1);
+  // The same rows per workgroup as the emitting pass: the same unroll.
+  launch_probe_kernel<kCount, false, probe_unroll<kEmit>()>(ctx, ht, key, sel, n, oc, part, grid);
+  if (hipcub::DeviceScan::ExclusiveSum(tmp, tb, bc, bo, grid + 1, ctx->stream) != hipSuccess)
+    return kg_fail(ctx, KG_ERR_HIP, "probe count scan failed");
+  hipLaunchKernelGGL(k_copy_u64, dim3(1), dim3(1), 0, ctx->stream, bo + grid, o.count);
+  o.block_offs = bo;
+  launch_probe_kernel<kEmit, FIRST>(ctx, ht, key, sel, n, o, part, grid);
+  return KG_OK;
+}
+
+// Device scratch bytes run_probe needs for `grid` workgroups.
+size_t run_probe_scratch(kg_ctx* ctx, int grid) {
+  size_t tb = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, static_cast<unsigned long long*>(nullptr),
+                                         static_cast<unsigned long long*>(nullptr), grid + 1, ctx->stream);
+  return (static_cast<size_t>(grid) + 1) * 16 + tb + 256;
 }
 
 // Probes n keys (rows sel[i] or i), directly or through a region partition of
@@ -1314,10 +1514,25 @@ template <int MODE, bool FIRST>
 int launch_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const int32_t* sel, int64_t n,
                  ProbeOut o) {
   const int P = probe_regions(ctx);
+  // kAgg: the hot-key sample runs first, inside the timed region.
+  auto hot_pass = [&]() {
+    if (MODE == kAgg && o.hot_keys)
+      hipLaunchKernelGGL(k_hot_keys, dim3(1), dim3(1024), 0, ctx->stream, key, sel, n,
+                         const_cast<int64_t*>(o.hot_keys), const_cast<int32_t*>(o.hot_n));
+  };
   if (P == 0) {
+    const int grid = probe_grid(ctx, n);
+    char* scratch = nullptr;
+    if (MODE == kEmit) {
+      const int st = kg_work_reserve(ctx, run_probe_scratch(ctx, grid));
+      if (st != KG_OK) return st;
+      scratch = static_cast<char*>(ctx->work);
+    }
     kg_time_begin(ctx);
-    launch_probe_kernel<MODE, FIRST>(ctx, ht, key, sel, n, o, PartIn{}, probe_grid(ctx, n));
+    hot_pass();
+    const int st = run_probe<MODE, FIRST>(ctx, ht, key, sel, n, o, PartIn{}, grid, scratch);
     kg_time_end(ctx);
+    if (st != KG_OK) return st;
     KG_LAUNCH_CHECK(ctx);
     return KG_OK;
   }
@@ -1331,8 +1546,10 @@ int launch_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const 
     off += (bytes + 255) / 256 * 256;
     return o2;
   };
+  const int grid = (probe_grid(ctx, n) + kXcds - 1) / kXcds * kXcds;
   const size_t o_hist = carve(m * 4), o_offs = carve((m + 1) * 4), o_keys = carve(n * 8),
-               o_vals = carve(vals ? n * 8 : 8), o_rows = carve(rows ? n * 4 : 8);
+               o_vals = carve(vals ? n * 8 : 8), o_rows = carve(rows ? n * 4 : 8),
+               o_run = carve(MODE == kEmit ? run_probe_scratch(ctx, grid) : 8);
   int st = kg_work_reserve(ctx, off);
   if (st != KG_OK) return st;
   char* w = static_cast<char*>(ctx->work);
@@ -1345,6 +1562,7 @@ int launch_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const 
                              (rows ? kPartTile * 4 : 0) + kPartTile * 2;
   KG_REQUIRE(ctx, scatter_lds <= 64 * 1024, "region count %d too large", P);
   kg_time_begin(ctx);
+  hot_pass();
   hipLaunchKernelGGL(k_part_hist, dim3(nblk), dim3(kJBlock), P * 4, ctx->stream, ht->view(), key, sel, n, P, hist);
   hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(1024), 0, ctx->stream, hist, offs, m);
   if (vals && rows)
@@ -1363,9 +1581,9 @@ int launch_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const 
   ProbeOut po = o;
   if (vals) po.vals = pvals;
   PartIn part{offs, rows ? prows : nullptr, nblk, P};
-  const int grid = (probe_grid(ctx, n) + kXcds - 1) / kXcds * kXcds;
-  launch_probe_kernel<MODE, FIRST>(ctx, ht, pk, nullptr, n, po, part, grid);
+  st = run_probe<MODE, FIRST>(ctx, ht, pk, nullptr, n, po, part, grid, w + o_run);
   kg_time_end(ctx);
+  if (st != KG_OK) return st;
   KG_LAUNCH_CHECK(ctx);
   return KG_OK;
 }
@@ -1452,10 +1670,13 @@ extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_co
     KG_LAUNCH_CHECK(ctx);
   }
   if (nrows > 0) {
+    if (!t->hot && (st = alloc(ctx, reinterpret_cast<void**>(&t->hot), kHot * 8 + 64)) != KG_OK) return st;
     ProbeOut o{};
     o.acc = t->acc;
     o.first = t->first;
     o.vals = d_vals;
+    o.hot_keys = t->hot;
+    o.hot_n = reinterpret_cast<const int32_t*>(t->hot + kHot);
     st = d_out_first ? launch_probe<kAgg, true>(ctx, t, *key, nullptr, nrows, o)
                      : launch_probe<kAgg, false>(ctx, t, *key, nullptr, nrows, o);
     if (st != KG_OK) return st;
@@ -1494,6 +1715,7 @@ extern "C" int kg_ht_free(kg_ctx* ctx, kg_hash_table* ht) {
   if (ht->keys) hipFree(ht->keys);
   if (ht->rows) hipFree(ht->rows);
   if (ht->acc) hipFree(ht->acc);
+  if (ht->hot) hipFree(ht->hot);
   if (ht->first) hipFree(ht->first);
   if (ht->side_rows) hipFree(ht->side_rows);
   if (ht->counters) hipFree(ht->counters);

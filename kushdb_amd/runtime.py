@@ -278,14 +278,21 @@ class HashTable:
         d = key_col.desc()
         self.ctx.check(self.ctx.lib.kg_ht_build(self.ctx.h, self.h, ctypes.byref(d), _ptr(rows), nrows))
 
-    def probe(self, key_col, rows=None, out_cap=None):
-        """All (probe row, build row) pairs with equal keys (unordered)."""
+    def probe(self, key_col, rows=None, out_cap=None, out=None):
+        """All (probe row, build row) pairs with equal keys (unordered); out =
+        (probe rows, build rows) int32 device tensors of out_cap entries to
+        reuse, or None to allocate them."""
         nrows = rows.numel() if rows is not None else len(key_col)
         if out_cap is None:
             out_cap = self.probe_count(key_col, rows)
         dev = key_col.data.device
-        op = torch.empty(max(out_cap, 1), dtype=torch.int32, device=dev)
-        ob = torch.empty(max(out_cap, 1), dtype=torch.int32, device=dev)
+        if out is not None:
+            op, ob = out
+            if op.numel() < out_cap or ob.numel() < out_cap:
+                raise ValueError("output tensors smaller than out_cap")
+        else:
+            op = torch.empty(max(out_cap, 1), dtype=torch.int32, device=dev)
+            ob = torch.empty(max(out_cap, 1), dtype=torch.int32, device=dev)
         cnt = ctypes.c_int64()
         d = key_col.desc()
         self.ctx.check(self.ctx.lib.kg_ht_probe(self.ctx.h, self.h, ctypes.byref(d), _ptr(rows), nrows, _ptr(op),

@@ -63,6 +63,7 @@
 #include "kush_oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -1406,4 +1407,102 @@ double kor_det_sum(const double* v, int64_t n) {
     cur.swap(nxt);
   }
   return kor_det_chunk(cur.data(), static_cast<int64_t>(cur.size()));
+}
+
+// C2's CPU column: the reference's chained join table (runtime/hash_table.cc
+// via kor_q_join_count_sum's loop) built from nb keys, then np probes with
+// SUM(value) over the matches, each phase timed with steady_clock.
+int64_t kor_time_join(int64_t nb, const int64_t* bkeys, int64_t np, const int64_t* pkeys, const int64_t* pvals,
+                      double* build_s, double* probe_s, int64_t* out_sum) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::unordered_map<int32_t, std::vector<int64_t>> table;
+  table.reserve(nb);
+  for (int64_t i = 0; i < nb; i++) {
+    int32_t h = 0;
+    kor_hash_combine(&h, bkeys[i]);
+    table[h].push_back(bkeys[i]);
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  int64_t matches = 0;
+  uint64_t sum = 0;
+  for (int64_t j = 0; j < np; j++) {
+    int32_t h = 0;
+    kor_hash_combine(&h, pkeys[j]);
+    auto it = table.find(h);
+    if (it == table.end()) continue;
+    for (int64_t k : it->second)
+      if (k == pkeys[j]) {
+        matches++;
+        sum += static_cast<uint64_t>(pvals[j]);
+      }
+  }
+  const auto t2 = std::chrono::steady_clock::now();
+  *build_s = std::chrono::duration<double>(t1 - t0).count();
+  *probe_s = std::chrono::duration<double>(t2 - t1).count();
+  *out_sum = static_cast<int64_t>(sum);
+  return matches;
+}
+
+// C3's CPU column: the five-table chain t0 <- t1 <- t2 <- t3 <- t4 (t_i.fk =
+// t_{i-1}.id, id = row index) with the filters t0.name STARTS_WITH p0
+// (runtime/string.cc StartsWith), t1.s LIKE p1 (the planner's regex,
+// kor_like), t4.v < vmax, then MIN(t0.name) and MIN(t4.v) over the result
+// tuples. Left-deep hash joins, each building the reference's chained table
+// (HashCombine of the key) over the tuples so far and probing it with the next
+// table's qualifying rows. Returns the number of result tuples; the row of
+// the smallest name (first among equals) and the smallest v.
+int64_t kor_q_chain(const int64_t* sizes, const int32_t* const* fks, const int32_t* v4, const int64_t* name_off,
+                    const uint8_t* name_bytes, const int64_t* s_off, const uint8_t* s_bytes, const uint8_t* p0,
+                    int32_t l0, const uint8_t* p1, int32_t l1, int32_t vmax, int64_t* out_min_name_row,
+                    int32_t* out_min_v) {
+  std::vector<int32_t> sel0(sizes[0]), sel1(sizes[1]), sel4(sizes[4]);
+  sel0.resize(kor_text_select(sizes[0], name_off, name_bytes, nullptr, KOR_TEXT_STARTS_WITH, p0, l0, nullptr,
+                              sel0.data()));
+  sel1.resize(kor_text_select(sizes[1], s_off, s_bytes, nullptr, KOR_TEXT_LIKE, p1, l1, nullptr, sel1.data()));
+  int64_t k4 = 0;
+  for (int64_t r = 0; r < sizes[4]; r++)
+    if (v4[r] < vmax) sel4[k4++] = static_cast<int32_t>(r);
+  sel4.resize(k4);
+  std::vector<std::vector<int32_t>> cols(5);
+  cols[0] = sel0;
+  for (int i = 1; i < 5; i++) {
+    std::unordered_map<int32_t, std::vector<int32_t>> table;
+    const std::vector<int32_t>& prev = cols[i - 1];
+    for (size_t j = 0; j < prev.size(); j++) {
+      int32_t h = 0;
+      kor_hash_combine(&h, prev[j]);
+      table[h].push_back(static_cast<int32_t>(j));
+    }
+    std::vector<std::vector<int32_t>> next(i + 1);
+    auto probe = [&](int64_t r) {
+      const int32_t key = fks[i][r];
+      int32_t h = 0;
+      kor_hash_combine(&h, key);
+      auto it = table.find(h);
+      if (it == table.end()) return;
+      for (int32_t j : it->second) {
+        if (prev[j] != key) continue;
+        for (int c = 0; c < i; c++) next[c].push_back(cols[c][j]);
+        next[i].push_back(static_cast<int32_t>(r));
+      }
+    };
+    if (i == 1) for (int32_t r : sel1) probe(r);
+    else if (i == 4) for (int32_t r : sel4) probe(r);
+    else for (int64_t r = 0; r < sizes[i]; r++) probe(r);
+    cols.swap(next);
+    cols.resize(5);
+  }
+  const int64_t nt = static_cast<int64_t>(cols[0].size());
+  int64_t best = -1;
+  int32_t vmin = INT32_MAX;
+  for (int64_t j = 0; j < nt; j++) {
+    const int32_t r = cols[0][j];
+    if (best < 0 || kor_text_cmp(name_bytes + name_off[r], name_off[r + 1] - name_off[r], name_bytes + name_off[best],
+                                 name_off[best + 1] - name_off[best]) < 0)
+      best = r;
+    vmin = std::min(vmin, v4[cols[4][j]]);
+  }
+  *out_min_name_row = best;
+  *out_min_v = vmin;
+  return nt;
 }

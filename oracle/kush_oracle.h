@@ -180,6 +180,15 @@ int64_t kor_q_star(int64_t nf, const int32_t* fk, const int32_t* qty, const doub
                    const int32_t* cat, int64_t max_groups, int32_t* out_cat, double* out_sum,
                    int64_t* out_cnt);
 
+// CPU columns of the bench: the C2 join with its build and probe phases timed,
+// and the five-table chain query of C3.
+int64_t kor_time_join(int64_t nb, const int64_t* bkeys, int64_t np, const int64_t* pkeys, const int64_t* pvals,
+                      double* build_s, double* probe_s, int64_t* out_sum);
+int64_t kor_q_chain(const int64_t* sizes, const int32_t* const* fks, const int32_t* v4, const int64_t* name_off,
+                    const uint8_t* name_bytes, const int64_t* s_off, const uint8_t* s_bytes, const uint8_t* p0,
+                    int32_t l0, const uint8_t* p1, int32_t l1, int32_t vmax, int64_t* out_min_name_row,
+                    int32_t* out_min_v);
+
 // ---- deterministic fixed-order reduction (GPU deterministic mode order) ----
 void kor_dtree_scan_agg(int64_t n, const int32_t* day, int32_t cutoff, const int32_t* g1,
                         const int32_t* g2, int32_t card2, int32_t ngroups, int32_t nf,

@@ -134,9 +134,46 @@ def lib():
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_void_p]
+        L.kor_time_join.restype = ctypes.c_int64
+        L.kor_time_join.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_int64)]
+        L.kor_q_chain.restype = ctypes.c_int64
+        L.kor_q_chain.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_char_p,
+                                  ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.POINTER(ctypes.c_int32)]
         L.kor_det_sum.restype = ctypes.c_double
         L.kor_det_sum.argtypes = [ctypes.c_void_p, ctypes.c_int64]
     return _lib
+
+
+def time_join(bkeys, pkeys, pvals):
+    """(matches, SUM(value) over matches, build seconds, probe seconds) of the
+    reference-style chained join table on the CPU, one thread."""
+    b = np.ascontiguousarray(bkeys, dtype=np.int64)
+    p = np.ascontiguousarray(pkeys, dtype=np.int64)
+    v = np.ascontiguousarray(pvals, dtype=np.int64)
+    bs, ps, s = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+    m = lib().kor_time_join(len(b), b.ctypes.data, len(p), p.ctypes.data, v.ctypes.data, ctypes.byref(bs),
+                            ctypes.byref(ps), ctypes.byref(s))
+    return m, s.value, bs.value, ps.value
+
+
+def chain_query(htabs, p0=b"a", p1=b"%e_a%", vmax=50):
+    """The five-table chain (t_i.fk = t_{i-1}.id; t0.name STARTS_WITH p0,
+    t1.s LIKE p1, t4.v < vmax): (result tuples, MIN(t0.name) bytes, MIN(t4.v))."""
+    sizes = np.array([len(t["id"]) for t in htabs], dtype=np.int64)
+    fks = [np.zeros(1, np.int32)] + [np.ascontiguousarray(t["fk"], dtype=np.int32) for t in htabs[1:]]
+    fkp = (ctypes.c_void_p * 5)(*[f.ctypes.data for f in fks])
+    v4 = np.ascontiguousarray(htabs[4]["v"], dtype=np.int32)
+    no, nb = htabs[0]["name_off"], htabs[0]["name_heap"]
+    so, sb = htabs[1]["s_off"], htabs[1]["s_heap"]
+    row, vmin = ctypes.c_int64(), ctypes.c_int32()
+    n = lib().kor_q_chain(sizes.ctypes.data, fkp, v4.ctypes.data, no.ctypes.data, nb.ctypes.data, so.ctypes.data,
+                          sb.ctypes.data, p0, len(p0), p1, len(p1), vmax, ctypes.byref(row), ctypes.byref(vmin))
+    name = bytes(nb[no[row.value]:no[row.value + 1]]) if n else None
+    return n, name, (vmin.value if n else None)
 
 
 def det_sum(values):
