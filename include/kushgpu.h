@@ -169,6 +169,12 @@ int kg_ctx_agg_order(const kg_ctx* ctx, int64_t n, int32_t* nblocks, int32_t* ve
  * brackets its dominant kernel with hipEvents on the context's stream.
  * kg_ctx_kernel_stats synchronizes, returns the summed kernel time in ms and
  * the number of timed launches since the last reset, and resets if asked. */
+/* The GROUP BY path the last aggregation call on this context took:
+ * "register" (at most 8 dense groups, 4 with MIN/MAX, per-lane registers),
+ * "lds" (dense groups in a workgroup-private LDS table of up to the device's
+ * LDS per workgroup, 160 KB on gfx950), "hashed" (kg_hash_group_agg), or ""
+ * before any. No counterpart in the reference. */
+const char* kg_ctx_last_path(const kg_ctx* ctx);
 int kg_ctx_set_timing(kg_ctx* ctx, int on);
 int kg_ctx_kernel_stats(kg_ctx* ctx, double* total_ms, int64_t* launches, int reset);
 

@@ -224,6 +224,8 @@ int kg_ctx_set_probe_regions(kg_ctx* ctx, int32_t regions) {
   return KG_OK;
 }
 
+const char* kg_ctx_last_path(const kg_ctx* ctx) { return ctx ? ctx->last_path : ""; }
+
 int kg_ctx_set_build_regions(kg_ctx* ctx, int32_t regions) {
   if (!ctx) return KG_ERR_INVALID;
   KG_REQUIRE(ctx, regions >= -1 && regions <= 32768, "build regions must be -1, 0 or up to 32768 (got %d)", regions);

@@ -273,6 +273,7 @@ extern "C" int kg_hash_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   bool minmax = false;
   if ((st = build_measures(ctx, cols, ncols, aggs, naggs, n, M, plan, minmax)) != KG_OK) return st;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  ctx->last_path = "hashed";
   const int nm = M.nm;
   const int64_t cap = std::max<int64_t>(std::min<int64_t>(n, max_groups + 1), 1);
   uint64_t nslots = 64;

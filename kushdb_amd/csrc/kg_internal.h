@@ -34,6 +34,8 @@ struct kg_ctx {
   // Rows per lane step of the last fixed-order aggregation (1 or 4): with
   // kg_agg_blocks_for(n) it defines that aggregation's FP64 addition order.
   int last_agg_vec = 1;
+  // The GROUP BY path of the last aggregation call (kg_ctx_last_path).
+  const char* last_path = "";
   int num_cus = 256;
   int64_t max_lds = 64 * 1024;  // LDS bytes one workgroup may use (160 KB on gfx950)
   std::string err;

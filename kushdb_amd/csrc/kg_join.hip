@@ -1873,8 +1873,13 @@ extern "C" int kg_scan_join_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols
   const int G = group_card;
   const int nm = M.nm;
   const size_t lds_bytes = (static_cast<size_t>(G) * nm * 2 + G * 2 + (minmax ? 2 * G * nm : 0)) * 8;
-  if (lds_bytes > 64 * 1024)
+  if (static_cast<int64_t>(lds_bytes) > ctx->max_lds)
     return kg_fail(ctx, KG_ERR_UNSUPPORTED, "group table of %zu bytes exceeds the LDS budget", lds_bytes);
+  {
+    const void* kfn = minmax ? reinterpret_cast<const void*>(&k_join_group_agg<true>)
+                             : reinterpret_cast<const void*>(&k_join_group_agg<false>);
+    if ((st = kg_allow_lds(ctx, kfn, lds_bytes)) != KG_OK) return st;
+  }
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   *out_ngroups = 0;
   size_t off = 0;

@@ -919,8 +919,9 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   }
   if (nkeys == 2) K.card1 = key_card[1];
   // A dense table that cannot stay on chip (more than kMaxLdsGroups groups, or
-  // more LDS than one workgroup gets) runs as the hashed GROUP BY over the same
-  // key columns: the same groups, aggregates and insertion order.
+  // more LDS than one workgroup gets: 160 KB on gfx950) runs as the hashed
+  // GROUP BY over the same key columns: the same groups, aggregates and
+  // insertion order.
   bool hashed = G > kMaxLdsGroups;
 
   MeasureSet M;
@@ -932,10 +933,14 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   if (!hashed && G > (minmax ? 4 : 8)) {
     const size_t lds_bytes = (G * M.nm * 2 + G * 2 + (minmax ? 2 * G * M.nm : 0)) * 8;
-    hashed = lds_bytes > 64 * 1024;
+    hashed = static_cast<int64_t>(lds_bytes) > ctx->max_lds;
   }
-  if (hashed) return dense_as_hashed(ctx, n, cols, ncols, preds, npreds, key_cols, nkeys, aggs, naggs, max_groups,
-                                     out_ngroups, out_keys, out_vals, out_nulls);
+  if (hashed) {
+    st = dense_as_hashed(ctx, n, cols, ncols, preds, npreds, key_cols, nkeys, aggs, naggs, max_groups, out_ngroups,
+                         out_keys, out_vals, out_nulls);
+    ctx->last_path = "hashed";
+    return st;
+  }
   const int nm = M.nm;
   std::vector<AggFinal> fin(G * (nm > 0 ? nm : 1));
   std::vector<uint64_t> rowsv(G);
@@ -944,6 +949,7 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
   std::vector<std::vector<int8_t>> first_null(naggs);
 
   const bool use_reg = G <= (minmax ? 4 : 8);
+  ctx->last_path = use_reg ? "register" : "lds";
   if (use_reg) {
     const int nb = kg_agg_blocks_for(n);
     const size_t per_block = static_cast<size_t>(G) * (nm > 0 ? nm : 1);
@@ -1022,7 +1028,11 @@ extern "C" int kg_scan_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int
                        static_cast<int>(G), M, drows, dfirst);
     KG_LAUNCH_CHECK(ctx);
     const size_t lds_bytes = (G * nm * 2 + G * 2 + (minmax ? 2 * G * nm : 0)) * 8;
-    KG_REQUIRE(ctx, lds_bytes <= 64 * 1024, "group table of %zu bytes exceeds LDS budget", lds_bytes);
+    // Up to the whole LDS of a CU (160 KB on gfx950): one workgroup per CU
+    // when the table is large, more when it is small.
+    const void* kfn = minmax ? reinterpret_cast<const void*>(&k_group_agg_lds<true>)
+                             : reinterpret_cast<const void*>(&k_group_agg_lds<false>);
+    if ((st = kg_allow_lds(ctx, kfn, lds_bytes)) != KG_OK) return st;
     int nb = static_cast<int>(std::min<int64_t>(std::max<int64_t>(ceil_div(n, kBlock * 16), 1), ctx->num_cus * 4));
     kg_time_begin(ctx);
     if (minmax)

@@ -142,6 +142,10 @@ class Context:
         self.check(self.lib.kg_file_index(self.h, str(path).encode(), key_type, ctypes.byref(h)))
         return ColumnIndex.from_handle(self, h)
 
+    def last_path(self):
+        """GROUP BY path of the last aggregation: "register", "lds" or "hashed"."""
+        return self.lib.kg_ctx_last_path(self.h).decode()
+
     def set_build_regions(self, regions):
         """-1: region-partitioned join builds when large; 0: direct inserts; > 0: at least that many regions."""
         self.check(self.lib.kg_ctx_set_build_regions(self.h, regions))

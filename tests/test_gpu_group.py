@@ -148,3 +148,26 @@ def test_dense_keys_beyond_on_chip_table(ctx):
                                  [(po.SUM, po.Col(po.INT, v)), (po.MIN, po.Col(po.INT, v)), (po.COUNT, None)],
                                  rows=rows, max_groups=6001)
     assert np.array_equal(codes.astype(np.int64), kv) and np.array_equal(vals, ov) and np.array_equal(nulls, onl)
+
+
+def test_dense_1024_groups_7_measures_stay_in_lds(ctx):
+    """1024 dense groups x 7 SUM measures need a 131 KB group table: it stays
+    on the LDS path (gfx950 gives a workgroup up to 160 KB) and equals the
+    oracle (integer SUMs exact, REAL SUMs within 1e-12)."""
+    rng = np.random.default_rng(21)
+    n = 2_000_000
+    g = rng.integers(0, 1024, n).astype(np.int32)
+    ints = [rng.integers(-1000, 1000, n).astype(np.int32) for _ in range(4)]
+    reals = [rng.uniform(-1e5, 1e5, n) for _ in range(3)]
+    cols = [Column(abi.KG_INT, torch.from_numpy(g).to(DEV))]
+    cols += [Column(abi.KG_INT, torch.from_numpy(x).to(DEV)) for x in ints]
+    cols += [Column(abi.KG_REAL, torch.from_numpy(x).to(DEV)) for x in reals]
+    aggs = [(abi.KG_AGG_SUM, 1 + i) for i in range(7)] + [(abi.KG_AGG_COUNT, -1)]
+    codes, vals, nulls = ctx.scan_group_agg(cols, [], [(0, 1024)], aggs, max_groups=1024)
+    assert ctx.last_path() == "lds"
+    ocols = [po.Col(po.INT, x) for x in ints] + [po.Col(po.REAL, x) for x in reals]
+    kv, _, ov, _ = po.group_by([po.Col(po.INT, g)], [(po.SUM, c) for c in ocols] + [(po.COUNT, None)],
+                               max_groups=1025)
+    assert np.array_equal(codes[:, 0].astype(np.int64), kv[:, 0] if kv.ndim > 1 else kv)
+    assert np.array_equal(vals[:, :4], ov[:, :4]) and np.array_equal(vals[:, 7], ov[:, 7])
+    assert np.allclose(vals[:, 4:7].view(np.float64), ov[:, 4:7].view(np.float64), rtol=1e-12, atol=1e-6)
