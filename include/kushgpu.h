@@ -351,10 +351,13 @@ int kg_hash_group_agg(kg_ctx* ctx, int64_t n, const kg_col* cols, int32_t ncols,
  * (runtime/memory_column_index.h:18-56, compile/proxy/memory_column_index.cc:282-428)
  * and, for unfiltered children, the disk index runtime::ColumnIndex
  * (runtime/column_index.cc:73-95): key -> bucket of tuple ids in ascending
- * order. Keys are one SMALLINT/INT/DATE/ENUM column; NULL keys are not indexed
- * (permutable_skinner_join_translator.cc:972-994). The tuple id of a row is its
- * position in d_rows (its position in the materialized child), or its row id
- * when d_rows is NULL. */
+ * order. Keys are one fixed-width column (BOOLEAN, SMALLINT, INT, DATE, ENUM,
+ * BIGINT, REAL; ColumnIndex::GetInt8..GetFloat64, runtime/column_index.h:50-54;
+ * TEXT keys index their kg_text_hash codes). NULL keys are not indexed
+ * (permutable_skinner_join_translator.cc:972-994); REAL keys compare as
+ * doubles: -0.0 and 0.0 are one key and NaN is neither indexed nor found. The
+ * tuple id of a row is its position in d_rows (its position in the
+ * materialized child), or its row id when d_rows is NULL. */
 typedef struct kg_index kg_index;
 int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_rows, int64_t nrows, kg_index** out);
 /* Device tuple-id array (grouped by key, ascending per key), its length and
@@ -403,12 +406,15 @@ int kg_tuple_set_free(kg_ctx* ctx, kg_tuple_set* s);
  * with its table functions and valid-tuple handler
  * (compile/translators/permutable_skinner_join_translator.cc:82-994): the
  * multi-way equi-join of ntables materialized children under the conditions
- * conds, with the join order chosen per episode by the reference's UCT agent
- * (exploration weight 1e-5, connectivity heuristic; skinner_join_executor.cc:
- * 559-838). An episode joins the next slice of its leftmost table's remaining
- * tuples through the other tables on the device (column-index expansion for
- * 16/32-bit keys, checked equality otherwise) and inserts the result tuples
- * into a tuple-id set; the join ends when a leftmost table is complete.
+ * conds, with the join order chosen per episode by a UCT search over join
+ * orders (the role of UctJoinAgent, skinner_join_executor.cc:559-838). An
+ * episode joins the next slice of its leftmost table's remaining tuples
+ * through the other tables on the device: a table linked by one condition
+ * through that condition's column index, a table linked by several through
+ * the sorted intersection of all their index buckets
+ * (permutable_skinner_join_translator.cc:359-662), a table linked by none as a
+ * cross product; the result tuples go into a tuple-id set, and the join ends
+ * when a leftmost table is complete.
  * *out_set receives a new set (caller frees with kg_tuple_set_free) holding
  * each result tuple once: int32[ntables] tuple ids, tuple id i of table t being
  * position i of its child (row d_rows[i] of the key columns, or row i when
@@ -658,9 +664,9 @@ int kg_file_release(kg_ctx* ctx, const char* path);
 int kg_file_enum(kg_ctx* ctx, const char* path, char* buf, int64_t buf_cap, int64_t* offsets, int32_t max_codes,
                  int32_t* out_count);
 /* ColumnIndex::Open + Get* (runtime/column_index.cc:150-240, .kdbindex with
- * SMALLINT, INT, DATE or ENUM keys, those kg_index_build takes; other key
- * types return KG_ERR_UNSUPPORTED): the file's key -> tuple-id lists as a
- * device column index, equal to kg_index_build over the listed tuples. */
+ * SMALLINT, INT, DATE, ENUM, BIGINT or REAL keys; other key types return
+ * KG_ERR_UNSUPPORTED): the file's key -> tuple-id lists as a device column
+ * index, equal to kg_index_build over the listed tuples. */
 int kg_file_index(kg_ctx* ctx, const char* path, int32_t key_type, kg_index** out);
 
 /* ---- ORDER BY and result rows -------------------------------------------------- */

@@ -257,9 +257,9 @@ extern "C" int kg_file_index(kg_ctx* ctx, const char* path, int32_t key_type, kg
   KG_REQUIRE(ctx, path && out, "path or out is NULL");
   *out = nullptr;
   const int w = width_of(key_type);
-  if (key_type != KG_SMALLINT && key_type != KG_INT && key_type != KG_DATE && key_type != KG_ENUM)
-    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "index key type %d: device column indexes take 16/32-bit keys",
-                   key_type);
+  if (w == 0 || key_type == KG_BOOLEAN)
+    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "index key type %d: fixed-width keys (SMALLINT, INT, DATE, ENUM, "
+                   "BIGINT, REAL) only", key_type);
   std::vector<char> b;
   KG_REQUIRE(ctx, read_file(path, b), "cannot read %s", path);
   KG_REQUIRE(ctx, b.size() >= 8, "%s is too short for a column index", path);

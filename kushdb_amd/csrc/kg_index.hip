@@ -25,62 +25,72 @@
 #include <cstring>
 #include <new>
 
-#include "kg_ops.h"
+#include "kg_index.h"
 
 namespace kg {
 
 constexpr int kIBlock = 256;
-constexpr int kDirSlots = 8;
-constexpr int64_t kDirEmpty = INT64_MIN;
 constexpr int kMaxTupleLen = 8;
 constexpr int kMaxIntersect = 8;
 
-__device__ __forceinline__ uint64_t dir_bucket(int64_t key, uint64_t nbuckets) {
-  return __umul64hi(mix64(static_cast<uint64_t>(key)), nbuckets);
+// Sortable form of an index key: I32 keys as a 32-bit order-preserving word
+// (sorted on 32 bits), I64 keys with the sign bit flipped, F64 keys as their
+// bits (equal keys only need to be adjacent).
+__device__ __forceinline__ uint64_t enc_key(int64_t k, int enc) {
+  if (enc == kEncI32) return static_cast<uint64_t>(static_cast<uint32_t>(static_cast<int32_t>(k)) ^ 0x80000000u);
+  if (enc == kEncI64) return static_cast<uint64_t>(k) ^ (1ull << 63);
+  return static_cast<uint64_t>(k);
 }
 
-// Keys of the selected rows as sortable uint32 (sign bit flipped) plus their
-// tuple ids (selection positions). NULL keys get the flag in `valid`.
-__global__ void k_index_gather(kg_col key, const int32_t* sel, int64_t n, uint32_t* okey, int32_t* otid,
+__host__ __device__ __forceinline__ int64_t dec_key(uint64_t u, int enc) {
+  if (enc == kEncI32) return static_cast<int32_t>(static_cast<uint32_t>(u) ^ 0x80000000u);
+  if (enc == kEncI64) return static_cast<int64_t>(u ^ (1ull << 63));
+  return static_cast<int64_t>(u);
+}
+
+// Sortable keys of the selected rows plus their tuple ids (selection
+// positions); rows without a key get valid = 0.
+__global__ void k_index_gather(kg_col key, const int32_t* sel, int64_t n, int enc, uint64_t* okey, int32_t* otid,
                                int32_t* valid) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int64_t r = sel ? sel[i] : i;
-    const bool null = key.nulls && key.nulls[r];
-    const int32_t k = null ? 0 : static_cast<int32_t>(load_int(key.type, key.data, r));
-    okey[i] = static_cast<uint32_t>(k) ^ 0x80000000u;
+    int64_t k = 0;
+    const bool ok = index_key(key, r, enc, k);
+    okey[i] = enc_key(ok ? k : 0, enc);
     otid[i] = static_cast<int32_t>(i);
-    valid[i] = null ? 0 : 1;
+    valid[i] = ok ? 1 : 0;
   }
 }
 
-// Directory from key to its run of tuple ids. Hashed: 8-slot buckets of
-// {key, offset, count}. Dense (dspan > 0, when the distinct keys span at most
-// about three times their number): offset/count indexed by key - dmin, count 0
-// for an absent key, written in key order by one streaming pass.
+// Owning view of a directory (see IndexDir).
 struct Dir {
-  int64_t* keys;     // [nbuckets * 8] (hashed)
-  int64_t* offset;   // [nbuckets * 8] or [dspan]
-  int32_t* count;    // [nbuckets * 8] or [dspan]
-  uint64_t nbuckets;
-  int64_t dmin;
-  uint64_t dspan;
+  int64_t* keys = nullptr;     // [nbuckets * 8] (hashed)
+  int64_t* offset = nullptr;   // [nbuckets * 8] or [dspan]
+  int32_t* count = nullptr;    // [nbuckets * 8] or [dspan]
+  uint64_t nbuckets = 0;
+  int64_t dmin = 0;
+  uint64_t dspan = 0;
+  int64_t min_off = 0;
+  int32_t min_cnt = 0;
+  int32_t enc = kEncI32;
+  IndexDir view() const { return IndexDir{keys, offset, count, nbuckets, dmin, dspan, min_off, min_cnt, enc}; }
 };
 
 // Dense directory: one thread per run, runs in ascending key order.
-__global__ void k_dir_dense(Dir d, const uint32_t* ukeys, const int32_t* counts, const int64_t* offsets,
+__global__ void k_dir_dense(Dir d, const uint64_t* ukeys, const int32_t* counts, const int64_t* offsets,
                             const int32_t* nruns) {
   const int64_t n = *nruns;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t u = static_cast<int64_t>(static_cast<int32_t>(ukeys[i] ^ 0x80000000u)) - d.dmin;
+    const uint64_t u = static_cast<uint64_t>(dec_key(ukeys[i], d.enc)) - static_cast<uint64_t>(d.dmin);
     d.offset[u] = offsets[i];
     d.count[u] = counts[i];
   }
 }
 
 // Sets *unsorted when some key is greater than the next one.
-__global__ void k_keys_sorted(const uint32_t* k, int64_t n, int* unsorted) {
+__global__ void k_keys_sorted(const uint64_t* k, int64_t n, int* unsorted) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   bool bad = false;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i + 1 < n; i += stride)
@@ -88,13 +98,15 @@ __global__ void k_keys_sorted(const uint32_t* k, int64_t n, int* unsorted) {
   if (__any(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(unsorted, 1);
 }
 
-// One thread per distinct key (run): claim a directory slot.
-__global__ void k_dir_insert(Dir d, const uint32_t* ukeys, const int32_t* counts, const int64_t* offsets,
+// One thread per distinct key (run): claim a directory slot. The run of the
+// key kDirEmpty (never a slot key) is kept by the host in min_off / min_cnt.
+__global__ void k_dir_insert(Dir d, const uint64_t* ukeys, const int32_t* counts, const int64_t* offsets,
                              const int32_t* nruns, int* overflow) {
   const int64_t n = *nruns;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t k = static_cast<int32_t>(ukeys[i] ^ 0x80000000u);
+    const int64_t k = dec_key(ukeys[i], d.enc);
+    if (k == kDirEmpty) continue;
     uint64_t b = dir_bucket(k, d.nbuckets);
     bool done = false;
     for (uint64_t probes = 0; probes < d.nbuckets && !done; probes++) {
@@ -118,47 +130,22 @@ __global__ void k_dir_insert(Dir d, const uint32_t* ukeys, const int32_t* counts
 
 // Batched GetBucket: bucket (pointer into tids, size) of each probe key; a
 // missing or NULL key yields (tids, 0), the EMPTY bucket.
-__global__ void k_dir_lookup(Dir d, const int32_t* tids, kg_col key, const int32_t* sel, int64_t n,
+__global__ void k_dir_lookup(IndexDir d, const int32_t* tids, kg_col key, const int32_t* sel, int64_t n,
                              const int32_t** out_ptr, int32_t* out_cnt) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int64_t r = sel ? sel[i] : i;
-    const int32_t* ptr = tids;
+    int64_t k = 0, off = 0;
     int32_t cnt = 0;
-    if (d.dspan && !(key.nulls && key.nulls[r])) {
-      const int64_t u = static_cast<int64_t>(static_cast<int32_t>(load_int(key.type, key.data, r))) - d.dmin;
-      if (u >= 0 && static_cast<uint64_t>(u) < d.dspan) {
-        cnt = d.count[u];
-        if (cnt) ptr = tids + d.offset[u];
-      }
-    } else if (!(key.nulls && key.nulls[r])) {
-      const int64_t k = static_cast<int32_t>(load_int(key.type, key.data, r));
-      uint64_t b = dir_bucket(k, d.nbuckets);
-      for (uint64_t probes = 0; probes < d.nbuckets; probes++) {
-        const longlong2* bk = reinterpret_cast<const longlong2*>(d.keys + b * kDirSlots);
-        bool empty = false, found = false;
-#pragma unroll
-        for (int q = 0; q < kDirSlots / 2; q++) {
-          const longlong2 kv = bk[q];
-          if (!found && kv.x == k) {
-            ptr = tids + d.offset[b * kDirSlots + 2 * q];
-            cnt = d.count[b * kDirSlots + 2 * q];
-            found = true;
-          }
-          if (!found && kv.y == k) {
-            ptr = tids + d.offset[b * kDirSlots + 2 * q + 1];
-            cnt = d.count[b * kDirSlots + 2 * q + 1];
-            found = true;
-          }
-          empty |= kv.x == kDirEmpty || kv.y == kDirEmpty;
-        }
-        if (found || empty) break;
-        b = (b + 1 == d.nbuckets) ? 0 : b + 1;
-      }
-    }
-    out_ptr[i] = ptr;
+    if (index_key(key, r, d.enc, k)) index_find(d, k, off, cnt);
+    out_ptr[i] = tids + (cnt ? off : 0);
     out_cnt[i] = cnt;
   }
+}
+
+__global__ void k_fill_i32x(int32_t* p, int32_t v, int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
 __global__ void k_fill_i64x(int64_t* p, int64_t v, int64_t n) {
@@ -381,15 +368,10 @@ int read_u64(kg_ctx* ctx, const void* d, void* h, size_t bytes) {
 
 int check_index_key(kg_ctx* ctx, const kg_col* key, int64_t n) {
   KG_REQUIRE(ctx, key && (n == 0 || key->data), "key column is NULL");
-  switch (key->type) {
-    case KG_SMALLINT:
-    case KG_INT:
-    case KG_DATE:
-    case KG_ENUM:
-      return KG_OK;
-    default:
-      return kg_fail(ctx, KG_ERR_UNSUPPORTED, "index keys must be 16/32-bit integers (type %d)", key->type);
-  }
+  if (type_width(key->type) == 0)
+    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "index keys must be fixed-width (type %d; TEXT keys index their "
+                   "kg_text_hash codes)", key->type);
+  return KG_OK;
 }
 
 }  // namespace
@@ -408,29 +390,31 @@ extern "C" int kg_index_free(kg_ctx* ctx, kg_index* idx) {
   return KG_OK;
 }
 
-extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_rows, int64_t nrows,
-                              kg_index** out) {
+int kg::index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_rows, int64_t nrows, int enc,
+                    kg_index** out) {
   if (!ctx) return KG_ERR_INVALID;
   KG_REQUIRE(ctx, out, "out is NULL");
   KG_REQUIRE(ctx, nrows >= 0 && nrows < INT32_MAX, "row count out of range");
+  KG_REQUIRE(ctx, enc == kEncI32 || enc == kEncI64 || enc == kEncF64, "bad index encoding %d", enc);
   int st = check_index_key(ctx, key, nrows);
   if (st != KG_OK) return st;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   kg_index* idx = new (std::nothrow) kg_index();
   if (!idx) return kg_fail(ctx, KG_ERR_OOM, "host allocation failed");
+  idx->dir.enc = enc;
   const int64_t n = nrows;
+  const int end_bit = enc == kEncI32 ? 32 : 64;
+  // Rows without a key (NULL; NaN under F64) are dropped first.
+  const bool select = key->nulls || enc == kEncF64 || key->type == KG_REAL;
   // Workspace: keys/tids in and out, valid flags, run outputs, cub temp.
   size_t cub_sort = 0, cub_sel = 0, cub_rle = 0, cub_scan = 0;
-  uint32_t* nk = nullptr;
+  uint64_t* nk = nullptr;
   int32_t* ni = nullptr;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, cub_sort, nk, nk, ni, ni, static_cast<int>(std::max<int64_t>(n, 1)), 0,
-                                     32, ctx->stream);
-  hipcub::DeviceSelect::Flagged(nullptr, cub_sel, nk, ni, nk, ni, static_cast<int>(std::max<int64_t>(n, 1)),
-                                ctx->stream);
-  hipcub::DeviceRunLengthEncode::Encode(nullptr, cub_rle, nk, nk, ni, ni, static_cast<int>(std::max<int64_t>(n, 1)),
-                                        ctx->stream);
-  hipcub::DeviceScan::ExclusiveSum(nullptr, cub_scan, ni, reinterpret_cast<int64_t*>(ni),
-                                   static_cast<int>(std::max<int64_t>(n, 1)), ctx->stream);
+  const int nmax = static_cast<int>(std::max<int64_t>(n, 1));
+  hipcub::DeviceRadixSort::SortPairs(nullptr, cub_sort, nk, nk, ni, ni, nmax, 0, end_bit, ctx->stream);
+  hipcub::DeviceSelect::Flagged(nullptr, cub_sel, nk, ni, nk, ni, nmax, ctx->stream);
+  hipcub::DeviceRunLengthEncode::Encode(nullptr, cub_rle, nk, nk, ni, ni, nmax, ctx->stream);
+  hipcub::DeviceScan::ExclusiveSum(nullptr, cub_scan, ni, reinterpret_cast<int64_t*>(ni), nmax, ctx->stream);
   const size_t cub_bytes = std::max(std::max(cub_sort, cub_sel), std::max(cub_rle, cub_scan));
   size_t off = 0;
   auto carve = [&](size_t bytes) {
@@ -438,8 +422,8 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
     off += (bytes + 255) / 256 * 256;
     return o;
   };
-  const size_t nn = static_cast<size_t>(std::max<int64_t>(n, 1));
-  const size_t o_k0 = carve(nn * 4), o_t0 = carve(nn * 4), o_k1 = carve(nn * 4), o_t1 = carve(nn * 4),
+  const size_t nn = static_cast<size_t>(nmax);
+  const size_t o_k0 = carve(nn * 8), o_t0 = carve(nn * 4), o_k1 = carve(nn * 8), o_t1 = carve(nn * 4),
                o_valid = carve(nn * 4), o_cnt = carve(nn * 4), o_off = carve(nn * 8), o_nsel = carve(16),
                o_nruns = carve(16), o_flag = carve(16), o_cub = carve(cub_bytes);
   if ((st = kg_work_reserve(ctx, off)) != KG_OK) {
@@ -447,9 +431,9 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
     return st;
   }
   char* w = static_cast<char*>(ctx->work);
-  uint32_t* k0 = reinterpret_cast<uint32_t*>(w + o_k0);
+  uint64_t* k0 = reinterpret_cast<uint64_t*>(w + o_k0);
   int32_t* t0 = reinterpret_cast<int32_t*>(w + o_t0);
-  uint32_t* k1 = reinterpret_cast<uint32_t*>(w + o_k1);
+  uint64_t* k1 = reinterpret_cast<uint64_t*>(w + o_k1);
   int32_t* t1 = reinterpret_cast<int32_t*>(w + o_t1);
   int32_t* valid = reinterpret_cast<int32_t*>(w + o_valid);
   int32_t* cnt = reinterpret_cast<int32_t*>(w + o_cnt);
@@ -462,12 +446,12 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
 
   if (n > 0) {
     kg_time_begin(ctx);
-    hipLaunchKernelGGL(k_index_gather, dim3(igrid(ctx, n)), dim3(kIBlock), 0, ctx->stream, *key, d_rows, n, k0, t0,
-                       valid);
+    hipLaunchKernelGGL(k_index_gather, dim3(igrid(ctx, n)), dim3(kIBlock), 0, ctx->stream, *key, d_rows, n, enc, k0,
+                       t0, valid);
     KG_LAUNCH_CHECK(ctx);
-    // Drop NULL keys (keeping order), then a stable sort by key keeps tuple ids
-    // ascending per key. Without a NULL column every key stays in place.
-    if (key->nulls) {
+    // Drop rows without a key (keeping order), then a stable sort by key keeps
+    // tuple ids ascending per key. Otherwise every key stays in place.
+    if (select) {
       cb = cub_bytes;
       KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(cub, cb, k0, valid, k1, nsel, static_cast<int>(n), ctx->stream));
       cb = cub_bytes;
@@ -475,11 +459,11 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
     }
   }
   int32_t hsel = static_cast<int32_t>(n);
-  if (n > 0 && key->nulls && (st = read_u64(ctx, nsel, &hsel, 4)) != KG_OK) {
+  if (n > 0 && select && (st = read_u64(ctx, nsel, &hsel, 4)) != KG_OK) {
     delete idx;
     return st;
   }
-  if (n > 0 && !key->nulls) {
+  if (n > 0 && !select) {
     std::swap(k0, k1);
     std::swap(t0, t1);
   }
@@ -492,7 +476,7 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
   if (hsel > 0) {
     // Keys already in order (an id column over an ascending selection) need no
     // sort: the stable sort of sorted keys is the identity.
-    KG_HIP_CHECK(ctx, hipMemsetAsync(flag, 0, 16, ctx->stream));
+    hipLaunchKernelGGL(k_fill_i32x, dim3(1), dim3(64), 0, ctx->stream, flag, 0, 4);
     hipLaunchKernelGGL(k_keys_sorted, dim3(igrid(ctx, hsel)), dim3(kIBlock), 0, ctx->stream, k1, hsel, flag);
     KG_LAUNCH_CHECK(ctx);
     int unsorted = 0;
@@ -502,9 +486,10 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
     }
     cb = cub_bytes;
     if (unsorted) {
-      KG_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(cub, cb, k1, k0, t1, idx->tids, hsel, 0, 32, ctx->stream));
+      KG_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(cub, cb, k1, k0, t1, idx->tids, hsel, 0, end_bit,
+                                                           ctx->stream));
     } else {
-      KG_HIP_CHECK(ctx, hipMemcpyAsync(k0, k1, static_cast<size_t>(hsel) * 4, hipMemcpyDeviceToDevice, ctx->stream));
+      KG_HIP_CHECK(ctx, hipMemcpyAsync(k0, k1, static_cast<size_t>(hsel) * 8, hipMemcpyDeviceToDevice, ctx->stream));
       KG_HIP_CHECK(ctx, hipMemcpyAsync(idx->tids, t1, static_cast<size_t>(hsel) * 4, hipMemcpyDeviceToDevice,
                                        ctx->stream));
     }
@@ -519,14 +504,15 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
   }
   idx->nkeys = hruns;
   if (hruns > 0) {
-    // Key range of the runs (ascending): first and last run key.
-    uint32_t ends[2] = {0, 0};
-    KG_HIP_CHECK(ctx, hipMemcpyAsync(&ends[0], k1, 4, hipMemcpyDeviceToHost, ctx->stream));
-    KG_HIP_CHECK(ctx, hipMemcpyAsync(&ends[1], k1 + hruns - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    // Key range of the runs (ascending in I32 / I64): first and last run key.
+    uint64_t ends[2] = {0, 0};
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(&ends[0], k1, 8, hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(&ends[1], k1 + hruns - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
     KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const int64_t kmin = static_cast<int32_t>(ends[0] ^ 0x80000000u), kmax = static_cast<int32_t>(ends[1] ^ 0x80000000u);
-    const int64_t span = kmax - kmin + 1;
-    if (span <= 3 * static_cast<int64_t>(hruns) + 1024) {
+    const int64_t kmin = dec_key(ends[0], enc), kmax = dec_key(ends[1], enc);
+    const uint64_t span1 = static_cast<uint64_t>(kmax) - static_cast<uint64_t>(kmin);  // span - 1
+    if (enc != kEncF64 && span1 < 3 * static_cast<uint64_t>(hruns) + 1024) {
+      const int64_t span = static_cast<int64_t>(span1) + 1;
       idx->dir.dmin = kmin;
       idx->dir.dspan = static_cast<uint64_t>(span);
       if ((st = ialloc(ctx, reinterpret_cast<void**>(&idx->dir.offset), span * 8)) != KG_OK ||
@@ -534,13 +520,22 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
         kg_index_free(ctx, idx);
         return st;
       }
-      KG_HIP_CHECK(ctx, hipMemsetAsync(idx->dir.count, 0, span * 4, ctx->stream));
+      hipLaunchKernelGGL(k_fill_i32x, dim3(igrid(ctx, span)), dim3(kIBlock), 0, ctx->stream, idx->dir.count, 0, span);
       hipLaunchKernelGGL(k_dir_dense, dim3(igrid(ctx, hruns)), dim3(kIBlock), 0, ctx->stream, idx->dir, k1, cnt, offs,
                          nruns);
       KG_LAUNCH_CHECK(ctx);
       kg_time_end(ctx);
       *out = idx;
       return KG_OK;
+    }
+    if (enc == kEncI64 && kmin == kDirEmpty) {  // the run of INT64_MIN is the first run
+      int32_t c0 = 0;
+      if ((st = read_u64(ctx, cnt, &c0, 4)) != KG_OK) {
+        kg_index_free(ctx, idx);
+        return st;
+      }
+      idx->dir.min_off = 0;
+      idx->dir.min_cnt = c0;
     }
   }
   idx->dir.nbuckets = static_cast<uint64_t>(std::max<int64_t>(ceil_div(std::max<int64_t>(hruns, 1) * 2, kDirSlots), 1));
@@ -554,7 +549,7 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
   hipLaunchKernelGGL(k_fill_i64x, dim3(igrid(ctx, ns)), dim3(kIBlock), 0, ctx->stream, idx->dir.keys, kDirEmpty,
                      static_cast<int64_t>(ns));
   KG_LAUNCH_CHECK(ctx);
-  KG_HIP_CHECK(ctx, hipMemsetAsync(flag, 0, 16, ctx->stream));
+  hipLaunchKernelGGL(k_fill_i32x, dim3(1), dim3(64), 0, ctx->stream, flag, 0, 4);
   if (hruns > 0) {
     hipLaunchKernelGGL(k_dir_insert, dim3(igrid(ctx, hruns)), dim3(kIBlock), 0, ctx->stream, idx->dir, k1, cnt, offs,
                        nruns, flag);
@@ -574,16 +569,19 @@ extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_r
   return KG_OK;
 }
 
-void kg::index_dir(const kg_index* idx, const int64_t** keys, const int64_t** offset, const int32_t** count,
-                   uint64_t* nbuckets, int64_t* dmin, uint64_t* dspan, const int32_t** tids) {
-  *keys = idx->dir.keys;
-  *offset = idx->dir.offset;
-  *count = idx->dir.count;
-  *nbuckets = idx->dir.nbuckets;
-  *dmin = idx->dir.dmin;
-  *dspan = idx->dir.dspan;
+extern "C" int kg_index_build(kg_ctx* ctx, const kg_col* key, const int32_t* d_rows, int64_t nrows,
+                              kg_index** out) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, key, "key column is NULL");
+  return index_build(ctx, key, d_rows, nrows, index_enc_of(key->type), out);
+}
+
+void kg::index_view(const kg_index* idx, IndexDir* dir, const int32_t** tids) {
+  *dir = idx->dir.view();
   *tids = idx->tids;
 }
+
+int kg::index_enc(const kg_index* idx) { return idx->dir.enc; }
 
 extern "C" int kg_index_info(kg_ctx* ctx, const kg_index* idx, const int32_t** d_tids, int64_t* ntids,
                              int64_t* nkeys) {
@@ -606,8 +604,8 @@ extern "C" int kg_index_lookup(kg_ctx* ctx, const kg_index* idx, const kg_col* k
   if (n == 0) return KG_OK;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
   kg_time_begin(ctx);
-  hipLaunchKernelGGL(k_dir_lookup, dim3(igrid(ctx, n)), dim3(kIBlock), 0, ctx->stream, idx->dir, idx->tids, *key,
-                     d_rows, n, d_out_ptr, d_out_count);
+  hipLaunchKernelGGL(k_dir_lookup, dim3(igrid(ctx, n)), dim3(kIBlock), 0, ctx->stream, idx->dir.view(), idx->tids,
+                     *key, d_rows, n, d_out_ptr, d_out_count);
   kg_time_end(ctx);
   KG_LAUNCH_CHECK(ctx);
   return KG_OK;

@@ -156,9 +156,5 @@ int emit_groups(kg_ctx* ctx, int64_t G, int32_t nkeys, int32_t card1, const Meas
 int det_fp_sums(kg_ctx* ctx, int64_t N, const int32_t* gid, const int32_t* src, int64_t G, const MeasureSet& M,
                 uint64_t* dst, int32_t stride);
 
-// Directory and tid array of a column index (kg_index.hip).
-// Dense directories have dspan > 0 (offset/count indexed by key - dmin).
-void index_dir(const kg_index* idx, const int64_t** keys, const int64_t** offset, const int32_t** count,
-               uint64_t* nbuckets, int64_t* dmin, uint64_t* dspan, const int32_t** tids);
 
 }  // namespace kg

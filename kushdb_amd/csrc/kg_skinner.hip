@@ -39,7 +39,7 @@
 #include <utility>
 #include <vector>
 
-#include "kg_internal.h"
+#include "kg_index.h"
 
 namespace kg {
 namespace {
@@ -299,6 +299,61 @@ __global__ void k_sk_compact(const int32_t* t, int64_t n, int32_t width, const i
       for (int q = 0; q < width; q++) out[static_cast<int64_t>(pos[i]) * width + q] = t[i * width + q];
 }
 
+// Several conditions link the next table to the tables joined so far: for
+// each partial tuple, the index bucket of every condition's key (the
+// per-predicate GetBucket of permutable_skinner_join_translator.cc:359-433),
+// laid out lists[i * k + j] for kg_intersect_batch (the bucket intersection
+// of :435-662).
+constexpr int kMaxLinks = 8;
+
+struct BucketSrc {
+  int32_t k;
+  int32_t width;
+  int32_t pos[kMaxLinks];            // tuple position of the probing table
+  kg_col key[kMaxLinks];             // its key column
+  const int32_t* rows[kMaxLinks];    // its tuple id -> row (NULL: identity)
+  IndexDir dir[kMaxLinks];
+  const int32_t* tids[kMaxLinks];
+};
+
+__global__ void k_sk_buckets(BucketSrc b, const int32_t* t, int64_t n, const int32_t** lists, int32_t* sizes) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (int j = 0; j < b.k; j++) {
+      const int32_t tid = t[i * b.width + b.pos[j]];
+      const int64_t r = b.rows[j] ? b.rows[j][tid] : tid;
+      int64_t key = 0, off = 0;
+      int32_t cnt = 0;
+      if (index_key(b.key[j], r, b.dir[j].enc, key)) index_find(b.dir[j], key, off, cnt);
+      lists[i * b.k + j] = b.tids[j] + off;
+      sizes[i * b.k + j] = cnt;
+    }
+  }
+}
+
+// New partial tuples from the intersections: output j extends tuple i (the
+// last i with offs[i] <= j) by ids[j].
+__global__ void k_sk_from_lists(const int32_t* t, int64_t n, int32_t width, const int64_t* offs, const int32_t* ids,
+                                int64_t total, int32_t* out) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int w1 = width + 1;
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < total; j += stride) {
+    int64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (offs[mid] <= j) lo = mid;
+      else hi = mid;
+    }
+    for (int q = 0; q < width; q++) out[j * w1 + q] = t[lo * width + q];
+    out[j * w1 + width] = ids[j];
+  }
+}
+
+__global__ void k_sk_fill(int32_t* p, int32_t v, int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
 struct Perm {
   int32_t p[kMaxTables];
 };
@@ -342,13 +397,7 @@ struct DevBuf {
   T* as() const { return static_cast<T*>(p); }
 };
 
-bool indexable(int32_t type) {
-  return type == KG_SMALLINT || type == KG_INT || type == KG_DATE || type == KG_ENUM;
-}
-
-bool fixed_width_key(int32_t type) {
-  return indexable(type) || type == KG_BIGINT || type == KG_REAL || type == KG_BOOLEAN;
-}
+bool fixed_width_key(int32_t type) { return type_width(type) > 0; }
 
 // One condition side: table and key column.
 struct Side {
@@ -373,32 +422,34 @@ class SkinnerJoin {
   ~SkinnerJoin() {
     for (size_t i = 0; i < index_.size(); i++)
       if (index_[i] && !shared_.count(static_cast<int>(i))) kg_index_free(ctx_, index_[i]);
-    for (DevBuf* b : {&a_, &b_, &keep_, &pos_, &cub_}) b->release();
+    for (DevBuf* b : {&a_, &b_, &keep_, &pos_, &cub_, &lists_, &sizes_, &nt_, &offs_, &ids_}) b->release();
   }
 
   // The device column index of one condition side, built on first use over
   // the table's materialized tuples (the reference builds a MemoryColumnIndex
   // per join column of a filtered child up front,
   // permutable_skinner_join_translator.cc:116-132, 934-994; here only the
-  // sides some chosen order probes are built). Sides with the same table and
-  // column share one index. Null when the side cannot be indexed.
+  // sides some chosen order probes are built). Keys compare as doubles when
+  // either side is REAL. Sides with the same table, column and key encoding
+  // share one index.
   int Index(int c, int s, kg_index** out) {
     *out = nullptr;
     const Side sd = side(c, s);
-    if (!indexable(sd.col.type) || !indexable(side(c, 1 - s).col.type)) return KG_OK;
     if (index_[c * 2 + s]) {
       *out = index_[c * 2 + s];
       return KG_OK;
     }
+    const int enc = IndexEncOf(c);
     for (size_t e = 0; e < index_.size(); e++) {
       const Side o = side(e / 2, e % 2);
-      if (index_[e] && o.table == sd.table && o.col.data == sd.col.data && o.col.type == sd.col.type) {
+      if (index_[e] && o.table == sd.table && o.col.data == sd.col.data && o.col.type == sd.col.type &&
+          index_enc(index_[e]) == enc) {
         index_[c * 2 + s] = *out = index_[e];
         shared_.insert(c * 2 + s);
         return KG_OK;
       }
     }
-    int st = kg_index_build(ctx_, &sd.col, tables_[sd.table].d_rows, tables_[sd.table].card, &index_[c * 2 + s]);
+    int st = index_build(ctx_, &sd.col, tables_[sd.table].d_rows, tables_[sd.table].card, enc, &index_[c * 2 + s]);
     if (st != KG_OK) return st;
     *out = index_[c * 2 + s];
     return KG_OK;
@@ -443,7 +494,10 @@ class SkinnerJoin {
       }
       int64_t next = 0;
       const int32_t min_id = static_cast<int32_t>(offset_[t] + 1);
-      if (via >= 0) {
+      const bool intersect = link.size() >= 2 && link.size() <= static_cast<size_t>(kMaxLinks);
+      if (intersect) {
+        if ((st = Intersect(link, pos_of, width, cur, min_id, &next)) != KG_OK) return st;
+      } else if (via >= 0) {
         const int c = link[via].first, s = link[via].second;
         const Side src = side(c, 1 - s);
         for (;;) {
@@ -479,8 +533,8 @@ class SkinnerJoin {
       Filter f{};
       f.width = width;
       f.pos = pos_of[t];
-      f.min_id = via >= 0 ? min_id : 0;
-      for (size_t l = 0; l < link.size(); l++) {
+      f.min_id = (via >= 0 && !intersect) ? min_id : 0;
+      for (size_t l = 0; l < link.size() && !intersect; l++) {
         if (static_cast<int>(l) == via) continue;
         if (f.nchecks == kMaxChecks) return kg_fail(ctx_, KG_ERR_UNSUPPORTED, "skinner join: more than %d "
                                                     "conditions close on one table", kMaxChecks);
@@ -547,8 +601,70 @@ class SkinnerJoin {
   const std::vector<int64_t>& offsets() const { return offset_; }
 
  private:
+  // Key encoding of condition c's indexes: REAL comparison when either side
+  // is REAL, else the integer width of the wider side.
+  int IndexEncOf(int c) const {
+    const int32_t a = conds_[c].left.type, b = conds_[c].right.type;
+    if (a == KG_REAL || b == KG_REAL) return kEncF64;
+    return (a == KG_BIGINT || b == KG_BIGINT) ? kEncI64 : kEncI32;
+  }
+
   Side side(size_t c, int s) const {
     return s == 0 ? Side{conds_[c].left_table, conds_[c].left} : Side{conds_[c].right_table, conds_[c].right};
+  }
+
+  // Joins table t through every linking condition at once: the buckets of
+  // each condition's index for every partial tuple in a_, their sorted
+  // intersection from t's first incomplete tuple id on, and the extended
+  // tuples into a_'s successor b_ (swapped by the caller).
+  int Intersect(const std::vector<std::pair<int, int>>& link, const std::vector<int>& pos_of, int width, int64_t cur,
+                int32_t min_id, int64_t* next) {
+    const int k = static_cast<int>(link.size());
+    BucketSrc bs{};
+    bs.k = k;
+    bs.width = width;
+    int st;
+    for (int j = 0; j < k; j++) {
+      const int c = link[j].first, s = link[j].second;
+      kg_index* idx = nullptr;
+      if ((st = Index(c, s, &idx)) != KG_OK) return st;
+      const Side src = side(c, 1 - s);
+      bs.pos[j] = pos_of[src.table];
+      bs.key[j] = src.col;
+      bs.rows[j] = tables_[src.table].d_rows;
+      index_view(idx, &bs.dir[j], &bs.tids[j]);
+    }
+    if ((st = lists_.reserve(ctx_, static_cast<size_t>(cur) * k * 8)) != KG_OK ||
+        (st = sizes_.reserve(ctx_, static_cast<size_t>(cur) * k * 4)) != KG_OK ||
+        (st = nt_.reserve(ctx_, static_cast<size_t>(cur) * 4)) != KG_OK ||
+        (st = offs_.reserve(ctx_, static_cast<size_t>(cur + 1) * 8)) != KG_OK)
+      return st;
+    hipLaunchKernelGGL(k_sk_buckets, dim3(sgrid(ctx_, cur)), dim3(kSBlock), 0, ctx_->stream, bs, a_.as<int32_t>(), cur,
+                       lists_.as<const int32_t*>(), sizes_.as<int32_t>());
+    hipLaunchKernelGGL(k_sk_fill, dim3(sgrid(ctx_, cur)), dim3(kSBlock), 0, ctx_->stream, nt_.as<int32_t>(), min_id,
+                       cur);
+    KG_LAUNCH_CHECK(ctx_);
+    int64_t total = 0;
+    for (;;) {
+      const int64_t cap = static_cast<int64_t>(ids_.bytes / 4);
+      st = kg_intersect_batch(ctx_, cur, k, lists_.as<const int32_t*>(), sizes_.as<int32_t>(), nt_.as<int32_t>(),
+                              offs_.as<int64_t>(), ids_.as<int32_t>(), cap, &total);
+      if (st == KG_ERR_CAPACITY && total > cap) {
+        if (total > INT32_MAX) return kg_fail(ctx_, KG_ERR_CAPACITY, "skinner join: %lld partial tuples in one "
+                                              "step; lower budget_per_episode", (long long)total);
+        if ((st = ids_.reserve(ctx_, static_cast<size_t>(total) * 4)) != KG_OK) return st;
+        continue;
+      }
+      if (st != KG_OK) return st;
+      break;
+    }
+    if ((st = b_.reserve(ctx_, static_cast<size_t>(total) * 4 * (width + 1))) != KG_OK) return st;
+    if (total > 0)
+      hipLaunchKernelGGL(k_sk_from_lists, dim3(sgrid(ctx_, total)), dim3(kSBlock), 0, ctx_->stream, a_.as<int32_t>(),
+                         cur, width, offs_.as<int64_t>(), ids_.as<int32_t>(), total, b_.as<int32_t>());
+    KG_LAUNCH_CHECK(ctx_);
+    *next = total;
+    return KG_OK;
   }
 
   int Compact(const Filter& f, int64_t* n) {
@@ -621,6 +737,7 @@ class SkinnerJoin {
   std::set<int> shared_;          // sides that reuse an earlier side's index
   std::map<std::vector<int>, double> per_leftmost_;
   DevBuf a_, b_, keep_, pos_, cub_;
+  DevBuf lists_, sizes_, nt_, offs_, ids_;  // bucket intersection (Intersect)
 };
 
 }  // namespace

@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "kg_ops.h"
+#include "kg_index.h"
 #include "kg_strops.h"
 
 namespace kg {
@@ -110,52 +111,21 @@ struct Expand {
   const int32_t* tids;    // the index's tid array
 };
 
-struct DirView {
-  const int64_t* keys;
-  const int64_t* offset;
-  const int32_t* count;
-  uint64_t nbuckets;
-  int64_t dmin;     // dense directory when dspan > 0
-  uint64_t dspan;
-};
-
-__device__ __forceinline__ void dir_find(const DirView& d, int64_t k, int64_t& off, int32_t& cnt) {
-  off = 0;
-  cnt = 0;
-  if (d.dspan) {
-    const int64_t u = k - d.dmin;
-    if (u >= 0 && static_cast<uint64_t>(u) < d.dspan) {
-      cnt = d.count[u];
-      if (cnt) off = d.offset[u];
-    }
-    return;
-  }
-  uint64_t b = __umul64hi(mix64(static_cast<uint64_t>(k)), d.nbuckets);
-  for (uint64_t probes = 0; probes < d.nbuckets; probes++) {
-    bool empty = false;
-    for (int s = 0; s < 8; s++) {
-      const int64_t v = d.keys[b * 8 + s];
-      if (v == k) {
-        off = d.offset[b * 8 + s];
-        cnt = d.count[b * 8 + s];
-        return;
-      }
-      empty |= v == INT64_MIN;
-    }
-    if (empty) return;
-    b = (b + 1 == d.nbuckets) ? 0 : b + 1;
-  }
+__global__ void k_zero_i64(int64_t* p, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 // Pass 1: bucket (offset, size) per partial tuple.
-__global__ void k_expand_count(Expand e, DirView d, const int32_t* tuples, int64_t n, int64_t* boff, int64_t* bcnt) {
+__global__ void k_expand_count(Expand e, IndexDir d, const int32_t* tuples, int64_t n, int64_t* boff,
+                               int64_t* bcnt) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     const int32_t tid = tuples[i * e.width + e.src_pos];
     const int64_t r = e.src_rows ? e.src_rows[tid] : tid;
-    int64_t off = 0;
+    int64_t k = 0, off = 0;
     int32_t cnt = 0;
-    if (!(e.key.nulls && e.key.nulls[r])) dir_find(d, static_cast<int32_t>(load_int(e.key.type, e.key.data, r)), off, cnt);
+    if (index_key(e.key, r, d.enc, k)) index_find(d, k, off, cnt);
     boff[i] = off;
     bcnt[i] = cnt;
   }
@@ -363,14 +333,14 @@ extern "C" int kg_index_join_expand(kg_ctx* ctx, int64_t n, int32_t width, const
   KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "tuple count out of range");
   KG_REQUIRE(ctx, n == 0 || d_tuples, "d_tuples is NULL");
   KG_REQUIRE(ctx, out_cap == 0 || d_out, "d_out is NULL");
-  if (src_key->type != KG_INT && src_key->type != KG_DATE && src_key->type != KG_ENUM && src_key->type != KG_SMALLINT)
-    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "join keys must be 16/32-bit integers");
+  if (type_width(src_key->type) == 0)
+    return kg_fail(ctx, KG_ERR_UNSUPPORTED, "join keys must be fixed-width (type %d)", src_key->type);
   *out_n = 0;
   if (n == 0) return KG_OK;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  DirView d;
+  IndexDir d;
   const int32_t* tids = nullptr;
-  index_dir(idx, &d.keys, &d.offset, &d.count, &d.nbuckets, &d.dmin, &d.dspan, &tids);
+  index_view(idx, &d, &tids);
   Expand e{width, src_pos, *src_key, d_src_rows, tids};
   size_t cub_scan = 0;
   hipcub::DeviceScan::ExclusiveSum(nullptr, cub_scan, static_cast<int64_t*>(nullptr), static_cast<int64_t*>(nullptr),
@@ -389,7 +359,7 @@ extern "C" int kg_index_join_expand(kg_ctx* ctx, int64_t n, int32_t width, const
   int64_t* boff = reinterpret_cast<int64_t*>(w + o_boff);
   int64_t* bcnt = reinterpret_cast<int64_t*>(w + o_bcnt);
   int64_t* starts = reinterpret_cast<int64_t*>(w + o_start);
-  KG_HIP_CHECK(ctx, hipMemsetAsync(bcnt + n, 0, 8, ctx->stream));
+  hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(64), 0, ctx->stream, bcnt + n, 1);
   kg_time_begin(ctx);
   hipLaunchKernelGGL(k_expand_count, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream, e, d, d_tuples, n, boff, bcnt);
   KG_LAUNCH_CHECK(ctx);

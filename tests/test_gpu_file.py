@@ -67,10 +67,6 @@ def test_index_file(ctx, name):
     print("index", name, flush=True)
     sql = TYPES[typ]
     col = ctx.file_column(path_of(name), sql)
-    if typ in ("bigint", "real"):  # device column indexes take 16/32-bit keys
-        with pytest.raises(RuntimeError, match="status 5"):
-            ctx.file_index(path_of(name, ".kdbindex"), sql)
-        return
     idx = ctx.file_index(path_of(name, ".kdbindex"), sql)
     nulls = col.nulls.cpu().numpy() if col.nulls is not None else np.zeros(len(col.data), np.int8)
     live = torch.as_tensor(np.nonzero(nulls == 0)[0].astype(np.int32)).to(col.data.device)

@@ -72,6 +72,57 @@ def test_index_build_and_lookup(ctx, n, domain, with_nulls, with_sel, order):
     idx.free()
 
 
+@pytest.mark.parametrize("kind", ["bigint_sparse", "bigint_dense", "bigint_extremes", "real"])
+def test_index_64bit_keys(ctx, kind):
+    """BIGINT and REAL keys: runs grouped by value (REAL compares as double:
+    -0.0 and 0.0 are one key, NaN is never indexed nor found), tuple ids
+    ascending per key, NULLs skipped; the hashed directory with INT64_MIN as a
+    key, and the dense directory for a small BIGINT span far from zero."""
+    rng = np.random.default_rng(len(kind))
+    n = 300_000
+    if kind == "bigint_sparse":
+        keys = rng.integers(-2**63, 2**63 - 1, 20_000, dtype=np.int64)[rng.integers(0, 20_000, n)]
+        typ = abi.KG_BIGINT
+    elif kind == "bigint_dense":
+        keys = rng.integers(0, 50_000, n).astype(np.int64) + 7 * 10**15
+        typ = abi.KG_BIGINT
+    elif kind == "bigint_extremes":
+        keys = rng.choice(np.array([-2**63, 2**63 - 1, 0, -1, 5], dtype=np.int64), n)
+        typ = abi.KG_BIGINT
+    else:
+        keys = rng.integers(-500, 500, n).astype(np.float64) / 4
+        keys[rng.random(n) < 0.02] = -0.0
+        keys[rng.random(n) < 0.02] = np.nan
+        keys[rng.random(n) < 0.01] = np.inf
+        typ = abi.KG_REAL
+    nulls = (rng.random(n) < 0.05).astype(np.int8)
+    col = Column(typ, t(keys), t(nulls))
+    idx = ColumnIndex(ctx, col)
+    want = {}
+    for i in range(n):
+        v = keys[i]
+        if nulls[i] or (typ == abi.KG_REAL and np.isnan(v)):
+            continue
+        kk = float(v) + 0.0 if typ == abi.KG_REAL else int(v)   # + 0.0 folds -0.0
+        want.setdefault(kk, []).append(i)
+    assert idx.nkeys == len(want) and idx.ntids == sum(len(v) for v in want.values())
+    probe = list(want.keys())
+    if typ == abi.KG_REAL:
+        probe += [-0.0, np.nan, 1e300, -np.inf]
+    else:
+        probe += [int(x) for x in rng.integers(-2**63, 2**63 - 1, 500, dtype=np.int64)] + [-2**63, 2**63 - 1]
+    pa = np.array(probe, dtype=np.float64 if typ == abi.KG_REAL else np.int64)
+    ptrs, cnts = idx.lookup(Column(typ, t(pa)))
+    offs = idx.offsets_of(ptrs).cpu().numpy()
+    cnts = cnts.cpu().numpy()
+    tids = idx.tids().cpu().numpy()
+    for j, k in enumerate(pa.tolist()):
+        kk = (k + 0.0) if typ == abi.KG_REAL else k
+        exp = [] if (typ == abi.KG_REAL and k != k) else want.get(kk, [])
+        assert tids[offs[j]:offs[j] + cnts[j]].tolist() == exp, (kind, k)
+    idx.free()
+
+
 @pytest.mark.parametrize("k", [1, 2, 3, 5])
 def test_intersect_batch(ctx, k):
     rng = np.random.default_rng(100 + k)

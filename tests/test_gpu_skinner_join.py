@@ -57,7 +57,7 @@ FANOUT = {"chain5": 4, "star4": 4, "cycle4": 8, "clique3": 25, "disconnected": 4
 
 def random_case(graph, seed, sel=False):
     """Tables of 20..400 tuples, keys over small domains, NULLs, mixed key
-    types (and widths, joined by checked equality instead of an index);
+    types and widths (mixed widths joined through a 64-bit index);
     optionally children that are selections of larger tables."""
     rng = np.random.default_rng(seed)
     nt, edges = GRAPHS[graph]
@@ -70,7 +70,7 @@ def random_case(graph, seed, sel=False):
     for i, (a, b) in enumerate(edges):
         kt, dt = KEY_TYPES[(seed + i) % len(KEY_TYPES)]
         if kt in (abi.KG_INT, abi.KG_SMALLINT, abi.KG_DATE, abi.KG_ENUM) and (seed + i) % 3 == 0:
-            kt2, dt2 = abi.KG_BIGINT, np.int64   # mixed widths: checked equality, not an index
+            kt2, dt2 = abi.KG_BIGINT, np.int64   # mixed widths: a 64-bit index
         else:
             kt2, dt2 = kt, dt
         dom = max(4, max(base) // FANOUT[graph])  # about FANOUT matches per key
@@ -99,6 +99,60 @@ def test_skinner_random_graphs_vs_oracle(graph, seed):
         g = got.cpu().numpy()
         assert np.array_equal(g[np.lexsort(g.T[::-1])], want), (graph, budget, jseed)
         assert st["distinct_tuples"] == len(want)
+    ctx.close()
+
+
+def test_skinner_bigint_1m_by_1m():
+    """1M x 1M BIGINT equi-join through the device column index (keys spread
+    over the int64 range, so no 32-bit narrowing applies), against the
+    oracle's chained hash join."""
+    ctx = Context(0)
+    rng = np.random.default_rng(8)
+    n = 1_000_000
+    dom = rng.integers(-2**62, 2**62, 600_000)
+    a = dom[rng.integers(0, len(dom), n)]
+    b = dom[rng.integers(0, len(dom), n)]
+    miss = rng.random(n) < 0.2
+    b[miss] = rng.integers(-2**62, 2**62, int(miss.sum()))
+    ca = Column(abi.KG_BIGINT, torch.from_numpy(a).to(DEV))
+    cb = Column(abi.KG_BIGINT, torch.from_numpy(b).to(DEV))
+    op, ob = po.hash_join([po.Col(po.BIGINT, a)], [po.Col(po.BIGINT, b)])
+    want = np.stack([ob, op], 1).astype(np.int32)
+    want = want[np.lexsort(want.T[::-1])]
+    got, st = skinner_join(ctx, [(n, None), (n, None)], [(0, ca, 1, cb)], budget=1 << 22, seed=3)
+    g = got.cpu().numpy()
+    assert np.array_equal(g[np.lexsort(g.T[::-1])], want)
+    assert st["distinct_tuples"] == len(want)
+    ctx.close()
+
+
+@pytest.mark.parametrize("kt", [abi.KG_INT, abi.KG_BIGINT, abi.KG_REAL])
+def test_skinner_cycle4_intersection(kt):
+    """A cyclic 4-table join (t0-t1-t2-t3-t0): the last table of every order
+    is linked by two conditions, which the episode joins through the
+    intersection of both index buckets. Tables of 600 tuples with NULLs,
+    against the oracle's result set."""
+    ctx = Context(0)
+    rng = np.random.default_rng(int(kt) + 40)
+    dt = {abi.KG_INT: np.int32, abi.KG_BIGINT: np.int64, abi.KG_REAL: np.float64}[kt]
+    card = 600
+    edges = [(0, 1), (1, 2), (2, 3), (3, 0)]
+    hconds, dconds = [], []
+    for i, (x, y) in enumerate(edges):
+        lk = rng.integers(0, 100, card).astype(dt)
+        rk = rng.integers(0, 100, card).astype(dt)
+        if kt == abi.KG_REAL:
+            lk[lk == 0] = -0.0                        # -0.0 joins 0.0
+        ln = (rng.random(card) < 0.05).astype(np.int8)
+        hconds.append((x, lk, ln, y, rk, None))
+        dconds.append((x, Column(kt, torch.from_numpy(lk).to(DEV), torch.from_numpy(ln).to(DEV)), y,
+                       Column(kt, torch.from_numpy(rk).to(DEV))))
+    want = np.array(po.skinner_join([card] * 4, hconds), dtype=np.int32).reshape(-1, 4)
+    print(f"  cycle4 {len(want)} result tuples", flush=True)
+    for budget, seed in [(0, 1), (5000, 7)]:
+        got, st = skinner_join(ctx, [(card, None)] * 4, dconds, budget=budget, seed=seed)
+        g = got.cpu().numpy()
+        assert np.array_equal(g[np.lexsort(g.T[::-1])], want), (kt, budget, seed)
     ctx.close()
 
 
