@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-probe-rows", type=int, default=1 << 25, help="probe rows of the C2 CPU baseline sample")
     ap.add_argument("--lf", type=float, default=0.0, help="join table load factor (0: autotuned)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-skinner", action="store_true", help="C3 without the Skinner-driven runs")
+    ap.add_argument("--only", choices=["c0_star", "c2_join", "c3_chain", "serving"],
+                    help="run one workload alone (for profiling) and print only its section")
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-saturation-only", action="store_true",
@@ -99,6 +102,45 @@ def copy_calibration():
     return 0
 
 
+def pmc_query_summary(fetch_csv, write_csv, name, first_kernel):
+    """HBM bytes of one query run alone under --pmc (bench.py --only <name>
+    --warmup 0 --steps 1): FETCH_SIZE + WRITE_SIZE summed over every dispatch
+    from the first one whose kernel name contains first_kernel (the query's
+    first kernel; everything before it is table generation) to the end,
+    scaled by the calibration already in profiles/traffic.json, stored under
+    "queries"/name with the kernels dispatched."""
+    import csv
+
+    def dispatches(path, counter):
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            d = acc.setdefault(int(r["Dispatch_Id"]), [r["Kernel_Name"], 0.0])
+            d[1] += float(r["Counter_Value"])
+        seq = [acc[k] for k in sorted(acc)]
+        start = next(i for i, (k, _) in enumerate(seq) if first_kernel in k)
+        return seq[start:]
+
+    with open(TRAFFIC_JSON) as fh:
+        out = json.load(fh)
+    ff, wf = out["calibration"]["fetch_factor"], out["calibration"]["write_factor"]
+    fseq, wseq = dispatches(fetch_csv, "FETCH_SIZE"), dispatches(write_csv, "WRITE_SIZE")
+    assert [k for k, _ in fseq] == [k for k, _ in wseq], "the two passes dispatched different kernels"
+    f, w = sum(v for _, v in fseq), sum(v for _, v in wseq)
+    kernels = {}
+    for k, _ in fseq:
+        short = k.split("(")[0].replace("void ", "")[:60]
+        kernels[short] = kernels.get(short, 0) + 1
+    out.setdefault("queries", {})[name] = {
+        "first_kernel": first_kernel, "dispatches": len(fseq), "fetch_kib": f, "write_kib": w,
+        "bytes_per_query": round((f * ff + w * wf) * 1024), "kernels": kernels}
+    with open(TRAFFIC_JSON, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out["queries"][name], indent=1))
+    return 0
+
+
 def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
     """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of the reported kernels,
     scaled by the copy kernel's reading of its known size, into
@@ -136,6 +178,13 @@ def pmc_summary(fetch_csv, write_csv, copy_fetch_csv, copy_write_csv):
         f, w = max(fk), max(wk)
         out["kernels"][short] = {"fetch_kib": f, "write_kib": w,
                                  "bytes_per_launch": round((f * ff + w * wf) * 1024)}
+    try:
+        with open(TRAFFIC_JSON) as fh:
+            prev = json.load(fh).get("queries")
+        if prev:
+            out["queries"] = prev
+    except (OSError, ValueError):
+        pass
     with open(TRAFFIC_JSON, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out, indent=1))
@@ -337,10 +386,11 @@ def bench_star_c0(ctx, a):
         r = ht.scan_probe_group_agg(cols, preds, 0, cat, 25, aggs, max_groups=64)
         ht.free()
         return r
-    for _ in range(3):
+    # Alone (--only, e.g. under a profiler) the given warmup/steps; else 3 / 20.
+    warm, steps = (a.warmup, max(1, a.steps)) if a.only else (3, 20)
+    for _ in range(warm):
         query()
     torch.cuda.synchronize()
-    steps = 20
     ctx.set_timing(True)
     ctx.kernel_stats(reset=True)
     t0 = time.perf_counter()
@@ -354,7 +404,7 @@ def bench_star_c0(ctx, a):
     out = {"workload": "1M-row fact (30% day range) join 10k-row dim, GROUP BY 25 cats: SUM(price*qty), COUNT(*)",
            "groups": int(len(codes)), "query_ms": round(wall * 1e3, 4), "kernel_ms": round(per_query_kernel_ms, 4),
            "rows_per_s": 1_000_000 / wall,
-           "roofline": roofline(C0_BYTES, per_query_kernel_ms, measured_traffic("k_join_group_agg"))}
+           "roofline": roofline(C0_BYTES, per_query_kernel_ms, query_traffic("c0_star"))}
     if not a.no_cpu:
         import numpy as np
         from oracle import pyoracle as po
@@ -388,9 +438,11 @@ def bench_chain_c3(ctx, a):
     tabs, spec, htabs = skinner_tables(scale=1.0, theta=0.0, device="cuda:0")
     torch.cuda.synchronize()
     order = [0, 1, 2, 3, 4]
-    chain_query(ctx, tabs, spec, order)
+    # Alone (--only, e.g. under a profiler) the given warmup/steps; else 1 / 3.
+    warm, steps = (a.warmup, max(1, a.steps)) if a.only else (1, 3)
+    for _ in range(warm):
+        chain_query(ctx, tabs, spec, order)
     torch.cuda.synchronize()
-    steps = 3
     t0 = time.perf_counter()
     for _ in range(steps):
         tuples, mn_text, mn_int, stats = chain_query(ctx, tabs, spec, order)
@@ -422,7 +474,7 @@ def bench_chain_c3(ctx, a):
         out["oracle_equal"] = bool(same)
     # The same query with the join order chosen per episode by the UCT agent (kg_skinner_join).
     from kushdb_amd.chain import skinner_chain_query
-    for budget in (1 << 20, 1 << 22):
+    for budget in () if a.no_skinner else (1 << 20, 1 << 22):
         skinner_chain_query(ctx, tabs, spec, budget=budget, seed=1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -440,16 +492,20 @@ def bench_chain_c3(ctx, a):
     return out
 
 
-def c3_traffic():
-    """HBM bytes of one C3 chain query: every dispatch of that query, FETCH_SIZE
-    + WRITE_SIZE scaled by the copy calibration (profiles/traffic.json), or
-    None when not measured."""
+def query_traffic(name):
+    """HBM bytes of one whole query (every dispatch of it, FETCH_SIZE +
+    WRITE_SIZE scaled by the copy calibration; profiles/traffic.json
+    "queries"), or None when not measured."""
     try:
         with open(TRAFFIC_JSON) as f:
-            v = json.load(f).get("queries", {}).get("c3_chain", {}).get("bytes_per_query")
+            v = json.load(f).get("queries", {}).get(name, {}).get("bytes_per_query")
             return int(v) if v is not None else None
     except (OSError, ValueError):
         return None
+
+
+def c3_traffic():
+    return query_traffic("c3_chain")
 
 
 def bench_serving(ctx, a):
@@ -548,6 +604,8 @@ def main():
         return copy_calibration()
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
         return pmc_summary(*sys.argv[2:6])
+    if len(sys.argv) > 1 and sys.argv[1] == "--pmc-query-summary":
+        return pmc_query_summary(*sys.argv[2:6])
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design note, "
@@ -557,6 +615,11 @@ def main():
     from kushdb_amd.runtime import Context
     torch.cuda.init()
     ctx = Context(0)
+    if a.only:
+        run_one = {"c0_star": bench_star_c0, "c2_join": bench_join, "c3_chain": bench_chain_c3,
+                   "serving": bench_serving}[a.only]
+        print(json.dumps({a.only: run_one(ctx, a)}), flush=True)
+        return 0
     t, wall, kms, groups = bench_scan_agg(ctx, a)
     n = a.rows
     rows_per_s = n * a.steps / wall

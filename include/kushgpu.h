@@ -506,7 +506,9 @@ int kg_text_pairs_equal(kg_ctx* ctx, const kg_text_col* a, const int32_t* d_rows
  * NULL): the GetBucket + handler chain of
  * permutable_skinner_join_translator.cc:359-433. Output tuples have width+1
  * ids, grouped by input tuple in input order, bucket order (ascending ids)
- * inside a group. *out_n receives the total even when it exceeds out_cap
+ * inside a group. d_tuples NULL (width 1) stands for the n one-table tuples
+ * (0), (1), ..., (n-1): the first table of a join order over its whole
+ * selection. *out_n receives the total even when it exceeds out_cap
  * (KG_ERR_CAPACITY). */
 int kg_index_join_expand(kg_ctx* ctx, int64_t n, int32_t width, const int32_t* d_tuples, int32_t src_pos,
                          const kg_col* src_key, const int32_t* d_src_rows, const kg_index* idx,

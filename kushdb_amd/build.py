@@ -1,6 +1,6 @@
 """Build recipe for the kushgpu shared library (gfx950) and the CPU oracle.
 
-    python -m kushdb_amd.build            # libkushgpu.so + oracle
+    python -m kushdb_amd.build            # libkushgpu.so + oracle + tests/cxx drivers
     python -m kushdb_amd.build --lib-only
 
 The library is compiled in-tree to kushdb_amd/libkushgpu.so with hipcc for
@@ -85,6 +85,15 @@ def build_oracle(verbose=False):
     subprocess.run(cmd, check=True)
 
 
+def build_cxx_tests(verbose=False):
+    """The C++ drivers of the C ABI under tests/cxx (need the library and the
+    oracle built first)."""
+    cmd = ["make", "-s", "-C", os.path.join(ROOT, "tests", "cxx")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib-only", action="store_true")
@@ -93,6 +102,7 @@ def main(argv=None):
     build_lib(a.verbose)
     if not a.lib_only:
         build_oracle(a.verbose)
+        build_cxx_tests(a.verbose)
     return 0
 
 

@@ -121,7 +121,7 @@ __global__ void k_expand_count(Expand e, IndexDir d, const int32_t* tuples, int6
                                int64_t* bcnt) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int32_t tid = tuples[i * e.width + e.src_pos];
+    const int32_t tid = tuples ? tuples[i * e.width + e.src_pos] : static_cast<int32_t>(i);
     const int64_t r = e.src_rows ? e.src_rows[tid] : tid;
     int64_t k = 0, off = 0;
     int32_t cnt = 0;
@@ -145,7 +145,11 @@ __global__ void k_expand_write(Expand e, const int32_t* tuples, int64_t n, const
       else hi = mid;
     }
     const int64_t i = lo;
-    for (int q = 0; q < e.width; q++) out[j * w1 + q] = tuples[i * e.width + q];
+    if (tuples) {
+      for (int q = 0; q < e.width; q++) out[j * w1 + q] = tuples[i * e.width + q];
+    } else {
+      out[j * w1] = static_cast<int32_t>(i);
+    }
     out[j * w1 + e.width] = e.tids[boff[i] + (j - starts[i])];
   }
 }
@@ -331,7 +335,7 @@ extern "C" int kg_index_join_expand(kg_ctx* ctx, int64_t n, int32_t width, const
   KG_REQUIRE(ctx, idx && src_key && (n == 0 || src_key->data) && out_n, "null pointer argument");
   KG_REQUIRE(ctx, width >= 1 && width <= 15 && src_pos >= 0 && src_pos < width, "bad width / position");
   KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "tuple count out of range");
-  KG_REQUIRE(ctx, n == 0 || d_tuples, "d_tuples is NULL");
+  KG_REQUIRE(ctx, d_tuples || width == 1, "d_tuples is NULL with width %d (the identity tuples have width 1)", width);
   KG_REQUIRE(ctx, out_cap == 0 || d_out, "d_out is NULL");
   if (type_width(src_key->type) == 0)
     return kg_fail(ctx, KG_ERR_UNSUPPORTED, "join keys must be fixed-width (type %d)", src_key->type);

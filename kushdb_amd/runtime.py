@@ -640,9 +640,15 @@ def text_minmax(ctx, col, rows=None, is_max=False, cap=4096):
     return pos.value, buf.raw[: min(ln.value, cap)]
 
 
-def index_join_expand(ctx, tuples, src_pos, src_key, src_rows, index, out_cap=None):
-    """Extend [n, width] partial tuples by the index bucket of each tuple's key."""
-    n, width = tuples.shape
+def index_join_expand(ctx, tuples, src_pos, src_key, src_rows, index, out_cap=None, n=None):
+    """Extend [n, width] partial tuples by the index bucket of each tuple's key.
+    tuples None: the n one-table tuples (0), ..., (n-1)."""
+    if tuples is None:
+        width = 1
+        dev = f"cuda:{ctx.device}"
+    else:
+        n, width = tuples.shape
+        dev = tuples.device
     total = ctypes.c_int64()
     d = src_key.desc()
     if out_cap is None:
@@ -652,7 +658,7 @@ def index_join_expand(ctx, tuples, src_pos, src_key, src_rows, index, out_cap=No
         if st not in (abi.KG_OK, abi.KG_ERR_CAPACITY):
             ctx.check(st)
         out_cap = total.value
-    out = torch.empty((max(out_cap, 1), width + 1), dtype=torch.int32, device=tuples.device)
+    out = torch.empty((max(out_cap, 1), width + 1), dtype=torch.int32, device=dev)
     ctx.check(ctx.lib.kg_index_join_expand(ctx.h, n, width, _ptr(tuples), src_pos, ctypes.byref(d), _ptr(src_rows),
                                            index.h, out_cap, _ptr(out), ctypes.byref(total)))
     return out[: total.value]
