@@ -293,7 +293,7 @@ def bench_join_theta(ctx, a, theta, cpu):
     wall = time.perf_counter() - t0
     kms, launches = ctx.kernel_stats(reset=True)
     ctx.set_timing(False)
-    pms = kms / launches
+    pms = kms / steps   # every timed span of the steps' calls
     out["join_probe"] = {"rows": npr, "theta": theta, "matches": card, "kernel_ms": round(pms, 3),
                          "ms_per_step": round(wall / steps * 1e3, 3), "rows_per_s": npr / (pms * 1e-3),
                          "roofline": roofline(npr * PROBE_BYTES_PER_ROW, pms,
@@ -308,7 +308,7 @@ def bench_join_theta(ctx, a, theta, cpu):
         gp, _ = ht.probe(pcol, out_cap=card, out=(op, ob))
     kms, launches = ctx.kernel_stats(reset=True)
     ctx.set_timing(False)
-    ems = kms / launches
+    ems = kms / steps
     assert gp.numel() == card
     out["join_probe_emit"] = {"rows": npr, "theta": theta, "pairs": card, "kernel_ms": round(ems, 3),
                               "rows_per_s": npr / (ems * 1e-3),
@@ -322,7 +322,7 @@ def bench_join_theta(ctx, a, theta, cpu):
         k, s, c, _ = ht.probe_agg(pcol, pv, max_groups=nb)
     kms, launches = ctx.kernel_stats(reset=True)
     ctx.set_timing(False)
-    ams = kms / launches
+    ams = kms / steps
     groups = int(k.numel())
     # SURVEY cost model: matched rows (key 8 + value 8 + 64 B update), every
     # probe row's key and bucket (8 + 64), 24 B per group out.

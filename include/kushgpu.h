@@ -156,6 +156,21 @@ int kg_ctx_set_probe_regions(kg_ctx* ctx, int32_t regions);
  * many regions (rounded up to a multiple of 8) for any build. No counterpart
  * in the reference (its inserts are per tuple, runtime/hash_table.cc:18-27). */
 int kg_ctx_set_build_regions(kg_ctx* ctx, int32_t regions);
+/* Probe + GROUP BY matched build key (kg_ht_probe_agg without first rows) of
+ * many rows: the probe rows are grouped by the table region of their key's home
+ * bucket (two counting-sort passes), then one workgroup per region probes its
+ * rows against that slice of the table and keeps SUM/COUNT per matched slot in
+ * LDS, writing its groups straight to the output, instead of two device atomics
+ * per matched row on a table-sized accumulator array. Frequent keys (sampled)
+ * are summed per workgroup during the partition and folded in by their region.
+ * Results are the same (integer sums and counts are exact). regions: -1
+ * (default) partitions when at least 2^22 rows probe a table of at least 64 MB
+ * of accumulators; 0 never; > 0 partitions into at least that many regions (a
+ * multiple of 128) for any probe. A key whose slot lies more than 4 buckets past
+ * its home region's end, or a wide table's free-marker key, makes the call run
+ * again on the atomic path. No counterpart in the reference (per-tuple
+ * UpdateOrInsert, compile/proxy/aggregate_hash_table.cc:187-243). */
+int kg_ctx_set_agg_regions(kg_ctx* ctx, int32_t regions);
 /* The FP64 addition order of the last kg_scan_group_agg call that used the
  * fixed-order kernels: T = *nblocks * 256 lanes; lane t folds, from +0.0, the
  * qualifying rows of chunks c = t + k*T (k ascending) of *vec consecutive rows
@@ -272,8 +287,9 @@ int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, con
  * key in a probe-key sample (uniform / moderate / heavy). The first call for a
  * class builds a table over the build keys (at most 64M rows) and times probes
  * of a probe sample (at most 32M rows) under each candidate, by coordinate
- * descent over load factor {0.35, 0.5, 0.65}, workgroups per CU {8, 16, 32}
- * and regions {0, 64, 128}; later calls for the class reuse the choice. The
+ * descent over load factor {0.35, 0.5, 0.65} (by build time plus the probe
+ * time scaled to all probe rows), workgroups per CU {8, 16, 32} and regions
+ * {0, 64, 128} (by probe time); later calls for the class reuse the choice. The
  * probe settings are applied to the context; the load factor is for
  * kg_ht_create. No counterpart in the reference (fixed std::unordered_map). */
 typedef struct kg_join_tuning {

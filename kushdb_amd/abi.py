@@ -145,6 +145,7 @@ SIGNATURES = {
     "kg_ctx_synchronize": (_i32, [_vp]),
     "kg_ctx_set_deterministic": (_i32, [_vp, _i32]),
     "kg_ctx_set_build_regions": (_i32, [_vp, _i32]),
+    "kg_ctx_set_agg_regions": (_i32, [_vp, _i32]),
     "kg_ctx_last_path": (ctypes.c_char_p, [_vp]),
     "kg_ctx_set_probe_regions": (_i32, [_vp, _i32]),
     "kg_ctx_set_timing": (_i32, [_vp, _i32]),

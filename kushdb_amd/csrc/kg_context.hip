@@ -233,4 +233,12 @@ int kg_ctx_set_build_regions(kg_ctx* ctx, int32_t regions) {
   return KG_OK;
 }
 
+int kg_ctx_set_agg_regions(kg_ctx* ctx, int32_t regions) {
+  if (!ctx) return KG_ERR_INVALID;
+  KG_REQUIRE(ctx, regions >= -1 && regions <= 65536, "aggregation regions must be -1, 0 or up to 65536 (got %d)",
+             regions);
+  ctx->agg_regions = regions;
+  return KG_OK;
+}
+
 }  // extern "C"

@@ -23,6 +23,12 @@ struct kg_ctx {
   // beyond the caches), 0 always inserts directly, > 0 at least that many
   // regions (kg_ctx_set_build_regions).
   int build_regions = -1;
+  // Probe + GROUP BY matched key (kg_ht_probe_agg): -1 chooses (probe rows
+  // partitioned by table region and aggregated per region in LDS for large
+  // probes into tables beyond the caches), 0 always uses per-slot device
+  // atomics, > 0 partitions with at least that many regions
+  // (kg_ctx_set_agg_regions).
+  int agg_regions = -1;
   // Join tuning choices per (build rows, key cardinality, skew) class.
   struct JoinTune {
     uint64_t cls;

@@ -49,12 +49,21 @@ __global__ __launch_bounds__(kJBlock) void k_key_range(kg_col key, const int32_t
                                                         unsigned long long* out) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kJBlock;
   int64_t lo = INT64_MAX, hi = INT64_MIN;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x; i < n; i += stride) {
-    const int64_t r = sel ? sel[i] : i;
-    int64_t k;
-    if (!load_key(key, r, k)) continue;
-    lo = k < lo ? k : lo;
-    hi = k > hi ? k : hi;
+  constexpr int U = 8;  // loads in flight per thread
+  for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * kJBlock + threadIdx.x; i0 < n; i0 += stride * U) {
+    int64_t kk[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + u * stride;
+      ok[u] = i < n && load_key_nt(key, sel ? sel[i] : i, kk[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (!ok[u]) continue;
+      lo = kk[u] < lo ? kk[u] : lo;
+      hi = kk[u] > hi ? kk[u] : hi;
+    }
   }
   for (int s = kWave / 2; s >= 1; s >>= 1) {
     const int64_t a = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(lo), s));
@@ -129,17 +138,32 @@ constexpr int kHotHash = 512;
 constexpr int kHotSample = 1 << 16;   // probe keys sampled by k_hot_keys
 constexpr int kHotTable = 4096;       // distinct sampled keys counted
 constexpr int kHotMinCount = 3;       // sampled occurrences that make a key hot
+constexpr int kHotProbes = 16;        // table entries a sampled key may try
 
 __device__ __forceinline__ uint32_t hot_pos(int64_t k) {
   return static_cast<uint32_t>(mix64(static_cast<uint64_t>(k)) >> 32) & (kHotHash - 1);
 }
 
-// One workgroup: kHotSample probe keys at an even stride counted in an LDS
+// The sample: kHotSample probe keys at an even stride (kEmpty where the row
+// has no key), gathered by many workgroups so that the random reads overlap.
+__global__ __launch_bounds__(256) void k_hot_sample(kg_col key, const int32_t* sel, int64_t n, int64_t* out) {
+  const int64_t stride = n > kHotSample ? n / kHotSample : 1;
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (s >= kHotSample) return;
+  int64_t k = kEmpty;
+  if (s * stride < n) {
+    int64_t v;
+    if (load_key(key, sel ? sel[s * stride] : s * stride, v)) k = v;
+  }
+  out[s] = k;
+}
+
+// One workgroup: the sampled probe keys counted in an LDS
 // open-addressing table (a key is claimed by a 64-bit CAS of the free marker;
-// keys that find no entry within 64 probes are not counted, nor is the free
-// marker itself); keys seen at least kHotMinCount times become the hot list.
-__global__ __launch_bounds__(1024) void k_hot_keys(kg_col key, const int32_t* sel, int64_t n, int64_t* hot,
-                                                   int32_t* nhot) {
+// keys that find no entry within kHotProbes probes are not counted, nor is the
+// free marker itself); keys seen at least kHotMinCount times become the hot
+// list.
+__global__ __launch_bounds__(1024) void k_hot_keys(const int64_t* sample, int64_t* hot, int32_t* nhot) {
   __shared__ unsigned long long tkey[kHotTable];
   __shared__ uint32_t tcnt[kHotTable];
   __shared__ int32_t out_n;
@@ -149,15 +173,16 @@ __global__ __launch_bounds__(1024) void k_hot_keys(kg_col key, const int32_t* se
   }
   if (threadIdx.x == 0) out_n = 0;
   __syncthreads();
-  const int64_t stride = n > kHotSample ? n / kHotSample : 1;
-  for (int64_t s = threadIdx.x; s < kHotSample && s * stride < n; s += 1024) {
-    const int64_t r = sel ? sel[s * stride] : s * stride;
-    int64_t k;
-    if (!load_key(key, r, k) || k == kEmpty) continue;
+  for (int s = threadIdx.x; s < kHotSample; s += 1024) {
+    const int64_t k = sample[s];
+    if (k == kEmpty) continue;
     uint32_t p = static_cast<uint32_t>(mix64(static_cast<uint64_t>(k))) & (kHotTable - 1);
-    for (int probes = 0; probes < 64; probes++) {
-      const unsigned long long prev = atomicCAS(&tkey[p], static_cast<unsigned long long>(kEmpty),
-                                                static_cast<unsigned long long>(k));
+    for (int probes = 0; probes < kHotProbes; probes++) {
+      // A plain read first: once the table is full (more distinct keys than
+      // entries, as under uniform keys) most probes end without a CAS.
+      unsigned long long prev = tkey[p];
+      if (prev == static_cast<unsigned long long>(kEmpty))
+        prev = atomicCAS(&tkey[p], static_cast<unsigned long long>(kEmpty), static_cast<unsigned long long>(k));
       if (prev == static_cast<unsigned long long>(kEmpty) || prev == static_cast<unsigned long long>(k)) {
         atomicAdd(&tcnt[p], 1u);
         break;
@@ -188,6 +213,16 @@ __global__ __launch_bounds__(1024) void k_hot_keys(kg_col key, const int32_t* se
   }
   __syncthreads();
   if (threadIdx.x == 0) *nhot = out_n < kHot ? out_n : kHot;
+}
+
+// Device bytes of a table's hot-key buffer: the list (kHot keys), its length
+// (in the next 64 bytes), then the sample.
+constexpr size_t kHotBufBytes = kHot * 8 + 64 + static_cast<size_t>(kHotSample) * 8;
+
+void launch_hot_keys(hipStream_t stream, const kg_col& key, const int32_t* sel, int64_t n, int64_t* hot) {
+  int64_t* sample = hot + kHot + 8;
+  hipLaunchKernelGGL(k_hot_sample, dim3(kHotSample / 256), dim3(256), 0, stream, key, sel, n, sample);
+  hipLaunchKernelGGL(k_hot_keys, dim3(1), dim3(1024), 0, stream, sample, hot, reinterpret_cast<int32_t*>(hot + kHot));
 }
 
 template <bool FIRST>
@@ -590,10 +625,18 @@ __global__ __launch_bounds__(kJBlock) void k_part_hist(TableView t, kg_col key, 
   __syncthreads();
   int64_t lo, hi;
   block_range(n, lo, hi);
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kJBlock) {
-    int64_t k;
-    const int q = key_region(t, key, sel ? sel[i] : i, P, k);
-    if (q >= 0) atomicAdd(&cnt[q], 1u);
+  constexpr int U = 8;  // loads in flight per thread
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += static_cast<int64_t>(kJBlock) * U) {
+    int q[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + static_cast<int64_t>(u) * kJBlock;
+      int64_t k;
+      q[u] = i < hi ? key_region(t, key, sel ? sel[i] : i, P, k) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (q[u] >= 0) atomicAdd(&cnt[q[u]], 1u);
   }
   __syncthreads();
   for (int q = threadIdx.x; q < P; q += kJBlock) hist[static_cast<int64_t>(q) * gridDim.x + blockIdx.x] = cnt[q];
@@ -922,6 +965,475 @@ __global__ __launch_bounds__(kJBlock) void k_collect(TableView t, const Acc* acc
   }
 }
 
+// ---- probe + GROUP BY matched key, partitioned by table region -------------------
+// The atomic form above adds every matched probe row into a per-slot
+// accumulator array the size of the table: two device atomics per match on
+// random lines of an array far beyond the caches. Here the probe rows are
+// first grouped by table region (fine region f = the multiply-high of the
+// key's hash by PF, which fixes a range of home buckets), in two counting-sort
+// passes (coarse regions
+// of kAggFine fine regions, then fine regions inside each coarse one); then
+// one workgroup per fine region probes its rows against that slice of the
+// table (read from L2), keeps SUM and COUNT per matched slot in LDS, and writes
+// its groups straight to the output. All rows of a key share its home bucket,
+// so a key's group is owned by exactly one region, also when the key was
+// placed in a later bucket of its chain: the LDS window covers kAggMargin
+// buckets past the region's last home bucket, and a match beyond that (or a
+// wide table's free-marker key, whose rows live in the side list) sets a flag
+// on which the host runs the atomic form instead.
+// Frequent keys (the k_hot_keys sample) skip the partition: each scatter
+// workgroup sums their rows in LDS and adds the totals to a small device
+// array, which the region owning each such key folds in before it writes.
+constexpr int kAggFine = 128;          // fine regions per coarse region
+constexpr int kAggMargin = 4;          // chain buckets past a region covered by its window
+constexpr int kAggTile = 2048;         // rows per scatter tile
+constexpr int kAggBlock = 1024;        // threads of a region workgroup
+constexpr int64_t kAggLdsBudget = 96 * 1024;
+
+struct AggGeo {
+  uint64_t nb;  // buckets (16-slot buckets for a narrow table)
+  int32_t PF;   // fine regions, a multiple of kAggFine
+  int32_t S;    // slots per bucket
+};
+
+// Fine region of a key: the multiply-high of the key's hash (the one that
+// picks its home bucket) by PF, so region f holds exactly the keys whose home
+// buckets lie in [floor(f * nb / PF), ceil((f + 1) * nb / PF)); neighbouring
+// regions may share a boundary bucket (a key's group still has one owner).
+__device__ __forceinline__ int agg_fine(const AggGeo& g, int64_t k) {
+  return static_cast<int>(__umul64hi(mix64(static_cast<uint64_t>(k)), static_cast<uint64_t>(g.PF)));
+}
+
+// Probe key of row r; false when the row cannot match (NULL, NaN, outside a
+// narrow table's range). A wide table's free-marker key sets *fallback.
+__device__ __forceinline__ bool agg_key(const TableView& t, const kg_col& key, int64_t r, int64_t& k,
+                                        int* fallback) {
+  if (!load_key_nt(key, r, k)) return false;
+  if (t.narrow) {
+    uint32_t k32;
+    return narrow_key(t, k, k32);
+  }
+  if (k == kEmpty) {
+    atomicOr(fallback, 1);
+    return false;
+  }
+  return true;
+}
+
+// The hot-key list in LDS: an open-addressing index (positions by hot_pos,
+// claimed in parallel with LDS CAS; kHotHash > kHot leaves a free entry, so
+// every search ends). Returns the list length. Ends synchronized.
+__device__ __forceinline__ int hot_setup(const int64_t* hot, const int32_t* hot_n, int32_t* hidx, int64_t* hkey) {
+  const int nhot = hot ? *hot_n : 0;
+  for (int e = threadIdx.x; e < kHotHash; e += blockDim.x) hidx[e] = -1;
+  for (int j = threadIdx.x; j < nhot; j += blockDim.x) hkey[j] = hot[j];
+  __syncthreads();
+  for (int j = threadIdx.x; j < nhot; j += blockDim.x) {
+    uint32_t p = hot_pos(hkey[j]);
+    for (int probes = 0; probes < kHotHash; probes++) {
+      if (atomicCAS(&hidx[p], -1, j) == -1) break;
+      p = (p + 1) & (kHotHash - 1);
+    }
+  }
+  __syncthreads();
+  return nhot;
+}
+
+__device__ __forceinline__ int hot_find(const int32_t* hidx, const int64_t* hkey, int nhot, int64_t k) {
+  if (nhot == 0) return -1;
+  uint32_t p = hot_pos(k);
+  for (int probes = 0; probes < kHotHash; probes++) {
+    const int j = hidx[p];
+    if (j < 0) return -1;
+    if (hkey[j] == k) return j;
+    p = (p + 1) & (kHotHash - 1);
+  }
+  return -1;
+}
+
+// Pass 1: per workgroup (a contiguous range of probe rows), the number of
+// non-hot matchable rows per coarse region; hist[c * gridDim.x + block].
+__global__ __launch_bounds__(kJBlock) void k_agg_hist(TableView t, kg_col key, int64_t n, AggGeo g, int PC,
+                                                       const int64_t* hot, const int32_t* hot_n, uint32_t* hist,
+                                                       int* fallback) {
+  extern __shared__ uint32_t ccnt[];  // [kJBlock / kWave][PC]: one histogram per wave
+  __shared__ int32_t hidx[kHotHash];
+  __shared__ int64_t hkey[kHot];
+  constexpr int NW = kJBlock / kWave;
+  for (int q = threadIdx.x; q < NW * PC; q += kJBlock) ccnt[q] = 0;
+  const int nhot = hot_setup(hot, hot_n, hidx, hkey);
+  uint32_t* wcnt = ccnt + (threadIdx.x / kWave) * PC;
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  constexpr int U = 8;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += static_cast<int64_t>(kJBlock) * U) {
+    int64_t kk[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + static_cast<int64_t>(u) * kJBlock;
+      ok[u] = i < hi && agg_key(t, key, i, kk[u], fallback);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (ok[u] && hot_find(hidx, hkey, nhot, kk[u]) < 0) atomicAdd(&wcnt[agg_fine(g, kk[u]) / kAggFine], 1u);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < PC; q += kJBlock) {
+    uint32_t c = 0;
+    for (int w = 0; w < NW; w++) c += ccnt[w * PC + q];
+    hist[static_cast<int64_t>(q) * gridDim.x + blockIdx.x] = c;
+  }
+}
+
+// Pass 2: the same rows, tile by tile counting-sorted by coarse region in LDS
+// and written as one run per region from the workgroup's scanned offsets
+// (okeys/ovals); hot rows are summed in LDS instead and added to hot_acc
+// ({sum, count} per hot key) when the workgroup ends.
+__global__ __launch_bounds__(kJBlock) void k_agg_scatter1(TableView t, kg_col key, const int64_t* vals, int64_t n,
+                                                           AggGeo g, int PC, const int64_t* hot, const int32_t* hot_n,
+                                                           const uint32_t* off, int64_t* okeys, int64_t* ovals,
+                                                           unsigned long long* hot_acc, int* fallback) {
+  extern __shared__ unsigned char smem[];
+  __shared__ uint32_t wtot[kJBlock / kWave];
+  __shared__ int32_t hidx[kHotHash];
+  __shared__ int64_t hkey[kHot];
+  __shared__ unsigned long long hsum[kHot], hcnt[kHot];
+  uint32_t* cursor = reinterpret_cast<uint32_t*>(smem);
+  uint32_t* lcnt = cursor + PC;
+  uint32_t* loff = lcnt + PC;
+  int64_t* tkeys = reinterpret_cast<int64_t*>(smem + ((static_cast<size_t>(3 * PC) * 4 + 15) / 16) * 16);
+  int64_t* tvals = tkeys + kAggTile;
+  uint16_t* treg = reinterpret_cast<uint16_t*>(tvals + kAggTile);
+  constexpr int kPer = kAggTile / kJBlock;
+  for (int q = threadIdx.x; q < PC; q += kJBlock) cursor[q] = off[static_cast<int64_t>(q) * gridDim.x + blockIdx.x];
+  for (int j = threadIdx.x; j < kHot; j += kJBlock) {
+    hsum[j] = 0;
+    hcnt[j] = 0;
+  }
+  const int nhot = hot_setup(hot, hot_n, hidx, hkey);
+  int64_t lo, hi;
+  block_range(n, lo, hi);
+  for (int64_t t0 = lo; t0 < hi; t0 += kAggTile) {
+    for (int q = threadIdx.x; q < PC; q += kJBlock) lcnt[q] = 0;
+    __syncthreads();
+    int reg[kPer];
+    uint32_t rank[kPer];
+    int64_t kk[kPer], vv[kPer];
+    bool ok[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int64_t i = t0 + threadIdx.x + static_cast<int64_t>(j) * kJBlock;
+      ok[j] = i < hi && agg_key(t, key, i, kk[j], fallback);
+      vv[j] = ok[j] ? __builtin_nontemporal_load(vals + i) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      reg[j] = -1;
+      if (!ok[j]) continue;
+      const int h = hot_find(hidx, hkey, nhot, kk[j]);
+      if (h >= 0) {
+        atomicAdd(&hsum[h], static_cast<unsigned long long>(vv[j]));
+        atomicAdd(&hcnt[h], 1ull);
+        continue;
+      }
+      reg[j] = agg_fine(g, kk[j]) / kAggFine;
+      rank[j] = atomicAdd(&lcnt[reg[j]], 1u);
+    }
+    __syncthreads();
+    block_excl_scan(lcnt, loff, PC, wtot);
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (reg[j] < 0) continue;
+      const uint32_t slot = loff[reg[j]] + rank[j];
+      tkeys[slot] = kk[j];
+      tvals[slot] = vv[j];
+      treg[slot] = static_cast<uint16_t>(reg[j]);
+    }
+    __syncthreads();
+    const uint32_t total = loff[PC - 1] + lcnt[PC - 1];
+    for (uint32_t slot = threadIdx.x; slot < total; slot += kJBlock) {
+      const int q = treg[slot];
+      const uint32_t o = cursor[q] + (slot - loff[q]);
+      okeys[o] = tkeys[slot];
+      ovals[o] = tvals[slot];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < PC; q += kJBlock) cursor[q] += lcnt[q];
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < nhot; j += kJBlock) {
+    if (hcnt[j] == 0) continue;
+    atomicAdd(&hot_acc[2 * j], hsum[j]);
+    atomicAdd(&hot_acc[2 * j + 1], hcnt[j]);
+  }
+}
+
+// Workgroup (c, q) of passes 3 and 4: the q-th of Q chunks of coarse region
+// c's rows (off1: the scanned pass-1 histogram, nblk1 workgroups).
+__device__ __forceinline__ void agg_chunk(const uint32_t* off1, int nblk1, int Q, int& c, int& q, int64_t& lo,
+                                          int64_t& hi) {
+  c = blockIdx.x / Q;
+  q = blockIdx.x % Q;
+  const int64_t s = off1[static_cast<int64_t>(c) * nblk1], e = off1[static_cast<int64_t>(c + 1) * nblk1];
+  lo = s + (e - s) * q / Q;
+  hi = s + (e - s) * (q + 1) / Q;
+}
+
+// Pass 3: per chunk, the number of rows per fine region of its coarse region;
+// hist2[(c * kAggFine + fl) * Q + q], so that one exclusive scan gives every
+// (fine region, chunk) its output position.
+__global__ __launch_bounds__(kJBlock) void k_agg_hist2(AggGeo g, const int64_t* keys, int nblk1, int Q,
+                                                        const uint32_t* off1, uint32_t* hist2) {
+  __shared__ uint32_t fcnt[kAggFine];
+  for (int f = threadIdx.x; f < kAggFine; f += kJBlock) fcnt[f] = 0;
+  __syncthreads();
+  int c, q;
+  int64_t lo, hi;
+  agg_chunk(off1, nblk1, Q, c, q, lo, hi);
+  constexpr int U = 4;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += static_cast<int64_t>(kJBlock) * U) {
+    int64_t kk[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + static_cast<int64_t>(u) * kJBlock;
+      kk[u] = i < hi ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (i0 + static_cast<int64_t>(u) * kJBlock >= hi) continue;
+      const uint32_t fl = static_cast<uint32_t>(agg_fine(g, kk[u]) - c * kAggFine);
+      if (fl < static_cast<uint32_t>(kAggFine)) atomicAdd(&fcnt[fl], 1u);
+    }
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < kAggFine; f += kJBlock)
+    hist2[(static_cast<int64_t>(c) * kAggFine + f) * Q + q] = fcnt[f];
+}
+
+// Pass 4: each chunk counting-sorted by fine region (LDS tiles) into
+// keys2/vals2 at the positions of the scanned pass-3 histogram.
+__global__ __launch_bounds__(kJBlock) void k_agg_scatter2(AggGeo g, const int64_t* keys, const int64_t* vals,
+                                                           int nblk1, int Q, const uint32_t* off1,
+                                                           const uint32_t* off2, int64_t* okeys, int64_t* ovals) {
+  __shared__ uint32_t cursor[kAggFine], lcnt[kAggFine], loff[kAggFine];
+  __shared__ uint32_t wtot[kJBlock / kWave];
+  __shared__ int64_t tkeys[kAggTile], tvals[kAggTile];
+  __shared__ uint8_t treg[kAggTile];
+  constexpr int kPer = kAggTile / kJBlock;
+  int c, q;
+  int64_t lo, hi;
+  agg_chunk(off1, nblk1, Q, c, q, lo, hi);
+  for (int f = threadIdx.x; f < kAggFine; f += kJBlock)
+    cursor[f] = off2[(static_cast<int64_t>(c) * kAggFine + f) * Q + q];
+  for (int64_t t0 = lo; t0 < hi; t0 += kAggTile) {
+    for (int f = threadIdx.x; f < kAggFine; f += kJBlock) lcnt[f] = 0;
+    __syncthreads();
+    int reg[kPer];
+    uint32_t rank[kPer];
+    int64_t kk[kPer], vv[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int64_t i = t0 + threadIdx.x + static_cast<int64_t>(j) * kJBlock;
+      kk[j] = i < hi ? keys[i] : 0;
+      vv[j] = i < hi ? vals[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      reg[j] = -1;
+      if (t0 + threadIdx.x + static_cast<int64_t>(j) * kJBlock >= hi) continue;
+      const uint32_t fl = static_cast<uint32_t>(agg_fine(g, kk[j]) - c * kAggFine);
+      if (fl >= static_cast<uint32_t>(kAggFine)) continue;
+      reg[j] = static_cast<int>(fl);
+      rank[j] = atomicAdd(&lcnt[fl], 1u);
+    }
+    __syncthreads();
+    block_excl_scan(lcnt, loff, kAggFine, wtot);
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (reg[j] < 0) continue;
+      const uint32_t slot = loff[reg[j]] + rank[j];
+      tkeys[slot] = kk[j];
+      tvals[slot] = vv[j];
+      treg[slot] = static_cast<uint8_t>(reg[j]);
+    }
+    __syncthreads();
+    const uint32_t total = loff[kAggFine - 1] + lcnt[kAggFine - 1];
+    for (uint32_t slot = threadIdx.x; slot < total; slot += kJBlock) {
+      const int f = treg[slot];
+      const uint32_t o = cursor[f] + (slot - loff[f]);
+      okeys[o] = tkeys[slot];
+      ovals[o] = tvals[slot];
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < kAggFine; f += kJBlock) cursor[f] += lcnt[f];
+    __syncthreads();
+  }
+}
+
+// Matches of k in its bucket chain from home bucket b (the probe rule: until a
+// bucket with a free slot): their number, and the first matching slot.
+template <bool NARROW>
+__device__ __forceinline__ unsigned agg_find(const TableView& t, uint64_t nb, int64_t k, uint64_t b,
+                                             int64_t& first) {
+  unsigned nm = 0;
+  first = -1;
+  if (NARROW) {
+    uint32_t k32;
+    if (!narrow_key(t, k, k32)) return 0;
+    const uint32_t* keys = keys32(t);
+    for (uint64_t scanned = 0; scanned < nb; scanned++) {
+      const uint4* bk = reinterpret_cast<const uint4*>(keys + b * kSlots32);
+      bool empty = false;
+#pragma unroll
+      for (int q = 0; q < kSlots32 / 4; q++) {
+        const uint4 x = bk[q];
+        const uint32_t v[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          if (v[e] == k32) {
+            if (first < 0) first = static_cast<int64_t>(b) * kSlots32 + 4 * q + e;
+            nm++;
+          }
+          empty |= v[e] == kEmpty32;
+        }
+      }
+      if (empty) break;
+      b = (b + 1 == nb) ? 0 : b + 1;
+    }
+  } else {
+    for (uint64_t scanned = 0; scanned < nb; scanned++) {
+      const longlong2* bk = reinterpret_cast<const longlong2*>(t.keys + b * kSlots);
+      bool empty = false;
+#pragma unroll
+      for (int q = 0; q < kSlots / 2; q++) {
+        const longlong2 x = bk[q];
+        if (x.x == k) {
+          if (first < 0) first = static_cast<int64_t>(b) * kSlots + 2 * q;
+          nm++;
+        }
+        if (x.y == k) {
+          if (first < 0) first = static_cast<int64_t>(b) * kSlots + 2 * q + 1;
+          nm++;
+        }
+        empty |= x.x == kEmpty || x.y == kEmpty;
+      }
+      if (empty) break;
+      b = (b + 1 == nb) ? 0 : b + 1;
+    }
+  }
+  return nm;
+}
+
+// Pass 5: workgroup f aggregates fine region f. LDS holds {sum} and {count}
+// (64-bit) for the slots of the region's home buckets and the kAggMargin
+// buckets after them; a matched row adds val * matches and matches to its
+// key's first matching slot (the group of the atomic form). Hot keys of this
+// region add their sampled totals the same way. Then the live slots are
+// written out (one output range per workgroup, slot order inside it).
+template <bool NARROW>
+__global__ __launch_bounds__(kAggBlock) void k_agg_region(TableView t, AggGeo g, int Q, const uint32_t* off2,
+                                                           const int64_t* keys, const int64_t* vals,
+                                                           const int64_t* hot, const int32_t* hot_n,
+                                                           const unsigned long long* hot_acc,
+                                                           unsigned long long* count, int64_t cap, int64_t* out_key,
+                                                           int64_t* out_sum, int64_t* out_cnt, int* fallback) {
+  extern __shared__ unsigned long long lacc[];
+  constexpr int NW = kAggBlock / kWave;
+  __shared__ unsigned int wsum[NW];
+  __shared__ unsigned int wc[2][NW];
+  __shared__ unsigned long long obase;
+  const uint64_t nb = g.nb;
+  const int S = g.S;
+  const uint64_t f = blockIdx.x;
+  const uint64_t blo = (f * nb) / g.PF, bhi = ((f + 1) * nb + g.PF - 1) / g.PF;
+  const uint64_t wb = min(bhi - blo + kAggMargin, nb - blo);
+  const int64_t nsl = static_cast<int64_t>(wb) * S;
+  const int64_t slot0 = static_cast<int64_t>(blo) * S;
+  unsigned long long* lsum = lacc;
+  unsigned long long* lcnt = lacc + nsl;
+  for (int64_t s = threadIdx.x; s < 2 * nsl; s += kAggBlock) lacc[s] = 0;
+  __syncthreads();
+  const int64_t lo = off2[f * Q], hi = off2[(f + 1) * Q];
+  constexpr int U = 4;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += static_cast<int64_t>(kAggBlock) * U) {
+    int64_t kk[U], vv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + static_cast<int64_t>(u) * kAggBlock;
+      kk[u] = i < hi ? __builtin_nontemporal_load(keys + i) : 0;
+      vv[u] = i < hi ? __builtin_nontemporal_load(vals + i) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (i0 + static_cast<int64_t>(u) * kAggBlock >= hi) continue;
+      int64_t first;
+      const unsigned nm = agg_find<NARROW>(t, nb, kk[u], bucket_of(kk[u], nb), first);
+      if (nm == 0) continue;
+      const int64_t rel = first - slot0;
+      if (rel < 0 || rel >= nsl) {
+        atomicOr(fallback, 1);
+        continue;
+      }
+      atomicAdd(&lsum[rel], static_cast<unsigned long long>(vv[u]) * nm);
+      atomicAdd(&lcnt[rel], static_cast<unsigned long long>(nm));
+    }
+  }
+  const int nhot = hot ? *hot_n : 0;
+  for (int j = threadIdx.x; j < nhot; j += kAggBlock) {
+    const int64_t k = hot[j];
+    if (hot_acc[2 * j + 1] == 0 || agg_fine(g, k) != static_cast<int>(f)) continue;
+    int64_t first;
+    const unsigned nm = agg_find<NARROW>(t, nb, k, bucket_of(k, nb), first);
+    if (nm == 0) continue;
+    const int64_t rel = first - slot0;
+    if (rel < 0 || rel >= nsl) {
+      atomicOr(fallback, 1);
+      continue;
+    }
+    atomicAdd(&lsum[rel], hot_acc[2 * j] * nm);
+    atomicAdd(&lcnt[rel], hot_acc[2 * j + 1] * nm);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  unsigned int mine = 0;
+  for (int64_t s = threadIdx.x; s < nsl; s += kAggBlock) mine += lcnt[s] > 0 ? 1u : 0u;
+  for (int st = kWave / 2; st >= 1; st >>= 1) mine += __shfl_xor(mine, st);
+  if (lane == 0) wsum[wave] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < NW; w++) tot += wsum[w];
+    obase = tot ? atomicAdd(count, tot) : 0;
+  }
+  __syncthreads();
+  unsigned long long pos = obase;
+  int it = 0;
+  for (int64_t s0 = 0; s0 < nsl; s0 += kAggBlock, it++) {
+    const int64_t s = s0 + threadIdx.x;
+    const bool live = s < nsl && lcnt[s] > 0;
+    const uint64_t m = __ballot(live);
+    if (lane == 0) wc[it & 1][wave] = __popcll(m);
+    __syncthreads();
+    unsigned int before = 0, tot = 0;
+    for (int w = 0; w < NW; w++) {
+      const unsigned int cw = wc[it & 1][w];
+      before += w < wave ? cw : 0;
+      tot += cw;
+    }
+    if (live) {
+      const unsigned long long p = pos + before + __popcll(m & ((1ull << lane) - 1ull));
+      if (p < static_cast<unsigned long long>(cap)) {
+        const int64_t gs = slot0 + s;
+        out_key[p] = NARROW ? static_cast<int64_t>(static_cast<uint64_t>(t.base) + keys32(t)[gs]) : t.keys[gs];
+        out_sum[p] = static_cast<int64_t>(lsum[s]);
+        out_cnt[p] = static_cast<int64_t>(lcnt[s]);
+      }
+    }
+    pos += tot;
+  }
+}
+
 // High-cardinality group-by: insert-or-update one (key, value) per row into an
 // open-addressing table of keys + accumulators. A row scans its bucket's eight
 // keys; a match updates; a free slot is claimed with atomicCAS (if the CAS
@@ -1157,6 +1669,7 @@ struct kg_hash_table {
                                            // [4..5] key range of a build
   int device = 0;
   int form = -1;          // -1 until the first build with keys, then 0 wide / 1 narrow
+  bool narrow_unfilled = false;  // narrow form chosen, its free markers not yet written
   int64_t base = 0;       // narrow: smallest key
   TableView view() const {
     TableView v{keys, rows, nbuckets, side_rows, counters + 1, form == 1 ? 1 : 0, 0, base};
@@ -1241,8 +1754,10 @@ int choose_form(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t
                      1);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, rng + 1,
                      static_cast<unsigned long long>(INT64_MIN), 1);
+  kg_time_begin(ctx);
   hipLaunchKernelGGL(k_key_range, dim3(std::min(grid_for(ctx, nrows, 16), ctx->num_cus * 8)), dim3(kJBlock), 0,
                      ctx->stream, *key, d_rows, nrows, rng);
+  kg_time_end(ctx);
   KG_LAUNCH_CHECK(ctx);
   int st = kg_pinned_reserve(ctx, 64);
   if (st != KG_OK) return st;
@@ -1253,12 +1768,11 @@ int choose_form(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t
   const uint64_t limit = static_cast<uint64_t>(kEmpty32) - 1;
   if (ht->form == -1) {
     if (static_cast<uint64_t>(hi) - static_cast<uint64_t>(lo) <= limit) {
+      // The narrow free marker is written by kg_ht_build before it inserts
+      // (or not at all when a fresh region build writes every slot).
       ht->form = 1;
       ht->base = lo;
-      const int64_t words = static_cast<int64_t>(ht->nbuckets) * kSlots / 2;  // 32-bit slots / 2
-      hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, words, 4)), dim3(kJBlock), 0, ctx->stream,
-                         reinterpret_cast<unsigned long long*>(ht->keys), ~0ull, words);
-      KG_LAUNCH_CHECK(ctx);
+      ht->narrow_unfilled = true;
     } else {
       ht->form = 0;
     }
@@ -1425,6 +1939,17 @@ extern "C" int kg_ht_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, co
     return kg_fail(ctx, KG_ERR_CAPACITY, "table capacity %lld rows exceeded", (long long)ht->capacity);
   if ((st = choose_form(ctx, ht, key, d_rows, nrows)) != KG_OK) return st;
   const BuildLayout L = ht->form >= 0 ? build_layout(ctx, ht, nrows) : BuildLayout{};
+  if (ht->narrow_unfilled) {
+    ht->narrow_unfilled = false;
+    if (!(L.P > 0 && inserted == 0)) {
+      const int64_t words = static_cast<int64_t>(ht->nbuckets) * kSlots / 2;  // 32-bit slots / 2
+      kg_time_begin(ctx);
+      hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, words, 4)), dim3(kJBlock), 0, ctx->stream,
+                         reinterpret_cast<unsigned long long*>(ht->keys), ~0ull, words);
+      kg_time_end(ctx);
+      KG_LAUNCH_CHECK(ctx);
+    }
+  }
   if (L.P > 0) {
     if ((st = region_build(ctx, ht, *key, d_rows, nrows, L, inserted == 0)) != KG_OK) return st;
   } else {
@@ -1517,8 +2042,7 @@ int launch_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col& key, const 
   // kAgg: the hot-key sample runs first, inside the timed region.
   auto hot_pass = [&]() {
     if (MODE == kAgg && o.hot_keys)
-      hipLaunchKernelGGL(k_hot_keys, dim3(1), dim3(1024), 0, ctx->stream, key, sel, n,
-                         const_cast<int64_t*>(o.hot_keys), const_cast<int32_t*>(o.hot_n));
+      launch_hot_keys(ctx->stream, key, sel, n, const_cast<int64_t*>(o.hot_keys));
   };
   if (P == 0) {
     const int grid = probe_grid(ctx, n);
@@ -1647,6 +2171,125 @@ extern "C" int kg_ht_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* k
   return KG_OK;
 }
 
+namespace {
+
+// Layout of a partitioned probe + GROUP BY (see k_agg_region); PF = 0 means
+// the atomic form. The LDS window of a region (its home buckets plus
+// kAggMargin, 16 bytes per slot) fits kAggLdsBudget.
+struct RegionAggPlan {
+  int PC = 0, PF = 0, Q = 0, nblk1 = 0;
+  int64_t span = 0;  // window buckets of the largest region
+};
+
+RegionAggPlan region_agg_plan(kg_ctx* ctx, const kg_hash_table* ht, int64_t n) {
+  RegionAggPlan p;
+  if (ctx->agg_regions == 0 || ht->form < 0) return p;
+  const bool narrow = ht->form == 1;
+  const int S = narrow ? kSlots32 : kSlots;
+  const int64_t nb = static_cast<int64_t>(narrow ? ht->nbuckets / 2 : ht->nbuckets);
+  if (ctx->agg_regions < 0 && (n < (1 << 22) || nb * S * 16 < (64ll << 20))) return p;
+  const int64_t budget = std::min<int64_t>(kAggLdsBudget, ctx->max_lds - 4096);
+  const int64_t win = budget / (16 * S);  // window buckets that fit
+  if (win <= kAggMargin + 2) return p;
+  int64_t PF = ceil_div(nb, win - kAggMargin - 1);
+  if (ctx->agg_regions > 0) PF = std::max<int64_t>(PF, ctx->agg_regions);
+  const int64_t PC = ceil_div(PF, kAggFine);
+  if (PC > 2048) return p;
+  PF = PC * kAggFine;
+  p.PC = static_cast<int>(PC);
+  p.PF = static_cast<int>(PF);
+  p.nblk1 = static_cast<int>(std::min<int64_t>(2048, std::max<int64_t>(kXcds, ceil_div(n, 4 * kAggTile))));
+  p.Q = static_cast<int>(std::max<int64_t>(1, ceil_div(4 * ctx->num_cus, PC)));
+  p.span = std::min<int64_t>(ceil_div(nb, PF) + 1 + kAggMargin, nb);
+  return p;
+}
+
+// Runs the partitioned form; *fallback is set when a key's group lies outside
+// its region's window (the caller then runs the atomic form). The whole
+// sequence, from the hot-key sample to the written groups, is the timed region.
+int run_agg_regions(kg_ctx* ctx, kg_hash_table* t, const kg_col& key, const int64_t* vals, int64_t n,
+                    const RegionAggPlan& p, int64_t cap, int64_t* out_key, int64_t* out_sum, int64_t* out_cnt,
+                    unsigned long long* count, bool* fallback) {
+  const bool narrow = t->form == 1;
+  const AggGeo g{narrow ? t->nbuckets / 2 : t->nbuckets, p.PF, narrow ? kSlots32 : kSlots};
+  const int64_t m1 = static_cast<int64_t>(p.PC) * p.nblk1, m2 = static_cast<int64_t>(p.PF) * p.Q;
+  size_t cub1 = 0, cub2 = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, cub1, static_cast<uint32_t*>(nullptr),
+                                         static_cast<uint32_t*>(nullptr), static_cast<int>(m1 + 1), ctx->stream);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, cub2, static_cast<uint32_t*>(nullptr),
+                                         static_cast<uint32_t*>(nullptr), static_cast<int>(m2 + 1), ctx->stream);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o2 = off;
+    off += (bytes + 255) / 256 * 256;
+    return o2;
+  };
+  const size_t o_h1 = carve((m1 + 1) * 4), o_o1 = carve((m1 + 1) * 4), o_h2 = carve((m2 + 1) * 4),
+               o_o2 = carve((m2 + 1) * 4), o_k1 = carve(n * 8), o_v1 = carve(n * 8), o_k2 = carve(n * 8),
+               o_v2 = carve(n * 8), o_hot = carve(kHot * 16), o_flag = carve(8),
+               o_cub = carve(std::max(cub1, cub2));
+  int st = kg_work_reserve(ctx, off);
+  if (st != KG_OK) return st;
+  if (!t->hot && (st = alloc(ctx, reinterpret_cast<void**>(&t->hot), kHotBufBytes)) != KG_OK) return st;
+  char* w = static_cast<char*>(ctx->work);
+  uint32_t* h1 = reinterpret_cast<uint32_t*>(w + o_h1);
+  uint32_t* o1 = reinterpret_cast<uint32_t*>(w + o_o1);
+  uint32_t* h2 = reinterpret_cast<uint32_t*>(w + o_h2);
+  uint32_t* o2 = reinterpret_cast<uint32_t*>(w + o_o2);
+  int64_t* k1 = reinterpret_cast<int64_t*>(w + o_k1);
+  int64_t* v1 = reinterpret_cast<int64_t*>(w + o_v1);
+  int64_t* k2 = reinterpret_cast<int64_t*>(w + o_k2);
+  int64_t* v2 = reinterpret_cast<int64_t*>(w + o_v2);
+  unsigned long long* hot_acc = reinterpret_cast<unsigned long long*>(w + o_hot);
+  int* flag = reinterpret_cast<int*>(w + o_flag);
+  const int64_t* hot = t->hot;
+  const int32_t* hot_n = reinterpret_cast<const int32_t*>(t->hot + kHot);
+  const size_t lds1 = static_cast<size_t>(p.PC) * 4 * (kJBlock / kWave);
+  const size_t lds2 = (static_cast<size_t>(3 * p.PC) * 4 + 15) / 16 * 16 + static_cast<size_t>(kAggTile) * 18;
+  const size_t lds5 = static_cast<size_t>(p.span) * g.S * 16;
+  if ((st = kg_allow_lds(ctx, reinterpret_cast<const void*>(&k_agg_scatter1), lds2)) != KG_OK) return st;
+  const void* region_kernel = narrow ? reinterpret_cast<const void*>(&k_agg_region<true>)
+                                     : reinterpret_cast<const void*>(&k_agg_region<false>);
+  if ((st = kg_allow_lds(ctx, region_kernel, lds5)) != KG_OK) return st;
+  const TableView v = t->view();
+  kg_time_begin(ctx);
+  hipLaunchKernelGGL(k_fill_i32j, dim3(1), dim3(64), 0, ctx->stream, reinterpret_cast<int32_t*>(h1 + m1), 0, 1);
+  hipLaunchKernelGGL(k_fill_i32j, dim3(1), dim3(64), 0, ctx->stream, reinterpret_cast<int32_t*>(h2 + m2), 0, 1);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kJBlock), 0, ctx->stream, hot_acc, 0ull, 2 * kHot);
+  hipLaunchKernelGGL(k_fill_i32j, dim3(1), dim3(64), 0, ctx->stream, flag, 0, 1);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, count, 0ull, 1);
+  launch_hot_keys(ctx->stream, key, nullptr, n, t->hot);
+  hipLaunchKernelGGL(k_agg_hist, dim3(p.nblk1), dim3(kJBlock), lds1, ctx->stream, v, key, n, g, p.PC, hot, hot_n, h1,
+                     flag);
+  size_t tb = std::max(cub1, cub2);
+  if (hipcub::DeviceScan::ExclusiveSum(w + o_cub, tb, h1, o1, static_cast<int>(m1 + 1), ctx->stream) != hipSuccess)
+    return kg_fail(ctx, KG_ERR_HIP, "aggregation histogram scan failed");
+  hipLaunchKernelGGL(k_agg_scatter1, dim3(p.nblk1), dim3(kJBlock), lds2, ctx->stream, v, key, vals, n, g, p.PC, hot,
+                     hot_n, o1, k1, v1, hot_acc, flag);
+  const int grid2 = p.PC * p.Q;
+  hipLaunchKernelGGL(k_agg_hist2, dim3(grid2), dim3(kJBlock), 0, ctx->stream, g, k1, p.nblk1, p.Q, o1, h2);
+  tb = std::max(cub1, cub2);
+  if (hipcub::DeviceScan::ExclusiveSum(w + o_cub, tb, h2, o2, static_cast<int>(m2 + 1), ctx->stream) != hipSuccess)
+    return kg_fail(ctx, KG_ERR_HIP, "aggregation histogram scan failed");
+  hipLaunchKernelGGL(k_agg_scatter2, dim3(grid2), dim3(kJBlock), 0, ctx->stream, g, k1, v1, p.nblk1, p.Q, o1, o2, k2,
+                     v2);
+  if (narrow)
+    hipLaunchKernelGGL(k_agg_region<true>, dim3(p.PF), dim3(kAggBlock), lds5, ctx->stream, v, g, p.Q, o2, k2, v2,
+                       hot, hot_n, hot_acc, count, cap, out_key, out_sum, out_cnt, flag);
+  else
+    hipLaunchKernelGGL(k_agg_region<false>, dim3(p.PF), dim3(kAggBlock), lds5, ctx->stream, v, g, p.Q, o2, k2, v2,
+                       hot, hot_n, hot_acc, count, cap, out_key, out_sum, out_cnt, flag);
+  kg_time_end(ctx);
+  KG_LAUNCH_CHECK(ctx);
+  if ((st = kg_pinned_reserve(ctx, 64)) != KG_OK) return st;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, flag, 4, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *fallback = *static_cast<const int32_t*>(ctx->pinned) != 0;
+  return KG_OK;
+}
+
+}  // namespace
+
 extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, const int64_t* d_vals,
                                int64_t nrows, int64_t max_groups, int64_t* d_out_key, int64_t* d_out_sum,
                                int64_t* d_out_count, int32_t* d_out_first, int64_t* out_ngroups) {
@@ -1657,10 +2300,31 @@ extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_co
   KG_REQUIRE(ctx, max_groups == 0 || (d_out_key && d_out_sum && d_out_count), "output arrays are NULL");
   *out_ngroups = 0;
   kg_hash_table* t = const_cast<kg_hash_table*>(ht);
+  unsigned long long* cnt = t->counters + 3;
+  const RegionAggPlan plan = d_out_first || nrows == 0 ? RegionAggPlan{} : region_agg_plan(ctx, t, nrows);
+  bool fallback = true;
+  if (plan.PF > 0) {
+    st = run_agg_regions(ctx, t, *key, d_vals, nrows, plan, max_groups, d_out_key, d_out_sum, d_out_count, cnt,
+                         &fallback);
+    if (st != KG_OK) return st;
+    ctx->last_path = fallback ? "hashed" : "regions";
+  }
+  if (!fallback) {
+    unsigned long long h = 0;
+    if ((st = read_counter(ctx, cnt, &h)) != KG_OK) return st;
+    *out_ngroups = static_cast<int64_t>(h);
+    if (*out_ngroups > max_groups)
+      return kg_fail(ctx, KG_ERR_CAPACITY, "%lld groups exceed max_groups %lld", (long long)*out_ngroups,
+                     (long long)max_groups);
+    return KG_OK;
+  }
+  if (plan.PF == 0) ctx->last_path = "hashed";
   const int64_t nslots = static_cast<int64_t>(t->nbuckets) * kSlots;
   if (!t->acc && (st = alloc(ctx, reinterpret_cast<void**>(&t->acc), (nslots + 1) * sizeof(Acc))) != KG_OK) return st;
   if (d_out_first && !t->first && (st = alloc(ctx, reinterpret_cast<void**>(&t->first), (nslots + 1) * 4)) != KG_OK)
     return st;
+  // The atomic form: accumulator clear, probe, collect; all timed.
+  kg_time_begin(ctx);
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, 2 * (nslots + 1), 4)), dim3(kJBlock), 0, ctx->stream,
                      reinterpret_cast<unsigned long long*>(t->acc), 0ull, 2 * (nslots + 1));
   KG_LAUNCH_CHECK(ctx);
@@ -1669,8 +2333,9 @@ extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_co
                        INT32_MAX, nslots + 1);
     KG_LAUNCH_CHECK(ctx);
   }
+  kg_time_end(ctx);
   if (nrows > 0) {
-    if (!t->hot && (st = alloc(ctx, reinterpret_cast<void**>(&t->hot), kHot * 8 + 64)) != KG_OK) return st;
+    if (!t->hot && (st = alloc(ctx, reinterpret_cast<void**>(&t->hot), kHotBufBytes)) != KG_OK) return st;
     ProbeOut o{};
     o.acc = t->acc;
     o.first = t->first;
@@ -1681,11 +2346,12 @@ extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_co
                      : launch_probe<kAgg, false>(ctx, t, *key, nullptr, nrows, o);
     if (st != KG_OK) return st;
   }
-  unsigned long long* cnt = t->counters + 3;
+  kg_time_begin(ctx);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, cnt, 0ull, 1);
   KG_LAUNCH_CHECK(ctx);
   hipLaunchKernelGGL(k_collect, dim3(grid_for(ctx, nslots + 1, 4)), dim3(kJBlock), 0, ctx->stream, t->view(), t->acc,
                      t->first, nslots, nslots, cnt, max_groups, d_out_key, d_out_sum, d_out_count, d_out_first);
+  kg_time_end(ctx);
   KG_LAUNCH_CHECK(ctx);
   unsigned long long h = 0;
   if ((st = read_counter(ctx, cnt, &h)) != KG_OK) return st;
@@ -2036,6 +2702,27 @@ int time_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, int64_t 
   return st;
 }
 
+// Kernel time of one build of n keys into ht: the sum of the build's timed
+// spans (the context's own pending timings are set aside meanwhile).
+int time_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, int64_t n, double* ms) {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> saved;
+  saved.swap(ctx->ev_pending);
+  const int timing = ctx->timing;
+  ctx->timing = 1;
+  int st = kg_ht_build(ctx, ht, key, nullptr, n);
+  double total = 0;
+  int64_t spans = 0;
+  const int st2 = kg_ctx_kernel_stats(ctx, &total, &spans, 1);
+  for (auto& pr : ctx->ev_pending) {  // left over only when the stats failed
+    ctx->ev_pool.push_back(pr.first);
+    ctx->ev_pool.push_back(pr.second);
+  }
+  ctx->ev_pending.swap(saved);
+  ctx->timing = timing;
+  *ms = total;
+  return st != KG_OK ? st : st2;
+}
+
 }  // namespace
 
 extern "C" int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild, const kg_col* probe_key,
@@ -2081,21 +2768,27 @@ extern "C" int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild
   // a switch on noise (16 instead of 32 probe workgroups per CU) cost 3% at
   // full size.
   constexpr double kMargin = 0.97;
-  // 1. load factor (direct probes, 32 workgroups per CU)
+  // 1. load factor (direct probes, 32 workgroups per CU), by the build time
+  // plus the probe time scaled to all nprobe rows: a load factor whose probes
+  // are no faster but whose build is (fewer table lines to write) wins.
   ctx->probe_regions = 0;
   ctx->probe_wg_per_cu = 32;
+  double best_score = 1e30;
   for (double lf : {0.5, 0.35, 0.65}) {
     kg_hash_table* ht = nullptr;
     if ((st = kg_ht_create(ctx, std::max<int64_t>(nb, 1), lf, &ht)) != KG_OK) return restore(st);
-    st = kg_ht_build(ctx, ht, build_key, nullptr, nb);
-    double ms = 0;
+    double bms = 0, ms = 0;
+    st = time_build(ctx, ht, build_key, nb, &bms);
     if (st == KG_OK) st = time_probe(ctx, ht, probe_key, np, 2, &ms);
     if (st != KG_OK) {
       kg_ht_free(ctx, ht);
       return restore(st);
     }
     trials++;
-    if (ms < best_ms * kMargin) {
+    const double score = bms * static_cast<double>(nbuild) / static_cast<double>(std::max<int64_t>(nb, 1)) +
+                         ms * static_cast<double>(nprobe) / static_cast<double>(std::max<int64_t>(np, 1));
+    if (score < best_score * kMargin) {
+      best_score = score;
       best_ms = ms;
       best_lf = lf;
       if (keep) kg_ht_free(ctx, keep);

@@ -150,6 +150,10 @@ class Context:
         """-1: region-partitioned join builds when large; 0: direct inserts; > 0: at least that many regions."""
         self.check(self.lib.kg_ctx_set_build_regions(self.h, regions))
 
+    def set_agg_regions(self, regions):
+        """Region count of partitioned probe + GROUP BY (-1 auto, 0 off)."""
+        self.check(self.lib.kg_ctx_set_agg_regions(self.h, regions))
+
     def set_probe_regions(self, regions):
         """Region count of partitioned join probes (-1 auto, 0 direct)."""
         self.check(self.lib.kg_ctx_set_probe_regions(self.h, regions))

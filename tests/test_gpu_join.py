@@ -327,3 +327,64 @@ def test_region_build_equals_direct(kind, regions, lf):
     op, ob = po.hash_join([po.Col(po.BIGINT, np.where(bmask, keys, 0), (~bmask).astype(np.int8))],
                           [po.Col(po.BIGINT, probe)])
     assert np.array_equal(p1, sorted_pairs(op, ob))
+
+
+@pytest.mark.parametrize("kind,regions", [("uniform", 1024), ("uniform", 128), ("zipf1", 1024), ("dup_nulls", 512),
+                                          ("wide", 1024), ("wide_marker", 1024)])
+def test_region_agg_equals_atomic(kind, regions):
+    """Probe + GROUP BY partitioned by table region (kg_ctx_set_agg_regions)
+    against the per-slot atomic form on the same table: uniform and Zipf(1)
+    probes (frequent keys summed during the partition) with 10% misses,
+    duplicate build keys with NULL probe keys (narrow table), a wide table
+    (keys across int64), and a wide table whose probes carry the free-marker key
+    INT64_MIN (the call then runs the atomic form). Groups equal as sets; the
+    unique-key cases also equal the oracle's."""
+    rng = np.random.default_rng(sum(kind.encode()) + regions)
+    pnull = None
+    if kind in ("uniform", "zipf1"):
+        nb, npr = 300_000, 3_000_000
+        bk, _, pk, pv = join_tables(nb, npr, theta=1.0 if kind == "zipf1" else 0.0, device=DEV,
+                                    seed=7 + regions)
+        keys, probe, vals = bk.cpu().numpy(), pk.cpu().numpy(), pv.cpu().numpy()
+    elif kind == "dup_nulls":
+        nb, npr = 400_000, 2_000_000
+        keys = rng.integers(0, 250_000, nb) + 5 * 10**9
+        probe = np.concatenate([keys[rng.integers(0, nb, npr - 200_000)], rng.integers(0, 10**10, 200_000)])
+        vals = rng.integers(-(1 << 40), 1 << 40, npr)
+        pnull = (rng.random(npr) < 0.03).astype(np.int8)
+    else:
+        nb, npr = 300_000, 2_000_000
+        keys = rng.integers(-2**63, 2**63 - 1, nb, dtype=np.int64)
+        keys[: nb // 4] = keys[nb // 4: nb // 2]
+        probe = np.concatenate([keys[rng.integers(0, nb, npr - 100_000)], rng.integers(-2**62, 2**62, 100_000)])
+        if kind == "wide_marker":
+            keys[rng.integers(0, nb, 3)] = np.iinfo(np.int64).min
+            probe[rng.integers(0, npr, 50)] = np.iinfo(np.int64).min
+        vals = rng.integers(0, 1 << 32, npr)
+    bcol = col64(keys)
+    pcol = Column(abi.KG_BIGINT, torch.as_tensor(np.ascontiguousarray(probe, dtype=np.int64)).to(DEV),
+                  torch.as_tensor(pnull).to(DEV) if pnull is not None else None)
+    pvals = torch.as_tensor(np.ascontiguousarray(vals, dtype=np.int64)).to(DEV)
+    results, paths = [], []
+    for mode in (0, regions):
+        c = Context(0)
+        c.set_agg_regions(mode)
+        ht = HashTable(c, nb, load_factor=0.5)
+        ht.insert(bcol)
+        print(f"region agg {kind} mode={mode}", flush=True)
+        k, s, n, _ = ht.probe_agg(pcol, pvals, max_groups=nb)
+        paths.append(c.last_path())
+        agg = np.stack([x.cpu().numpy() for x in (k, s, n)], 1)
+        results.append(agg[np.lexsort((agg[:, 2], agg[:, 1], agg[:, 0]))])
+        ht.free()
+        c.close()
+    assert paths == ["hashed", "hashed" if kind == "wide_marker" else "regions"]
+    assert np.array_equal(results[0], results[1])
+    if kind in ("uniform", "zipf1"):
+        ok = np.empty(nb, dtype=np.int64)
+        osum = np.empty(nb, dtype=np.int64)
+        ocnt = np.empty(nb, dtype=np.int64)
+        g = po.lib().kor_q_join_agg(nb, keys.ctypes.data, npr, probe.ctypes.data, vals.ctypes.data, nb,
+                                    ok.ctypes.data, osum.ctypes.data, ocnt.ctypes.data)
+        want = np.stack([ok[:g], osum[:g], ocnt[:g]], 1)
+        assert np.array_equal(results[1], want[np.lexsort((want[:, 2], want[:, 1], want[:, 0]))])
