@@ -509,7 +509,7 @@ def c3_traffic():
 
 
 def bench_serving(ctx, a):
-    """Batched/served mix: Poisson arrivals of queries from 8 templates over
+    """Batched/served mix: Poisson arrivals of queries from 9 templates over
     resident fact tables of 10k..50M rows. Plain one-at-a-time GPU path vs
     continuous batching. The p99 bar is the plain path's p99 at 70% of its
     saturation rate; for each mode the highest offered rate whose p99 stays
@@ -526,17 +526,49 @@ def bench_serving(ctx, a):
     facts, dim, build = serve_columns(tables)
     specs = serve_queries(a.serve_queries)
     queries = [make_query(s, facts, dim, build) for s in specs]
+    # The join tuner's load factors for the two build tables (cached on the
+    # context: join queries of both serving modes take them).
+    for bname, pcol in (("build", 11), ("big", 12)):
+        log(f"serving: join tuning for the {bname} table: {ctx.join_tune(build[bname][0], facts[-1][pcol])}")
     nq = len(queries)
     rng = np.random.RandomState(7)
     expo = rng.exponential(1.0, size=nq)
 
-    def run(mode, rate, cap=0):
+    def run(mode, rate, cap=0, keep=False):
         arr = np.zeros(nq) if rate is None else np.cumsum(expo / rate)
-        _, lat, st = serve(ctx, queries, arr, mode, slots=a.serve_slots, max_batch=a.serve_max_batch,
-                           arena_bytes=a.serve_arena << 20, max_groups=64, max_batch_rows=cap)
-        return {"offered": rate, "cap_rows": cap, "qps": st["qps"], "p50_ms": st["p50_s"] * 1e3,
-                "p99_ms": st["p99_s"] * 1e3, "mean_batch": st["mean_batch"], "batches": st["batches"],
-                "fallbacks": st["fallbacks"]}
+        res, lat, st = serve(ctx, queries, arr, mode, slots=a.serve_slots, max_batch=a.serve_max_batch,
+                             arena_bytes=a.serve_arena << 20, max_groups=64, max_batch_rows=cap)
+        out = {"offered": rate, "cap_rows": cap, "qps": st["qps"], "p50_ms": st["p50_s"] * 1e3,
+               "p99_ms": st["p99_s"] * 1e3, "mean_batch": st["mean_batch"], "batches": st["batches"],
+               "fallbacks": st["fallbacks"]}
+        return (out, res) if keep else out
+
+    def same_results(ra, rb):
+        """Per query: equal status, group keys and order, counts and integer
+        sums exactly, FP64 sums and averages within 1e-12 relative."""
+        from kushdb_amd.workloads import FACT_COLS
+        bad = 0
+        for i, (sp, x, y) in enumerate(zip(specs, ra, rb)):
+            ok = x[0] == y[0] and x[1].shape == y[1].shape and np.array_equal(x[1], y[1]) and \
+                np.array_equal(x[3], y[3])
+            if not ok and len(examples) < 8:
+                examples.append({"query": i, "template": sp["template"], "table": sp["table"],
+                                 "status": [int(x[0]), int(y[0])], "groups": [len(x[1]), len(y[1])]})
+            for j, ag in enumerate(sp["aggs"]):
+                if not ok:
+                    break
+                real = ag[0] == abi.KG_AGG_AVG or (ag[0] == abi.KG_AGG_SUM and (
+                    (len(ag) > 2 and ag[2] >= 0) or FACT_COLS[ag[1]].startswith("m")))
+                if real:
+                    u, v = x[2][:, j].view(np.float64), y[2][:, j].view(np.float64)
+                    ok = bool(np.all(np.abs(u - v) <= 1e-12 * np.maximum(np.abs(u), np.abs(v)) + 0.0))
+                else:
+                    ok = np.array_equal(x[2][:, j], y[2][:, j])
+                if not ok and len(examples) < 8:
+                    examples.append({"query": i, "template": sp["template"], "table": sp["table"], "agg": j,
+                                     "plain": x[2][:3, j].tolist(), "batched": y[2][:3, j].tolist()})
+            bad += 0 if ok else 1
+        return bad
 
     def best_rate(mode, start, bar, cap=0, factor=1.25, limit=64, refine=0):
         """Highest offered rate (start * factor^k, then `refine` geometric
@@ -569,9 +601,12 @@ def bench_serving(ctx, a):
         log(f"saturation: batched {batched_sat['qps']:.0f} q/s")
         return {"batched_saturation": batched_sat}
     run(abi.KG_SERVE_PLAIN, None)
-    plain_sat = run(abi.KG_SERVE_PLAIN, None)
-    batched_sat = run(abi.KG_SERVE_BATCHED, None)
-    log(f"saturation: plain {plain_sat['qps']:.0f} q/s, batched {batched_sat['qps']:.0f} q/s")
+    plain_sat, plain_res = run(abi.KG_SERVE_PLAIN, None, keep=True)
+    batched_sat, batched_res = run(abi.KG_SERVE_BATCHED, None, keep=True)
+    examples = []
+    mismatched = same_results(plain_res, batched_res)
+    log(f"saturation: plain {plain_sat['qps']:.0f} q/s, batched {batched_sat['qps']:.0f} q/s; "
+        f"{mismatched} of {nq} results differ between the two")
     base = 0.7 * plain_sat["qps"]
     plain_at = run(abi.KG_SERVE_PLAIN, base)
     bar = plain_at["p99_ms"]
@@ -595,6 +630,9 @@ def bench_serving(ctx, a):
         "plain_best_within_bar": plain_best, "plain_runs": plain_runs,
         "batched_best_within_bar": bbest, "batched_by_cap": batched,
         "qps_ratio_at_equal_p99": ratio,
+        "results_plain_vs_batched": {"queries": nq, "mismatched": mismatched, "examples": examples,
+                                     "bar": "status, group keys/order, counts and integer sums exact; FP64 "
+                                            "within 1e-12 relative"},
         "saturation_qps_ratio": batched_sat["qps"] / plain_sat["qps"],
     }
 

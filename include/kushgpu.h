@@ -582,7 +582,9 @@ typedef struct kg_query {
   const kg_pred* build_preds;
   int32_t build_code_col;       /* group code column in build_cols, or -1: one group */
   int32_t group_card;           /* codes are in [0, group_card) */
-  double load_factor;           /* join table load factor; 0 = 0.5 */
+  double load_factor;           /* join table load factor; 0 = what kg_join_tune chose on
+                                   this context for builds of the same power-of-two size
+                                   (unique keys preferred), else 0.5 */
 } kg_query;
 typedef struct kg_query_result {
   int32_t status;               /* KG_OK or this query's error */
@@ -629,7 +631,8 @@ int kg_batch_run(kg_batch* b, const kg_query* qs, int32_t nq, kg_query_result* r
  * (ascending) after the call starts, on columns already resident in device
  * memory, and records each query's latency (completion - arrival) and result.
  * KG_SERVE_PLAIN runs queries one at a time, in arrival order, through the
- * plain entry points. KG_SERVE_BATCHED keeps up to `slots` batches in flight
+ * plain entry points (join queries reuse one pooled join table, reset per
+ * query, as the batched path reuses its arenas). KG_SERVE_BATCHED keeps up to `slots` batches in flight
  * on their own streams (continuous batching): whenever a slot is free, the
  * queries that have arrived are admitted in arrival order while their projected
  * device footprint (kg_batch_footprint) fits the slot's arena, the batch holds

@@ -2057,6 +2057,12 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   KG_REQUIRE(ctx, nq >= 0 && nq <= kBMaxQueries, "1 to %d queries per batch (got %d)", kBMaxQueries, nq);
   KG_REQUIRE(ctx, nq == 0 || qs, "queries are NULL");
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // Join queries without a load factor take the context's tuned choice for
+  // their build size (kg_join_tune), else 0.5.
+  std::vector<kg_query> resolved(qs, qs + std::max(nq, 0));
+  for (kg_query& q : resolved)
+    if (q.kind == KG_Q_JOIN_AGG && !(q.load_factor > 0)) q.load_factor = kg::tuned_load_factor(ctx, q.build_n);
+  qs = resolved.data();
   BHeader* H = b->hdr();
   BQuery* Q = b->hq();
   std::memset(H, 0, sizeof(BHeader));

@@ -115,6 +115,15 @@ namespace kg {
 
 constexpr int kWave = 64;
 
+// Join-table memory reuse (kg_join.hip): whether t's key / row memory holds a
+// table of capacity_rows at load_factor, and resetting it to such an empty
+// table (free markers, counters, form) without allocating.
+bool ht_fits(const kg_hash_table* t, int64_t capacity_rows, double load_factor);
+int ht_reset(kg_ctx* ctx, kg_hash_table* t, int64_t capacity_rows, double load_factor);
+// The load factor kg_join_tune chose on this context for builds of about
+// build_rows rows (same power-of-two class, unique keys preferred), or 0.5.
+double tuned_load_factor(const kg_ctx* ctx, int64_t build_rows);
+
 // The static device scratch of the no-allocation entry points: tile counts
 // (int32) and tile offsets (int64) of kg_scan_select_noalloc for up to
 // kNoAllocMaxTiles tiles of 4096 rows, then the selected-row total.

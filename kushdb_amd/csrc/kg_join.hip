@@ -1506,8 +1506,10 @@ __global__ __launch_bounds__(kJBlock) void k_join_group_agg(PredSet P, kg_col ke
   unsigned long long* lfirst = lrows + G;
   unsigned long long* lmn = lfirst + G;
   unsigned long long* lmx = lmn + (MINMAX ? G * nm : 0);
-  const int words = G * nm * 2 + G * 2 + (MINMAX ? 2 * G * nm : 0);
-  for (int i = threadIdx.x; i < words; i += kJBlock) lds[i] = 0;
+  // Each LDS word is set by exactly one of these loops (two loops writing
+  // one word from different waves would race without a barrier).
+  const int zero_words = G * nm * 2 + G;  // sums, non-NULL counts, rows
+  for (int i = threadIdx.x; i < zero_words; i += kJBlock) lds[i] = 0;
   for (int i = threadIdx.x; i < G; i += kJBlock) lfirst[i] = ~0ull;
   if (MINMAX)
     for (int i = threadIdx.x; i < G * nm; i += kJBlock) {
@@ -1670,6 +1672,8 @@ struct kg_hash_table {
   int device = 0;
   int form = -1;          // -1 until the first build with keys, then 0 wide / 1 narrow
   bool narrow_unfilled = false;  // narrow form chosen, its free markers not yet written
+  int64_t alloc_slots = 0;       // slots the key / row memory holds (kg::ht_reset reuses it)
+  int64_t acc_slots = 0;         // entries of acc (and first, when allocated)
   int64_t base = 0;       // narrow: smallest key
   TableView view() const {
     TableView v{keys, rows, nbuckets, side_rows, counters + 1, form == 1 ? 1 : 0, 0, base};
@@ -1891,6 +1895,58 @@ int region_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col& key, const int32_
 
 }  // namespace
 
+namespace kg {
+
+// Buckets of a table for capacity_rows at load_factor (an even count: the
+// narrow form has half as many buckets of 16 slots).
+uint64_t table_buckets_for(int64_t capacity_rows, double load_factor) {
+  const double slots = std::max<double>(static_cast<double>(capacity_rows) / load_factor, kSlots);
+  uint64_t nb = static_cast<uint64_t>(ceil_div(static_cast<int64_t>(slots), kSlots));
+  return nb + (nb & 1);
+}
+
+double tuned_load_factor(const kg_ctx* ctx, int64_t build_rows) {
+  int lg = 0;
+  while ((int64_t{1} << lg) < build_rows && lg < 63) lg++;
+  double lf = 0.5;
+  int best = 1 << 30;
+  for (const auto& t : ctx->join_tunes) {
+    if (static_cast<int>(t.cls >> 8) != lg) continue;
+    const int rank = static_cast<int>((t.cls >> 4) & 0xF) * 4 + static_cast<int>(t.cls & 0xF);  // unique, uniform first
+    if (rank < best) {
+      best = rank;
+      lf = t.load_factor;
+    }
+  }
+  return lf;
+}
+
+bool ht_fits(const kg_hash_table* t, int64_t capacity_rows, double load_factor) {
+  if (load_factor == 0) load_factor = 0.5;
+  return t && load_factor > 0 && load_factor <= 0.9 && capacity_rows >= 0 && capacity_rows < INT32_MAX &&
+         static_cast<int64_t>(table_buckets_for(capacity_rows, load_factor) * kSlots) <= t->alloc_slots;
+}
+
+int ht_reset(kg_ctx* ctx, kg_hash_table* t, int64_t capacity_rows, double load_factor) {
+  KG_REQUIRE(ctx, ht_fits(t, capacity_rows, load_factor), "table memory does not fit %lld rows",
+             (long long)capacity_rows);
+  if (load_factor == 0) load_factor = 0.5;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  t->capacity = capacity_rows;
+  t->nbuckets = table_buckets_for(capacity_rows, load_factor);
+  t->form = -1;
+  t->base = 0;
+  t->narrow_unfilled = false;
+  const int64_t nslots = static_cast<int64_t>(t->nbuckets) * kSlots;
+  hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ctx, nslots, 4)), dim3(kJBlock), 0, ctx->stream, t->keys, kEmpty,
+                     nslots);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0ull, 8);
+  KG_LAUNCH_CHECK(ctx);
+  return KG_OK;
+}
+
+}  // namespace kg
+
 extern "C" int kg_ht_create(kg_ctx* ctx, int64_t capacity_rows, double load_factor, kg_hash_table** out) {
   if (!ctx) return KG_ERR_INVALID;
   KG_REQUIRE(ctx, out, "out is NULL");
@@ -1903,10 +1959,9 @@ extern "C" int kg_ht_create(kg_ctx* ctx, int64_t capacity_rows, double load_fact
   if (!t) return kg_fail(ctx, KG_ERR_OOM, "host allocation failed");
   t->device = ctx->device;
   t->capacity = capacity_rows;
-  const double slots = std::max<double>(static_cast<double>(capacity_rows) / load_factor, kSlots);
-  t->nbuckets = static_cast<uint64_t>(ceil_div(static_cast<int64_t>(slots), kSlots));
-  t->nbuckets += t->nbuckets & 1;  // even: the narrow form has nbuckets / 2 buckets of 16 slots
+  t->nbuckets = kg::table_buckets_for(capacity_rows, load_factor);
   const size_t nslots = t->nbuckets * kSlots;
+  t->alloc_slots = static_cast<int64_t>(nslots);
   int st;
   if ((st = alloc(ctx, reinterpret_cast<void**>(&t->keys), nslots * 8)) != KG_OK ||
       (st = alloc(ctx, reinterpret_cast<void**>(&t->rows), nslots * 4)) != KG_OK ||
@@ -2320,8 +2375,19 @@ extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_co
   }
   if (plan.PF == 0) ctx->last_path = "hashed";
   const int64_t nslots = static_cast<int64_t>(t->nbuckets) * kSlots;
-  if (!t->acc && (st = alloc(ctx, reinterpret_cast<void**>(&t->acc), (nslots + 1) * sizeof(Acc))) != KG_OK) return st;
-  if (d_out_first && !t->first && (st = alloc(ctx, reinterpret_cast<void**>(&t->first), (nslots + 1) * 4)) != KG_OK)
+  if (t->acc_slots < nslots + 1) {  // a reset table may have grown past its accumulators
+    if (t->acc) (void)hipFree(t->acc);
+    if (t->first) (void)hipFree(t->first);
+    t->acc = nullptr;
+    t->first = nullptr;
+    t->acc_slots = 0;
+  }
+  if (!t->acc) {
+    if ((st = alloc(ctx, reinterpret_cast<void**>(&t->acc), (t->alloc_slots + 1) * sizeof(Acc))) != KG_OK) return st;
+    t->acc_slots = t->alloc_slots + 1;
+  }
+  if (d_out_first && !t->first &&
+      (st = alloc(ctx, reinterpret_cast<void**>(&t->first), t->acc_slots * 4)) != KG_OK)
     return st;
   // The atomic form: accumulator clear, probe, collect; all timed.
   kg_time_begin(ctx);

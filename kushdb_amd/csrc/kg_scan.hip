@@ -523,8 +523,10 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_lds(PredSet P, KeySet K, M
   unsigned long long* lfirst = lrows + G;                // [G]
   unsigned long long* lmn = lfirst + G;                  // [G][nm] (MINMAX)
   unsigned long long* lmx = lmn + (MINMAX ? G * nm : 0); // [G][nm] (MINMAX)
-  const int words = G * nm * 2 + G * 2 + (MINMAX ? 2 * G * nm : 0);
-  for (int i = threadIdx.x; i < words; i += kBlock) lds[i] = 0;
+  // Each LDS word is set by exactly one of these loops (two loops writing
+  // one word from different waves would race without a barrier).
+  const int zero_words = G * nm * 2 + G;  // sums, non-NULL counts, rows
+  for (int i = threadIdx.x; i < zero_words; i += kBlock) lds[i] = 0;
   for (int i = threadIdx.x; i < G; i += kBlock) lfirst[i] = INT32_MAX;
   if (MINMAX)
     for (int i = threadIdx.x; i < G * nm; i += kBlock) {

@@ -44,8 +44,32 @@ struct Scratch {
   }
 };
 
+// The plain path's join table: one table whose device memory is reused by
+// every join query (reset to an empty table of the query's size), replaced
+// by a larger one only when a query needs more slots, as the batched path
+// reuses its arena.
+struct TablePool {
+  kg_hash_table* t = nullptr;
+  kg_ctx* ctx = nullptr;
+  ~TablePool() {
+    if (t) kg_ht_free(ctx, t);
+  }
+  int get(kg_ctx* c, int64_t rows, double lf, kg_hash_table** out) {
+    ctx = c;
+    if (t && kg::ht_fits(t, rows, lf)) {
+      *out = t;
+      return kg::ht_reset(c, t, rows, lf);
+    }
+    if (t) kg_ht_free(c, t);
+    t = nullptr;
+    const int st = kg_ht_create(c, rows, lf, &t);
+    *out = t;
+    return st;
+  }
+};
+
 // One query through the plain entry points (what a host plan would call).
-int run_plain(kg_ctx* ctx, const kg_query& q, kg_query_result& R, Scratch& S) {
+int run_plain(kg_ctx* ctx, const kg_query& q, kg_query_result& R, Scratch& S, TablePool& T) {
   R.ngroups = 0;
   if (q.kind == KG_Q_SCAN_AGG)
     return kg_scan_group_agg(ctx, q.n, q.cols, q.ncols, q.preds, q.npreds, q.key_cols, q.key_card, q.nkeys, q.aggs,
@@ -63,7 +87,7 @@ int run_plain(kg_ctx* ctx, const kg_query& q, kg_query_result& R, Scratch& S) {
     sel = S.sel;
   }
   kg_hash_table* ht = nullptr;
-  if ((st = kg_ht_create(ctx, nb, q.load_factor, &ht)) != KG_OK) return st;
+  if ((st = T.get(ctx, nb, q.load_factor, &ht)) != KG_OK) return st;
   st = kg_ht_build(ctx, ht, &q.build_cols[q.build_key_col], sel, nb);
   if (st == KG_OK) {
     const kg_col* codes = q.build_code_col >= 0 ? &q.build_cols[q.build_code_col] : nullptr;
@@ -71,7 +95,6 @@ int run_plain(kg_ctx* ctx, const kg_query& q, kg_query_result& R, Scratch& S) {
                                 codes ? q.group_card : 1, q.aggs, q.naggs, R.max_groups, &R.ngroups, R.keys, R.vals,
                                 R.nulls);
   }
-  kg_ht_free(ctx, ht);
   return st;
 }
 
@@ -107,7 +130,14 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
   const int max_batch = opts->max_batch > 0 ? std::min(opts->max_batch, 256) : 128;
   KG_REQUIRE(ctx, nslots <= 8, "at most 8 batch slots");
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // Join queries without a load factor take the context's tuned choice (the
+  // batched path resolves them the same way, so footprints agree).
+  std::vector<kg_query> resolved(qs, qs + nq);
+  for (kg_query& q : resolved)
+    if (q.kind == KG_Q_JOIN_AGG && !(q.load_factor > 0)) q.load_factor = kg::tuned_load_factor(ctx, q.build_n);
+  qs = resolved.data();
   Scratch S;
+  TablePool T;
   std::vector<Slot> slots;
   kg_serve_stats st_{};
   auto cleanup = [&]() {
@@ -139,7 +169,7 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
     ndone++;
   };
   auto plain = [&](int32_t i) {
-    res[i].status = run_plain(ctx, qs[i], res[i], S);
+    res[i].status = run_plain(ctx, qs[i], res[i], S, T);
     if (res[i].status != KG_OK && first_err == KG_OK) first_err = res[i].status;
     finish(i);
   };

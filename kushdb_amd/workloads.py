@@ -182,12 +182,18 @@ def skinner_tables(scale=1.0, theta=0.0, device="cuda", seed=SEED + 4):
 # (10% misses).
 SERVE_FACT_SIZES = (10_000, 100_000, 1_000_000, 4_000_000, 16_000_000, 50_000_000)
 SERVE_BUILD_ROWS = 1 << 20
-FACT_COLS = ("day", "g1", "g2", "fk", "m1", "m2", "m3", "m4", "i1", "i2", "i3", "k64")
+FACT_COLS = ("day", "g1", "g2", "fk", "m1", "m2", "m3", "m4", "i1", "i2", "i3", "k64", "k64b")
 SERVE_TEMPLATES = ("c1_scan_agg", "c1_small", "two_filter", "two_filter_ungrouped", "c0_star", "star_dim_filter",
-                   "c2_join", "selective_join")
+                   "c2_join", "c2_join_16m", "selective_join")
 
 
-def serve_tables(fact_sizes=SERVE_FACT_SIZES, ndim=10_000, nbuild=SERVE_BUILD_ROWS, device="cuda", seed=SEED + 5):
+def serve_tables(fact_sizes=SERVE_FACT_SIZES, ndim=10_000, nbuild=SERVE_BUILD_ROWS, device="cuda", seed=SEED + 5,
+                 nbig=None):
+    """Fact tables, the 10k-row dimension, the build table of nbuild int64
+    keys and the big build table of nbig keys (16 * nbuild by default: 16M for
+    the serving mix), each a permutation of 0..rows-1. Fact k64 / k64b keys are
+    uniform over 1.1 times the build / big table's keys (10% misses)."""
+    nbig = 16 * nbuild if nbig is None else nbig
     facts = []
     for j, n in enumerate(fact_sizes):
         g = _gen(device, seed + 16 * j)
@@ -203,19 +209,25 @@ def serve_tables(fact_sizes=SERVE_FACT_SIZES, ndim=10_000, nbuild=SERVE_BUILD_RO
         for i in range(1, 4):
             t[f"i{i}"] = torch.randint(1, 51, (n,), generator=g, device=device, dtype=torch.int32)
         t["k64"] = torch.randint(0, int(nbuild * 1.1), (n,), generator=g, device=device, dtype=torch.int64)
+        t["k64b"] = torch.randint(0, int(nbig * 1.1), (n,), generator=g, device=device, dtype=torch.int64)
         facts.append(t)
     g = _gen(device, seed + 7)
     dim = {"pk": torch.arange(ndim, device=device, dtype=torch.int32),
            "cat": torch.randint(0, 25, (ndim,), generator=g, device=device, dtype=torch.int32)}
     build = {"key": torch.randperm(nbuild, generator=g, device=device).to(torch.int64),
              "payload": torch.randint(0, 1 << 30, (nbuild,), generator=g, device=device, dtype=torch.int32)}
-    return {"facts": facts, "dim": dim, "build": build}
+    g = _gen(device, seed + 9)
+    big = {"key": torch.randperm(nbig, generator=g, device=device).to(torch.int64),
+           "payload": torch.randint(0, 1 << 30, (nbig,), generator=g, device=device, dtype=torch.int32)}
+    return {"facts": facts, "dim": dim, "build": build, "big": big}
 
 
-def serve_queries(nq, fact_sizes=SERVE_FACT_SIZES, nbuild=SERVE_BUILD_ROWS, seed=SEED + 6, templates=None):
+def serve_queries(nq, fact_sizes=SERVE_FACT_SIZES, nbuild=SERVE_BUILD_ROWS, seed=SEED + 6, templates=None,
+                  nbig=None):
     """nq query specs: template, fact table index and parameters. Column
-    indices refer to FACT_COLS, the dimension's (pk, cat) and the build
-    table's (key, payload). Templates and tables are drawn uniformly."""
+    indices refer to FACT_COLS, the dimension's (pk, cat) and the build / big
+    build table's (key, payload). Templates and tables are drawn uniformly."""
+    nbig = 16 * nbuild if nbig is None else nbig
     import numpy as np
     from . import abi
     rng = np.random.RandomState(seed & 0x7FFFFFFF)
@@ -262,10 +274,14 @@ def serve_queries(nq, fact_sizes=SERVE_FACT_SIZES, nbuild=SERVE_BUILD_ROWS, seed
                 w = max(1, int(NDAYS * rng.uniform(0.01, 0.1)))
                 s["preds"] = [(0, abi.KG_GEQ, d0), (0, abi.KG_LT, d0 + w), (9, abi.KG_LT, int(rng.randint(10, 51)))]
                 s["aggs"] = [(C, -1), (S, 8)]
-        elif tpl == "c2_join":
-            lo = int(rng.randint(0, nbuild // 2))
-            hi = lo + int(nbuild * rng.uniform(0.1, 0.5))
-            s.update(kind="join", build="build", probe_key=11, build_key=0, build_codes=-1, group_card=1,
+        elif tpl in ("c2_join", "c2_join_16m"):
+            # A key range of 10-50% of the build (or the 16M-key big build)
+            # table is built; the fact's k64 (k64b) keys probe it.
+            nb = nbuild if tpl == "c2_join" else nbig
+            lo = int(rng.randint(0, nb // 2))
+            hi = lo + int(nb * rng.uniform(0.1, 0.5))
+            s.update(kind="join", build="build" if tpl == "c2_join" else "big",
+                     probe_key=11 if tpl == "c2_join" else 12, build_key=0, build_codes=-1, group_card=1,
                      build_preds=[(0, abi.KG_GEQ, lo), (0, abi.KG_LT, hi)])
             s["preds"] = [(0, abi.KG_GEQ, D + int(rng.randint(0, int(NDAYS * 0.5))))]
             s["aggs"] = [(C, -1), (S, 10)]
@@ -276,25 +292,27 @@ def serve_queries(nq, fact_sizes=SERVE_FACT_SIZES, nbuild=SERVE_BUILD_ROWS, seed
 
 
 def serve_columns(tables):
-    """Column objects of serve_tables: (facts [FACT_COLS order], dim, build)."""
+    """Column objects of serve_tables: (facts [FACT_COLS order], dim, builds),
+    builds = {"build": [key, payload], "big": [key, payload]}."""
     from . import abi
     from .runtime import Column
     types = {"day": abi.KG_DATE, "g1": abi.KG_ENUM, "g2": abi.KG_ENUM, "fk": abi.KG_INT, "m1": abi.KG_REAL,
              "m2": abi.KG_REAL, "m3": abi.KG_REAL, "m4": abi.KG_REAL, "i1": abi.KG_INT, "i2": abi.KG_INT,
-             "i3": abi.KG_INT, "k64": abi.KG_BIGINT}
+             "i3": abi.KG_INT, "k64": abi.KG_BIGINT, "k64b": abi.KG_BIGINT}
     facts = [[Column(types[c], t[c]) for c in FACT_COLS] for t in tables["facts"]]
     dim = [Column(abi.KG_INT, tables["dim"]["pk"]), Column(abi.KG_ENUM, tables["dim"]["cat"])]
-    build = [Column(abi.KG_BIGINT, tables["build"]["key"]), Column(abi.KG_INT, tables["build"]["payload"])]
-    return facts, dim, build
+    builds = {b: [Column(abi.KG_BIGINT, tables[b]["key"]), Column(abi.KG_INT, tables[b]["payload"])]
+              for b in ("build", "big")}
+    return facts, dim, builds
 
 
-def make_query(spec, facts, dim, build, load_factor=0.0):
+def make_query(spec, facts, dim, builds, load_factor=0.0):
     """runtime.Query for a spec of serve_queries over serve_columns."""
     from .runtime import Query
     cols = facts[spec["table"]]
     if spec["kind"] == "scan":
         return Query.scan_agg(cols, spec["preds"], spec["keys"], spec["aggs"], table_id=spec["table"])
-    bcols = dim if spec["build"] == "dim" else build
+    bcols = dim if spec["build"] == "dim" else builds[spec["build"]]
     return Query.join_agg(cols, spec["preds"], spec["probe_key"], bcols, spec["build_key"], spec["aggs"],
                           build_preds=spec["build_preds"], build_codes=spec["build_codes"],
                           group_card=spec["group_card"], table_id=spec["table"], load_factor=load_factor)

@@ -11,13 +11,14 @@ from kushdb_amd.workloads import FACT_COLS
 
 REL = 1e-12
 PO_T = {"day": po.DATE, "g1": po.ENUM, "g2": po.ENUM, "fk": po.INT, "m1": po.REAL, "m2": po.REAL, "m3": po.REAL,
-        "m4": po.REAL, "i1": po.INT, "i2": po.INT, "i3": po.INT, "k64": po.BIGINT}
+        "m4": po.REAL, "i1": po.INT, "i2": po.INT, "i3": po.INT, "k64": po.BIGINT, "k64b": po.BIGINT}
 
 
 def host_tables(tables):
     def h(d):
         return {k: v.cpu().numpy() for k, v in d.items()}
-    return {"facts": [h(t) for t in tables["facts"]], "dim": h(tables["dim"]), "build": h(tables["build"])}
+    return {"facts": [h(t) for t in tables["facts"]], "dim": h(tables["dim"]), "build": h(tables["build"]),
+            "big": h(tables["big"])}
 
 
 def _agg_col(f, a, at=None):
@@ -55,7 +56,7 @@ def oracle_spec(spec, H):
         B = H["dim"]
         bcols = [po.Col(po.INT, B["pk"]), po.Col(po.ENUM, B["cat"])]
     else:
-        B = H["build"]
+        B = H[spec["build"]]
         bcols = [po.Col(po.BIGINT, B["key"]), po.Col(po.INT, B["payload"])]
     brows = po.scan_select(bcols, spec["build_preds"]) if spec["build_preds"] else None
     pk = FACT_COLS[spec["probe_key"]]

@@ -36,7 +36,7 @@ def test_batch_mix_vs_oracle(env):
     ctx, _, cols, H = env
     specs = serve_queries(48, SIZES, NBUILD, seed=11)
     print("batch mix of", len(specs), flush=True)
-    b = Batch(ctx, arena_bytes=256 << 20)
+    b = Batch(ctx, arena_bytes=1 << 30)
     got = b.run(_queries(cols, specs), max_groups=64)
     for s, g in zip(specs, got):
         assert_same(s, g, oracle_spec(s, H), what="mix")
@@ -47,7 +47,7 @@ def test_alone_equals_mix(env):
     ctx, _, cols, _ = env
     specs = serve_queries(32, SIZES, NBUILD, seed=12)
     print("alone vs mix", flush=True)
-    b = Batch(ctx, arena_bytes=256 << 20)
+    b = Batch(ctx, arena_bytes=1 << 30)
     mix = b.run(_queries(cols, specs), max_groups=64)
     for s, m in zip(specs, mix):
         alone = b.run(_queries(cols, [s]), max_groups=64)[0]
@@ -218,4 +218,22 @@ def test_queries_complete_individually(env):
     b.release()
     for s, g in zip(specs, got):
         assert_same(s, g, oracle_spec(s, H), what="streamed")
+    b.free()
+
+
+def test_alone_equals_mix_join_templates(env):
+    """The join templates, including the big-build join (a 10-50% key range of
+    the 16x larger build table, probed by k64b), each alone equal to its
+    result in a mix with scans, and both against the oracle."""
+    ctx, _, cols, H = env
+    specs = serve_queries(60, SIZES, NBUILD, seed=19, templates=["c2_join_16m", "c2_join", "c1_scan_agg",
+                                                                 "selective_join"])[:16]
+    assert any(s["template"] == "c2_join_16m" for s in specs)
+    print("alone vs mix (join templates)", flush=True)
+    b = Batch(ctx, arena_bytes=1 << 30)
+    mix = b.run(_queries(cols, specs), max_groups=64)
+    for s, m in zip(specs, mix):
+        assert_same(s, m, oracle_spec(s, H), what="join mix")
+        alone = b.run(_queries(cols, [s]), max_groups=64)[0]
+        assert_same(s, alone, (m[1], m[2], m[3]), what="join alone")
     b.free()

@@ -78,3 +78,50 @@ def test_star_join_duplicates_and_misses():
     assert np.array_equal(codes, oc)
     assert np.array_equal(vals[:, 1], ocnt)
     assert np.allclose(vals[:, 0].view(np.float64), osum, rtol=REL, atol=0)
+
+
+def test_group_order_at_scale():
+    """A 12M-row fact with a measure (the star kernel's LDS table holds the
+    sums ahead of the first-row words, so the first-row slots start in a later
+    wave's range than the threads that set them): group order and every value
+    against numpy, for the star path and for the dense LDS GROUP BY
+    (50 groups, SUM, MIN and MAX). Regression for a race between the zeroing
+    and the first-row / MIN / MAX initialisation of the LDS table."""
+    ctx = Context(0)
+    rng = np.random.default_rng(8)
+    n = 12_000_000
+    fk = rng.integers(0, 10_000, n).astype(np.int32)
+    day = rng.integers(0, 2557, n).astype(np.int32)
+    qty = rng.integers(1, 51, n).astype(np.int32)
+    cat = rng.integers(0, 25, 10_000).astype(np.int32)
+    dv = lambda x: torch.as_tensor(x).to("cuda:0")
+    cols = [Column(abi.KG_INT, dv(fk)), Column(abi.KG_INT, dv(qty)), Column(abi.KG_DATE, dv(day))]
+    ht = HashTable(ctx, 10_000)
+    ht.insert(Column(abi.KG_INT, dv(np.arange(10_000, dtype=np.int32))))
+    codes, vals, _ = ht.scan_probe_group_agg(cols, [(2, abi.KG_GEQ, 100), (2, abi.KG_LT, 867)], 0,
+                                             Column(abi.KG_ENUM, dv(cat)), 25,
+                                             [(abi.KG_AGG_SUM, 1), (abi.KG_AGG_COUNT, -1)])
+    ht.free()
+    rows = np.nonzero((day >= 100) & (day < 867))[0]
+    g = cat[fk[rows]]
+    u, first = np.unique(g, return_index=True)
+    order = u[np.argsort(first)]
+    assert np.asarray(codes).reshape(-1).tolist() == order.tolist()
+    for i, c in enumerate(order):
+        assert vals[i, 0] == qty[rows][g == c].astype(np.int64).sum() and vals[i, 1] == (g == c).sum()
+    # dense LDS GROUP BY over 50 groups with MIN / MAX
+    key = rng.integers(0, 50, n).astype(np.int32)
+    kc = [Column(abi.KG_INT, dv(key)), Column(abi.KG_INT, dv(qty)), Column(abi.KG_DATE, dv(day))]
+    codes, vals, _ = ctx.scan_group_agg(kc, [(2, abi.KG_GEQ, 100)], [(0, 50)],
+                                        [(abi.KG_AGG_SUM, 1), (abi.KG_AGG_MIN, 1), (abi.KG_AGG_MAX, 1),
+                                         (abi.KG_AGG_COUNT, -1)], max_groups=64)
+    assert ctx.last_path() == "lds"
+    sel = np.nonzero(day >= 100)[0]
+    kk = key[sel]
+    u, first = np.unique(kk, return_index=True)
+    order = u[np.argsort(first)]
+    assert codes[:, 0].tolist() == order.tolist()
+    for i, c in enumerate(order):
+        q = qty[sel][kk == c].astype(np.int64)
+        assert vals[i].tolist() == [q.sum(), q.min(), q.max(), len(q)]
+    ctx.close()

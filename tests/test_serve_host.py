@@ -36,7 +36,7 @@ def test_composed_oracle_closed_forms(H, tpl):
         kv, vals, nulls = oracle_spec(s, H)
         m = _mask(f, s["preds"])
         if s["kind"] == "join":
-            B = H["dim"] if s["build"] == "dim" else H["build"]
+            B = H["dim"] if s["build"] == "dim" else H[s["build"]]
             bkey = B["pk"] if s["build"] == "dim" else B["key"]
             bm = np.ones(len(bkey), dtype=bool)
             for c, op, lit in s["build_preds"]:
