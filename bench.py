@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--lf", type=float, default=0.0, help="join table load factor (0: autotuned)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-skinner", action="store_true", help="C3 without the Skinner-driven runs")
-    ap.add_argument("--only", choices=["c0_star", "c2_join", "c3_chain", "serving"],
+    ap.add_argument("--only", choices=["c0_star", "c2_join", "c3_chain", "serving", "c2_build", "c2_probe_agg"],
                     help="run one workload alone (for profiling) and print only its section")
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--serve-slots", type=int, default=3)
@@ -282,7 +282,7 @@ def bench_join_theta(ctx, a, theta, cpu):
     out["join_build"] = {"rows": nb, "theta": theta, "load_factor": lf, "kernel_ms": round(bms, 3),
                          "rows_per_s": nb / (bms * 1e-3),
                          "roofline": roofline(nb * BUILD_BYTES_PER_ROW, bms,
-                                              measured_traffic("k_ht_build") if theta == 0 else None)}
+                                              query_traffic("c2_build") if theta == 0 else None)}
     # probe (join cardinality)
     ht.probe_count(pcol)
     ctx.set_timing(True)
@@ -330,7 +330,7 @@ def bench_join_theta(ctx, a, theta, cpu):
     out["join_probe_agg"] = {"rows": npr, "theta": theta, "groups": groups, "kernel_ms": round(ams, 3),
                              "rows_per_s": npr / (ams * 1e-3),
                              "roofline": roofline(agg_bytes, ams,
-                                                  measured_traffic("k_ht_probe_agg") if theta == 0 else None)}
+                                                  query_traffic("c2_probe_agg") if theta == 0 else None)}
     ht.free()
     if cpu:
         from oracle import pyoracle as po
@@ -346,6 +346,61 @@ def bench_join_theta(ctx, a, theta, cpu):
     del bk, pay, pk, pv
     torch.cuda.empty_cache()
     return out
+
+
+def bench_c2_op(ctx, a, op):
+    """One C2 operation at full size and theta = --thetas[0], once, for
+    counter runs: 'c2_build' generates the tables and builds the 16M-key table
+    (load factor --lf, 0.35 when 0: the tuner's choice at this size);
+    'c2_probe_agg' also runs one probe + GROUP BY matched key after the build.
+    The operation's dispatches are the last ones of the process."""
+    import torch
+    from kushdb_amd import abi
+    from kushdb_amd.runtime import Column, HashTable
+    from kushdb_amd.workloads import join_tables
+    bk, _, pk, pv = join_tables(a.build_rows, a.probe_rows, theta=a.thetas[0], device="cuda:0")
+    torch.cuda.synchronize()
+    ht = HashTable(ctx, a.build_rows, load_factor=a.lf or 0.35)
+    ht.insert(Column(abi.KG_BIGINT, bk))
+    out = {"op": op, "build_rows": a.build_rows, "load_factor": a.lf or 0.35}
+    if op == "c2_probe_agg":
+        k, _, _, _ = ht.probe_agg(Column(abi.KG_BIGINT, pk), pv, max_groups=a.build_rows)
+        out.update(probe_rows=a.probe_rows, groups=int(k.numel()))
+    torch.cuda.synchronize()
+    ht.free()
+    return out
+
+
+def pmc_counters(name, *csvs):
+    """Per kernel (largest dispatch of each), every counter of the given
+    rocprofv3 --pmc counter_collection CSVs, summed over the dispatch's
+    instances, as JSON under profiles/counters.json[name]."""
+    import csv
+    per = {}
+    for path in csvs:
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            k = (r["Kernel_Name"], r["Dispatch_Id"])
+            acc.setdefault(k, {}).setdefault(r["Counter_Name"], 0.0)
+            acc[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        for (kern, did), cs in acc.items():
+            short = kern.split("(")[0].replace("void ", "").replace("kg::", "")[:70]
+            for cn, v in cs.items():
+                d = per.setdefault(short, {}).setdefault(cn, [])
+                d.append(v)
+    out = {k: {cn: {"dispatches": len(v), "max": max(v), "sum": sum(v)} for cn, v in cs.items()}
+           for k, cs in per.items()}
+    path = os.path.join(ROOT, "profiles", "counters.json")
+    try:
+        with open(path) as fh:
+            allc = json.load(fh)
+    except (OSError, ValueError):
+        allc = {}
+    allc[name] = out
+    with open(path, "w") as fh:
+        json.dump(allc, fh, indent=1, sort_keys=True)
+    print(json.dumps(out, indent=1))
+    return 0
 
 
 def bench_join(ctx, a):
@@ -644,6 +699,8 @@ def main():
         return pmc_summary(*sys.argv[2:6])
     if len(sys.argv) > 1 and sys.argv[1] == "--pmc-query-summary":
         return pmc_query_summary(*sys.argv[2:6])
+    if len(sys.argv) > 1 and sys.argv[1] == "--pmc-counters":
+        return pmc_counters(*sys.argv[2:])
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design note, "
@@ -655,7 +712,8 @@ def main():
     ctx = Context(0)
     if a.only:
         run_one = {"c0_star": bench_star_c0, "c2_join": bench_join, "c3_chain": bench_chain_c3,
-                   "serving": bench_serving}[a.only]
+                   "serving": bench_serving, "c2_build": lambda c, x: bench_c2_op(c, x, "c2_build"),
+                   "c2_probe_agg": lambda c, x: bench_c2_op(c, x, "c2_probe_agg")}[a.only]
         print(json.dumps({a.only: run_one(ctx, a)}), flush=True)
         return 0
     t, wall, kms, groups = bench_scan_agg(ctx, a)

@@ -306,7 +306,11 @@ int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild, const kg_
 
 /* Number of build rows inserted so far. */
 int kg_ht_size(kg_ctx* ctx, const kg_hash_table* ht, int64_t* out_rows);
-/* Release the table (runtime::HashTable::Free, hash_table.cc:53-59). */
+/* Release the table (runtime::HashTable::Free, hash_table.cc:53-59). Its
+ * device memory goes back to ctx's block cache (at most 2 GiB per context,
+ * released by kg_ctx_destroy), where the next kg_ht_create on ctx takes it, in
+ * the order of ctx's stream: a plan that creates and frees a table per query
+ * allocates nothing after its first query. */
 int kg_ht_free(kg_ctx* ctx, kg_hash_table* ht);
 
 /* ---- star pipeline: scan-select -> join probe -> group-by ----------------------

@@ -1,4 +1,5 @@
 // Context, errors and workspace for the kushgpu C ABI.
+#include <algorithm>
 #include <cstdarg>
 #include <mutex>
 #include <new>
@@ -44,6 +45,54 @@ int kg_fail(kg_ctx* ctx, int status, const char* fmt, ...) {
     ctx->err = buf;
   }
   return status;
+}
+
+int kg_dev_alloc(kg_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  bytes = (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+  // The smallest cached block that holds `bytes` without wasting more than
+  // about half of itself.
+  auto it = ctx->block_cache.lower_bound(bytes);
+  if (it != ctx->block_cache.end() && it->first <= 2 * bytes + (size_t{1} << 20)) {
+    *p = it->second;
+    ctx->block_live[*p] = it->first;
+    ctx->block_cache_bytes -= it->first;
+    ctx->block_cache.erase(it);
+    return KG_OK;
+  }
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    // Give the cached blocks back to the device and try once more.
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->block_cache) (void)hipFree(e.second);
+    ctx->block_cache.clear();
+    ctx->block_cache_bytes = 0;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return kg_fail(ctx, KG_ERR_OOM, "device allocation of %zu bytes failed", bytes);
+    }
+  }
+  ctx->block_live[*p] = bytes;
+  return KG_OK;
+}
+
+void kg_dev_release(kg_ctx* ctx, void* p) {
+  if (!p) return;
+  auto it = ctx ? ctx->block_live.find(p) : decltype(ctx->block_live.end()){};
+  if (!ctx || it == ctx->block_live.end()) {
+    (void)hipFree(p);
+    return;
+  }
+  const size_t bytes = it->second;
+  ctx->block_live.erase(it);
+  if (ctx->block_cache_bytes + bytes > kg::kBlockCacheBytes) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(p);
+    return;
+  }
+  ctx->block_cache.emplace(bytes, p);
+  ctx->block_cache_bytes += bytes;
 }
 
 int kg_allow_lds(kg_ctx* ctx, const void* kernel, size_t bytes) {
@@ -182,6 +231,7 @@ int kg_ctx_destroy(kg_ctx* ctx) {
   }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   for (auto& f : ctx->files) kg_file_col_free(f);
+  for (auto& e : ctx->block_cache) (void)hipFree(e.second);
   if (ctx->work) hipFree(ctx->work);
   if (ctx->pinned) hipHostFree(ctx->pinned);
   if (ctx->own_stream) hipStreamDestroy(ctx->stream);
@@ -194,8 +244,10 @@ const char* kg_last_error(const kg_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 int kg_ctx_set_stream(kg_ctx* ctx, void* hip_stream) {
   if (!ctx) return KG_ERR_INVALID;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // Cached device blocks are reused in the order of the context's stream:
+  // everything queued on the old stream ends before the new one is used.
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->own_stream) {
-    KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     KG_HIP_CHECK(ctx, hipStreamDestroy(ctx->stream));
     ctx->own_stream = false;
     ctx->stream = nullptr;

@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "kushgpu.h"
@@ -69,7 +71,23 @@ struct kg_ctx {
   // Device address of this device's static scratch array (kg_context.hip),
   // used by the entry points that allocate nothing.
   void* scratch = nullptr;
+  // Device blocks handed out by kg_dev_alloc (pointer -> bytes), and released
+  // blocks kept for reuse (bytes -> pointer), at most kg::kBlockCacheBytes in
+  // all. Reuse is ordered on ctx->stream, where every user of these blocks
+  // enqueues its work.
+  std::multimap<size_t, void*> block_cache;
+  std::unordered_map<void*, size_t> block_live;
+  size_t block_cache_bytes = 0;
 };
+
+// Device memory for per-query structures (join tables, their accumulators):
+// a released block of at least `bytes` (and at most about twice that) from
+// the context's cache, else hipMalloc. kg_dev_release returns a block to the
+// cache (or frees it when the cache is full or the block is not the
+// context's). A plain plan creates and frees a join table per query; on the
+// device a hipMalloc / hipFree pair costs more than a small build and probe.
+int kg_dev_alloc(kg_ctx* ctx, void** p, size_t bytes);
+void kg_dev_release(kg_ctx* ctx, void* p);
 
 // Frees the device memory of a loaded file column.
 void kg_file_col_free(kg_ctx::FileCol& f);
@@ -129,6 +147,9 @@ double tuned_load_factor(const kg_ctx* ctx, int64_t build_rows);
 // kNoAllocMaxTiles tiles of 4096 rows, then the selected-row total.
 constexpr int64_t kNoAllocMaxTiles = 131072;
 constexpr int64_t kNoAllocScratchBytes = kNoAllocMaxTiles * 12 + 256;
+
+// Bytes of released device blocks a context keeps for reuse (kg_dev_alloc).
+constexpr size_t kBlockCacheBytes = size_t{2} << 30;
 
 // Width in bytes of a column's values for the SQL type ids of kushgpu.h.
 inline int type_width(int32_t type) {

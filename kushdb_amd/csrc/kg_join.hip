@@ -1683,14 +1683,9 @@ struct kg_hash_table {
 
 namespace {
 
-int alloc(kg_ctx* ctx, void** p, size_t bytes) {
-  if (hipMalloc(p, bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    *p = nullptr;
-    return kg_fail(ctx, KG_ERR_OOM, "device allocation of %zu bytes failed", bytes);
-  }
-  return KG_OK;
-}
+// Table memory comes from the context's block cache (kg_dev_alloc) and goes
+// back to it (kg_dev_release).
+int alloc(kg_ctx* ctx, void** p, size_t bytes) { return kg_dev_alloc(ctx, p, bytes); }
 
 // A column's data may be NULL only when it has no rows (an empty device tensor).
 int check_key(kg_ctx* ctx, const kg_col* key, int64_t n) {
@@ -1726,13 +1721,12 @@ int widen(kg_ctx* ctx, kg_hash_table* ht) {
   int st;
   if ((st = alloc(ctx, reinterpret_cast<void**>(&tk), nslots * 4)) != KG_OK) return st;
   if ((st = alloc(ctx, reinterpret_cast<void**>(&tr), nslots * 4)) != KG_OK) {
-    (void)hipFree(tk);
+    kg_dev_release(ctx, tk);
     return st;
   }
   auto done = [&](int s2) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(tk);
-    (void)hipFree(tr);
+    kg_dev_release(ctx, tk);
+    kg_dev_release(ctx, tr);
     return s2;
   };
   if (hipMemcpyAsync(tk, ht->keys, nslots * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess ||
@@ -2376,8 +2370,8 @@ extern "C" int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_co
   if (plan.PF == 0) ctx->last_path = "hashed";
   const int64_t nslots = static_cast<int64_t>(t->nbuckets) * kSlots;
   if (t->acc_slots < nslots + 1) {  // a reset table may have grown past its accumulators
-    if (t->acc) (void)hipFree(t->acc);
-    if (t->first) (void)hipFree(t->first);
+    kg_dev_release(ctx, t->acc);
+    kg_dev_release(ctx, t->first);
     t->acc = nullptr;
     t->first = nullptr;
     t->acc_slots = 0;
@@ -2440,17 +2434,13 @@ extern "C" int kg_ht_size(kg_ctx* ctx, const kg_hash_table* ht, int64_t* out_row
 
 extern "C" int kg_ht_free(kg_ctx* ctx, kg_hash_table* ht) {
   if (!ht) return KG_OK;
-  if (ctx) {
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-  }
-  if (ht->keys) hipFree(ht->keys);
-  if (ht->rows) hipFree(ht->rows);
-  if (ht->acc) hipFree(ht->acc);
-  if (ht->hot) hipFree(ht->hot);
-  if (ht->first) hipFree(ht->first);
-  if (ht->side_rows) hipFree(ht->side_rows);
-  if (ht->counters) hipFree(ht->counters);
+  if (ctx) (void)hipSetDevice(ctx->device);
+  // Back to the context's block cache (freed when the table is not the
+  // context's or there is no context).
+  for (void* p : {static_cast<void*>(ht->keys), static_cast<void*>(ht->rows), static_cast<void*>(ht->acc),
+                  static_cast<void*>(ht->hot), static_cast<void*>(ht->first), static_cast<void*>(ht->side_rows),
+                  static_cast<void*>(ht->counters)})
+    kg_dev_release(ctx && ctx->device == ht->device ? ctx : nullptr, p);
   delete ht;
   return KG_OK;
 }
@@ -2531,10 +2521,8 @@ int star_det_sums(kg_ctx* ctx, int64_t n, const PredSet& P, const kg_col& key, c
                                          static_cast<int64_t*>(nullptr), static_cast<int>(n + 1), s);
   char* buf = nullptr;
   const size_t cnt_b = (n + 1) * 8, off_b = (n + 1) * 8;
-  if (hipMalloc(&buf, cnt_b + off_b + scan_tmp + 256) != hipSuccess) {
-    (void)hipGetLastError();
+  if (kg_dev_alloc(ctx, reinterpret_cast<void**>(&buf), cnt_b + off_b + scan_tmp + 256) != KG_OK)
     return kg_fail(ctx, KG_ERR_OOM, "deterministic star: device allocation failed");
-  }
   int64_t* cnt = reinterpret_cast<int64_t*>(buf);
   int64_t* off = reinterpret_cast<int64_t*>(buf + cnt_b);
   void* tmp = buf + cnt_b + off_b;
@@ -2550,8 +2538,7 @@ int star_det_sums(kg_ctx* ctx, int64_t n, const PredSet& P, const kg_col& key, c
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   char* items = nullptr;
   if (e == hipSuccess && N > 0) {
-    if (hipMalloc(&items, N * 8) != hipSuccess) {
-      (void)hipGetLastError();
+    if (kg_dev_alloc(ctx, reinterpret_cast<void**>(&items), N * 8) != KG_OK) {
       st = kg_fail(ctx, KG_ERR_OOM, "deterministic star: %lld join rows", (long long)N);
     } else {
       int32_t* gid = reinterpret_cast<int32_t*>(items);
@@ -2564,8 +2551,8 @@ int star_det_sums(kg_ctx* ctx, int64_t n, const PredSet& P, const kg_col& key, c
     }
   }
   (void)hipStreamSynchronize(s);
-  if (items) (void)hipFree(items);
-  (void)hipFree(buf);
+  kg_dev_release(ctx, items);
+  kg_dev_release(ctx, buf);
   if (st != KG_OK) return st;
   if (e != hipSuccess) return kg_fail(ctx, KG_ERR_HIP, "deterministic star failed: %s", hipGetErrorString(e));
   return KG_OK;
