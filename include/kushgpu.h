@@ -146,15 +146,18 @@ int kg_ctx_set_deterministic(kg_ctx* ctx, int on);
  * probes are per tuple). */
 int kg_ctx_set_probe_regions(kg_ctx* ctx, int32_t regions);
 /* Join builds (kg_ht_build) of many rows group their (key, row) pairs by
- * table region first (histogram, scan, LDS-staged scatter), then build each
- * region's slice of the table in one workgroup's LDS and write it back with
- * coalesced stores, instead of one random CAS insert per row; the table's
- * contents are the same (the first free slot from the home bucket on), and
- * keys that overflow a region's last bucket are inserted directly afterwards.
- * regions: -1 (default) builds by regions when at least 2^20 rows go into a
- * table of at least 32 MB; 0 always inserts directly; > 0 uses at least that
- * many regions (rounded up to a multiple of 8) for any build. No counterpart
- * in the reference (its inserts are per tuple, runtime/hash_table.cc:18-27). */
+ * table region first (a histogram pass that also finds the key range, then two
+ * LDS counting-sort passes: coarse regions, then fine regions), then build each
+ * fine region's slice of the table in one workgroup's LDS (one LDS atomic per
+ * row on its bucket's fill count) and write it back with coalesced stores,
+ * instead of one random CAS insert per row; the table holds the same (key,
+ * row) entries per bucket chain, and keys that overflow a slice's last bucket
+ * are inserted directly afterwards. regions: -1 (default) builds by regions
+ * when at least 2^20 rows go into a table of at least 32 MB; 0 always inserts
+ * directly; > 0 partitions every build, with at least that many fine regions
+ * where the table allows (a power of two dividing its bucket count, slices
+ * within the LDS). No counterpart in the reference (its inserts are per
+ * tuple, runtime/hash_table.cc:18-27). */
 int kg_ctx_set_build_regions(kg_ctx* ctx, int32_t regions);
 /* Probe + GROUP BY matched build key (kg_ht_probe_agg without first rows) of
  * many rows: the probe rows are grouped by the table region of their key's home
@@ -286,7 +289,7 @@ int kg_ht_probe_agg(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, con
  * sample (unique / few duplicates / many) and the share of the most frequent
  * key in a probe-key sample (uniform / moderate / heavy). The first call for a
  * class builds a table over the build keys (at most 64M rows) and times probes
- * of a probe sample (at most 32M rows) under each candidate, by coordinate
+ * of a probe sample (at most 128M rows) under each candidate, by coordinate
  * descent over load factor {0.35, 0.5, 0.65} (by build time plus the probe
  * time scaled to all probe rows), workgroups per CU {8, 16, 32} and regions
  * {0, 64, 128} (by probe time); later calls for the class reuse the choice. The

@@ -671,8 +671,9 @@ __global__ __launch_bounds__(1024) void k_scan_u32(const uint32_t* in, uint32_t*
 // workgroup: each thread sums a run of consecutive counters, the run totals
 // are scanned across the wave (shuffles) and the waves (wtot in LDS), and each
 // thread writes its run's offsets. Ends with the workgroup synchronized.
+template <int NT = kJBlock>
 __device__ __forceinline__ void block_excl_scan(const uint32_t* cnt, uint32_t* off, int P, uint32_t* wtot) {
-  const int per = (P + kJBlock - 1) / kJBlock;
+  const int per = (P + NT - 1) / NT;
   const int lo = min(static_cast<int>(threadIdx.x) * per, P), hi = min(lo + per, P);
   uint32_t s = 0;
   for (int q = lo; q < hi; q++) s += cnt[q];
@@ -758,127 +759,417 @@ __global__ __launch_bounds__(kJBlock) void k_part_scatter(TableView t, kg_col ke
   }
 }
 
-// ---- region-partitioned build ---------------------------------------------------
-// For a large build, random CAS inserts into an HBM table cost two random
-// line round trips each (the key CAS, then the row id). Instead the (key, row)
-// pairs are first grouped by table region with the probe partitioner
-// (k_part_hist / scan / k_part_scatter: region = floor(bucket * P / buckets),
-// partitions XCD-major); then one workgroup per region builds the region's
-// slice of the table in LDS, one window of the slice at a time (W windows, each
-// at most the LDS budget): it copies the window into LDS, inserts the region's
-// keys whose home bucket lies in the window with LDS CAS (the same rule: first
-// free slot from the home bucket on), and writes the window back with
-// coalesced stores. A key that finds no free slot before its window's last
-// bucket is spilled and inserted afterwards by the global-atomic path, which
-// continues into the next window exactly as a direct insert would; so is a
-// wide table's free-marker key (it goes to the side list). The region's keys
-// are read once per window, from L2 after the first.
-constexpr int kRegionBlock = 1024;
-constexpr int kBuildTile = 4096;   // keys per scatter tile of a build
-constexpr int kItemBatch = 8;      // items per thread per step of the region kernel
+// ---- partitioned build ------------------------------------------------------------
+// A large build into a table beyond the caches would cost every row a random
+// CAS and a random row-id store on lines far apart in HBM. Instead the
+// (key, row) pairs are grouped by table region in two counting-sort passes
+// and each region's slice of the table is built in one workgroup's LDS and
+// written back with coalesced stores:
+//   A  k_bp_hist      per workgroup (a contiguous range of input rows), the
+//                     number of pairs per coarse region, and the key range
+//                     that decides the table form (one pass over the keys);
+//   B  k_bp_scatter1  the same rows, tile by tile counting-sorted by coarse
+//                     region in LDS and written as one run per region;
+//   C  k_bp_hist2     per chunk of a coarse region, the pairs per fine region;
+//   D  k_bp_scatter2  each chunk counting-sorted by fine region;
+//   E  k_bp_region    one workgroup per fine region: its slice of the table in
+//                     LDS (free, or read back when the table already holds
+//                     rows), one LDS atomic per pair on its bucket's fill count
+//                     (occupied slots are always a prefix of a bucket, so the
+//                     count is the first free slot), the slice written back.
+// Fine region f of a key = multiply-high of its hash by PF, coarse region =
+// f / (PF / PC). The table's bucket count (of the form in use) is a multiple
+// of PF, so region f holds exactly the keys whose home buckets lie in
+// [f m, (f + 1) m), m = buckets / PF: slices are disjoint. A key that finds
+// every bucket from its home to the slice's end full is spilled, and inserted
+// after all regions by the global-atomic path, which continues into the next
+// slice exactly as a direct insert would; so is a wide table's free-marker key
+// (it goes to the side list). The slots a key takes in a bucket may differ
+// from a direct build's; the set of (key, row) per bucket chain does not.
+constexpr int kBPCoarse = 128;         // coarse regions (at most)
+constexpr int kBPTile = 8192;          // pairs per scatter tile
+constexpr int kBPBlock = 256;          // threads of a histogram / region workgroup
+constexpr int kBPSBlock = 512;         // threads of a scatter workgroup
+constexpr int kBPHBlock = 1024;        // threads of a pass-A workgroup
+// Dynamic LDS of a scatter workgroup: the tile's keys, row ids and bins.
+constexpr size_t kBPScatterLds = kBPTile * (8 + 4 + 1);
+constexpr int64_t kBPSliceBudget = 48 * 1024;  // LDS bytes of a region's slice (3 or more workgroups per CU)
 
+struct BPGeo {
+  uint64_t nbf;   // buckets of the table's form (16-slot buckets for a narrow table)
+  int64_t m;      // buckets per fine region
+  int32_t PF;     // fine regions, a power of two
+  int32_t PC;     // coarse regions, a power of two dividing PF
+  int32_t F2;     // fine regions per coarse region
+};
+
+__device__ __forceinline__ uint32_t bp_fine(const BPGeo& g, int64_t k) {
+  return static_cast<uint32_t>(__umul64hi(mix64(static_cast<uint64_t>(k)), static_cast<uint64_t>(g.PF)));
+}
+
+// A workgroup's contiguous range of input rows, of even length (pairs of
+// consecutive rows are read together).
+__device__ __forceinline__ void bp_block_range(int64_t n, int64_t& lo, int64_t& hi) {
+  int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  chunk += chunk & 1;
+  lo = static_cast<int64_t>(blockIdx.x) * chunk;
+  hi = lo + chunk < n ? lo + chunk : n;
+  if (lo > n) lo = n;
+}
+
+// Build keys and row ids of input rows i and i + 1 (below hi). FAST: a
+// non-NULL BIGINT key column, 16-byte aligned, no row selection, so both keys
+// come from one 16-byte load. ok*: the row exists and its key is not NULL.
+template <bool FAST>
+__device__ __forceinline__ void bp_load2(const kg_col& key, const int32_t* sel, int64_t i, int64_t hi, int64_t& k0,
+                                         int64_t& k1, int32_t& r0, int32_t& r1, bool& ok0, bool& ok1) {
+  k0 = k1 = 0;
+  if (FAST) {
+    const int64_t* kp = static_cast<const int64_t*>(key.data);
+    r0 = static_cast<int32_t>(i);
+    r1 = static_cast<int32_t>(i + 1);
+    ok0 = i < hi;
+    ok1 = i + 1 < hi;
+    if (ok1) {
+      typedef long long v2i64 __attribute__((ext_vector_type(2)));
+      const v2i64 v = __builtin_nontemporal_load(reinterpret_cast<const v2i64*>(kp + i));
+      k0 = v[0];
+      k1 = v[1];
+    } else if (ok0) {
+      k0 = __builtin_nontemporal_load(kp + i);
+    }
+    return;
+  }
+  r0 = i < hi ? (sel ? sel[i] : static_cast<int32_t>(i)) : 0;
+  r1 = i + 1 < hi ? (sel ? sel[i + 1] : static_cast<int32_t>(i + 1)) : 0;
+  ok0 = i < hi && load_key_nt(key, r0, k0);
+  ok1 = i + 1 < hi && load_key_nt(key, r1, k1);
+}
+
+// Pass A. hist[c * gridDim.x + block] (and hist[PC * gridDim.x] = 0);
+// range[0] / range[1] receive the smallest / largest non-NULL key (signed, by
+// atomicMin / atomicMax).
+template <bool FAST>
+__global__ __launch_bounds__(kBPHBlock) void k_bp_hist(kg_col key, const int32_t* sel, int64_t n, BPGeo g,
+                                                        uint32_t* hist, unsigned long long* range) {
+  constexpr int NW = kBPHBlock / kWave;
+  __shared__ uint32_t cnt[NW][kBPCoarse];
+  __shared__ int64_t wlo[NW], whi[NW];
+  for (int q = threadIdx.x; q < NW * kBPCoarse; q += kBPHBlock) (&cnt[0][0])[q] = 0;
+  __syncthreads();
+  const int wave = threadIdx.x / kWave;
+  const int shift = __builtin_ctz(static_cast<unsigned>(g.F2));
+  int64_t lo, hi;
+  bp_block_range(n, lo, hi);
+  int64_t kmin = INT64_MAX, kmax = INT64_MIN;
+  constexpr int U = 4;  // row pairs in flight per thread
+  for (int64_t i0 = lo + 2 * threadIdx.x; i0 < hi; i0 += static_cast<int64_t>(kBPHBlock) * 2 * U) {
+    int64_t kk[2 * U];
+    bool ok[2 * U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int32_t r0, r1;
+      bp_load2<FAST>(key, sel, i0 + static_cast<int64_t>(u) * 2 * kBPHBlock, hi, kk[2 * u], kk[2 * u + 1], r0, r1,
+                     ok[2 * u], ok[2 * u + 1]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2 * U; u++) {
+      if (!ok[u]) continue;
+      kmin = kk[u] < kmin ? kk[u] : kmin;
+      kmax = kk[u] > kmax ? kk[u] : kmax;
+      atomicAdd(&cnt[wave][bp_fine(g, kk[u]) >> shift], 1u);
+    }
+  }
+  for (int s = kWave / 2; s >= 1; s >>= 1) {
+    const int64_t a = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(kmin), s));
+    const int64_t b = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(kmax), s));
+    kmin = a < kmin ? a : kmin;
+    kmax = b > kmax ? b : kmax;
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    wlo[wave] = kmin;
+    whi[wave] = kmax;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < g.PC; q += kBPHBlock) {
+    uint32_t c = 0;
+    for (int w = 0; w < NW; w++) c += cnt[w][q];
+    hist[static_cast<int64_t>(q) * gridDim.x + blockIdx.x] = c;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) hist[static_cast<int64_t>(g.PC) * gridDim.x] = 0;  // the scan's last input
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < NW; w++) {
+      kmin = wlo[w] < kmin ? wlo[w] : kmin;
+      kmax = whi[w] > kmax ? whi[w] : kmax;
+    }
+    if (kmin <= kmax) {
+      atomicMin(reinterpret_cast<long long*>(&range[0]), static_cast<long long>(kmin));
+      atomicMax(reinterpret_cast<long long*>(&range[1]), static_cast<long long>(kmax));
+    }
+  }
+}
+
+// Counting sort of one tile of up to kBPTile pairs (kk / rr, reg = bin or -1
+// per item, kPer items per thread) into LDS, then one run per bin written at
+// the bin's cursor; cursors advance by the bin counts. Ends synchronized.
+template <int kPer, typename BinT, int NT = kBPSBlock>
+__device__ __forceinline__ void bp_tile_scatter(const int64_t (&kk)[kPer], const int32_t (&rr)[kPer],
+                                                const int (&reg)[kPer], int nbin, uint32_t* cursor, uint32_t* lcnt,
+                                                uint32_t* loff, uint32_t* wtot, int64_t* tkeys, int32_t* trows,
+                                                BinT* tbin, int64_t* okeys, int32_t* orows) {
+  uint32_t rank[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; j++)
+    if (reg[j] >= 0) rank[j] = atomicAdd(&lcnt[reg[j]], 1u);
+  __syncthreads();
+  block_excl_scan<NT>(lcnt, loff, nbin, wtot);
+#pragma unroll
+  for (int j = 0; j < kPer; j++) {
+    if (reg[j] < 0) continue;
+    const uint32_t slot = loff[reg[j]] + rank[j];
+    tkeys[slot] = kk[j];
+    trows[slot] = rr[j];
+    tbin[slot] = static_cast<BinT>(reg[j]);
+  }
+  __syncthreads();
+  const uint32_t total = loff[nbin - 1] + lcnt[nbin - 1];
+  for (uint32_t slot = threadIdx.x; slot < total; slot += NT) {
+    const int q = tbin[slot];
+    const uint32_t o = cursor[q] + (slot - loff[q]);
+    okeys[o] = tkeys[slot];
+    orows[o] = trows[slot];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nbin; q += NT) {
+    cursor[q] += lcnt[q];
+    lcnt[q] = 0;
+  }
+  __syncthreads();
+}
+
+// Pass B: rows of the workgroup's range into coarse runs at the scanned pass-A
+// offsets (off[c * gridDim.x + block]).
+template <bool FAST>
+__global__ __launch_bounds__(kBPSBlock) void k_bp_scatter1(kg_col key, const int32_t* sel, int64_t n, BPGeo g,
+                                                           const uint32_t* off, int64_t* okeys, int32_t* orows) {
+  __shared__ uint32_t cursor[kBPCoarse], lcnt[kBPCoarse], loff[kBPCoarse];
+  __shared__ uint32_t wtot[kBPSBlock / kWave];
+  extern __shared__ int64_t tkeys[];  // kBPTile keys, then kBPTile row ids and bins
+  int32_t* trows = reinterpret_cast<int32_t*>(tkeys + kBPTile);
+  uint8_t* tbin = reinterpret_cast<uint8_t*>(trows + kBPTile);
+  constexpr int kPer = kBPTile / kBPSBlock;
+  const int shift = __builtin_ctz(static_cast<unsigned>(g.F2));
+  for (int q = threadIdx.x; q < g.PC; q += kBPSBlock) {
+    cursor[q] = off[static_cast<int64_t>(q) * gridDim.x + blockIdx.x];
+    lcnt[q] = 0;
+  }
+  __syncthreads();
+  int64_t lo, hi;
+  bp_block_range(n, lo, hi);
+  for (int64_t t0 = lo; t0 < hi; t0 += kBPTile) {
+    int64_t kk[kPer];
+    int32_t rr[kPer];
+    int reg[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer / 2; j++) {
+      bool ok0, ok1;
+      bp_load2<FAST>(key, sel, t0 + 2 * threadIdx.x + static_cast<int64_t>(j) * 2 * kBPSBlock, hi, kk[2 * j],
+                     kk[2 * j + 1], rr[2 * j], rr[2 * j + 1], ok0, ok1);
+      reg[2 * j] = ok0 ? static_cast<int>(bp_fine(g, kk[2 * j]) >> shift) : -1;
+      reg[2 * j + 1] = ok1 ? static_cast<int>(bp_fine(g, kk[2 * j + 1]) >> shift) : -1;
+    }
+    bp_tile_scatter<kPer>(kk, rr, reg, g.PC, cursor, lcnt, loff, wtot, tkeys, trows, tbin, okeys, orows);
+  }
+}
+
+// Workgroup (c, q) of passes C and D: the q-th of Q chunks of coarse region
+// c's pairs (off1: the scanned pass-A histogram over nblk1 workgroups).
+__device__ __forceinline__ void bp_chunk(const uint32_t* off1, int nblk1, int Q, int& c, int& q, int64_t& lo,
+                                         int64_t& hi) {
+  c = blockIdx.x / Q;
+  q = blockIdx.x % Q;
+  const int64_t s = off1[static_cast<int64_t>(c) * nblk1], e = off1[static_cast<int64_t>(c + 1) * nblk1];
+  lo = s + (e - s) * q / Q;
+  hi = s + (e - s) * (q + 1) / Q;
+}
+
+constexpr int kBPMaxF2 = 256;  // fine regions per coarse region (at most)
+
+// Pass C: hist2[(c * F2 + fl) * Q + q] = pairs of chunk (c, q) in fine region
+// c * F2 + fl (and hist2[PF * Q] = 0); clears the spill count of pass E.
+__global__ __launch_bounds__(kBPHBlock) void k_bp_hist2(BPGeo g, const int64_t* keys, int nblk1, int Q,
+                                                        const uint32_t* off1, uint32_t* hist2,
+                                                        unsigned long long* spill_count) {
+  __shared__ uint32_t fcnt[kBPMaxF2];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hist2[static_cast<int64_t>(g.PF) * Q] = 0;  // the scan's last input
+    *spill_count = 0;
+  }
+  for (int f = threadIdx.x; f < g.F2; f += kBPHBlock) fcnt[f] = 0;
+  __syncthreads();
+  int c, q;
+  int64_t lo, hi;
+  bp_chunk(off1, nblk1, Q, c, q, lo, hi);
+  const uint32_t base = static_cast<uint32_t>(c) * static_cast<uint32_t>(g.F2);
+  constexpr int U = 8;
+  for (int64_t i0 = lo + threadIdx.x; i0 < hi; i0 += static_cast<int64_t>(kBPHBlock) * U) {
+    int64_t kk[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + static_cast<int64_t>(u) * kBPHBlock;
+      kk[u] = i < hi ? __builtin_nontemporal_load(keys + i) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t fl = bp_fine(g, kk[u]) - base;
+      if (i0 + static_cast<int64_t>(u) * kBPHBlock < hi && fl < static_cast<uint32_t>(g.F2)) atomicAdd(&fcnt[fl], 1u);
+    }
+  }
+  __syncthreads();
+  for (int f = threadIdx.x; f < g.F2; f += kBPHBlock) hist2[(static_cast<int64_t>(base) + f) * Q + q] = fcnt[f];
+}
+
+// Pass D: chunk (c, q) counting-sorted by fine region into okeys / orows at
+// the scanned pass-C offsets.
+__global__ __launch_bounds__(kBPSBlock) void k_bp_scatter2(BPGeo g, const int64_t* keys, const int32_t* rows,
+                                                           int nblk1, int Q, const uint32_t* off1,
+                                                           const uint32_t* off2, int64_t* okeys, int32_t* orows) {
+  __shared__ uint32_t cursor[kBPMaxF2], lcnt[kBPMaxF2], loff[kBPMaxF2];
+  __shared__ uint32_t wtot[kBPSBlock / kWave];
+  extern __shared__ int64_t tkeys[];  // kBPTile keys, then kBPTile row ids and bins
+  int32_t* trows = reinterpret_cast<int32_t*>(tkeys + kBPTile);
+  uint8_t* tbin = reinterpret_cast<uint8_t*>(trows + kBPTile);
+  constexpr int kPer = kBPTile / kBPSBlock;
+  int c, q;
+  int64_t lo, hi;
+  bp_chunk(off1, nblk1, Q, c, q, lo, hi);
+  const uint32_t base = static_cast<uint32_t>(c) * static_cast<uint32_t>(g.F2);
+  for (int f = threadIdx.x; f < g.F2; f += kBPSBlock) {
+    cursor[f] = off2[(static_cast<int64_t>(base) + f) * Q + q];
+    lcnt[f] = 0;
+  }
+  __syncthreads();
+  for (int64_t t0 = lo; t0 < hi; t0 += kBPTile) {
+    int64_t kk[kPer];
+    int32_t rr[kPer];
+    int reg[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int64_t i = t0 + threadIdx.x + static_cast<int64_t>(j) * kBPSBlock;
+      const bool in = i < hi;
+      kk[j] = in ? __builtin_nontemporal_load(keys + i) : 0;
+      rr[j] = in ? __builtin_nontemporal_load(rows + i) : 0;
+      const uint32_t fl = bp_fine(g, kk[j]) - base;
+      reg[j] = (in && fl < static_cast<uint32_t>(g.F2)) ? static_cast<int>(fl) : -1;
+    }
+    bp_tile_scatter<kPer>(kk, rr, reg, g.F2, cursor, lcnt, loff, wtot, tkeys, trows, tbin, okeys, orows);
+  }
+}
+
+// Pass E: workgroup f builds buckets [f m, (f + 1) m) of the table from fine
+// region f's pairs [off2[f * Q], off2[(f + 1) * Q]). fresh: the table holds no
+// rows yet, so the slice starts free (and every slot of the table is written,
+// free markers included); otherwise it is read back. Spilled pairs go to
+// spill_keys / spill_rows.
 template <bool NARROW>
-__global__ __launch_bounds__(kRegionBlock) void k_build_region(TableView t, int P, int W, int nblk,
-                                                                const uint32_t* offs, const int64_t* pkeys,
-                                                                const int32_t* prows, int fresh,
-                                                                unsigned long long* inserted, int64_t* spill_keys,
-                                                                int32_t* spill_rows, unsigned long long* spill_count) {
+__global__ __launch_bounds__(kBPBlock) void k_bp_region(TableView t, BPGeo g, int Q, const uint32_t* off2,
+                                                         const int64_t* keys, const int32_t* rows, int fresh,
+                                                         unsigned long long* inserted, int64_t* spill_keys,
+                                                         int32_t* spill_rows, unsigned long long* spill_count) {
   using KeyT = typename std::conditional<NARROW, uint32_t, unsigned long long>::type;
   constexpr int S = NARROW ? kSlots32 : kSlots;
   const KeyT kFree = NARROW ? static_cast<KeyT>(kEmpty32) : static_cast<KeyT>(kEmpty);
   extern __shared__ unsigned long long lds_words[];
-  const int q = blockIdx.x;
-  const int per = P / kXcds;
-  const uint64_t p = static_cast<uint64_t>(q % per) * kXcds + q / per;  // region of partition q
-  const uint64_t nb = NARROW ? nbuckets32(t) : t.nbuckets;
-  const uint64_t lo = (p * nb + P - 1) / P, hi = ((p + 1) * nb + P - 1) / P;
-  const int64_t wbk = static_cast<int64_t>((hi - lo + W - 1) / W);  // buckets per window
+  const int64_t m = g.m;
+  const int64_t f = blockIdx.x;
   KeyT* lkey = reinterpret_cast<KeyT*>(lds_words);
-  int32_t* lrow = reinterpret_cast<int32_t*>(lkey + wbk * S);
+  int32_t* lrow = reinterpret_cast<int32_t*>(lkey + m * S);
+  int32_t* lfill = lrow + m * S;
   KeyT* gkey = NARROW ? reinterpret_cast<KeyT*>(keys32(t)) : reinterpret_cast<KeyT*>(t.keys);
-  const int64_t a = offs[static_cast<int64_t>(q) * nblk], b = offs[static_cast<int64_t>(q + 1) * nblk];
-  for (int w = 0; w < W; w++) {
-    const int64_t wlo = static_cast<int64_t>(lo) + w * wbk;
-    const int64_t whi = min(wlo + wbk, static_cast<int64_t>(hi));
-    if (wlo >= whi) break;
-    const int64_t ns = (whi - wlo) * S;
-    const int64_t g0 = wlo * S;
-    // 16-byte transfers: a bucket is 64 bytes of keys, and S row ids
-    // (S * 4 bytes, a multiple of 16) start at a multiple of S.
-    {
-      uint4* lk4 = reinterpret_cast<uint4*>(lkey);
-      uint4* lr4 = reinterpret_cast<uint4*>(lrow);
-      const uint4* gk4 = reinterpret_cast<const uint4*>(gkey + g0);
-      const uint4* gr4 = reinterpret_cast<const uint4*>(t.rows + g0);
-      const int64_t nk4 = ns * static_cast<int64_t>(sizeof(KeyT)) / 16, nr4 = ns / 4;
-      const uint4 free4 = NARROW ? make_uint4(kEmpty32, kEmpty32, kEmpty32, kEmpty32)
-                                 : make_uint4(0u, 0x80000000u, 0u, 0x80000000u);
-      for (int64_t i = threadIdx.x; i < nk4; i += kRegionBlock) lk4[i] = fresh ? free4 : gk4[i];
-      for (int64_t i = threadIdx.x; i < nr4; i += kRegionBlock) lr4[i] = fresh ? make_uint4(0u, 0u, 0u, 0u) : gr4[i];
+  const int64_t g0 = f * m * S;  // first slot of the slice
+  const int64_t ns = m * S;
+  {
+    uint4* lk4 = reinterpret_cast<uint4*>(lkey);
+    uint4* lr4 = reinterpret_cast<uint4*>(lrow);
+    const uint4* gk4 = reinterpret_cast<const uint4*>(gkey + g0);
+    const uint4* gr4 = reinterpret_cast<const uint4*>(t.rows + g0);
+    const int64_t nk4 = ns * static_cast<int64_t>(sizeof(KeyT)) / 16, nr4 = ns / 4;
+    const uint4 free4 = NARROW ? make_uint4(kEmpty32, kEmpty32, kEmpty32, kEmpty32)
+                               : make_uint4(0u, 0x80000000u, 0u, 0x80000000u);
+    if (fresh) {
+      for (int64_t i = threadIdx.x; i < nk4; i += kBPBlock) lk4[i] = free4;
+      for (int64_t i = threadIdx.x; i < nr4; i += kBPBlock) lr4[i] = make_uint4(0u, 0u, 0u, 0u);
+      for (int64_t b = threadIdx.x; b < m; b += kBPBlock) lfill[b] = 0;
+    } else {
+      for (int64_t i = threadIdx.x; i < nk4; i += kBPBlock) lk4[i] = gk4[i];
+      for (int64_t i = threadIdx.x; i < nr4; i += kBPBlock) lr4[i] = gr4[i];
     }
-    __syncthreads();
-    // kItemBatch items per thread per step, their loads issued together.
-    for (int64_t i0 = a + threadIdx.x; i0 < b; i0 += static_cast<int64_t>(kRegionBlock) * kItemBatch) {
-      int64_t kb[kItemBatch];
-      int32_t rb[kItemBatch];
-#pragma unroll
-      for (int u = 0; u < kItemBatch; u++) {
-        const int64_t i = i0 + static_cast<int64_t>(u) * kRegionBlock;
-        kb[u] = i < b ? pkeys[i] : 0;
-        rb[u] = i < b ? prows[i] : -1;
-      }
-#pragma unroll
-      for (int u = 0; u < kItemBatch; u++) {
-        if (i0 + static_cast<int64_t>(u) * kRegionBlock >= b) continue;
-        const int64_t k = kb[u];
-        bool ok;
-        KeyT kk;
-        if (NARROW) {
-          uint32_t k32;
-          ok = narrow_key(t, k, k32);
-          kk = static_cast<KeyT>(k32);
-        } else {
-          ok = k != kEmpty;
-          kk = static_cast<KeyT>(k);
-        }
-        const int64_t home = static_cast<int64_t>(bucket_of(k, nb));
-        // A key is inserted (or spilled) only by the window of its home
-        // bucket; keys no window can take are spilled by window 0.
-        if (ok ? (home < wlo || home >= whi) : w != 0) continue;
-        bool placed = false;
-        if (ok) {
-          for (int64_t bb = home - wlo; bb < whi - wlo && !placed; bb++) {
-            for (int s = 0; s < S; s++) {
-              KeyT* slot = lkey + bb * S + s;
-              if (*slot != kFree) continue;
-              if (atomicCAS(slot, kFree, kk) == kFree) {
-                lrow[bb * S + s] = rb[u];
-                placed = true;
-                break;
-              }
-            }
-          }
-        }
-        if (!placed) {
-          const unsigned long long pos = atomicAdd(spill_count, 1ull);
-          spill_keys[pos] = k;
-          spill_rows[pos] = rb[u];
-        }
-      }
-    }
-    __syncthreads();
-    {
-      const uint4* lk4 = reinterpret_cast<const uint4*>(lkey);
-      const uint4* lr4 = reinterpret_cast<const uint4*>(lrow);
-      uint4* gk4 = reinterpret_cast<uint4*>(gkey + g0);
-      uint4* gr4 = reinterpret_cast<uint4*>(t.rows + g0);
-      const int64_t nk4 = ns * static_cast<int64_t>(sizeof(KeyT)) / 16, nr4 = ns / 4;
-      for (int64_t i = threadIdx.x; i < nk4; i += kRegionBlock) gk4[i] = lk4[i];
-      for (int64_t i = threadIdx.x; i < nr4; i += kRegionBlock) gr4[i] = lr4[i];
+  }
+  __syncthreads();
+  if (!fresh) {
+    // Fill count of each bucket: its first free slot.
+    for (int64_t b = threadIdx.x; b < m; b += kBPBlock) {
+      int c = 0;
+      while (c < S && lkey[b * S + c] != kFree) c++;
+      lfill[b] = c;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && b > a) atomicAdd(inserted, static_cast<unsigned long long>(b - a));
+  const int64_t a = off2[f * Q], e = off2[(f + 1) * Q];
+  const uint64_t first = static_cast<uint64_t>(f * m);
+  // One pair: the first bucket from its home with a free slot (its fill count
+  // below S), else the spill list.
+  auto insert_one = [&](int64_t k, int32_t r) {
+    bool ok;
+    KeyT kv;
+    if (NARROW) {
+      uint32_t k32;
+      ok = narrow_key(t, k, k32);
+      kv = static_cast<KeyT>(k32);
+    } else {
+      ok = k != kEmpty;
+      kv = static_cast<KeyT>(k);
+    }
+    const int64_t home = static_cast<int64_t>(bucket_of(k, g.nbf)) - static_cast<int64_t>(first);
+    if (ok && home >= 0) {
+      for (int64_t bb = home; bb < m; bb++) {
+        const int c = atomicAdd(&lfill[bb], 1);
+        if (c < S) {
+          lkey[bb * S + c] = kv;
+          lrow[bb * S + c] = r;
+          return;
+        }
+      }
+    }
+    const unsigned long long pos = atomicAdd(spill_count, 1ull);
+    spill_keys[pos] = k;
+    spill_rows[pos] = r;
+  };
+  constexpr int U = 8;  // pairs in flight per thread
+  for (int64_t i0 = a + threadIdx.x; i0 < e; i0 += static_cast<int64_t>(kBPBlock) * U) {
+    int64_t kk[U];
+    int32_t rr[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + static_cast<int64_t>(u) * kBPBlock;
+      kk[u] = i < e ? __builtin_nontemporal_load(keys + i) : 0;
+      rr[u] = i < e ? __builtin_nontemporal_load(rows + i) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (i0 + static_cast<int64_t>(u) * kBPBlock < e) insert_one(kk[u], rr[u]);
+  }
+  __syncthreads();
+  {
+    const uint4* lk4 = reinterpret_cast<const uint4*>(lkey);
+    const uint4* lr4 = reinterpret_cast<const uint4*>(lrow);
+    uint4* gk4 = reinterpret_cast<uint4*>(gkey + g0);
+    uint4* gr4 = reinterpret_cast<uint4*>(t.rows + g0);
+    const int64_t nk4 = ns * static_cast<int64_t>(sizeof(KeyT)) / 16, nr4 = ns / 4;
+    for (int64_t i = threadIdx.x; i < nk4; i += kBPBlock) gk4[i] = lk4[i];
+    for (int64_t i = threadIdx.x; i < nr4; i += kBPBlock) gr4[i] = lr4[i];
+  }
+  if (threadIdx.x == 0 && e > a) atomicAdd(inserted, static_cast<unsigned long long>(e - a));
 }
 
 // Inserts the spilled (key, row) pairs with the global-atomic path.
@@ -985,7 +1276,7 @@ __global__ __launch_bounds__(kJBlock) void k_collect(TableView t, const Acc* acc
 // workgroup sums their rows in LDS and adds the totals to a small device
 // array, which the region owning each such key folds in before it writes.
 constexpr int kAggFine = 128;          // fine regions per coarse region
-constexpr int kAggMargin = 4;          // chain buckets past a region covered by its window
+constexpr int kAggMargin = 8;          // chain buckets past a region covered by its window
 constexpr int kAggTile = 2048;         // rows per scatter tile
 constexpr int kAggBlock = 1024;        // threads of a region workgroup
 constexpr int64_t kAggLdsBudget = 96 * 1024;
@@ -1347,7 +1638,10 @@ __global__ __launch_bounds__(kAggBlock) void k_agg_region(TableView t, AggGeo g,
   const int S = g.S;
   const uint64_t f = blockIdx.x;
   const uint64_t blo = (f * nb) / g.PF, bhi = ((f + 1) * nb + g.PF - 1) / g.PF;
-  const uint64_t wb = min(bhi - blo + kAggMargin, nb - blo);
+  // The window may run past the table's last bucket onto its first ones (a
+  // chain wraps around there); slot positions are taken modulo the table.
+  const uint64_t wb = min(bhi - blo + kAggMargin, nb);
+  const int64_t tslots = static_cast<int64_t>(nb) * S;
   const int64_t nsl = static_cast<int64_t>(wb) * S;
   const int64_t slot0 = static_cast<int64_t>(blo) * S;
   unsigned long long* lsum = lacc;
@@ -1370,8 +1664,9 @@ __global__ __launch_bounds__(kAggBlock) void k_agg_region(TableView t, AggGeo g,
       int64_t first;
       const unsigned nm = agg_find<NARROW>(t, nb, kk[u], bucket_of(kk[u], nb), first);
       if (nm == 0) continue;
-      const int64_t rel = first - slot0;
-      if (rel < 0 || rel >= nsl) {
+      int64_t rel = first - slot0;
+      if (rel < 0) rel += tslots;
+      if (rel >= nsl) {
         atomicOr(fallback, 1);
         continue;
       }
@@ -1386,8 +1681,9 @@ __global__ __launch_bounds__(kAggBlock) void k_agg_region(TableView t, AggGeo g,
     int64_t first;
     const unsigned nm = agg_find<NARROW>(t, nb, k, bucket_of(k, nb), first);
     if (nm == 0) continue;
-    const int64_t rel = first - slot0;
-    if (rel < 0 || rel >= nsl) {
+    int64_t rel = first - slot0;
+    if (rel < 0) rel += tslots;
+    if (rel >= nsl) {
       atomicOr(fallback, 1);
       continue;
     }
@@ -1424,7 +1720,7 @@ __global__ __launch_bounds__(kAggBlock) void k_agg_region(TableView t, AggGeo g,
     if (live) {
       const unsigned long long p = pos + before + __popcll(m & ((1ull << lane) - 1ull));
       if (p < static_cast<unsigned long long>(cap)) {
-        const int64_t gs = slot0 + s;
+        const int64_t gs = slot0 + s < tslots ? slot0 + s : slot0 + s - tslots;
         out_key[p] = NARROW ? static_cast<int64_t>(static_cast<uint64_t>(t.base) + keys32(t)[gs]) : t.keys[gs];
         out_sum[p] = static_cast<int64_t>(lsum[s]);
         out_cnt[p] = static_cast<int64_t>(lcnt[s]);
@@ -1780,107 +2076,179 @@ int choose_form(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t
   return KG_OK;
 }
 
-// Layout of a region-partitioned build: P regions (a multiple of 8) of W LDS
-// windows each; P = 0 inserts directly. Builds of fewer than 2^20 rows, or into
-// tables small enough for the L2 caches, insert directly. A window (8 B per
-// narrow slot, 12 B per wide slot) fits the LDS budget; W grows so that there
-// are at most about four regions per CU (fewer regions give longer runs in the
-// scatter, more windows per region cost the region kernel a pass over its keys
-// each).
-constexpr int64_t kRegionLdsBudget = 128 * 1024;
-
-struct BuildLayout {
-  int P = 0, W = 1;
-};
-
-BuildLayout build_layout(kg_ctx* ctx, const kg_hash_table* ht, int64_t nrows) {
-  BuildLayout L;
-  if (ctx->build_regions == 0 || (ctx->build_regions < 0 && nrows < (1 << 20))) return L;
-  const bool narrow = ht->form == 1;
-  const int S = narrow ? kSlots32 : kSlots;
-  const int64_t nb = static_cast<int64_t>(narrow ? ht->nbuckets / 2 : ht->nbuckets);
-  const int64_t slot_bytes = narrow ? 8 : 12;
-  if (ctx->build_regions < 0 && nb * S * slot_bytes < (32ll << 20)) return L;
-  const int64_t budget = std::min<int64_t>(kRegionLdsBudget, ctx->max_lds);
-  const int64_t max_wb = budget / (S * slot_bytes);  // buckets per window
-  const int64_t windows = ceil_div(nb, max_wb);
-  int64_t W = 1, P;
-  if (ctx->build_regions > 0) {
-    P = std::max<int64_t>(ctx->build_regions, ceil_div(windows, 8));
-    P = ceil_div(P, kXcds) * kXcds;
-    W = ceil_div(windows, P);
-  } else {
-    W = std::min<int64_t>(8, std::max<int64_t>(1, ceil_div(windows, 4 * ctx->num_cus)));
-    P = ceil_div(ceil_div(windows, W), kXcds) * kXcds;
-  }
-  // The scatter keeps 3 words per region in LDS beside its tile.
-  const int64_t max_p = (ctx->max_lds - (8 + kBuildTile * 14)) / 12 / kXcds * kXcds;
-  if (ctx->build_regions > 0) P = std::min(P, max_p);
-  if (P > max_p || P > nb || P < kXcds) return L;
-  W = ceil_div(windows, P);
-  if (ceil_div(ceil_div(nb, P), W) > max_wb) W++;
-  L.P = static_cast<int>(P);
-  L.W = static_cast<int>(W);
-  return L;
+// Partitioned build (passes A-E above). Used for builds of at least 2^20 rows
+// into tables of at least 32 MB (kg_ctx_set_build_regions: 0 never, > 0 for
+// every build, with at least that many fine regions where the table allows).
+bool use_part_build(const kg_ctx* ctx, const kg_hash_table* ht, int64_t nrows) {
+  if (ctx->build_regions == 0) return false;
+  if (ctx->build_regions > 0) return true;
+  return nrows >= (1 << 20) && ht->nbuckets * kSlots * 8 >= (32ull << 20);
 }
 
-// The region-partitioned build (see k_build_region); the whole sequence is the
-// timed region.
-int region_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col& key, const int32_t* sel, int64_t n,
-                 const BuildLayout& L, bool fresh) {
-  const int P = L.P, W = L.W;
-  const bool narrow = ht->form == 1;
-  const int nblk = static_cast<int>(std::min<int64_t>(4096, std::max<int64_t>(kXcds, ceil_div(n, 4 * kBuildTile))));
-  const int64_t m = static_cast<int64_t>(P) * nblk;
-  size_t scan_tmp = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, static_cast<uint32_t*>(nullptr),
-                                         static_cast<uint32_t*>(nullptr), static_cast<int>(m + 1), ctx->stream);
+// Fine regions for a table form of nbf buckets of bpb LDS bytes each: the
+// smallest power of two dividing nbf whose slices fit kBPSliceBudget (or the
+// requested count), at most 32768.
+int bp_regions(const kg_ctx* ctx, uint64_t nbf, int64_t bpb) {
+  const uint64_t pmax = std::min<uint64_t>(nbf & (~nbf + 1), 32768);
+  uint64_t pf = 1;
+  while (pf < pmax && static_cast<int64_t>(nbf / pf) * bpb > kBPSliceBudget) pf <<= 1;
+  while (pf < pmax && ctx->build_regions > 0 && pf < static_cast<uint64_t>(ctx->build_regions)) pf <<= 1;
+  return static_cast<int>(pf);
+}
+
+// LDS bytes per bucket of a region slice: keys, row ids, fill count.
+int64_t bp_bucket_bytes(bool narrow) { return narrow ? kSlots32 * 8 + 4 : kSlots * 12 + 4; }
+
+// Sets the table form from the key range [lo, hi] of this build (the rule of
+// choose_form); widens a narrow table the keys leave.
+int form_from_range(kg_ctx* ctx, kg_hash_table* ht, int32_t key_type, int64_t lo, int64_t hi) {
+  if (ht->form == 0) return KG_OK;
+  if (key_type == KG_REAL) return ht->form == 1 ? widen(ctx, ht) : (ht->form = 0, KG_OK);
+  if (lo > hi) return KG_OK;  // no non-NULL keys
+  const uint64_t limit = static_cast<uint64_t>(kEmpty32) - 1;
+  if (ht->form == -1) {
+    if (static_cast<uint64_t>(hi) - static_cast<uint64_t>(lo) <= limit) {
+      ht->form = 1;
+      ht->base = lo;
+      ht->narrow_unfilled = true;
+    } else {
+      ht->form = 0;
+    }
+    return KG_OK;
+  }
+  if (lo < ht->base || static_cast<uint64_t>(hi) - static_cast<uint64_t>(ht->base) > limit) return widen(ctx, ht);
+  return KG_OK;
+}
+
+// Inserts directly (one CAS per row): the form from a key-range pass, the
+// narrow free markers when first needed, then k_ht_build.
+int direct_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t* d_rows, int64_t nrows) {
+  int st;
+  if ((st = choose_form(ctx, ht, key, d_rows, nrows)) != KG_OK) return st;
+  if (ht->narrow_unfilled) {
+    ht->narrow_unfilled = false;
+    const int64_t words = static_cast<int64_t>(ht->nbuckets) * kSlots / 2;  // 32-bit slots / 2
+    kg_time_begin(ctx);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, words, 4)), dim3(kJBlock), 0, ctx->stream,
+                       reinterpret_cast<unsigned long long*>(ht->keys), ~0ull, words);
+    kg_time_end(ctx);
+    KG_LAUNCH_CHECK(ctx);
+  }
+  kg_time_begin(ctx);
+  hipLaunchKernelGGL(k_ht_build, dim3(grid_for(ctx, nrows, 4)), dim3(kJBlock), 0, ctx->stream, ht->view(), *key,
+                     d_rows, nrows, ht->counters, reinterpret_cast<int*>(ht->counters + 2));
+  kg_time_end(ctx);
+  KG_LAUNCH_CHECK(ctx);
+  return KG_OK;
+}
+
+// The partitioned build; falls back to direct_build when the table's bucket
+// count cannot be split into slices that fit the LDS.
+int part_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, const int32_t* sel, int64_t n, bool fresh) {
+  const uint64_t nb = ht->nbuckets;
+  const int pf_wide = bp_regions(ctx, nb, bp_bucket_bytes(false));
+  const int pf_narrow = bp_regions(ctx, nb / 2, bp_bucket_bytes(true));
+  const int PC = std::min(kBPCoarse, std::min(pf_wide, pf_narrow));
+  const int nblk = static_cast<int>(std::min<int64_t>(4096, std::max<int64_t>(8, ceil_div(n, 4 * kBPTile))));
+  const int Q = std::max(1, nblk / PC);
+  const int pf_cap = std::max(pf_wide, pf_narrow);
+  const int64_t m1 = static_cast<int64_t>(PC) * nblk, m2 = static_cast<int64_t>(pf_cap) * Q;
+  size_t tmp1 = 0, tmp2 = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp1, static_cast<uint32_t*>(nullptr),
+                                         static_cast<uint32_t*>(nullptr), static_cast<int>(m1 + 1), ctx->stream);
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, static_cast<uint32_t*>(nullptr),
+                                         static_cast<uint32_t*>(nullptr), static_cast<int>(m2 + 1), ctx->stream);
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     const size_t o2 = off;
     off += (bytes + 255) / 256 * 256;
     return o2;
   };
-  const size_t o_hist = carve((m + 1) * 4), o_offs = carve((m + 1) * 4), o_keys = carve(n * 8), o_rows = carve(n * 4),
-               o_skeys = carve(n * 8), o_srows = carve(n * 4), o_scnt = carve(8), o_tmp = carve(scan_tmp);
+  const size_t o_h1 = carve((m1 + 1) * 4), o_o1 = carve((m1 + 1) * 4), o_h2 = carve((m2 + 1) * 4),
+               o_o2 = carve((m2 + 1) * 4), o_k1 = carve(n * 8), o_r1 = carve(n * 4), o_k2 = carve(n * 8),
+               o_r2 = carve(n * 4), o_sc = carve(8), o_tmp = carve(std::max(tmp1, tmp2));
   int st = kg_work_reserve(ctx, off);
   if (st != KG_OK) return st;
   char* w = static_cast<char*>(ctx->work);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(w + o_hist);
-  uint32_t* offs = reinterpret_cast<uint32_t*>(w + o_offs);
-  int64_t* pkeys = reinterpret_cast<int64_t*>(w + o_keys);
-  int32_t* prows = reinterpret_cast<int32_t*>(w + o_rows);
-  int64_t* skeys = reinterpret_cast<int64_t*>(w + o_skeys);
-  int32_t* srows = reinterpret_cast<int32_t*>(w + o_srows);
-  unsigned long long* scnt = reinterpret_cast<unsigned long long*>(w + o_scnt);
-  const size_t scatter_lds = static_cast<size_t>(3 * P + 1) * 4 + 8 + kBuildTile * 8 + kBuildTile * 4 + kBuildTile * 2;
-  const int S = narrow ? kSlots32 : kSlots;
-  const int64_t nb = static_cast<int64_t>(narrow ? ht->nbuckets / 2 : ht->nbuckets);
-  const size_t region_lds = static_cast<size_t>(ceil_div(ceil_div(nb, P), W)) * S * (narrow ? 8 : 12) + 64;
-  if ((st = kg_allow_lds(ctx, reinterpret_cast<const void*>(&k_part_scatter<false, true, kBuildTile>), scatter_lds)) !=
-      KG_OK)
-    return st;
+  uint32_t* h1 = reinterpret_cast<uint32_t*>(w + o_h1);
+  uint32_t* o1 = reinterpret_cast<uint32_t*>(w + o_o1);
+  uint32_t* h2 = reinterpret_cast<uint32_t*>(w + o_h2);
+  uint32_t* o2 = reinterpret_cast<uint32_t*>(w + o_o2);
+  int64_t* k1 = reinterpret_cast<int64_t*>(w + o_k1);
+  int32_t* r1 = reinterpret_cast<int32_t*>(w + o_r1);
+  int64_t* k2 = reinterpret_cast<int64_t*>(w + o_k2);
+  int32_t* r2 = reinterpret_cast<int32_t*>(w + o_r2);
+  unsigned long long* scnt = reinterpret_cast<unsigned long long*>(w + o_sc);
+  unsigned long long* rng = ht->counters + 4;
+  // Pass A (coarse regions only: PF = PC, F2 = 1) and the key range.
+  BPGeo g{0, 0, PC, PC, 1};
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, rng, static_cast<unsigned long long>(INT64_MAX),
+                     1);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, rng + 1,
+                     static_cast<unsigned long long>(INT64_MIN), 1);
+  kg_time_begin(ctx);
+  const bool fast = key->type == KG_BIGINT && !key->nulls && !sel && (reinterpret_cast<uintptr_t>(key->data) & 15) == 0;
+  if (fast)
+    hipLaunchKernelGGL(k_bp_hist<true>, dim3(nblk), dim3(kBPHBlock), 0, ctx->stream, *key, sel, n, g, h1, rng);
+  else
+    hipLaunchKernelGGL(k_bp_hist<false>, dim3(nblk), dim3(kBPHBlock), 0, ctx->stream, *key, sel, n, g, h1, rng);
+  kg_time_end(ctx);
+  KG_LAUNCH_CHECK(ctx);
+  if ((st = kg_pinned_reserve(ctx, 64)) != KG_OK) return st;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, rng, 16, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  const int64_t lo = static_cast<const int64_t*>(ctx->pinned)[0], hi = static_cast<const int64_t*>(ctx->pinned)[1];
+  if ((st = form_from_range(ctx, ht, key->type, lo, hi)) != KG_OK) return st;
+  if (ht->form < 0) return KG_OK;  // every key is NULL: nothing to insert
+  const bool narrow = ht->form == 1;
+  g.nbf = narrow ? nb / 2 : nb;
+  g.PF = std::min(narrow ? pf_narrow : pf_wide, PC * kBPMaxF2);
+  g.F2 = g.PF / PC;
+  g.m = static_cast<int64_t>(g.nbf / g.PF);
+  const size_t lds = static_cast<size_t>(g.m * bp_bucket_bytes(narrow)) + 64;
+  if (g.nbf % g.PF != 0 || static_cast<int64_t>(lds) > ctx->max_lds) return direct_build(ctx, ht, key, sel, n);
   const void* region_kernel =
-      narrow ? reinterpret_cast<const void*>(&k_build_region<true>) : reinterpret_cast<const void*>(&k_build_region<false>);
-  if ((st = kg_allow_lds(ctx, region_kernel, region_lds)) != KG_OK) return st;
+      narrow ? reinterpret_cast<const void*>(&k_bp_region<true>) : reinterpret_cast<const void*>(&k_bp_region<false>);
+  if ((st = kg_allow_lds(ctx, region_kernel, lds)) != KG_OK ||
+      (st = kg_allow_lds(ctx, reinterpret_cast<const void*>(fast ? &k_bp_scatter1<true> : &k_bp_scatter1<false>),
+                         kBPScatterLds)) != KG_OK ||
+      (st = kg_allow_lds(ctx, reinterpret_cast<const void*>(&k_bp_scatter2), kBPScatterLds)) != KG_OK)
+    return st;
+  if (ht->narrow_unfilled) {
+    ht->narrow_unfilled = false;
+    if (!fresh) {  // slices are read back: the free markers must be there first
+      const int64_t words = static_cast<int64_t>(nb) * kSlots / 2;
+      hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, words, 4)), dim3(kJBlock), 0, ctx->stream,
+                         reinterpret_cast<unsigned long long*>(ht->keys), ~0ull, words);
+    }
+  }
   const TableView v = ht->view();
   kg_time_begin(ctx);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, scnt, 0ull, 1);
-  hipLaunchKernelGGL(k_fill_i32j, dim3(1), dim3(64), 0, ctx->stream, reinterpret_cast<int32_t*>(hist + m), 0, 1);
-  hipLaunchKernelGGL(k_part_hist, dim3(nblk), dim3(kJBlock), P * 4, ctx->stream, v, key, sel, n, P, hist);
-  size_t tb = scan_tmp;
-  if (hipcub::DeviceScan::ExclusiveSum(w + o_tmp, tb, hist, offs, static_cast<int>(m + 1), ctx->stream) != hipSuccess)
+  size_t tb = tmp1;
+  if (hipcub::DeviceScan::ExclusiveSum(w + o_tmp, tb, h1, o1, static_cast<int>(m1 + 1), ctx->stream) != hipSuccess)
     return kg_fail(ctx, KG_ERR_HIP, "build histogram scan failed");
-  hipLaunchKernelGGL((k_part_scatter<false, true, kBuildTile>), dim3(nblk), dim3(kJBlock), scatter_lds, ctx->stream, v,
-                     key, sel, static_cast<const int64_t*>(nullptr), n, P, offs, pkeys, static_cast<int64_t*>(nullptr),
-                     prows);
-  if (narrow)
-    hipLaunchKernelGGL(k_build_region<true>, dim3(P), dim3(kRegionBlock), region_lds, ctx->stream, v, P, W, nblk, offs,
-                       pkeys, prows, fresh ? 1 : 0, ht->counters, skeys, srows, scnt);
+  if (fast)
+    hipLaunchKernelGGL(k_bp_scatter1<true>, dim3(nblk), dim3(kBPSBlock), kBPScatterLds, ctx->stream, *key, sel, n, g,
+                       o1, k1, r1);
   else
-    hipLaunchKernelGGL(k_build_region<false>, dim3(P), dim3(kRegionBlock), region_lds, ctx->stream, v, P, W, nblk,
-                       offs, pkeys, prows, fresh ? 1 : 0, ht->counters, skeys, srows, scnt);
-  hipLaunchKernelGGL(k_build_spill, dim3(ctx->num_cus), dim3(kJBlock), 0, ctx->stream, v, skeys, srows, scnt,
+    hipLaunchKernelGGL(k_bp_scatter1<false>, dim3(nblk), dim3(kBPSBlock), kBPScatterLds, ctx->stream, *key, sel, n,
+                       g, o1, k1, r1);
+  hipLaunchKernelGGL(k_bp_hist2, dim3(PC * Q), dim3(kBPHBlock), 0, ctx->stream, g, k1, nblk, Q, o1, h2, scnt);
+  tb = tmp2;
+  const int64_t mf = static_cast<int64_t>(g.PF) * Q;
+  if (hipcub::DeviceScan::ExclusiveSum(w + o_tmp, tb, h2, o2, static_cast<int>(mf + 1), ctx->stream) != hipSuccess)
+    return kg_fail(ctx, KG_ERR_HIP, "build region histogram scan failed");
+  hipLaunchKernelGGL(k_bp_scatter2, dim3(PC * Q), dim3(kBPSBlock), kBPScatterLds, ctx->stream, g, k1, r1, nblk, Q, o1,
+                     o2, k2,
+                     r2);
+  // The coarse pairs are consumed: their buffers take the spilled pairs.
+  if (narrow)
+    hipLaunchKernelGGL(k_bp_region<true>, dim3(g.PF), dim3(kBPBlock), lds, ctx->stream, v, g, Q, o2, k2, r2,
+                       fresh ? 1 : 0, ht->counters, k1, r1, scnt);
+  else
+    hipLaunchKernelGGL(k_bp_region<false>, dim3(g.PF), dim3(kBPBlock), lds, ctx->stream, v, g, Q, o2, k2, r2,
+                       fresh ? 1 : 0, ht->counters, k1, r1, scnt);
+  hipLaunchKernelGGL(k_build_spill, dim3(ctx->num_cus), dim3(kJBlock), 0, ctx->stream, v, k1, r1, scnt,
                      reinterpret_cast<int*>(ht->counters + 2));
   kg_time_end(ctx);
   KG_LAUNCH_CHECK(ctx);
@@ -1893,9 +2261,17 @@ namespace kg {
 
 // Buckets of a table for capacity_rows at load_factor (an even count: the
 // narrow form has half as many buckets of 16 slots).
+// Tables of 2^15 buckets or more take a multiple of a power of two g of at
+// least nb / 256 buckets (at most 0.8% more), so that the partitioned build
+// can split either form into equal slices.
 uint64_t table_buckets_for(int64_t capacity_rows, double load_factor) {
   const double slots = std::max<double>(static_cast<double>(capacity_rows) / load_factor, kSlots);
   uint64_t nb = static_cast<uint64_t>(ceil_div(static_cast<int64_t>(slots), kSlots));
+  if (nb >= (uint64_t{1} << 15)) {
+    uint64_t g = 1;
+    while (g * 256 < nb) g <<= 1;
+    nb = (nb + g - 1) / g * g;
+  }
   return nb + (nb & 1);
 }
 
@@ -1986,28 +2362,9 @@ extern "C" int kg_ht_build(kg_ctx* ctx, kg_hash_table* ht, const kg_col* key, co
   if ((st = read_counter(ctx, ht->counters, &inserted)) != KG_OK) return st;
   if (static_cast<int64_t>(inserted) + nrows > ht->capacity)
     return kg_fail(ctx, KG_ERR_CAPACITY, "table capacity %lld rows exceeded", (long long)ht->capacity);
-  if ((st = choose_form(ctx, ht, key, d_rows, nrows)) != KG_OK) return st;
-  const BuildLayout L = ht->form >= 0 ? build_layout(ctx, ht, nrows) : BuildLayout{};
-  if (ht->narrow_unfilled) {
-    ht->narrow_unfilled = false;
-    if (!(L.P > 0 && inserted == 0)) {
-      const int64_t words = static_cast<int64_t>(ht->nbuckets) * kSlots / 2;  // 32-bit slots / 2
-      kg_time_begin(ctx);
-      hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx, words, 4)), dim3(kJBlock), 0, ctx->stream,
-                         reinterpret_cast<unsigned long long*>(ht->keys), ~0ull, words);
-      kg_time_end(ctx);
-      KG_LAUNCH_CHECK(ctx);
-    }
-  }
-  if (L.P > 0) {
-    if ((st = region_build(ctx, ht, *key, d_rows, nrows, L, inserted == 0)) != KG_OK) return st;
-  } else {
-    kg_time_begin(ctx);
-    hipLaunchKernelGGL(k_ht_build, dim3(grid_for(ctx, nrows, 4)), dim3(kJBlock), 0, ctx->stream, ht->view(), *key,
-                       d_rows, nrows, ht->counters, reinterpret_cast<int*>(ht->counters + 2));
-    kg_time_end(ctx);
-    KG_LAUNCH_CHECK(ctx);
-  }
+  st = use_part_build(ctx, ht, nrows) ? part_build(ctx, ht, key, d_rows, nrows, inserted == 0)
+                                      : direct_build(ctx, ht, key, d_rows, nrows);
+  if (st != KG_OK) return st;
   unsigned long long overflow = 0;
   if ((st = read_counter(ctx, ht->counters + 2, &overflow)) != KG_OK) return st;
   if (overflow) return kg_fail(ctx, KG_ERR_CAPACITY, "join table overflow (side list or buckets full)");
@@ -2734,24 +3091,30 @@ uint64_t tuning_class(int64_t nbuild, std::vector<int64_t> b, std::vector<int64_
 
 // Mean time of `reps` probe_count calls (after one warm call) in ms.
 int time_probe(kg_ctx* ctx, const kg_hash_table* ht, const kg_col* key, int64_t n, int reps, double* ms) {
+  // Kernel time (the sum of one call's timed spans, not the host round trips
+  // around it), the fastest of `reps` calls after one untimed call.
   int64_t cnt = 0;
   int st = kg_ht_probe_count(ctx, ht, key, nullptr, n, &cnt);
   if (st != KG_OK) return st;
-  hipEvent_t a, b;
-  KG_HIP_CHECK(ctx, hipEventCreate(&a));
-  KG_HIP_CHECK(ctx, hipEventCreate(&b));
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> saved;
+  saved.swap(ctx->ev_pending);
   const int timing = ctx->timing;
-  ctx->timing = 0;
-  (void)hipEventRecord(a, ctx->stream);
-  for (int i = 0; i < reps && st == KG_OK; i++) st = kg_ht_probe_count(ctx, ht, key, nullptr, n, &cnt);
-  (void)hipEventRecord(b, ctx->stream);
-  (void)hipEventSynchronize(b);
-  float t = 0;
-  (void)hipEventElapsedTime(&t, a, b);
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  ctx->timing = 1;
+  double best = 1e30;
+  for (int i = 0; i < reps && st == KG_OK; i++) {
+    st = kg_ht_probe_count(ctx, ht, key, nullptr, n, &cnt);
+    double total = 0;
+    int64_t spans = 0;
+    if (st == KG_OK) st = kg_ctx_kernel_stats(ctx, &total, &spans, 1);
+    if (st == KG_OK) best = std::min(best, total);
+  }
+  for (auto& pr : ctx->ev_pending) {  // left over only when a call failed
+    ctx->ev_pool.push_back(pr.first);
+    ctx->ev_pool.push_back(pr.second);
+  }
+  ctx->ev_pending.swap(saved);
   ctx->timing = timing;
-  *ms = t / reps;
+  *ms = best;
   return st;
 }
 
@@ -2805,7 +3168,7 @@ extern "C" int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild
       return KG_OK;
     }
   const int64_t nb = std::min<int64_t>(nbuild, int64_t{1} << 26);
-  const int64_t np = std::min<int64_t>(nprobe, int64_t{1} << 25);
+  const int64_t np = std::min<int64_t>(nprobe, int64_t{1} << 27);
   double best_lf = 0.5, best_ms = 1e30;
   int best_wg = 32, best_regions = 0, trials = 0;
   kg_hash_table* keep = nullptr;
@@ -2832,7 +3195,7 @@ extern "C" int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild
     if ((st = kg_ht_create(ctx, std::max<int64_t>(nb, 1), lf, &ht)) != KG_OK) return restore(st);
     double bms = 0, ms = 0;
     st = time_build(ctx, ht, build_key, nb, &bms);
-    if (st == KG_OK) st = time_probe(ctx, ht, probe_key, np, 2, &ms);
+    if (st == KG_OK) st = time_probe(ctx, ht, probe_key, np, 3, &ms);
     if (st != KG_OK) {
       kg_ht_free(ctx, ht);
       return restore(st);
@@ -2854,7 +3217,7 @@ extern "C" int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild
   for (int wg : {16, 8}) {
     ctx->probe_wg_per_cu = wg;
     double ms = 0;
-    if ((st = time_probe(ctx, keep, probe_key, np, 2, &ms)) != KG_OK) return restore(st);
+    if ((st = time_probe(ctx, keep, probe_key, np, 3, &ms)) != KG_OK) return restore(st);
     trials++;
     if (ms < best_ms * kMargin) {
       best_ms = ms;
@@ -2865,7 +3228,7 @@ extern "C" int kg_join_tune(kg_ctx* ctx, const kg_col* build_key, int64_t nbuild
   for (int reg : {64, 128}) {
     ctx->probe_regions = reg;
     double ms = 0;
-    if ((st = time_probe(ctx, keep, probe_key, np, 2, &ms)) != KG_OK) return restore(st);
+    if ((st = time_probe(ctx, keep, probe_key, np, 3, &ms)) != KG_OK) return restore(st);
     trials++;
     if (ms < best_ms * kMargin) {
       best_ms = ms;
