@@ -54,12 +54,13 @@ constexpr int kBRegG = 8;             // register path: at most this many groups
 constexpr int kBLdsWords = 2048;      // LDS path: 8-byte words of the item table
 constexpr int kBFastPreds = 4;
 constexpr int kBMaxAggs = 16;
-constexpr int kBMaxFills = 6 * kBMaxQueries + 8;
+constexpr int kBMaxFills = 8 * kBMaxQueries + 8;
 constexpr int kBMapMax = 1 << 21;     // direct-map joins: at most this many key offsets
+constexpr int kBBigMapMax = 1 << 25;  // map-only joins (build sides beyond kBMapMax rows): map entries
 constexpr uint32_t kMapEmpty = 0xFFFFFFFFu;
 
-// kFlagNoMap is internal (the direct map of a join cannot be used) and is not
-// reported.
+// kFlagNoMap (the direct map of a join cannot be used) is reported only for a
+// map-only build source, as KG_ERR_UNSUPPORTED.
 enum BFlag : uint32_t { kFlagEmptyKey = 1u, kFlagBadCode = 2u, kFlagOverflow = 4u, kFlagNoMap = 8u };
 
 // Register-path view of a scan: 4-byte non-NULL predicate, key and INT
@@ -119,6 +120,19 @@ struct BQuery {
   uint32_t* map;
   int64_t* krange;
   int32_t msize;
+  // Build source: typed joins over the same build key column, row count and
+  // group codes in one batch share one map (and table) built over the build
+  // rows that pass any of their build predicates (build_all, on the source's
+  // first query); each then re-checks its own
+  // build predicates at a match (recheck): on the probe key when every one is
+  // a conjunct on the build key column (kp_*), else on the build row. A build
+  // side beyond kBMapMax rows has the map only (map_only): if it cannot be used
+  // the query reports KG_ERR_UNSUPPORTED. sflag: the source's build-phase flags.
+  uint32_t* sflag;
+  int32_t map_only, build_all, recheck, nkp, kp_other;
+  int32_t bmem_lo, bmem_hi;   // build_all: the source's queries, BHeader::bmem[bmem_lo .. bmem_hi)
+  int32_t kp_op[kBFastPreds];
+  int64_t kp_lit[kBFastPreds];
   TableView t;          // JOIN_AGG table (slots in the arena)
   int64_t bn;           // JOIN_AGG build rows
   PredSet PB;           // build side predicates
@@ -170,6 +184,7 @@ struct BHeader {
   int32_t nq, nfills, nbq, pad;
   int32_t bpre[kBMaxQueries + 1];             // build phase: item prefix over bq entries
   int32_t bq[kBMaxQueries];                   // build phase: query of entry j
+  int32_t bmem[kBMaxQueries];                 // queries of each shared build source
   int32_t nrq;                                // key-range phase (direct-map joins only):
   int32_t rpre[kBMaxQueries + 1];             //   item prefix over rq entries
   int32_t rq[kBMaxQueries];                   //   query of entry j
@@ -223,6 +238,15 @@ __device__ __forceinline__ uint64_t coherent_load(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Whether build row r goes into query d's map / table: its build predicates,
+// or for a shared source (build_all) those of any of the source's queries.
+__device__ __forceinline__ bool build_row_passes(const BHeader* h, const BQuery* qs, const BQuery& d, int64_t r) {
+  if (!d.build_all) return eval_preds(d.PB, r);
+  for (int m = d.bmem_lo; m < d.bmem_hi; m++)
+    if (eval_preds(qs[h->bmem[m]].PB, r)) return true;
+  return false;
+}
+
 // Key range of a direct-map join: min and max of the qualifying build keys,
 // over the build items (before the build phase). Items are dealt by a grid
 // stride (no shared counter); each workgroup reduces its item's range and
@@ -240,7 +264,7 @@ __global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQu
     const int64_t hi = lo + kBBuildTile < d.bn ? lo + kBBuildTile : d.bn;
     int64_t nmin = INT64_MIN, mx = INT64_MIN;  // max of -key, max of key
     for (int64_t r = lo + threadIdx.x; r < hi; r += kBThreads) {
-      if (!eval_preds(d.PB, r)) continue;
+      if (!build_row_passes(h, qs, d, r)) continue;
       int64_t k;
       if (!load_key(d.bkey, r, k) || k == kEmpty) continue;
       nmin = -k > nmin ? -k : nmin;
@@ -279,7 +303,7 @@ __device__ __forceinline__ bool map_usable(const BQuery& d, int64_t* mn, int64_t
   if (!d.map) return false;
   const int64_t nmin = static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[0])));
   const int64_t mx = static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[1])));
-  const uint32_t fl = __hip_atomic_load(d.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t fl = __hip_atomic_load(d.sflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (fl & kFlagNoMap) return false;
   if (mx == INT64_MIN) {  // no qualifying build row: every probe misses
     *mn = 0;
@@ -306,24 +330,29 @@ __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQu
     const TableView t = d.t;
     int64_t mn = 0, span = 0;
     const bool use_map = map_usable(d, &mn, &span);
+    if (d.map_only && !use_map) {  // no table to fall back to: the queries report it
+      if (threadIdx.x == 0) atomicOr(d.sflag, kFlagNoMap);
+      continue;
+    }
     for (int64_t r = lo + threadIdx.x; r < hi; r += kBThreads) {
-      if (!eval_preds(d.PB, r)) continue;
+      if (!build_row_passes(h, qs, d, r)) continue;
       int64_t k;
       if (!load_key(d.bkey, r, k)) continue;  // NULL build keys are skipped
       if (k == kEmpty) {
-        atomicOr(d.flag, kFlagEmptyKey);
+        atomicOr(d.sflag, kFlagEmptyKey);
         continue;
       }
       if (use_map) {
         const uint64_t off = static_cast<uint64_t>(k) - static_cast<uint64_t>(mn);
         const int g = d.codes ? d.codes[r] : 0;
         if (off >= static_cast<uint64_t>(span) || g < 0 || g >= d.G || r >= (int64_t{1} << 24)) {
-          atomicOr(d.flag, kFlagNoMap);
+          atomicOr(d.sflag, kFlagNoMap);
         } else {
           const uint32_t e = (static_cast<uint32_t>(g) << 24) | static_cast<uint32_t>(r);
-          if (atomicCAS(&d.map[off], kMapEmpty, e) != kMapEmpty) atomicOr(d.flag, kFlagNoMap);
+          if (atomicCAS(&d.map[off], kMapEmpty, e) != kMapEmpty) atomicOr(d.sflag, kFlagNoMap);
         }
       }
+      if (d.map_only) continue;
       uint64_t b = bucket_of(k, t.nbuckets);
       bool done = false;
       for (uint64_t probes = 0; probes < t.nbuckets && !done; probes++) {
@@ -339,9 +368,21 @@ __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQu
         }
         b = (b + 1 == t.nbuckets) ? 0 : b + 1;
       }
-      if (!done) atomicOr(d.flag, kFlagOverflow);
+      if (!done) atomicOr(d.sflag, kFlagOverflow);
     }
   }
+}
+
+// Whether build row brow (key k, equal to the probe key) passes query d's own
+// build predicates, for a query whose map or table holds its source's rows.
+__device__ __forceinline__ bool build_recheck(const BQuery& d, int64_t k, int32_t brow) {
+  if (!d.recheck) return true;
+  if (d.kp_other) return eval_preds(d.PB, brow);
+  bool ok = true;
+#pragma unroll
+  for (int p = 0; p < kBFastPreds; p++)
+    if (p < d.nkp) ok = ok && cmp_i64(d.kp_op[p], k, d.kp_lit[p]);
+  return ok;
 }
 
 // Calls f(brow) for every build row whose key equals k (bucket order).
@@ -1039,7 +1080,7 @@ __device__ __forceinline__ void ld4_jmeasure(int kind, int fkind, const void* co
   }
 }
 
-template <bool FULL>
+template <bool FULL, bool ONEG>
 __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int64_t hi, int slot, uint64_t* lds) {
   const BJoinFast& J = d.J;
   const int G = d.G, nm = d.nm;
@@ -1048,9 +1089,15 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
   uint64_t* lrows = lsum + G * nm;
   uint64_t* lfirst = lrows + G;
   const int words = G * nm + G * 2;
-  for (int i = lane; i < words; i += kWave) lds[i] = i < G * nm + G ? 0ull : ~0ull;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
+  if (!ONEG) {
+    for (int i = lane; i < words; i += kWave) lds[i] = i < G * nm + G ? 0ull : ~0ull;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  // One group (an ungrouped join): the lane's row count, sums and first output
+  // row stay in registers until the item ends (every match would otherwise
+  // add to the same LDS words).
+  uint64_t rc = 0, rs0 = 0, rs1 = 0, rfirst = ~0ull;
   const TableView& t = d.t;
   int64_t mn = 0, span = 0;
   const bool use_map = map_usable(d, &mn, &span);
@@ -1090,6 +1137,14 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
     for (int e = 0; e < 4; e++) pass[e] = pass[e] && key[e] != kEmpty;
     // Adds probe row `row` matched with build row brow of group g.
     auto consume_g = [&](int64_t row, int32_t brow, int g, uint64_t a0, uint64_t a1) {
+      if (ONEG) {
+        rc++;
+        const uint64_t fr = (static_cast<uint64_t>(row) << 32) | static_cast<uint32_t>(brow);
+        rfirst = fr < rfirst ? fr : rfirst;
+        if (nm > 0) rs0 = real0 ? fadd_bits(rs0, a0) : rs0 + a0;
+        if (nm > 1) rs1 = real1 ? fadd_bits(rs1, a1) : rs1 + a1;
+        return;
+      }
       atomicAdd(reinterpret_cast<unsigned long long*>(&lrows[g]), 1ull);
       if (g >= 64 || !((seen >> g) & 1ull)) {
         atomicMin(reinterpret_cast<unsigned long long*>(&lfirst[g]),
@@ -1115,10 +1170,11 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
       }
 #pragma unroll
       for (int e = 0; e < 4; e++)
-        if (me[e] != kMapEmpty)
+        if (me[e] != kMapEmpty && build_recheck(d, key[e], static_cast<int32_t>(me[e] & 0xFFFFFFu)))
           consume_g(base + e, static_cast<int32_t>(me[e] & 0xFFFFFFu), static_cast<int>(me[e] >> 24), v0[e], v1[e]);
       continue;
     }
+    if (d.map_only) continue;  // the map is off and there is no table: the query reports it
     // First buckets of the passing rows, loaded together.
     uint64_t b[4];
     longlong2 kv[4][kSlots / 2];
@@ -1135,6 +1191,7 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
       if (!pass[e]) continue;
       const int64_t row = base + e;
       auto consume = [&](int32_t brow) {
+        if (!build_recheck(d, key[e], brow)) return;
         const int g = d.codes ? d.codes[brow] : 0;
         if (static_cast<unsigned>(g) >= static_cast<unsigned>(G)) {
           atomicOr(d.flag, kFlagBadCode);
@@ -1173,6 +1230,27 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
       }
     }
   }
+  if (ONEG) {
+    for (int st = kWave / 2; st >= 1; st >>= 1) {
+      rc += shfl_xor_u64(rc, st);
+      const uint64_t of = shfl_xor_u64(rfirst, st);
+      rfirst = of < rfirst ? of : rfirst;
+    }
+    if (nm > 0) rs0 = real0 ? wave_sum<1>(rs0) : wave_sum<2>(rs0);
+    if (nm > 1) rs1 = real1 ? wave_sum<1>(rs1) : wave_sum<2>(rs1);
+    if (lane == 0 && rc) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&d.rows[slot]), rc);
+      atomicMin(reinterpret_cast<unsigned long long*>(&d.first[slot]), rfirst);
+      for (int m = 0; m < nm; m++) {
+        const int64_t o = static_cast<int64_t>(slot) * nm + m;
+        const uint64_t v = m == 0 ? rs0 : rs1;
+        atomicAdd(reinterpret_cast<unsigned long long*>(&d.nn[o]), rc);
+        if (d.M.is_real[m]) atomicAdd(reinterpret_cast<double*>(&d.sum[o]), __longlong_as_double(v));
+        else atomicAdd(reinterpret_cast<unsigned long long*>(&d.sum[o]), v);
+      }
+    }
+    return;
+  }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
   for (int i = lane; i < G; i += kWave) {
@@ -1204,7 +1282,10 @@ __device__ void fold_query(const BQuery& d) {
   uint64_t* rows = reinterpret_cast<uint64_t*>(d.res + L.rows);
   uint64_t* first = reinterpret_cast<uint64_t*>(d.res + L.first);
   int8_t* fnull = reinterpret_cast<int8_t*>(d.res + L.fnull);
-  if (lane == 0) *reinterpret_cast<uint32_t*>(d.res) = __hip_atomic_load(d.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0)
+    *reinterpret_cast<uint32_t*>(d.res) =
+        __hip_atomic_load(d.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+        (d.sflag ? __hip_atomic_load(d.sflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u);
   for (int i = lane; i < G * nm; i += kWave) {
     const bool real = d.M.is_real[i % nm];
     uint64_t s = 0, c = 0;
@@ -1472,8 +1553,13 @@ __global__ __launch_bounds__(kBThreads) void k_batch_join(BHeader* h, const BQue
   const int slot = acc_slot();
   uint64_t* mine = lds + (threadIdx.x / kWave) * kBWaveWords;
   drain_wave(h->join, qs, [&](const BQuery& d, int64_t lo, int64_t hi) {
-    if (hi - lo == kBTile) item_join_fast<true>(d, lo, hi, slot, mine);
-    else item_join_fast<false>(d, lo, hi, slot, mine);
+    if (d.G == 1) {
+      if (hi - lo == kBTile) item_join_fast<true, true>(d, lo, hi, slot, mine);
+      else item_join_fast<false, true>(d, lo, hi, slot, mine);
+    } else {
+      if (hi - lo == kBTile) item_join_fast<true, false>(d, lo, hi, slot, mine);
+      else item_join_fast<false, false>(d, lo, hi, slot, mine);
+    }
   });
 }
 
@@ -1512,8 +1598,9 @@ struct BPlan {
   std::vector<AggPlan> plan;
   size_t res_off = 0;
   ResLayout L{};
-  int64_t table = -1;  // index of the shared table entry
   bool join = false;
+  bool map_only = false;     // its build source has the map only
+  bool build_entry = true;   // it adds the build phase's items of its source
   int64_t bn = 0;
 };
 
@@ -1562,20 +1649,70 @@ int32_t map_entries(bool typed_join, int64_t bn, int64_t G) {
   return m;
 }
 
-size_t query_footprint(const kg_query& q) {
+// Map entries of a shared build source of bn rows (0: none): twice the rows,
+// rounded up to a power of two, at most kBMapMax for build sides of at most
+// kBMapMax rows (which also get a table) and kBBigMapMax beyond.
+int32_t source_map_entries(int64_t bn, int64_t G) {
+  if (bn <= 0 || G > 254 || bn > (int64_t{1} << 24)) return 0;  // map entries hold 24-bit build rows
+  const int64_t cap = bn > kBMapMax ? kBBigMapMax : kBMapMax;
+  int64_t m = 2;
+  while (m < 2 * bn && m < cap) m *= 2;
+  return m >= bn ? static_cast<int32_t>(m) : 0;
+}
+
+// Bytes of a query's own result area and accumulators in a batch.
+size_t query_own_bytes(const kg_query& q) {
   const int64_t G = std::min<int64_t>(query_groups(q), kMaxLdsGroups);
   const int nm = std::min(std::max(q.naggs, 1), kMaxMeasures);
   size_t b = res_layout(static_cast<int>(G), nm, std::min(q.naggs, kBMaxAggs)).bytes;
   b += round256(static_cast<size_t>(kBSlots) * G * (2 * nm + 1) * 8 + 16);
   b += round256(static_cast<size_t>(kBSlots) * G * 8);
-  if (q.kind == KG_Q_JOIN_AGG) {
-    const uint64_t nb = table_buckets(std::max<int64_t>(q.build_n, 0), query_lf(q));
-    b += round256(nb * kSlots * 8) + round256(nb * kSlots * 4);
-    const int32_t m = map_entries(true, q.build_n, G);
-    if (m) b += round256(static_cast<size_t>(m) * 4) + 256;
-  }
   return b;
 }
+
+// Bytes of a join's build structures: map-only (build sides beyond kBMapMax
+// rows that a map can take), or table plus the map of a small build side.
+size_t join_table_bytes(const kg_query& q) {
+  if (q.kind != KG_Q_JOIN_AGG) return 0;
+  const int64_t G = std::min<int64_t>(query_groups(q), kMaxLdsGroups);
+  const int64_t bn = std::max<int64_t>(q.build_n, 0);
+  const int32_t sm = source_map_entries(bn, G);
+  if (bn > kBMapMax && sm) return round256(static_cast<size_t>(sm) * 4) + 512;
+  const uint64_t nb = table_buckets(bn, query_lf(q));
+  size_t b = round256(nb * kSlots * 8) + round256(nb * kSlots * 4) + 256;
+  const int32_t m = map_entries(true, bn, G);
+  if (m) b += round256(static_cast<size_t>(m) * 4) + 256;
+  return b;
+}
+
+size_t query_footprint(const kg_query& q) { return query_own_bytes(q) + join_table_bytes(q); }
+
+}  // namespace
+
+namespace kg {
+
+size_t batch_own_bytes(const kg_query& q) { return query_own_bytes(q); }
+
+size_t batch_source_bytes(const kg_query& q) { return join_table_bytes(q); }
+
+bool batch_source_id(const kg_query& q, BatchSource* id) {
+  if (q.kind != KG_Q_JOIN_AGG || !q.build_cols || q.build_key_col < 0 || q.build_key_col >= q.build_ncols) return false;
+  const int64_t G = query_groups(q);
+  if (!source_map_entries(q.build_n, G)) return false;
+  const kg_col& k = q.build_cols[q.build_key_col];
+  id->key = k.data;
+  id->nulls = k.nulls;
+  id->codes = q.build_code_col >= 0 && q.build_code_col < q.build_ncols ? q.build_cols[q.build_code_col].data
+                                                                       : nullptr;
+  id->bn = q.build_n;
+  id->type = k.type;
+  id->G = static_cast<int32_t>(G);
+  return true;
+}
+
+}  // namespace kg
+
+namespace {
 
 }  // namespace
 
@@ -2092,9 +2229,54 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     ok.push_back(i);
   }
   b->res_used = off;
+  // Build sources: typed joins that a direct map may serve, by build key
+  // column, build rows and group codes. A source with several queries, or with
+  // a build side beyond kBMapMax rows, is built once for the batch.
+  struct Source {
+    const void* key;
+    int32_t type;
+    const int8_t* nulls;
+    int64_t bn;
+    const int32_t* codes;
+    int32_t G;
+    double lf;
+    std::vector<int32_t> members;
+    int32_t msize = 0;
+    bool map_only = false, ok = true;
+  };
+  std::vector<Source> srcs;
+  std::vector<int> src_of(nq, -1);
+  for (int i : ok) {
+    const BPlan& P = b->plans[i];
+    const BQuery& D = Q[i];
+    if (!P.join || !D.jfast || !source_map_entries(P.bn, P.G)) continue;
+    auto it = std::find_if(srcs.begin(), srcs.end(), [&](const Source& x) {
+      return x.key == D.bkey.data && x.type == D.bkey.type && x.nulls == D.bkey.nulls && x.bn == P.bn &&
+             x.codes == D.codes && x.G == P.G;
+    });
+    if (it == srcs.end()) {
+      srcs.push_back(Source{D.bkey.data, D.bkey.type, D.bkey.nulls, P.bn, D.codes, P.G, query_lf(qs[i]), {}});
+      it = srcs.end() - 1;
+    }
+    it->members.push_back(i);
+    src_of[i] = static_cast<int>(it - srcs.begin());
+  }
+  for (size_t si = 0; si < srcs.size(); si++) {
+    Source& S = srcs[si];
+    if (S.members.size() < 2 && S.bn <= kBMapMax) {  // one query, small build side: its own map and table
+      src_of[S.members[0]] = -1;
+      S.members.clear();
+      continue;
+    }
+    S.map_only = S.bn > kBMapMax;
+    S.msize = source_map_entries(S.bn, S.G);
+  }
   off = 0;
   int nf = 0;
   std::vector<int> rem_fill(nq, -1);  // fill entry of each query's remaining-item count
+  // Arena: each query's accumulators (and the table of a join that is not in
+  // a shared source), then the shared sources. A query or source that does not
+  // fit is left to the plain path (KG_ERR_UNSUPPORTED).
   for (int i : ok) {
     BPlan& P = b->plans[i];
     BQuery& D = Q[i];
@@ -2103,14 +2285,15 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     const size_t first_bytes = round256(static_cast<size_t>(kBSlots) * G * 8);
     size_t need = acc_bytes + first_bytes;
     uint64_t nbuckets = 0;
-    const int32_t msize = map_entries(P.join && D.jfast, P.bn, G);
-    if (P.join) {
+    const bool own_table = P.join && src_of[i] < 0;
+    const int32_t msize = own_table ? map_entries(D.jfast != 0, P.bn, G) : 0;
+    if (own_table) {
       nbuckets = table_buckets(P.bn, query_lf(qs[i]));
       need += round256(nbuckets * kSlots * 8) + round256(nbuckets * kSlots * 4);
       if (msize) need += round256(static_cast<size_t>(msize) * 4) + 256;
     }
     if (off + need > b->arena_bytes) {
-      P.status = KG_ERR_CAPACITY;
+      P.status = KG_ERR_UNSUPPORTED;
       P.err = "the batch arena cannot hold this query's tables";
       continue;
     }
@@ -2120,6 +2303,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     D.rows = D.nn + static_cast<size_t>(kBSlots) * G * std::max(nm, 1);
     D.flag = reinterpret_cast<uint32_t*>(D.rows + static_cast<size_t>(kBSlots) * G);
     D.rem = reinterpret_cast<unsigned long long*>(D.rows + static_cast<size_t>(kBSlots) * G + 1);
+    D.sflag = P.join ? D.flag : nullptr;
     // Fill ranges must not overlap: the fill kernel writes them in no fixed order.
     H->fills[nf++] = BFill{D.sum, static_cast<int64_t>(kBSlots) * G * (2 * std::max(nm, 1) + 1) + 1, 0};
     D.items = static_cast<int32_t>(ceil_div(qs[i].n, kBTile));  // shared scans: set by layout_groups
@@ -2127,7 +2311,10 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.rem), 1, static_cast<uint64_t>(D.items)};
     D.first = reinterpret_cast<uint64_t*>(base + acc_bytes);
     H->fills[nf++] = BFill{D.first, static_cast<int64_t>(kBSlots) * G, ~0ull};
-    if (P.join) {
+    D.map = nullptr;
+    D.krange = nullptr;
+    D.msize = 0;
+    if (own_table) {
       char* tb = base + acc_bytes + first_bytes;
       D.t.keys = reinterpret_cast<int64_t*>(tb);
       D.t.rows = reinterpret_cast<int32_t*>(tb + round256(nbuckets * kSlots * 8));
@@ -2136,9 +2323,6 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
       D.t.side_count = nullptr;
       H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.t.keys), static_cast<int64_t>(nbuckets * kSlots),
                              static_cast<uint64_t>(kEmpty)};
-      D.map = nullptr;
-      D.krange = nullptr;
-      D.msize = 0;
       if (msize) {
         char* mb = tb + round256(nbuckets * kSlots * 8) + round256(nbuckets * kSlots * 4);
         D.map = reinterpret_cast<uint32_t*>(mb);
@@ -2153,6 +2337,79 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     D.seq = b->seq;
     off += need;
   }
+  int32_t nbmem = 0;
+  for (Source& S : srcs) {
+    if (S.members.empty()) continue;
+    // flag word, key range, map, then (small build sides) the table
+    const uint64_t nbuckets = S.map_only ? 0 : table_buckets(S.bn, S.lf);
+    const size_t map_b = round256(static_cast<size_t>(S.msize) * 4);
+    const size_t keys_b = round256(nbuckets * kSlots * 8), rows_b = round256(nbuckets * kSlots * 4);
+    const size_t need = 256 + map_b + keys_b + rows_b;
+    bool live = false;
+    for (int i : S.members) live |= b->plans[i].status == KG_OK;
+    if (!live) continue;
+    if (off + need > b->arena_bytes) {
+      for (int i : S.members)
+        if (b->plans[i].status == KG_OK) {
+          b->plans[i].status = KG_ERR_UNSUPPORTED;
+          b->plans[i].err = "the batch arena cannot hold this query's tables";
+        }
+      continue;
+    }
+    char* base = b->arena + off;
+    uint32_t* sflag = reinterpret_cast<uint32_t*>(base);
+    int64_t* krange = reinterpret_cast<int64_t*>(base + 64);
+    uint32_t* map = reinterpret_cast<uint32_t*>(base + 256);
+    H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(sflag), 1, 0};
+    H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(krange), 2, static_cast<uint64_t>(INT64_MIN)};
+    H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(map), S.msize / 2, ~0ull};
+    TableView tv{};
+    if (!S.map_only) {
+      tv.keys = reinterpret_cast<int64_t*>(base + 256 + map_b);
+      tv.rows = reinterpret_cast<int32_t*>(base + 256 + map_b + keys_b);
+      tv.nbuckets = nbuckets;
+      H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(tv.keys), static_cast<int64_t>(nbuckets * kSlots),
+                             static_cast<uint64_t>(kEmpty)};
+    }
+    off += need;
+    const bool several = S.members.size() > 1;
+    bool leader = true;
+    const int32_t blo = nbmem;
+    for (int i : S.members)
+      if (b->plans[i].status == KG_OK) H->bmem[nbmem++] = i;
+    for (int i : S.members) {
+      BPlan& P = b->plans[i];
+      if (P.status != KG_OK) continue;
+      BQuery& D = Q[i];
+      P.map_only = S.map_only;
+      D.map = map;
+      D.krange = krange;
+      D.msize = S.msize;
+      D.t = tv;
+      D.sflag = sflag;
+      D.map_only = S.map_only ? 1 : 0;
+      D.build_all = (leader && several) ? 1 : 0;  // the leader builds the source (the others add no build items)
+      D.bmem_lo = blo;
+      D.bmem_hi = nbmem;
+      D.recheck = several ? 1 : 0;
+      P.build_entry = leader;
+      leader = false;
+      // Build predicates that are conjuncts on the build key column are
+      // checked on the (equal) probe key; any other form on the build row.
+      D.nkp = 0;
+      D.kp_other = 0;
+      if (D.PB.n > 0 && (!D.PB.conj || D.PB.n > kBFastPreds)) D.kp_other = 1;
+      for (int p = 0; p < D.PB.n && !D.kp_other; p++) {
+        if (D.PB.data[p] == D.bkey.data && D.PB.type[p] == D.bkey.type && D.PB.type[p] != KG_REAL) {
+          D.kp_op[D.nkp] = D.PB.op[p];
+          D.kp_lit[D.nkp] = D.PB.ilit[p];
+          D.nkp++;
+        } else {
+          D.kp_other = 1;
+        }
+      }
+    }
+  }
   H->nfills = nf;
   // Build phase entries; tables (queries sharing table_id and n) for the scan
   // phase, in order of first appearance.
@@ -2163,7 +2420,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   int nbq = 0;
   H->bpre[0] = 0;
   for (int i : live) {
-    if (!b->plans[i].join || b->plans[i].bn == 0) continue;
+    if (!b->plans[i].join || b->plans[i].bn == 0 || !b->plans[i].build_entry) continue;
     H->bq[nbq] = i;
     H->bpre[nbq + 1] = H->bpre[nbq] + static_cast<int32_t>(ceil_div(b->plans[i].bn, kBBuildTile));
     nbq++;
@@ -2237,7 +2494,7 @@ void emit_query(kg_batch* b, int i, kg_query_result& R) {
   if (P.status != KG_OK) return;
   const char* base = b->h_res + P.res_off;
   const uint32_t flag = *reinterpret_cast<const volatile uint32_t*>(base);
-  if (flag & kFlagEmptyKey) {
+  if ((flag & kFlagEmptyKey) || ((flag & kFlagNoMap) && P.map_only)) {
     R.status = KG_ERR_UNSUPPORTED;
     return;
   }

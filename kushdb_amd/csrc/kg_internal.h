@@ -142,6 +142,24 @@ int ht_reset(kg_ctx* ctx, kg_hash_table* t, int64_t capacity_rows, double load_f
 // build_rows rows (same power-of-two class, unique keys preferred), or 0.5.
 double tuned_load_factor(const kg_ctx* ctx, int64_t build_rows);
 
+// Batch footprint parts (kg_batch.hip), for admission: a query's own result
+// area and accumulators, and its build structures, which the queries of one
+// batch that share a build source (same build key column, rows and group
+// codes; see kg_batch_submit) hold once.
+struct BatchSource {
+  const void* key;
+  const int8_t* nulls;
+  const void* codes;
+  int64_t bn;
+  int32_t type, G;
+  bool operator==(const BatchSource& o) const {
+    return key == o.key && nulls == o.nulls && codes == o.codes && bn == o.bn && type == o.type && G == o.G;
+  }
+};
+size_t batch_own_bytes(const kg_query& q);
+size_t batch_source_bytes(const kg_query& q);
+bool batch_source_id(const kg_query& q, BatchSource* id);
+
 // The static device scratch of the no-allocation entry points: tile counts
 // (int32) and tile offsets (int64) of kg_scan_select_noalloc for up to
 // kNoAllocMaxTiles tiles of 4096 rows, then the selected-row total.

@@ -235,6 +235,7 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
         s.qbuf.clear();
         const int64_t arena = kg_batch_arena_bytes(s.b);
         int64_t used = 0, rows = 0;
+        std::vector<kg::BatchSource> sources;  // build sources already admitted to this batch
         while (!pending.empty() && static_cast<int>(s.ids.size()) < max_batch) {
           const int32_t i = pending.front();
           int64_t fp = 0;
@@ -246,7 +247,13 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
             plain(i);
             continue;
           }
+          // A build source already in the batch is not counted again.
+          kg::BatchSource src{};
+          const bool has_src = kg::batch_source_id(qs[i], &src);
+          if (has_src && std::find(sources.begin(), sources.end(), src) != sources.end())
+            fp = static_cast<int64_t>(kg::batch_own_bytes(qs[i]));
           if (used + fp > arena) break;
+          if (has_src) sources.push_back(src);
           const int64_t qrows = qs[i].n + (qs[i].kind == KG_Q_JOIN_AGG ? qs[i].build_n : 0);
           if (opts->max_batch_rows > 0 && !s.ids.empty() && rows + qrows > opts->max_batch_rows) break;
           rows += qrows;

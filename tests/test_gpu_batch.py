@@ -237,3 +237,48 @@ def test_alone_equals_mix_join_templates(env):
         alone = b.run(_queries(cols, [s]), max_groups=64)[0]
         assert_same(s, alone, (m[1], m[2], m[3]), what="join alone")
     b.free()
+
+
+def test_map_only_build_sources(env):
+    """Joins whose build side exceeds the row limit of a table-backed direct map
+    (here a 3M-key build table) run map-only: one map per build source per
+    batch, built over all build rows when several queries share the source,
+    each query re-checking its own key range on the probe key. Every query
+    against the oracle in a mix with table-backed joins and scans, and alone
+    (the map then holds only its own range); a source whose keys repeat cannot
+    use a map and is reported unsupported, and serving runs such queries on
+    the plain path with the same results."""
+    from kushdb_amd.runtime import Column
+    ctx, _, _, _ = env
+    nbig = 3_000_000
+    tables = serve_tables(SIZES, nbuild=NBUILD, nbig=nbig, device="cuda:0")
+    torch.cuda.synchronize()
+    cols = serve_columns(tables)
+    H = host_tables(tables)
+    specs = serve_queries(80, SIZES, NBUILD, seed=23, templates=["c2_join_16m", "c2_join", "c0_star", "c1_small"],
+                          nbig=nbig)[:24]
+    assert sum(s["template"] == "c2_join_16m" for s in specs) >= 3
+    print("map-only sources: mix of", len(specs), flush=True)
+    b = Batch(ctx, arena_bytes=1 << 30)
+    mix = b.run(_queries(cols, specs), max_groups=64)
+    for s, m in zip(specs, mix):
+        assert_same(s, m, oracle_spec(s, H), what="map-only mix")
+    big = [s for s in specs if s["template"] == "c2_join_16m"]
+    for s in big[:2]:
+        alone = b.run(_queries(cols, [s]), max_groups=64)[0]
+        assert_same(s, alone, oracle_spec(s, H), what="map-only alone")
+    # Repeated build keys: the map is off and there is no table to fall back to.
+    facts, dim, builds = cols
+    dup = dict(builds)
+    dup["big"] = [Column(abi.KG_BIGINT, (tables["big"]["key"] // 2).contiguous()), builds["big"][1]]
+    qd = [make_query(s, facts, dim, dup) for s in big[:2]]
+    got = b.run(qd, max_groups=64)
+    assert all(g[0] == abi.KG_ERR_UNSUPPORTED for g in got), [g[0] for g in got]
+    b.free()
+    Hd = dict(H, big={"key": H["big"]["key"] // 2, "payload": H["big"]["payload"]})
+    arrivals = np.zeros(len(qd))
+    res, _, st = serve(ctx, qd, arrivals, abi.KG_SERVE_BATCHED, slots=1, max_batch=8, arena_bytes=1 << 30,
+                       max_groups=64)
+    assert st["fallbacks"] == len(qd)
+    for s, g in zip(big[:2], res):
+        assert_same(s, g, oracle_spec(s, Hd), what="repeated keys served")

@@ -76,10 +76,15 @@ def test_footprint_counts_join_table():
     table = nbuckets * 8 * 12                    # 8-byte keys + 4-byte row ids per slot
     dmap = (1 << 21) * 4                         # direct map: 2 x rows to a power of two, <= 2^21 entries
     assert table + dmap <= b.value < table + dmap + (1 << 16)
-    q.build_n = 3_000_000                        # beyond the direct map: the table alone
+    q.build_n = 3_000_000                        # beyond a table-backed map: the map alone
     assert lib.kg_batch_footprint(ctypes.byref(q), ctypes.byref(b)) == abi.KG_OK
-    table = -(-6_000_000 // 8) * 8 * 12
-    assert table <= b.value < table + (1 << 16)
+    dmap = (1 << 23) * 4                         # 2 x rows to a power of two, <= 2^25 entries
+    assert dmap <= b.value < dmap + (1 << 16)
+    q.build_n = 20_000_000                       # beyond any map (2^25 entries): the table
+    assert lib.kg_batch_footprint(ctypes.byref(q), ctypes.byref(b)) == abi.KG_OK
+    table = -(-40_000_000 // 8)
+    table = table * 8 * 12
+    assert table <= b.value < table * 1.01 + (1 << 16)
     q.build_n = 0
     q.kind = abi.KG_Q_SCAN_AGG
     assert lib.kg_batch_footprint(ctypes.byref(q), ctypes.byref(b)) == abi.KG_OK
