@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-saturation-only", action="store_true",
                     help="serving: one batched pass of the mix at once (for kernel traces)")
+    ap.add_argument("--serve-rate-only", type=float, default=0.0,
+                    help="serving: one batched pass of the mix at this offered rate, q/s (for kernel traces)")
     ap.add_argument("--serve-caps", type=int, nargs="+", default=[100_000_000, 0],
                     help="batch work caps (rows) tried for the batched mode; 0 = no cap")
     ap.add_argument("--serve-max-batch", type=int, default=128)
@@ -651,6 +653,10 @@ def bench_serving(ctx, a):
         return best, runs
 
     run(abi.KG_SERVE_BATCHED, None)
+    if a.serve_rate_only > 0:  # one batched pass at a given rate (for kernel traces)
+        r = run(abi.KG_SERVE_BATCHED, a.serve_rate_only)
+        log(f"batched at {a.serve_rate_only:.0f} q/s offered: {r}")
+        return {"batched_at_rate": r}
     if a.serve_saturation_only:  # one batched pass of the mix (for kernel traces)
         batched_sat = run(abi.KG_SERVE_BATCHED, None)
         log(f"saturation: batched {batched_sat['qps']:.0f} q/s")

@@ -86,10 +86,21 @@ void kg_dev_release(kg_ctx* ctx, void* p) {
   }
   const size_t bytes = it->second;
   ctx->block_live.erase(it);
-  if (ctx->block_cache_bytes + bytes > kg::kBlockCacheBytes) {
+  if (bytes > kg::kBlockCacheBytes) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(p);
     return;
+  }
+  // Room for the block: the largest cached blocks go back to the device (the
+  // newest release is the likeliest to be asked for again).
+  if (ctx->block_cache_bytes + bytes > kg::kBlockCacheBytes) {
+    (void)hipStreamSynchronize(ctx->stream);
+    while (ctx->block_cache_bytes + bytes > kg::kBlockCacheBytes && !ctx->block_cache.empty()) {
+      auto big = std::prev(ctx->block_cache.end());
+      (void)hipFree(big->second);
+      ctx->block_cache_bytes -= big->first;
+      ctx->block_cache.erase(big);
+    }
   }
   ctx->block_cache.emplace(bytes, p);
   ctx->block_cache_bytes += bytes;
