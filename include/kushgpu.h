@@ -521,6 +521,18 @@ int kg_text_hash(kg_ctx* ctx, const kg_text_col* col, const int32_t* d_rows, int
                  int8_t* d_out_nulls);
 int kg_text_pairs_equal(kg_ctx* ctx, const kg_text_col* a, const int32_t* d_rows_a, const kg_text_col* b,
                         const int32_t* d_rows_b, int64_t n, int32_t* d_out_idx, int64_t* out_count);
+/* Exactness of grouping (or joining) TEXT values by their kg_text_hash codes:
+ * of the n values (rows d_rows[i], or i when d_rows is NULL) with codes
+ * d_hash[i], *out_mismatches receives the number whose string differs from
+ * the string of the smallest row with the same code (NULL values are
+ * skipped). 0 means equal codes imply equal strings on these rows, so a GROUP
+ * BY or join on the codes equals the one on the strings (String::Equals,
+ * runtime/string.cc:42-46); otherwise two strings collided. One pass builds a
+ * device table of codes (2n slots), a second compares each row with its
+ * code's row: O(n). No counterpart in the reference, which compares strings
+ * inside its hash tables (compile/proxy/aggregate_hash_table.cc:187-243). */
+int kg_text_hash_verify(kg_ctx* ctx, const kg_text_col* col, const int32_t* d_rows, const int64_t* d_hash,
+                        int64_t n, int64_t* out_mismatches);
 
 /* ---- index-nested-loop join step (Skinner table functions) ----------------------
  * Extends each partial join tuple d_tuples[i*width .. +width) (tuple ids per

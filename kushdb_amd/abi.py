@@ -200,6 +200,7 @@ SIGNATURES = {
     "kg_text_hash": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, _i64, _vp, _vp]),
     "kg_text_pairs_equal": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, ctypes.POINTER(KgTextCol), _vp, _i64, _vp,
                                    _pi64]),
+    "kg_text_hash_verify": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),
     "kg_tuple_set_extract": (_i32, [_vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_free": (_i32, [_vp, _vp]),

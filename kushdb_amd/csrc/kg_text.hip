@@ -233,6 +233,68 @@ __global__ void k_text_pairs_eq(kg_text_col a, const int32_t* ra, kg_text_col b,
   }
 }
 
+// Exactness check of TEXT hash codes (kg_text_hash_verify): an open-addressing
+// table of codes (kCodeFree = free; a code equal to it is taken as
+// kCodeFree + 1 in both passes) with the smallest row of each code, then every
+// row compared byte for byte with its code's smallest row.
+constexpr int64_t kCodeFree = INT64_MIN;
+
+__device__ __forceinline__ int64_t code_of(const int64_t* hash, int64_t i) {
+  const int64_t h = hash[i];
+  return h == kCodeFree ? kCodeFree + 1 : h;
+}
+
+__global__ void k_code_fill(int64_t* keys, int32_t* reps, int64_t nslots, unsigned long long* count) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < nslots; i += stride) {
+    keys[i] = kCodeFree;
+    reps[i] = INT32_MAX;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = 0;
+}
+
+__global__ void k_code_claim(kg_text_col col, const int32_t* rows, const int64_t* hash, int64_t n, int64_t* keys,
+                             int32_t* reps, uint64_t mask) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t r = rows ? rows[i] : i;
+    if (col.nulls && col.nulls[r]) continue;
+    const int64_t h = code_of(hash, i);
+    uint64_t s = mix64(static_cast<uint64_t>(h)) & mask;
+    for (uint64_t probes = 0; probes <= mask; probes++) {
+      int64_t cur = keys[s];
+      if (cur == kCodeFree)
+        cur = static_cast<int64_t>(atomicCAS(reinterpret_cast<unsigned long long*>(keys + s),
+                                             static_cast<unsigned long long>(kCodeFree),
+                                             static_cast<unsigned long long>(h)));
+      if (cur == kCodeFree || cur == h) {
+        atomicMin(reps + s, static_cast<int32_t>(r));
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+  }
+}
+
+__global__ void k_code_check(kg_text_col col, const int32_t* rows, const int64_t* hash, int64_t n,
+                             const int64_t* keys, const int32_t* reps, uint64_t mask, unsigned long long* count) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t r = rows ? rows[i] : i;
+    if (col.nulls && col.nulls[r]) continue;
+    const int64_t h = code_of(hash, i);
+    uint64_t s = mix64(static_cast<uint64_t>(h)) & mask;
+    for (uint64_t probes = 0; probes <= mask && keys[s] != h; probes++) s = (s + 1) & mask;
+    if (keys[s] != h) continue;  // not reached: every code was claimed by the first pass
+    const int64_t q = reps[s];
+    const int64_t ab = col.offsets[r], al = col.offsets[r + 1] - ab;
+    const int64_t bb = col.offsets[q], bl = col.offsets[q + 1] - bb;
+    bool eq = al == bl;
+    for (int64_t k = 0; eq && k < al; k++) eq = col.bytes[ab + k] == col.bytes[bb + k];
+    if (!eq) atomicAdd(count, 1ull);
+  }
+}
+
 }  // namespace
 
 extern "C" int kg_text_select(kg_ctx* ctx, const kg_text_col* col, int32_t op, const char* pattern, int32_t plen,
@@ -468,5 +530,40 @@ extern "C" int kg_text_pairs_equal(kg_ctx* ctx, const kg_text_col* a, const int3
   KG_HIP_CHECK(ctx, hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
   KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   *out_count = h;
+  return KG_OK;
+}
+
+extern "C" int kg_text_hash_verify(kg_ctx* ctx, const kg_text_col* col, const int32_t* d_rows, const int64_t* d_hash,
+                                   int64_t n, int64_t* out_mismatches) {
+  if (!ctx) return KG_ERR_INVALID;
+  int st = check_text(ctx, col);
+  if (st != KG_OK) return st;
+  KG_REQUIRE(ctx, n >= 0 && n < INT32_MAX, "row count out of range");
+  KG_REQUIRE(ctx, out_mismatches && (n == 0 || d_hash), "null pointer argument");
+  *out_mismatches = 0;
+  if (n == 0) return KG_OK;
+  KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  uint64_t nslots = 64;
+  while (nslots < 2 * static_cast<uint64_t>(n)) nslots <<= 1;
+  char* buf = nullptr;
+  const size_t kb = nslots * 8, rb = (nslots * 4 + 255) / 256 * 256;
+  if ((st = kg_dev_alloc(ctx, reinterpret_cast<void**>(&buf), kb + rb + 256)) != KG_OK) return st;
+  int64_t* keys = reinterpret_cast<int64_t*>(buf);
+  int32_t* reps = reinterpret_cast<int32_t*>(buf + kb);
+  unsigned long long* count = reinterpret_cast<unsigned long long*>(buf + kb + rb);
+  const int grid = tgrid(ctx, static_cast<int64_t>(std::max<uint64_t>(nslots, static_cast<uint64_t>(n))));
+  hipLaunchKernelGGL(k_code_fill, dim3(grid), dim3(kTBlock), 0, ctx->stream, keys, reps,
+                     static_cast<int64_t>(nslots), count);
+  hipLaunchKernelGGL(k_code_claim, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream, *col, d_rows, d_hash, n, keys,
+                     reps, nslots - 1);
+  hipLaunchKernelGGL(k_code_check, dim3(tgrid(ctx, n)), dim3(kTBlock), 0, ctx->stream, *col, d_rows, d_hash, n,
+                     keys, reps, nslots - 1, count);
+  hipError_t e = hipGetLastError();
+  unsigned long long h = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, count, 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  kg_dev_release(ctx, buf);
+  if (e != hipSuccess) return kg_fail(ctx, KG_ERR_HIP, "text code check failed: %s", hipGetErrorString(e));
+  *out_mismatches = static_cast<int64_t>(h);
   return KG_OK;
 }

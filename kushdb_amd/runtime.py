@@ -630,6 +630,18 @@ def text_pairs_equal(ctx, a, rows_a, b, rows_b):
                                           _ptr(rows_b.contiguous()), n, _ptr(out), ctypes.byref(cnt)))
     return out[: cnt.value]
 
+def text_hash_verify(ctx, col, hashes, rows=None):
+    """Number of values (rows, or all) whose string differs from the string of
+    the smallest row with the same hash code (kg_text_hash_verify); 0 means a
+    GROUP BY or join on the codes is exact."""
+    n = hashes.numel()
+    out = ctypes.c_int64()
+    d = col.desc()
+    ctx.check(ctx.lib.kg_text_hash_verify(ctx.h, ctypes.byref(d), _ptr(rows), _ptr(hashes.contiguous()), n,
+                                          ctypes.byref(out)))
+    return out.value
+
+
 def text_minmax(ctx, col, rows=None, is_max=False, cap=4096):
     """(position, bytes) of the first MIN (MAX) non-NULL value, or (-1, None)."""
     n = rows.numel() if rows is not None else len(col)

@@ -29,7 +29,7 @@ DATA = (
        "l_partkey.kdb", "p_partkey.kdb", "p_type.kdb", "p_size.kdb", "p_brand.kdb", "p_container.kdb",
        "l_shipinstruct.kdb", "c_acctbal.kdb", "c_name.kdb", "c_phone.kdb", "c_address.kdb", "c_comment.kdb",
        "l_returnflag.kdb", "ps_partkey.kdb", "ps_suppkey.kdb", "ps_supplycost.kdb", "ps_availqty.kdb",
-       "p_name.kdb", "l_quantity.kdb"]
+       "p_name.kdb", "l_quantity.kdb", "o_totalprice.kdb"]
 )
 
 EXPECTED = (

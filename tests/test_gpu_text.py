@@ -70,3 +70,33 @@ def test_expr_like_text_and_enum(ctx):
         wenc = [w.encode() for w in words]
         hit = np.array([po.text_select(_oracle_col([w]), abi.KG_TEXT_LIKE, pat).size == 1 for w in wenc])
         assert np.array_equal(got.cpu().numpy(), np.nonzero(hit[codes])[0]), pat
+
+
+def test_text_hash_verify(ctx):
+    """kg_text_hash_verify counts the values whose string differs from the
+    string of the smallest row with the same code: 0 for kg_text_hash's own
+    codes (duplicates, NULLs, empty strings, a row subset), and for codes that
+    collide on purpose (the string's length) exactly the count restated from
+    that definition in Python, on all rows and on a subset."""
+    from kushdb_amd.runtime import text_hash, text_hash_verify
+    rng = np.random.default_rng(31)
+    base = _strings(300, 5) + [b"", b"x", "é".encode(), b"a\nb"]
+    strs = [base[int(i)] for i in rng.integers(0, len(base), 20000)]
+    nulls = (rng.random(len(strs)) < 0.05).astype(np.int8)
+    col = TextColumn.from_strings(strs, nulls=nulls)
+    dev = "cuda:0"
+    sub = torch.as_tensor(np.sort(rng.choice(len(strs), 7000, replace=False)).astype(np.int32)).to(dev)
+    for rows in (None, sub):
+        codes, _ = text_hash(ctx, col, rows)
+        assert text_hash_verify(ctx, col, codes, rows) == 0
+        idx = np.arange(len(strs)) if rows is None else sub.cpu().numpy()
+        lens = np.array([len(strs[i]) for i in idx], dtype=np.int64)
+        fake = torch.as_tensor(lens).to(dev)
+        rep = {}
+        for i, ln in zip(idx, lens):
+            if not nulls[i]:
+                rep[ln] = min(rep.get(ln, i), i)
+        want = sum(1 for i, ln in zip(idx, lens) if not nulls[i] and strs[i] != strs[rep[ln]])
+        assert want > 0
+        assert text_hash_verify(ctx, col, fake, rows) == want
+    assert text_hash_verify(ctx, col, torch.empty(0, dtype=torch.int64, device=dev), None) == 0

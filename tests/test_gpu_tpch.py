@@ -82,8 +82,8 @@ def device_q12(ctx):
     three DATE columns (kg_expr_select), the join (kg_ht_build on the selected
     lineitem keys, kg_ht_probe with all orders), the CASE counts over the
     orders' TEXT priority (kg_expr_eval at the joined rows), GROUP BY l_shipmode
-    through its string hash (kg_text_hash codes, kg_hash_group_agg), each
-    group's row count checked against an exact string select so that no two
+    through its string hash (kg_text_hash codes, kg_hash_group_agg), the
+    codes checked exact in one pass (kg_text_hash_verify) so that no two
     strings share a group, ORDER BY l_shipmode (kg_order_by over TEXT)."""
     import tpch_q12
     from kushdb_amd.expr import And, Case, Eq, Geq, Lt, Neq, Or, expr_select
@@ -104,6 +104,7 @@ def device_q12(ctx):
     ht.free()
     lrows = rows_l[bi.long()].contiguous()
     codes, _ = text_hash(ctx, sm, lrows)
+    from kushdb_amd.runtime import text_hash_verify
     urgent = Or([Eq(ColRef(0, T), Literal("1-URGENT")), Eq(ColRef(0, T), Literal("2-HIGH"))])
     high = expr_eval(ctx, len(c["o_orderkey"]), [pri], Case(urgent, Literal(1), Literal(0)), rows=orows.contiguous())
     low = expr_eval(ctx, len(c["o_orderkey"]), [pri],
@@ -113,9 +114,8 @@ def device_q12(ctx):
                                         [(abi.KG_AGG_SUM, 1), (abi.KG_AGG_SUM, 2), (abi.KG_AGG_COUNT, -1)],
                                         max_groups=64)
     names = [c["l_shipmode"][int(r)] for r in lrows.cpu().numpy()[first]]
-    for g, s in enumerate(names):   # every row of a group has the group's string
-        exact = text_select(ctx, sm, abi.KG_TEXT_EQ, s, lrows)
-        assert len(exact) == vals[g, 2]
+    # every row of a group has the group's string: no two strings share a code
+    assert text_hash_verify(ctx, sm, codes, lrows) == 0
     gsm = TextColumn.from_strings(names, device=DEV)
     perm = order_by(ctx, [gsm], [True]).cpu().numpy()
     return [(names[i], int(vals[i, 0]), int(vals[i, 1])) for i in perm]
@@ -480,3 +480,53 @@ def test_tpch_q09():
     assert [g[:2] for g in got] == [w[:2] for w in want]
     assert np.allclose([g[2] for g in got], [w[2] for w in want], rtol=1e-12, atol=1e-9)
     ctx.close()
+
+
+def test_tpch_q18_lower_bar():
+    """Q18's whole plan with a HAVING bar low enough that about 60 orders
+    pass: the subquery GROUP BY and HAVING select, the joins with orders,
+    customer and lineitem again, the GROUP BY over (c_name, c_custkey,
+    o_orderkey, o_orderdate, o_totalprice) (c_name is an attribute of the
+    uniquely keyed customer row, so the device groups by the other four) with
+    SUM(l_quantity), and ORDER BY o_totalprice DESC, o_orderdate, against the
+    oracle's restatement of the plan."""
+    import tpch_q18
+    c = tpch_q18.full_columns()
+    assert len(np.unique(c["c_custkey"])) == len(c["c_custkey"])
+    assert len(np.unique(c["o_orderkey"])) == len(c["o_orderkey"])
+    keys, sums = tpch_q18.oracle_subquery(c)
+    bar = float(np.sort(sums)[-60])
+    want = tpch_q18.oracle_rows(c, bar)
+    assert 40 <= len(want) <= 60
+    ctx = Context(0)
+    I, R = abi.KG_INT, abi.KG_REAL
+    n = len(c["l_orderkey"])
+    first, vals, _ = ctx.hash_group_agg([dev(c["l_orderkey"], I), dev(c["l_quantity"], R)], [], [0],
+                                        [(abi.KG_AGG_SUM, 1)], max_groups=n + 1)
+    got_sums = vals[:, 0].view(np.float64).copy()
+    s = Column(R, torch.from_numpy(got_sums).to(DEV))
+    having = ctx.scan_select([s], [(0, abi.KG_GT, bar)])
+    hk = gather(ctx, Column(I, torch.from_numpy(c["l_orderkey"][first].astype(np.int32)).to(DEV)), having)
+    print(f"q18 bar {bar}: {len(having)} orders above it", flush=True)
+    # subquery JOIN orders ON l_orderkey = o_orderkey
+    orow, _ = _dev_join(ctx, hk, dev(c["o_orderkey"], I))
+    # JOIN customer ON o_custkey = c_custkey
+    pr, crow = _dev_join(ctx, dev(c["c_custkey"], I), gather(ctx, dev(c["o_custkey"], I), orow.int()))
+    orow = orow[pr].contiguous()
+    # JOIN lineitem ON o_orderkey = l_orderkey
+    lrow, br = _dev_join(ctx, gather(ctx, dev(c["o_orderkey"], I), orow.int()), dev(c["l_orderkey"], I))
+    orow, crow = orow[br].int().contiguous(), crow[br].int().contiguous()
+    gcols = [gather(ctx, dev(c["c_custkey"], I), crow), gather(ctx, dev(c["o_orderkey"], I), orow),
+             gather(ctx, dev(c["o_orderdate"], abi.KG_DATE), orow), gather(ctx, dev(c["o_totalprice"], R), orow),
+             gather(ctx, dev(c["l_quantity"], R), lrow.int())]
+    gfirst, gvals, _ = ctx.hash_group_agg(gcols, [], [0, 1, 2, 3], [(abi.KG_AGG_SUM, 4)], max_groups=len(lrow) + 1)
+    fo, fc = orow.cpu().numpy()[gfirst], crow.cpu().numpy()[gfirst]
+    tp = Column(R, torch.from_numpy(c["o_totalprice"][fo].copy()).to(DEV))
+    od = Column(abi.KG_DATE, torch.from_numpy(c["o_orderdate"][fo].copy()).to(DEV))
+    perm = order_by(ctx, [tp, od], [False, True]).cpu().numpy()
+    rs = gvals[:, 0].view(np.float64)
+    got = [(c["c_name"][fc[i]], int(c["c_custkey"][fc[i]]), int(c["o_orderkey"][fo[i]]), int(c["o_orderdate"][fo[i]]),
+            float(c["o_totalprice"][fo[i]]), float(rs[i])) for i in perm]
+    ctx.close()
+    assert [g[:5] for g in got] == [w[:5] for w in want]
+    assert all(abs(g[5] - w[5]) <= 1e-12 * abs(w[5]) for g, w in zip(got, want))

@@ -1,5 +1,6 @@
 """The oracle (CPU restatement) against the reference's own end-to-end
 fixtures: the checked-in column files and *_expected.tbl outputs."""
+import numpy as np
 import pytest
 
 import kdb
@@ -86,3 +87,28 @@ def test_tpch_q07():
 def test_tpch_q09():
     import tpch_q09
     tpch_q09.check(tpch_q09.oracle_q09(tpch_q09.columns()), kdb.expected("tpch/q09_expected.tbl"))
+
+
+def test_q18_plan_restatement_vs_pandas():
+    """tpch_q18.oracle_rows (the whole Q18 plan at a HAVING bar low enough for
+    rows to reach the joins, the GROUP BY and the ORDER BY) against the same
+    SQL evaluated independently with pandas merges, groupby and sort; at the
+    plan's own bar (315) both are empty, as the reference's expected table is."""
+    import pandas as pd
+    import tpch_q18
+    c = tpch_q18.full_columns()
+    li = pd.DataFrame({"k": c["l_orderkey"], "q": c["l_quantity"]})
+    sub = li.groupby("k", sort=False)["q"].sum()
+    bar = float(np.sort(sub.to_numpy())[-60])
+    sel = sub[sub > bar].index.to_numpy()
+    o = pd.DataFrame({"ok": c["o_orderkey"], "od": c["o_orderdate"], "tp": c["o_totalprice"], "ck": c["o_custkey"]})
+    cu = pd.DataFrame({"ck": c["c_custkey"], "name": c["c_name"]})
+    j = o[o["ok"].isin(sel)].merge(cu, on="ck").merge(li, left_on="ok", right_on="k")
+    g = j.groupby(["name", "ck", "ok", "od", "tp"], sort=False)["q"].sum().reset_index()
+    g = g.sort_values(["tp", "od"], ascending=[False, True])
+    want = [(r.name, int(r.ck), int(r.ok), int(r.od), float(r.tp), float(r.q)) for r in g.itertuples(index=False)]
+    got = tpch_q18.oracle_rows(c, bar)
+    assert len(got) == len(want) > 0
+    assert [x[:5] for x in got] == [x[:5] for x in want]
+    assert all(abs(x[5] - y[5]) <= 1e-9 * abs(y[5]) for x, y in zip(got, want))
+    assert tpch_q18.oracle_rows(c, tpch_q18.BAR) == [] and kdb.expected("tpch/q18_expected.tbl") == []
