@@ -529,6 +529,23 @@ def bench_chain_c3(ctx, a):
                                "sample": f"the whole query over the same {n_in} rows, {ct:.2f} s: filters, four "
                                          f"left-deep chained hash joins, MIN; result equal to the device's: {same}"}
         out["oracle_equal"] = bool(same)
+    # The same query as one native plan (kg_chain_run: the same entry points
+    # called from C++, intermediate memory from the context's block cache).
+    from kushdb_amd.runtime import chain_run
+    from kushdb_amd.workloads import chain_query_for
+    q = chain_query_for(tabs, "a", "%e_a%", 50)
+    chain_run(ctx, q)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nn, nvmin, nrow = chain_run(ctx, q)
+    nwall = (time.perf_counter() - t0) / steps
+    ntext = bytes(htabs[0]["name_heap"][htabs[0]["name_off"][nrow]:htabs[0]["name_off"][nrow + 1]]) \
+        if nrow is not None else None
+    as_b = lambda x: x.encode() if isinstance(x, str) else (bytes(x) if x is not None else None)
+    out["native_plan"] = {"query_ms": round(nwall * 1e3, 3), "rows_per_s": n_in / nwall,
+                          "equal_to_stepwise": bool(nn == tuples.shape[0] and nvmin == mn_int and
+                                                    ntext == as_b(mn_text))}
+    log(f"C3 native plan: {nwall * 1e3:.2f} ms ({out['native_plan']})")
     # The same query with the join order chosen per episode by the UCT agent (kg_skinner_join).
     from kushdb_amd.chain import skinner_chain_query
     for budget in () if a.no_skinner else (1 << 20, 1 << 22):
@@ -575,12 +592,14 @@ def bench_serving(ctx, a):
     import torch
     from kushdb_amd import abi
     from kushdb_amd.runtime import serve
-    from kushdb_amd.workloads import (SERVE_FACT_SIZES, SERVE_TEMPLATES, make_query, serve_columns, serve_queries,
-                                      serve_tables)
+    from kushdb_amd.workloads import (SERVE_FACT_SIZES, SERVE_TEMPLATES, chain_indexes, make_query, serve_columns,
+                                      serve_queries, serve_tables)
     log(f"serving: generating tables {SERVE_FACT_SIZES}")
     tables = serve_tables(device="cuda:0")
     torch.cuda.synchronize()
     facts, dim, build = serve_columns(tables)
+    # Stored indexes on the chain tables' fk columns (unfiltered children probe them).
+    build["chain_idx"] = chain_indexes(ctx, build["chain"])
     specs = serve_queries(a.serve_queries)
     queries = [make_query(s, facts, dim, build) for s in specs]
     # The join tuner's load factors for the two build tables (cached on the
@@ -588,6 +607,7 @@ def bench_serving(ctx, a):
     for bname, pcol in (("build", 11), ("big", 12)):
         log(f"serving: join tuning for the {bname} table: {ctx.join_tune(build[bname][0], facts[-1][pcol])}")
     nq = len(queries)
+    tpl_of = np.array([sp["template"] for sp in specs])
     rng = np.random.RandomState(7)
     expo = rng.exponential(1.0, size=nq)
 
@@ -598,7 +618,16 @@ def bench_serving(ctx, a):
         out = {"offered": rate, "cap_rows": cap, "qps": st["qps"], "p50_ms": st["p50_s"] * 1e3,
                "p99_ms": st["p99_s"] * 1e3, "mean_batch": st["mean_batch"], "batches": st["batches"],
                "fallbacks": st["fallbacks"]}
+        # Latency by template: [p50, p99] in ms.
+        lat_ms = np.asarray(lat[:nq]) * 1e3
+        out["by_template"] = {t: [round(float(np.percentile(lat_ms[tpl_of == t], q)), 3) for q in (50, 99)]
+                              for t in SERVE_TEMPLATES if (tpl_of == t).any()}
         return (out, res) if keep else out
+
+    t0 = tables["chain"]["host"][0]
+
+    def chain_text(row):
+        return bytes(t0["name_heap"][t0["name_off"][row]:t0["name_off"][row + 1]])
 
     def same_results(ra, rb):
         """Per query: equal status, group keys and order, counts and integer
@@ -608,6 +637,10 @@ def bench_serving(ctx, a):
         for i, (sp, x, y) in enumerate(zip(specs, ra, rb)):
             ok = x[0] == y[0] and x[1].shape == y[1].shape and np.array_equal(x[1], y[1]) and \
                 np.array_equal(x[3], y[3])
+            if ok and sp["kind"] == "chain" and x[0] == abi.KG_OK:
+                # tuple count and MIN int exact; the MIN strings' bytes equal
+                ok = x[2][0, 0] == y[2][0, 0] and (x[3][0, 1] or x[2][0, 1] == y[2][0, 1]) and \
+                    (x[3][0, 2] or chain_text(x[2][0, 2]) == chain_text(y[2][0, 2]))
             if not ok and len(examples) < 8:
                 examples.append({"query": i, "template": sp["template"], "table": sp["table"],
                                  "status": [int(x[0]), int(y[0])], "groups": [len(x[1]), len(y[1])]})

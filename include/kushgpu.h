@@ -578,7 +578,51 @@ int kg_gather(kg_ctx* ctx, const kg_col* src, const int32_t* d_rows, int64_t n, 
  * are exact. MIN/MAX and more than 16 aggregates are not batched
  * (KG_ERR_UNSUPPORTED in that query's result; run it on the plain path). */
 typedef struct kg_batch kg_batch;
-enum kg_query_kind { KG_Q_SCAN_AGG = 0, KG_Q_JOIN_AGG = 1 };
+/* KG_Q_CHAIN: the multi-way join shape of the reference's Skinner queries
+ * (PermutableSkinnerJoinTranslator over filtered children,
+ * compile/translators/permutable_skinner_join_translator.cc:82-994, then the
+ * ungrouped MIN aggregates of aggregate_translator.cc:34-126) as one served
+ * query: tables t0..t(k-1) joined along a chain t_i.fk = t_(i-1).id, each
+ * child filtered by conjunctive predicates over its columns and at most one
+ * TEXT predicate (kg_text_select), the result tuples deduplicated, then
+ * MIN(TEXT) over one table's string column and MIN over one table's integer
+ * column. Run by kg_chain_run; batches report it KG_ERR_UNSUPPORTED and the
+ * serving loop runs it on the plain path. With skinner_budget > 0 the join
+ * builds its own indexes (fk_index is not used). */
+typedef struct kg_chain_table {
+  int64_t n;                    /* rows */
+  const kg_col* cols;
+  int32_t ncols;
+  int32_t npreds;
+  const kg_pred* preds;         /* conjunctive, over cols */
+  const kg_text_col* text;      /* TEXT predicate column, or NULL for none */
+  const char* text_pattern;
+  int32_t text_op;              /* kg_text_op */
+  int32_t text_plen;
+  int32_t id_col;               /* key the next table references (cols index; -1 for the last table) */
+  int32_t fk_col;               /* key that references the previous table's id_col (-1 for the first) */
+  const kg_index* fk_index;     /* optional kg_index_build of fk_col over all rows, used when the table has
+                                   no predicate: the stored column index of an unfiltered child
+                                   (permutable_skinner_join_translator.cc:116-132); NULL: built per query */
+} kg_chain_table;
+typedef struct kg_chain {
+  int32_t ntables;              /* 2..8 */
+  int32_t min_text_table;       /* MIN over min_text at that table's rows, or -1 */
+  const kg_chain_table* tables;
+  const kg_text_col* min_text;
+  int32_t min_int_table;        /* MIN over tables[min_int_table].cols[min_int_col] (an integer type), or -1 */
+  int32_t min_int_col;
+  int64_t skinner_budget;       /* 0: the order t0, t1, ... by index-nested-loop steps; > 0: kg_skinner_join
+                                   with this budget per episode (UCT-chosen orders) */
+  int64_t seed;                 /* kg_skinner_join seed (> 0) */
+} kg_chain;
+/* One KG_Q_CHAIN query: *out_count = distinct result tuples; *out_min_int =
+ * the MIN (null flag *min_int_null = 1 when there is no tuple or no MIN
+ * column); *out_min_text_row = the row of min_text holding the first minimal
+ * string (-1 when there is none). Synchronizes the context's stream. */
+int kg_chain_run(kg_ctx* ctx, const kg_chain* q, int64_t* out_count, int64_t* out_min_int, int8_t* min_int_null,
+                 int64_t* out_min_text_row);
+enum kg_query_kind { KG_Q_SCAN_AGG = 0, KG_Q_JOIN_AGG = 1, KG_Q_CHAIN = 2 };
 typedef struct kg_query {
   int32_t kind;                 /* kg_query_kind */
   int32_t table_id;             /* scan/probe table identity for tile sharing */
@@ -604,6 +648,9 @@ typedef struct kg_query {
   double load_factor;           /* join table load factor; 0 = what kg_join_tune chose on
                                    this context for builds of the same power-of-two size
                                    (unique keys preferred), else 0.5 */
+  const kg_chain* chain;        /* KG_Q_CHAIN: the query (the fields above are unused); its
+                                   result is one group of 3 values: COUNT of distinct tuples,
+                                   MIN int (null when none), row of the MIN string (null when none) */
 } kg_query;
 typedef struct kg_query_result {
   int32_t status;               /* KG_OK or this query's error */
@@ -657,9 +704,12 @@ int kg_batch_run(kg_batch* b, const kg_query* qs, int32_t nq, kg_query_result* r
  * device footprint (kg_batch_footprint) fits the slot's arena, the batch holds
  * at most max_batch queries and its rows (scanned + build) stay within
  * max_batch_rows (the first query is always admitted), which bounds how long a
- * batch keeps its slot; a query the batch engine does not support
- * is re-run on the plain path. Each query completes as soon as its own work
- * is done (kg_batch_query_ready), not at the end of its batch. */
+ * batch keeps its slot; chain queries (KG_Q_CHAIN), queries larger than an
+ * arena and queries a batch reports unsupported run on the plain path in a
+ * lane of their own: a host thread with a second context and stream (kept
+ * with ctx for later calls), one query at a time, alongside the batches. Each
+ * query completes as soon as its own work is done (kg_batch_query_ready), not
+ * at the end of its batch. */
 enum kg_serve_mode { KG_SERVE_PLAIN = 0, KG_SERVE_BATCHED = 1 };
 typedef struct kg_serve_opts {
   int32_t mode;                 /* kg_serve_mode */
@@ -675,7 +725,7 @@ typedef struct kg_serve_stats {
   double p50_s, p99_s, max_s;   /* latency percentiles */
   double mean_batch;            /* queries per batch (batched mode) */
   int64_t batches;
-  int64_t fallbacks;            /* queries re-run on the plain path */
+  int64_t fallbacks;            /* batched mode: queries run on the plain lane */
 } kg_serve_stats;
 int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const double* arrival_s, const kg_serve_opts* opts,
              kg_query_result* res, double* latency_s, kg_serve_stats* stats);

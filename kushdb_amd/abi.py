@@ -65,8 +65,23 @@ class KgAgg(ctypes.Structure):
 
 
 # enum kg_query_kind / kg_serve_mode
-KG_Q_SCAN_AGG, KG_Q_JOIN_AGG = range(2)
+KG_Q_SCAN_AGG, KG_Q_JOIN_AGG, KG_Q_CHAIN = range(3)
 KG_SERVE_PLAIN, KG_SERVE_BATCHED = range(2)
+
+
+class KgChainTable(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("cols", ctypes.POINTER(KgCol)), ("ncols", ctypes.c_int32),
+                ("npreds", ctypes.c_int32), ("preds", ctypes.POINTER(KgPred)),
+                ("text", ctypes.POINTER(KgTextCol)), ("text_pattern", ctypes.c_char_p), ("text_op", ctypes.c_int32),
+                ("text_plen", ctypes.c_int32), ("id_col", ctypes.c_int32), ("fk_col", ctypes.c_int32),
+                ("fk_index", ctypes.c_void_p)]
+
+
+class KgChain(ctypes.Structure):
+    _fields_ = [("ntables", ctypes.c_int32), ("min_text_table", ctypes.c_int32),
+                ("tables", ctypes.POINTER(KgChainTable)), ("min_text", ctypes.POINTER(KgTextCol)),
+                ("min_int_table", ctypes.c_int32), ("min_int_col", ctypes.c_int32),
+                ("skinner_budget", ctypes.c_int64), ("seed", ctypes.c_int64)]
 
 
 class KgQuery(ctypes.Structure):
@@ -79,7 +94,7 @@ class KgQuery(ctypes.Structure):
                 ("build_cols", ctypes.POINTER(KgCol)), ("build_ncols", ctypes.c_int32),
                 ("build_npreds", ctypes.c_int32), ("build_preds", ctypes.POINTER(KgPred)),
                 ("build_code_col", ctypes.c_int32), ("group_card", ctypes.c_int32),
-                ("load_factor", ctypes.c_double)]
+                ("load_factor", ctypes.c_double), ("chain", ctypes.POINTER(KgChain))]
 
 
 class KgQueryResult(ctypes.Structure):
@@ -201,6 +216,7 @@ SIGNATURES = {
     "kg_text_pairs_equal": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, ctypes.POINTER(KgTextCol), _vp, _i64, _vp,
                                    _pi64]),
     "kg_text_hash_verify": (_i32, [_vp, ctypes.POINTER(KgTextCol), _vp, _vp, _i64, _pi64]),
+    "kg_chain_run": (_i32, [_vp, ctypes.POINTER(KgChain), _pi64, _pi64, ctypes.POINTER(ctypes.c_int8), _pi64]),
     "kg_tuple_set_insert": (_i32, [_vp, _vp, _vp, _i64, _vp, _pi64]),
     "kg_tuple_set_extract": (_i32, [_vp, _vp, _vp, _i64, _pi64]),
     "kg_tuple_set_free": (_i32, [_vp, _vp]),

@@ -32,6 +32,7 @@ SOURCES = [
     "kg_skinner.hip",
     "kg_group.hip",
     "kg_detsum.hip",
+    "kg_chain.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

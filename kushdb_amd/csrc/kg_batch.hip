@@ -1844,6 +1844,7 @@ bool join_view(const kg_query& q, const PredSet& P, const MeasureSet& M, BJoinFa
 // Validates and plans query q into the device descriptor D and host plan P.
 int plan_query(kg_ctx* ctx, const kg_query& q, BQuery& D, BPlan& P) {
   std::memset(&D, 0, sizeof(D));
+  if (q.kind == KG_Q_CHAIN) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "chain queries are not batched");
   KG_REQUIRE(ctx, q.kind == KG_Q_SCAN_AGG || q.kind == KG_Q_JOIN_AGG, "bad query kind %d", q.kind);
   KG_REQUIRE(ctx, q.n >= 0 && q.n < INT32_MAX, "row count %lld out of range", (long long)q.n);
   KG_REQUIRE(ctx, q.naggs >= 0 && q.naggs <= kBMaxAggs, "at most %d aggregates", kBMaxAggs);

@@ -351,13 +351,18 @@ struct kg_tuple_set {
 
 namespace {
 
-int ialloc(kg_ctx* ctx, void** p, size_t bytes) {
-  if (hipMalloc(p, bytes ? bytes : 8) != hipSuccess) {
-    (void)hipGetLastError();
-    *p = nullptr;
-    return kg_fail(ctx, KG_ERR_OOM, "device allocation of %zu bytes failed", bytes);
-  }
-  return KG_OK;
+// Index and tuple-set memory comes from the context's block cache: a plan
+// builds and frees several of them per query.
+int ialloc(kg_ctx* ctx, void** p, size_t bytes) { return kg_dev_alloc(ctx, p, bytes ? bytes : 8); }
+
+// Returns a block to ctx's cache (stream-ordered reuse), or frees it when
+// there is no context.
+void ifree(kg_ctx* ctx, void* p) {
+  if (!p) return;
+  if (ctx)
+    kg_dev_release(ctx, p);
+  else
+    (void)hipFree(p);
 }
 
 int read_u64(kg_ctx* ctx, const void* d, void* h, size_t bytes) {
@@ -378,14 +383,11 @@ int check_index_key(kg_ctx* ctx, const kg_col* key, int64_t n) {
 
 extern "C" int kg_index_free(kg_ctx* ctx, kg_index* idx) {
   if (!idx) return KG_OK;
-  if (ctx) {
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-  }
-  if (idx->tids) (void)hipFree(idx->tids);
-  if (idx->dir.keys) (void)hipFree(idx->dir.keys);
-  if (idx->dir.offset) (void)hipFree(idx->dir.offset);
-  if (idx->dir.count) (void)hipFree(idx->dir.count);
+  if (ctx) (void)hipSetDevice(ctx->device);
+  ifree(ctx, idx->tids);
+  ifree(ctx, idx->dir.keys);
+  ifree(ctx, idx->dir.offset);
+  ifree(ctx, idx->dir.count);
   delete idx;
   return KG_OK;
 }
@@ -664,15 +666,12 @@ extern "C" int kg_intersect_batch(kg_ctx* ctx, int64_t ntasks, int32_t k, const 
 
 extern "C" int kg_tuple_set_free(kg_ctx* ctx, kg_tuple_set* s) {
   if (!s) return KG_OK;
-  if (ctx) {
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-  }
-  if (s->state) (void)hipFree(s->state);
-  if (s->data) (void)hipFree(s->data);
-  if (s->claimed) (void)hipFree(s->claimed);
-  if (s->first_pos) (void)hipFree(s->first_pos);
-  if (s->counters) (void)hipFree(s->counters);
+  if (ctx) (void)hipSetDevice(ctx->device);
+  ifree(ctx, s->state);
+  ifree(ctx, s->data);
+  ifree(ctx, s->claimed);
+  ifree(ctx, s->first_pos);
+  ifree(ctx, s->counters);
   delete s;
   return KG_OK;
 }

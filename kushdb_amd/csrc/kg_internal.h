@@ -78,6 +78,10 @@ struct kg_ctx {
   std::multimap<size_t, void*> block_cache;
   std::unordered_map<void*, size_t> block_live;
   size_t block_cache_bytes = 0;
+  // Context of the batched serving loop's plain lane (kg_serve): queries the
+  // batches do not run execute there, on its own stream and host thread.
+  // Created on first use, destroyed with this context.
+  kg_ctx* serve_lane = nullptr;
 };
 
 // Device memory for per-query structures (join tables, their accumulators):

@@ -6,10 +6,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -71,6 +74,21 @@ struct TablePool {
 // One query through the plain entry points (what a host plan would call).
 int run_plain(kg_ctx* ctx, const kg_query& q, kg_query_result& R, Scratch& S, TablePool& T) {
   R.ngroups = 0;
+  if (q.kind == KG_Q_CHAIN) {
+    KG_REQUIRE(ctx, R.max_groups >= 1 && R.vals && R.nulls, "a chain query's result takes one group of 3 values");
+    int64_t count = 0, min_int = 0, text_row = -1;
+    int8_t int_null = 1;
+    const int st = kg_chain_run(ctx, q.chain, &count, &min_int, &int_null, &text_row);
+    if (st != KG_OK) return st;
+    R.ngroups = 1;
+    R.vals[0] = count;
+    R.nulls[0] = 0;
+    R.vals[1] = min_int;
+    R.nulls[1] = int_null;
+    R.vals[2] = text_row;
+    R.nulls[2] = text_row < 0 ? 1 : 0;
+    return KG_OK;
+  }
   if (q.kind == KG_Q_SCAN_AGG)
     return kg_scan_group_agg(ctx, q.n, q.cols, q.ncols, q.preds, q.npreds, q.key_cols, q.key_card, q.nkeys, q.aggs,
                              q.naggs, R.max_groups, &R.ngroups, R.keys, R.vals, R.nulls);
@@ -107,6 +125,52 @@ double percentile(std::vector<double> v, double p) {
   return v[k - 1];
 }
 
+// The batched mode's plain lane: a host thread that runs the queries batches
+// do not take (chain queries, unsupported or oversized ones) one at a time on
+// its own context and stream, while the serving loop keeps admitting and
+// collecting batches.
+struct Lane {
+  std::mutex m;
+  std::condition_variable cv;
+  std::deque<int32_t> q;
+  bool stop = false;
+  std::thread th;
+  void push(int32_t i) {
+    {
+      std::lock_guard<std::mutex> lk(m);
+      q.push_back(i);
+    }
+    cv.notify_one();
+  }
+  void finish() {
+    if (!th.joinable()) return;
+    {
+      std::lock_guard<std::mutex> lk(m);
+      stop = true;
+    }
+    cv.notify_one();
+    th.join();
+  }
+};
+
+// The lane's context: created once per serving context, with the same
+// execution settings (deterministic mode, partitioning choices, join tunes).
+int lane_context(kg_ctx* ctx, kg_ctx** out) {
+  if (!ctx->serve_lane) {
+    const int st = kg_ctx_create(ctx->device, &ctx->serve_lane);
+    if (st != KG_OK) return kg_fail(ctx, st, "serving lane context: status %d", st);
+  }
+  kg_ctx* l = ctx->serve_lane;
+  l->deterministic = ctx->deterministic;
+  l->probe_regions = ctx->probe_regions;
+  l->probe_wg_per_cu = ctx->probe_wg_per_cu;
+  l->build_regions = ctx->build_regions;
+  l->agg_regions = ctx->agg_regions;
+  l->join_tunes = ctx->join_tunes;
+  *out = l;
+  return KG_OK;
+}
+
 struct Slot {
   kg_batch* b = nullptr;
   bool busy = false;
@@ -136,15 +200,20 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
   for (kg_query& q : resolved)
     if (q.kind == KG_Q_JOIN_AGG && !(q.load_factor > 0)) q.load_factor = kg::tuned_load_factor(ctx, q.build_n);
   qs = resolved.data();
-  Scratch S;
-  TablePool T;
+  Scratch S, LS;
+  TablePool T, LT;
   std::vector<Slot> slots;
   kg_serve_stats st_{};
+  Lane lane;
+  kg_ctx* lctx = nullptr;
   auto cleanup = [&]() {
+    lane.finish();
     for (Slot& s : slots)
       if (s.b) kg_batch_free(s.b);
   };
   if (batched) {
+    const int st = lane_context(ctx, &lctx);
+    if (st != KG_OK) return st;
     kg_batch_opts bo{};
     bo.arena_bytes = opts->arena_bytes;
     slots.resize(nslots);
@@ -157,23 +226,47 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
     }
   }
   // Pre-fault the plain path's code and the batch graphs before the clock starts.
-  KG_HIP_CHECK(ctx, hipDeviceSynchronize());
+  if (const hipError_t e = hipDeviceSynchronize(); e != hipSuccess) {
+    cleanup();
+    return kg_fail(ctx, KG_ERR_HIP, "hipDeviceSynchronize failed: %s", hipGetErrorString(e));
+  }
   std::vector<double> done_at(nq, -1.0);
   std::deque<int32_t> pending;
-  int32_t next = 0, ndone = 0;
-  int first_err = KG_OK;
+  int32_t next = 0;
+  std::atomic<int32_t> ndone{0};
   const Clock::time_point t0 = Clock::now();
   auto finish = [&](int32_t i) {
     done_at[i] = seconds_since(t0);
     latency_s[i] = done_at[i] - arrival_s[i];
-    ndone++;
+    ndone.fetch_add(1, std::memory_order_release);
   };
+  // Plain mode: in the loop. Batched mode: handed to the lane's thread, which
+  // writes res[i], done_at[i] and latency_s[i] of its queries only.
   auto plain = [&](int32_t i) {
+    if (batched) {
+      lane.push(i);
+      return;
+    }
     res[i].status = run_plain(ctx, qs[i], res[i], S, T);
-    if (res[i].status != KG_OK && first_err == KG_OK) first_err = res[i].status;
     finish(i);
   };
-  while (ndone < nq) {
+  if (batched)
+    lane.th = std::thread([&]() {
+      (void)hipSetDevice(lctx->device);
+      for (;;) {
+        int32_t i;
+        {
+          std::unique_lock<std::mutex> lk(lane.m);
+          lane.cv.wait(lk, [&]() { return lane.stop || !lane.q.empty(); });
+          if (lane.q.empty()) return;
+          i = lane.q.front();
+          lane.q.pop_front();
+        }
+        res[i].status = run_plain(lctx, qs[i], res[i], LS, LT);
+        finish(i);
+      }
+    });
+  while (ndone.load(std::memory_order_acquire) < nq) {
     const double now = seconds_since(t0);
     while (next < nq && arrival_s[next] <= now) pending.push_back(next++);
     bool progressed = false;
@@ -210,7 +303,6 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
             st_.fallbacks++;
             plain(i);
           } else {
-            if (res[i].status != KG_OK && first_err == KG_OK) first_err = res[i].status;
             finish(i);
           }
         }
@@ -240,8 +332,8 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
           const int32_t i = pending.front();
           int64_t fp = 0;
           kg_batch_footprint(&qs[i], &fp);
-          if (fp > arena) {
-            // Never fits a slot: run it alone on the plain path.
+          if (fp > arena || qs[i].kind == KG_Q_CHAIN) {
+            // Never fits a slot, or a kind batches do not run: the plain path.
             pending.pop_front();
             st_.fallbacks++;
             plain(i);
@@ -288,7 +380,7 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
       }
     }
   }
-  cleanup();
+  cleanup();  // joins the lane: its writes are visible from here on
   std::vector<double> lat(latency_s, latency_s + nq);
   double last = 0;
   for (double d : done_at) last = std::max(last, d);
@@ -299,6 +391,5 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
   st_.max_s = percentile(lat, 1.0);
   if (st_.batches > 0) st_.mean_batch /= static_cast<double>(st_.batches);
   if (stats) *stats = st_;
-  (void)first_err;
   return KG_OK;
 }

@@ -378,14 +378,23 @@ extern "C" int kg_text_minmax(kg_ctx* ctx, const kg_text_col* col, const int32_t
   KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   *out_pos = pos;
   if (pos < 0) return KG_OK;
+  // Read-backs on the context's stream (a null-stream copy would also wait
+  // for every other stream's work on the device).
   int32_t row = static_cast<int32_t>(pos);
-  if (d_rows) KG_HIP_CHECK(ctx, hipMemcpy(&row, d_rows + pos, 4, hipMemcpyDeviceToHost));
+  if (d_rows) {
+    KG_HIP_CHECK(ctx, hipMemcpyAsync(&row, d_rows + pos, 4, hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  }
   int64_t ab[2];
-  KG_HIP_CHECK(ctx, hipMemcpy(ab, col->offsets + row, 16, hipMemcpyDeviceToHost));
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(ab, col->offsets + row, 16, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   *out_len = ab[1] - ab[0];
   if (out_buf && buf_cap > 0) {
     const int64_t m = std::min(buf_cap, *out_len);
-    if (m > 0) KG_HIP_CHECK(ctx, hipMemcpy(out_buf, col->bytes + ab[0], m, hipMemcpyDeviceToHost));
+    if (m > 0) {
+      KG_HIP_CHECK(ctx, hipMemcpyAsync(out_buf, col->bytes + ab[0], m, hipMemcpyDeviceToHost, ctx->stream));
+      KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
   }
   return KG_OK;
 }

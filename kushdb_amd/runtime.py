@@ -741,10 +741,66 @@ class Query:
                      build_cols=build_cols, build_key=build_key, build_preds=build_preds, build_codes=build_codes,
                      group_card=group_card if build_codes >= 0 else 1, load_factor=load_factor)
 
+    @staticmethod
+    def chain(tables, min_text=None, min_int=None, skinner_budget=0, seed=1, table_id=0):
+        """A KG_Q_CHAIN query: tables t0..t(k-1) joined t_i.fk = t_(i-1).id.
+        tables: [{"cols": [Column], "preds": [(col, op, lit)], "text": TextColumn
+        or None, "text_op": op, "pattern": bytes, "id_col": i, "fk_col": j,
+        "fk_index": ColumnIndex over fk_col (all rows) or None}];
+        min_text = (table, TextColumn) and min_int = (table, col) name the MIN
+        aggregates. Its result is one group: (COUNT of distinct tuples, MIN
+        int, row of the MIN string)."""
+        self = Query.__new__(Query)
+        self.kind = abi.KG_Q_CHAIN
+        self.cols, self.build_cols = [], []
+        self.naggs, self.nkeys_out = 3, 0
+        keep = []
+        tarr = (abi.KgChainTable * len(tables))()
+        for i, t in enumerate(tables):
+            cols = list(t["cols"])
+            carr = (KgCol * len(cols))(*[c.desc() for c in cols])
+            preds = list(t.get("preds", ()))
+            parr = (KgPred * max(len(preds), 1))(*[_pred(cols, p) for p in preds])
+            keep += [carr, parr]
+            tc = t.get("text")
+            tdesc = ctypes.pointer(tc.desc()) if tc is not None else None
+            pat = bytes(t.get("pattern", b""))
+            keep += [tdesc, pat]
+            fki = t.get("fk_index")
+            keep.append(fki)
+            tarr[i] = abi.KgChainTable(len(cols[0]), carr, len(cols), len(preds), parr, tdesc, pat,
+                                       t.get("text_op", 0), len(pat), t.get("id_col", -1), t.get("fk_col", -1),
+                                       fki.h.value if fki is not None else None)
+        mt = ctypes.pointer(min_text[1].desc()) if min_text else None
+        keep += [tarr, mt]
+        self.chain_desc = abi.KgChain(len(tables), min_text[0] if min_text else -1, tarr, mt,
+                                      min_int[0] if min_int else -1, min_int[1] if min_int else -1,
+                                      skinner_budget, seed)
+        self._keep = tuple(keep)
+        self.q = abi.KgQuery()
+        self.q.kind = abi.KG_Q_CHAIN
+        self.q.table_id = table_id
+        self.q.naggs = 3
+        self.q.build_code_col = -1
+        self.q.probe_key_col = -1
+        self.q.build_key_col = -1
+        self.q.chain = ctypes.pointer(self.chain_desc)
+        return self
+
     def footprint(self):
         b = ctypes.c_int64()
         abi.load().kg_batch_footprint(ctypes.byref(self.q), ctypes.byref(b))
         return b.value
+
+
+def chain_run(ctx, query):
+    """(COUNT of distinct tuples, MIN int or None, row of the MIN string or
+    None) of a Query.chain query (kg_chain_run)."""
+    c, v, t = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    nl = ctypes.c_int8()
+    ctx.check(ctx.lib.kg_chain_run(ctx.h, ctypes.byref(query.chain_desc), ctypes.byref(c), ctypes.byref(v),
+                                   ctypes.byref(nl), ctypes.byref(t)))
+    return c.value, (None if nl.value else v.value), (None if t.value < 0 else t.value)
 
 
 def _result_buffers(queries, max_groups):

@@ -149,6 +149,7 @@ def skinner_tables(scale=1.0, theta=0.0, device="cuda", seed=SEED + 4):
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     sizes = [max(4, int(n * scale)) for n in SKINNER_SIZES]
+    v_is_dev = torch.device(device).type == "cuda"
     vocab = vocabulary(seed=seed)
     tabs, htabs = [], []
     for i, n in enumerate(sizes):
@@ -160,7 +161,9 @@ def skinner_tables(scale=1.0, theta=0.0, device="cuda", seed=SEED + 4):
                 perm = torch.randperm(sizes[i - 1], generator=g, device=device).to(torch.int32)
                 t["fk"] = perm[t["fk"].long()]
         ht = {k: v.cpu().numpy() for k, v in t.items()}
-        dt = {k: Column(abi.KG_INT, v) for k, v in t.items()}
+        # Column objects need device tensors; on the CPU the tensors stay raw
+        # (host copies are what CPU callers use).
+        dt = {k: Column(abi.KG_INT, v) for k, v in t.items()} if v_is_dev else dict(t)
         if i in (0, 1):
             name = "name" if i == 0 else "s"
             off, heap, lst = text_column(n, vocab, rng)
@@ -183,16 +186,27 @@ def skinner_tables(scale=1.0, theta=0.0, device="cuda", seed=SEED + 4):
 SERVE_FACT_SIZES = (10_000, 100_000, 1_000_000, 4_000_000, 16_000_000, 50_000_000)
 SERVE_BUILD_ROWS = 1 << 20
 FACT_COLS = ("day", "g1", "g2", "fk", "m1", "m2", "m3", "m4", "i1", "i2", "i3", "k64", "k64b")
-SERVE_TEMPLATES = ("c1_scan_agg", "c1_small", "two_filter", "two_filter_ungrouped", "c0_star", "star_dim_filter",
+# Templates the batch engine runs, and the served mix: those plus the five-table
+# chain join with TEXT filters and MIN aggregates (c3_chain), which serving runs
+# on the plain path.
+BATCH_TEMPLATES = ("c1_scan_agg", "c1_small", "two_filter", "two_filter_ungrouped", "c0_star", "star_dim_filter",
                    "c2_join", "c2_join_16m", "selective_join")
+SERVE_TEMPLATES = BATCH_TEMPLATES + ("c3_chain",)
+# Chain tables of the served mix: the five chained tables at this fraction of
+# their full sizes (10k / 50k / 200k / 1.25M / 2.5M rows).
+SERVE_CHAIN_SCALE = 0.05
+# LIKE patterns of served chain queries (t1.s LIKE p).
+CHAIN_LIKES = ("%e_a%", "%a_e%", "%o_i%", "%i_e%", "%a_a%", "%e_o%")
 
 
 def serve_tables(fact_sizes=SERVE_FACT_SIZES, ndim=10_000, nbuild=SERVE_BUILD_ROWS, device="cuda", seed=SEED + 5,
-                 nbig=None):
+                 nbig=None, chain_scale=SERVE_CHAIN_SCALE):
     """Fact tables, the 10k-row dimension, the build table of nbuild int64
     keys and the big build table of nbig keys (16 * nbuild by default: 16M for
     the serving mix), each a permutation of 0..rows-1. Fact k64 / k64b keys are
-    uniform over 1.1 times the build / big table's keys (10% misses)."""
+    uniform over 1.1 times the build / big table's keys (10% misses). "chain"
+    holds the five chained tables of skinner_tables at chain_scale (device
+    columns and host copies)."""
     nbig = 16 * nbuild if nbig is None else nbig
     facts = []
     for j, n in enumerate(fact_sizes):
@@ -219,14 +233,18 @@ def serve_tables(fact_sizes=SERVE_FACT_SIZES, ndim=10_000, nbuild=SERVE_BUILD_RO
     g = _gen(device, seed + 9)
     big = {"key": torch.randperm(nbig, generator=g, device=device).to(torch.int64),
            "payload": torch.randint(0, 1 << 30, (nbig,), generator=g, device=device, dtype=torch.int32)}
-    return {"facts": facts, "dim": dim, "build": build, "big": big}
+    ctabs, _, chost = skinner_tables(scale=chain_scale, device=device)
+    return {"facts": facts, "dim": dim, "build": build, "big": big, "chain": {"tabs": ctabs, "host": chost}}
 
 
 def serve_queries(nq, fact_sizes=SERVE_FACT_SIZES, nbuild=SERVE_BUILD_ROWS, seed=SEED + 6, templates=None,
                   nbig=None):
     """nq query specs: template, fact table index and parameters. Column
     indices refer to FACT_COLS, the dimension's (pk, cat) and the build / big
-    build table's (key, payload). Templates and tables are drawn uniformly."""
+    build table's (key, payload). Templates and tables are drawn uniformly.
+    c3_chain specs (kind "chain") join the five chain tables with
+    t0.name STARTS_WITH p0, t1.s LIKE p1 and t4.v < vmax, for MIN(t0.name),
+    MIN(t4.v) and the count of result tuples."""
     nbig = 16 * nbuild if nbig is None else nbig
     import numpy as np
     from . import abi
@@ -285,6 +303,9 @@ def serve_queries(nq, fact_sizes=SERVE_FACT_SIZES, nbuild=SERVE_BUILD_ROWS, seed
                      build_preds=[(0, abi.KG_GEQ, lo), (0, abi.KG_LT, hi)])
             s["preds"] = [(0, abi.KG_GEQ, D + int(rng.randint(0, int(NDAYS * 0.5))))]
             s["aggs"] = [(C, -1), (S, 10)]
+        elif tpl == "c3_chain":
+            s.update(kind="chain", p0=chr(97 + int(rng.randint(26))), p1=CHAIN_LIKES[rng.randint(len(CHAIN_LIKES))],
+                     vmax=int(rng.randint(10, 101)), aggs=[])
         else:
             raise ValueError(tpl)
         specs.append(s)
@@ -303,12 +324,17 @@ def serve_columns(tables):
     dim = [Column(abi.KG_INT, tables["dim"]["pk"]), Column(abi.KG_ENUM, tables["dim"]["cat"])]
     builds = {b: [Column(abi.KG_BIGINT, tables[b]["key"]), Column(abi.KG_INT, tables[b]["payload"])]
               for b in ("build", "big")}
+    if "chain" in tables:
+        builds["chain"] = tables["chain"]["tabs"]
     return facts, dim, builds
 
 
 def make_query(spec, facts, dim, builds, load_factor=0.0):
     """runtime.Query for a spec of serve_queries over serve_columns."""
     from .runtime import Query
+    if spec["kind"] == "chain":
+        return chain_query_for(builds["chain"], spec["p0"], spec["p1"], spec["vmax"],
+                               indexes=builds.get("chain_idx"))
     cols = facts[spec["table"]]
     if spec["kind"] == "scan":
         return Query.scan_agg(cols, spec["preds"], spec["keys"], spec["aggs"], table_id=spec["table"])
@@ -316,3 +342,33 @@ def make_query(spec, facts, dim, builds, load_factor=0.0):
     return Query.join_agg(cols, spec["preds"], spec["probe_key"], bcols, spec["build_key"], spec["aggs"],
                           build_preds=spec["build_preds"], build_codes=spec["build_codes"],
                           group_card=spec["group_card"], table_id=spec["table"], load_factor=load_factor)
+
+
+def chain_indexes(ctx, tabs):
+    """Stored column indexes on the fk columns of chain tables t1.. over all
+    rows (None for t0): what an unfiltered child probes instead of an index
+    built per query."""
+    from .runtime import ColumnIndex
+    return [None] + [ColumnIndex(ctx, t["fk"]) for t in tabs[1:]]
+
+
+def chain_query_for(tabs, p0, p1, vmax, skinner_budget=0, seed=1, indexes=None):
+    """runtime.Query (KG_Q_CHAIN) over the five chain tables of
+    skinner_tables: t0.name STARTS_WITH p0, t1.s LIKE p1, t4.v < vmax;
+    MIN(t0.name), MIN(t4.v). indexes: chain_indexes (optional)."""
+    from . import abi
+    from .runtime import Query
+    tables = []
+    for i, t in enumerate(tabs):
+        cols = [t["id"], t["v"]] + ([t["fk"]] if i > 0 else [])
+        d = {"cols": cols, "id_col": 0, "fk_col": 2 if i > 0 else -1,
+             "fk_index": indexes[i] if indexes is not None else None}
+        if i == 0:
+            d.update(text=t["name"], text_op=abi.KG_TEXT_STARTS_WITH, pattern=p0.encode())
+        elif i == 1:
+            d.update(text=t["s"], text_op=abi.KG_TEXT_LIKE, pattern=p1.encode())
+        if i == len(tabs) - 1:
+            d["preds"] = [(1, abi.KG_LT, vmax)]
+        tables.append(d)
+    return Query.chain(tables, min_text=(0, tabs[0]["name"]), min_int=(len(tabs) - 1, 1),
+                       skinner_budget=skinner_budget, seed=seed)

@@ -17,8 +17,11 @@ PO_T = {"day": po.DATE, "g1": po.ENUM, "g2": po.ENUM, "fk": po.INT, "m1": po.REA
 def host_tables(tables):
     def h(d):
         return {k: v.cpu().numpy() for k, v in d.items()}
-    return {"facts": [h(t) for t in tables["facts"]], "dim": h(tables["dim"]), "build": h(tables["build"]),
-            "big": h(tables["big"])}
+    H = {"facts": [h(t) for t in tables["facts"]], "dim": h(tables["dim"]), "build": h(tables["build"]),
+         "big": h(tables["big"])}
+    if "chain" in tables:
+        H["chain"] = tables["chain"]["host"]
+    return H
 
 
 def _agg_col(f, a, at=None):
@@ -36,7 +39,14 @@ def _agg_col(f, a, at=None):
 
 
 def oracle_spec(spec, H):
-    """(keys int64 [G, nk], vals int64 [G, na], nulls [G, na]) in insertion order."""
+    """(keys int64 [G, nk], vals int64 [G, na], nulls [G, na]) in insertion order.
+    A chain spec: (MIN(t0.name) bytes or None, [[tuples, MIN(t4.v), -1]],
+    [[0, MIN null, name null]]) from the oracle's chain restatement."""
+    if spec["kind"] == "chain":
+        n, name, vmin = po.chain_query(H["chain"], spec["p0"].encode(), spec["p1"].encode(), spec["vmax"])
+        vals = np.array([[n, vmin if vmin is not None else 0, -1]], dtype=np.int64)
+        nulls = np.array([[0, vmin is None, name is None]], dtype=np.int8)
+        return name, vals, nulls
     f = H["facts"][spec["table"]]
     n = len(f["day"])
     fcols = [po.Col(PO_T[c], f[c]) for c in FACT_COLS]
@@ -82,12 +92,31 @@ def is_real_agg(a):
     return (len(a) > 2 and a[2] >= 0) or FACT_COLS[a[1]].startswith("m")
 
 
-def assert_same(spec, got, want, what=""):
+def chain_name(H, vals, nulls):
+    """MIN(t0.name) bytes of a chain query result (its row in vals[0, 2])."""
+    if nulls[0, 2]:
+        return None
+    t0 = H["chain"][0]
+    r = int(vals[0, 2])
+    return bytes(t0["name_heap"][t0["name_off"][r]:t0["name_off"][r + 1]])
+
+
+def assert_same(spec, got, want, what="", H=None):
     """got = (status, codes, vals, nulls) from the GPU; want = oracle_spec(...)
-    or another GPU result in the same 3-tuple form."""
+    or another GPU result in the same 3-tuple form. Chain specs compare the
+    MIN string's bytes (host tables H), not its row: equal strings may sit in
+    several rows."""
     status, codes, vals, nulls = got
     wk, wv, wn = want
     assert status == abi.KG_OK, (what, spec, status)
+    if spec["kind"] == "chain":
+        assert len(vals) == 1 and len(wv) == 1, (what, spec, len(vals), len(wv))
+        wv, wn = np.asarray(wv), np.asarray(wn)
+        assert np.array_equal(nulls, wn), (what, spec, nulls, wn)
+        assert vals[0, 0] == wv[0, 0] and (nulls[0, 1] or vals[0, 1] == wv[0, 1]), (what, spec, vals, wv)
+        want_name = wk if (wk is None or isinstance(wk, bytes)) else chain_name(H, wv, wn)
+        assert chain_name(H, vals, nulls) == want_name, (what, spec)
+        return
     assert len(vals) == len(wv), (what, spec, len(vals), len(wv))
     if codes.shape[1] and len(wv):
         assert np.array_equal(codes.astype(np.int64), np.asarray(wk, dtype=np.int64)), (what, spec)

@@ -9,21 +9,24 @@ import torch
 
 from kushdb_amd import abi
 from kushdb_amd.runtime import Batch, Context, Query, serve
-from kushdb_amd.workloads import make_query, serve_columns, serve_queries, serve_tables
+from kushdb_amd.workloads import (BATCH_TEMPLATES, chain_indexes, make_query, serve_columns, serve_queries,
+                                  serve_tables)
 from serve_oracle import assert_same, host_tables, oracle_spec
 
 pytestmark = pytest.mark.gpu
 
 SIZES = (3_000, 40_000, 250_000)
 NBUILD = 100_000
+CHAIN_SCALE = 0.01          # chain tables of 2k / 10k / 40k / 250k / 500k rows
 
 
 @pytest.fixture(scope="module")
 def env():
     ctx = Context(0)
-    tables = serve_tables(SIZES, nbuild=NBUILD, device="cuda:0")
+    tables = serve_tables(SIZES, nbuild=NBUILD, device="cuda:0", chain_scale=CHAIN_SCALE)
     torch.cuda.synchronize()
     cols = serve_columns(tables)
+    cols[2]["chain_idx"] = chain_indexes(ctx, cols[2]["chain"])
     return ctx, tables, cols, host_tables(tables)
 
 
@@ -34,7 +37,7 @@ def _queries(cols, specs):
 
 def test_batch_mix_vs_oracle(env):
     ctx, _, cols, H = env
-    specs = serve_queries(48, SIZES, NBUILD, seed=11)
+    specs = serve_queries(48, SIZES, NBUILD, seed=11, templates=BATCH_TEMPLATES)
     print("batch mix of", len(specs), flush=True)
     b = Batch(ctx, arena_bytes=1 << 30)
     got = b.run(_queries(cols, specs), max_groups=64)
@@ -45,7 +48,7 @@ def test_batch_mix_vs_oracle(env):
 
 def test_alone_equals_mix(env):
     ctx, _, cols, _ = env
-    specs = serve_queries(32, SIZES, NBUILD, seed=12)
+    specs = serve_queries(32, SIZES, NBUILD, seed=12, templates=BATCH_TEMPLATES)
     print("alone vs mix", flush=True)
     b = Batch(ctx, arena_bytes=1 << 30)
     mix = b.run(_queries(cols, specs), max_groups=64)
@@ -126,7 +129,7 @@ def test_repeated_submits_and_empty(env):
     ctx, _, cols, H = env
     b = Batch(ctx, arena_bytes=64 << 20)
     assert b.run([], max_groups=8) == []
-    specs = serve_queries(8, SIZES, NBUILD, seed=13)
+    specs = serve_queries(8, SIZES, NBUILD, seed=13, templates=BATCH_TEMPLATES)
     first = b.run(_queries(cols, specs), max_groups=64)
     for _ in range(3):
         again = b.run(_queries(cols, specs), max_groups=64)
@@ -153,10 +156,12 @@ def test_unsupported_and_bad_codes_per_query(env):
     q_bad = Query.scan_agg(facts[0], [], [(3, 10)], [(abi.KG_AGG_COUNT, -1)], table_id=0)
     # the same on the register path (at most 8 groups)
     q_bad8 = Query.scan_agg(facts[2], [(0, abi.KG_GEQ, 0)], [(3, 8)], [(abi.KG_AGG_SUM, 4)], table_id=2)
-    r = b.run([q_min, q_ok, q_bad, q_bad8], max_groups=64)
+    q_chain = make_query({"kind": "chain", "p0": "a", "p1": "%e_a%", "vmax": 50}, *cols)
+    r = b.run([q_min, q_ok, q_bad, q_bad8, q_chain], max_groups=64)
     assert r[0][0] == abi.KG_ERR_UNSUPPORTED
     assert r[2][0] == abi.KG_ERR_INVALID
     assert r[3][0] == abi.KG_ERR_INVALID
+    assert r[4][0] == abi.KG_ERR_UNSUPPORTED
     assert_same(ok_spec, r[1], oracle_spec(ok_spec, H), what="ok next to failures")
     b.free()
 
@@ -173,10 +178,13 @@ def test_serve_plain_and_batched_agree(env):
                                  arena_bytes=128 << 20, max_groups=64)
     print("served", st_b, flush=True)
     for s, p, q in zip(specs, plain, batched):
-        assert_same(s, p, oracle_spec(s, H), what="plain")
-        assert_same(s, q, (p[1], p[2], p[3]), what="batched vs plain")
+        assert_same(s, p, oracle_spec(s, H), what="plain", H=H)
+        assert_same(s, q, (p[1], p[2], p[3]), what="batched vs plain", H=H)
     assert (lat_p > 0).all() and (lat_b > 0).all()
-    assert st_b["batches"] >= 1 and st_b["fallbacks"] == 0
+    # chain queries are not batched: serving runs them on the plain path
+    nchain = sum(s["kind"] == "chain" for s in specs)
+    assert nchain >= 1
+    assert st_b["batches"] >= 1 and st_b["fallbacks"] == nchain
     assert st_p["p99_s"] >= st_p["p50_s"] > 0
 
 
@@ -205,7 +213,7 @@ def test_queries_complete_individually(env):
     answers."""
     import time
     ctx, _, cols, H = env
-    specs = serve_queries(24, SIZES, NBUILD, seed=17)
+    specs = serve_queries(24, SIZES, NBUILD, seed=17, templates=BATCH_TEMPLATES)
     b = Batch(ctx, arena_bytes=128 << 20)
     b.submit(_queries(cols, specs), max_groups=64)
     got = [None] * len(specs)
@@ -251,7 +259,7 @@ def test_map_only_build_sources(env):
     from kushdb_amd.runtime import Column
     ctx, _, _, _ = env
     nbig = 3_000_000
-    tables = serve_tables(SIZES, nbuild=NBUILD, nbig=nbig, device="cuda:0")
+    tables = serve_tables(SIZES, nbuild=NBUILD, nbig=nbig, device="cuda:0", chain_scale=CHAIN_SCALE)
     torch.cuda.synchronize()
     cols = serve_columns(tables)
     H = host_tables(tables)
@@ -282,3 +290,35 @@ def test_map_only_build_sources(env):
     assert st["fallbacks"] == len(qd)
     for s, g in zip(big[:2], res):
         assert_same(s, g, oracle_spec(s, Hd), what="repeated keys served")
+
+
+CHAIN_CASES = [("a", "%e_a%", 50), ("t", "%o_i%", 10), ("q", "%a_e%", 100), ("s", "%i_e%", 37),
+               ("zz", "%e_a%", 50), ("a", "%e_a%", 0)]
+
+
+@pytest.mark.parametrize("budget,stored", [(0, False), (0, True), (4096, False)])
+def test_chain_query_vs_oracle(env, budget, stored):
+    """The chain kind through kg_chain_run: five tables, t0.name STARTS_WITH,
+    t1.s LIKE, t4.v < x, joined in the fixed order (budget 0; indexes built
+    per query, or the stored fk indexes of the unfiltered t2 and t3) or by
+    the Skinner driver over several episodes; tuple count, MIN(t4.v) and the
+    MIN(t0.name) string equal to the oracle's chain restatement, including
+    empty results (no prefix match, no v below 0)."""
+    from kushdb_amd.runtime import chain_run
+    from kushdb_amd.workloads import chain_query_for
+    from oracle import pyoracle as po
+    from serve_oracle import chain_name
+    ctx, _, cols, H = env
+    tabs = cols[2]["chain"]
+    for p0, p1, vmax in CHAIN_CASES:
+        print("chain", p0, p1, vmax, "budget", budget, flush=True)
+        q = chain_query_for(tabs, p0, p1, vmax, skinner_budget=budget, seed=3,
+                            indexes=cols[2]["chain_idx"] if stored else None)
+        n, vmin, row = chain_run(ctx, q)
+        wn, wname, wvmin = po.chain_query(H["chain"], p0.encode(), p1.encode(), vmax)
+        assert n == wn, (p0, p1, vmax, n, wn)
+        assert vmin == wvmin, (p0, p1, vmax, vmin, wvmin)
+        vals = np.array([[n, 0, -1 if row is None else row]], dtype=np.int64)
+        nulls = np.array([[0, 0, row is None]], dtype=np.int8)
+        assert chain_name(H, vals, nulls) == wname, (p0, p1, vmax)
+    assert any(po.chain_query(H["chain"], p0.encode(), p1.encode(), v)[0] > 0 for p0, p1, v in CHAIN_CASES)

@@ -3,6 +3,7 @@ served query templates against numpy closed forms (row counts, integer sums,
 float sums), and the batch arena footprint (kg_batch_footprint, host code in
 libkushgpu.so that needs no GPU)."""
 import ctypes
+import re
 
 import numpy as np
 import pytest
@@ -17,7 +18,25 @@ NBUILD = 20_000
 
 @pytest.fixture(scope="module")
 def H():
-    return host_tables(serve_tables(SIZES, nbuild=NBUILD, device="cpu"))
+    # chain tables of 1k / 5k / 20k / 125k / 250k rows
+    return host_tables(serve_tables(SIZES, nbuild=NBUILD, device="cpu", chain_scale=0.005))
+
+
+def _chain_closed_form(T, p0, p1, vmax):
+    """(tuples, MIN(t4.v), MIN(t0.name) bytes) of the chain query: ids are the
+    row numbers, so each qualifying t4 row fixes its whole chain through the
+    fk columns."""
+    like = re.compile(p1.replace("%", ".*").replace("_", "."), re.S)
+    ok0 = np.array([x.startswith(p0) for x in T[0]["name_list"]])
+    ok1 = np.array([like.fullmatch(x) is not None for x in T[1]["s_list"]])
+    r4 = np.nonzero(T[4]["v"] < vmax)[0]
+    r3 = T[4]["fk"][r4]
+    r1 = T[2]["fk"][T[3]["fk"][r3]]
+    r0 = T[1]["fk"][r1]
+    m = ok1[r1] & ok0[r0]
+    if not m.any():
+        return 0, None, None
+    return int(m.sum()), int(T[4]["v"][r4[m]].min()), min(T[0]["name_list"][i].encode() for i in r0[m])
 
 
 def _mask(f, preds):
@@ -32,6 +51,12 @@ def _mask(f, preds):
 @pytest.mark.parametrize("tpl", SERVE_TEMPLATES)
 def test_composed_oracle_closed_forms(H, tpl):
     for s in serve_queries(6, SIZES, NBUILD, seed=3, templates=[tpl]):
+        if s["kind"] == "chain":
+            name, vals, nulls = oracle_spec(s, H)
+            n, vmin, wname = _chain_closed_form(H["chain"], s["p0"], s["p1"], s["vmax"])
+            assert vals[0, 0] == n and name == wname
+            assert (vmin is None and nulls[0, 1]) or vals[0, 1] == vmin
+            continue
         f = H["facts"][s["table"]]
         kv, vals, nulls = oracle_spec(s, H)
         m = _mask(f, s["preds"])
