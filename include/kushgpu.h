@@ -601,9 +601,10 @@ typedef struct kg_chain_table {
   int32_t text_plen;
   int32_t id_col;               /* key the next table references (cols index; -1 for the last table) */
   int32_t fk_col;               /* key that references the previous table's id_col (-1 for the first) */
-  const kg_index* fk_index;     /* optional kg_index_build of fk_col over all rows, used when the table has
-                                   no predicate: the stored column index of an unfiltered child
-                                   (permutable_skinner_join_translator.cc:116-132); NULL: built per query */
+  const kg_index* fk_index;     /* optional stored index: kg_index_build of fk_col over all rows (the
+                                   column index of permutable_skinner_join_translator.cc:116-132). The
+                                   fixed order then joins through it and tests this table's predicates on
+                                   the joined rows; NULL: filter first, index the selection per query */
 } kg_chain_table;
 typedef struct kg_chain {
   int32_t ntables;              /* 2..8 */

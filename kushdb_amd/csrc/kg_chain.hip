@@ -7,16 +7,73 @@
 // (kg_skinner_join), the result tuples deduplicated in a tuple-id set (the
 // valid-tuple handler, :842-912), then the ungrouped MIN aggregates
 // (aggregate_translator.cc:34-126; MinMaxAggregator, aggregator.cc:155-171).
-// Device memory comes from the context's block cache, so a served stream of
-// these queries allocates nothing once warm.
+// A table with a stored index on its fk column (over all rows) is joined
+// through that index instead, and its own predicates are evaluated on the
+// expanded tuples afterwards (the reference's table functions likewise test
+// flagged predicates after the index probe, :663-832); its tuple ids are then
+// row ids. Results are the same either way. Device memory comes from the
+// context's block cache, so a served stream of these queries allocates
+// nothing once warm.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "kg_internal.h"
+#include "kg_ops.h"
+#include "kg_strops.h"
 
 namespace {
+
+constexpr int kCBlock = 256;
+constexpr int kChainPattern = 240;
+
+// A table's TEXT predicate for the device (offsets NULL: none).
+struct TextPred {
+  kg_text_col col;
+  int32_t op;
+  int32_t len;
+  uint8_t p[kChainPattern];
+};
+
+// keep[i] = the table's predicates hold at row tuples[i * w + pos]; idx[i] = i.
+__global__ __launch_bounds__(kCBlock) void k_tuple_keep(const int32_t* tuples, int32_t w, int32_t pos, int64_t m,
+                                                         kg::PredSet P, TextPred F, uint8_t* keep, int32_t* idx) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kCBlock;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCBlock + threadIdx.x; i < m; i += stride) {
+    const int64_t r = tuples[i * w + pos];
+    bool ok = kg::eval_preds(P, r);
+    if (ok && F.col.offsets) {
+      if (F.col.nulls && F.col.nulls[r]) {
+        ok = false;
+      } else {
+        const int64_t a = F.col.offsets[r], b = F.col.offsets[r + 1];
+        ok = F.op == KG_TEXT_LIKE ? kg::like_match(F.col.bytes + a, b - a, F.p, F.len)
+                                  : kg::text_match(F.op, F.col.bytes + a, b - a, F.p, F.len);
+      }
+    }
+    keep[i] = ok ? 1 : 0;
+    idx[i] = static_cast<int32_t>(i);
+  }
+}
+
+// out[j] = tuples[idx[j]] (w ids each), j < *count.
+__global__ __launch_bounds__(kCBlock) void k_tuple_take(const int32_t* tuples, int32_t w, const int32_t* idx,
+                                                         const int64_t* count, int32_t* out) {
+  const int64_t m = *count * w;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kCBlock;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * kCBlock + threadIdx.x; e < m; e += stride) {
+    const int64_t j = e / w;
+    out[e] = tuples[static_cast<int64_t>(idx[j]) * w + (e - j * w)];
+  }
+}
+
+int cgrid(kg_ctx* ctx, int64_t work) {
+  return static_cast<int>(std::min<int64_t>(kg::ceil_div(std::max<int64_t>(work, 1), kCBlock),
+                                            static_cast<int64_t>(ctx->num_cus) * 16));
+}
 
 // Device blocks of one query, returned to the context's cache at the end.
 struct Blocks {
@@ -103,6 +160,56 @@ int select_table(kg_ctx* ctx, Blocks& B, const kg_chain_table& t, const int32_t*
   return KG_OK;
 }
 
+// Keeps the tuples (width w) whose row at position pos satisfies table t's
+// predicates; *cur / *ncur are replaced by the kept tuples.
+int filter_tuples(kg_ctx* ctx, Blocks& B, const kg_chain_table& t, int32_t w, int32_t pos, const int32_t** cur,
+                  int64_t* ncur) {
+  kg::PredSet P{};
+  int st = kg::build_preds(ctx, t.cols, t.ncols, t.preds, t.npreds, 1, t.n, P);
+  if (st != KG_OK) return st;
+  TextPred F{};
+  if (t.text) {
+    KG_REQUIRE(ctx, t.text->offsets && t.text->bytes, "TEXT column is NULL");
+    KG_REQUIRE(ctx, t.text_op >= KG_TEXT_EQ && t.text_op <= KG_TEXT_LIKE, "bad text operator %d", t.text_op);
+    KG_REQUIRE(ctx, t.text_plen >= 0 && t.text_plen <= kChainPattern, "pattern length %d out of 0..%d", t.text_plen,
+               kChainPattern);
+    const uint8_t* pat = reinterpret_cast<const uint8_t*>(t.text_pattern);
+    if (t.text_op == KG_TEXT_LIKE && !kg::like_pattern_ok(pat, t.text_plen)) {  // re2 rejects it: no match
+      *ncur = 0;
+      return KG_OK;
+    }
+    F.col = *t.text;
+    F.op = t.text_op;
+    F.len = t.text_plen;
+    if (t.text_plen) std::memcpy(F.p, pat, t.text_plen);
+  }
+  const int64_t m = *ncur;
+  uint8_t* keep = nullptr;
+  int32_t *idx = nullptr, *kept = nullptr, *out = nullptr;
+  int64_t* count = nullptr;
+  size_t cub_bytes = 0;
+  hipcub::DeviceSelect::Flagged(nullptr, cub_bytes, idx, keep, kept, count, static_cast<int>(m), ctx->stream);
+  uint8_t* cub = nullptr;
+  if ((st = B.get(m, &keep)) != KG_OK || (st = B.get(m, &idx)) != KG_OK || (st = B.get(m, &kept)) != KG_OK ||
+      (st = B.get(1, &count)) != KG_OK || (st = B.get(static_cast<int64_t>(cub_bytes), &cub)) != KG_OK ||
+      (st = B.get(m * w, &out)) != KG_OK)
+    return st;
+  hipLaunchKernelGGL(k_tuple_keep, dim3(cgrid(ctx, m)), dim3(kCBlock), 0, ctx->stream, *cur, w, pos, m, P, F, keep,
+                     idx);
+  KG_LAUNCH_CHECK(ctx);
+  KG_HIP_CHECK(ctx, hipcub::DeviceSelect::Flagged(cub, cub_bytes, idx, keep, kept, count, static_cast<int>(m),
+                                                  ctx->stream));
+  hipLaunchKernelGGL(k_tuple_take, dim3(cgrid(ctx, m * w)), dim3(kCBlock), 0, ctx->stream, *cur, w, kept, count,
+                     out);
+  KG_LAUNCH_CHECK(ctx);
+  int64_t h = 0;
+  KG_HIP_CHECK(ctx, hipMemcpyAsync(&h, count, 8, hipMemcpyDeviceToHost, ctx->stream));
+  KG_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *cur = out;
+  *ncur = h;
+  return KG_OK;
+}
+
 }  // namespace
 
 extern "C" int kg_chain_run(kg_ctx* ctx, const kg_chain* q, int64_t* out_count, int64_t* out_min_int,
@@ -119,10 +226,15 @@ extern "C" int kg_chain_run(kg_ctx* ctx, const kg_chain* q, int64_t* out_count, 
   const int k = q->ntables;
   const kg_chain_table* T = q->tables;
   Blocks B(ctx);
-  std::vector<const int32_t*> sel(k);
+  // Tables joined through a stored fk index keep no selection: their
+  // predicates are applied to the expanded tuples.
+  auto through_stored = [&](int i) { return q->skinner_budget == 0 && i > 0 && T[i].fk_index != nullptr; };
+  std::vector<const int32_t*> sel(k, nullptr);
   std::vector<int64_t> cnt(k);
-  for (int i = 0; i < k; i++)
-    if ((st = select_table(ctx, B, T[i], &sel[i], &cnt[i])) != KG_OK) return st;
+  for (int i = 0; i < k; i++) {
+    cnt[i] = T[i].n;
+    if (!through_stored(i) && (st = select_table(ctx, B, T[i], &sel[i], &cnt[i])) != KG_OK) return st;
+  }
 
   // Result tuples: int32[k] tuple ids (positions in each table's selection),
   // table order, each once.
@@ -147,14 +259,18 @@ extern "C" int kg_chain_run(kg_ctx* ctx, const kg_chain* q, int64_t* out_count, 
     int64_t ncur = cnt[0];
     for (int i = 1; i < k && ncur > 0; i++) {
       IndexGuard built{ctx};
-      const kg_index* idx = sel[i] ? nullptr : T[i].fk_index;  // an unfiltered table's stored index
+      const kg_index* idx = through_stored(i) ? T[i].fk_index : nullptr;
       if (!idx) {
         if ((st = kg_index_build(ctx, &T[i].cols[T[i].fk_col], sel[i], cnt[i], &built.idx)) != KG_OK) return st;
         idx = built.idx;
       }
-      // A unique id column gives each new row at most one partial tuple, so
-      // t_i's selection bounds the output; other keys retry at the total.
+      // Over a selection with a unique id column each new row joins at most
+      // one partial tuple, so the selection bounds the output; through a
+      // stored index, the average fan-out (rows per referenced row) twice
+      // over. Other cases retry at the total.
       int64_t cap = std::max<int64_t>(cnt[i], 1), total = 0;
+      if (through_stored(i))
+        cap = std::min<int64_t>(cap, ncur * (2 * kg::ceil_div(T[i].n, std::max<int64_t>(T[i - 1].n, 1)) + 2));
       int32_t* out = nullptr;
       if ((st = B.get(cap * (i + 1), &out)) != KG_OK) return st;
       const kg_col* src_key = &T[i - 1].cols[T[i - 1].id_col];
@@ -167,6 +283,8 @@ extern "C" int kg_chain_run(kg_ctx* ctx, const kg_chain* q, int64_t* out_count, 
       if (st != KG_OK) return st;
       cur = out;
       ncur = total;
+      if (through_stored(i) && (T[i].npreds > 0 || T[i].text) && ncur > 0)
+        if ((st = filter_tuples(ctx, B, T[i], i + 1, i, &cur, &ncur)) != KG_OK) return st;
     }
     if ((st = kg_tuple_set_create(ctx, k, std::max<int64_t>(ncur, 1), &set.s)) != KG_OK) return st;
     if (ncur > 0 && (st = kg_tuple_set_insert(ctx, set.s, cur, ncur, nullptr, &ntup)) != KG_OK) return st;

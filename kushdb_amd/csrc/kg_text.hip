@@ -37,22 +37,60 @@ struct Pattern {
   uint8_t p[kMaxPattern];
 };
 
-__device__ __forceinline__ bool text_pred(const Pattern& P, const uint8_t* s, int64_t n) {
-  return text_match(P.op, s, n, P.p, P.len);
+// `s op p` with a LIKE pattern already checked (like_pattern_ok) on the host.
+__device__ __forceinline__ bool text_pred(int32_t op, const uint8_t* s, int64_t n, const uint8_t* p, int32_t m) {
+  return op == KG_TEXT_LIKE ? like_match(s, n, p, m) : text_match(op, s, n, p, m);
 }
 
-__global__ void k_text_flags(kg_text_col col, Pattern P, const int32_t* sel, int64_t n, int32_t* ids,
-                             uint8_t* flags) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t r = sel ? sel[i] : i;
-    bool ok = false;
-    if (!(col.nulls && col.nulls[r])) {
-      const int64_t a = col.offsets[r], b = col.offsets[r + 1];
-      ok = text_pred(P, col.bytes + a, b - a);
+// Heap bytes a workgroup stages in LDS for its 256 rows: the bytes of
+// consecutive rows are one contiguous range of the heap.
+constexpr int kTextTileBytes = 32 * 1024;
+
+// One row per thread. Without a selection (and with a 4-byte aligned heap), a
+// workgroup's rows are consecutive: their heap range is copied to LDS with
+// coalesced dword loads
+// (when it fits kTextTileBytes) and matched there, so the per-character loads
+// of the matching loop hit LDS instead of waiting on global memory. Rows of a
+// selection, and tiles whose strings exceed the LDS range, match from global
+// memory. The pattern is read from LDS.
+__global__ __launch_bounds__(kTBlock) void k_text_flags(kg_text_col col, Pattern P, const int32_t* sel, int64_t n,
+                                                         int32_t* ids, uint8_t* flags) {
+  __shared__ uint32_t tile[kTextTileBytes / 4];
+  __shared__ uint8_t pat[kMaxPattern];
+  for (int j = threadIdx.x; j < P.len; j += kTBlock) pat[j] = P.p[j];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kTBlock;
+  for (int64_t t0 = static_cast<int64_t>(blockIdx.x) * kTBlock; t0 < n; t0 += stride) {
+    const int64_t i = t0 + threadIdx.x;
+    const int64_t tn = n - t0 < kTBlock ? n - t0 : kTBlock;
+    int64_t base = 0;
+    bool staged = false;
+    if (!sel && (reinterpret_cast<uintptr_t>(col.bytes) & 3) == 0) {
+      const int64_t lo = col.offsets[t0], hi = col.offsets[t0 + tn];
+      base = lo & ~int64_t{3};
+      if (hi - base <= kTextTileBytes) {
+        staged = true;
+        const int64_t nfull = (hi - base) / 4;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(col.bytes + base);
+        for (int64_t w = threadIdx.x; w < nfull; w += kTBlock) tile[w] = __builtin_nontemporal_load(src + w);
+        // The last partial word byte by byte: nothing past the heap's end is read.
+        const int64_t rem = (hi - base) - nfull * 4;
+        if (threadIdx.x < rem)
+          reinterpret_cast<uint8_t*>(tile)[nfull * 4 + threadIdx.x] = col.bytes[base + nfull * 4 + threadIdx.x];
+      }
     }
-    ids[i] = static_cast<int32_t>(r);
-    flags[i] = ok ? 1 : 0;
+    __syncthreads();
+    if (i < n) {
+      const int64_t r = sel ? sel[i] : i;
+      bool ok = false;
+      if (!(col.nulls && col.nulls[r])) {
+        const int64_t a = col.offsets[r], b = col.offsets[r + 1];
+        const uint8_t* s = staged ? reinterpret_cast<const uint8_t*>(tile) + (a - base) : col.bytes + a;
+        ok = text_pred(P.op, s, b - a, pat, P.len);
+      }
+      ids[i] = static_cast<int32_t>(r);
+      flags[i] = ok ? 1 : 0;
+    }
+    __syncthreads();
   }
 }
 
@@ -180,8 +218,10 @@ __global__ void k_gather(const uint8_t* src, const int8_t* src_nulls, int w, con
   }
 }
 
+// One item per thread up to 16 workgroups per CU (grid-stride beyond): these
+// kernels are bound by dependent loads, so more waves hide more latency.
 inline int tgrid(kg_ctx* ctx, int64_t work) {
-  int64_t g = ceil_div(std::max<int64_t>(work, 1), kTBlock * 4);
+  int64_t g = ceil_div(std::max<int64_t>(work, 1), kTBlock);
   return static_cast<int>(std::min<int64_t>(g, static_cast<int64_t>(ctx->num_cus) * 16));
 }
 
@@ -310,6 +350,8 @@ extern "C" int kg_text_select(kg_ctx* ctx, const kg_text_col* col, int32_t op, c
   *out_count = 0;
   if (n == 0) return KG_OK;
   KG_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  // A LIKE pattern re2 would reject (not UTF-8) matches nothing.
+  if (op == KG_TEXT_LIKE && !like_pattern_ok(reinterpret_cast<const uint8_t*>(pattern), plen)) return KG_OK;
   Pattern P{};
   P.op = op;
   P.len = plen;
