@@ -528,6 +528,7 @@ constexpr int kSMaxRG = kSMaxTile / (kWave * 4);   // row groups per lane in a w
 struct SGroup {
   int64_t n;                          // table rows
   int32_t nq, ncol, rows, nunits;     // rows: tile rows
+  int32_t tile_bytes;                 // staged bytes of a tile (columns' tiles back to back, 16-byte multiple)
   int32_t q[kSMaxQ];
   const void* col[kSMaxCols];
   int32_t width[kSMaxCols];           // bytes per value (4 or 8)
@@ -539,6 +540,11 @@ struct SGroup {
   int8_t uq[kSWaves], um[kSWaves], ug[kSWaves], urg[kSWaves], ucnt[kSWaves];
   int16_t ur[kSWaves];
 };
+
+// s_waitcnt immediate (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt in
+// 6:4, lgkmcnt in 11:8) that waits for LDS operations only: lgkmcnt(0), with
+// vmcnt and expcnt at their maximum.
+constexpr int kWaitLgkm0 = 0xC07F;
 
 // LDS pointers (address space 3), so that tile reads compile to ds_read.
 typedef __attribute__((address_space(3))) char lds_char;
@@ -1411,7 +1417,15 @@ __device__ __forceinline__ void group_leave(const SGroup& S, const BQuery* qs, b
 // belongs to another group, and at the end. Two workgroups of 8 waves per CU.
 __global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_shared(
     BHeader* h, const BQuery* qs, const SGroup* gs) {
-  __shared__ __attribute__((aligned(16))) char sh[2][kSBufBytes];
+  // Tiles are staged into `stg` with direct global-to-LDS loads and copied to
+  // `cur_tile` (LDS to LDS) before they are computed. The compiler can tell
+  // that the computation's LDS reads (of cur_tile) do not alias the
+  // global-to-LDS loads in flight (into stg), so it does not make them wait
+  // for the next tile's loads: staging overlaps computation. (With one array
+  // of two buffers indexed by parity, every LDS read waited for the whole next
+  // tile.)
+  __shared__ __attribute__((aligned(16))) char stg[kSBufBytes];
+  __shared__ __attribute__((aligned(16))) char cur_tile[kSBufBytes];
   // Tile prefix, rows and tile rows of every group, read once: finding a
   // tile's group and rows then takes LDS reads only.
   __shared__ int32_t s_tpre[kBMaxQueries + 1];
@@ -1451,46 +1465,54 @@ __global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     const int64_t n = s_n[g];
     vr = static_cast<int>(n - lo < rows ? n - lo : rows);
   };
-  // Staging: whole 16-byte chunks with direct global-to-LDS loads (one 1 KB
-  // wave instruction covers 64 chunks; the tile's instructions are dealt round
-  // robin over the waves, across columns), the bytes of a trailing partial
-  // chunk with 4-byte loads; rows past vr are left unwritten (the units never
-  // read them). Lane c holds staging column c's address, width and LDS offset
-  // of group cg, reloaded when the staged group changes.
-  int cg = -1, cncol = 0;
-  const char* ccol = nullptr;
-  int cw = 0, coff = 0;
+  // Staging: 16-byte chunks with direct global-to-LDS loads (one 1 KB wave
+  // instruction covers 64 chunks; the tile's instructions are dealt round
+  // robin over the waves, across columns). The last chunk of a tile's column
+  // may run past the table's last row: the column starts 16-byte aligned
+  // (register-path columns are), so that chunk starts at a 16-byte boundary
+  // and lies in the same 4 KB page as the valid bytes it holds; its extra
+  // bytes land in LDS past the valid rows, inside the column's tile, and are
+  // never read. The staged group's column table lives in LDS (s_col, s_cw,
+  // s_coff, s_ncol), rewritten when the staged group changes: reading it
+  // from global memory per column would make each column wait for the loads
+  // of the previous one, and holding it in vector registers across the loop
+  // spills.
+  __shared__ uint64_t s_col[kSMaxCols];
+  __shared__ int32_t s_cw[kSMaxCols], s_coff[kSMaxCols];
+  __shared__ int32_t s_ncol;
+  int staged_g = -1;
   auto stage = [&](int g, int64_t lo, int vr, char* buf) {
-    if (g != cg) {
+    if (g != staged_g) {  // uniform: every wave stages the same tiles
+      // No wave reads the table here: its last reads were in the previous
+      // stage call, before this iteration's first barrier.
       const SGroup& S = gs[g];
-      cncol = uni(S.ncol);
-      ccol = lane < kSMaxCols ? static_cast<const char*>(S.col[lane]) : nullptr;
-      cw = lane < kSMaxCols ? S.width[lane] : 0;
-      coff = lane < kSMaxCols ? S.off[lane] : 0;
-      cg = g;
+      if (threadIdx.x < kSMaxCols) {
+        s_col[threadIdx.x] = reinterpret_cast<uint64_t>(S.col[threadIdx.x]);
+        s_cw[threadIdx.x] = S.width[threadIdx.x];
+        s_coff[threadIdx.x] = S.off[threadIdx.x];
+      }
+      if (threadIdx.x == 0) s_ncol = S.ncol;
+      __syncthreads();
+      staged_g = g;
     }
+    const int ncol = uni(s_ncol);
     const int rows = s_rows[g];
     int base = 0;
-    for (int c = 0; c < cncol; c++) {
-      const uint64_t pa = reinterpret_cast<uint64_t>(ccol);
-      const uint32_t plo = __builtin_amdgcn_readlane(static_cast<uint32_t>(pa), c);
-      const uint32_t phi = __builtin_amdgcn_readlane(static_cast<uint32_t>(pa >> 32), c);
-      const int w = __builtin_amdgcn_readlane(cw, c);
-      const int off = __builtin_amdgcn_readlane(coff, c);
+    for (int c = 0; c < ncol; c++) {
+      const uint64_t pa = s_col[c];
+      const uint32_t plo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(pa));
+      const uint32_t phi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(pa >> 32));
+      const int w = uni(s_cw[c]);
+      const int off = uni(s_coff[c]);
       const char* src = reinterpret_cast<const char*>((static_cast<uint64_t>(phi) << 32) | plo) + lo * w;
       char* dst = buf + off;
-      const int bytes = vr * w;
-      const int full = bytes >> 4;
+      const int chunks = (vr * w + 15) >> 4;
       const int ninst = (rows * w) >> 10;
       for (int i = (wave - base % kSWaves + kSWaves) % kSWaves; i < ninst; i += kSWaves) {
         const int chunk = i * kWave + lane;
-        if (chunk < full) __builtin_amdgcn_global_load_lds(src + chunk * 16, dst + i * 1024, 16, 0, 0);
+        if (chunk < chunks) __builtin_amdgcn_global_load_lds(src + chunk * 16, dst + i * 1024, 16, 0, 0);
       }
       base += ninst;
-      const int tail = (bytes & 15) >> 2;  // 4-byte words of a partial last chunk
-      if (static_cast<int>(threadIdx.x) < tail)
-        reinterpret_cast<int32_t*>(dst + full * 16)[threadIdx.x] =
-            reinterpret_cast<const int32_t*>(src + full * 16)[threadIdx.x];
     }
   };
   int32_t t = static_cast<int32_t>(blockIdx.x);
@@ -1498,13 +1520,22 @@ __global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(4))) 
   int64_t lo = 0;
   if (t < total) {
     decode(t, g, lo, vr);
-    stage(g, lo, vr, sh[0]);
+    stage(g, lo, vr, stg);
   }
-  for (int k = 0; t < total; k++) {
-    // This tile's staging is complete, and no wave still reads the other buffer.
+  const lds_char* buf = (const lds_char*)cur_tile;
+  while (t < total) {
+    // This tile's staging is complete; move it to the compute buffer, which no
+    // wave still reads.
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    {
+      const int nv = uni(gs[uni(g)].tile_bytes) >> 4;
+      const v4i* src = reinterpret_cast<const v4i*>(stg);
+      v4i* dst = reinterpret_cast<v4i*>(cur_tile);
+      for (int i = threadIdx.x; i < nv; i += kSThreads) dst[i] = src[i];
+    }
     __syncthreads();
     if (g != cur) {
       if (cur >= 0) {
@@ -1524,12 +1555,17 @@ __global__ __launch_bounds__(kSThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     int64_t lon = 0;
     if (tn < total) {
       decode(tn, gn, lon, vrn);
-      stage(gn, lon, vrn, sh[(k + 1) & 1]);
+      stage(gn, lon, vrn, stg);
     }
-    const lds_char* buf = (const lds_char*)(sh[k & 1]);
     __attribute__((address_space(3))) uint32_t* codes = (__attribute__((address_space(3))) uint32_t*)(s_codes);
     cbad |= tile_codes(cd, buf, s_rows[g], vr, codes);
-    __syncthreads();  // the codes are complete
+    // The codes are complete: an LDS-only barrier (a full __syncthreads would
+    // also wait for the next tile's global-to-LDS loads, which must stay in
+    // flight during the units' work).
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (active) {
       switch (u.gc) {
         case 1: unit_scan<1>(u, buf, lo, codes, A); break;
@@ -2085,6 +2121,7 @@ bool layout_groups(STiles& ST, SGroup* gs, const std::vector<int32_t>& ids, cons
       S.off[c] = off;
       off += S.rows * S.width[c];
     }
+    S.tile_bytes = (off + 15) / 16 * 16;  // <= kSBufBytes: a group's row bytes are bounded (kSRowBytes)
     // Units: each query's measures in pairs, then every pair over rs row
     // slices (whole row groups of 4 rows per lane), at most one unit per wave.
     auto pairs = [&](int q) { return std::max(1, (Q[S.q[q]].nm + kSUnitM - 1) / kSUnitM); };
