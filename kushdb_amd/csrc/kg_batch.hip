@@ -738,9 +738,10 @@ __device__ __forceinline__ void unit_scan(const UnitDesc& u, const lds_char* sh,
                                           const __attribute__((address_space(3))) uint32_t* codes, UnitAcc& A) {
   const int nmu = u.nmu;
   const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int j = 0; j < kSMaxRG; j++) {
-    if (j >= u.rg) break;
+  // Row groups one at a time (no unrolling: copies of the measure loop per
+  // row group cost registers and spill).
+#pragma unroll 1
+  for (int j = 0; j < u.rg; j++) {
     const int row = u.r0 + 4 * (j * kWave + lane);
     const uint32_t word = codes[u.q * (kSMaxTile / 4) + row / 4];
     if (word == 0xFFFFFFFFu) continue;  // no qualifying row
