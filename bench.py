@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--serve-caps", type=int, nargs="+", default=[100_000_000, 0],
                     help="batch work caps (rows) tried for the batched mode; 0 = no cap")
     ap.add_argument("--serve-max-batch", type=int, default=128)
+    ap.add_argument("--serve-lanes", type=int, default=1, help="plain lanes of batched serving")
+    ap.add_argument("--serve-lane-scan-rows", type=int, default=0,
+                    help="batched serving: scans of at least this many rows arriving with nothing pending run "
+                         "on an idle plain lane (0: never)")
     ap.add_argument("--serve-arena", type=int, default=1024, help="MiB of device arena per batch slot")
     return ap.parse_args()
 
@@ -636,10 +640,11 @@ def bench_serving(ctx, a):
     def run(mode, rate, cap=0, keep=False):
         arr = np.zeros(nq) if rate is None else np.cumsum(expo / rate)
         res, lat, st = serve(ctx, queries, arr, mode, slots=a.serve_slots, max_batch=a.serve_max_batch,
-                             arena_bytes=a.serve_arena << 20, max_groups=64, max_batch_rows=cap)
+                             arena_bytes=a.serve_arena << 20, max_groups=64, max_batch_rows=cap,
+                             lanes=a.serve_lanes, lane_scan_rows=a.serve_lane_scan_rows)
         out = {"offered": rate, "cap_rows": cap, "qps": st["qps"], "p50_ms": st["p50_s"] * 1e3,
                "p99_ms": st["p99_s"] * 1e3, "mean_batch": st["mean_batch"], "batches": st["batches"],
-               "fallbacks": st["fallbacks"]}
+               "fallbacks": st["fallbacks"], "lane_scans": st["lane_scans"]}
         # Latency by template: [p50, p99] in ms.
         lat_ms = np.asarray(lat[:nq]) * 1e3
         out["by_template"] = {t: [round(float(np.percentile(lat_ms[tpl_of == t], q)), 3) for q in (50, 99)]

@@ -707,8 +707,9 @@ int kg_batch_run(kg_batch* b, const kg_query* qs, int32_t nq, kg_query_result* r
  * max_batch_rows (the first query is always admitted), which bounds how long a
  * batch keeps its slot; chain queries (KG_Q_CHAIN), queries larger than an
  * arena and queries a batch reports unsupported run on the plain path in a
- * lane of their own: a host thread with a second context and stream (kept
- * with ctx for later calls), one query at a time, alongside the batches. Each
+ * lane: a host thread with a context and stream of its own (kept with ctx for
+ * later calls), one query at a time, alongside the batches; with
+ * lane_scan_rows, so do large scans that arrive when nothing is pending. Each
  * query completes as soon as its own work is done (kg_batch_query_ready), not
  * at the end of its batch. */
 enum kg_serve_mode { KG_SERVE_PLAIN = 0, KG_SERVE_BATCHED = 1 };
@@ -716,9 +717,11 @@ typedef struct kg_serve_opts {
   int32_t mode;                 /* kg_serve_mode */
   int32_t slots;                /* batches in flight; 0 = 3 */
   int32_t max_batch;            /* queries per batch; 0 = 128 */
-  int32_t reserved;
+  int32_t lanes;                /* batched mode: plain lanes (host threads with their own contexts); 0 = 1 */
   int64_t arena_bytes;          /* per slot; 0 = 1 GiB */
   int64_t max_batch_rows;       /* projected work cap of a batch: scanned + build rows; 0 = none */
+  int64_t lane_scan_rows;       /* batched mode: a KG_Q_SCAN_AGG query of at least this many rows that arrives
+                                   with no backlog (nothing pending) runs on an idle lane; 0 = never */
 } kg_serve_opts;
 typedef struct kg_serve_stats {
   double wall_s;                /* first arrival to last completion */
@@ -726,7 +729,8 @@ typedef struct kg_serve_stats {
   double p50_s, p99_s, max_s;   /* latency percentiles */
   double mean_batch;            /* queries per batch (batched mode) */
   int64_t batches;
-  int64_t fallbacks;            /* batched mode: queries run on the plain lane */
+  int64_t fallbacks;            /* batched mode: queries run on a plain lane because batches do not take them */
+  int64_t lane_scans;           /* batched mode: large scans sent to an idle lane (lane_scan_rows) */
 } kg_serve_stats;
 int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const double* arrival_s, const kg_serve_opts* opts,
              kg_query_result* res, double* latency_s, kg_serve_stats* stats);

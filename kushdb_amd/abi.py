@@ -109,13 +109,14 @@ class KgBatchOpts(ctypes.Structure):
 
 class KgServeOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("slots", ctypes.c_int32), ("max_batch", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("arena_bytes", ctypes.c_int64), ("max_batch_rows", ctypes.c_int64)]
+                ("lanes", ctypes.c_int32), ("arena_bytes", ctypes.c_int64), ("max_batch_rows", ctypes.c_int64),
+                ("lane_scan_rows", ctypes.c_int64)]
 
 
 class KgServeStats(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("qps", ctypes.c_double), ("p50_s", ctypes.c_double),
                 ("p99_s", ctypes.c_double), ("max_s", ctypes.c_double), ("mean_batch", ctypes.c_double),
-                ("batches", ctypes.c_int64), ("fallbacks", ctypes.c_int64)]
+                ("batches", ctypes.c_int64), ("fallbacks", ctypes.c_int64), ("lane_scans", ctypes.c_int64)]
 
 
 class KgJoinTuning(ctypes.Structure):

@@ -1758,6 +1758,12 @@ struct kg_batch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   hipEvent_t ready = nullptr;  // the context stream's work before a submit
+  // Whether a submit orders the batch after the context stream's work (its
+  // columns may have been written there). The serving loop, which reads only
+  // resident columns and synchronizes the device before its clock starts,
+  // turns it off: a batch must not wait for work another stream shares a
+  // hardware queue with.
+  bool after_ctx = true;
   int grid = 0;
   int grid_gen = 0;
   int grid_shared = 0;
@@ -2218,6 +2224,8 @@ extern "C" int kg_batch_free(kg_batch* b) {
   return KG_OK;
 }
 
+void kg::batch_order_after_ctx(kg_batch* b, bool on) { b->after_ctx = on; }
+
 extern "C" int kg_batch_footprint(const kg_query* q, int64_t* bytes) {
   if (!q || !bytes) return KG_ERR_INVALID;
   *bytes = static_cast<int64_t>(query_footprint(*q));
@@ -2491,8 +2499,10 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     }
   const size_t bytes = round256(sizeof(BHeader)) + sizeof(BQuery) * static_cast<size_t>(nq);
   // Columns written on the context's stream before this call are visible to the batch.
-  KG_HIP_CHECK(ctx, hipEventRecord(b->ready, ctx->stream));
-  KG_HIP_CHECK(ctx, hipStreamWaitEvent(b->stream, b->ready, 0));
+  if (b->after_ctx) {
+    KG_HIP_CHECK(ctx, hipEventRecord(b->ready, ctx->stream));
+    KG_HIP_CHECK(ctx, hipStreamWaitEvent(b->stream, b->ready, 0));
+  }
   KG_HIP_CHECK(ctx, hipMemcpyAsync(b->d_desc, b->h_desc, bytes, hipMemcpyHostToDevice, b->stream));
   if (H->shared.ng)
     KG_HIP_CHECK(ctx, hipMemcpyAsync(b->dg(), b->hg(), sizeof(SGroup) * H->shared.ng, hipMemcpyHostToDevice,

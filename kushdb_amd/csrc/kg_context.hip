@@ -234,7 +234,7 @@ int kg_ctx_create(int device, kg_ctx** out) {
 
 int kg_ctx_destroy(kg_ctx* ctx) {
   if (!ctx) return KG_OK;
-  if (ctx->serve_lane) kg_ctx_destroy(ctx->serve_lane);
+  for (kg_ctx* l : ctx->serve_lanes) kg_ctx_destroy(l);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& pr : ctx->ev_pending) {

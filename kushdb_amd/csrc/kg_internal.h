@@ -78,10 +78,10 @@ struct kg_ctx {
   std::multimap<size_t, void*> block_cache;
   std::unordered_map<void*, size_t> block_live;
   size_t block_cache_bytes = 0;
-  // Context of the batched serving loop's plain lane (kg_serve): queries the
-  // batches do not run execute there, on its own stream and host thread.
-  // Created on first use, destroyed with this context.
-  kg_ctx* serve_lane = nullptr;
+  // Contexts of the batched serving loop's plain lanes (kg_serve): queries
+  // the batches do not run execute there, each lane on its own stream and
+  // host thread. Created on first use, destroyed with this context.
+  std::vector<kg_ctx*> serve_lanes;
 };
 
 // Device memory for per-query structures (join tables, their accumulators):
@@ -161,6 +161,9 @@ struct BatchSource {
   }
 };
 size_t batch_own_bytes(const kg_query& q);
+// Whether kg_batch_submit orders the batch after the context stream's prior
+// work (default on; the serving loop turns it off, see kg_batch).
+void batch_order_after_ctx(kg_batch* b, bool on);
 size_t batch_source_bytes(const kg_query& q);
 bool batch_source_id(const kg_query& q, BatchSource* id);
 

@@ -12,6 +12,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -125,17 +126,22 @@ double percentile(std::vector<double> v, double p) {
   return v[k - 1];
 }
 
-// The batched mode's plain lane: a host thread that runs the queries batches
-// do not take (chain queries, unsupported or oversized ones) one at a time on
-// its own context and stream, while the serving loop keeps admitting and
-// collecting batches.
+// The batched mode's plain lanes: host threads that run the queries batches
+// do not take (chain queries, unsupported or oversized ones, and large scans
+// that arrive with no backlog) one at a time, each on its own context and
+// stream, while the serving loop keeps admitting and collecting batches.
 struct Lane {
   std::mutex m;
   std::condition_variable cv;
   std::deque<int32_t> q;
   bool stop = false;
   std::thread th;
+  std::atomic<int32_t> load{0};  // queued + running
+  kg_ctx* ctx = nullptr;
+  Scratch S;
+  TablePool T;
   void push(int32_t i) {
+    load.fetch_add(1);
     {
       std::lock_guard<std::mutex> lk(m);
       q.push_back(i);
@@ -153,21 +159,24 @@ struct Lane {
   }
 };
 
-// The lane's context: created once per serving context, with the same
-// execution settings (deterministic mode, partitioning choices, join tunes).
-int lane_context(kg_ctx* ctx, kg_ctx** out) {
-  if (!ctx->serve_lane) {
-    const int st = kg_ctx_create(ctx->device, &ctx->serve_lane);
+// Lane contexts: created once per serving context (kept for later calls),
+// with the same execution settings (deterministic mode, partitioning choices,
+// join tunes).
+int lane_contexts(kg_ctx* ctx, int n) {
+  while (static_cast<int>(ctx->serve_lanes.size()) < n) {
+    kg_ctx* l = nullptr;
+    const int st = kg_ctx_create(ctx->device, &l);
     if (st != KG_OK) return kg_fail(ctx, st, "serving lane context: status %d", st);
+    ctx->serve_lanes.push_back(l);
   }
-  kg_ctx* l = ctx->serve_lane;
-  l->deterministic = ctx->deterministic;
-  l->probe_regions = ctx->probe_regions;
-  l->probe_wg_per_cu = ctx->probe_wg_per_cu;
-  l->build_regions = ctx->build_regions;
-  l->agg_regions = ctx->agg_regions;
-  l->join_tunes = ctx->join_tunes;
-  *out = l;
+  for (kg_ctx* l : ctx->serve_lanes) {
+    l->deterministic = ctx->deterministic;
+    l->probe_regions = ctx->probe_regions;
+    l->probe_wg_per_cu = ctx->probe_wg_per_cu;
+    l->build_regions = ctx->build_regions;
+    l->agg_regions = ctx->agg_regions;
+    l->join_tunes = ctx->join_tunes;
+  }
   return KG_OK;
 }
 
@@ -200,20 +209,25 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
   for (kg_query& q : resolved)
     if (q.kind == KG_Q_JOIN_AGG && !(q.load_factor > 0)) q.load_factor = kg::tuned_load_factor(ctx, q.build_n);
   qs = resolved.data();
-  Scratch S, LS;
-  TablePool T, LT;
+  Scratch S;
+  TablePool T;
   std::vector<Slot> slots;
   kg_serve_stats st_{};
-  Lane lane;
-  kg_ctx* lctx = nullptr;
+  const int nlanes = batched ? (opts->lanes > 0 ? opts->lanes : 1) : 0;
+  KG_REQUIRE(ctx, nlanes <= 8, "at most 8 plain lanes");
+  std::vector<std::unique_ptr<Lane>> lanes;
   auto cleanup = [&]() {
-    lane.finish();
+    for (auto& l : lanes) l->finish();
     for (Slot& s : slots)
       if (s.b) kg_batch_free(s.b);
   };
   if (batched) {
-    const int st = lane_context(ctx, &lctx);
+    const int st = lane_contexts(ctx, nlanes);
     if (st != KG_OK) return st;
+    for (int k = 0; k < nlanes; k++) {
+      lanes.emplace_back(new Lane());
+      lanes.back()->ctx = ctx->serve_lanes[k];
+    }
     kg_batch_opts bo{};
     bo.arena_bytes = opts->arena_bytes;
     slots.resize(nslots);
@@ -225,11 +239,15 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
       }
     }
   }
-  // Pre-fault the plain path's code and the batch graphs before the clock starts.
+  // Pre-fault the plain path's code and the batch graphs before the clock
+  // starts. After this synchronization every column is complete, so batches
+  // need not wait for the context stream (which may share a hardware queue
+  // with a slot or a lane).
   if (const hipError_t e = hipDeviceSynchronize(); e != hipSuccess) {
     cleanup();
     return kg_fail(ctx, KG_ERR_HIP, "hipDeviceSynchronize failed: %s", hipGetErrorString(e));
   }
+  for (Slot& sl : slots) kg::batch_order_after_ctx(sl.b, false);
   std::vector<double> done_at(nq, -1.0);
   std::deque<int32_t> pending;
   int32_t next = 0;
@@ -240,35 +258,62 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
     latency_s[i] = done_at[i] - arrival_s[i];
     ndone.fetch_add(1, std::memory_order_release);
   };
-  // Plain mode: in the loop. Batched mode: handed to the lane's thread, which
-  // writes res[i], done_at[i] and latency_s[i] of its queries only.
+  // Plain mode: in the loop. Batched mode: handed to the least loaded lane,
+  // whose thread writes res[i], done_at[i] and latency_s[i] of its queries
+  // only.
+  auto least_loaded = [&]() {
+    Lane* best = lanes[0].get();
+    for (auto& l : lanes)
+      if (l->load.load() < best->load.load()) best = l.get();
+    return best;
+  };
   auto plain = [&](int32_t i) {
     if (batched) {
-      lane.push(i);
+      least_loaded()->push(i);
       return;
     }
     res[i].status = run_plain(ctx, qs[i], res[i], S, T);
     finish(i);
   };
-  if (batched)
-    lane.th = std::thread([&]() {
-      (void)hipSetDevice(lctx->device);
+  for (auto& lp : lanes) {
+    Lane* L = lp.get();
+    L->th = std::thread([&, L]() {
+      (void)hipSetDevice(L->ctx->device);
       for (;;) {
         int32_t i;
         {
-          std::unique_lock<std::mutex> lk(lane.m);
-          lane.cv.wait(lk, [&]() { return lane.stop || !lane.q.empty(); });
-          if (lane.q.empty()) return;
-          i = lane.q.front();
-          lane.q.pop_front();
+          std::unique_lock<std::mutex> lk(L->m);
+          L->cv.wait(lk, [&]() { return L->stop || !L->q.empty(); });
+          if (L->q.empty()) return;
+          i = L->q.front();
+          L->q.pop_front();
         }
-        res[i].status = run_plain(lctx, qs[i], res[i], LS, LT);
+        res[i].status = run_plain(L->ctx, qs[i], res[i], L->S, L->T);
         finish(i);
+        L->load.fetch_sub(1);
       }
     });
+  }
+  // Large scans that arrive with no backlog run plain on an idle lane: a lone
+  // scan gains nothing from a batch (no other query shares its tiles), and the
+  // plain scan kernel streams its table faster than a batch's shared scan.
+  auto to_lane = [&](int32_t i) {
+    if (!batched || opts->lane_scan_rows <= 0 || qs[i].kind != KG_Q_SCAN_AGG || qs[i].n < opts->lane_scan_rows ||
+        !pending.empty())
+      return false;
+    return least_loaded()->load.load() == 0;
+  };
   while (ndone.load(std::memory_order_acquire) < nq) {
     const double now = seconds_since(t0);
-    while (next < nq && arrival_s[next] <= now) pending.push_back(next++);
+    while (next < nq && arrival_s[next] <= now) {
+      const int32_t i = next++;
+      if (to_lane(i)) {
+        st_.lane_scans++;
+        plain(i);
+      } else {
+        pending.push_back(i);
+      }
+    }
     bool progressed = false;
     if (!batched) {
       if (!pending.empty()) {
@@ -380,7 +425,7 @@ extern "C" int kg_serve(kg_ctx* ctx, const kg_query* qs, int32_t nq, const doubl
       }
     }
   }
-  cleanup();  // joins the lane: its writes are visible from here on
+  cleanup();  // joins the lanes: their writes are visible from here on
   std::vector<double> lat(latency_s, latency_s + nq);
   double last = 0;
   for (double d : done_at) last = std::max(last, d);

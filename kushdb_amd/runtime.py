@@ -887,15 +887,18 @@ class Batch:
             pass
 
 
-def serve(ctx, queries, arrivals, mode, slots=3, max_batch=128, arena_bytes=0, max_groups=64, max_batch_rows=0):
+def serve(ctx, queries, arrivals, mode, slots=3, max_batch=128, arena_bytes=0, max_groups=64, max_batch_rows=0,
+          lanes=1, lane_scan_rows=0):
     """Replays `queries` arriving at `arrivals` seconds (ascending). Returns
-    (results as Batch.collect, latencies in s, stats dict)."""
+    (results as Batch.collect, latencies in s, stats dict). Batched mode runs
+    what batches do not take on `lanes` plain lanes, and scans of at least
+    lane_scan_rows rows arriving with nothing pending on an idle lane."""
     n = len(queries)
     qarr = (abi.KgQuery * max(n, 1))(*[q.q for q in queries])
     arr = np.ascontiguousarray(arrivals, dtype=np.float64)
     lat = np.zeros(max(n, 1), dtype=np.float64)
     res, bufs = _result_buffers(queries, max_groups)
-    opts = abi.KgServeOpts(mode, slots, max_batch, 0, arena_bytes, max_batch_rows)
+    opts = abi.KgServeOpts(mode, slots, max_batch, lanes, arena_bytes, max_batch_rows, lane_scan_rows)
     st = abi.KgServeStats()
     ctx.check(ctx.lib.kg_serve(ctx.h, qarr, n, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                ctypes.byref(opts), res, lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
