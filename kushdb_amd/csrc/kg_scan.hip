@@ -159,8 +159,14 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_reg(PredSet P, KeySet K, M
         }
         sum[g][m] = M.is_real[m] ? fadd_bits(sum[g][m], v[m]) : sum[g][m] + v[m];
         if constexpr (MINMAX) {
-          if (fresh || val_less(M.is_real[m], v[m], mn[g][m])) mn[g][m] = v[m];
-          if (fresh || val_less(M.is_real[m], mx[g][m], v[m])) mx[g][m] = v[m];
+          // Written as selects: the equivalent conditional stores
+          // `if (fresh || less) mn = v` were compiled so that rows after a
+          // thread's first grid-stride step never updated MIN/MAX.
+          const uint64_t cur_mn = mn[g][m], cur_mx = mx[g][m];
+          const bool lt = val_less(M.is_real[m], v[m], cur_mn);
+          const bool gt = val_less(M.is_real[m], cur_mx, v[m]);
+          mn[g][m] = (fresh || lt) ? v[m] : cur_mn;
+          mx[g][m] = (fresh || gt) ? v[m] : cur_mx;
         }
       }
     }
@@ -192,8 +198,10 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_reg(PredSet P, KeySet K, M
         }
         if constexpr (MINMAX) {
           const uint64_t olo = shfl_xor_u64(lo, st), ohi = shfl_xor_u64(hi, st);
-          if (oc > 0 && (c == 0 || val_less(M.is_real[m], olo, lo))) lo = olo;
-          if (oc > 0 && (c == 0 || val_less(M.is_real[m], hi, ohi))) hi = ohi;
+          const bool take_lo = oc > 0 && (c == 0 || val_less(M.is_real[m], olo, lo));
+          const bool take_hi = oc > 0 && (c == 0 || val_less(M.is_real[m], hi, ohi));
+          lo = take_lo ? olo : lo;
+          hi = take_hi ? ohi : hi;
         }
         c += oc;
       }
@@ -237,8 +245,10 @@ __global__ __launch_bounds__(kBlock) void k_group_agg_reg(PredSet P, KeySet K, M
       for (int w = 0; w < st; w++) {
         p[w] = M.is_real[m] ? fadd_bits(p[w], p[w + st]) : p[w] + p[w + st];
         if constexpr (MINMAX) {
-          if (pc[w + st] > 0 && (pc[w] == 0 || val_less(M.is_real[m], plo[w + st], plo[w]))) plo[w] = plo[w + st];
-          if (pc[w + st] > 0 && (pc[w] == 0 || val_less(M.is_real[m], phi[w], phi[w + st]))) phi[w] = phi[w + st];
+          const bool take_lo = pc[w + st] > 0 && (pc[w] == 0 || val_less(M.is_real[m], plo[w + st], plo[w]));
+          const bool take_hi = pc[w + st] > 0 && (pc[w] == 0 || val_less(M.is_real[m], phi[w], phi[w + st]));
+          plo[w] = take_lo ? plo[w + st] : plo[w];
+          phi[w] = take_hi ? phi[w + st] : phi[w];
         }
         pc[w] += pc[w + st];
       }
@@ -310,8 +320,10 @@ __global__ __launch_bounds__(64) void k_group_agg_final(const AggOut in, int nbl
       for (int b = 0; b < nblocks; b++) {
         const uint64_t bc = snn[b];
         if (minmax && bc) {
-          if (c == 0 || val_less(real, smn[b], lo)) lo = smn[b];
-          if (c == 0 || val_less(real, hi, smx[b])) hi = smx[b];
+          const bool take_lo = c == 0 || val_less(real, smn[b], lo);
+          const bool take_hi = c == 0 || val_less(real, hi, smx[b]);
+          lo = take_lo ? smn[b] : lo;
+          hi = take_hi ? smx[b] : hi;
         }
         c += bc;
       }

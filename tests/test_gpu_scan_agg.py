@@ -141,3 +141,52 @@ def test_scan_select_vs_oracle(ctx):
     got = ctx.scan_select(cols, preds, conjunction=False).cpu().numpy()
     want = po.scan_select([po.Col(po.DATE, h["day"]), po.Col(po.ENUM, h["g1"])], preds, conjunction=False)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [310, 5000, 700_003])
+@pytest.mark.parametrize("ngroups", [1, 3, 12])
+def test_minmax_extremes_anywhere(ctx, n, ngroups):
+    """MIN / MAX over INT, BIGINT and REAL columns with NULLs, grouped by a
+    dense code (1 group: ungrouped; 3: the register path; 12: the LDS path),
+    with each group's extremes planted at positions that fall in every
+    grid-stride step of a thread (the first row, row 256, the last rows, and
+    random ones), against numpy per group in insertion order."""
+    rng = np.random.default_rng(n * 31 + ngroups)
+    g = rng.integers(0, ngroups, n).astype(np.int32)
+    i32 = rng.integers(-1000, 1000, n).astype(np.int32)
+    i64 = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    f = rng.uniform(-1e6, 1e6, n)
+    nul = (rng.random(n) < 0.05).astype(np.int8)
+    plant = sorted({0, min(256, n - 1), min(257, n - 1), n - 1, n - 2, *rng.integers(0, n, 8).tolist()})
+    for k, p in enumerate(plant):
+        sign = 1 if k % 2 else -1
+        i32[p] = sign * (5000 + k)
+        i64[p] = sign * ((1 << 50) + k)
+        f[p] = sign * (1e9 + k)
+        nul[p] = 0
+    dev = "cuda:0"
+    cols = [Column(abi.KG_ENUM, torch.from_numpy(g).to(dev)),
+            Column(abi.KG_INT, torch.from_numpy(i32).to(dev), torch.from_numpy(nul).to(dev)),
+            Column(abi.KG_BIGINT, torch.from_numpy(i64).to(dev)),
+            Column(abi.KG_REAL, torch.from_numpy(f).to(dev), torch.from_numpy(nul).to(dev))]
+    aggs = [(abi.KG_AGG_MIN, 1), (abi.KG_AGG_MAX, 1), (abi.KG_AGG_MIN, 2), (abi.KG_AGG_MAX, 2),
+            (abi.KG_AGG_MIN, 3), (abi.KG_AGG_MAX, 3)]
+    print("minmax", n, ngroups, flush=True)
+    keys, vals, nulls = ctx.scan_group_agg(cols, [], [(0, ngroups)], aggs)
+    order = [int(x) for x in dict.fromkeys(g.tolist())]  # insertion order
+    assert [int(k) for k in np.asarray(keys).reshape(-1)] == order
+    ok = nul == 0
+    for row, code in enumerate(order):
+        m = g == code
+        want = [i32[m & ok].min(), i32[m & ok].max(), i64[m].min(), i64[m].max()]
+        assert [int(x) for x in vals[row, :4]] == [int(x) for x in want], (n, ngroups, code)
+        assert vals[row, 4:].view(np.float64).tolist() == [f[m & ok].min(), f[m & ok].max()], (n, ngroups, code)
+        assert not nulls[row].any()
+    # The hashed GROUP BY over the same key column.
+    first, hvals, hnulls = ctx.hash_group_agg(cols, [], [0], aggs)
+    hk = g[np.asarray(first)]
+    for row, code in enumerate(hk.tolist()):
+        m = g == code
+        want = [i32[m & ok].min(), i32[m & ok].max(), i64[m].min(), i64[m].max()]
+        assert [int(x) for x in hvals[row, :4]] == [int(x) for x in want], ("hashed", n, ngroups, code)
+        assert hvals[row, 4:].view(np.float64).tolist() == [f[m & ok].min(), f[m & ok].max()], ("hashed", n, code)
