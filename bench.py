@@ -58,6 +58,8 @@ def parse():
                     help="batch work caps (rows) tried for the batched mode; 0 = no cap")
     ap.add_argument("--serve-max-batch", type=int, default=128)
     ap.add_argument("--serve-lanes", type=int, default=1, help="plain lanes of batched serving")
+    ap.add_argument("--serve-hw-queues", type=int, default=8,
+                    help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's value)")
     ap.add_argument("--serve-lane-scan-rows", type=int, default=0,
                     help="batched serving: scans of at least this many rows arriving with nothing pending run "
                          "on an idle plain lane (0: never)")
@@ -751,6 +753,9 @@ def bench_serving(ctx, a):
         "plain_best_within_bar": plain_best, "plain_runs": plain_runs,
         "batched_best_within_bar": bbest, "batched_by_cap": batched,
         "qps_ratio_at_equal_p99": ratio,
+        "serving_config": {"slots": a.serve_slots, "lanes": a.serve_lanes, "max_batch": a.serve_max_batch,
+                           "arena_mib": a.serve_arena, "lane_scan_rows": a.serve_lane_scan_rows,
+                           "GPU_MAX_HW_QUEUES": os.environ.get("GPU_MAX_HW_QUEUES")},
         "results_plain_vs_batched": {"queries": nq, "mismatched": mismatched, "examples": examples,
                                      "bar": "status, group keys/order, counts and integer sums exact; FP64 "
                                             "within 1e-12 relative"},
@@ -772,6 +777,12 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design note, "
               "not a launched configuration.", file=sys.stderr)
         return 2
+    # Hardware queues of this process, read when the HIP runtime starts: batched serving keeps the
+    # context stream, its batch slots and a plain lane busy at once, and with HIP's default of 4
+    # queues some of those streams share a queue and wait for each other's work. Single-stream
+    # workloads (every other measurement, and the plain serving path) do not depend on it.
+    if a.serve_hw_queues > 0:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(a.serve_hw_queues)
     import torch
     from kushdb_amd.runtime import Context
     torch.cuda.init()
