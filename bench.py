@@ -574,6 +574,20 @@ def bench_chain_c3(ctx, a):
         "index_build_once_ms": round(ibuild * 1e3, 2),
         "equal_to_stepwise": bool(sn == tuples.shape[0] and svmin == mn_int and stext == as_b(mn_text))}
     log(f"C3 native plan, stored fk indexes: {swall_i * 1e3:.2f} ms ({out['native_plan_stored_indexes']})")
+    # The reference's choice per child: the stored index of an unfiltered child
+    # (t2, t3), an index built per query over a filtered child's selection (t1, t4).
+    qr = chain_query_for(tabs, "a", "%e_a%", 50, indexes=[None, None, idx[2], idx[3], None])
+    chain_run(ctx, qr)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rn, rvmin, rrow = chain_run(ctx, qr)
+    rwall = (time.perf_counter() - t0) / steps
+    rtext = bytes(htabs[0]["name_heap"][htabs[0]["name_off"][rrow]:htabs[0]["name_off"][rrow + 1]]) \
+        if rrow is not None else None
+    out["native_plan_stored_unfiltered"] = {
+        "query_ms": round(rwall * 1e3, 3), "rows_per_s": n_in / rwall,
+        "equal_to_stepwise": bool(rn == tuples.shape[0] and rvmin == mn_int and rtext == as_b(mn_text))}
+    log(f"C3 native plan, stored indexes of the unfiltered children: {rwall * 1e3:.2f} ms")
     # The same query with the join order chosen per episode by the UCT agent (kg_skinner_join).
     from kushdb_amd.chain import skinner_chain_query
     for budget in () if a.no_skinner else (1 << 20, 1 << 22):

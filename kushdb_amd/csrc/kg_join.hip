@@ -348,16 +348,21 @@ __device__ __forceinline__ void probe_one(const TableView& t, const ProbeOut& o,
 
 // The same for a narrow table: each lane of the group holds 4 of the bucket's
 // 16 offsets (16 bytes); slot order within the bucket is lane-major.
+// kv0_valid = false: kv0 is not used, bucket b is read from memory (a chain
+// continued past its first bucket); max_buckets bounds the buckets visited.
 template <int MODE, bool FIRST>
 __device__ __forceinline__ void probe_one32(const TableView& t, const ProbeOut& o, int64_t r, int64_t vi, uint32_t k32,
                                             bool active, uint64_t b, uint4 kv0, int sub, int grp,
-                                            unsigned long long& local) {
+                                            unsigned long long& local, bool kv0_valid = true,
+                                            uint64_t max_buckets = ~uint64_t{0}) {
   const uint32_t* keys = keys32(t);
   const uint64_t nb = nbuckets32(t);
+  const uint64_t limit = max_buckets < nb ? max_buckets : nb;
   unsigned nmatch = 0;
   int64_t first_slot = -1;
   uint64_t scanned = 0;
-  bool first_bucket = true;
+  bool first_bucket = kv0_valid;
+  if (limit == 0) active = false;
   while (__any(active)) {
     bool m0 = false, m1 = false, m2 = false, m3 = false, e = false;
     int64_t slot0 = 0;
@@ -405,13 +410,67 @@ __device__ __forceinline__ void probe_one32(const TableView& t, const ProbeOut& 
     }
     if (active) {
       scanned++;
-      if (ge || scanned >= nb) active = false;
+      if (ge || scanned >= limit) active = false;
       b = (b + 1 == nb) ? 0 : b + 1;
     }
   }
   if (MODE == kAgg && sub == 0 && nmatch)
     agg_add<FIRST>(o, first_slot, static_cast<int64_t>(static_cast<uint64_t>(t.base) + k32),
                    static_cast<unsigned long long>(o.vals[vi]) * nmatch, nmatch, static_cast<int32_t>(r));
+}
+
+// Emitting probes of one wave step into a narrow table: the first buckets of
+// all U probes are already loaded; the row ids of every first-bucket match are
+// loaded together (U * 4 independent reads in flight instead of one probe's at
+// a time), one wave-aggregated append reserves the step's output, and only
+// chains that run past a full first bucket continue bucket by bucket.
+template <int U>
+__device__ __forceinline__ void emit_first32(const TableView& t, const ProbeOut& o, const int64_t (&r)[U],
+                                             const uint32_t (&k32)[U], const bool (&active)[U],
+                                             const uint64_t (&b)[U], const uint4 (&kv)[U], int sub, int grp,
+                                             unsigned long long& local) {
+  const uint64_t nb = nbuckets32(t);
+  const unsigned sh = grp * kProbeLanes;
+  int32_t rows[U][4];
+  unsigned mask[U];
+  bool more[U];
+  unsigned want = 0;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    unsigned mk = 0;
+    bool e = false;
+    if (active[u]) {
+      mk = (kv[u].x == k32[u] ? 1u : 0u) | (kv[u].y == k32[u] ? 2u : 0u) | (kv[u].z == k32[u] ? 4u : 0u) |
+           (kv[u].w == k32[u] ? 8u : 0u);
+      e = kv[u].x == kEmpty32 || kv[u].y == kEmpty32 || kv[u].z == kEmpty32 || kv[u].w == kEmpty32;
+    }
+    const unsigned ge = (__ballot(e) >> sh) & 0xF;
+    more[u] = active[u] && ge == 0;
+    mask[u] = mk;
+    want += __popc(mk);
+    const int64_t slot0 = static_cast<int64_t>(b[u]) * kSlots32 + sub * 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) rows[u][q] = ((mk >> q) & 1u) ? t.rows[slot0 + q] : 0;
+  }
+  unsigned long long pos = wave_append(o.count, want);
+  const unsigned long long cap = static_cast<unsigned long long>(o.cap);
+#pragma unroll
+  for (int u = 0; u < U; u++)
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if ((mask[u] >> q) & 1u) {
+        if (pos < cap) {
+          o.out_probe[pos] = static_cast<int32_t>(r[u]);
+          o.out_build[pos] = rows[u][q];
+        }
+        pos++;
+      }
+  // Chains past a full first bucket: the following buckets, at most nb - 1 of
+  // them (the first bucket is not visited again).
+#pragma unroll
+  for (int u = 0; u < U; u++)
+    probe_one32<kEmit, false>(t, o, r[u], r[u], k32[u], more[u], b[u] + 1 == nb ? 0 : b[u] + 1, kv[u], sub, grp,
+                              local, false, nb - 1);
 }
 
 // Probe kernel: every 4-lane group handles kUnroll probes per wave step; the
@@ -546,9 +605,13 @@ __global__ __launch_bounds__(kJBlock) void k_ht_probe(TableView t, kg_col key, c
           kv[u] = *reinterpret_cast<const uint4*>(keys32(t) + static_cast<int64_t>(b[u]) * kSlots32 + sub * 4);
       }
       if (base + nwaves * kStep < hi) load_step(base + nwaves * kStep);
+      if constexpr (MODE == kEmit) {
+        emit_first32<kUnroll>(t, o, r, k32, active, b, kv, sub, grp, local);
+      } else {
 #pragma unroll
-      for (int u = 0; u < kUnroll; u++)
-        probe_one32<MODE, FIRST>(t, o, r[u], v[u], k32[u], active[u], b[u], kv[u], sub, grp, local);
+        for (int u = 0; u < kUnroll; u++)
+          probe_one32<MODE, FIRST>(t, o, r[u], v[u], k32[u], active[u], b[u], kv[u], sub, grp, local);
+      }
     } else {
       longlong2 kv[kUnroll];
 #pragma unroll
