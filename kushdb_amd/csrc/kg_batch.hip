@@ -1087,6 +1087,13 @@ __device__ __forceinline__ void ld4_jmeasure(int kind, int fkind, const void* co
   }
 }
 
+// Element e (< 4) of a 4-element register array, chosen by selects: a
+// runtime index into the array itself would move it to scratch memory.
+template <typename T>
+__device__ __forceinline__ T pick4(const T* a, int e) {
+  return e == 0 ? a[0] : e == 1 ? a[1] : e == 2 ? a[2] : a[3];
+}
+
 template <bool FULL, bool ONEG>
 __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int64_t hi, int slot, uint64_t* lds) {
   const BJoinFast& J = d.J;
@@ -1112,130 +1119,169 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
   const bool real0 = J.mkind[0] == 1 || J.fkind[0] != 0;
   const bool real1 = J.mkind[1] == 1 || J.fkind[1] != 0;
   uint64_t seen = 0;  // groups (< 64) whose first row this lane has reported
-#pragma unroll 1
-  for (int k = 0; k < kWRowGroups; k++) {
-    const int64_t base = lo + 4 * lane + static_cast<int64_t>(k) * kWave * 4;
-    const int nv = FULL ? 4 : rows_left(hi, base);
-    if (!FULL && __builtin_amdgcn_readfirstlane(__ballot(nv > 0) != 0 ? 1 : 0) == 0) break;
+  // A row group's streamed values: 4 consecutive rows per lane.
+  struct JRows {
+    int64_t base;
     bool pass[4];
+    int64_t key[4];
+    uint64_t v0[4], v1[4];
+  };
+  // False when no lane of the wave has rows left (tail items).
+  auto load_rows = [&](int k, JRows& R) -> bool {
+    R.base = lo + 4 * lane + static_cast<int64_t>(k) * kWave * 4;
+    const int64_t base = R.base;
+    const int nv = FULL ? 4 : rows_left(hi, base);
+    if (!FULL && __builtin_amdgcn_readfirstlane(__ballot(nv > 0) != 0 ? 1 : 0) == 0) return false;
 #pragma unroll
-    for (int e = 0; e < 4; e++) pass[e] = e < nv;
+    for (int e = 0; e < 4; e++) R.pass[e] = e < nv;
 #pragma unroll
     for (int p = 0; p < kBFastPreds; p++) {
       if (p >= J.npred) break;
       int32_t x[4];
       ld4_i32<FULL>(J.pcol[p], base, nv, x);
 #pragma unroll
-      for (int e = 0; e < 4; e++) pass[e] = pass[e] && cmp_i64(J.op[p], x[e], J.lit[p]);
+      for (int e = 0; e < 4; e++) R.pass[e] = R.pass[e] && cmp_i64(J.op[p], x[e], J.lit[p]);
     }
-    int64_t key[4];
     if (J.key64) {
-      ld4_i64<FULL>(static_cast<const int64_t*>(J.key), base, nv, key);
+      ld4_i64<FULL>(static_cast<const int64_t*>(J.key), base, nv, R.key);
     } else {
       int32_t k32[4];
       ld4_i32<FULL>(static_cast<const int32_t*>(J.key), base, nv, k32);
 #pragma unroll
-      for (int e = 0; e < 4; e++) key[e] = k32[e];
-    }
-    uint64_t v0[4] = {0, 0, 0, 0}, v1[4] = {0, 0, 0, 0};
-    if (nm > 0) ld4_jmeasure<FULL>(J.mkind[0], J.fkind[0], J.mcol[0], J.fcol[0], base, nv, v0);
-    if (nm > 1) ld4_jmeasure<FULL>(J.mkind[1], J.fkind[1], J.mcol[1], J.fcol[1], base, nv, v1);
-#pragma unroll
-    for (int e = 0; e < 4; e++) pass[e] = pass[e] && key[e] != kEmpty;
-    // Adds probe row `row` matched with build row brow of group g.
-    auto consume_g = [&](int64_t row, int32_t brow, int g, uint64_t a0, uint64_t a1) {
-      if (ONEG) {
-        rc++;
-        const uint64_t fr = (static_cast<uint64_t>(row) << 32) | static_cast<uint32_t>(brow);
-        rfirst = fr < rfirst ? fr : rfirst;
-        if (nm > 0) rs0 = real0 ? fadd_bits(rs0, a0) : rs0 + a0;
-        if (nm > 1) rs1 = real1 ? fadd_bits(rs1, a1) : rs1 + a1;
-        return;
-      }
-      atomicAdd(reinterpret_cast<unsigned long long*>(&lrows[g]), 1ull);
-      if (g >= 64 || !((seen >> g) & 1ull)) {
-        atomicMin(reinterpret_cast<unsigned long long*>(&lfirst[g]),
-                  (static_cast<unsigned long long>(row) << 32) | static_cast<uint32_t>(brow));
-        if (g < 64) seen |= 1ull << g;
-      }
-      if (nm > 0) {
-        if (real0) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm]), __longlong_as_double(a0));
-        else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm]), a0);
-      }
-      if (nm > 1) {
-        if (real1) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm + 1]), __longlong_as_double(a1));
-        else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm + 1]), a1);
-      }
-    };
-    if (use_map) {
-      // Direct map: one 4-byte entry per probe, the four loaded together.
-      uint32_t me[4];
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const uint64_t off = static_cast<uint64_t>(key[e]) - static_cast<uint64_t>(mn);
-        me[e] = (pass[e] && off < static_cast<uint64_t>(span)) ? d.map[off] : kMapEmpty;
-      }
-#pragma unroll
-      for (int e = 0; e < 4; e++)
-        if (me[e] != kMapEmpty && build_recheck(d, key[e], static_cast<int32_t>(me[e] & 0xFFFFFFu)))
-          consume_g(base + e, static_cast<int32_t>(me[e] & 0xFFFFFFu), static_cast<int>(me[e] >> 24), v0[e], v1[e]);
-      continue;
-    }
-    if (d.map_only) continue;  // the map is off and there is no table: the query reports it
-    // First buckets of the passing rows, loaded together.
-    uint64_t b[4];
-    longlong2 kv[4][kSlots / 2];
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      b[e] = pass[e] ? bucket_of(key[e], t.nbuckets) : 0;
-#pragma unroll
-      for (int q = 0; q < kSlots / 2; q++)
-        kv[e][q] = pass[e] ? reinterpret_cast<const longlong2*>(t.keys + b[e] * kSlots)[q]
-                           : make_longlong2(kEmpty, kEmpty);
+      for (int e = 0; e < 4; e++) R.key[e] = k32[e];
     }
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      if (!pass[e]) continue;
-      const int64_t row = base + e;
-      auto consume = [&](int32_t brow) {
-        if (!build_recheck(d, key[e], brow)) return;
-        const int g = d.codes ? d.codes[brow] : 0;
-        if (static_cast<unsigned>(g) >= static_cast<unsigned>(G)) {
-          atomicOr(d.flag, kFlagBadCode);
+    for (int e = 0; e < 4; e++) R.v0[e] = R.v1[e] = 0;
+    if (nm > 0) ld4_jmeasure<FULL>(J.mkind[0], J.fkind[0], J.mcol[0], J.fcol[0], base, nv, R.v0);
+    if (nm > 1) ld4_jmeasure<FULL>(J.mkind[1], J.fkind[1], J.mcol[1], J.fcol[1], base, nv, R.v1);
+#pragma unroll
+    for (int e = 0; e < 4; e++) R.pass[e] = R.pass[e] && R.key[e] != kEmpty;
+    return true;
+  };
+#pragma unroll 1
+  for (int k = 0; k < kWRowGroups; k++) {
+    JRows cur;
+    if (!load_rows(k, cur)) break;
+    const int64_t base = cur.base;
+    const bool* pass = cur.pass;
+    const int64_t* key = cur.key;
+    const uint64_t* v0 = cur.v0;
+    const uint64_t* v1 = cur.v1;
+    // One row group (returns early where the group's rows are done).
+    auto step = [&]() {
+      // Adds probe row `row` matched with build row brow of group g.
+      auto consume_g = [&](int64_t row, int32_t brow, int g, uint64_t a0, uint64_t a1) {
+        if (ONEG) {
+          rc++;
+          const uint64_t fr = (static_cast<uint64_t>(row) << 32) | static_cast<uint32_t>(brow);
+          rfirst = fr < rfirst ? fr : rfirst;
+          if (nm > 0) rs0 = real0 ? fadd_bits(rs0, a0) : rs0 + a0;
+          if (nm > 1) rs1 = real1 ? fadd_bits(rs1, a1) : rs1 + a1;
           return;
         }
-        consume_g(row, brow, g, v0[e], v1[e]);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&lrows[g]), 1ull);
+        if (g >= 64 || !((seen >> g) & 1ull)) {
+          atomicMin(reinterpret_cast<unsigned long long*>(&lfirst[g]),
+                    (static_cast<unsigned long long>(row) << 32) | static_cast<uint32_t>(brow));
+          if (g < 64) seen |= 1ull << g;
+        }
+        if (nm > 0) {
+          if (real0) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm]), __longlong_as_double(a0));
+          else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm]), a0);
+        }
+        if (nm > 1) {
+          if (real1) atomicAdd(reinterpret_cast<double*>(&lsum[g * nm + 1]), __longlong_as_double(a1));
+          else atomicAdd(reinterpret_cast<unsigned long long*>(&lsum[g * nm + 1]), a1);
+        }
       };
-      uint32_t hit = 0;
-      bool empty = false;
+      if (use_map) {
+        // Direct map: one 4-byte entry per probe, the four loaded together.
+        uint32_t me[4];
 #pragma unroll
-      for (int q = 0; q < kSlots / 2; q++) {
-        hit |= (kv[e][q].x == key[e] ? 1u : 0u) << (2 * q);
-        hit |= (kv[e][q].y == key[e] ? 1u : 0u) << (2 * q + 1);
-        empty |= kv[e][q].x == kEmpty || kv[e][q].y == kEmpty;
-      }
-      while (hit) {
-        const int sidx = __builtin_ctz(hit);
-        hit &= hit - 1;
-        consume(t.rows[b[e] * kSlots + sidx]);
-      }
-      if (!empty) {
-        uint64_t bb = (b[e] + 1 == t.nbuckets) ? 0 : b[e] + 1;
-        for (uint64_t scanned = 1; scanned < t.nbuckets; scanned++) {
-          const longlong2* bk = reinterpret_cast<const longlong2*>(t.keys + bb * kSlots);
-          bool emp = false;
+        for (int e = 0; e < 4; e++) {
+          const uint64_t off = static_cast<uint64_t>(key[e]) - static_cast<uint64_t>(mn);
+          me[e] = (pass[e] && off < static_cast<uint64_t>(span)) ? d.map[off] : kMapEmpty;
+        }
 #pragma unroll
-          for (int q = 0; q < kSlots / 2; q++) {
-            const longlong2 x = bk[q];
-            if (x.x == key[e]) consume(t.rows[bb * kSlots + 2 * q]);
-            if (x.y == key[e]) consume(t.rows[bb * kSlots + 2 * q + 1]);
-            emp |= x.x == kEmpty || x.y == kEmpty;
+        for (int e = 0; e < 4; e++)
+          if (me[e] != kMapEmpty && build_recheck(d, key[e], static_cast<int32_t>(me[e] & 0xFFFFFFu)))
+            consume_g(base + e, static_cast<int32_t>(me[e] & 0xFFFFFFu), static_cast<int>(me[e] >> 24), v0[e], v1[e]);
+        return;
+      }
+      if (d.map_only) return;  // the map is off and there is no table: the query reports it
+      // First buckets of the passing rows, loaded together.
+      uint64_t b[4];
+      longlong2 kv[4][kSlots / 2];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        b[e] = pass[e] ? bucket_of(key[e], t.nbuckets) : 0;
+#pragma unroll
+        for (int q = 0; q < kSlots / 2; q++)
+          kv[e][q] = pass[e] ? reinterpret_cast<const longlong2*>(t.keys + b[e] * kSlots)[q]
+                             : make_longlong2(kEmpty, kEmpty);
+      }
+      // Matching slots of the first buckets (bit kSlots e + s: row e, slot s)
+      // and the rows whose first bucket is full, so their chain goes on.
+      static_assert(4 * kSlots <= 32, "four rows' slot bits fit one word");
+      uint32_t hits = 0, more = 0;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        uint32_t h = 0;
+        bool empty = false;
+#pragma unroll
+        for (int q = 0; q < kSlots / 2; q++) {
+          h |= (kv[e][q].x == key[e] ? 1u : 0u) << (2 * q);
+          h |= (kv[e][q].y == key[e] ? 1u : 0u) << (2 * q + 1);
+          empty |= kv[e][q].x == kEmpty || kv[e][q].y == kEmpty;
+        }
+        hits |= (pass[e] ? h : 0u) << (kSlots * e);
+        more |= (pass[e] && !empty ? 1u : 0u) << e;
+      }
+      // Rows in ascending order, each one's first-bucket matches before its
+      // chain; the row's values are selected, not indexed, so they stay in
+      // registers.
+#pragma unroll 1
+      for (int e = 0; e < 4; e++) {
+        uint32_t h = (hits >> (kSlots * e)) & ((1u << kSlots) - 1u);
+        const bool chain = (more >> e) & 1u;
+        if (h == 0 && !chain) continue;
+        const int64_t row = base + e;
+        const int64_t ke = pick4(key, e);
+        const uint64_t be = pick4(b, e);
+        const uint64_t a0 = pick4(v0, e), a1 = pick4(v1, e);
+        auto consume = [&](int32_t brow) {
+          if (!build_recheck(d, ke, brow)) return;
+          const int g = d.codes ? d.codes[brow] : 0;
+          if (static_cast<unsigned>(g) >= static_cast<unsigned>(G)) {
+            atomicOr(d.flag, kFlagBadCode);
+            return;
           }
-          if (emp) break;
-          bb = (bb + 1 == t.nbuckets) ? 0 : bb + 1;
+          consume_g(row, brow, g, a0, a1);
+        };
+        while (h) {
+          const int sidx = __builtin_ctz(h);
+          h &= h - 1;
+          consume(t.rows[be * kSlots + sidx]);
+        }
+        if (chain) {
+          uint64_t bb = (be + 1 == t.nbuckets) ? 0 : be + 1;
+          for (uint64_t scanned = 1; scanned < t.nbuckets; scanned++) {
+            const longlong2* bk = reinterpret_cast<const longlong2*>(t.keys + bb * kSlots);
+            bool emp = false;
+#pragma unroll
+            for (int q = 0; q < kSlots / 2; q++) {
+              const longlong2 x = bk[q];
+              if (x.x == ke) consume(t.rows[bb * kSlots + 2 * q]);
+              if (x.y == ke) consume(t.rows[bb * kSlots + 2 * q + 1]);
+              emp |= x.x == kEmpty || x.y == kEmpty;
+            }
+            if (emp) break;
+            bb = (bb + 1 == t.nbuckets) ? 0 : bb + 1;
+          }
         }
       }
-    }
+    };
+    step();
   }
   if (ONEG) {
     for (int st = kWave / 2; st >= 1; st >>= 1) {
