@@ -20,8 +20,10 @@
 // shared scans: the queries over one table form groups whose tiles (512 or
 // 1024 rows) are dealt round robin to the workgroups; the group's columns are
 // staged once into LDS and every query of the group reads the tile from
-// there. Other scan/probe work items are (query, tile of kBTile rows); queries
-// over the same table share its tiles through L2: the items of tile k of a table are adjacent and
+// there. Other scan/probe work items are (query, tile of 256 to kBTile rows,
+// one size per queue, smaller when the queue's queries hold few rows in all);
+// queries over the same table share its tiles through L2: the items of tile k
+// of a table are adjacent and
 // sit in queue k mod 8, one queue per XCD (workgroup b starts on queue b mod 8,
 // the XCD the dispatcher places it on). A workgroup drains its own queue, then
 // the others, and exits when all eight are empty; there is no inter-workgroup
@@ -43,11 +45,16 @@
 
 namespace kg {
 
-constexpr int kBTile = 16384;         // rows per scan/probe work item
+constexpr int kBTile = 16384;         // most rows per scan/probe work item
+constexpr int kBTileMin = 256;        // fewest rows per scan/probe work item (one row group per lane)
+constexpr int64_t kBItemTarget = 2048;  // fewest scan/probe items a queue aims for (before kBTileMin)
 constexpr int kBBuildTile = 512;      // rows per build work item
 constexpr int kBThreads = 256;        // threads per workgroup
 constexpr int kBWaves = kBThreads / kWave;
 constexpr int kBQueues = 8;           // scan/probe work queues (one per XCD)
+constexpr int kRangeCopies = 16;      // copies of a direct map's key range, one 128-byte line each
+constexpr int kRangeStride = 16;      // int64 words between copies
+constexpr size_t kRangeBytes = sizeof(int64_t) * kRangeCopies * kRangeStride;
 constexpr int kBSlots = 64;           // accumulator copies per query (8 per XCD)
 constexpr int kBMaxQueries = 256;     // queries per batch
 constexpr int kBRegG = 8;             // register path: at most this many groups
@@ -116,7 +123,8 @@ struct BQuery {
   // Direct map of a typed join with a small build side: entry key - min for
   // keys in [min, min + msize) = group code << 24 | build row (kMapEmpty: no
   // row), usable when the qualifying build keys are unique and span at most
-  // msize values. krange = {-min, max} of the qualifying build keys.
+  // msize values. krange = {-min, max} of the qualifying build keys, held in
+  // kRangeCopies copies kRangeStride words apart (see k_batch_range).
   uint32_t* map;
   int64_t* krange;
   int32_t msize;
@@ -172,11 +180,11 @@ struct STiles {
 
 struct BQueues {
   int32_t nt;
+  int32_t tile;                               // rows per item (a power of two, kBTileMin..kBTile)
   int32_t qpre[kBQueues][kBMaxQueries + 1];   // per queue, item prefix over tables
   int32_t toff[kBMaxQueries + 1];             // table t's queries: tq[toff[t] .. toff[t+1])
   int32_t tq[kBMaxQueries];
   int64_t tn[kBMaxQueries];                   // rows of table t
-  unsigned int heads[kBQueues];
 };
 
 // Batch header, copied to the device with the query descriptors.
@@ -250,8 +258,10 @@ __device__ __forceinline__ bool build_row_passes(const BHeader* h, const BQuery*
 // Key range of a direct-map join: min and max of the qualifying build keys,
 // over the build items (before the build phase). Items are dealt by a grid
 // stride (no shared counter); each workgroup reduces its item's range and
-// touches the query's two words only when it widens them, so the atomics do
-// not queue up on one address.
+// touches the words of copy blockIdx mod kRangeCopies, each copy on its own
+// cache line, only when it widens them. Every workgroup starts at once and
+// finds the initial range, so with one copy all their atomics would queue on
+// one line.
 __global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQuery* qs) {
   __shared__ int64_t s_red[2][kBWaves];
   const int32_t total = h->rpre[h->nrq];
@@ -287,10 +297,11 @@ __global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQu
         mx = s_red[1][w] > mx ? s_red[1][w] : mx;
       }
       if (mx != INT64_MIN) {
-        if (nmin > static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[0]))))
-          atomicMax(reinterpret_cast<long long*>(&d.krange[0]), static_cast<long long>(nmin));
-        if (mx > static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[1]))))
-          atomicMax(reinterpret_cast<long long*>(&d.krange[1]), static_cast<long long>(mx));
+        int64_t* kr = d.krange + static_cast<int64_t>(blockIdx.x % kRangeCopies) * kRangeStride;
+        if (nmin > static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&kr[0]))))
+          atomicMax(reinterpret_cast<long long*>(&kr[0]), static_cast<long long>(nmin));
+        if (mx > static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&kr[1]))))
+          atomicMax(reinterpret_cast<long long*>(&kr[1]), static_cast<long long>(mx));
       }
     }
     __syncthreads();
@@ -298,11 +309,24 @@ __global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQu
 }
 
 // Whether a direct-map join can use its map (range known, within the map, no
-// repeated key); *mn and *span receive the map's base key and span.
+// repeated key); *mn and *span receive the map's base key and span. Called by
+// all 64 lanes of a wave (lane c reads range copy c).
 __device__ __forceinline__ bool map_usable(const BQuery& d, int64_t* mn, int64_t* span) {
   if (!d.map) return false;
-  const int64_t nmin = static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[0])));
-  const int64_t mx = static_cast<int64_t>(coherent_load(reinterpret_cast<const uint64_t*>(&d.krange[1])));
+  const int lane = threadIdx.x & (kWave - 1);
+  uint64_t a = static_cast<uint64_t>(INT64_MIN), b = static_cast<uint64_t>(INT64_MIN);
+  if (lane < kRangeCopies) {
+    a = coherent_load(reinterpret_cast<const uint64_t*>(d.krange + static_cast<int64_t>(lane) * kRangeStride));
+    b = coherent_load(reinterpret_cast<const uint64_t*>(d.krange + static_cast<int64_t>(lane) * kRangeStride + 1));
+  }
+  for (int st = kRangeCopies / 2; st >= 1; st >>= 1) {
+    const int64_t x = static_cast<int64_t>(shfl_xor_u64(a, st));
+    const int64_t y = static_cast<int64_t>(shfl_xor_u64(b, st));
+    a = static_cast<uint64_t>(x > static_cast<int64_t>(a) ? x : static_cast<int64_t>(a));
+    b = static_cast<uint64_t>(y > static_cast<int64_t>(b) ? y : static_cast<int64_t>(b));
+  }
+  const int64_t nmin = static_cast<int64_t>(__shfl(static_cast<unsigned long long>(a), 0));
+  const int64_t mx = static_cast<int64_t>(__shfl(static_cast<unsigned long long>(b), 0));
   const uint32_t fl = __hip_atomic_load(d.sflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (fl & kFlagNoMap) return false;
   if (mx == INT64_MIN) {  // no qualifying build row: every probe misses
@@ -434,10 +458,9 @@ __device__ __forceinline__ void visit_rows(const BQuery& d, int64_t lo, int64_t 
 }
 
 // Vector-load helpers of the typed join path: lane l owns the row groups
-// lo + 4 l + 256 k (k < kWRowGroups) of 4 consecutive rows of a kBTile item,
+// lo + 4 l + 256 k (k < item rows / 256) of 4 consecutive rows of an item,
 // read with 16-byte vector loads (FULL) or element by element at the end of a
 // table.
-constexpr int kWRowGroups = kBTile / (kWave * 4);   // row groups per lane
 
 template <bool FULL>
 __device__ __forceinline__ void ld4_i32(const int32_t* col, int64_t base, int nv, int32_t (&o)[4]) {
@@ -1047,7 +1070,8 @@ __device__ __forceinline__ void item_join(const BQuery& d, int64_t lo, int64_t h
 }
 
 // Typed join items (one wave per item): lane l owns the row groups
-// lo + 4 l + 256 k (k < 16) of 4 consecutive rows. Per row group, the
+// lo + 4 l + 256 k (k < ngroups, the item's rows / 256) of 4 consecutive rows.
+// Per row group, the
 // predicate, key and measure columns are read with 16-byte vector loads, then
 // the first buckets of the passing rows are loaded together, then their
 // matches' row ids and group codes. Aggregation into the wave's LDS table: row
@@ -1095,7 +1119,8 @@ __device__ __forceinline__ T pick4(const T* a, int e) {
 }
 
 template <bool FULL, bool ONEG>
-__device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int64_t hi, int slot, uint64_t* lds) {
+__device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int64_t hi, int slot, uint64_t* lds,
+                                               int ngroups) {
   const BJoinFast& J = d.J;
   const int G = d.G, nm = d.nm;
   const int lane = threadIdx.x & (kWave - 1);
@@ -1159,7 +1184,7 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
     return true;
   };
 #pragma unroll 1
-  for (int k = 0; k < kWRowGroups; k++) {
+  for (int k = 0; k < ngroups; k++) {
     JRows cur;
     if (!load_rows(k, cur)) break;
     const int64_t base = cur.base;
@@ -1393,34 +1418,31 @@ __device__ __forceinline__ int acc_slot() {
   return static_cast<int>(blockIdx.x % kBQueues) + kBQueues * (wave % (kBSlots / kBQueues));
 }
 
-// Wave-level form of drain(): each wave takes items on its own.
+// Scan/probe items of one queue set, one wave per item, dealt with no shared
+// counter: the waves of XCD x (workgroups b with b mod 8 = x) take the items of
+// queue x, wave j of the XCD taking items j, j + W, j + 2W, ... (W waves on the
+// XCD), so every item is taken once. Workgroups are dispatched in order as
+// earlier ones finish, so waves that start late take what is left. (Dealing
+// through an atomic head per queue queued thousands of atomics on the heads'
+// cache line when every wave of the grid arrived at once.) Needs a grid of at
+// least kBQueues workgroups, which the host checks.
 template <typename Run>
-__device__ __forceinline__ void drain_wave(BQueues& Q, const BQuery* qs, Run&& run) {
+__device__ __forceinline__ void drain_wave(const BQueues& Q, const BQuery* qs, Run&& run) {
   const int nt = Q.nt;
-  const int lane = threadIdx.x & (kWave - 1);
-  int x = blockIdx.x % kBQueues;
-  for (int exhausted = 0; exhausted < kBQueues;) {
-    const int32_t total = Q.qpre[x][nt];
-    // Heads only grow: a queue seen at or past its end stays empty, and is
-    // skipped without an atomic (all waves finish by probing every queue).
-    uint32_t v = 0;
-    if (lane == 0) {
-      v = __hip_atomic_load(&Q.heads[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (static_cast<int32_t>(v) < total) v = atomicAdd(&Q.heads[x], 1u);
-    }
-    const int32_t it = static_cast<int32_t>(__builtin_amdgcn_readfirstlane(v));
-    if (it >= total) {
-      x = (x + 1) % kBQueues;
-      exhausted++;
-      continue;
-    }
-    const int t = find_prefix(Q.qpre[x], nt, it);
+  const int x = static_cast<int>(blockIdx.x % kBQueues);
+  const int32_t total = Q.qpre[x][nt];
+  const int32_t nwg = (static_cast<int32_t>(gridDim.x) - x + kBQueues - 1) / kBQueues;
+  const int32_t W = nwg * kBWaves;
+  int t = 0;
+  for (int32_t it = static_cast<int32_t>(blockIdx.x / kBQueues) * kBWaves + uni(static_cast<int>(threadIdx.x / kWave));
+       it < total; it += W) {
+    while (Q.qpre[x][t + 1] <= it) t++;  // a wave's items ascend
     const int32_t local = it - Q.qpre[x][t];
     const int32_t m = Q.toff[t + 1] - Q.toff[t];
     const int32_t tile = x + kBQueues * (local / m);
     const BQuery& d = qs[Q.tq[Q.toff[t] + local % m]];
-    const int64_t lo = static_cast<int64_t>(tile) * kBTile;
-    const int64_t hi = lo + kBTile < d.n ? lo + kBTile : d.n;
+    const int64_t lo = static_cast<int64_t>(tile) * Q.tile;
+    const int64_t hi = lo + Q.tile < d.n ? lo + Q.tile : d.n;
     run(d, lo, hi);
     item_done(d);
   }
@@ -1635,13 +1657,15 @@ __global__ __launch_bounds__(kBThreads) void k_batch_join(BHeader* h, const BQue
   __shared__ uint64_t lds[kBLdsWords];
   const int slot = acc_slot();
   uint64_t* mine = lds + (threadIdx.x / kWave) * kBWaveWords;
+  const int tile = h->join.tile;
+  const int ngroups = tile / (kWave * 4);
   drain_wave(h->join, qs, [&](const BQuery& d, int64_t lo, int64_t hi) {
     if (d.G == 1) {
-      if (hi - lo == kBTile) item_join_fast<true, true>(d, lo, hi, slot, mine);
-      else item_join_fast<false, true>(d, lo, hi, slot, mine);
+      if (hi - lo == tile) item_join_fast<true, true>(d, lo, hi, slot, mine, ngroups);
+      else item_join_fast<false, true>(d, lo, hi, slot, mine, ngroups);
     } else {
-      if (hi - lo == kBTile) item_join_fast<true, false>(d, lo, hi, slot, mine);
-      else item_join_fast<false, false>(d, lo, hi, slot, mine);
+      if (hi - lo == tile) item_join_fast<true, false>(d, lo, hi, slot, mine, ngroups);
+      else item_join_fast<false, false>(d, lo, hi, slot, mine, ngroups);
     }
   });
 }
@@ -1760,11 +1784,11 @@ size_t join_table_bytes(const kg_query& q) {
   const int64_t G = std::min<int64_t>(query_groups(q), kMaxLdsGroups);
   const int64_t bn = std::max<int64_t>(q.build_n, 0);
   const int32_t sm = source_map_entries(bn, G);
-  if (bn > kBMapMax && sm) return round256(static_cast<size_t>(sm) * 4) + 512;
+  if (bn > kBMapMax && sm) return round256(static_cast<size_t>(sm) * 4) + 256 + kRangeBytes;
   const uint64_t nb = table_buckets(bn, query_lf(q));
-  size_t b = round256(nb * kSlots * 8) + round256(nb * kSlots * 4) + 256;
+  size_t b = round256(nb * kSlots * 8) + round256(nb * kSlots * 4) + 256 + kRangeBytes;
   const int32_t m = map_entries(true, bn, G);
-  if (m) b += round256(static_cast<size_t>(m) * 4) + 256;
+  if (m) b += round256(static_cast<size_t>(m) * 4) + kRangeBytes;
   return b;
 }
 
@@ -2028,6 +2052,21 @@ int plan_query(kg_ctx* ctx, const kg_query& q, BQuery& D, BPlan& P) {
   return KG_OK;
 }
 
+// Rows per item of a queue over the queries `ids`: the largest power of two
+// up to kBTile that still gives kBItemTarget items over all their rows (the
+// waves of about two thirds of the chip at the join kernel's occupancy), so a
+// batch of a few queries over small or mid-size tables spreads over the waves
+// instead of running one long item per wave (an item's rows are walked by one
+// wave, one dependent map or bucket lookup per row group). Larger items keep
+// the per-item partial flushes (global atomics) few.
+int queue_tile(const std::vector<int32_t>& ids, const kg_query* qs) {
+  int64_t rows = 0;
+  for (int i : ids) rows += qs[i].n;
+  int tile = kBTile;
+  while (tile > kBTileMin && rows / tile < kBItemTarget) tile /= 2;
+  return tile;
+}
+
 // Tables of the queries `ids` (equal table_id and n share tiles), in order of
 // first appearance, and the item prefix of each XCD queue.
 bool layout_queues(BQueues& Q, const std::vector<int32_t>& ids, const kg_query* qs) {
@@ -2056,13 +2095,12 @@ bool layout_queues(BQueues& Q, const std::vector<int32_t>& ids, const kg_query* 
   for (int x = 0; x < kBQueues; x++) {
     Q.qpre[x][0] = 0;
     for (int t = 0; t < nt; t++) {
-      const int64_t tiles = ceil_div(Q.tn[t], kBTile);
+      const int64_t tiles = ceil_div(Q.tn[t], Q.tile);
       const int64_t mine = tiles > x ? (tiles - x + kBQueues - 1) / kBQueues : 0;
       const int64_t items = mine * (Q.toff[t + 1] - Q.toff[t]);
       if (Q.qpre[x][t] + items >= INT32_MAX) return false;
       Q.qpre[x][t + 1] = Q.qpre[x][t] + static_cast<int32_t>(items);
     }
-    Q.heads[x] = 0;
   }
   return true;
 }
@@ -2218,7 +2256,7 @@ extern "C" int kg_batch_create(kg_ctx* ctx, const kg_batch_opts* opts, kg_batch*
   b->arena_bytes = static_cast<size_t>(arena);
   b->res_cap = std::min<size_t>(b->arena_bytes, size_t{64} << 20);
   b->grid = ctx->num_cus * 3;
-  b->grid_gen = ctx->num_cus * 8;
+  b->grid_gen = std::max(ctx->num_cus * 8, kBQueues);  // drain_wave needs a workgroup per XCD queue
   b->grid_shared = ctx->num_cus * 2;
   auto fail = [&](int st) {
     kg_batch_free(b);
@@ -2383,7 +2421,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     if (own_table) {
       nbuckets = table_buckets(P.bn, query_lf(qs[i]));
       need += round256(nbuckets * kSlots * 8) + round256(nbuckets * kSlots * 4);
-      if (msize) need += round256(static_cast<size_t>(msize) * 4) + 256;
+      if (msize) need += round256(static_cast<size_t>(msize) * 4) + kRangeBytes;
     }
     if (off + need > b->arena_bytes) {
       P.status = KG_ERR_UNSUPPORTED;
@@ -2399,7 +2437,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     D.sflag = P.join ? D.flag : nullptr;
     // Fill ranges must not overlap: the fill kernel writes them in no fixed order.
     H->fills[nf++] = BFill{D.sum, static_cast<int64_t>(kBSlots) * G * (2 * std::max(nm, 1) + 1) + 1, 0};
-    D.items = static_cast<int32_t>(ceil_div(qs[i].n, kBTile));  // shared scans: set by layout_groups
+    D.items = 0;  // set with the layout of the query's queue or shared-scan group
     rem_fill[i] = nf;
     H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.rem), 1, static_cast<uint64_t>(D.items)};
     D.first = reinterpret_cast<uint64_t*>(base + acc_bytes);
@@ -2422,7 +2460,8 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
         D.krange = reinterpret_cast<int64_t*>(mb + round256(static_cast<size_t>(msize) * 4));
         D.msize = msize;
         H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.map), msize / 2, ~0ull};
-        H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.krange), 2, static_cast<uint64_t>(INT64_MIN)};
+        H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(D.krange), kRangeCopies * kRangeStride,
+                               static_cast<uint64_t>(INT64_MIN)};
       }
     }
     D.res = b->d_res + P.res_off;
@@ -2433,11 +2472,11 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
   int32_t nbmem = 0;
   for (Source& S : srcs) {
     if (S.members.empty()) continue;
-    // flag word, key range, map, then (small build sides) the table
+    // flag word, key range copies, map, then (small build sides) the table
     const uint64_t nbuckets = S.map_only ? 0 : table_buckets(S.bn, S.lf);
     const size_t map_b = round256(static_cast<size_t>(S.msize) * 4);
     const size_t keys_b = round256(nbuckets * kSlots * 8), rows_b = round256(nbuckets * kSlots * 4);
-    const size_t need = 256 + map_b + keys_b + rows_b;
+    const size_t need = 256 + kRangeBytes + map_b + keys_b + rows_b;
     bool live = false;
     for (int i : S.members) live |= b->plans[i].status == KG_OK;
     if (!live) continue;
@@ -2451,15 +2490,16 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     }
     char* base = b->arena + off;
     uint32_t* sflag = reinterpret_cast<uint32_t*>(base);
-    int64_t* krange = reinterpret_cast<int64_t*>(base + 64);
-    uint32_t* map = reinterpret_cast<uint32_t*>(base + 256);
+    int64_t* krange = reinterpret_cast<int64_t*>(base + 256);
+    uint32_t* map = reinterpret_cast<uint32_t*>(base + 256 + kRangeBytes);
     H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(sflag), 1, 0};
-    H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(krange), 2, static_cast<uint64_t>(INT64_MIN)};
+    H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(krange), kRangeCopies * kRangeStride,
+                           static_cast<uint64_t>(INT64_MIN)};
     H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(map), S.msize / 2, ~0ull};
     TableView tv{};
     if (!S.map_only) {
-      tv.keys = reinterpret_cast<int64_t*>(base + 256 + map_b);
-      tv.rows = reinterpret_cast<int32_t*>(base + 256 + map_b + keys_b);
+      tv.keys = reinterpret_cast<int64_t*>(base + 256 + kRangeBytes + map_b);
+      tv.rows = reinterpret_cast<int32_t*>(base + 256 + kRangeBytes + map_b + keys_b);
       tv.nbuckets = nbuckets;
       H->fills[nf++] = BFill{reinterpret_cast<uint64_t*>(tv.keys), static_cast<int64_t>(nbuckets * kSlots),
                              static_cast<uint64_t>(kEmpty)};
@@ -2534,10 +2574,14 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     else if (Q[i].jfast) join_ids.push_back(i);
     else gen_ids.push_back(i);
   }
+  H->join.tile = queue_tile(join_ids, qs);
+  H->gen.tile = queue_tile(gen_ids, qs);
   if (!layout_groups(H->shared, b->hg(), fast_ids, qs, Q) || !layout_queues(H->join, join_ids, qs) ||
       !layout_queues(H->gen, gen_ids, qs))
     return kg_fail(ctx, KG_ERR_INVALID, "batch too large");
-  for (int i : fast_ids) H->fills[rem_fill[i]].v = static_cast<uint64_t>(Q[i].items);
+  for (int i : join_ids) Q[i].items = static_cast<int32_t>(ceil_div(qs[i].n, H->join.tile));
+  for (int i : gen_ids) Q[i].items = static_cast<int32_t>(ceil_div(qs[i].n, H->gen.tile));
+  for (int i : live) H->fills[rem_fill[i]].v = static_cast<uint64_t>(Q[i].items);
   // Queries that failed planning keep a zero descriptor: G = 0, nothing to fold.
   for (int i = 0; i < nq; i++)
     if (b->plans[i].status != KG_OK) {
