@@ -1111,13 +1111,6 @@ __device__ __forceinline__ void ld4_jmeasure(int kind, int fkind, const void* co
   }
 }
 
-// Element e (< 4) of a 4-element register array, chosen by selects: a
-// runtime index into the array itself would move it to scratch memory.
-template <typename T>
-__device__ __forceinline__ T pick4(const T* a, int e) {
-  return e == 0 ? a[0] : e == 1 ? a[1] : e == 2 ? a[2] : a[3];
-}
-
 template <bool FULL, bool ONEG>
 __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int64_t hi, int slot, uint64_t* lds,
                                                int ngroups) {
@@ -1234,74 +1227,81 @@ __device__ __forceinline__ void item_join_fast(const BQuery& d, int64_t lo, int6
         return;
       }
       if (d.map_only) return;  // the map is off and there is no table: the query reports it
-      // First buckets of the passing rows, loaded together.
-      uint64_t b[4];
-      longlong2 kv[4][kSlots / 2];
+      // First buckets of the passing rows, two rows at a time (a bucket is 16
+      // registers; four in flight would raise the register budget of every
+      // join item, map path included, and cut the waves per SIMD).
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        b[e] = pass[e] ? bucket_of(key[e], t.nbuckets) : 0;
+      for (int h2 = 0; h2 < 4; h2 += 2) {
+        uint64_t b[2];
+        longlong2 kv[2][kSlots / 2];
 #pragma unroll
-        for (int q = 0; q < kSlots / 2; q++)
-          kv[e][q] = pass[e] ? reinterpret_cast<const longlong2*>(t.keys + b[e] * kSlots)[q]
-                             : make_longlong2(kEmpty, kEmpty);
-      }
-      // Matching slots of the first buckets (bit kSlots e + s: row e, slot s)
-      // and the rows whose first bucket is full, so their chain goes on.
-      static_assert(4 * kSlots <= 32, "four rows' slot bits fit one word");
-      uint32_t hits = 0, more = 0;
+        for (int e = 0; e < 2; e++) {
+          b[e] = pass[h2 + e] ? bucket_of(key[h2 + e], t.nbuckets) : 0;
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        uint32_t h = 0;
-        bool empty = false;
-#pragma unroll
-        for (int q = 0; q < kSlots / 2; q++) {
-          h |= (kv[e][q].x == key[e] ? 1u : 0u) << (2 * q);
-          h |= (kv[e][q].y == key[e] ? 1u : 0u) << (2 * q + 1);
-          empty |= kv[e][q].x == kEmpty || kv[e][q].y == kEmpty;
+          for (int q = 0; q < kSlots / 2; q++)
+            kv[e][q] = pass[h2 + e] ? reinterpret_cast<const longlong2*>(t.keys + b[e] * kSlots)[q]
+                                    : make_longlong2(kEmpty, kEmpty);
         }
-        hits |= (pass[e] ? h : 0u) << (kSlots * e);
-        more |= (pass[e] && !empty ? 1u : 0u) << e;
-      }
-      // Rows in ascending order, each one's first-bucket matches before its
-      // chain; the row's values are selected, not indexed, so they stay in
-      // registers.
-#pragma unroll 1
-      for (int e = 0; e < 4; e++) {
-        uint32_t h = (hits >> (kSlots * e)) & ((1u << kSlots) - 1u);
-        const bool chain = (more >> e) & 1u;
-        if (h == 0 && !chain) continue;
-        const int64_t row = base + e;
-        const int64_t ke = pick4(key, e);
-        const uint64_t be = pick4(b, e);
-        const uint64_t a0 = pick4(v0, e), a1 = pick4(v1, e);
-        auto consume = [&](int32_t brow) {
-          if (!build_recheck(d, ke, brow)) return;
-          const int g = d.codes ? d.codes[brow] : 0;
-          if (static_cast<unsigned>(g) >= static_cast<unsigned>(G)) {
-            atomicOr(d.flag, kFlagBadCode);
-            return;
+        // Matching slots of the first buckets (bit kSlots e + s: row h2 + e,
+        // slot s) and the rows whose first bucket is full, so their chain
+        // goes on.
+        static_assert(2 * kSlots <= 32, "two rows' slot bits fit one word");
+        uint32_t hits = 0, more = 0;
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          uint32_t h = 0;
+          bool empty = false;
+#pragma unroll
+          for (int q = 0; q < kSlots / 2; q++) {
+            h |= (kv[e][q].x == key[h2 + e] ? 1u : 0u) << (2 * q);
+            h |= (kv[e][q].y == key[h2 + e] ? 1u : 0u) << (2 * q + 1);
+            empty |= kv[e][q].x == kEmpty || kv[e][q].y == kEmpty;
           }
-          consume_g(row, brow, g, a0, a1);
-        };
-        while (h) {
-          const int sidx = __builtin_ctz(h);
-          h &= h - 1;
-          consume(t.rows[be * kSlots + sidx]);
+          hits |= (pass[h2 + e] ? h : 0u) << (kSlots * e);
+          more |= (pass[h2 + e] && !empty ? 1u : 0u) << e;
         }
-        if (chain) {
-          uint64_t bb = (be + 1 == t.nbuckets) ? 0 : be + 1;
-          for (uint64_t scanned = 1; scanned < t.nbuckets; scanned++) {
-            const longlong2* bk = reinterpret_cast<const longlong2*>(t.keys + bb * kSlots);
-            bool emp = false;
-#pragma unroll
-            for (int q = 0; q < kSlots / 2; q++) {
-              const longlong2 x = bk[q];
-              if (x.x == ke) consume(t.rows[bb * kSlots + 2 * q]);
-              if (x.y == ke) consume(t.rows[bb * kSlots + 2 * q + 1]);
-              emp |= x.x == kEmpty || x.y == kEmpty;
+        // Rows in ascending order, each one's first-bucket matches before its
+        // chain; the row's values are selected, not indexed, so they stay in
+        // registers.
+#pragma unroll 1
+        for (int e = 0; e < 2; e++) {
+          uint32_t h = (hits >> (kSlots * e)) & ((1u << kSlots) - 1u);
+          const bool chain = (more >> e) & 1u;
+          if (h == 0 && !chain) continue;
+          const int64_t row = base + h2 + e;
+          const int64_t ke = e == 0 ? key[h2] : key[h2 + 1];
+          const uint64_t be = e == 0 ? b[0] : b[1];
+          const uint64_t a0 = e == 0 ? v0[h2] : v0[h2 + 1];
+          const uint64_t a1 = e == 0 ? v1[h2] : v1[h2 + 1];
+          auto consume = [&](int32_t brow) {
+            if (!build_recheck(d, ke, brow)) return;
+            const int g = d.codes ? d.codes[brow] : 0;
+            if (static_cast<unsigned>(g) >= static_cast<unsigned>(G)) {
+              atomicOr(d.flag, kFlagBadCode);
+              return;
             }
-            if (emp) break;
-            bb = (bb + 1 == t.nbuckets) ? 0 : bb + 1;
+            consume_g(row, brow, g, a0, a1);
+          };
+          while (h) {
+            const int sidx = __builtin_ctz(h);
+            h &= h - 1;
+            consume(t.rows[be * kSlots + sidx]);
+          }
+          if (chain) {
+            uint64_t bb = (be + 1 == t.nbuckets) ? 0 : be + 1;
+            for (uint64_t scanned = 1; scanned < t.nbuckets; scanned++) {
+              const longlong2* bk = reinterpret_cast<const longlong2*>(t.keys + bb * kSlots);
+              bool emp = false;
+#pragma unroll
+              for (int q = 0; q < kSlots / 2; q++) {
+                const longlong2 x = bk[q];
+                if (x.x == ke) consume(t.rows[bb * kSlots + 2 * q]);
+                if (x.y == ke) consume(t.rows[bb * kSlots + 2 * q + 1]);
+                emp |= x.x == kEmpty || x.y == kEmpty;
+              }
+              if (emp) break;
+              bb = (bb + 1 == t.nbuckets) ? 0 : bb + 1;
+            }
           }
         }
       }
