@@ -52,6 +52,7 @@ constexpr int kBBuildTile = 512;      // rows per build work item
 constexpr int kBThreads = 256;        // threads per workgroup
 constexpr int kBWaves = kBThreads / kWave;
 constexpr int kBQueues = 8;           // scan/probe work queues (one per XCD)
+constexpr int kBMaxIvl = 8;           // key intervals of a shared build source's predicate union
 constexpr int kRangeCopies = 16;      // copies of a direct map's key range, one 128-byte line each
 constexpr int kRangeStride = 16;      // int64 words between copies
 constexpr size_t kRangeBytes = sizeof(int64_t) * kRangeCopies * kRangeStride;
@@ -139,6 +140,12 @@ struct BQuery {
   uint32_t* sflag;
   int32_t map_only, build_all, recheck, nkp, kp_other;
   int32_t bmem_lo, bmem_hi;   // build_all: the source's queries, BHeader::bmem[bmem_lo .. bmem_hi)
+  // build_all, when every query's build predicates compare the build key with
+  // literals in one conjunction: the union of their key intervals (nivl
+  // merged, ascending, inclusive bounds), tested in place of each query's
+  // predicates; nivl < 0 otherwise.
+  int32_t nivl;
+  int64_t ivl_lo[kBMaxIvl], ivl_hi[kBMaxIvl];
   int32_t kp_op[kBFastPreds];
   int64_t kp_lit[kBFastPreds];
   TableView t;          // JOIN_AGG table (slots in the arena)
@@ -250,6 +257,13 @@ __device__ __forceinline__ uint64_t coherent_load(const uint64_t* p) {
 // or for a shared source (build_all) those of any of the source's queries.
 __device__ __forceinline__ bool build_row_passes(const BHeader* h, const BQuery* qs, const BQuery& d, int64_t r) {
   if (!d.build_all) return eval_preds(d.PB, r);
+  if (d.nivl >= 0) {  // a NULL key passes no interval (its row is skipped as a NULL key anyway)
+    int64_t k;
+    if (!load_key(d.bkey, r, k)) return false;
+    for (int i = 0; i < d.nivl; i++)
+      if (k >= d.ivl_lo[i] && k <= d.ivl_hi[i]) return true;
+    return false;
+  }
   for (int m = d.bmem_lo; m < d.bmem_hi; m++)
     if (eval_preds(qs[h->bmem[m]].PB, r)) return true;
   return false;
@@ -1890,6 +1904,64 @@ int capture(kg_batch* b) {
 
 bool fits_group(const BQuery& D);
 
+// For the leader of a shared build source: when every member's build
+// predicates are one conjunction of comparisons of the build key column (not
+// REAL) with literals, the union of the members' key intervals, merged, into
+// lead.nivl / ivl_lo / ivl_hi (at most kBMaxIvl intervals, else nivl stays -1).
+// A row passes some member's predicates exactly when its non-NULL key lies in
+// the union; a member without predicates passes every row.
+void key_interval_union(BQuery& lead, const int32_t* mem, int nmem, const BQuery* Q) {
+  std::vector<std::pair<int64_t, int64_t>> iv;
+  for (int m = 0; m < nmem; m++) {
+    const BQuery& M = Q[mem[m]];
+    if (M.PB.n > 0 && !M.PB.conj) return;
+    int64_t lo = INT64_MIN, hi = INT64_MAX;
+    for (int p = 0; p < M.PB.n; p++) {
+      if (M.PB.data[p] != M.bkey.data || M.PB.type[p] != M.bkey.type || M.PB.type[p] == KG_REAL ||
+          M.PB.nulls[p] != M.bkey.nulls)
+        return;
+      const int64_t v = M.PB.ilit[p];
+      switch (M.PB.op[p]) {
+        case KG_EQ: lo = std::max(lo, v); hi = std::min(hi, v); break;
+        case KG_LEQ: hi = std::min(hi, v); break;
+        case KG_GEQ: lo = std::max(lo, v); break;
+        case KG_LT:  // key < INT64_MIN: empty
+          if (v == INT64_MIN) {
+            lo = 1;
+            hi = 0;
+          } else {
+            hi = std::min(hi, v - 1);
+          }
+          break;
+        case KG_GT:  // key > INT64_MAX: empty
+          if (v == INT64_MAX) {
+            lo = 1;
+            hi = 0;
+          } else {
+            lo = std::max(lo, v + 1);
+          }
+          break;
+        default: return;  // <> splits the key range
+      }
+    }
+    if (lo <= hi) iv.push_back({lo, hi});
+  }
+  std::sort(iv.begin(), iv.end());
+  std::vector<std::pair<int64_t, int64_t>> u;
+  for (const auto& x : iv) {
+    if (!u.empty() && (u.back().second == INT64_MAX || x.first <= u.back().second + 1))
+      u.back().second = std::max(u.back().second, x.second);
+    else
+      u.push_back(x);
+  }
+  if (u.size() > static_cast<size_t>(kBMaxIvl)) return;
+  lead.nivl = static_cast<int32_t>(u.size());
+  for (size_t i = 0; i < u.size(); i++) {
+    lead.ivl_lo[i] = u[i].first;
+    lead.ivl_hi[i] = u[i].second;
+  }
+}
+
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 // The register-path view of a scan, when its columns allow vector loads.
@@ -2507,6 +2579,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     off += need;
     const bool several = S.members.size() > 1;
     bool leader = true;
+    int lead = -1;
     const int32_t blo = nbmem;
     for (int i : S.members)
       if (b->plans[i].status == KG_OK) H->bmem[nbmem++] = i;
@@ -2522,6 +2595,8 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
       D.sflag = sflag;
       D.map_only = S.map_only ? 1 : 0;
       D.build_all = (leader && several) ? 1 : 0;  // the leader builds the source (the others add no build items)
+      D.nivl = -1;
+      if (leader) lead = i;
       D.bmem_lo = blo;
       D.bmem_hi = nbmem;
       D.recheck = several ? 1 : 0;
@@ -2542,6 +2617,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
         }
       }
     }
+    if (lead >= 0 && several) key_interval_union(Q[lead], H->bmem + blo, nbmem - blo, Q);
   }
   H->nfills = nf;
   // Build phase entries; tables (queries sharing table_id and n) for the scan

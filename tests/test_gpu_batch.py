@@ -292,6 +292,38 @@ def test_map_only_build_sources(env):
         assert_same(s, g, oracle_spec(s, Hd), what="repeated keys served")
 
 
+def test_shared_source_key_intervals(env):
+    """Joins over one build source whose build predicates compare the build key
+    with literals: the source's map is built over the union of their key
+    intervals (merged; each query's own predicates instead when the union has
+    more than eight pieces or a query uses <>), and every query re-checks its
+    own range at a match. Each query against the oracle: overlapping and
+    adjacent ranges, a point, open ends, an empty range, a query without build
+    predicates, a <> query and twelve separate points."""
+    ctx, _, cols, H = env
+    base = serve_queries(64, SIZES, NBUILD, seed=31, templates=["c2_join"])
+    EQ, NE, LT, LE, GT, GE = abi.KG_EQ, abi.KG_NEQ, abi.KG_LT, abi.KG_LEQ, abi.KG_GT, abi.KG_GEQ
+    cases = [
+        [[(0, GE, 100), (0, LT, 5000)], [(0, GE, 5000), (0, LE, 9000)], [(0, EQ, 20000)],
+         [(0, GT, NBUILD - 50)], [(0, LT, 10)], [(0, GT, 500), (0, LT, 400)]],
+        [[(0, GE, 100), (0, LT, 5000)], []],
+        [[(0, NE, 77)], [(0, LT, 3000)]],
+        [[(0, EQ, 1000 * i)] for i in range(12)],
+    ]
+    b = Batch(ctx, arena_bytes=1 << 30)
+    for preds in cases:
+        specs = []
+        for j, bp in enumerate(preds):
+            s = dict(base[j])
+            s["build_preds"] = bp
+            specs.append(s)
+        print("shared source, build predicates", preds, flush=True)
+        got = b.run(_queries(cols, specs), max_groups=64)
+        for s, g in zip(specs, got):
+            assert_same(s, g, oracle_spec(s, H), what="key intervals")
+    b.free()
+
+
 CHAIN_CASES = [("a", "%e_a%", 50), ("t", "%o_i%", 10), ("q", "%a_e%", 100), ("s", "%i_e%", 37),
                ("zz", "%e_a%", 50), ("a", "%e_a%", 0)]
 
