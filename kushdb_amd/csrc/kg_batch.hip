@@ -140,10 +140,10 @@ struct BQuery {
   uint32_t* sflag;
   int32_t map_only, build_all, recheck, nkp, kp_other;
   int32_t bmem_lo, bmem_hi;   // build_all: the source's queries, BHeader::bmem[bmem_lo .. bmem_hi)
-  // build_all, when every query's build predicates compare the build key with
-  // literals in one conjunction: the union of their key intervals (nivl
-  // merged, ascending, inclusive bounds), tested in place of each query's
-  // predicates; nivl < 0 otherwise.
+  // The query that builds (alone, or build_all for a source), when every
+  // query's build predicates compare the build key with literals in one
+  // conjunction: the union of their key intervals (nivl merged, ascending,
+  // inclusive bounds), tested in place of the predicates; nivl < 0 otherwise.
   int32_t nivl;
   int64_t ivl_lo[kBMaxIvl], ivl_hi[kBMaxIvl];
   int32_t kp_op[kBFastPreds];
@@ -253,47 +253,46 @@ __device__ __forceinline__ uint64_t coherent_load(const uint64_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Whether build row r goes into query d's map / table: its build predicates,
-// or for a shared source (build_all) those of any of the source's queries.
-__device__ __forceinline__ bool build_row_passes(const BHeader* h, const BQuery* qs, const BQuery& d, int64_t r) {
-  if (!d.build_all) return eval_preds(d.PB, r);
-  if (d.nivl >= 0) {  // a NULL key passes no interval (its row is skipped as a NULL key anyway)
-    int64_t k;
+// Whether build row r goes into query d's map / table (its build predicates,
+// or for a shared source (build_all) those of any of the source's queries,
+// tested as key intervals where they have that form), with its key in *k. A
+// row with a NULL key goes nowhere.
+__device__ __forceinline__ bool build_row_key(const BHeader* h, const BQuery* qs, const BQuery& d, int64_t r,
+                                              int64_t& k) {
+  if (d.nivl >= 0) {
     if (!load_key(d.bkey, r, k)) return false;
     for (int i = 0; i < d.nivl; i++)
       if (k >= d.ivl_lo[i] && k <= d.ivl_hi[i]) return true;
     return false;
   }
-  for (int m = d.bmem_lo; m < d.bmem_hi; m++)
-    if (eval_preds(qs[h->bmem[m]].PB, r)) return true;
-  return false;
+  if (!d.build_all) {
+    if (!eval_preds(d.PB, r)) return false;
+  } else {
+    bool any = false;
+    for (int m = d.bmem_lo; m < d.bmem_hi && !any; m++) any = eval_preds(qs[h->bmem[m]].PB, r);
+    if (!any) return false;
+  }
+  return load_key(d.bkey, r, k);
 }
 
 // Key range of a direct-map join: min and max of the qualifying build keys,
 // over the build items (before the build phase). Items are dealt by a grid
-// stride (no shared counter); each workgroup reduces its item's range and
+// stride (no shared counter), so a workgroup's items ascend and those of one
+// query are consecutive: the workgroup accumulates its range over them and
+// reduces it once per query, when the query changes and at the end. It then
 // touches the words of copy blockIdx mod kRangeCopies, each copy on its own
-// cache line, only when it widens them. Every workgroup starts at once and
-// finds the initial range, so with one copy all their atomics would queue on
-// one line.
+// cache line, only when it widens them (every workgroup finds the initial
+// range, so with one copy all their atomics would queue on one line).
 __global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQuery* qs) {
   __shared__ int64_t s_red[2][kBWaves];
+  constexpr int R = kBBuildTile / kBThreads;  // rows per thread and item
+  static_assert(R * kBThreads == kBBuildTile, "build items are whole rows per thread");
   const int32_t total = h->rpre[h->nrq];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
-  for (int32_t it = blockIdx.x; it < total; it += gridDim.x) {
-    const int j = find_prefix(h->rpre, h->nrq, it);
-    const BQuery& d = qs[h->rq[j]];
-    const int64_t lo = static_cast<int64_t>(it - h->rpre[j]) * kBBuildTile;
-    const int64_t hi = lo + kBBuildTile < d.bn ? lo + kBBuildTile : d.bn;
-    int64_t nmin = INT64_MIN, mx = INT64_MIN;  // max of -key, max of key
-    for (int64_t r = lo + threadIdx.x; r < hi; r += kBThreads) {
-      if (!build_row_passes(h, qs, d, r)) continue;
-      int64_t k;
-      if (!load_key(d.bkey, r, k) || k == kEmpty) continue;
-      nmin = -k > nmin ? -k : nmin;
-      mx = k > mx ? k : mx;
-    }
+  int64_t nmin = INT64_MIN, mx = INT64_MIN;  // max of -key, max of key
+  // Uniform over the workgroup (it and j depend on blockIdx only).
+  auto flush = [&](const BQuery& d) {
     for (int s = kWave / 2; s >= 1; s >>= 1) {
       const int64_t x = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(nmin), s));
       const int64_t y = static_cast<int64_t>(shfl_xor_u64(static_cast<uint64_t>(mx), s));
@@ -319,7 +318,29 @@ __global__ __launch_bounds__(kBThreads) void k_batch_range(BHeader* h, const BQu
       }
     }
     __syncthreads();
+    nmin = mx = INT64_MIN;
+  };
+  int j = -1;
+  for (int32_t it = blockIdx.x; it < total; it += gridDim.x) {
+    int jj = j < 0 ? find_prefix(h->rpre, h->nrq, it) : j;
+    while (h->rpre[jj + 1] <= it) jj++;
+    if (jj != j) {
+      if (j >= 0) flush(qs[h->rq[j]]);
+      j = jj;
+    }
+    const BQuery& d = qs[h->rq[j]];
+    const int64_t lo = static_cast<int64_t>(it - h->rpre[j]) * kBBuildTile;
+    const int64_t hi = lo + kBBuildTile < d.bn ? lo + kBBuildTile : d.bn;
+#pragma unroll
+    for (int e = 0; e < R; e++) {
+      const int64_t r = lo + threadIdx.x + static_cast<int64_t>(e) * kBThreads;
+      int64_t k;
+      if (r >= hi || !build_row_key(h, qs, d, r, k) || k == kEmpty) continue;
+      nmin = -k > nmin ? -k : nmin;
+      mx = k > mx ? k : mx;
+    }
   }
+  if (j >= 0) flush(qs[h->rq[j]]);
 }
 
 // Whether a direct-map join can use its map (range known, within the map, no
@@ -357,49 +378,77 @@ __device__ __forceinline__ bool map_usable(const BQuery& d, int64_t* mn, int64_t
 // Build phase: items (query, build tile); same insertion as k_ht_build, and
 // for a direct-map join the entry of the key in the map (a repeated key, or a
 // group code outside [0, G), turns the map off). Items are dealt by a grid
-// stride.
+// stride; a workgroup checks the map once per query (its items of one query
+// are consecutive), and the map entries of a thread's rows of an item are
+// claimed together.
 __global__ __launch_bounds__(kBThreads) void k_batch_build(BHeader* h, const BQuery* qs) {
+  constexpr int R = kBBuildTile / kBThreads;  // rows per thread and item
   const int32_t total = h->bpre[h->nbq];
+  int j = -1;
+  int64_t mn = 0, span = 0;
+  bool use_map = false;
   for (int32_t it = blockIdx.x; it < total; it += gridDim.x) {
-    const int j = find_prefix(h->bpre, h->nbq, it);
-    const BQuery& d = qs[h->bq[j]];
+    int jj = j < 0 ? find_prefix(h->bpre, h->nbq, it) : j;
+    while (h->bpre[jj + 1] <= it) jj++;
+    const BQuery& d = qs[h->bq[jj]];
+    if (jj != j) {
+      j = jj;
+      use_map = map_usable(d, &mn, &span);
+    }
     const int64_t lo = static_cast<int64_t>(it - h->bpre[j]) * kBBuildTile;
     const int64_t hi = lo + kBBuildTile < d.bn ? lo + kBBuildTile : d.bn;
     const TableView t = d.t;
-    int64_t mn = 0, span = 0;
-    const bool use_map = map_usable(d, &mn, &span);
     if (d.map_only && !use_map) {  // no table to fall back to: the queries report it
       if (threadIdx.x == 0) atomicOr(d.sflag, kFlagNoMap);
       continue;
     }
-    for (int64_t r = lo + threadIdx.x; r < hi; r += kBThreads) {
-      if (!build_row_passes(h, qs, d, r)) continue;
-      int64_t k;
-      if (!load_key(d.bkey, r, k)) continue;  // NULL build keys are skipped
-      if (k == kEmpty) {
+    int64_t key[R];
+    bool ok[R];
+#pragma unroll
+    for (int e = 0; e < R; e++) {
+      const int64_t r = lo + threadIdx.x + static_cast<int64_t>(e) * kBThreads;
+      key[e] = 0;
+      ok[e] = r < hi && build_row_key(h, qs, d, r, key[e]);
+      if (ok[e] && key[e] == kEmpty) {
         atomicOr(d.sflag, kFlagEmptyKey);
-        continue;
+        ok[e] = false;
       }
-      if (use_map) {
-        const uint64_t off = static_cast<uint64_t>(k) - static_cast<uint64_t>(mn);
+    }
+    if (use_map) {
+      uint32_t prev[R];
+#pragma unroll
+      for (int e = 0; e < R; e++) {
+        const int64_t r = lo + threadIdx.x + static_cast<int64_t>(e) * kBThreads;
+        prev[e] = kMapEmpty;
+        if (!ok[e]) continue;
+        const uint64_t off = static_cast<uint64_t>(key[e]) - static_cast<uint64_t>(mn);
         const int g = d.codes ? d.codes[r] : 0;
         if (off >= static_cast<uint64_t>(span) || g < 0 || g >= d.G || r >= (int64_t{1} << 24)) {
           atomicOr(d.sflag, kFlagNoMap);
         } else {
-          const uint32_t e = (static_cast<uint32_t>(g) << 24) | static_cast<uint32_t>(r);
-          if (atomicCAS(&d.map[off], kMapEmpty, e) != kMapEmpty) atomicOr(d.sflag, kFlagNoMap);
+          const uint32_t ent = (static_cast<uint32_t>(g) << 24) | static_cast<uint32_t>(r);
+          prev[e] = atomicCAS(&d.map[off], kMapEmpty, ent);
         }
       }
-      if (d.map_only) continue;
+#pragma unroll
+      for (int e = 0; e < R; e++)
+        if (prev[e] != kMapEmpty) atomicOr(d.sflag, kFlagNoMap);
+    }
+    if (d.map_only) continue;
+#pragma unroll 1
+    for (int e = 0; e < R; e++) {
+      if (!ok[e]) continue;
+      const int64_t r = lo + threadIdx.x + static_cast<int64_t>(e) * kBThreads;
+      const int64_t k = key[e];
       uint64_t b = bucket_of(k, t.nbuckets);
       bool done = false;
       for (uint64_t probes = 0; probes < t.nbuckets && !done; probes++) {
         for (int s = 0; s < kSlots && !done; s++) {
           if (t.keys[b * kSlots + s] != kEmpty) continue;
-          const unsigned long long prev =
+          const unsigned long long p =
               atomicCAS(reinterpret_cast<unsigned long long*>(t.keys + b * kSlots + s),
                         static_cast<unsigned long long>(kEmpty), static_cast<unsigned long long>(k));
-          if (prev == static_cast<unsigned long long>(kEmpty)) {
+          if (p == static_cast<unsigned long long>(kEmpty)) {
             t.rows[b * kSlots + s] = static_cast<int32_t>(r);
             done = true;
           }
@@ -1904,10 +1953,11 @@ int capture(kg_batch* b) {
 
 bool fits_group(const BQuery& D);
 
-// For the leader of a shared build source: when every member's build
-// predicates are one conjunction of comparisons of the build key column (not
-// REAL) with literals, the union of the members' key intervals, merged, into
-// lead.nivl / ivl_lo / ivl_hi (at most kBMaxIvl intervals, else nivl stays -1).
+// For the query that builds a map or table (alone, or the leader of a shared
+// build source with members mem): when every member's build predicates are
+// one conjunction of comparisons of the build key column (not REAL) with
+// literals, the union of the members' key intervals, merged, into lead.nivl /
+// ivl_lo / ivl_hi (at most kBMaxIvl intervals, else nivl stays -1).
 // A row passes some member's predicates exactly when its non-NULL key lies in
 // the union; a member without predicates passes every row.
 void key_interval_union(BQuery& lead, const int32_t* mem, int nmem, const BQuery* Q) {
@@ -2029,6 +2079,7 @@ bool join_view(const kg_query& q, const PredSet& P, const MeasureSet& M, BJoinFa
 // Validates and plans query q into the device descriptor D and host plan P.
 int plan_query(kg_ctx* ctx, const kg_query& q, BQuery& D, BPlan& P) {
   std::memset(&D, 0, sizeof(D));
+  D.nivl = -1;
   if (q.kind == KG_Q_CHAIN) return kg_fail(ctx, KG_ERR_UNSUPPORTED, "chain queries are not batched");
   KG_REQUIRE(ctx, q.kind == KG_Q_SCAN_AGG || q.kind == KG_Q_JOIN_AGG, "bad query kind %d", q.kind);
   KG_REQUIRE(ctx, q.n >= 0 && q.n < INT32_MAX, "row count %lld out of range", (long long)q.n);
@@ -2491,6 +2542,8 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
     const bool own_table = P.join && src_of[i] < 0;
     const int32_t msize = own_table ? map_entries(D.jfast != 0, P.bn, G) : 0;
     if (own_table) {
+      const int32_t self = i;
+      key_interval_union(D, &self, 1, Q);
       nbuckets = table_buckets(P.bn, query_lf(qs[i]));
       need += round256(nbuckets * kSlots * 8) + round256(nbuckets * kSlots * 4);
       if (msize) need += round256(static_cast<size_t>(msize) * 4) + kRangeBytes;
@@ -2617,7 +2670,7 @@ extern "C" int kg_batch_submit(kg_batch* b, const kg_query* qs, int32_t nq) {
         }
       }
     }
-    if (lead >= 0 && several) key_interval_union(Q[lead], H->bmem + blo, nbmem - blo, Q);
+    if (lead >= 0) key_interval_union(Q[lead], H->bmem + blo, nbmem - blo, Q);
   }
   H->nfills = nf;
   // Build phase entries; tables (queries sharing table_id and n) for the scan
