@@ -47,7 +47,7 @@ namespace kg {
 
 constexpr int kBTile = 16384;         // most rows per scan/probe work item
 constexpr int kBTileMin = 256;        // fewest rows per scan/probe work item (one row group per lane)
-constexpr int64_t kBItemTarget = 2048;  // fewest scan/probe items a queue aims for (before kBTileMin)
+constexpr int64_t kBItemTarget = 1024;  // fewest scan/probe items a queue aims for (before kBTileMin)
 constexpr int kBBuildTile = 512;      // rows per build work item
 constexpr int kBThreads = 256;        // threads per workgroup
 constexpr int kBWaves = kBThreads / kWave;
@@ -2176,8 +2176,9 @@ int plan_query(kg_ctx* ctx, const kg_query& q, BQuery& D, BPlan& P) {
 }
 
 // Rows per item of a queue over the queries `ids`: the largest power of two
-// up to kBTile that still gives kBItemTarget items over all their rows (the
-// waves of about two thirds of the chip at the join kernel's occupancy), so a
+// up to kBTile that still gives kBItemTarget items over all their rows (about
+// a quarter of the waves the chip holds at the join kernel's occupancy: more,
+// smaller items measured slower on 1M- to 50M-row probes), so a
 // batch of a few queries over small or mid-size tables spreads over the waves
 // instead of running one long item per wave (an item's rows are walked by one
 // wave, one dependent map or bucket lookup per row group). Larger items keep
