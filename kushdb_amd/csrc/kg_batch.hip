@@ -52,6 +52,7 @@ constexpr int kBBuildTile = 512;      // rows per build work item
 constexpr int kBThreads = 256;        // threads per workgroup
 constexpr int kBWaves = kBThreads / kWave;
 constexpr int kBQueues = 8;           // scan/probe work queues (one per XCD)
+constexpr int kBHeadStride = 32;      // 4-byte words between the queues' item counters
 constexpr int kBMaxIvl = 8;           // key intervals of a shared build source's predicate union
 constexpr int kRangeCopies = 16;      // copies of a direct map's key range, one 128-byte line each
 constexpr int kRangeStride = 16;      // int64 words between copies
@@ -192,6 +193,9 @@ struct BQueues {
   int32_t toff[kBMaxQueries + 1];             // table t's queries: tq[toff[t] .. toff[t+1])
   int32_t tq[kBMaxQueries];
   int64_t tn[kBMaxQueries];                   // rows of table t
+  // Per queue, the count of items taken past the first W (see drain_wave),
+  // each on its own 128-byte line.
+  unsigned int next[kBQueues * kBHeadStride];
 };
 
 // Batch header, copied to the device with the query descriptors.
@@ -1481,24 +1485,26 @@ __device__ __forceinline__ int acc_slot() {
   return static_cast<int>(blockIdx.x % kBQueues) + kBQueues * (wave % (kBSlots / kBQueues));
 }
 
-// Scan/probe items of one queue set, one wave per item, dealt with no shared
-// counter: the waves of XCD x (workgroups b with b mod 8 = x) take the items of
-// queue x, wave j of the XCD taking items j, j + W, j + 2W, ... (W waves on the
-// XCD), so every item is taken once. Workgroups are dispatched in order as
-// earlier ones finish, so waves that start late take what is left. (Dealing
-// through an atomic head per queue queued thousands of atomics on the heads'
-// cache line when every wave of the grid arrived at once.) Needs a grid of at
-// least kBQueues workgroups, which the host checks.
+// Scan/probe items of one queue set, one wave per item. The waves of XCD x
+// (workgroups b with b mod 8 = x) take the items of queue x: wave j of the XCD
+// (W waves on it) starts with item j, with no atomic, and takes each further
+// item from the queue's counter, W + count. Items are thus taken in queue
+// order, so the items of one tile (one per query of its table) run at about
+// the same time and share the tile in L2; starting every wave through the
+// counter would queue the atomics of the whole grid on its line at once.
+// Every item is taken once. Needs a grid of at least kBQueues workgroups,
+// which the host checks.
 template <typename Run>
-__device__ __forceinline__ void drain_wave(const BQueues& Q, const BQuery* qs, Run&& run) {
+__device__ __forceinline__ void drain_wave(BQueues& Q, const BQuery* qs, Run&& run) {
   const int nt = Q.nt;
+  const int lane = threadIdx.x & (kWave - 1);
   const int x = static_cast<int>(blockIdx.x % kBQueues);
   const int32_t total = Q.qpre[x][nt];
   const int32_t nwg = (static_cast<int32_t>(gridDim.x) - x + kBQueues - 1) / kBQueues;
   const int32_t W = nwg * kBWaves;
   int t = 0;
-  for (int32_t it = static_cast<int32_t>(blockIdx.x / kBQueues) * kBWaves + uni(static_cast<int>(threadIdx.x / kWave));
-       it < total; it += W) {
+  int32_t it = static_cast<int32_t>(blockIdx.x / kBQueues) * kBWaves + uni(static_cast<int>(threadIdx.x / kWave));
+  while (it < total) {
     while (Q.qpre[x][t + 1] <= it) t++;  // a wave's items ascend
     const int32_t local = it - Q.qpre[x][t];
     const int32_t m = Q.toff[t + 1] - Q.toff[t];
@@ -1508,6 +1514,10 @@ __device__ __forceinline__ void drain_wave(const BQueues& Q, const BQuery* qs, R
     const int64_t hi = lo + Q.tile < d.n ? lo + Q.tile : d.n;
     run(d, lo, hi);
     item_done(d);
+    if (total <= W) break;  // the first items were all: no counter traffic
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(&Q.next[x * kBHeadStride], 1u);
+    it = W + static_cast<int32_t>(__builtin_amdgcn_readfirstlane(v));
   }
 }
 
@@ -2216,6 +2226,7 @@ bool layout_queues(BQueues& Q, const std::vector<int32_t>& ids, const kg_query* 
     for (size_t j = 0; j < tables[t].second.size(); j++) Q.tq[Q.toff[t] + j] = tables[t].second[j];
     Q.toff[t + 1] = Q.toff[t] + static_cast<int32_t>(tables[t].second.size());
   }
+  std::memset(Q.next, 0, sizeof(Q.next));
   for (int x = 0; x < kBQueues; x++) {
     Q.qpre[x][0] = 0;
     for (int t = 0; t < nt; t++) {
