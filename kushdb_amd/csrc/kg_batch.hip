@@ -23,11 +23,11 @@
 // there. Other scan/probe work items are (query, tile of 256 to kBTile rows,
 // one size per queue, smaller when the queue's queries hold few rows in all);
 // queries over the same table share its tiles through L2: the items of tile k
-// of a table are adjacent and
-// sit in queue k mod 8, one queue per XCD (workgroup b starts on queue b mod 8,
-// the XCD the dispatcher places it on). A workgroup drains its own queue, then
-// the others, and exits when all eight are empty; there is no inter-workgroup
-// waiting, so any grid size is correct.
+// of a table are adjacent and sit in queue k mod 8, one queue per XCD (the
+// waves of workgroup b take the items of queue b mod 8, the XCD the
+// dispatcher places it on: each wave's first item by its index, the rest in
+// queue order from the queue's counter). There is no inter-workgroup waiting,
+// so any grid of at least eight workgroups is correct.
 //
 // Register-path partials live in lane registers, reduced across the wave by a
 // butterfly; other queries use a wave-private LDS table updated with LDS
