@@ -292,6 +292,30 @@ def test_map_only_build_sources(env):
         assert_same(s, g, oracle_spec(s, Hd), what="repeated keys served")
 
 
+def test_large_batch_counter_dealing(env):
+    """A batch whose join queue holds more items than an XCD has waves (about
+    18 joins over a 16M-row table: some 2k items per queue), so waves take the
+    items past their first from the queue counters. Every query equals its
+    result alone, where each item is some wave's first."""
+    ctx = env[0]
+    n = 16_000_000
+    tables = serve_tables((n,), nbuild=NBUILD, device="cuda:0", chain_scale=CHAIN_SCALE)
+    torch.cuda.synchronize()
+    cols = serve_columns(tables)
+    specs = serve_queries(64, (n,), NBUILD, seed=41,
+                          templates=["c2_join", "c0_star", "selective_join", "star_dim_filter", "two_filter"])[:24]
+    assert sum(s["kind"] == "join" for s in specs) >= 16
+    print("large batch of", len(specs), "over", n, "rows", flush=True)
+    b = Batch(ctx, arena_bytes=1 << 30)
+    mix = b.run(_queries(cols, specs), max_groups=64)
+    for s, m in zip(specs, mix):
+        alone = b.run(_queries(cols, [s]), max_groups=64)[0]
+        assert_same(s, alone, (m[1], m[2], m[3]), what="counter dealing")
+    b.free()
+    del tables, cols
+    torch.cuda.empty_cache()
+
+
 def test_shared_source_key_intervals(env):
     """Joins over one build source whose build predicates compare the build key
     with literals: the source's map is built over the union of their key
